@@ -1,0 +1,60 @@
+"""Extract the reference's own golden data for the raster RCWA path into small fixtures.
+
+Run once in a container that holds the reference checkout; the outputs are committed, and no
+test or GPU run reads the reference at run time. Nothing is imported from the reference: its
+``.npz`` files are read with ``numpy.load(allow_pickle=False)`` and its Li-nesting totals are
+constants printed in its test source (``tests/references/test_reticolo_li_nesting.py:34-39``),
+restated below as data.
+
+Sources (paths relative to the reference root):
+* ``tests/references/golden_reticolo.npz`` -- RETICOLO 1D per-order efficiencies; the inputs
+  are the case table at ``tests/references/test_reticolo_golden.py:57-93``.
+* ``tests/fixtures/legacy_rcwa/golden_legacy.npz`` -- legacy per-order efficiencies, boundary
+  amplitudes, ``epz_i`` and ``q`` per layer; inputs at
+  ``tests/rcwa/test_golden_legacy_parity.py:250-264`` (layers listed top-to-bottom there).
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+REF = Path(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
+HERE = Path(__file__).resolve().parent
+
+RETICOLO_CASES = ['te_normal', 'te_oblique', 'tm_normal', 'tm_oblique', 'te_quarter',
+                  'tm_quarter', 'te_two_layer', 'tm_two_layer', 'te_high_order', 'tm_high_order']
+
+LEGACY_CASES = ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique', 'te_2layer',
+                'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer']
+
+# tests/references/test_reticolo_li_nesting.py:34-39 -- RETICOLO V10 total T, (TE, TM).
+LI_NESTING_FIXED = {'3': [0.9822061471, 0.8913451356], '5': [0.9816211625, 0.8830613359]}
+
+
+def main():
+    g = np.load(REF / 'tests/references/golden_reticolo.npz', allow_pickle=False)
+    out = {k: g[k] for k in g.files if k.split('__')[0] in RETICOLO_CASES}
+    np.savez_compressed(HERE / 'reticolo_1d.npz', **out)
+
+    lg = np.load(REF / 'tests/fixtures/legacy_rcwa/golden_legacy.npz', allow_pickle=False)
+    keep = {}
+    for k in lg.files:
+        case = k.split('__')[0]
+        if case not in LEGACY_CASES:
+            continue
+        parts = k.split('__')
+        if parts[1] in ('res', 'res_te_inc', 'res_tm_inc') or parts[1] == 'grating_type':
+            keep[k] = lg[k]
+        elif parts[1].startswith('layer') and parts[2] in ('epz_i', 'q', 'd'):
+            keep[k] = lg[k]
+    np.savez_compressed(HERE / 'legacy_subset.npz', **keep)
+
+    (HERE / 'li_nesting_fixed.json').write_text(json.dumps(
+        {'source': 'tests/references/test_reticolo_li_nesting.py:34-39 (RETICOLO V10, nesting fixed)',
+         'tolerance': 1e-9, 'total_T_te_tm': LI_NESTING_FIXED}, indent=1))
+    print('wrote', len(out), 'reticolo arrays,', len(keep), 'legacy arrays')
+
+
+if __name__ == '__main__':
+    main()

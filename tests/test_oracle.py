@@ -1,0 +1,258 @@
+"""Pin the CPU oracle against the reference's golden vectors and against closed forms.
+
+These run on the CPU only. The oracle is the checker for every GPU parity test, so each of its
+functions is pinned here by data the reference ships or by mathematics a mistake would break:
+exact pixel integrals, uniform layers, Fresnel/Airy closed forms, energy conservation,
+mode residuals.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import rcwa_oracle as orc
+
+GOLDEN = Path(__file__).resolve().parent / 'golden'
+
+# tests/references/test_reticolo_golden.py:57-93 (reference): inputs of the RETICOLO 1D cases.
+_HALF = np.array([[[9.0] * 5 + [1.0] * 5]])
+_QUARTER = np.array([[[12.25] * 3 + [1.0] * 9]])
+_TWO_LAYER = np.array([[[9.0] * 6 + [1.0] * 6], [[4.840000000000001] * 8 + [1.0] * 4]])
+RETICOLO = {
+    'te_normal': dict(tetm='s', theta=0.0, top=1.0, bot=2.25, M=10, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+    'te_oblique': dict(tetm='s', theta=0.2, top=1.0, bot=2.25, M=10, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+    'tm_normal': dict(tetm='p', theta=0.0, top=1.0, bot=2.25, M=10, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+    'tm_oblique': dict(tetm='p', theta=0.2, top=1.0, bot=2.25, M=10, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+    'te_quarter': dict(tetm='s', theta=0.15, top=1.0, bot=2.25, M=12, period=0.8, wl=0.7, eps=_QUARTER, d=[0.25]),
+    'tm_quarter': dict(tetm='p', theta=0.15, top=1.0, bot=2.25, M=12, period=0.8, wl=0.7, eps=_QUARTER, d=[0.25]),
+    'te_two_layer': dict(tetm='s', theta=0.25, top=1.0, bot=4.0, M=10, period=0.7, wl=0.85, eps=_TWO_LAYER[::-1], d=[0.22, 0.15]),
+    'tm_two_layer': dict(tetm='p', theta=0.25, top=1.0, bot=4.0, M=10, period=0.7, wl=0.85, eps=_TWO_LAYER[::-1], d=[0.22, 0.15]),
+    'te_high_order': dict(tetm='s', theta=0.2, top=1.0, bot=2.25, M=30, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+    'tm_high_order': dict(tetm='p', theta=0.2, top=1.0, bot=2.25, M=30, period=0.7, wl=0.9, eps=_HALF, d=[0.15]),
+}
+
+
+def _rand(shape, seed):
+    # tests/rcwa/test_golden_legacy_parity.py:221-229 (reference)
+    return np.random.default_rng(seed).uniform(1.0, 4.0, size=shape) ** 2
+
+
+# tests/rcwa/test_golden_legacy_parity.py:235-249 (reference); layers listed TOP to BOTTOM.
+LEGACY = {
+    'te_1layer': dict(pol='s', phi=None, eps=[[[9., 9., 9., 1., 1., 1.]]], d=[0.15], top=4., bot=1., theta=0., M=(6, 0), period=(0.77, 1.), wl=0.777),
+    'tm_1layer': dict(pol='p', phi=None, eps=[[[9., 9., 9., 1., 1., 1.]]], d=[0.15], top=4., bot=1., theta=0., M=(6, 0), period=(0.77, 1.), wl=0.777),
+    'te_oblique': dict(pol='s', phi=None, eps=[[[9., 9., 9., 1., 1., 1.]]], d=[0.2], top=1., bot=2.25, theta=0.3, M=(6, 0), period=(0.77, 1.), wl=0.777),
+    'tm_oblique': dict(pol='p', phi=None, eps=[[[9., 9., 9., 1., 1., 1.]]], d=[0.2], top=1., bot=2.25, theta=0.3, M=(6, 0), period=(0.77, 1.), wl=0.777),
+    'te_2layer': dict(pol='s', phi=None, eps=[[[9., 9., 1., 1., 9., 1.]], [[4., 4., 4., 1., 1., 1.]]], d=[0.15, 0.22], top=1., bot=1., theta=0., M=(5, 0), period=(0.7, 1.), wl=0.9),
+    'te_same_mat': dict(pol='s', phi=None, eps=[[[9., 9., 1., 1., 9., 1.]], [[1., 1., 1., 1., 1., 1.]]], d=[0.2, 0.1], top=1., bot=1., theta=0., M=(5, 0), period=(0.7, 1.), wl=0.9),
+    '2d_normal': dict(psi=0., phi=0., eps=_rand((1, 5, 5), 0), d=[0.2], top=1., bot=1., theta=0., M=(2, 2), period=(1., 1.), wl=0.9),
+    '2d_oblique': dict(psi=0.9, phi=0.4, eps=_rand((1, 6, 6), 1), d=[0.3], top=1., bot=2.25, theta=0.3, M=(2, 2), period=(1., 1.2), wl=0.8),
+    '2d_2layer': dict(psi=1.1, phi=0.5, eps=_rand((2, 4, 4), 2), d=[0.15, 0.22], top=1.44, bot=1., theta=0.2, M=(2, 2), period=(0.9, 0.9), wl=0.7),
+}
+
+
+def legacy_case(name):
+    """Inputs of a legacy case, restated bottom-to-top, and the solve's pol argument."""
+    c = LEGACY[name]
+    eps = np.asarray(c['eps'], dtype=float)[::-1]
+    d = list(c['d'])[::-1]
+    pol = c['pol'] if 'pol' in c else (np.sin(c['psi']), np.cos(c['psi']))
+    return c, eps, d, pol
+
+
+def legacy_expectations(name, res_sub, kx, sub):
+    """Golden efficiencies and converted amplitudes for one incidence variant.
+
+    Conversions restate tests/rcwa/test_golden_legacy_parity.py:166-218 (reference): the
+    amplitude factors of ``_amp_map``, the frame sign ``sigma(kx)*sigma_inc`` and the centre
+    parity ``(-1)**(m+n)``.
+    """
+    g = np.load(GOLDEN / 'legacy_subset.npz')
+    c = LEGACY[name]
+    one_d = int(g[f'{name}__grating_type']) == 0
+    n_top, n_bot = np.sqrt(c['top']), np.sqrt(c['bot'])
+    if one_d:
+        amap = (('R_s', 'r_s', 1.0), ('R_p', 'r_p', 1.0), ('T_s', 't_s', 1.0), ('T_p', 't_p', n_top / n_bot))
+    else:
+        amap = (('R_s', 'r_s', 1.0), ('R_p', 'r_p', 1 / (-1j * n_top)), ('T_s', 't_s', 1.0), ('T_p', 't_p', 1 / (-1j * n_bot)))
+    kxr = np.real(kx)
+    sigma = np.where(kxr < 0, -1.0, 1.0)
+    frame = (sigma * sigma[kxr.size // 2]).reshape(1, -1)
+    hu, hv = c['M']
+    m, n = np.arange(-hu, hu + 1), np.arange(-hv, hv + 1)
+    centre = np.where((n[:, None] + m[None, :]) % 2 == 1, -1.0, 1.0)
+    effs = {k: g[f'{name}__{sub}__{gk}'] for gk, k in (('de_ri', 'R'), ('de_ri_s', 'R_s'), ('de_ri_p', 'R_p'),
+                                                       ('de_ti', 'T'), ('de_ti_s', 'T_s'), ('de_ti_p', 'T_p'))}
+    amps = {attr: g[f'{name}__{sub}__{key}'] * f * frame * centre for key, attr, f in amap}
+    return effs, amps
+
+
+def test_series_is_exact_pixel_integral():
+    """Sampled coefficients equal the exact integral of the staircase (closed form), any order,
+    including |k| > n where the DFT has wrapped (factorization.py:289-311)."""
+    rng = np.random.default_rng(3)
+    a = rng.uniform(1, 5, 7) + 1j * rng.uniform(0, 1, 7)
+    n = a.size
+    half = 6
+    got = orc.raster_series(a, half)
+    for i, k in enumerate(range(-2 * half, 2 * half + 1)):
+        if k == 0:
+            want = a.mean()
+        else:
+            j = np.arange(n)
+            want = np.sum(a * (np.exp(-2j * np.pi * k * j / n) - np.exp(-2j * np.pi * k * (j + 1) / n))
+                          / (2j * np.pi * k))
+        assert abs(got[i] - want) < 1e-14, (k, got[i], want)
+
+
+def test_uniform_layer_is_exactly_eps_identity():
+    eps = np.full((7, 9), 3.3 + 0.2j)
+    epx, epy, epz_i = orc.li_conv_matrices(eps, 3, 2)
+    N = 35
+    assert np.abs(epx - eps[0, 0] * np.eye(N)).max() < 1e-14
+    assert np.abs(epy - eps[0, 0] * np.eye(N)).max() < 1e-14
+    assert np.abs(epz_i - np.eye(N) / eps[0, 0]).max() < 1e-14
+
+
+def test_y_invariant_layer_reduces_to_1d_li():
+    """On a y-invariant raster epx is the 1D inverse rule per v order and epy the direct rule
+    (factorization.py:610-617)."""
+    row = np.array([2.0, 2.0, 7.0, 7.0, 7.0, 1.5])
+    eps = np.tile(row, (4, 1))
+    hu, hv = 3, 1
+    epx, epy, _ = orc.li_conv_matrices(eps, hu, hv)
+    tp = orc.toeplitz_positions(hu)
+    nu = 2 * hu + 1
+    inv_rule = np.linalg.inv(orc.raster_series(1 / row, hu)[tp].reshape(nu, nu))
+    direct = orc.raster_series(row, hu)[tp].reshape(nu, nu)
+    for b in range(2 * hv + 1):
+        sl = slice(b * nu, (b + 1) * nu)
+        assert np.abs(epx[sl, sl] - inv_rule).max() < 1e-13
+        assert np.abs(epy[sl, sl] - direct).max() < 1e-13
+
+
+@pytest.mark.parametrize('name', list(RETICOLO))
+def test_oracle_matches_reticolo_1d(name):
+    c = RETICOLO[name]
+    g = np.load(GOLDEN / 'reticolo_1d.npz')
+    res = orc.solve_raster(c['eps'], c['d'], c['top'], c['bot'], (c['period'], 1.0), c['wl'],
+                           c['theta'], None, c['tetm'], (c['M'], 0))
+    for side, arr in (('ri', res['incident']['R']), ('ti', res['incident']['T'])):
+        idx = g[f'{name}__idx_{side}']
+        want = g[f'{name}__de_{side}']
+        got = np.real(arr).ravel()[idx]
+        assert np.abs(got - want).max() < 1e-11, (name, side, np.abs(got - want).max())
+        rest = np.delete(np.real(arr).ravel(), idx)
+        assert np.abs(rest).max() < 1e-12
+
+
+@pytest.mark.parametrize('name', list(LEGACY))
+def test_oracle_matches_legacy_golden(name):
+    c, eps, d, pol = legacy_case(name)
+    res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'],
+                           pol, c['M'])
+    hu, hv = c['M']
+    kx = (orc.incident_kxy(c['top'], c['theta'], c['phi'])[0]
+          + np.arange(-hu, hu + 1) * c['wl'] / c['period'][0])
+    subs = (('res', 'incident'),) if res['path'].startswith('1d') else (
+        ('res', 'incident'), ('res_te_inc', 's_inc'), ('res_tm_inc', 'p_inc'))
+    for sub, attr in subs:
+        effs, amps = legacy_expectations(name, None, kx, sub)
+        got = res[attr]
+        for k, want in effs.items():
+            assert np.abs(np.reshape(want, got[k].shape) - got[k]).max() < 1e-12, (sub, k)
+        for k, want in amps.items():
+            assert np.abs(np.reshape(want, got[k].shape) - got[k]).max() < 1e-12, (sub, k)
+
+
+@pytest.mark.parametrize('name', ['2d_normal', '2d_oblique', '2d_2layer'])
+def test_oracle_epz_i_and_q_match_legacy(name):
+    g = np.load(GOLDEN / 'legacy_subset.npz')
+    c, eps, d, pol = legacy_case(name)
+    hu, hv = c['M']
+    res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'],
+                           pol, c['M'])
+    conv = [orc.li_conv_matrices(e, hu, hv) for e in eps]
+    kx, ky = orc.order_kxy_2d(c['wl'], c['period'], c['top'], c['theta'], c['phi'], hu, hv)
+    kxf = np.broadcast_to(kx[None, :], (2 * hv + 1, 2 * hu + 1)).reshape(-1)
+    kyf = np.broadcast_to(ky[:, None], (2 * hv + 1, 2 * hu + 1)).reshape(-1)
+    L = len(d)
+    for i in range(L):
+        # golden layers are in the legacy (top-to-bottom) kernel order: layer i of the golden is
+        # our layer L-1-i? The legacy kernel listed layers in its own order; compare as sets.
+        want_epz = [g[f'{name}__layer{j}__epz_i'] for j in range(L)]
+        assert min(np.abs(w - conv[i][2]).max() for w in want_epz) < 1e-12
+        W, V, q, _, _ = orc.layer_modes_2d(kxf, kyf, *conv[i])
+        want_q = [np.sort(np.abs(g[f'{name}__layer{j}__q'])) for j in range(L)]
+        assert min(np.abs(w - np.sort(np.abs(q))).max() for w in want_q) < 1e-10
+    assert res['path'] == '2d'
+
+
+def test_oracle_matches_reticolo_li_nesting_totals():
+    """2D crossed grating vs RETICOLO (fixed nesting), tests/references/test_reticolo_li_nesting.py."""
+    spec = json.loads((GOLDEN / 'li_nesting_fixed.json').read_text())
+    n, period = 650, 325.0
+    pixel = period / n
+    x0, x1 = round((period - 200.0) / 2 / pixel), round((period + 200.0) / 2 / pixel)
+    y0, y1 = round((period - 150.0) / 2 / pixel), round((period + 150.0) / 2 / pixel)
+    eps = np.ones((1, n, n))
+    eps[0, y0:y1, x0:x1] = 6.0
+    for M, (te, tm) in spec['total_T_te_tm'].items():
+        M = int(M)
+        for psi, want in ((np.pi / 2, te), (0.0, tm)):
+            res = orc.solve_raster(eps, [600.0], 1.0, 1.46 ** 2, (period, period), 532.0, 0.0, 0.0,
+                                   (np.sin(psi), np.cos(psi)), (M, M))
+            got = float(np.sum(res['incident']['T']))
+            assert abs(got - want) < spec['tolerance'], (M, psi, got, want)
+
+
+def test_uniform_slab_matches_airy():
+    """Uniform lossy slab on a substrate: zeroth-order R/T equal the Airy closed form."""
+    eps_l, eps_sub, d, wl, theta = 4.0 + 0.3j, 2.25, 0.37, 0.8, 0.4
+    eps = np.full((1, 3, 3), eps_l)
+    for pol, name in (((1.0, 0.0), 's'), ((0.0, 1.0), 'p')):
+        res = orc.solve_raster(eps, [d], 1.0, eps_sub, (0.5, 0.5), wl, theta, 0.0, pol, (1, 1))
+        k0 = 2 * np.pi / wl
+        kx = np.sin(theta)
+        kz = [np.sqrt(e - kx ** 2 + 0j) for e in (1.0, eps_l, eps_sub)]
+        kz = [k if (k.real + k.imag) >= 0 else -k for k in kz]
+        eps_list = (1.0, eps_l, eps_sub)
+        adm = kz if name == 's' else [k / e for k, e in zip(kz, eps_list)]
+        r01 = (adm[0] - adm[1]) / (adm[0] + adm[1])
+        r12 = (adm[1] - adm[2]) / (adm[1] + adm[2])
+        t01 = 2 * adm[0] / (adm[0] + adm[1])
+        t12 = 2 * adm[1] / (adm[1] + adm[2])
+        ph = np.exp(1j * k0 * kz[1] * d)
+        r = (r01 + r12 * ph ** 2) / (1 + r01 * r12 * ph ** 2)
+        t = t01 * t12 * ph / (1 + r01 * r12 * ph ** 2)
+        R = abs(r) ** 2
+        # s: t is the E ratio, power ~ |E|^2 Re(kz); p: t is the H ratio, power ~ |H|^2 Re(kz/eps)
+        T = abs(t) ** 2 * adm[2].real / adm[0].real
+        got = res['incident']
+        c = got['R'].shape[0] // 2, got['R'].shape[1] // 2
+        assert abs(got['R'][c] - R) < 1e-12, (name, got['R'][c], R)
+        assert abs(got['T'][c] - T) < 1e-12, (name, got['T'][c], T)
+
+
+def test_lossless_2d_stack_conserves_energy():
+    rng = np.random.default_rng(11)
+    eps = rng.choice([1.0, 12.25], size=(2, 16, 16))
+    res = orc.solve_raster(eps, [0.2, 0.31], 1.0, 2.25, (1.0, 1.0), 0.83, 0.3, 1.1, 'p', (3, 3))
+    for k in ('incident', 's_inc', 'p_inc'):
+        tot = res[k]['R'].sum() + res[k]['T'].sum()
+        assert abs(tot - 1) < 1e-10, (k, tot)
+
+
+def test_modes_satisfy_both_maxwell_factors():
+    """Omega2 W = W q^2 and Omega_R W = V q (tests/rcwa/test_solve_2d_core.py:195 idea)."""
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(1, 6, size=(8, 8)) + 1j * rng.uniform(0, 0.3, size=(8, 8))
+    epx, epy, epz_i = orc.li_conv_matrices(eps, 2, 2)
+    kx, ky = orc.order_kxy_2d(0.7, (1.0, 1.0), 1.0, 0.2, 0.3, 2, 2)
+    kxf = np.broadcast_to(kx[None, :], (5, 5)).reshape(-1)
+    kyf = np.broadcast_to(ky[:, None], (5, 5)).reshape(-1)
+    W, V, q, O2, OR = orc.layer_modes_2d(kxf, kyf, epx, epy, epz_i)
+    assert np.abs(O2 @ W - W * q ** 2).max() < 1e-11 * np.abs(O2).max()
+    assert np.abs(OR @ W - V * q).max() < 1e-11 * np.abs(OR).max()
+    assert np.all(q.real >= 0)
