@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: RCWA solves/s at orders (9,9), complex FP64, batched vs plain launch, on one MI355X.
+
+Workload (synthetic, seeded): 4 layers of 128x128 continuous-valued permittivity rasters
+(eps U[1,12]), 200 nm each, square period 1000 nm, air over an eps 2.25 substrate; orders
+(9,9) = 361 harmonics, 722x722 complex128 layer eigenproblems; 64 normal-incidence p-polarized
+points at wavelengths evenly spread over [500, 900] nm.
+
+A step is one batched solve of all 64 points from the device-resident rasters: Li-rule
+convolution-matrix assembly, 256 layer eigensolves, gap-embedded layer S-matrices, the cascade
+and the per-order efficiencies, in one launch sequence. ``value`` is points solved per second
+(one solve = one (problem, wavelength, angle, polarization) point). The plain form solves the
+same points one point per launch sequence; its rate and the ratio are reported beside it.
+
+Also reported: the dominant kernel's roofline (algorithmic flops per launch / average launch
+time, from HIP events recorded on the launching stream during the timed steps) and the NumPy
+FP64 oracle timed on the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--points', type=int, default=64)
+    ap.add_argument('--plain-points', type=int, default=2)
+    ap.add_argument('--cpu-points', type=int, default=1)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--orders', type=int, default=9)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the points of a batch are '
+              'independent, so a multi-GPU run would shard them with no exchange -- not built '
+              'in this engine.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from wavebit_amd import _native, workloads
+    from wavebit_amd.rcwa import _engine as eng
+
+    dev = torch.device('cuda:0')
+    M = args.orders
+    wl = workloads.continuous_stack(wavelengths=args.points, orders=(M, M))
+    hu, hv = wl['orders']
+    L = wl['rasters'].shape[0]
+    P = len(wl['points'])
+    N = (2 * hu + 1) * (2 * hv + 1)
+
+    def records(points):
+        recs = []
+        for p in points:
+            pt = eng.Point(stack=0, wavelength=p['wavelength'], theta=p['theta'], phi=p['phi'],
+                           polarization=p['polarization'], eps_top=wl['eps_top'],
+                           eps_bottom=wl['eps_bottom'], period=wl['period'],
+                           thickness=tuple(wl['thickness']))
+            recs.append(eng.point_record(pt, eng.PATH_2D))
+        return torch.tensor(np.array(recs), dtype=torch.float64, device=dev)
+
+    eps_d = torch.from_numpy(wl['rasters']).to(dev)
+    pts_d = records(wl['points'])
+    thick_d = torch.tensor(np.tile(wl['thickness'], (P, 1)), dtype=torch.float64, device=dev)
+    idx_d = torch.tensor(np.tile(np.arange(L, dtype=np.int32), (P, 1)), device=dev)
+    ws = torch.empty(_native.solve_workspace_bytes(P, L, hu, hv, _native.MODE_2D), dtype=torch.uint8,
+                     device=dev)
+
+    def step(pts, thick, idx, workspace):
+        epx, epy, epz, info = _native.assemble_li(eps_d, hu, hv)
+        return _native.solve(pts, thick, idx, epx, epy, epz, hu, hv, _native.MODE_2D, workspace)
+
+    for _ in range(args.warmup):
+        step(pts_d, thick_d, idx_d, ws)
+    torch.cuda.synchronize()
+    names = list(_native.TIMER_IDS)
+    _native.timing_enable(True)
+    for n_ in names:
+        _native.timing_read(n_, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(pts_d, thick_d, idx_d, ws)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _native.timing_enable(False)
+    kernels = {}
+    for n_ in names:
+        ms, cnt, fl = _native.timing_read(n_, reset=True)
+        if cnt:
+            kernels[n_] = dict(total_ms=ms, launches=cnt, flops=fl)
+    eff, amp, pe, flags = out
+    n_flagged = int((flags != 0).sum())
+    batched_rate = P * args.steps / dt
+
+    # plain: one point per launch sequence
+    plain_n = max(1, args.plain_points)
+    ws1 = torch.empty(_native.solve_workspace_bytes(1, L, hu, hv, _native.MODE_2D), dtype=torch.uint8,
+                      device=dev)
+    step(pts_d[:1].contiguous(), thick_d[:1].contiguous(), idx_d[:1].contiguous(), ws1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(plain_n):
+        step(pts_d[i:i + 1].contiguous(), thick_d[i:i + 1].contiguous(), idx_d[i:i + 1].contiguous(), ws1)
+    torch.cuda.synchronize()
+    plain_rate = plain_n / (time.perf_counter() - t0)
+
+    # dominant kernel roofline
+    dom = max(kernels.items(), key=lambda kv: kv[1]['total_ms']) if kernels else None
+    roofline = None
+    if dom is not None:
+        name, k = dom
+        avg_s = k['total_ms'] / k['launches'] / 1e3
+        flops_per_launch = k['flops'] / k['launches']
+        achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+        roofline = {'bound': 'mfma', 'kernel': name, 'achieved': round(achieved, 4),
+                    'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                    'frac': round(achieved / PEAK_FP64_TFLOPS, 5), 'traffic': None,
+                    'avg_launch_ms': round(k['total_ms'] / k['launches'], 4),
+                    'flops_per_launch': flops_per_launch,
+                    'share_of_step': round(k['total_ms'] / 1e3 / dt, 4)}
+
+    # CPU baseline (oracle, NumPy FP64) on a bounded sample; parity of the same points
+    cpu = None
+    oracle_err = None
+    if not args.no_cpu and args.cpu_points > 0:
+        from oracle import rcwa_oracle as orc
+        n_cpu = min(args.cpu_points, P)
+        effh = eff.cpu().numpy()
+        t0 = time.perf_counter()
+        errs = []
+        for i in range(n_cpu):
+            p = wl['points'][i]
+            ref = orc.solve_raster(wl['rasters'], wl['thickness'], wl['eps_top'], wl['eps_bottom'],
+                                   wl['period'], p['wavelength'], p['theta'], p['phi'],
+                                   p['polarization'], (hu, hv))
+            got = eng.unpack(effh, amp.cpu().numpy(), pe.cpu().numpy(), i, hu, hv, eng.PATH_2D)
+            errs.append(max(float(np.abs(got['incident'][k] - ref['incident'][k]).max())
+                            for k in ('R', 'T')))
+        cpu_dt = time.perf_counter() - t0
+        cpu = {'value': round(n_cpu / cpu_dt, 6), 'unit': 'solves/s', 'cores': os.cpu_count(),
+               'kind': 'port',
+               'sample': f'{n_cpu} point(s) of the same workload (4 layers, orders (9,9)) through '
+                         f'the NumPy FP64 oracle, {cpu_dt:.1f} s'}
+        oracle_err = max(errs)
+
+    fl_pt = workloads.flops_per_point((hu, hv), L)
+    line = {
+        'metric': 'RCWA solves/s @ (9,9) orders FP64 vs plain launch; p50/p99 serve latency',
+        'value': round(batched_rate, 4), 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(dt / args.steps * 1e3, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'c128',
+        'data': 'synthetic',
+        'config': {'workload': f'{L} layers 128x128 continuous eps U[1,12], 200 nm, period 1000 nm, '
+                               f'orders ({hu},{hv}) = {N} harmonics (n={2 * N}), {P} wavelengths '
+                               '[500,900] nm, normal incidence, p-pol, one batched launch sequence',
+                   'points_per_step': P, 'layers': L, 'orders': [hu, hv],
+                   'parallelism': 'single GPU, batched points'},
+        'plain_solves_per_s': round(plain_rate, 4),
+        'batched_vs_plain': round(batched_rate / plain_rate, 3),
+        'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
+        'roofline': roofline, 'cpu_baseline': cpu,
+        'oracle_max_abs_err_RT': oracle_err, 'flagged_points': n_flagged,
+        'kernels_ms': {k: round(v['total_ms'] / args.steps, 3) for k, v in kernels.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -354,6 +354,8 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
                    R_s=np.zeros(N), R_p=de_ri, T_s=np.zeros(N), T_p=de_ti)
     out['R'] = out['R_s'] + out['R_p']
     out['T'] = out['T_s'] + out['T_p']
+    # power entering the lit face (observations/power.py:37-48 net_power, summed / incoming)
+    out['power_entering'] = float(np.sum(((delta + R) * np.conj(adm_ref * (delta - R))).real) / incoming)
     return out
 
 
@@ -472,8 +474,10 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
             tetm = 0 if polarization == 's' else 1
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
                          theta, hu, phi)
-            return {'path': '1d_te' if tetm == 0 else '1d_tm',
-                    'incident': {k: np.asarray(v).reshape(1, -1) for k, v in out.items() if k != 'kx'}}
+            inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
+                   if k not in ('kx', 'power_entering')}
+            inc['power_entering'] = out['power_entering']
+            return {'path': '1d_te' if tetm == 0 else '1d_tm', 'incident': inc}
         pol = (1.0, 0.0) if polarization == 's' else (0.0, 1.0)
     else:
         pol = np.asarray(polarization, dtype=np.complex128)

@@ -1,0 +1,153 @@
+"""Lifecycle and admission on the CPU: declarations, error classes and codes, the C-ABI exports.
+
+No GPU work here: these mirror the reference's API tests (tests/api/test_unified_rcwa.py,
+tests/api/test_error_codes.py) for the raster path this engine implements.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import wavebit_amd
+from wavebit_amd import rcwa, _native
+from wavebit_amd.errors import IncompatibleProblemError, UnsupportedFeatureError, ValidationError
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _problem(eps=None, wavelength=0.8, pol='p', theta=0.2, phi=0.3, layers=1):
+    eps = np.full((layers, 4, 4), 4.0) if eps is None else eps
+    return rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.2, epsilon=e) for e in eps],
+                         top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25),
+                         period=(1.0, 1.0)),
+        excitation=rcwa.Excitation(wavelength=wavelength,
+                                   sources=(rcwa.PlaneWave(polarization=pol, theta=theta, phi=phi),)))
+
+
+def test_header_declares_exactly_the_library_exports():
+    header = (ROOT / 'include' / 'wavebit_amd.h').read_text()
+    declared = set(re.findall(r'\b(wb_[a-z_]+)\s*\(', header))
+    assert declared == set(_native.EXPORTS)
+    lib = _native.load_library()
+    for name in _native.EXPORTS:
+        assert hasattr(lib, name), name
+    assert lib.wb_version() == 1
+
+
+def test_workspace_queries_need_no_gpu():
+    lib = _native.load_library()
+    assert lib.wb_zgeev_workspace_size(100, 4) > 2 * 100 * 100 * 16 * 4
+    assert lib.wb_solve_workspace_size(2, 3, 2, 2, 0) > 0
+    assert lib.wb_solve_workspace_size(0, 3, 2, 2, 0) == 0
+    assert lib.wb_assemble_li_workspace_size(1, 8, 8, 3, 3) > 0
+
+
+def test_check_reports_estimates_without_solving():
+    rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(3, 3)), backend='torch', device='cuda')
+    assert rep.ok
+    assert rep.estimates['retained_spatial_modes'] == 49
+    assert rep.resolved['solve_path'] == '2d'
+
+
+def test_one_row_raster_with_a_polarization_word_routes_to_the_pure_1d_path():
+    p = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.3, epsilon=[[1, 1, 12.25, 12.25]])],
+                         bottom=wavebit_amd.Material(2.25), period=(0.7, 0.7)),
+        excitation=rcwa.Excitation(wavelength=0.9, sources=(rcwa.PlaneWave(polarization='s', theta=0.0),)))
+    rep = rcwa.check(p, numerics=rcwa.Numerics(orders=(10, 0)), backend='torch', device='cuda')
+    assert rep.ok and rep.resolved['solve_path'] == '1d_te'
+
+
+@pytest.mark.parametrize('kwargs,code', [
+    (dict(backend='numpy', device='cpu'), 'rcwa.backend'),
+    (dict(backend='torch', device='cpu'), 'rcwa.device'),
+    (dict(backend='torch', device='cuda', precision='single'), 'rcwa.precision'),
+])
+def test_cpu_backends_are_refused_not_rerouted(kwargs, code):
+    rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(1, 1)), **kwargs)
+    assert not rep.ok and rep.issues[0].code == code
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(_problem(), numerics=rcwa.Numerics(orders=(1, 1)), **kwargs)
+    assert e.value.code == code
+
+
+@pytest.mark.parametrize('numerics', [rcwa.Numerics(orders=(1, 1), algorithm='etm'),
+                                      rcwa.Numerics(orders=(1, 1), factorization='nvf'),
+                                      rcwa.Numerics(orders=(1, 1), li_nesting='symmetric'),
+                                      rcwa.Numerics(orders=(1, 1), fourier_coefficients='integrated')])
+def test_non_default_numerics_are_unsupported(numerics):
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert e.value.code == 'rcwa.unsupported'
+
+
+def test_declaration_errors_carry_codes():
+    with pytest.raises(ValidationError) as e:
+        rcwa.Numerics(orders=(1, -1))
+    assert e.value.code == 'numerics.orders'
+    with pytest.raises(ValidationError) as e:
+        rcwa.PlaneWave(polarization='x')
+    assert e.value.code == 'source.polarization'
+    with pytest.raises(ValidationError) as e:
+        rcwa.Layer(thickness=1.0, epsilon=[[1.0]], background=wavebit_amd.Material(2.0))
+    assert e.value.code == 'layer.ambiguous'
+    with pytest.raises(ValidationError) as e:
+        rcwa.Stack(period=(1, 1), lattice=((1, 0), (0, 1)))
+    assert e.value.code == 'stack.cell_ambiguous'
+
+
+def test_y_orders_on_a_one_row_raster_are_refused():
+    p = _problem(eps=np.full((1, 1, 4), 3.0))
+    rep = rcwa.check(p, numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda')
+    assert not rep.ok and rep.issues[0].code == 'rcwa.invalid'
+
+
+def test_mixed_raster_and_uniform_layers_are_refused():
+    p = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.1, epsilon=[[2.0, 3.0]]),
+                                 rcwa.Layer(thickness=0.1, background=wavebit_amd.Material(2.0))]),
+        excitation=rcwa.Excitation(wavelength=0.5, sources=(rcwa.PlaneWave(),)))
+    rep = rcwa.check(p, numerics=rcwa.Numerics(orders=(1, 0)), backend='torch', device='cuda')
+    assert not rep.ok and rep.issues[0].code == 'rcwa.mixed_layer_encoding'
+
+
+def test_runtime_guards_refuse_nonfinite_and_negative():
+    prep_kw = dict(numerics=rcwa.Numerics(orders=(1, 1)), backend='torch', device='cuda')
+    bad = _problem(eps=np.array([[[np.nan, 1.0], [1.0, 1.0]]]))
+    with pytest.raises(ValidationError) as e:
+        rcwa.prepare(bad, **prep_kw)
+    assert e.value.code == 'prepared.finite_inputs'
+
+
+def test_prepared_refuses_a_structural_change():
+    prep = rcwa.prepare(_problem(), numerics=rcwa.Numerics(orders=(1, 1)), backend='torch', device='cuda')
+    other = _problem(layers=2)
+    with pytest.raises(IncompatibleProblemError) as e:
+        prep.run(other)
+    assert e.value.code == 'prepared.structure'
+    rep = prep.validate(other)
+    assert not rep.ok and rep.issues[0].code == 'prepared.structure'
+
+
+def test_engine_fails_loudly_without_a_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip('a GPU is visible')
+    prep = rcwa.prepare(_problem(), numerics=rcwa.Numerics(orders=(1, 1)), backend='torch', device='cuda')
+    with pytest.raises(_native.EngineUnavailable):
+        prep.run(_problem())
+
+
+def test_incident_kxy_follows_the_reference_rules():
+    from wavebit_amd.rcwa import _engine as eng
+    kx, ky = eng.incident_kxy(2.25, 0.3, 0.4)
+    assert abs(kx - 1.5 * np.sin(0.3) * np.cos(0.4)) < 1e-15
+    assert abs(ky - 1.5 * np.sin(0.3) * np.sin(0.4)) < 1e-15
+    with pytest.raises(ValueError):
+        eng.incident_kxy(-1.0, 0.1, 0.0)
+    # grazing is held just below pi/2 (single-precision bound)
+    kx, _ = eng.incident_kxy(1.0, float(np.float32(np.pi / 2)), 0.0)
+    assert 0.999999 < kx < 1.0

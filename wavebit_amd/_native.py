@@ -1,0 +1,273 @@
+"""ctypes binding of the HIP engine's C ABI (``include/wavebit_amd.h``).
+
+Device memory and streams come from PyTorch-ROCm (``torch.cuda`` is HIP on ROCm); every
+buffer handed across the boundary is a contiguous tensor on the engine's device, complex128
+as interleaved doubles. There is no fallback: if ``libwavebit_hip.so`` is missing or no GPU is
+visible, these functions raise.
+"""
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+_LIB_PATH = Path(__file__).resolve().parent / 'libwavebit_hip.so'
+_lock = threading.Lock()
+_lib = None
+_ctxs = {}
+
+WB_OK, WB_BADARG, WB_OOM, WB_SINGULAR, WB_NONFINITE, WB_NOCONVERGE, WB_FLAGGED, WB_HIP = range(8)
+MODE_2D, MODE_1D_TE, MODE_1D_TM = 0, 1, 2
+FLAG_NOCONVERGE, FLAG_SINGULAR = 1, 2
+PT_STRIDE = 16
+
+EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb_zgemm_batched',
+           'wb_zgetrf_batched', 'wb_zgetrs_batched', 'wb_zgeev_workspace_size',
+           'wb_zgeev_batched', 'wb_assemble_li_workspace_size', 'wb_assemble_li',
+           'wb_solve_workspace_size', 'wb_solve', 'wb_timing_enable', 'wb_timing_read')
+
+TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
+             'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9}
+
+
+class EngineUnavailable(RuntimeError):
+    """The HIP engine cannot run here (library not built, or no GPU visible)."""
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load_library():
+    """Load ``libwavebit_hip.so`` and declare its signatures (no GPU needed)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not _LIB_PATH.exists():
+            raise EngineUnavailable(
+                f'{_LIB_PATH} is not built; run `python -m wavebit_amd.build` (hipcc, gfx950)')
+        lib = ctypes.CDLL(str(_LIB_PATH))
+        c_int, c_ll, c_sz, c_vp, c_dp = (ctypes.c_int, ctypes.c_longlong, ctypes.c_size_t,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_double))
+        sig = {
+            'wb_version': (c_int, []),
+            'wb_ctx_create': (c_int, [c_int, ctypes.POINTER(c_vp)]),
+            'wb_ctx_destroy': (c_int, [c_vp]),
+            'wb_last_error': (ctypes.c_char_p, [c_vp]),
+            'wb_zgemm_batched': (c_int, [c_vp, c_int, c_int, c_int, c_dp, c_vp, c_int, c_ll, c_vp,
+                                         c_int, c_ll, c_dp, c_vp, c_int, c_ll, c_int, c_vp]),
+            'wb_zgetrf_batched': (c_int, [c_vp, c_int, c_vp, c_int, c_ll, c_vp, c_vp, c_int, c_vp]),
+            'wb_zgetrs_batched': (c_int, [c_vp, c_int, c_int, c_vp, c_int, c_ll, c_vp, c_vp, c_int,
+                                          c_ll, c_int, c_vp]),
+            'wb_zgeev_workspace_size': (c_sz, [c_int, c_int]),
+            'wb_zgeev_batched': (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp, c_int,
+                                         c_vp]),
+            'wb_assemble_li_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_assemble_li': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp,
+                                       c_vp, c_vp, c_sz, c_vp, c_vp]),
+            'wb_solve_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_solve': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_timing_enable': (c_int, [c_int]),
+            'wb_timing_read': (c_int, [c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(c_ll), ctypes.POINTER(ctypes.c_double),
+                                       c_int]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+class NativeError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f'wavebit_amd status {status}: {message}')
+        self.status = status
+
+
+class Context:
+    """One ``wb_ctx`` per device, created on first use and kept for the process."""
+
+    def __init__(self, device):
+        self.lib = load_library()
+        self.device = device
+        handle = ctypes.c_void_p()
+        status = self.lib.wb_ctx_create(device, ctypes.byref(handle))
+        if status != WB_OK:
+            raise EngineUnavailable(f'wb_ctx_create({device}) failed with status {status}')
+        self.handle = handle
+
+    def check(self, status, what):
+        if status != WB_OK:
+            msg = self.lib.wb_last_error(self.handle)
+            raise NativeError(status, f'{what}: {msg.decode() if msg else ""}')
+
+
+def device_index(device):
+    device = torch.device(device)
+    if device.type != 'cuda':
+        raise EngineUnavailable(f'the HIP engine runs on a GPU device, got {device}')
+    return torch.cuda.current_device() if device.index is None else device.index
+
+
+def context(device):
+    if not torch.cuda.is_available():
+        raise EngineUnavailable('no GPU is visible to PyTorch-ROCm; the HIP engine cannot run')
+    idx = device_index(device)
+    with _lock:
+        ctx = _ctxs.get(idx)
+    if ctx is None:
+        ctx = Context(idx)
+        with _lock:
+            _ctxs.setdefault(idx, ctx)
+            ctx = _ctxs[idx]
+    return ctx
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _require(t, dtype, device, name):
+    if t.dtype != dtype or t.device != device or not t.is_contiguous():
+        raise ValueError(f'{name}: expected a contiguous {dtype} tensor on {device}, got '
+                         f'{t.dtype} on {t.device} (contiguous={t.is_contiguous()})')
+
+
+def _workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+# ----------------------------------------------------------------------------- primitives
+
+def zgemm(A, B, C=None, alpha=1.0, beta=0.0):
+    """Batched ``alpha A @ B + beta C`` for (batch, m, k) x (batch, k, n) complex128."""
+    device = A.device
+    ctx = context(device)
+    _require(A, torch.complex128, device, 'A')
+    _require(B, torch.complex128, device, 'B')
+    batch, m, k = A.shape
+    n = B.shape[-1]
+    if B.shape != (batch, k, n):
+        raise ValueError(f'shape mismatch {tuple(A.shape)} @ {tuple(B.shape)}')
+    if C is None:
+        C = torch.zeros((batch, m, n), dtype=torch.complex128, device=device)
+        beta = 0.0
+    _require(C, torch.complex128, device, 'C')
+    al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+    ctx.check(ctx.lib.wb_zgemm_batched(ctx.handle, m, n, k, al, _ptr(A), k, m * k, _ptr(B), n,
+                                       k * n, be, _ptr(C), n, m * n, batch, _stream(device)),
+              'zgemm')
+    return C
+
+
+def zgetrf(A):
+    """In-place batched LU with partial pivoting: returns (A, ipiv, info)."""
+    device = A.device
+    ctx = context(device)
+    _require(A, torch.complex128, device, 'A')
+    batch, n, _ = A.shape
+    ipiv = torch.empty((batch, n), dtype=torch.int32, device=device)
+    info = torch.empty((batch,), dtype=torch.int32, device=device)
+    ctx.check(ctx.lib.wb_zgetrf_batched(ctx.handle, n, _ptr(A), n, n * n, _ptr(ipiv), _ptr(info),
+                                        batch, _stream(device)), 'zgetrf')
+    return A, ipiv, info
+
+
+def zgetrs(LU, ipiv, B):
+    """Solve ``A X = B`` in place in B (batch, n, nrhs) from ``zgetrf`` factors."""
+    device = LU.device
+    ctx = context(device)
+    _require(B, torch.complex128, device, 'B')
+    batch, n, _ = LU.shape
+    nrhs = B.shape[-1]
+    ctx.check(ctx.lib.wb_zgetrs_batched(ctx.handle, n, nrhs, _ptr(LU), n, n * n, _ptr(ipiv),
+                                        _ptr(B), nrhs, n * nrhs, batch, _stream(device)), 'zgetrs')
+    return B
+
+
+def zgeev(A):
+    """Batched general eigendecomposition; A (batch, n, n) is consumed. Returns (w, W, info)."""
+    device = A.device
+    ctx = context(device)
+    _require(A, torch.complex128, device, 'A')
+    batch, n, _ = A.shape
+    w = torch.empty((batch, n), dtype=torch.complex128, device=device)
+    W = torch.empty((batch, n, n), dtype=torch.complex128, device=device)
+    info = torch.empty((batch,), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_zgeev_workspace_size(n, batch)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_zgeev_batched(ctx.handle, n, _ptr(A), _ptr(w), _ptr(W), _ptr(ws), nbytes,
+                                       _ptr(info), batch, _stream(device)), 'zgeev')
+    return w, W, info
+
+
+def assemble_li(eps, hu, hv):
+    """Li convolution matrices of a raster stack eps (L, Ny, Nx) -> (epx, epy, epz_i, info)."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    L, Ny, Nx = eps.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    out = [torch.empty((L, N, N), dtype=torch.complex128, device=device) for _ in range(3)]
+    info = torch.empty((2 * L,), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_assemble_li_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_assemble_li(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, _ptr(out[0]),
+                                     _ptr(out[1]), _ptr(out[2]), _ptr(ws), nbytes, _ptr(info),
+                                     _stream(device)), 'assemble_li')
+    return out[0], out[1], out[2], info
+
+
+def solve_workspace_bytes(P, L, hu, hv, mode):
+    return int(load_library().wb_solve_workspace_size(P, L, hu, hv, mode))
+
+
+def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
+    """Batch RCWA solve; see ``wb_solve``. Returns (eff, amp, pe, flags)."""
+    device = epx.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
+                               _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(eff),
+                               _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
+                               _stream(device)), 'solve')
+    return eff, amp, pe, flags
+
+
+# ----------------------------------------------------------------------------- timing
+
+def timing_enable(on=True):
+    load_library().wb_timing_enable(1 if on else 0)
+
+
+def timing_read(name, reset=True):
+    """(total milliseconds, launch count, algorithmic flops) of kernel ``name`` since reset."""
+    lib = load_library()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_longlong()
+    fl = ctypes.c_double()
+    lib.wb_timing_read(TIMER_IDS[name], ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(fl),
+                       1 if reset else 0)
+    return ms.value, cnt.value, fl.value

@@ -1,0 +1,175 @@
+// C ABI of the wavebit_amd engine: contexts, argument checks, workspace carving and the
+// launches behind each entry point declared in include/wavebit_amd.h.
+#include <string>
+#include <cstring>
+
+#include "wb_internal.h"
+#include "../../include/wavebit_amd.h"
+
+
+
+struct wb_ctx {
+  int device;
+  std::string error;
+};
+
+namespace {
+
+int fail(wb_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+int hip_status(wb_ctx* ctx, hipError_t e, const char* where) {
+  if (e == hipSuccess) {
+    if (ctx) ctx->error.clear();
+    return WB_OK;
+  }
+  return fail(ctx, WB_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int enter(wb_ctx* ctx) {
+  if (!ctx) return WB_BADARG;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return fail(ctx, WB_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return WB_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int wb_version(void) { return 1; }
+
+int wb_ctx_create(int device, wb_ctx** out) {
+  if (!out) return WB_BADARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return WB_BADARG;
+  wb_ctx* c = new (std::nothrow) wb_ctx;
+  if (!c) return WB_OOM;
+  c->device = device;
+  *out = c;
+  return WB_OK;
+}
+
+int wb_ctx_destroy(wb_ctx* ctx) {
+  delete ctx;
+  return WB_OK;
+}
+
+const char* wb_last_error(const wb_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int wb_zgemm_batched(wb_ctx* ctx, int m, int n, int k, const double* alpha, const double* A,
+                     int lda, long long strideA, const double* B, int ldb, long long strideB,
+                     const double* beta, double* C, int ldc, long long strideC, int batch,
+                     void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (m < 0 || n < 0 || k < 0 || batch < 0 || !alpha || !beta || lda < k || ldb < n || ldc < n)
+    return fail(ctx, WB_BADARG, "wb_zgemm_batched: bad dimensions");
+  return hip_status(ctx,
+                    wb_zgemm_launch(m, n, k, mkc(alpha[0], alpha[1]), bp_strided(A, strideA), lda,
+                                    bp_strided(B, strideB), ldb, mkc(beta[0], beta[1]),
+                                    bp_strided(C, strideC), ldc, batch, (hipStream_t)stream),
+                    "wb_zgemm_batched");
+}
+
+int wb_zgetrf_batched(wb_ctx* ctx, int n, double* A, int lda, long long strideA, int* ipiv,
+                      int* info, int batch, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (n < 0 || lda < n || batch < 0 || !ipiv || !info)
+    return fail(ctx, WB_BADARG, "wb_zgetrf_batched: bad arguments");
+  return hip_status(ctx,
+                    wb_getrf_launch(n, bp_strided(A, strideA), lda, ipiv, info, batch,
+                                    (hipStream_t)stream),
+                    "wb_zgetrf_batched");
+}
+
+int wb_zgetrs_batched(wb_ctx* ctx, int n, int nrhs, const double* LU, int lda, long long strideA,
+                      const int* ipiv, double* B, int ldb, long long strideB, int batch,
+                      void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (n < 0 || nrhs < 0 || lda < n || ldb < nrhs || batch < 0 || !ipiv)
+    return fail(ctx, WB_BADARG, "wb_zgetrs_batched: bad arguments");
+  return hip_status(ctx,
+                    wb_getrs_launch(n, nrhs, bp_strided(LU, strideA), lda, ipiv,
+                                    bp_strided(B, strideB), ldb, batch, (hipStream_t)stream),
+                    "wb_zgetrs_batched");
+}
+
+size_t wb_zgeev_workspace_size(int n, int batch) {
+  if (n <= 0 || batch <= 0) return 0;
+  const size_t nn = (size_t)n * n * sizeof(cd);
+  return 2 * align_up(nn * batch) + align_up((size_t)n * batch * sizeof(cd)) + 256;
+}
+
+int wb_zgeev_batched(wb_ctx* ctx, int n, double* A, double* w, double* W, void* workspace,
+                     size_t workspace_bytes, int* info, int batch, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (n < 0 || batch < 0 || !info) return fail(ctx, WB_BADARG, "wb_zgeev_batched: bad arguments");
+  if (n == 0 || batch == 0) return WB_OK;
+  if (workspace_bytes < wb_zgeev_workspace_size(n, batch))
+    return fail(ctx, WB_OOM, "wb_zgeev_batched: workspace too small");
+  char* p = (char*)workspace;
+  const size_t nn = (size_t)n * n * sizeof(cd);
+  cd* Z = (cd*)p; p += align_up(nn * batch);
+  cd* X = (cd*)p; p += align_up(nn * batch);
+  cd* tau = (cd*)p;
+  return hip_status(ctx,
+                    wb_zgeev_launch(n, (cd*)A, (cd*)w, (cd*)W, Z, X, tau, info, batch,
+                                    (hipStream_t)stream),
+                    "wb_zgeev_batched");
+}
+
+size_t wb_assemble_li_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0) return 0;
+  return wb_assemble_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_li(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                   double* epx, double* epy, double* epz_i, void* workspace,
+                   size_t workspace_bytes, int* info, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hv > 15 || !info)
+    return fail(ctx, WB_BADARG, "wb_assemble_li: bad dimensions (need L, Ny, Nx >= 1, 0 <= hv <= 15)");
+  if (workspace_bytes < wb_assemble_li_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_li: workspace too small");
+  return hip_status(ctx,
+                    wb_assemble_li_launch((const cd*)eps, L, Ny, Nx, hu, hv, (cd*)epx, (cd*)epy,
+                                          (cd*)epz_i, workspace, info, (hipStream_t)stream),
+                    "wb_assemble_li");
+}
+
+size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
+  return wb_solve_workspace_bytes(P, L, hu, hv, mode);
+}
+
+int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+             const double* thick, const int* conv_idx, const double* epx, const double* epy,
+             const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+             void* workspace, size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_solve: bad dimensions");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe)
+    return fail(ctx, WB_BADARG, "wb_solve: null buffer");
+  if (workspace_bytes < wb_solve_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags;
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve");
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// Li-rule convolution-matrix assembly for raster layers, sampled coefficients, fixed nesting.
+//
+// Restates, in device code of its own, reference factorization.py:289-642 (the staircase
+// series, the Toeplitz layout, the nested rule with v inside) for a stack of L rasters
+// (L, Ny, Nx) complex128, row-major:
+//   k_series_v     column series along v of eps and 1/eps: one workgroup per (column, layer,
+//                  slot), one lane per order; the direct DFT of (a - a0)/n times the exact
+//                  pixel transform sinc(k/n) exp(-i pi k/n), plus a0 at order 0.
+//   k_series_2d    the direct 2D coefficients of eps (series along u of the column series).
+//   k_inner_inv    every column's (2hv+1)^2 Toeplitz matrix inverted in LDS (Gauss-Jordan
+//                  with partial pivoting, one workgroup per column).
+//   k_outer_bttb   series along u of each inverse entry, scattered straight into the N x N
+//                  block-Toeplitz layout out[(n,m),(n',m')] = A_{m-m'}[n,n'].
+//   k_direct_bttb  the 2D direct rule M[(n,m),(n',m')] = c(n-n', m-m').
+// Then epx = inv(T_u([T_v(eps)]^-1)) and epz_i = inv(M) on the batched LU path.
+// The raster is read once per slot by coalesced column walks; the N x N outputs are written
+// once. Twiddles use exact integer phase reduction (k*j mod n) and sincospi.
+#include "wb_internal.h"
+
+
+namespace {
+
+__device__ inline cd pixel_series(const cd* vals, long long stride, int n, int k, bool recip) {
+  cd base = vals[0];
+  if (recip) base = crecip(base);
+  double sr = 0.0, si = 0.0;
+  const double inv_n = 1.0 / (double)n;
+  for (int j = 1; j < n; ++j) {
+    cd a = vals[(long long)j * stride];
+    if (recip) a = crecip(a);
+    const cd d = cscale(csub(a, base), inv_n);
+    long long idx = ((long long)k * j) % n;
+    if (idx < 0) idx += n;
+    double s, c;
+    sincospi(-2.0 * (double)idx / (double)n, &s, &c);
+    sr += d.x * c - d.y * s;
+    si += d.x * s + d.y * c;
+  }
+  cd out = mkc(sr, si);
+  if (n > 1) {
+    const double tt = (double)k / (double)n;
+    const double sinc = (k == 0) ? 1.0 : sinpi(tt) / (3.141592653589793 * tt);
+    double s, c;
+    sincospi(-tt, &s, &c);
+    out = cscale(cmul(out, mkc(c, s)), sinc);
+  } else {
+    out = mkc(0.0, 0.0);
+  }
+  if (k == 0) out = cadd(out, base);
+  return out;
+}
+
+// lines[slot][l][ju][kv], slot 0: eps, slot 1: 1/eps
+__global__ void k_series_v(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines) {
+  const int ju = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int Kv = 4 * hv + 1;
+  const cd* col = eps + (long long)l * Ny * Nx + ju;
+  for (int kk = threadIdx.x; kk < Kv; kk += blockDim.x) {
+    const int k = kk - 2 * hv;
+    lines[(((long long)slot * L + l) * Nx + ju) * Kv + kk] = pixel_series(col, Nx, Ny, k, slot == 1);
+  }
+}
+
+// c2d[l][kv][ku] = series_u over ju of lines[0][l][ju][kv]
+__global__ void k_series_2d(const cd* lines, int L, int Nx, int hu, int hv, cd* c2d) {
+  const int kv = blockIdx.x, l = blockIdx.y;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const cd* seq = lines + ((long long)l * Nx) * Kv + kv;
+  for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x)
+    c2d[((long long)l * Kv + kv) * Ku + kk] = pixel_series(seq, Kv, Nx, kk - 2 * hu, false);
+}
+
+// inner[slot][l][ju][nv*nv] = inv(Toeplitz(lines[slot][l][ju]))
+__global__ __launch_bounds__(64) void k_inner_inv(const cd* lines, int L, int Nx, int hv, cd* inner) {
+  const int ju = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int nv = 2 * hv + 1, Kv = 4 * hv + 1;
+  const cd* ln = lines + (((long long)slot * L + l) * Nx + ju) * Kv;
+  __shared__ cd M[31][63];
+  __shared__ int s_piv;
+  const int t = threadIdx.x;
+  for (int e = t; e < nv * 2 * nv; e += 64) {
+    const int r = e / (2 * nv), c = e % (2 * nv);
+    M[r][c] = (c < nv) ? ln[r - c + 2 * hv] : mkc((c - nv) == r ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int j = 0; j < nv; ++j) {
+    if (t == 0) {
+      int p = j;
+      double best = cabs1(M[j][j]);
+      for (int r = j + 1; r < nv; ++r) {
+        const double v = cabs1(M[r][j]);
+        if (v > best) { best = v; p = r; }
+      }
+      s_piv = p;
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != j) {
+      for (int c = t; c < 2 * nv; c += 64) { cd x = M[j][c]; M[j][c] = M[p][c]; M[p][c] = x; }
+    }
+    __syncthreads();
+    const cd rd = crecip(M[j][j]);
+    __syncthreads();
+    for (int c = t; c < 2 * nv; c += 64) M[j][c] = cmul(M[j][c], rd);
+    __syncthreads();
+    for (int e = t; e < nv * 2 * nv; e += 64) {
+      const int r = e / (2 * nv), c = e % (2 * nv);
+      if (r != j && c != j) M[r][c] = csub(M[r][c], cmul(M[r][j], M[j][c]));
+    }
+    __syncthreads();
+    for (int r = t; r < nv; r += 64)
+      if (r != j) M[r][j] = mkc(0.0, 0.0);
+    __syncthreads();
+  }
+  cd* out = inner + (((long long)slot * L + l) * Nx + ju) * nv * nv;
+  for (int e = t; e < nv * nv; e += 64) out[e] = M[e / nv][nv + e % nv];
+}
+
+// out[slot][l] (N x N) from the u-series of each inner entry.
+__global__ void k_outer_bttb(const cd* inner, int L, int Nx, int hu, int hv, cd* outA, cd* outB) {
+  const int e = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, Ku = 4 * hu + 1;
+  const int N = nu * nv;
+  const int n1 = e / nv, n2 = e % nv;
+  const cd* seq = inner + (((long long)slot * L + l) * Nx) * nv * nv + e;
+  cd* out = (slot == 0 ? outA : outB) + (long long)l * N * N;
+  for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x) {
+    const cd c = pixel_series(seq, (long long)nv * nv, Nx, kk - 2 * hu, false);
+    const int d = kk - 2 * hu;   // m - m'
+    for (int m = 0; m < nu; ++m) {
+      const int mp = m - d;
+      if (mp < 0 || mp >= nu) continue;
+      out[(long long)(n1 * nu + m) * N + (n2 * nu + mp)] = c;
+    }
+  }
+}
+
+__global__ void k_direct_bttb(const cd* c2d, int L, int hu, int hv, cd* out) {
+  const int l = blockIdx.y;
+  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv, Ku = 4 * hu + 1;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * N) return;
+  const int r = (int)(e / N), c = (int)(e % N);
+  const int n1 = r / nu, m1 = r % nu, n2 = c / nu, m2 = c % nu;
+  out[(long long)l * N * N + e] = c2d[((long long)l * (4 * hv + 1) + (n1 - n2 + 2 * hv)) * Ku + (m1 - m2 + 2 * hu)];
+}
+
+}  // namespace
+
+size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  size_t b = 0;
+  b += sizeof(cd) * 2 * L * Nx * Kv;       // lines
+  b += sizeof(cd) * L * Kv * Ku;           // c2d
+  b += sizeof(cd) * 2 * L * Nx * nv * nv;  // inner inverses
+  b += sizeof(cd) * 2 * L * N * N;         // pre-inversion epx, epz
+  b += sizeof(int) * 2 * L * N + sizeof(int) * 2 * L;
+  return b + 1024;
+}
+
+hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
+                                 cd* epy, cd* epz_i, void* ws, int* info_out, hipStream_t stream) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  char* p = (char*)ws;
+  cd* lines = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * Kv;
+  cd* c2d = (cd*)p; p += sizeof(cd) * (size_t)L * Kv * Ku;
+  cd* inner = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * nv * nv;
+  cd* pre = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * N * N;     // [0]: epx pre, [1]: epz pre
+  int* ipiv = (int*)p; p += sizeof(int) * 2 * (size_t)L * N;
+  int* info = info_out;
+  hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series_2d, dim3(Kv, L), dim3(64), 0, stream, lines, L, Nx, hu, hv, c2d);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
+  WB_CHECK_LAUNCH();
+  // slot 0 (eps) -> epx pre-inversion, slot 1 (1/eps) -> epy directly
+  hipLaunchKernelGGL(k_outer_bttb, dim3(nv * nv, L, 2), dim3(64), 0, stream, inner, L, Nx, hu, hv,
+                     pre, epy);
+  WB_CHECK_LAUNCH();
+  cd* epz_pre = pre + (size_t)L * N * N;
+  const long long NN = (long long)N * N;
+  hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), dim3(256), 0, stream, c2d,
+                     L, hu, hv, epz_pre);
+  WB_CHECK_LAUNCH();
+  // epx = inv(pre), epz_i = inv(epz_pre): one batched inversion of 2L matrices
+  // output slots are separate buffers: use a grid BatchPtr over (slot, layer)
+  BatchPtr A = bp_strided(pre, NN);
+  hipError_t e;
+  e = wb_inverse_launch(N, A, N, bp_strided(epx, NN), N, ipiv, info, L, stream);
+  if (e != hipSuccess) return e;
+  e = wb_inverse_launch(N, bp_strided(epz_pre, NN), N, bp_strided(epz_i, NN), N, ipiv + (size_t)L * N,
+                        info + L, L, stream);
+  return e;
+}

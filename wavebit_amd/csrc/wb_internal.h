@@ -1,0 +1,35 @@
+// Internal launch interfaces shared by the wavebit_amd translation units.
+#pragma once
+#include "wb_common.h"
+
+hipError_t wb_zgemm_launch(int m, int n, int k, cd alpha, BatchPtr A, int lda, BatchPtr B, int ldb,
+                           cd beta, BatchPtr C, int ldc, int batch, hipStream_t stream);
+hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int batch,
+                           hipStream_t stream);
+hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                           int ldb, int batch, hipStream_t stream);
+hipError_t wb_set_identity_launch(int n, BatchPtr B, int ldb, int batch, hipStream_t stream);
+hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
+                             int* info, int batch, hipStream_t stream);
+hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream);
+hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, int* info, int batch,
+                           hipStream_t stream);
+size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
+                                 cd* epy, cd* epz_i, void* ws, int* info_out, hipStream_t stream);
+
+struct wb_solve_args {
+  int P, L, hu, hv, mode;
+  const double* pts;        // P x PT_STRIDE
+  const double* thick;      // P x L
+  const int* conv_idx;      // P x L, index into the conv arrays
+  const cd* epx;            // C x N x N
+  const cd* epy;
+  const cd* epz;
+  double* eff;              // P x 3 x 4 x N
+  cd* amp;                  // P x 3 x 4 x N
+  double* pe;               // P x 3
+  int* flags;               // P x L: eig non-convergence / singular factorizations
+};
+size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
+hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

@@ -1,0 +1,305 @@
+// Batched complex FP64 LU with partial pivoting and the triangular solves built on it.
+//
+// Right-looking blocked factorization, one launch sequence per panel of WB_NB columns:
+//   k_getrf_panel   one workgroup per matrix factors the tall panel (pivot search, row swap,
+//                   scale, rank-1 update inside the panel);
+//   k_laswp_trsm    applies the panel's row swaps to every column outside it and solves
+//                   L11 U12 = A12 for the columns to its right (64 columns per workgroup);
+//   zgemm           A22 -= L21 U12 on the matrix cores.
+// Solves (getrs) reuse the same split: row permutation, then per diagonal block a small
+// triangular kernel followed by a GEMM update of the rows still to be solved. The pivots are
+// chosen by |re| + |im| with the first maximum winning, as LAPACK's izamax does.
+#include "wb_internal.h"
+#include "wb_timing.h"
+
+
+#define WB_NB 32
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, BatchPtr A, int lda,
+                                                     int* ipiv, int* info) {
+  const long long b = blockIdx.x;
+  cd* a = A.ptr(b);
+  int* piv = ipiv + b * (long long)n;
+  const int t = threadIdx.x;
+  __shared__ double s_val[256];
+  __shared__ int s_idx[256];
+  __shared__ cd s_prow[WB_NB];
+  for (int j = k0; j < k0 + jb; ++j) {
+    // pivot search over rows j..n-1 of column j
+    double best = -1.0;
+    int bi = n;
+    for (int i = j + t; i < n; i += 256) {
+      double v = cabs1(a[(long long)i * lda + j]);
+      if (v > best) { best = v; bi = i; }
+    }
+    s_val[t] = best;
+    s_idx[t] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        double v2 = s_val[t + s];
+        int i2 = s_idx[t + s];
+        if (v2 > s_val[t] || (v2 == s_val[t] && i2 < s_idx[t])) { s_val[t] = v2; s_idx[t] = i2; }
+      }
+      __syncthreads();
+    }
+    const int p = s_idx[0];
+    const double pv = s_val[0];
+    __syncthreads();
+    if (t == 0) {
+      piv[j] = p;
+      if (pv == 0.0 && info[b] == 0) info[b] = j + 1;
+    }
+    // swap rows j and p inside the panel, and stage the pivot row
+    for (int c = t; c < jb; c += 256) {
+      const int col = k0 + c;
+      cd x = a[(long long)j * lda + col];
+      cd y = a[(long long)p * lda + col];
+      if (p != j) {
+        a[(long long)j * lda + col] = y;
+        a[(long long)p * lda + col] = x;
+        s_prow[c] = y;
+      } else {
+        s_prow[c] = x;
+      }
+    }
+    __syncthreads();
+    const cd d = s_prow[j - k0];
+    const cd rd = (pv == 0.0) ? mkc(0.0, 0.0) : crecip(d);
+    for (int i = j + 1 + t; i < n; i += 256) {
+      cd* row = a + (long long)i * lda;
+      cd l = (pv == 0.0) ? row[j] : cmul(row[j], rd);
+      row[j] = l;
+      for (int c = j + 1 - k0; c < jb; ++c) row[k0 + c] = csub(row[k0 + c], cmul(l, s_prow[c]));
+    }
+    __syncthreads();
+  }
+}
+
+// Swaps for panel rows k0..k0+jb-1 applied to every column outside the panel; then the unit
+// lower triangular solve for the columns right of the panel. 64 columns per workgroup.
+__global__ __launch_bounds__(64) void k_laswp_trsm(int n, int k0, int jb, BatchPtr A, int lda,
+                                                   const int* ipiv) {
+  const long long b = blockIdx.y;
+  cd* a = A.ptr(b);
+  const int* piv = ipiv + b * (long long)n;
+  const int t = threadIdx.x;
+  __shared__ cd L[WB_NB][WB_NB + 1];
+  const int outside = n - jb;
+  const int ci = blockIdx.x * 64 + t;
+  int col = -1;
+  if (ci < outside) col = (ci < k0) ? ci : ci + jb;
+  if (col >= 0) {
+    for (int j = k0; j < k0 + jb; ++j) {
+      const int p = piv[j];
+      if (p != j) {
+        cd x = a[(long long)j * lda + col];
+        a[(long long)j * lda + col] = a[(long long)p * lda + col];
+        a[(long long)p * lda + col] = x;
+      }
+    }
+  }
+  // stage L11 (after the swaps of the panel itself, already applied by the panel kernel)
+  for (int e = t; e < jb * jb; e += 64) {
+    const int r = e / jb, c = e % jb;
+    L[r][c] = a[(long long)(k0 + r) * lda + k0 + c];
+  }
+  __syncthreads();
+  if (col >= k0 + jb) {
+    cd x[WB_NB];
+#pragma unroll
+    for (int r = 0; r < WB_NB; ++r)
+      if (r < jb) x[r] = a[(long long)(k0 + r) * lda + col];
+#pragma unroll
+    for (int r = 1; r < WB_NB; ++r) {
+      if (r < jb) {
+        cd s = x[r];
+#pragma unroll
+        for (int q = 0; q < WB_NB; ++q)
+          if (q < r) s = csub(s, cmul(L[r][q], x[q]));
+        x[r] = s;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < WB_NB; ++r)
+      if (r < jb) a[(long long)(k0 + r) * lda + col] = x[r];
+  }
+}
+
+// Row permutation of B by the LU pivots (sequential swaps, as LAPACK's zlaswp).
+__global__ __launch_bounds__(64) void k_laswp_rhs(int n, int nrhs, BatchPtr B, int ldb,
+                                                  const int* ipiv, int piv_stride) {
+  const long long b = blockIdx.y;
+  cd* x = B.ptr(b);
+  const int* piv = ipiv + b * (long long)piv_stride;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= nrhs) return;
+  for (int j = 0; j < n; ++j) {
+    const int p = piv[j];
+    if (p != j) {
+      cd v = x[(long long)j * ldb + col];
+      x[(long long)j * ldb + col] = x[(long long)p * ldb + col];
+      x[(long long)p * ldb + col] = v;
+    }
+  }
+}
+
+// Solve the jb x jb diagonal block (rows r0..r0+jb-1) of a triangular factor against the
+// same rows of B, in place. lower: unit lower (L); else non-unit upper (U).
+__global__ __launch_bounds__(64) void k_trsm_diag(int r0, int jb, int nrhs, BatchPtr A, int lda,
+                                                  BatchPtr B, int ldb, int lower) {
+  const long long b = blockIdx.y;
+  const cd* a = A.ptr(b);
+  cd* x = B.ptr(b);
+  const int t = threadIdx.x;
+  __shared__ cd T[WB_NB][WB_NB + 1];
+  for (int e = t; e < jb * jb; e += 64) {
+    const int r = e / jb, c = e % jb;
+    T[r][c] = a[(long long)(r0 + r) * lda + r0 + c];
+  }
+  __syncthreads();
+  const int col = blockIdx.x * 64 + t;
+  if (col >= nrhs) return;
+  cd v[WB_NB];
+#pragma unroll
+  for (int r = 0; r < WB_NB; ++r)
+    if (r < jb) v[r] = x[(long long)(r0 + r) * ldb + col];
+  if (lower) {
+#pragma unroll
+    for (int r = 1; r < WB_NB; ++r) {
+      if (r < jb) {
+        cd s = v[r];
+#pragma unroll
+        for (int q = 0; q < WB_NB; ++q)
+          if (q < r) s = csub(s, cmul(T[r][q], v[q]));
+        v[r] = s;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = WB_NB - 1; r >= 0; --r) {
+      if (r < jb) {
+        cd s = v[r];
+#pragma unroll
+        for (int q = 0; q < WB_NB; ++q)
+          if (q > r && q < jb) s = csub(s, cmul(T[r][q], v[q]));
+        v[r] = cdiv(s, T[r][r]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < WB_NB; ++r)
+    if (r < jb) x[(long long)(r0 + r) * ldb + col] = v[r];
+}
+
+__global__ void k_zero_int(int* x, int count) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < count) x[e] = 0;
+}
+
+__global__ void k_set_identity(int n, BatchPtr B, int ldb) {
+  const long long b = blockIdx.y;
+  cd* x = B.ptr(b);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  x[(long long)r * ldb + c] = mkc(r == c ? 1.0 : 0.0, 0.0);
+}
+
+inline BatchPtr offset(BatchPtr p, long long off) { p.base += off; return p; }
+
+}  // namespace
+
+hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int batch,
+                           hipStream_t stream) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  // a fill kernel rather than a memset node, so the sequence stays safe under graph capture
+  hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
+  WB_CHECK_LAUNCH();
+  for (int k0 = 0; k0 < n; k0 += WB_NB) {
+    const int jb = (n - k0 < WB_NB) ? n - k0 : WB_NB;
+    WB_TIMED(WB_T_GETRF_PANEL, stream,
+             hipLaunchKernelGGL(k_getrf_panel, dim3(batch), dim3(256), 0, stream, n, k0, jb, A, lda,
+                                ipiv, info));
+    WB_CHECK_LAUNCH();
+    const int outside = n - jb;
+    if (outside > 0) {
+      WB_TIMED(WB_T_LASWP_TRSM, stream,
+               hipLaunchKernelGGL(k_laswp_trsm, dim3((outside + 63) / 64, batch), dim3(64), 0, stream,
+                                  n, k0, jb, A, lda, ipiv));
+      WB_CHECK_LAUNCH();
+    }
+    const int rest = n - k0 - jb;
+    if (rest > 0) {
+      hipError_t e = wb_zgemm_launch(rest, rest, jb, mkc(-1.0, 0.0),
+                                     offset(A, (long long)(k0 + jb) * lda + k0), lda,
+                                     offset(A, (long long)k0 * lda + k0 + jb), lda, mkc(1.0, 0.0),
+                                     offset(A, (long long)(k0 + jb) * lda + k0 + jb), lda, batch,
+                                     stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+// Solve A X = B in place given the LU factors of A.
+hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                           int ldb, int batch, hipStream_t stream) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  dim3 g((nrhs + 63) / 64, batch);
+  hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n);
+  WB_CHECK_LAUNCH();
+  for (int r0 = 0; r0 < n; r0 += WB_NB) {
+    const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_trsm_diag, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb, 1));
+    WB_CHECK_LAUNCH();
+    const int rest = n - r0 - jb;
+    if (rest > 0) {
+      hipError_t e = wb_zgemm_launch(rest, nrhs, jb, mkc(-1.0, 0.0),
+                                     offset(LU, (long long)(r0 + jb) * lda + r0), lda,
+                                     offset(B, (long long)r0 * ldb), ldb, mkc(1.0, 0.0),
+                                     offset(B, (long long)(r0 + jb) * ldb), ldb, batch, stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  const int last = ((n - 1) / WB_NB) * WB_NB;
+  for (int r0 = last; r0 >= 0; r0 -= WB_NB) {
+    const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_trsm_diag, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb, 0));
+    WB_CHECK_LAUNCH();
+    if (r0 > 0) {
+      hipError_t e = wb_zgemm_launch(r0, nrhs, jb, mkc(-1.0, 0.0), offset(LU, (long long)r0), lda,
+                                     offset(B, (long long)r0 * ldb), ldb, mkc(1.0, 0.0), B, ldb,
+                                     batch, stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_int, dim3((count + 255) / 256), dim3(256), 0, stream, x, count);
+  return hipGetLastError();
+}
+
+hipError_t wb_set_identity_launch(int n, BatchPtr B, int ldb, int batch, hipStream_t stream) {
+  long long total = (long long)n * n;
+  dim3 g((unsigned)((total + 255) / 256), batch);
+  hipLaunchKernelGGL(k_set_identity, g, dim3(256), 0, stream, n, B, ldb);
+  return hipGetLastError();
+}
+
+// inv(A): A is overwritten by its LU factors, Ainv receives the inverse.
+hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
+                             int* info, int batch, hipStream_t stream) {
+  hipError_t e = wb_getrf_launch(n, A, lda, ipiv, info, batch, stream);
+  if (e != hipSuccess) return e;
+  e = wb_set_identity_launch(n, Ainv, ldi, batch, stream);
+  if (e != hipSuccess) return e;
+  return wb_getrs_launch(n, n, A, lda, ipiv, Ainv, ldi, batch, stream);
+}

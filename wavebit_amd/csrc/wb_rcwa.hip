@@ -1,0 +1,731 @@
+// RCWA batch solve on one MI355X: per-order conventions, layer modes, gap-embedded layer
+// S-matrices, the cascade and the order-resolved efficiencies, for a bucket of points that
+// share the order window and the layer count.
+//
+// Reference arithmetic restated here (paths under the reference's src/wavebit/):
+//   conventions   boundary_kz (_common/conventions.py:655-778), mode_q (:186-250),
+//                 order_azimuth (:1316-1430), order_frame_sign (:1433-1464), gap waves
+//                 (rcwa/_engine/scattering.py:128-160);
+//   modes (2D)    Omega2 = [[Ky^2 + B epx, Cc epy - KxKy], [E epx - KyKx, Kx^2 + D epy]],
+//                 V = Omega_R W / q, s/p rotation (rcwa/_engine/solve_2d.py:2377-2448,
+//                 2584-2616);
+//   modes (1D)    TE: Kx^2 - epy, V = W q; TM: epx (Kx epz_i Kx - I), V = epx^-1 W q
+//                 (rcwa/_engine/solve_1d.py:203-253);
+//   layer S       A, B = W^-1 Wg +- V^-1 Vg; S11 = M^-1 (X Y X A - B), S12 = M^-1 X (A - Y B)
+//                 with Y = B A^-1, M = A - X Y X B -- the algebra of scattering.py:116-174
+//                 with the two products it repeats formed once;
+//   cascade       the kernel's port order (engine.py:3402-3420): start port = scene bottom,
+//                 layers bottom-to-top, top incidence read as S22 / S12 (scattering.py:948-949).
+//                 Instead of full Redheffer stars, the reflection looking back to the start
+//                 port, G_k = s11 + s12 G (I - s11 G)^-1 s12, is carried as a matrix and the
+//                 transmitted wave is swept back through the stored LU factors as vectors: the
+//                 same S22/S12 columns the star cascade gives (scattering.py:191-200).
+//   efficiencies  scattering.py:907-994, solve_1d.py:1588-1610, observations/power.py.
+#include "wb_internal.h"
+
+
+// per-point parameter record (doubles), set by the host
+enum {
+  PT_WL = 0, PT_KX0 = 1, PT_KY0 = 2, PT_PHI = 3, PT_PX = 4, PT_PY = 5,
+  PT_ETOP = 6, PT_EBOT = 8, PT_AS = 10, PT_AP = 12, PT_STRIDE = 16
+};
+// per-order vectors kept per point: kx, ky, kz_kt, kz_kb, gap y, Fc, Fs  (7 x N complex)
+enum { OV_KX = 0, OV_KY, OV_KZT, OV_KZB, OV_Y, OV_FC, OV_FS, OV_COUNT };
+// per-point diagonal blocks (n each): ref11, ref12, ref21, ref22, trn11, trn12, trn21, trn22
+enum { WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2 };
+enum { DG_R11 = 0, DG_R12, DG_R21, DG_R22, DG_T11, DG_T12, DG_T21, DG_T22, DG_COUNT };
+
+namespace {
+
+constexpr double PI = 3.141592653589793;
+constexpr double HALF_PI = 1.5707963267948966;
+constexpr double FLAT = 1e-12;
+
+__device__ inline cd boundary_kz(cd eps, cd kpar2) {
+  cd v = csqrt_p(csub(eps, kpar2));
+  if (v.x + v.y < 0.0) v = cneg(v);
+  return v;
+}
+
+__device__ inline cd mode_q(cd w) {
+  cd v = csqrt_p(w);
+  const bool flip = (fabs(v.x) > FLAT) ? (v.x < 0.0) : (v.y > 0.0);
+  return flip ? cneg(v) : v;
+}
+
+// Real-angle order azimuth (kx, ky real): atan with the half-turn on Re(kx) < 0, the
+// +-pi/2 - kx/ky column and the origin rule; returns (cos, sin).
+__device__ inline void order_frame(double kx, double ky, double phi, double* c, double* s) {
+  double varphi;
+  if (kx == 0.0) {
+    if (ky == 0.0) {
+      double sp, cp;
+      sincos(phi, &sp, &cp);
+      varphi = atan(sp / cp) + (cp < 0.0 ? PI : 0.0);
+    } else {
+      varphi = (ky < 0.0 ? -HALF_PI : HALF_PI) - kx / ky;
+    }
+  } else {
+    varphi = atan(ky / kx) + (kx < 0.0 ? PI : 0.0);
+  }
+  sincos(varphi, s, c);
+}
+
+__device__ inline cd pt_c(const double* p, int off) { return mkc(p[off], p[off + 1]); }
+
+// One thread per (point, order). mode: 0 = 2D coupled, 1 = 1D TE, 2 = 1D TM.
+__global__ void k_point_setup(int P, int hu, int hv, int mode, const double* pts, cd* ov, cd* dg) {
+  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv;
+  const int n = (mode == 0) ? 2 * N : N;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)P * N) return;
+  const int p = (int)(gid / N), r = (int)(gid % N);
+  const double* pt = pts + (long long)p * PT_STRIDE;
+  const int mi = r % nu - hu, ni = r / nu - hv;
+  const double wl = pt[PT_WL];
+  const double kx = pt[PT_KX0] + (double)mi * (wl / pt[PT_PX]);
+  const double ky = (mode == 0) ? pt[PT_KY0] + (double)ni * (wl / pt[PT_PY]) : 0.0;
+  const cd kpar2 = mkc(kx * kx + ky * ky, 0.0);
+  // kernel top = scene bottom (substrate), kernel bottom = scene top (superstrate)
+  const cd e_kt = pt_c(pt, PT_EBOT), e_kb = pt_c(pt, PT_ETOP);
+  const cd n_kt = csqrt_p(e_kt), n_kb = csqrt_p(e_kb);
+  const cd kz_kt = boundary_kz(cmul(n_kt, n_kt), kpar2);
+  const cd kz_kb = boundary_kz(cmul(n_kb, n_kb), kpar2);
+  const double y = sqrt(1.0 + (kx * kx + ky * ky));
+  double fc = 1.0, fs = 0.0;
+  if (mode == 0) order_frame(kx, ky, pt[PT_PHI], &fc, &fs);
+  cd* o = ov + (long long)p * OV_COUNT * N;
+  o[OV_KX * N + r] = mkc(kx, 0.0);
+  o[OV_KY * N + r] = mkc(ky, 0.0);
+  o[OV_KZT * N + r] = kz_kt;
+  o[OV_KZB * N + r] = kz_kb;
+  o[OV_Y * N + r] = mkc(y, 0.0);
+  o[OV_FC * N + r] = mkc(fc, 0.0);
+  o[OV_FS * N + r] = mkc(fs, 0.0);
+  cd* d = dg + (long long)p * DG_COUNT * n;
+  // half-space -> gap interface blocks, diagonal: P = Wg^-1 Wh, Q = Vg^-1 Vh
+  for (int side = 0; side < 2; ++side) {
+    const cd kz = side == 0 ? kz_kt : kz_kb;
+    const cd e = side == 0 ? e_kt : e_kb;
+    const int halves = (mode == 0) ? 2 : 1;
+    for (int h = 0; h < halves; ++h) {
+      cd Pd, Qd;
+      if (mode == 0) {
+        if (h == 0) { Pd = mkc(1.0, 0.0); Qd = cscale(kz, 1.0 / y); }
+        else { Pd = cdiv(kz, cscale(e, y)); Qd = mkc(1.0, 0.0); }
+      } else if (mode == 1) {
+        Pd = mkc(1.0, 0.0); Qd = cscale(kz, 1.0 / y);
+      } else {
+        Pd = mkc(1.0, 0.0); Qd = cdiv(kz, cscale(e, y));
+      }
+      const cd A = cadd(Pd, Qd), B = csub(Pd, Qd);
+      const cd BA = cdiv(B, A);
+      const cd two_A = cdiv(mkc(2.0, 0.0), A);
+      const cd half = cscale(csub(A, cmul(B, BA)), 0.5);
+      const int idx = h * N + r;
+      if (side == 0) {   // S_ref (start port, scene bottom)
+        d[DG_R11 * n + idx] = cneg(BA);
+        d[DG_R12 * n + idx] = two_A;
+        d[DG_R21 * n + idx] = half;
+        d[DG_R22 * n + idx] = BA;
+      } else {           // S_trn (end port, scene top)
+        d[DG_T11 * n + idx] = BA;
+        d[DG_T12 * n + idx] = half;
+        d[DG_T21 * n + idx] = two_A;
+        d[DG_T22 * n + idx] = cneg(BA);
+      }
+    }
+  }
+}
+
+// 2D: BE = [[Kx epz_i Kx - I],[Ky epz_i Kx]], CD = [[Kx epz_i Ky],[Ky epz_i Ky - I]] (n x N each,
+// BE followed by CD in one buffer), and Omega2 initialised with its diagonal blocks.
+__global__ void k_omega_pre(int N, int L, BatchPtr epz, const cd* ov, cd* becd, cd* om) {
+  const long long b = blockIdx.y;           // pair (point * L + layer)
+  const int p = (int)(b / L);
+  const int n = 2 * N;
+  const cd* ez = epz.ptr(b);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  cd* be = becd + b * (long long)n * n;
+  cd* cdm = be + (long long)n * N;
+  cd* O = om + b * (long long)n * n;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * N) return;
+  const int i = (int)(e / N), j = (int)(e % N);
+  const cd kxi = o[OV_KX * N + i], kyi = o[OV_KY * N + i];
+  const cd kxj = o[OV_KX * N + j], kyj = o[OV_KY * N + j];
+  const cd z = ez[e];
+  cd Bm = cmul(cmul(kxi, z), kxj);
+  cd Dm = cmul(cmul(kyi, z), kyj);
+  if (i == j) { Bm = csub(Bm, mkc(1.0, 0.0)); Dm = csub(Dm, mkc(1.0, 0.0)); }
+  be[(long long)i * N + j] = Bm;
+  be[(long long)(N + i) * N + j] = cmul(cmul(kyi, z), kxj);
+  cdm[(long long)i * N + j] = cmul(cmul(kxi, z), kyj);
+  cdm[(long long)(N + i) * N + j] = Dm;
+  const cd zero = mkc(0.0, 0.0);
+  const bool dgn = (i == j);
+  O[(long long)i * n + j] = dgn ? cmul(kyi, kyi) : zero;
+  O[(long long)i * n + N + j] = dgn ? cneg(cmul(kxi, kyi)) : zero;
+  O[(long long)(N + i) * n + j] = dgn ? cneg(cmul(kyi, kxi)) : zero;
+  O[(long long)(N + i) * n + N + j] = dgn ? cmul(kxi, kxi) : zero;
+}
+
+// 1D TE: A = Kx^2 - epy. TM: Bm = Kx epz_i Kx - I into `aux` (the GEMM epx @ Bm follows).
+__global__ void k_op_1d(int N, int L, int tm, BatchPtr epy, BatchPtr epz, const cd* ov, cd* out) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * N) return;
+  const int i = (int)(e / N), j = (int)(e % N);
+  const cd kxi = o[OV_KX * N + i], kxj = o[OV_KX * N + j];
+  cd* A = out + b * (long long)N * N;
+  if (!tm) {
+    cd v = cneg(epy.ptr(b)[e]);
+    if (i == j) v = cadd(v, cmul(kxi, kxi));
+    A[e] = v;
+  } else {
+    cd v = cmul(cmul(kxi, epz.ptr(b)[e]), kxj);
+    if (i == j) v = csub(v, mkc(1.0, 0.0));
+    A[e] = v;
+  }
+}
+
+// 2D: q = mode_q(w); Vg holds [-epy W_bot ; epx W_top] from the GEMMs. Writes the s/p basis
+// big_W, big_V and q.
+__global__ void k_modes_post_2d(int N, int L, const cd* W, const cd* Vg, const cd* w, const cd* ov,
+                                cd* bigW, cd* bigV, cd* q) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const int n = 2 * N;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const long long base = b * (long long)n * n;
+  const cd qc = mode_q(w[b * n + c]);
+  if (r == 0) q[b * n + c] = qc;
+  const cd kx = o[OV_KX * N + r], ky = o[OV_KY * N + r];
+  const double fc = o[OV_FC * N + r].x, fs = o[OV_FS * N + r].x;
+  const cd Wt = W[base + (long long)r * n + c], Wb = W[base + (long long)(N + r) * n + c];
+  // Omega_R = [[-KxKy, Kx^2 - epy], [epx - Ky^2, KyKx]]
+  cd Vt = cadd(Vg[base + (long long)r * n + c],
+               cadd(cmul(cneg(cmul(kx, ky)), Wt), cmul(cmul(kx, kx), Wb)));
+  cd Vb = cadd(Vg[base + (long long)(N + r) * n + c],
+               cadd(cmul(cneg(cmul(ky, ky)), Wt), cmul(cmul(ky, kx), Wb)));
+  Vt = cdiv(Vt, qc);
+  Vb = cdiv(Vb, qc);
+  bigW[base + (long long)r * n + c] = csub(cscale(Wb, fc), cscale(Wt, fs));
+  bigW[base + (long long)(N + r) * n + c] = cadd(cscale(Wt, fc), cscale(Wb, fs));
+  bigV[base + (long long)r * n + c] = cadd(cscale(Vt, fc), cscale(Vb, fs));
+  bigV[base + (long long)(N + r) * n + c] = csub(cscale(Vb, fc), cscale(Vt, fs));
+}
+
+// 1D: q = mode_q(w); V = Bw * q by columns (Bw = W for TE, epx^-1 W for TM).
+__global__ void k_modes_post_1d(int N, const cd* Bw, const cd* w, cd* V, cd* q) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * N) return;
+  const int r = (int)(e / N), c = (int)(e % N);
+  const cd qc = mode_q(w[b * N + c]);
+  if (r == 0) q[b * N + c] = qc;
+  V[b * (long long)N * N + e] = cmul(Bw[b * (long long)N * N + e], qc);
+}
+
+// A = Wi diag(wg) + Vi diag(vg), B = Wi diag(wg) - Vi diag(vg) for the gap of each pair.
+// 2D gap: wg = [1, -i y], vg = [-i y, 1]; 1D TE: wg = 1, vg = -i y; 1D TM: same with z_p = y.
+__global__ void k_ab(int N, int n, int L, int mode, const cd* Wi, const cd* Vi, const cd* ov,
+                     cd* A, cd* B) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int c = (int)(e % n);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  cd wg, vg;
+  if (mode == 0) {
+    const double y = o[OV_Y * N + (c % N)].x;
+    if (c < N) { wg = mkc(1.0, 0.0); vg = mkc(0.0, -y); }
+    else { wg = mkc(0.0, -y); vg = mkc(1.0, 0.0); }
+  } else {
+    const double y = o[OV_Y * N + c].x;
+    wg = mkc(1.0, 0.0); vg = mkc(0.0, -y);
+  }
+  const long long idx = b * (long long)n * n + e;
+  const cd P = cmul(Wi[idx], wg), Q = cmul(Vi[idx], vg);
+  A[idx] = cadd(P, Q);
+  B[idx] = csub(P, Q);
+}
+
+// XYX[r][c] = X_r Y[r][c] X_c with X = exp(-k0 q d).
+__global__ void k_xyx(int n, int L, const cd* Y, const cd* q, const double* pts, const double* thick,
+                      cd* XYX) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
+  const double d = thick[b];
+  const cd xr = cexp_(cscale(q[b * n + r], -k0 * d));
+  const cd xc = cexp_(cscale(q[b * n + c], -k0 * d));
+  XYX[b * (long long)n * n + e] = cmul(cmul(xr, Y[b * (long long)n * n + e]), xc);
+}
+
+// dst (ld_dst) = src (ld_src), n x ncols per batch element
+__global__ void k_copy(int rows, int cols, BatchPtr src, int lds, BatchPtr dst, int ldd) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e / cols), c = (int)(e % cols);
+  dst.ptr(b)[(long long)r * ldd + c] = src.ptr(b)[(long long)r * lds + c];
+}
+
+// E2 rows scaled by X = exp(-k0 q d): E2 lives at columns n..2n-1 of a row stride 2n buffer.
+__global__ void k_scale_rows_x(int n, int L, cd* E, const cd* q, const double* pts,
+                               const double* thick) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
+  const cd xr = cexp_(cscale(q[b * n + r], -k0 * thick[b]));
+  cd* x = E + b * (long long)n * 2 * n + (long long)r * 2 * n + n + c;
+  *x = cmul(xr, *x);
+}
+
+// Gamma_0: full matrix with the diagonal of S_ref's 22 block.
+__global__ void k_gamma0(int n, const cd* dg, cd* G) {
+  const long long p = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  G[p * (long long)n * n + e] = (r == c) ? dg[p * DG_COUNT * n + DG_R22 * n + r] : mkc(0.0, 0.0);
+}
+
+// M = I - diag(left) * T (left == nullptr: plain I - T), in place on T.
+__global__ void k_i_minus(int n, BatchPtr T, int ldt, const cd* dg, int dg_block) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  cd* x = T.ptr(b) + (long long)r * ldt + c;
+  cd v = *x;
+  if (dg) v = cmul(dg[b * DG_COUNT * n + dg_block * n + r], v);
+  *x = csub(r == c ? mkc(1.0, 0.0) : mkc(0.0, 0.0), v);
+}
+
+// Right-hand side for the final interface: rhs = diag(t12) e_j for the basis incidences.
+// 2D: two columns (s and p unit at the specular order); 1D: one column (frame sign applied).
+__global__ void k_rhs_init(int N, int n, int nrhs, const cd* dg, const double* sigma_c, cd* rhs) {
+  const long long p = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int r = e / nrhs, j = e % nrhs;
+  const int hot = (nrhs == 2) ? (j == 0 ? N / 2 : N + N / 2) : N / 2;
+  cd v = mkc(0.0, 0.0);
+  if (r == hot) {
+    v = dg[p * DG_COUNT * n + DG_T12 * n + r];
+    if (sigma_c) v = cscale(v, sigma_c[p]);
+  }
+  rhs[p * (long long)n * nrhs + e] = v;
+}
+
+// R = t22 e + t21 (G u): gu holds G u; writes c_ref columns.
+__global__ void k_reflect(int N, int n, int nrhs, const cd* dg, const double* sigma_c, const cd* gu,
+                          cd* cref) {
+  const long long p = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int r = e / nrhs, j = e % nrhs;
+  const int hot = (nrhs == 2) ? (j == 0 ? N / 2 : N + N / 2) : N / 2;
+  cd v = cmul(dg[p * DG_COUNT * n + DG_T21 * n + r], gu[p * (long long)n * nrhs + e]);
+  if (r == hot) {
+    cd t22 = dg[p * DG_COUNT * n + DG_T22 * n + r];
+    if (sigma_c) t22 = cscale(t22, sigma_c[p]);
+    v = cadd(v, t22);
+  }
+  cref[p * (long long)n * nrhs + e] = v;
+}
+
+// c_trn = diag(ref12) d0
+__global__ void k_transmit(int n, int nrhs, const cd* dg, const cd* d0, cd* ctrn) {
+  const long long p = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int r = e / nrhs;
+  ctrn[p * (long long)n * nrhs + e] = cmul(dg[p * DG_COUNT * n + DG_R12 * n + r],
+                                           d0[p * (long long)n * nrhs + e]);
+}
+
+// Efficiencies, amplitudes and power entering, one workgroup per point.
+// eff: [P][3][4][N] doubles (R_s, R_p, T_s, T_p); amp: [P][3][4][N] complex (r_s, r_p, t_s, t_p);
+// pe: [P][3]. 1D fills variant 0 only.
+__global__ __launch_bounds__(256) void k_efficiency(int hu, int hv, int mode, const double* pts,
+                                                    const cd* ov, const cd* cref, const cd* ctrn,
+                                                    double* eff, cd* amp, double* pe) {
+  const int p = blockIdx.x;
+  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv;
+  const int n = (mode == 0) ? 2 * N : N;
+  const int nrhs = (mode == 0) ? 2 : 1;
+  const double* pt = pts + (long long)p * PT_STRIDE;
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const cd* cr = cref + (long long)p * n * nrhs;
+  const cd* ct = ctrn + (long long)p * n * nrhs;
+  // incidence side ("ref") is the scene top = kernel bottom
+  const cd e_ref = pt_c(pt, PT_ETOP), e_trn = pt_c(pt, PT_EBOT);
+  const cd n_ref = csqrt_p(pt_c(pt, PT_ETOP)), n_trn = csqrt_p(pt_c(pt, PT_EBOT));
+  const int c0 = N / 2;
+  const cd kz_c = o[OV_KZB * N + c0];
+  __shared__ double red[3][256];
+  const int t = threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (mode == 0) {
+    const cd as = pt_c(pt, PT_AS), ap = pt_c(pt, PT_AP);
+    // the three incidence variants as (alpha, beta) weights of the (s, p) basis columns
+    const cd mi_n = cmul(mkc(0.0, -1.0), n_ref);
+    const double cos_half_pi = cos(HALF_PI), sin_half_pi = sin(HALF_PI);
+    cd al[3] = {as, mkc(sin_half_pi, 0.0), mkc(0.0, 0.0)};
+    cd be[3] = {cmul(mi_n, ap), cmul(mi_n, mkc(cos_half_pi, 0.0)), cmul(mi_n, mkc(1.0, 0.0))};
+    const double s_power = kz_c.x;
+    const double p_power = cnorm2(n_ref) * cdiv(kz_c, e_ref).x;
+    double den[3] = {cnorm2(as) * s_power + cnorm2(ap) * p_power, s_power, p_power};
+    for (int r = t; r < N; r += 256) {
+      const int mi = r % nu - hu, ni = r / nu - hv;
+      const double par = ((mi + ni) & 1) ? -1.0 : 1.0;
+      const cd kzr = o[OV_KZB * N + r], kzt = o[OV_KZT * N + r];
+      const double adm_s_ref = kzr.x, adm_p_ref = cdiv(kzr, e_ref).x;
+      const double adm_s_trn = kzt.x, adm_p_trn = cdiv(kzt, e_trn).x;
+      for (int v = 0; v < 3; ++v) {
+        const cd Rs = cadd(cmul(cr[r * 2 + 0], al[v]), cmul(cr[r * 2 + 1], be[v]));
+        const cd Rp = cneg(cadd(cmul(cr[(N + r) * 2 + 0], al[v]), cmul(cr[(N + r) * 2 + 1], be[v])));
+        const cd Ts = cadd(cmul(ct[r * 2 + 0], al[v]), cmul(ct[r * 2 + 1], be[v]));
+        const cd Tp = cadd(cmul(ct[(N + r) * 2 + 0], al[v]), cmul(ct[(N + r) * 2 + 1], be[v]));
+        double* ef = eff + (((long long)p * 3 + v) * 4) * N;
+        ef[0 * N + r] = cnorm2(Rs) * adm_s_ref / den[v];
+        ef[1 * N + r] = cnorm2(Rp) * adm_p_ref / den[v];
+        ef[2 * N + r] = cnorm2(Ts) * adm_s_trn / den[v];
+        ef[3 * N + r] = cnorm2(Tp) * adm_p_trn / den[v];
+        cd* am = amp + (((long long)p * 3 + v) * 4) * N;
+        am[0 * N + r] = cscale(Rs, par);
+        am[1 * N + r] = cscale(cdiv(Rp, mi_n), par);
+        am[2 * N + r] = cscale(Ts, par);
+        am[3 * N + r] = cscale(cdiv(Tp, cmul(mkc(0.0, -1.0), n_trn)), par);
+        // net power at the lit face: incident column entries are nonzero at the specular order
+        const cd inc_s = (r == c0) ? al[v] : mkc(0.0, 0.0);
+        const cd inc_p = (r == c0) ? be[v] : mkc(0.0, 0.0);
+        const cd ps = cmul(cadd(inc_s, Rs), cconj(cmul(kzr, csub(inc_s, Rs))));
+        const cd pp = cmul(cadd(inc_p, Rp), cconj(cmul(cdiv(kzr, e_ref), csub(inc_p, Rp))));
+        acc[v] += (ps.x + pp.x) / den[v];
+      }
+    }
+  } else {
+    const int tm = (mode == 2);
+    const double sig_c = (o[OV_KX * N + c0].x < 0.0) ? -1.0 : 1.0;
+    const cd adm_c = tm ? cdiv(kz_c, e_ref) : kz_c;
+    const double incoming = adm_c.x;
+    for (int r = t; r < N; r += 256) {
+      const int mi = r % nu - hu;
+      const double par = (mi & 1) ? -1.0 : 1.0;
+      const double sig = (o[OV_KX * N + r].x < 0.0) ? -1.0 : 1.0;
+      const cd kzr = o[OV_KZB * N + r], kzt = o[OV_KZT * N + r];
+      const cd adm_ref = tm ? cdiv(kzr, e_ref) : kzr;
+      const cd adm_trn = tm ? cdiv(kzt, e_trn) : kzt;
+      const cd R = cr[r], T = ct[r];
+      double* ef = eff + ((long long)p * 3 * 4) * N;
+      cd* am = amp + ((long long)p * 3 * 4) * N;
+      const double er = cnorm2(R) * adm_ref.x / incoming;
+      const double et = cnorm2(T) * adm_trn.x / incoming;
+      const cd zero = mkc(0.0, 0.0);
+      if (!tm) {
+        ef[0 * N + r] = er; ef[1 * N + r] = 0.0; ef[2 * N + r] = et; ef[3 * N + r] = 0.0;
+        am[0 * N + r] = cscale(R, sig * par); am[1 * N + r] = zero;
+        am[2 * N + r] = cscale(T, sig * par); am[3 * N + r] = zero;
+      } else {
+        ef[0 * N + r] = 0.0; ef[1 * N + r] = er; ef[2 * N + r] = 0.0; ef[3 * N + r] = et;
+        am[0 * N + r] = zero; am[1 * N + r] = cscale(R, sig * par);
+        am[2 * N + r] = zero; am[3 * N + r] = cscale(cmul(T, cdiv(n_ref, n_trn)), sig * par);
+      }
+      const cd inc = (r == c0) ? mkc(sig_c, 0.0) : mkc(0.0, 0.0);
+      const cd pw = cmul(cadd(inc, R), cconj(cmul(adm_ref, csub(inc, R))));
+      acc[0] += pw.x / incoming;
+    }
+  }
+  for (int v = 0; v < 3; ++v) red[v][t] = acc[v];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int v = 0; v < 3; ++v) red[v][t] += red[v][t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int v = 0; v < 3; ++v) pe[p * 3 + v] = red[v][0];
+}
+
+// flags[b] |= bit where info[b] != 0 (b over pairs, or over points spread to their pairs)
+__global__ void k_flag(int count, int spread, const int* info, int bit, int* flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count) return;
+  if (info[b / spread] != 0) flags[b] |= bit;
+}
+
+__global__ void k_zero_flags(int count, int* flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < count) flags[b] = 0;
+}
+
+__global__ void k_sigma_c(int P, int hu, const cd* ov, int N, double* sig) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  sig[p] = (ov[(long long)p * OV_COUNT * N + OV_KX * N + N / 2].x < 0.0) ? -1.0 : 1.0;
+}
+
+inline dim3 grid2(long long elems, int batch) {
+  return dim3((unsigned)((elems + 255) / 256), (unsigned)batch);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------ batch solve
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
+  const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
+  const size_t n = (mode == 0) ? 2 * N : N;
+  const size_t B = (size_t)P * L;
+  const size_t nn = n * n * sizeof(cd);
+  size_t s = 0;
+  s += align_up(B * nn) * 6;          // M0..M5
+  s += align_up(B * nn * 2);          // M6: S11 | S12 (row stride 2n)
+  s += align_up(B * n * sizeof(cd)) * 3;   // w, q, tau
+  s += align_up(B * n * sizeof(int)) * 4;  // ipiv sets
+  s += align_up(B * sizeof(int)) * 4;      // infos
+  s += align_up((size_t)P * OV_COUNT * N * sizeof(cd));
+  s += align_up((size_t)P * DG_COUNT * n * sizeof(cd));
+  s += align_up((size_t)P * nn);           // Gamma_0
+  s += align_up((size_t)P * n * 2 * sizeof(cd)) * 4;  // vectors
+  s += align_up((size_t)P * sizeof(double));
+  return s + 4096;
+}
+
+hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
+  const int P = a.P, L = a.L, mode = a.mode;
+  const int N = (2 * a.hu + 1) * (2 * a.hv + 1);
+  const int n = (mode == 0) ? 2 * N : N;
+  const int B = P * L;
+  const long long nn = (long long)n * n;
+  const long long NN = (long long)N * N;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += align_up(bytes); return (void*)r; };
+  cd* M[6];
+  for (int i = 0; i < 6; ++i) M[i] = (cd*)take((size_t)B * nn * sizeof(cd));
+  cd* M6 = (cd*)take((size_t)B * nn * 2 * sizeof(cd));
+  cd* w = (cd*)take((size_t)B * n * sizeof(cd));
+  cd* q = (cd*)take((size_t)B * n * sizeof(cd));
+  cd* tau = (cd*)take((size_t)B * n * sizeof(cd));
+  int* ipiv[4];
+  for (int i = 0; i < 4; ++i) ipiv[i] = (int*)take((size_t)B * n * sizeof(int));
+  int* info[4];
+  for (int i = 0; i < 4; ++i) info[i] = (int*)take((size_t)B * sizeof(int));
+  cd* ov = (cd*)take((size_t)P * OV_COUNT * N * sizeof(cd));
+  cd* dg = (cd*)take((size_t)P * DG_COUNT * n * sizeof(cd));
+  cd* G0 = (cd*)take((size_t)P * nn * sizeof(cd));
+  const int nrhs = (mode == 0) ? 2 : 1;
+  cd* vec[4];
+  for (int i = 0; i < 4; ++i) vec[i] = (cd*)take((size_t)P * n * 2 * sizeof(cd));
+  double* sig = (double*)take((size_t)P * sizeof(double));
+  hipError_t e;
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+
+#define FLAG(count, spread, inf, bit)                                                          \
+  do {                                                                                       \
+    if (a.flags) {                                                                           \
+      hipLaunchKernelGGL(k_flag, dim3(((count) + 255) / 256), dim3(256), 0, st, (count), (spread), \
+                         (inf), (bit), a.flags);                                             \
+      WB_CHECK_LAUNCH();                                                                     \
+    }                                                                                        \
+  } while (0)
+  if (a.flags) {
+    hipLaunchKernelGGL(k_zero_flags, dim3((B + 255) / 256), dim3(256), 0, st, B, a.flags);
+    WB_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_point_setup, dim3((unsigned)(((long long)P * N + 255) / 256)), dim3(256), 0,
+                     st, P, a.hu, a.hv, mode, a.pts, ov, dg);
+  WB_CHECK_LAUNCH();
+
+  BatchPtr EPX = bp_gather(a.epx, NN, a.conv_idx);
+  BatchPtr EPY = bp_gather(a.epy, NN, a.conv_idx);
+  BatchPtr EPZ = bp_gather(a.epz, NN, a.conv_idx);
+  auto S = [&](cd* base, long long stride) { return bp_strided(base, stride); };
+
+  // ---------------------------------------------------------------- layer modes
+  cd *bigW, *bigV;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_omega_pre, grid2(NN, B), dim3(256), 0, st, N, L, EPZ, ov, M[1], M[0]);
+    WB_CHECK_LAUNCH();
+    // Omega2[:, 0:N] += [B; E] epx ; Omega2[:, N:2N] += [Cc; D] epy
+    if ((e = wb_zgemm_launch(n, N, N, one, S(M[1], nn), N, EPX, N, one, S(M[0], nn), n, B, st))) return e;
+    if ((e = wb_zgemm_launch(n, N, N, one, S(M[1] + (long long)n * N, nn), N, EPY, N, one,
+                             S(M[0] + N, nn), n, B, st))) return e;
+    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, info[0], B, st))) return e;
+    FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
+    // V GEMMs: top = -epy W_bot, bottom = epx W_top (into M4)
+    if ((e = wb_zgemm_launch(N, n, N, mone, EPY, N, S(M[3] + (long long)N * n, nn), n, zero,
+                             S(M[4], nn), n, B, st))) return e;
+    if ((e = wb_zgemm_launch(N, n, N, one, EPX, N, S(M[3], nn), n, zero,
+                             S(M[4] + (long long)N * n, nn), n, B, st))) return e;
+    hipLaunchKernelGGL(k_modes_post_2d, grid2((long long)N * n, B), dim3(256), 0, st, N, L, M[3],
+                       M[4], w, ov, M[1], M[2], q);
+    WB_CHECK_LAUNCH();
+    bigW = M[1]; bigV = M[2];
+  } else {
+    const int tm = (mode == 2);
+    if (!tm) {
+      hipLaunchKernelGGL(k_op_1d, grid2(NN, B), dim3(256), 0, st, N, L, 0, EPY, EPZ, ov, M[0]);
+      WB_CHECK_LAUNCH();
+    } else {
+      hipLaunchKernelGGL(k_op_1d, grid2(NN, B), dim3(256), 0, st, N, L, 1, EPY, EPZ, ov, M[5]);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_zgemm_launch(N, N, N, one, EPX, N, S(M[5], nn), N, zero, S(M[0], nn), N, B, st))) return e;
+    }
+    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, info[0], B, st))) return e;
+    FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
+    cd* Bw = M[3];
+    if (tm) {
+      // base = epx^-1 W
+      hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, EPX, N, S(M[5], nn), N);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_getrf_launch(N, S(M[5], nn), N, ipiv[3], info[3], B, st))) return e;
+      hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, S(M[3], nn), N, S(M[4], nn), N);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_getrs_launch(N, N, S(M[5], nn), N, ipiv[3], S(M[4], nn), N, B, st))) return e;
+      Bw = M[4];
+    }
+    // W stays in M3; V into M2
+    hipLaunchKernelGGL(k_modes_post_1d, grid2(NN, B), dim3(256), 0, st, N, Bw, w, M[2], q);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, S(M[3], nn), N, S(M[1], nn), N);
+    WB_CHECK_LAUNCH();
+    bigW = M[1]; bigV = M[2];
+  }
+
+  // ---------------------------------------------------------------- layer S-matrices
+  // Wi = bigW^-1 (M0), Vi = bigV^-1 (M3)
+  if ((e = wb_getrf_launch(n, S(bigW, nn), n, ipiv[0], info[1], B, st))) return e;
+  FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
+  if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
+  if ((e = wb_getrs_launch(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, st))) return e;
+  if ((e = wb_getrf_launch(n, S(bigV, nn), n, ipiv[1], info[2], B, st))) return e;
+  FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
+  if ((e = wb_set_identity_launch(n, S(M[3], nn), n, B, st))) return e;
+  if ((e = wb_getrs_launch(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, st))) return e;
+  // A -> M4, B -> M5
+  hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, M[0], M[3], ov, M[4], M[5]);
+  WB_CHECK_LAUNCH();
+  // Ainv: LU of a copy of A in M1, inverse in M0
+  hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M[1], nn), n);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_getrf_launch(n, S(M[1], nn), n, ipiv[0], info[1], B, st))) return e;
+  FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
+  if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
+  if ((e = wb_getrs_launch(n, n, S(M[1], nn), n, ipiv[0], S(M[0], nn), n, B, st))) return e;
+  // Y = B Ainv -> M2 ; XYX -> M3
+  if ((e = wb_zgemm_launch(n, n, n, one, S(M[5], nn), n, S(M[0], nn), n, zero, S(M[2], nn), n, B, st))) return e;
+  hipLaunchKernelGGL(k_xyx, grid2(nn, B), dim3(256), 0, st, n, L, M[2], q, a.pts, a.thick, M[3]);
+  WB_CHECK_LAUNCH();
+  // Mm = A - XYX B -> M0
+  hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M[0], nn), n);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_zgemm_launch(n, n, n, mone, S(M[3], nn), n, S(M[5], nn), n, one, S(M[0], nn), n, B, st))) return e;
+  // E1 = XYX A - B (cols 0..n-1 of M6), E2 = X (A - Y B) (cols n..2n-1)
+  const long long n2 = 2 * nn;
+  hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[5], nn), n, S(M6, n2), 2 * n);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_zgemm_launch(n, n, n, one, S(M[3], nn), n, S(M[4], nn), n, mone, S(M6, n2), 2 * n, B, st))) return e;
+  hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M6 + n, n2), 2 * n);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_zgemm_launch(n, n, n, mone, S(M[2], nn), n, S(M[5], nn), n, one, S(M6 + n, n2), 2 * n, B, st))) return e;
+  hipLaunchKernelGGL(k_scale_rows_x, grid2(nn, B), dim3(256), 0, st, n, L, M6, q, a.pts, a.thick);
+  WB_CHECK_LAUNCH();
+  // [S11 | S12] = Mm^-1 [E1 | E2]
+  if ((e = wb_getrf_launch(n, S(M[0], nn), n, ipiv[1], info[2], B, st))) return e;
+  FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
+  if ((e = wb_getrs_launch(n, 2 * n, S(M[0], nn), n, ipiv[1], S(M6, n2), 2 * n, B, st))) return e;
+
+  // ---------------------------------------------------------------- cascade
+  // pair (p, l) lives at index p * L + l; per-point views step by L pairs
+  hipLaunchKernelGGL(k_gamma0, grid2(nn, P), dim3(256), 0, st, n, dg, G0);
+  WB_CHECK_LAUNCH();
+  BatchPtr Gam = bp_strided(G0, nn);
+  int ldg = n;
+  for (int l = 0; l < L; ++l) {
+    BatchPtr s11 = bp_strided(M6 + (long long)l * n2, (long long)L * n2);
+    BatchPtr s12 = bp_offset(s11, n);
+    BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);     // -> LU_l
+    BatchPtr Xs = bp_strided(M[1] + (long long)l * nn, (long long)L * nn);
+    BatchPtr G2 = bp_strided(M[2] + (long long)l * nn, (long long)L * nn);
+    // T = s11 G ; Mx = I - T ; LU(Mx)
+    if ((e = wb_zgemm_launch(n, n, n, one, s11, 2 * n, Gam, ldg, zero, Tm, n, P, st))) return e;
+    hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tm, n, nullptr, 0);
+    WB_CHECK_LAUNCH();
+    // pivots of (p, l): a P-strided slice per layer
+    int* piv_pl = ipiv[2] + (long long)l * P * n;
+    int* info_pl = info[3];
+    if ((e = wb_getrf_launch(n, Tm, n, piv_pl, info_pl, P, st))) return e;
+    FLAG(B, L, info_pl, WB_FLAG_SINGULAR);
+    // Xs = Mx^-1 s12
+    hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, s12, 2 * n, Xs, n);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_getrs_launch(n, n, Tm, n, piv_pl, Xs, n, P, st))) return e;
+    // G2 = G Xs ; s11 += s12 G2  (s11 becomes Gamma_{l+1})
+    if ((e = wb_zgemm_launch(n, n, n, one, Gam, ldg, Xs, n, zero, G2, n, P, st))) return e;
+    if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, G2, n, one, s11, 2 * n, P, st))) return e;
+    Gam = s11;
+    ldg = 2 * n;
+  }
+  // final interface: u = (I - t11 G_L)^-1 t12 e ; R = t22 e + t21 G_L u
+  const int last_ld = ldg;
+  BatchPtr Tf = bp_strided(G0, nn);   // Gamma_0 no longer needed
+  hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, Gam, last_ld, Tf, n);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tf, n, dg, (int)DG_T11);
+  WB_CHECK_LAUNCH();
+  int* piv_f = ipiv[3];
+  if ((e = wb_getrf_launch(n, Tf, n, piv_f, info[3], P, st))) return e;
+  FLAG(B, L, info[3], WB_FLAG_SINGULAR);
+  double* sigc = nullptr;
+  if (mode != 0) {
+    hipLaunchKernelGGL(k_sigma_c, dim3((P + 255) / 256), dim3(256), 0, st, P, a.hu, ov, N, sig);
+    WB_CHECK_LAUNCH();
+    sigc = sig;
+  }
+  const long long vs = (long long)n * nrhs;
+  hipLaunchKernelGGL(k_rhs_init, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[0]);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_getrs_launch(n, nrhs, Tf, n, piv_f, S(vec[0], vs), nrhs, P, st))) return e;
+  // gu = G_L u
+  if ((e = wb_zgemm_launch(n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero, S(vec[1], vs),
+                           nrhs, P, st))) return e;
+  hipLaunchKernelGGL(k_reflect, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[1], vec[2]);
+  WB_CHECK_LAUNCH();
+  // sweep: d_{l} = (I - s11 G_{l})^-1 s12 d_{l+1}, layers top to bottom
+  cd* dcur = vec[0];
+  cd* dnext = vec[1];
+  for (int l = L - 1; l >= 0; --l) {
+    BatchPtr s12 = bp_offset(bp_strided(M6 + (long long)l * n2, (long long)L * n2), n);
+    BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);
+    int* piv_pl = ipiv[2] + (long long)l * P * n;
+    if ((e = wb_zgemm_launch(n, nrhs, n, one, s12, 2 * n, S(dcur, vs), nrhs, zero, S(dnext, vs),
+                             nrhs, P, st))) return e;
+    if ((e = wb_getrs_launch(n, nrhs, Tm, n, piv_pl, S(dnext, vs), nrhs, P, st))) return e;
+    cd* tmp = dcur; dcur = dnext; dnext = tmp;
+  }
+  hipLaunchKernelGGL(k_transmit, grid2(vs, P), dim3(256), 0, st, n, nrhs, dg, dcur, vec[3]);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_efficiency, dim3(P), dim3(256), 0, st, a.hu, a.hv, mode, a.pts, ov, vec[2],
+                     vec[3], a.eff, a.amp, a.pe);
+  WB_CHECK_LAUNCH();
+  return hipSuccess;
+}

@@ -1,0 +1,11 @@
+"""RCWA on one MI355X: the reference's ``wavebit.rcwa`` lifecycle and declarations over a
+HIP engine (raster layers, sampled Li coefficients, SMM, complex FP64)."""
+from .problem import Excitation, Layer, Outputs, Problem, Region, Stack
+from .sources import PlaneWave
+from .numerics import Numerics
+from .results import DiffractionOrders, Result
+from ._lifecycle import CheckReport, Execution, Prepared, check, prepare, run
+
+__all__ = ['Excitation', 'Layer', 'Outputs', 'Problem', 'Region', 'Stack', 'PlaneWave', 'Numerics',
+           'DiffractionOrders', 'Result', 'CheckReport', 'Execution', 'Prepared', 'check',
+           'prepare', 'run']

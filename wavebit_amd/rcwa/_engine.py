@@ -1,0 +1,177 @@
+"""Host side of the MI355X RCWA engine: points, buckets, device buffers and result unpacking.
+
+A *point* is one (problem, wavelength, theta, phi, polarization) plane-wave solve. Points that
+share an order window, a layer count and a solve path form a *bucket*, which is one
+``wb_solve`` launch sequence on the device; convolution matrices depend only on a layer's
+raster and the order window, so they are assembled once per (raster stack, window) and
+indexed by every point that uses them.
+
+Scalar host logic restated from the reference (paths under its ``src/wavebit/``):
+``_incident_kxy`` (rcwa/_engine/engine.py:2706-2795) with ``incident_phase_index``
+(_common/conventions.py:84-106), the routing of ``_routed_grating_type``
+(rcwa/_engine/engine.py:3148-3181) and the polarization resolution of ``_adapt.polarization``
+(rcwa/_adapt.py:264-296) and ``incident_sp_amplitudes`` (_common/conventions.py:1069-1115).
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _native
+
+_PI = 3.141592653589793
+_HALF_PI = 1.5707963267948966
+_HALF_PI_F32 = float(np.float32(_HALF_PI))
+_HALF_PI_F32_BELOW = float(np.nextafter(np.float32(_HALF_PI), np.float32(0)))
+
+PATH_2D, PATH_TE, PATH_TM = '2d', '1d_te', '1d_tm'
+_MODE = {PATH_2D: _native.MODE_2D, PATH_TE: _native.MODE_1D_TE, PATH_TM: _native.MODE_1D_TM}
+
+
+def incident_phase_index(eps_mu, theta):
+    """``|Re(k)|/k0`` for a real angle (conventions.py:84-106)."""
+    a, b = eps_mu.real, eps_mu.imag
+    loss = b / math.cos(theta)
+    radius = math.hypot(a, loss)
+    if a >= 0:
+        return math.sqrt((radius + a) / 2)
+    large = math.sqrt((radius - a) / 2)
+    return abs(loss) / (2 * (large if large > 0 else 1.0))
+
+
+def incident_kxy(eps_inc, theta, phi):
+    """Incident in-plane wavevector / k0 for real angles (engine.py:2706-2795)."""
+    theta = 0.0 if theta is None else float(theta)
+    phi = 0.0 if phi is None else float(phi)
+    eps_inc = complex(eps_inc)
+    if (eps_inc.imag == 0 and eps_inc.real <= 0) or (eps_inc.imag != 0 and abs(theta) >= _PI / 2):
+        raise ValueError('real-angle incidence requires a propagating incident wave: a lossless lit '
+                         'half-space must carry one (Re(eps*mu) > 0), and in an absorbing one '
+                         'abs(theta) must be less than pi/2 radians')
+    index = incident_phase_index(eps_inc, theta)
+    if theta >= _HALF_PI_F32:
+        theta = _HALF_PI_F32_BELOW
+    elif theta <= -_HALF_PI_F32:
+        theta = -_HALF_PI_F32_BELOW
+    s = math.sin(theta)
+    return index * s * math.cos(phi), index * s * math.sin(phi)
+
+
+def route(n_rows, polarization, phi, hv):
+    """The solve path a raster takes (engine.py:3148-3181, 1D conical excluded)."""
+    if isinstance(polarization, str) and n_rows == 1 and phi is None and hv == 0:
+        return PATH_TE if polarization == 's' else PATH_TM
+    return PATH_2D
+
+
+def jones(polarization):
+    """The unit ``(a_s, a_p)`` pair (``_adapt.polarization``)."""
+    if isinstance(polarization, str):
+        return (1.0 + 0j, 0.0 + 0j) if polarization == 's' else (0.0 + 0j, 1.0 + 0j)
+    a = np.asarray(polarization, dtype=np.complex128).reshape(-1)
+    if a.shape != (2,):
+        raise ValueError('Jones polarization must have exactly two components.')
+    norm = math.sqrt(float(np.sum(np.abs(a) ** 2)))
+    if norm == 0:
+        raise ValueError('polarization has zero norm')
+    return complex(a[0] / norm), complex(a[1] / norm)
+
+
+@dataclass
+class Point:
+    """One plane-wave solve of one layer stack."""
+    stack: int              # index of the conv-matrix stack in its bucket
+    wavelength: float
+    theta: float
+    phi: object              # None keeps the pure-1D routing available
+    polarization: object
+    eps_top: complex
+    eps_bottom: complex
+    period: tuple
+    thickness: tuple
+
+
+@dataclass
+class Bucket:
+    """Points sharing (orders, layer count, path): one device launch sequence."""
+    hu: int
+    hv: int
+    path: str
+    n_layers: int
+    rasters: list = field(default_factory=list)       # per stack: (L, Ny, Nx) complex tensor
+    points: list = field(default_factory=list)
+
+
+class ConvCache:
+    """Device-resident convolution matrices keyed by (raster identity, orders)."""
+
+    def __init__(self, device):
+        self.device = device
+        self._store = {}
+
+    def get(self, key, eps, hu, hv):
+        hit = self._store.get(key)
+        if hit is not None:
+            return hit
+        epx, epy, epz_i, info = _native.assemble_li(eps, hu, hv)
+        if int(info.abs().sum()) != 0:
+            raise FloatingPointError('a layer\'s convolution matrix is singular')
+        self._store[key] = (epx, epy, epz_i)
+        return self._store[key]
+
+    def __len__(self):
+        return len(self._store)
+
+    def clear(self):
+        self._store.clear()
+
+
+def point_record(pt, path):
+    """The WB_PT_* record of one point."""
+    kx0, ky0 = incident_kxy(pt.eps_top, pt.theta, pt.phi)
+    a_s, a_p = jones(pt.polarization) if path == PATH_2D else (1.0 + 0j, 0.0 + 0j)
+    eps_t, eps_b = complex(pt.eps_top), complex(pt.eps_bottom)
+    py = pt.period[1] if len(pt.period) > 1 else 1.0
+    return [pt.wavelength, kx0, ky0, 0.0 if pt.phi is None else float(pt.phi),
+            float(pt.period[0]), float(py), eps_t.real, eps_t.imag, eps_b.real, eps_b.imag,
+            a_s.real, a_s.imag, a_p.real, a_p.imag, 0.0, 0.0]
+
+
+def run_bucket(bucket, conv, device, workspace=None):
+    """Solve every point of a bucket; returns host numpy arrays (eff, amp, pe, flags).
+
+    ``conv`` is a list of (epx, epy, epz_i) per stack, each (L, N, N) on the device.
+    """
+    P, L = len(bucket.points), bucket.n_layers
+    N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
+    # concatenate the stacks' matrices: stack s layer l -> s * L + l
+    epx = torch.cat([c[0] for c in conv]).contiguous()
+    epy = torch.cat([c[1] for c in conv]).contiguous()
+    epz = torch.cat([c[2] for c in conv]).contiguous()
+    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
+    pts_d = torch.from_numpy(rec).to(device)
+    thick_d = torch.from_numpy(thick).to(device)
+    idx_d = torch.from_numpy(idx).to(device)
+    eff, amp, pe, flags = _native.solve(pts_d, thick_d, idx_d, epx, epy, epz, bucket.hu,
+                                        bucket.hv, _MODE[bucket.path], workspace)
+    return eff, amp, pe, flags, N
+
+
+def unpack(eff, amp, pe, p, hu, hv, path):
+    """One point's outputs as ``{'incident', 's_inc', 'p_inc'}`` dicts of order-grid arrays."""
+    nv, nu = 2 * hv + 1, 2 * hu + 1
+    variants = ('incident',) if path != PATH_2D else ('incident', 's_inc', 'p_inc')
+    out = {}
+    for v, name in enumerate(variants):
+        e = eff[p, v].reshape(4, nv, nu)
+        a = amp[p, v].reshape(4, nv, nu)
+        sub = {'R_s': e[0], 'R_p': e[1], 'T_s': e[2], 'T_p': e[3],
+               'r_s': a[0], 'r_p': a[1], 't_s': a[2], 't_p': a[3],
+               'power_entering': pe[p, v]}
+        sub['R'] = sub['R_s'] + sub['R_p']
+        sub['T'] = sub['T_s'] + sub['T_p']
+        out[name] = sub
+    return out

@@ -1,0 +1,450 @@
+"""``check`` / ``prepare`` / ``Prepared.run`` / ``run`` on the MI355X engine.
+
+Mirrors reference ``src/wavebit/rcwa/_lifecycle.py``: ``check`` (:421-457) admits a problem
+and reports resource estimates without solving; ``prepare`` (:736-797) validates concrete
+values and freezes what a run reuses; ``Prepared.run`` (:698-733) executes a live problem of
+the same structure after the runtime guards (:515-561) and packs a ``Result``; ``run``
+(:800-803) is prepare-then-run. Error classes and codes follow the reference: bad inputs are
+``ValidationError`` (``rcwa.invalid``), unsupported requests ``UnsupportedFeatureError``
+(``rcwa.unsupported`` and friends), reuse violations ``IncompatibleProblemError``
+(``prepared.*``), failed factorizations or eigensolves ``NumericalError('rcwa.singular')``
+and non-finite observables ``NumericalError('rcwa.nonfinite')``.
+
+Execution: ``backend='torch', device='cuda[:N]'`` (PyTorch-ROCm exposes HIP as ``cuda``) or
+the explicit ``backend='hip'`` run the HIP engine; ``precision='double'`` only. The
+reference's ``'numpy'`` and ``'jax'`` backends are CPU/JAX paths this engine does not have:
+they are refused with ``UnsupportedFeatureError`` rather than silently rerouted.
+
+Unlike the reference, which builds one solver per wavelength in a Python loop
+(:712-727), a run here is one batched device solve over all its wavelengths, and the
+convolution matrices of each raster stack are kept on the device and reused by every later
+run of the same rasters.
+"""
+import hashlib
+from dataclasses import dataclass, field
+from types import MappingProxyType
+
+import numpy as np
+
+from ..errors import (IncompatibleProblemError, NumericalError, UnsupportedFeatureError,
+                      ValidationError, WavebitError)
+from ..materials import Material
+from .numerics import Numerics
+from .problem import Problem
+from .sources import PlaneWave
+from . import _engine
+from .results import pack
+
+
+@dataclass(frozen=True)
+class Execution:
+    backend: str = 'numpy'
+    device: str = 'cpu'
+    precision: str = 'double'
+
+    def validate(self):
+        if self.backend not in ('numpy', 'jax', 'torch', 'hip'):
+            raise ValidationError("backend must be one of 'numpy', 'jax', 'torch', or 'hip'.",
+                                  code='execution.backend', path='execution.backend',
+                                  actual=self.backend)
+        if self.precision not in ('double', 'single'):
+            raise ValidationError("precision must be 'double' or 'single'.",
+                                  code='execution.precision', path='execution.precision')
+        if self.backend in ('numpy', 'jax'):
+            raise UnsupportedFeatureError(
+                f"backend={self.backend!r} is a CPU path; this engine runs the RCWA solve on the "
+                "MI355X through HIP: use backend='torch', device='cuda' (or backend='hip').",
+                code='rcwa.backend', path='execution.backend')
+        if not str(self.device).startswith('cuda'):
+            raise UnsupportedFeatureError(
+                f"device={self.device!r}: the HIP engine runs on a GPU device ('cuda' or 'cuda:N').",
+                code='rcwa.device', path='execution.device')
+        if self.precision != 'double':
+            raise UnsupportedFeatureError('Only precision="double" (complex128) is implemented.',
+                                          code='rcwa.precision', path='execution.precision')
+
+
+@dataclass(frozen=True)
+class CheckReport:
+    issues: tuple = ()
+    estimates: object = field(default_factory=dict)
+    resolved: object = field(default_factory=dict)
+    capabilities: object = field(default_factory=dict)
+
+    @property
+    def ok(self):
+        return not any(i.severity == 'error' for i in self.issues)
+
+    def raise_if_invalid(self):
+        for issue in self.issues:
+            if issue.severity == 'error':
+                raise ValidationError(issue.message, code=issue.code, path=issue.path)
+
+
+# --------------------------------------------------------------------------- admission
+
+def _host_array(value, path):
+    if hasattr(value, 'detach'):
+        if getattr(value, 'requires_grad', False):
+            raise UnsupportedFeatureError(
+                'Gradients through Prepared.run are provided by wavebit_amd.rcwa.grad, not by '
+                'autograd through this entry point.', code='rcwa.gradient', path=path)
+        value = value.detach().cpu().numpy()
+    try:
+        return np.asarray(value, dtype=np.complex128)
+    except (TypeError, ValueError):
+        raise ValidationError('expected numbers', code='rcwa.invalid', path=path) from None
+
+
+def _scalar(value, path, real=True):
+    if hasattr(value, 'detach'):
+        if getattr(value, 'requires_grad', False):
+            raise UnsupportedFeatureError('Autograd through Prepared.run is not implemented.',
+                                          code='rcwa.gradient', path=path)
+        value = value.detach().cpu().item()
+    z = complex(value)
+    if real:
+        if z.imag != 0:
+            raise UnsupportedFeatureError('complex values are not implemented here', code='rcwa.unsupported',
+                                          path=path)
+        return z.real
+    return z
+
+
+def _material(m, path):
+    if isinstance(m, (int, float, complex)):
+        raise ValidationError('Use a common Material model, not an untyped material number.',
+                              code='material.type', path=path)
+    if not hasattr(m, 'at'):
+        raise ValidationError('Use a common Material model.', code='material.type', path=path)
+    return complex(m.at(None).epsilon)
+
+
+def _wavelengths(problem):
+    value = problem.excitation.wavelength
+    if hasattr(value, 'detach'):
+        value = value.detach().cpu().numpy()
+    arr = np.asarray(value, dtype=np.float64) if not isinstance(value, tuple) else np.asarray(
+        [_scalar(v, 'excitation.wavelength') for v in value], dtype=np.float64)
+    if arr.ndim == 0:
+        arr = arr.reshape(1)
+    if arr.ndim != 1 or arr.size == 0:
+        raise ValidationError('wavelength must be a scalar or a nonempty one-dimensional spectrum.',
+                              code='wavelength.shape', path='excitation.wavelength')
+    return arr
+
+
+@dataclass
+class _Resolved:
+    rasters: np.ndarray        # (L, Ny, Nx) complex, bottom-to-top
+    thickness: np.ndarray      # (L,)
+    eps_top: complex
+    eps_bottom: complex
+    period: tuple
+    wavelengths: np.ndarray
+    source: PlaneWave
+    polarization: object
+    path: str
+    hu: int
+    hv: int
+    uniform: bool
+
+
+def _resolve(problem, numerics, execution):
+    if not isinstance(problem, Problem):
+        raise ValidationError('rcwa lifecycle functions require rcwa.Problem.', code='problem.type',
+                              path='problem')
+    if not isinstance(numerics, Numerics):
+        raise ValidationError('numerics must be rcwa.Numerics.', code='numerics.type', path='numerics')
+    execution.validate()
+    bad = numerics.unsupported()
+    if bad is not None:
+        raise UnsupportedFeatureError(
+            f'numerics.{bad[0]}={bad[1]!r} is not implemented by this engine (it implements the '
+            'raster defaults: SMM, sampled coefficients, Li rules with fixed nesting).',
+            code='rcwa.unsupported', path=f'numerics.{bad[0]}')
+    if problem.outputs.fields:
+        raise UnsupportedFeatureError('Field reconstruction is not implemented by this engine.',
+                                      code='rcwa.unsupported', path='outputs.fields')
+    if problem.outputs.scattering:
+        raise UnsupportedFeatureError('Scattering-matrix output is not implemented by this engine.',
+                                      code='rcwa.unsupported', path='outputs.scattering')
+    if not problem.outputs.diffraction:
+        raise UnsupportedFeatureError('Outputs(diffraction=False) leaves nothing this engine computes.',
+                                      code='rcwa.unsupported', path='outputs.diffraction')
+    stack = problem.stack
+    if stack.lattice is not None:
+        raise UnsupportedFeatureError('Oblique lattices are not implemented; give period=(Lx, Ly).',
+                                      code='rcwa.unsupported', path='stack.lattice')
+    period = tuple(_scalar(v, 'stack.period') for v in stack.period)
+    if len(period) != 2 or not all(p > 0 for p in period):
+        raise ValidationError('period must be two positive lengths.', code='stack.period',
+                              path='stack.period')
+    sources = problem.excitation.sources
+    if len(sources) != 1:
+        raise UnsupportedFeatureError('One incident plane wave per problem is implemented.',
+                                      code='rcwa.unsupported', path='excitation.sources')
+    src = sources[0]
+    if src.side != 'top':
+        raise UnsupportedFeatureError('Bottom incidence is not implemented by this engine.',
+                                      code='rcwa.unsupported', path='excitation.sources.side')
+    if src.kx is not None or src.ky is not None:
+        raise UnsupportedFeatureError('Direct kx/ky incidence is not implemented; use theta/phi.',
+                                      code='rcwa.unsupported', path='excitation.sources')
+    theta = None if src.theta is None else _scalar(src.theta, 'excitation.sources.theta')
+    phi = None if src.phi is None else _scalar(src.phi, 'excitation.sources.phi')
+    layers = stack.layers
+    raster = [l.epsilon is not None for l in layers]
+    if any(raster) and not all(raster):
+        raise UnsupportedFeatureError(
+            'A stack cannot mix sampled layers and region layers. Represent homogeneous layers on '
+            'the same raster, or use regions throughout.', code='rcwa.mixed_layer_encoding',
+            path='stack.layers')
+    eps_top = _material(stack.top, 'stack.top')
+    eps_bottom = _material(stack.bottom, 'stack.bottom')
+    if any(l.regions for l in layers):
+        raise UnsupportedFeatureError('Region (shape) layers are not implemented; give sampled '
+                                      'epsilon rasters.', code='rcwa.unsupported', path='stack.layers')
+    if any(l.mu is not None for l in layers):
+        raise UnsupportedFeatureError('Magnetic rasters are not implemented.', code='rcwa.unsupported',
+                                      path='stack.layers.mu')
+    uniform = not any(raster)
+    if not layers:
+        rasters = np.full((1, 1, 1), eps_top, dtype=np.complex128)
+        thickness = np.zeros(1)
+        uniform = True
+    elif uniform:
+        rasters = np.stack([np.full((1, 1), _material(l.background, f'stack.layers[{i}].background'),
+                                    dtype=np.complex128) for i, l in enumerate(layers)])
+        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness')
+                              for i, l in enumerate(layers)])
+    else:
+        arrs = [_host_array(l.epsilon, f'stack.layers[{i}].epsilon') for i, l in enumerate(layers)]
+        for i, a in enumerate(arrs):
+            if a.ndim != 2 or a.size == 0:
+                raise ValidationError('epsilon must be a two-dimensional (Ny, Nx) raster.',
+                                      code='layer.epsilon_shape', path=f'stack.layers[{i}].epsilon')
+        if len({a.shape for a in arrs}) != 1:
+            raise ValidationError('Every raster layer of a stack must have the same (Ny, Nx) shape.',
+                                  code='layer.epsilon_shape', path='stack.layers')
+        rasters = np.stack(arrs)
+        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness')
+                              for i, l in enumerate(layers)])
+    hu, hv = numerics.orders
+    if rasters.shape[1] == 1 and hv != 0 and not uniform:
+        raise ValidationError(
+            'A one-row raster is invariant along y, so it has no y orders: use orders=(M, 0).',
+            code='rcwa.invalid', path='numerics.orders')
+    if hv > 15:
+        raise UnsupportedFeatureError('orders[1] above 15 is not implemented by this engine.',
+                                      code='rcwa.unsupported', path='numerics.orders')
+    pol = src.polarization
+    if not isinstance(pol, str):
+        pol = tuple(_scalar(v, 'excitation.sources.polarization', real=False) for v in pol)
+        if len(pol) != 2:
+            raise ValidationError('Jones polarization must have exactly two components.',
+                                  code='source.polarization_shape',
+                                  path='excitation.sources.polarization')
+    n_rows = rasters.shape[1] if not uniform else 2
+    path = _engine.route(n_rows, pol, phi, hv)
+    ws = _wavelengths(problem)
+    return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
+                     period=period, wavelengths=ws, source=src, polarization=pol, path=path,
+                     hu=hu, hv=hv, uniform=uniform)
+
+
+def _structure_key(r):
+    return (r.rasters.shape, r.thickness.shape, r.wavelengths.shape, r.path, r.hu, r.hv,
+            isinstance(r.polarization, str))
+
+
+def _runtime_checks(r):
+    finite = bool(np.all(np.isfinite(r.rasters)) and np.all(np.isfinite(r.thickness))
+                  and np.all(np.isfinite(r.wavelengths)) and np.isfinite(r.eps_top)
+                  and np.isfinite(r.eps_bottom))
+    pol_ok = True
+    if not isinstance(r.polarization, str):
+        pol_ok = sum(abs(complex(v)) ** 2 for v in r.polarization) > 0
+    amp = r.source.amplitude
+    amp_ok = abs(complex(amp.item() if hasattr(amp, 'item') else amp)) > 0
+    return {'finite_inputs': finite,
+            'positive_wavelength': bool(np.all(r.wavelengths > 0)),
+            'nonnegative_thickness': bool(np.all(r.thickness >= 0)),
+            'nonzero_polarization': bool(pol_ok and amp_ok)}
+
+
+def _estimates(r):
+    N = (2 * r.hu + 1) * (2 * r.hv + 1)
+    n = 2 * N if r.path == _engine.PATH_2D else N
+    P, L = r.wavelengths.size, r.rasters.shape[0]
+    ws = _engine._native.solve_workspace_bytes(P, L, r.hu, r.hv, _engine._MODE[r.path]) if _native_ok() else None
+    return {'retained_spatial_modes': N, 'one_dense_material_matrix_bytes': N * N * 16,
+            'one_mode_matrix_bytes': n * n * 16, 'solve_workspace_bytes': ws,
+            'scope': 'device workspace of one batched run over every wavelength'}
+
+
+def _native_ok():
+    try:
+        _engine._native.load_library()
+        return True
+    except Exception:
+        return False
+
+
+def _report(problem, numerics, execution, r):
+    return CheckReport(
+        issues=(),
+        estimates=MappingProxyType(_estimates(r)),
+        resolved=MappingProxyType({'order_window': ((0, r.hu), (0, r.hv)),
+                                   'wavelength_count': int(r.wavelengths.size),
+                                   'layer_count': len(problem.stack.layers),
+                                   'fourier_coefficients': 'sampled', 'factorization': 'li',
+                                   'li_nesting': 'fixed', 'algorithm': 'smm',
+                                   'formulation': 'second_order', 'solve_path': r.path,
+                                   'backend': execution.backend, 'device': execution.device,
+                                   'precision': execution.precision, 'engine': 'wavebit_amd/hip-gfx950',
+                                   'length_units': problem.units.length}),
+        capabilities=MappingProxyType({'eager': True, 'batched_wavelengths': True,
+                                       'material_gradients': 'wavebit_amd.rcwa.grad',
+                                       'diffraction': True}))
+
+
+def check(problem, *, numerics, backend='numpy', device='cpu', precision='double'):
+    """Admission and scoped resource estimates without solving (reference _lifecycle.py:421)."""
+    execution = Execution(backend=backend, device=device, precision=precision)
+    try:
+        r = _resolve(problem, numerics, execution)
+        _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
+        return _report(problem, numerics, execution, r)
+    except WavebitError as error:
+        return CheckReport(issues=(error.issue,))
+    except (TypeError, ValueError) as error:
+        return CheckReport(issues=(ValidationError(str(error), code='rcwa.invalid', path='problem').issue,))
+
+
+class _Cache:
+    """Device-resident convolution matrices per (raster bytes, window) for one preparation."""
+
+    def __init__(self, device):
+        self.conv = _engine.ConvCache(device)
+
+    def get(self, rasters, hu, hv):
+        import torch
+        key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
+               rasters.shape, hu, hv)
+        eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.conv.device)
+        return self.conv.get(key, eps, hu, hv)
+
+
+@dataclass(frozen=True)
+class Prepared:
+    """A fixed numerical contract and structure; device caches of convolution matrices."""
+    numerics: Numerics
+    execution: Execution
+    report: CheckReport
+    contract: object
+    _structure: object = field(repr=False)
+    _cache: object = field(repr=False, default=None)
+
+    def _resolve_live(self, problem):
+        r = _resolve(problem, self.numerics, self.execution)
+        if _structure_key(r) != self._structure:
+            raise IncompatibleProblemError(
+                'The live problem changed a static part of the preparation (layer count, raster '
+                'shape, wavelength count, solve path or order window). Prepare again.',
+                code='prepared.structure', path='problem')
+        return r
+
+    def validate(self, problem):
+        try:
+            r = self._resolve_live(problem)
+            for name, ok in _runtime_checks(r).items():
+                if not ok:
+                    raise IncompatibleProblemError('The prepared runtime guard failed.',
+                                                   code=f'prepared.{name}', path='problem')
+            return _report(problem, self.numerics, self.execution, r)
+        except WavebitError as error:
+            return CheckReport(issues=(error.issue,))
+
+    def run(self, problem):
+        """Execute a live problem of the prepared structure; returns a ``Result``."""
+        r = self._resolve_live(problem)
+        checks = _runtime_checks(r)
+        for name, ok in checks.items():
+            if not ok:
+                raise IncompatibleProblemError('The prepared runtime guard failed.',
+                                               code=f'prepared.{name}', path='problem')
+        try:
+            per_wl, flags = _solve_resolved(r, self._cache)
+        except FloatingPointError as error:
+            raise NumericalError(str(error), code='rcwa.singular', path='problem') from error
+        except ValueError as error:
+            raise ValidationError(str(error), code='rcwa.invalid', path='problem') from error
+        if np.any(flags != 0):
+            raise NumericalError(
+                'A layer eigensolve did not converge or a scattering-matrix factorization was '
+                'singular.', code='rcwa.singular', path='problem')
+        prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950', 'length_units': problem.units.length,
+                'execution': self.execution, 'numerics': self.numerics,
+                'coordinate_frame': '+z up; bottom interface z=0; layers bottom-to-top',
+                'time_convention': 'exp(-i*omega*t)'}
+        diag = {'report': self.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
+                'effective_wavelengths': tuple(float(w) for w in r.wavelengths),
+                'effective_theta': tuple(r.source.theta for _ in r.wavelengths)}
+        result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov)
+        if not result.validity.checks['numerical_finite']:
+            raise NumericalError('The requested solve produced nonfinite observables; no implicit '
+                                 'detuning was applied.', code='rcwa.nonfinite', path='result')
+        return result
+
+
+def _solve_resolved(r, cache):
+    """One batched device solve over all wavelengths of a resolved problem."""
+    import torch
+    device = torch.device(cache.conv.device)
+    _engine._native.context(device)          # raises EngineUnavailable without a GPU or library
+    conv = cache.get(r.rasters, r.hu, r.hv)
+    bucket = _engine.Bucket(hu=r.hu, hv=r.hv, path=r.path, n_layers=r.rasters.shape[0])
+    for wl in r.wavelengths:
+        bucket.points.append(_engine.Point(
+            stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
+            polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
+            period=r.period, thickness=tuple(float(t) for t in r.thickness)))
+    eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device)
+    eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
+    per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
+    return per_wl, flags
+
+
+def prepare(problem, *, numerics, backend='numpy', device='cpu', precision='double'):
+    """Validate concrete values and freeze the structure a run reuses (reference :736)."""
+    execution = Execution(backend=backend, device=device, precision=precision)
+    try:
+        r = _resolve(problem, numerics, execution)
+        _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
+    except WavebitError:
+        raise
+    except NotImplementedError as error:
+        raise UnsupportedFeatureError(str(error), code='rcwa.unsupported', path='problem') from error
+    except (TypeError, ValueError) as error:
+        raise ValidationError(str(error), code='rcwa.invalid', path='problem') from error
+    for name, ok in _runtime_checks(r).items():
+        if not ok:
+            raise ValidationError(f'runtime guard {name} failed on the prepared problem.',
+                                  code=f'prepared.{name}', path='problem')
+    report = _report(problem, numerics, execution, r)
+    contract = MappingProxyType({
+        'static': ('numerics', 'execution', 'units', 'layer count', 'raster shapes',
+                   'wavelength count', 'solve path', 'order window'),
+        'live': ('raster values', 'half-space materials', 'wavelength values', 'incident angles',
+                 'polarization', 'layer thickness'),
+        'cache_policy': 'convolution matrices stay on the device per distinct raster stack; '
+                        'modes and propagation are rebuilt every run'})
+    return Prepared(numerics=numerics, execution=execution, report=report, contract=contract,
+                    _structure=_structure_key(r), _cache=_Cache(device))
+
+
+def run(problem, *, numerics, backend='numpy', device='cpu', precision='double'):
+    """Prepare and run one problem (reference :800-803)."""
+    return prepare(problem, numerics=numerics, backend=backend, device=device,
+                   precision=precision).run(problem)

@@ -1,0 +1,123 @@
+"""Result containers (subset of reference ``src/wavebit/rcwa/results.py:93-487`` and
+``src/wavebit/observations/contracts.py:288``).
+
+``Result.R`` / ``Result.T`` are per-order efficiencies of shape (wavelength, order_v, order_u);
+``Result.reflection`` / ``Result.transmission`` are ``DiffractionOrders`` with the Fresnel
+amplitude ratios in each order's own s/p frame, centre-referenced, as the reference reports
+them. Every array is an independent host (NumPy) snapshot: a later solve never changes an
+earlier result.
+"""
+from dataclasses import dataclass, field
+from types import MappingProxyType
+
+import numpy as np
+
+from ..errors import UnsupportedFeatureError
+
+
+@dataclass(frozen=True)
+class DiffractionOrders:
+    wavelengths: object
+    orders_u: object
+    orders_v: object
+    amplitude_ratio_s: object
+    amplitude_ratio_p: object
+    efficiency: object
+    efficiency_s: object
+    efficiency_p: object
+    k_parallel: object = None
+    reference: dict = field(default_factory=dict)
+
+
+@dataclass(frozen=True)
+class Validity:
+    checks: dict
+    convergence: dict
+
+
+@dataclass(frozen=True)
+class Result:
+    """Independent result of one prepared run."""
+    outputs: object
+    wavelengths: object
+    validity: object
+    diagnostics: object
+    provenance: object
+
+    def __post_init__(self):
+        for name in ('outputs', 'diagnostics', 'provenance'):
+            object.__setattr__(self, name, MappingProxyType(dict(getattr(self, name))))
+
+    def _efficiency(self, port):
+        observation = self.outputs.get(port)
+        if observation is None or observation.efficiency is None:
+            raise UnsupportedFeatureError(
+                'No single-incident-port diffraction efficiency was requested or is available for this result.',
+                code='outputs.unavailable', path=f'outputs.{port}.efficiency')
+        return observation.efficiency
+
+    @property
+    def R(self):
+        """Reflected efficiencies, shape (wavelength, order_v, order_u)."""
+        return self._efficiency('reflected')
+
+    @property
+    def T(self):
+        """Transmitted efficiencies, shape (wavelength, order_v, order_u)."""
+        return self._efficiency('transmitted')
+
+    @property
+    def reflection(self):
+        return self.outputs['reflected']
+
+    @property
+    def transmission(self):
+        return self.outputs['transmitted']
+
+    @property
+    def scattering(self):
+        raise UnsupportedFeatureError('Scattering-matrix output is not implemented by this engine.',
+                                      code='outputs.unavailable', path='outputs.scattering')
+
+
+def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance):
+    """Stack per-wavelength solve outputs (dicts from the engine) into a ``Result``.
+
+    NaN-masks every masked output when a validity check failed (reference
+    ``results.py:441-464`` ``admission_scale``).
+    """
+    orders_u = np.arange(-hu, hu + 1)
+    orders_v = np.arange(-hv, hv + 1)
+    ws = np.asarray(wavelengths, dtype=np.float64)
+
+    def stack(key):
+        return np.stack([np.asarray(d['incident'][key]) for d in per_wavelength])
+
+    finite = all(np.all(np.isfinite(stack(k))) for k in ('r_s', 'r_p', 't_s', 't_p', 'R', 'T'))
+    combined = bool(finite and all(bool(v) for v in checks.values()))
+    scale = 1.0 if combined else float('nan')
+    outputs = {}
+    for name, prefix, eff in (('reflected', 'r', 'R'), ('transmitted', 't', 'T')):
+        outputs[name] = DiffractionOrders(
+            wavelengths=ws, orders_u=orders_u, orders_v=orders_v,
+            amplitude_ratio_s=stack(prefix + '_s') * scale,
+            amplitude_ratio_p=stack(prefix + '_p') * scale,
+            efficiency=stack(eff) * scale,
+            efficiency_s=stack(eff + '_s') * scale,
+            efficiency_p=stack(eff + '_p') * scale,
+            reference={'port': 'top' if name == 'reflected' else 'bottom', 'wave': name,
+                       'incident_port': 'top', 'polarization_frame': "each order's outgoing s/p frame",
+                       'time_convention': 'exp(-i*omega*t)'})
+    outputs['power_entering'] = np.array([float(d['incident']['power_entering']) for d in per_wavelength]) * scale
+    variants = {}
+    for v in ('s_inc', 'p_inc'):
+        if all(v in d for d in per_wavelength):
+            variants[v] = {k: np.stack([np.asarray(d[v][k]) for d in per_wavelength]) * scale
+                           for k in ('R', 'T', 'r_s', 'r_p', 't_s', 't_p')}
+    diagnostics = dict(diagnostics)
+    diagnostics['incidence_variants'] = variants
+    return Result(outputs=outputs, wavelengths=ws,
+                  validity=Validity(checks={**checks, 'numerical_finite': combined},
+                                    convergence={'status': 'not assessed',
+                                                 'requires': 'independent refinement study'}),
+                  diagnostics=diagnostics, provenance=provenance)
