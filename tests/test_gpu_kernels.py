@@ -62,7 +62,8 @@ def _check_eig(A, w, W):
     assert np.allclose(np.linalg.norm(W, axis=0), 1.0, atol=1e-12)
 
 
-@pytest.mark.parametrize('n,batch', [(1, 1), (2, 2), (3, 1), (10, 3), (50, 2), (98, 2), (257, 1)])
+@pytest.mark.parametrize('n,batch', [(1, 1), (2, 2), (3, 1), (10, 3), (50, 2), (98, 2), (257, 1),
+                                     (450, 2), (722, 1)])
 def test_zgeev_random(n, batch):
     rng = np.random.default_rng(100 + n)
     A = crand(rng, batch, n, n)
@@ -81,3 +82,18 @@ def test_zgeev_diagonal_and_repeated():
     assert int(info[0]) == 0
     _check_eig(A[0], w.cpu().numpy()[0], W.cpu().numpy()[0])
     assert abs(abs(np.linalg.det(W.cpu().numpy()[0])) - 1.0) < 1e-12
+
+
+def test_zgeev_repeated_blocks_keep_a_full_basis():
+    """diag(B, B) has every eigenvalue twice (as a symmetric grating's operator does); the
+    eigenvector matrix must still be a basis."""
+    rng = np.random.default_rng(21)
+    Bm = crand(rng, 60, 60)
+    A = np.zeros((1, 120, 120), dtype=np.complex128)
+    A[0, :60, :60] = Bm
+    A[0, 60:, 60:] = Bm
+    w, W, info = _native.zgeev(torch.from_numpy(A.copy()).to(DEV))
+    assert int(info[0]) == 0
+    w, W = w.cpu().numpy()[0], W.cpu().numpy()[0]
+    _check_eig(A[0], w, W)
+    assert np.linalg.cond(W) < 1e10

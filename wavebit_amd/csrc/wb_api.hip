@@ -105,7 +105,8 @@ int wb_zgetrs_batched(wb_ctx* ctx, int n, int nrhs, const double* LU, int lda, l
 size_t wb_zgeev_workspace_size(int n, int batch) {
   if (n <= 0 || batch <= 0) return 0;
   const size_t nn = (size_t)n * n * sizeof(cd);
-  return 2 * align_up(nn * batch) + align_up((size_t)n * batch * sizeof(cd)) + 256;
+  return 2 * align_up(nn * batch) + align_up((size_t)n * batch * sizeof(cd)) +
+         align_up(wb_hqr_ms_rot_bytes(n, batch)) + 256;
 }
 
 int wb_zgeev_batched(wb_ctx* ctx, int n, double* A, double* w, double* W, void* workspace,
@@ -120,9 +121,10 @@ int wb_zgeev_batched(wb_ctx* ctx, int n, double* A, double* w, double* W, void* 
   const size_t nn = (size_t)n * n * sizeof(cd);
   cd* Z = (cd*)p; p += align_up(nn * batch);
   cd* X = (cd*)p; p += align_up(nn * batch);
-  cd* tau = (cd*)p;
+  cd* tau = (cd*)p; p += align_up((size_t)n * batch * sizeof(cd));
+  void* rot = p;
   return hip_status(ctx,
-                    wb_zgeev_launch(n, (cd*)A, (cd*)w, (cd*)W, Z, X, tau, info, batch,
+                    wb_zgeev_launch(n, (cd*)A, (cd*)w, (cd*)W, Z, X, tau, rot, info, batch,
                                     (hipStream_t)stream),
                     "wb_zgeev_batched");
 }

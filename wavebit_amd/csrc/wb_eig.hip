@@ -7,10 +7,9 @@
 //              and then from the left (a thread per column: coalesced column sweep).
 //   k_unghr    forms Q explicitly from the stored reflectors (backward accumulation) and
 //              clears the reflector storage below the subdiagonal.
-//   k_hqr      complex single-shift QR iteration on the Hessenberg matrix with Wilkinson
-//              shifts, exceptional shifts every 10 stalled sweeps, the Ahues-Tisseur
-//              deflation test, and Schur vectors accumulated into Z (Golub & Van Loan,
-//              Matrix Computations, 4th ed., 7.5; the zlahqr scheme).
+//   k_hqr_ms   (wb_hqr.hip) windowed small-bulge multishift QR iteration with Schur vectors
+//              accumulated into Z; small active blocks finish with single-shift QR
+//              (Wilkinson shifts, Ahues-Tisseur deflation test; Golub & Van Loan 7.5).
 //   k_trevc    eigenvectors of the triangular Schur factor by back substitution, one thread
 //              per eigenvector, synchronised by row so column i of T is a broadcast read.
 //   (GEMM)     W = Z X on the matrix cores, then k_colnorm scales each column to unit
@@ -23,6 +22,7 @@
 namespace {
 
 constexpr int NT = 256;
+constexpr int NTH = 1024;   // Hessenberg stages: 16 waves per matrix to hide global latency
 constexpr double ULP = DBL_EPSILON;            // 2^-52, LAPACK dlamch('P')
 constexpr double SAFMIN = DBL_MIN;
 
@@ -33,6 +33,7 @@ __device__ inline double wave_sum(double v) {
 }
 
 // Block-wide sum of one double per thread (all threads receive the total).
+template <int NTB>
 __device__ inline double block_sum(double v, double* red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -41,26 +42,26 @@ __device__ inline double block_sum(double v, double* red) {
   __syncthreads();
   double s = 0.0;
 #pragma unroll
-  for (int w = 0; w < NT / 64; ++w) s += red[w];
+  for (int w = 0; w < NTB / 64; ++w) s += red[w];
   return s;
 }
 
 // ----------------------------------------------------------------------------- Hessenberg
 
-__global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
+__global__ __launch_bounds__(NTH) void k_gehrd(int n, cd* Ab, cd* taub) {
   const long long b = blockIdx.x;
   cd* A = Ab + b * (long long)n * n;
   cd* tau = taub + b * (long long)n;
   extern __shared__ cd smem[];
   cd* v = smem;            // reflector, indices k+1..n-1 stored at v[0..m)
-  __shared__ double red[NT / 64];
+  __shared__ double red[NTH / 64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;   // reflector length
     // --- generate H = I - tau v v^H with H^H x = beta e1, x = A[k+1:n, k]
     double ss = 0.0;
-    for (int i = k + 2 + t; i < n; i += NT) ss += cnorm2(A[(long long)i * n + k]);
-    const double xnorm2 = block_sum(ss, red);
+    for (int i = k + 2 + t; i < n; i += NTH) ss += cnorm2(A[(long long)i * n + k]);
+    const double xnorm2 = block_sum<NTH>(ss, red);
     const cd alpha = A[(long long)(k + 1) * n + k];
     cd tk;
     double beta;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
       tk = mkc((beta - alpha.x) / beta, -alpha.y / beta);
       scale = crecip(mkc(alpha.x - beta, alpha.y));
     }
-    for (int i = t; i < m; i += NT) {
+    for (int i = t; i < m; i += NTH) {
       if (i == 0) v[0] = mkc(1.0, 0.0);
       else v[i] = cmul(A[(long long)(k + 1 + i) * n + k], scale);
     }
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
     if (t == 0) tau[k] = tk;
     if (tk.x != 0.0 || tk.y != 0.0) {
       // --- right: A[:, k+1:n] -= tau (A[:, k+1:n] v) v^H, a wave per row
-      for (int r = wave; r < n; r += NT / 64) {
+      for (int r = wave; r < n; r += NTH / 64) {
         cd* row = A + (long long)r * n + k + 1;
         double zr = 0.0, zi = 0.0;
         for (int j = lane; j < m; j += 64) {
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
       __syncthreads();
       // --- left: A[k+1:n, k+1:n] -= conj(tau) v (v^H A), a thread per column
       const cd tc = cconj(tk);
-      for (int j = k + 1 + t; j < n; j += NT) {
+      for (int j = k + 1 + t; j < n; j += NTH) {
         double sr = 0.0, si = 0.0;
         for (int i = 0; i < m; ++i) {
           cd p = cmulc(v[i], A[(long long)(k + 1 + i) * n + j]);
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
     }
     __syncthreads();
     // --- store beta and the reflector below the subdiagonal
-    for (int i = t; i < m; i += NT) {
+    for (int i = t; i < m; i += NTH) {
       cd* p = A + (long long)(k + 1 + i) * n + k;
       *p = (i == 0) ? mkc(beta, 0.0) : v[i];
     }
@@ -121,14 +122,14 @@ __global__ __launch_bounds__(NT) void k_gehrd(int n, cd* Ab, cd* taub) {
 }
 
 // Q = H_0 H_1 ... H_{n-3}; clears the reflectors below the subdiagonal of A afterwards.
-__global__ __launch_bounds__(NT) void k_unghr(int n, cd* Ab, const cd* taub, cd* Qb) {
+__global__ __launch_bounds__(NTH) void k_unghr(int n, cd* Ab, const cd* taub, cd* Qb) {
   const long long b = blockIdx.x;
   cd* A = Ab + b * (long long)n * n;
   const cd* tau = taub + b * (long long)n;
   cd* Q = Qb + b * (long long)n * n;
   extern __shared__ cd v[];
   const int t = threadIdx.x;
-  for (long long e = t; e < (long long)n * n; e += NT) {
+  for (long long e = t; e < (long long)n * n; e += NTH) {
     const int r = (int)(e / n), c = (int)(e % n);
     Q[e] = mkc(r == c ? 1.0 : 0.0, 0.0);
   }
@@ -136,10 +137,10 @@ __global__ __launch_bounds__(NT) void k_unghr(int n, cd* Ab, const cd* taub, cd*
   for (int k = n - 3; k >= 0; --k) {
     const int m = n - k - 1;
     const cd tk = tau[k];
-    for (int i = t; i < m; i += NT) v[i] = (i == 0) ? mkc(1.0, 0.0) : A[(long long)(k + 1 + i) * n + k];
+    for (int i = t; i < m; i += NTH) v[i] = (i == 0) ? mkc(1.0, 0.0) : A[(long long)(k + 1 + i) * n + k];
     __syncthreads();
     if (tk.x != 0.0 || tk.y != 0.0) {
-      for (int j = k + 1 + t; j < n; j += NT) {
+      for (int j = k + 1 + t; j < n; j += NTH) {
         double sr = 0.0, si = 0.0;
         for (int i = 0; i < m; ++i) {
           cd p = cmulc(v[i], Q[(long long)(k + 1 + i) * n + j]);
@@ -154,159 +155,10 @@ __global__ __launch_bounds__(NT) void k_unghr(int n, cd* Ab, const cd* taub, cd*
     }
     __syncthreads();
   }
-  for (long long e = t; e < (long long)n * n; e += NT) {
+  for (long long e = t; e < (long long)n * n; e += NTH) {
     const int r = (int)(e / n), c = (int)(e % n);
     if (r > c + 1) A[e] = mkc(0.0, 0.0);
   }
-}
-
-// ----------------------------------------------------------------------------- QR iteration
-
-__global__ __launch_bounds__(NT) void k_hqr(int n, cd* Hb, cd* Zb, cd* wb, int* info) {
-  const long long b = blockIdx.x;
-  cd* H = Hb + b * (long long)n * n;
-  cd* Z = Zb + b * (long long)n * n;
-  cd* wout = wb + b * (long long)n;
-  const int t = threadIdx.x;
-  __shared__ int s_k[NT];
-  const double smlnum = SAFMIN * ((double)n / ULP);
-  const int itmax = 30 * (n > 10 ? n : 10);
-  const int kexsh = 10;
-  const double dat1 = 0.75;
-#define HH(r, c) H[(long long)(r) * n + (c)]
-  int i = n - 1;
-  int kdefl = 0;
-  while (i >= 0) {
-    int l = 0;
-    bool converged = false;
-    for (int its = 0; its <= itmax; ++its) {
-      // --- largest k in (l, i] with a negligible subdiagonal, else l
-      int found = l;
-      for (int k = l + 1 + t; k <= i; k += NT) {
-        const cd hk = HH(k, k - 1);
-        bool small = cabs1(hk) <= smlnum;
-        if (!small) {
-          double tst = cabs1(HH(k - 1, k - 1)) + cabs1(HH(k, k));
-          if (tst == 0.0) {
-            if (k - 2 >= 0) tst += fabs(HH(k - 1, k - 2).x);
-            if (k + 1 <= n - 1) tst += fabs(HH(k + 1, k).x);
-          }
-          if (cabs1(hk) <= ULP * tst) {
-            const double h1 = cabs1(hk), h2 = cabs1(HH(k - 1, k));
-            const double ab = fmax(h1, h2), ba = fmin(h1, h2);
-            const double g1 = cabs1(HH(k, k)), g2 = cabs1(csub(HH(k - 1, k - 1), HH(k, k)));
-            const double aa = fmax(g1, g2), bb = fmin(g1, g2);
-            const double s = aa + ab;
-            small = ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
-          }
-        }
-        if (small && k > found) found = k;
-      }
-      s_k[t] = found;
-      __syncthreads();
-      for (int s = NT / 2; s > 0; s >>= 1) {
-        if (t < s && s_k[t + s] > s_k[t]) s_k[t] = s_k[t + s];
-        __syncthreads();
-      }
-      l = s_k[0];
-      __syncthreads();
-      if (l > 0) {
-        if (t == 0) HH(l, l - 1) = mkc(0.0, 0.0);
-      }
-      if (l >= i) { converged = true; break; }
-      ++kdefl;
-      // --- shift
-      cd sh;
-      if (kdefl % (2 * kexsh) == 0) {
-        const double s = dat1 * fabs(HH(i, i - 1).x);
-        sh = cadd(mkc(s, 0.0), HH(i, i));
-      } else if (kdefl % kexsh == 0) {
-        const double s = dat1 * fabs(HH(l + 1, l).x);
-        sh = cadd(mkc(s, 0.0), HH(l, l));
-      } else {
-        sh = HH(i, i);
-        const cd u = cmul(csqrt_p(HH(i - 1, i)), csqrt_p(HH(i, i - 1)));
-        double s = cabs1(u);
-        if (s != 0.0) {
-          const cd x = cscale(csub(HH(i - 1, i - 1), sh), 0.5);
-          const double sx = cabs1(x);
-          s = fmax(s, cabs1(x));
-          const cd xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
-          cd y = cscale(csqrt_p(cadd(cmul(xs, xs), cmul(us, us))), s);
-          if (sx > 0.0) {
-            const cd xn = cscale(x, 1.0 / sx);
-            if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
-          }
-          sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
-        }
-      }
-      // --- single-shift sweep over the active block [l, i]
-      for (int k = l; k < i; ++k) {
-        cd v1, v2;
-        if (k == l) {
-          v1 = csub(HH(l, l), sh);
-          v2 = HH(l + 1, l);
-          const double s = cabs1(v1) + cabs1(v2);
-          if (s != 0.0) { v1 = cscale(v1, 1.0 / s); v2 = cscale(v2, 1.0 / s); }
-        } else {
-          v1 = HH(k, k - 1);
-          v2 = HH(k + 1, k - 1);
-        }
-        // 2-element reflector: H = I - tau [1; u][1; u]^H, H^H [v1; v2] = [beta; 0]
-        cd tk, u;
-        double beta;
-        const double x2 = cnorm2(v2);
-        if (x2 == 0.0 && v1.y == 0.0) {
-          tk = mkc(0.0, 0.0); u = mkc(0.0, 0.0); beta = v1.x;
-        } else {
-          beta = -copysign(sqrt(v1.x * v1.x + v1.y * v1.y + x2), v1.x);
-          tk = mkc((beta - v1.x) / beta, -v1.y / beta);
-          u = cdiv(v2, mkc(v1.x - beta, v1.y));
-        }
-        __syncthreads();
-        if (k > l && t == 0) {
-          HH(k, k - 1) = mkc(beta, 0.0);
-          HH(k + 1, k - 1) = mkc(0.0, 0.0);
-        }
-        if (tk.x != 0.0 || tk.y != 0.0) {
-          const cd tc = cconj(tk), uc = cconj(u);
-          // left, rows k and k+1, columns k..n-1
-          for (int j = k + t; j < n; j += NT) {
-            const cd a0 = HH(k, j), a1 = HH(k + 1, j);
-            const cd s = cmul(tc, cadd(a0, cmul(uc, a1)));
-            HH(k, j) = csub(a0, s);
-            HH(k + 1, j) = csub(a1, cmul(s, u));
-          }
-          __syncthreads();
-          // right, columns k and k+1, rows 0..min(k+2, i); and Z rows 0..n-1
-          const int rmax = (k + 2 < i) ? k + 2 : i;
-          for (int r = t; r <= rmax; r += NT) {
-            const cd a0 = HH(r, k), a1 = HH(r, k + 1);
-            const cd s = cmul(tk, cadd(a0, cmul(u, a1)));
-            HH(r, k) = csub(a0, s);
-            HH(r, k + 1) = csub(a1, cmul(s, uc));
-          }
-          for (int r = t; r < n; r += NT) {
-            cd* zr = Z + (long long)r * n + k;
-            const cd a0 = zr[0], a1 = zr[1];
-            const cd s = cmul(tk, cadd(a0, cmul(u, a1)));
-            zr[0] = csub(a0, s);
-            zr[1] = csub(a1, cmul(s, uc));
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (!converged) {
-      if (t == 0) info[b] = i + 1;
-      break;
-    }
-    if (t == 0) wout[i] = HH(i, i);
-    kdefl = 0;
-    i = l - 1;
-    __syncthreads();
-  }
-#undef HH
 }
 
 // ----------------------------------------------------------------------------- eigenvectors
@@ -378,8 +230,8 @@ __global__ __launch_bounds__(NT) void k_colnorm(int n, cd* Wb) {
 
 // A (batch of n x n) is destroyed. Outputs: w (batch x n) eigenvalues, W (batch x n x n)
 // eigenvectors in columns. Workspace: Z (batch x n x n), X (batch x n x n), tau (batch x n).
-hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, int* info, int batch,
-                           hipStream_t stream) {
+hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
+                           int batch, hipStream_t stream) {
   if (n <= 0 || batch <= 0) return hipSuccess;
   const size_t sh = sizeof(cd) * (size_t)n;
   // Algorithmic flops per stage, from EIG(n) = 100 n^3 for the whole complex eigensolve:
@@ -388,13 +240,15 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, in
   const double n3 = (double)n * n * n;
   hipError_t z = wb_zero_int_launch(info, batch, stream);
   if (z != hipSuccess) return z;
-  WB_TIMED_F(WB_T_GEHRD, stream, 40.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(NT), sh, stream, n, A, tau));
+  WB_TIMED_F(WB_T_GEHRD, stream, 40.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(NTH), sh, stream, n, A, tau));
   WB_CHECK_LAUNCH();
   WB_TIMED_F(WB_T_UNGHR, stream, 16.0 / 3.0 * n3 * batch,
-           hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(NT), sizeof(cd) * (size_t)n, stream, n, A, tau, Z));
+           hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, A, tau, Z));
   WB_CHECK_LAUNCH();
-  WB_TIMED_F(WB_T_HQR, stream, 72.0 * n3 * batch, hipLaunchKernelGGL(k_hqr, dim3(batch), dim3(NT), 0, stream, n, A, Z, w, info));
-  WB_CHECK_LAUNCH();
+  {
+    hipError_t e2 = wb_hqr_ms_launch(n, A, Z, w, info, rot_ws, batch, stream);
+    if (e2 != hipSuccess) return e2;
+  }
   WB_TIMED_F(WB_T_TREVC, stream, 4.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(NT), 0, stream, n, A, X));
   WB_CHECK_LAUNCH();
   hipError_t e = wb_zgemm_launch(n, n, n, mkc(1.0, 0.0), bp_strided(Z, (long long)n * n), n,

@@ -1,0 +1,480 @@
+// Windowed small-bulge multishift QR for the complex Hessenberg eigenproblem, with Schur
+// vectors (one 256-thread workgroup per matrix).
+//
+// A multishift sweep chases M single-shift bulges (2-element reflectors) down the active block
+// [lo, hi] in a tightly packed chain: bulge j enters at step 3j and at step s acts at position
+// k = lo + s - 3j. Spacing 3 keeps the bulges' reflector inputs disjoint, so each step computes
+// all M reflectors, then applies all M row operations, then all M column operations, and the
+// result equals M consecutive single-shift sweeps (the implicit-Q argument behind
+// Braman-Byers-Mathias small-bulge multishift QR; Golub & Van Loan 7.5.7).
+//
+// The chase runs in a W x W diagonal window held in LDS. Everything outside the window is
+// deferred and applied from the recorded reflectors by a register "band": in step s, bulge j
+// always touches band slots 3(M-1-j) and 3(M-1-j)+1 of the band [lo+s-3(M-1), lo+s+1], so a
+// strip of H or Z streams through B = 3(M-1)+2 registers with compile-time indices, each element
+// read and written once per window (H right of and above the window) or once per sweep (Z).
+// Shifts are the eigenvalues of the trailing M x M block; small active blocks finish with the
+// single-shift iteration.
+#include "wb_internal.h"
+#include "wb_timing.h"
+#include <float.h>
+
+namespace {
+
+constexpr int NT = 256;
+constexpr double ULP = DBL_EPSILON;
+constexpr double SAFMIN = DBL_MIN;
+
+struct Rot {
+  cd tau;
+  cd u;
+};
+
+// 2-element reflector H = I - tau [1;u][1;u]^H with H^H [x1; x2] = [beta; 0].
+__device__ inline void refl2(cd x1, cd x2, cd* tau, cd* u, double* beta) {
+  const double x2n = cnorm2(x2);
+  if (x2n == 0.0 && x1.y == 0.0) {
+    *tau = mkc(0.0, 0.0); *u = mkc(0.0, 0.0); *beta = x1.x;
+    return;
+  }
+  const double b = -copysign(sqrt(x1.x * x1.x + x1.y * x1.y + x2n), x1.x);
+  *beta = b;
+  *tau = mkc((b - x1.x) / b, -x1.y / b);
+  *u = cdiv(x2, mkc(x1.x - b, x1.y));
+}
+
+// left application (rows k, k+1): [a0; a1] <- H^H [a0; a1]
+__device__ inline void rot_left(const Rot& r, cd& a0, cd& a1) {
+  const cd s = cmul(cconj(r.tau), cadd(a0, cmulc(r.u, a1)));
+  a0 = csub(a0, s);
+  a1 = csub(a1, cmul(s, r.u));
+}
+// right application (columns k, k+1): [a0, a1] <- [a0, a1] H
+__device__ inline void rot_right(const Rot& r, cd& a0, cd& a1) {
+  const cd s = cmul(r.tau, cadd(a0, cmul(r.u, a1)));
+  a0 = csub(a0, s);
+  a1 = csub(a1, cmul(s, cconj(r.u)));
+}
+
+// Eigenvalues of a small upper Hessenberg matrix (row-major, ld) by single-shift QR, in
+// place; one thread. Returns 0 on success.
+__device__ int small_eigvals(cd* A, int m, int ld, cd* out) {
+  const double smlnum = SAFMIN * ((double)m / ULP);
+  int i = m - 1;
+  int kdefl = 0;
+  while (i >= 0) {
+    int l = 0;
+    bool conv = false;
+    for (int its = 0; its <= 30 * 10; ++its) {
+      int k;
+      for (k = i; k > l; --k) {
+        const cd hk = A[k * ld + k - 1];
+        if (cabs1(hk) <= smlnum) break;
+        double tst = cabs1(A[(k - 1) * ld + k - 1]) + cabs1(A[k * ld + k]);
+        if (cabs1(hk) <= ULP * tst) break;
+      }
+      l = k;
+      if (l > 0) A[l * ld + l - 1] = mkc(0.0, 0.0);
+      if (l >= i) { conv = true; break; }
+      ++kdefl;
+      cd sh = A[i * ld + i];
+      if (kdefl % 10 == 0) {
+        sh = cadd(sh, mkc(0.75 * cabs1(A[i * ld + i - 1]), 0.0));
+      } else {
+        const cd u = cmul(csqrt_p(A[(i - 1) * ld + i]), csqrt_p(A[i * ld + i - 1]));
+        double s = cabs1(u);
+        if (s != 0.0) {
+          const cd x = cscale(csub(A[(i - 1) * ld + i - 1], sh), 0.5);
+          const double sx = cabs1(x);
+          s = fmax(s, sx);
+          cd y = cscale(csqrt_p(cadd(cmul(cscale(x, 1.0 / s), cscale(x, 1.0 / s)),
+                                     cmul(cscale(u, 1.0 / s), cscale(u, 1.0 / s)))), s);
+          if (sx > 0.0) {
+            const cd xn = cscale(x, 1.0 / sx);
+            if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
+          }
+          sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
+        }
+      }
+      for (int kk = l; kk < i; ++kk) {
+        cd x1, x2;
+        if (kk == l) { x1 = csub(A[l * ld + l], sh); x2 = A[(l + 1) * ld + l]; }
+        else { x1 = A[kk * ld + kk - 1]; x2 = A[(kk + 1) * ld + kk - 1]; }
+        Rot r;
+        double beta;
+        refl2(x1, x2, &r.tau, &r.u, &beta);
+        if (kk > l) { A[kk * ld + kk - 1] = mkc(beta, 0.0); A[(kk + 1) * ld + kk - 1] = mkc(0.0, 0.0); }
+        for (int j = kk; j <= i; ++j) rot_left(r, A[kk * ld + j], A[(kk + 1) * ld + j]);
+        const int rmax = (kk + 2 < i) ? kk + 2 : i;
+        for (int q = l; q <= rmax; ++q) rot_right(r, A[q * ld + kk], A[q * ld + kk + 1]);
+      }
+    }
+    if (!conv) return 1;
+    out[i] = A[i * ld + i];
+    kdefl = 0;
+    i = l - 1;
+  }
+  return 0;
+}
+
+// Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo.
+__device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
+  const int t = threadIdx.x;
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  int found = lo;
+  for (int k = lo + 1 + t; k <= hi; k += NT) {
+    const cd hk = H[(long long)k * n + k - 1];
+    bool small = cabs1(hk) <= smlnum;
+    if (!small) {
+      double tst = cabs1(H[(long long)(k - 1) * n + k - 1]) + cabs1(H[(long long)k * n + k]);
+      if (tst == 0.0) {
+        if (k - 2 >= 0) tst += fabs(H[(long long)(k - 1) * n + k - 2].x);
+        if (k + 1 <= n - 1) tst += fabs(H[(long long)(k + 1) * n + k].x);
+      }
+      if (cabs1(hk) <= ULP * tst) {
+        const double h1 = cabs1(hk), h2 = cabs1(H[(long long)(k - 1) * n + k]);
+        const double ab = fmax(h1, h2), ba = fmin(h1, h2);
+        const double g1 = cabs1(H[(long long)k * n + k]);
+        const double g2 = cabs1(csub(H[(long long)(k - 1) * n + k - 1], H[(long long)k * n + k]));
+        const double aa = fmax(g1, g2), bb = fmin(g1, g2);
+        const double s = aa + ab;
+        small = ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
+      }
+    }
+    if (small && k > found) found = k;
+  }
+  s_k[t] = found;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s && s_k[t + s] > s_k[t]) s_k[t] = s_k[t + s];
+    __syncthreads();
+  }
+  const int r = s_k[0];
+  __syncthreads();
+  return r;
+}
+
+// Single-shift iteration on the active block [lo_lim, hi] until all its eigenvalues converge
+// (the zlahqr scheme, rotations applied straight to global H and Z). Returns 0 on success.
+__device__ int single_shift_block(cd* H, cd* Z, cd* wout, int n, int lo_lim, int hi, int* s_k) {
+  const int t = threadIdx.x;
+#define HH(r, c) H[(long long)(r) * n + (c)]
+  int i = hi;
+  int kdefl = 0;
+  const int itmax = 30 * ((hi - lo_lim + 1) > 10 ? (hi - lo_lim + 1) : 10);
+  while (i >= lo_lim) {
+    int l = lo_lim;
+    bool converged = false;
+    for (int its = 0; its <= itmax; ++its) {
+      l = find_split(H, n, l, i, s_k);
+      if (t == 0 && l > 0) HH(l, l - 1) = mkc(0.0, 0.0);
+      if (l >= i) { converged = true; break; }
+      ++kdefl;
+      cd sh;
+      if (kdefl % 20 == 0) {
+        sh = cadd(mkc(0.75 * fabs(HH(i, i - 1).x), 0.0), HH(i, i));
+      } else if (kdefl % 10 == 0) {
+        sh = cadd(mkc(0.75 * fabs(HH(l + 1, l).x), 0.0), HH(l, l));
+      } else {
+        sh = HH(i, i);
+        const cd u = cmul(csqrt_p(HH(i - 1, i)), csqrt_p(HH(i, i - 1)));
+        double s = cabs1(u);
+        if (s != 0.0) {
+          const cd x = cscale(csub(HH(i - 1, i - 1), sh), 0.5);
+          const double sx = cabs1(x);
+          s = fmax(s, sx);
+          const cd xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
+          cd y = cscale(csqrt_p(cadd(cmul(xs, xs), cmul(us, us))), s);
+          if (sx > 0.0) {
+            const cd xn = cscale(x, 1.0 / sx);
+            if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
+          }
+          sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
+        }
+      }
+      for (int k = l; k < i; ++k) {
+        cd v1, v2;
+        if (k == l) {
+          v1 = csub(HH(l, l), sh);
+          v2 = HH(l + 1, l);
+          const double s = cabs1(v1) + cabs1(v2);
+          if (s != 0.0) { v1 = cscale(v1, 1.0 / s); v2 = cscale(v2, 1.0 / s); }
+        } else {
+          v1 = HH(k, k - 1);
+          v2 = HH(k + 1, k - 1);
+        }
+        Rot r;
+        double beta;
+        refl2(v1, v2, &r.tau, &r.u, &beta);
+        __syncthreads();
+        if (k > l && t == 0) {
+          HH(k, k - 1) = mkc(beta, 0.0);
+          HH(k + 1, k - 1) = mkc(0.0, 0.0);
+        }
+        if (r.tau.x != 0.0 || r.tau.y != 0.0) {
+          for (int j = k + t; j < n; j += NT) rot_left(r, HH(k, j), HH(k + 1, j));
+          __syncthreads();
+          const int rmax = (k + 2 < i) ? k + 2 : i;
+          for (int q = t; q <= rmax; q += NT) rot_right(r, HH(q, k), HH(q, k + 1));
+          for (int q = t; q < n; q += NT) {
+            cd* zr = Z + (long long)q * n + k;
+            rot_right(r, zr[0], zr[1]);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!converged) return i + 1;
+    if (t == 0) wout[i] = HH(i, i);
+    kdefl = 0;
+    i = l - 1;
+    __syncthreads();
+  }
+#undef HH
+  return 0;
+}
+
+// Apply the steps [s0, s1) of a sweep's reflectors to one strip of B-register band.
+// left: row operations on a column strip (rows), else column operations on a row strip.
+// get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid range.
+template <int M, bool LEFT, typename Get, typename Put>
+__device__ inline void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
+                                  int vlo, int vhi, Get get, Put put) {
+  constexpr int B = 3 * (M - 1) + 2;
+  cd band[B];
+  const int base0 = lo + s0 - 3 * (M - 1);
+#pragma unroll
+  for (int i = 0; i < B - 1; ++i) {
+    const int idx = base0 + i;
+    band[i] = (idx >= vlo && idx < vhi) ? get(idx) : mkc(0.0, 0.0);
+  }
+  for (int s = s0; s < s1; ++s) {
+    const int base = lo + s - 3 * (M - 1);
+    const int top = base + B - 1;
+    band[B - 1] = (top >= vlo && top < vhi) ? get(top) : mkc(0.0, 0.0);
+    const Rot* rs = rots + (long long)(s - s0) * rot_stride;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const Rot r = rs[j];
+      constexpr int dummy = 0;
+      (void)dummy;
+      const int slot = 3 * (M - 1 - j);
+      if (LEFT) rot_left(r, band[slot], band[slot + 1]);
+      else rot_right(r, band[slot], band[slot + 1]);
+    }
+    if (base >= vlo && base < vhi) put(base, band[0]);
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i) band[i] = band[i + 1];
+  }
+  const int base_end = lo + s1 - 3 * (M - 1);
+#pragma unroll
+  for (int i = 0; i < B - 1; ++i) {
+    const int idx = base_end + i;
+    if (idx >= vlo && idx < vhi) put(idx, band[i]);
+  }
+}
+
+// One multishift sweep over the active block [lo, hi] with shifts sh[0..M).
+template <int M>
+__device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* sh, int W,
+                                 cd* Hw, Rot* rot_win, Rot* rot_all) {
+  const int t = threadIdx.x;
+  const int T = (hi - 1 - lo) + 3 * (M - 1);       // last step
+  const int ldw = W + 1;
+  int s_begin = 0;
+  while (s_begin <= T) {
+    int kmin = lo + s_begin - 3 * (M - 1);
+    if (kmin < lo) kmin = lo;
+    const int a = (kmin == lo) ? lo : kmin - 1;
+    const int wend = (a + W < hi + 1) ? a + W : hi + 1;
+    const int ww = wend - a;
+    int s_end;
+    if (wend == hi + 1) s_end = T + 1;
+    else s_end = wend - 3 - lo + 1;
+    if (s_end > T + 1) s_end = T + 1;
+    // load window
+    for (int e = t; e < ww * ww; e += NT) {
+      const int r = e / ww, c = e % ww;
+      Hw[r * ldw + c] = H[(long long)(a + r) * n + a + c];
+    }
+    __syncthreads();
+    for (int s = s_begin; s < s_end; ++s) {
+      // reflectors
+      if (t < M) {
+        const int j = t;
+        const int k = lo + s - 3 * j;
+        Rot r;
+        r.tau = mkc(0.0, 0.0);
+        r.u = mkc(0.0, 0.0);
+        if (k >= lo && k <= hi - 1) {
+          cd x1, x2;
+          if (k == lo) {
+            x1 = csub(Hw[(k - a) * ldw + (k - a)], sh[j]);
+            x2 = Hw[(k + 1 - a) * ldw + (k - a)];
+            const double sc = cabs1(x1) + cabs1(x2);
+            if (sc != 0.0) { x1 = cscale(x1, 1.0 / sc); x2 = cscale(x2, 1.0 / sc); }
+          } else {
+            x1 = Hw[(k - a) * ldw + (k - 1 - a)];
+            x2 = Hw[(k + 1 - a) * ldw + (k - 1 - a)];
+          }
+          double beta;
+          refl2(x1, x2, &r.tau, &r.u, &beta);
+          if (k > lo) {
+            Hw[(k - a) * ldw + (k - 1 - a)] = mkc(beta, 0.0);
+            Hw[(k + 1 - a) * ldw + (k - 1 - a)] = mkc(0.0, 0.0);
+          }
+        }
+        rot_win[(s - s_begin) * M + j] = r;
+        rot_all[(long long)s * M + j] = r;
+      }
+      __syncthreads();
+      // row operations: bulge j on rows k, k+1, columns k..wend-1
+      for (int e = t; e < M * ww; e += NT) {
+        const int j = e / ww, c = a + e % ww;
+        const int k = lo + s - 3 * j;
+        if (k < lo || k > hi - 1 || c < k) continue;
+        const Rot r = rot_win[(s - s_begin) * M + j];
+        rot_left(r, Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
+      }
+      __syncthreads();
+      // column operations: bulge j on columns k, k+1, rows a..min(k+2, hi)
+      for (int e = t; e < M * ww; e += NT) {
+        const int j = e / ww, q = a + e % ww;
+        const int k = lo + s - 3 * j;
+        if (k < lo || k > hi - 1) continue;
+        const int rmax = (k + 2 < hi) ? k + 2 : hi;
+        if (q > rmax) continue;
+        const Rot r = rot_win[(s - s_begin) * M + j];
+        rot_right(r, Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+      }
+      __syncthreads();
+    }
+    // store window
+    for (int e = t; e < ww * ww; e += NT) {
+      const int r = e / ww, c = e % ww;
+      H[(long long)(a + r) * n + a + c] = Hw[r * ldw + c];
+    }
+    // deferred row operations on the columns right of the window (rows a..wend-1)
+    for (int c = wend + t; c < n; c += NT) {
+      band_apply<M, true>(lo, s_begin, s_end, rot_win, M, a, wend,
+                          [&](int i) { return H[(long long)i * n + c]; },
+                          [&](int i, cd v) { H[(long long)i * n + c] = v; });
+    }
+    // deferred column operations on the rows above the window (columns a..wend-1)
+    for (int q = t; q < a; q += NT) {
+      cd* row = H + (long long)q * n;
+      band_apply<M, false>(lo, s_begin, s_end, rot_win, M, a, wend,
+                           [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+    }
+    __syncthreads();
+    s_begin = s_end;
+  }
+  // Schur vectors: every row of Z gets the whole sweep's column operations
+  for (int q = t; q < n; q += NT) {
+    cd* row = Z + (long long)q * n;
+    band_apply<M, false>(lo, 0, T + 1, rot_all, M, lo, hi + 1,
+                         [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+  }
+  __syncthreads();
+}
+
+template <int M>
+__global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, int* info, Rot* rws,
+                                               int W, int nmin) {
+  const long long b = blockIdx.x;
+  cd* H = Hb + b * (long long)n * n;
+  cd* Z = Zb + b * (long long)n * n;
+  cd* wout = wb + b * (long long)n;
+  Rot* rot_all = rws + b * (long long)(n + 3 * M) * M;
+  extern __shared__ unsigned char smem[];
+  cd* Hw = (cd*)smem;
+  Rot* rot_win = (Rot*)(smem + sizeof(cd) * (size_t)W * (W + 1));
+  __shared__ int s_k[NT];
+  __shared__ cd s_sh[M];
+  __shared__ cd s_small[M * M];
+  __shared__ int s_flag;
+  const int t = threadIdx.x;
+  int hi = n - 1;
+  int lo = 0;
+  int stall = 0;
+  int sweeps = 0;
+  const int max_sweeps = 40 * (n / M + 10);
+  while (hi >= 0) {
+    lo = find_split(H, n, 0, hi, s_k);
+    if (lo > 0 && t == 0) H[(long long)lo * n + lo - 1] = mkc(0.0, 0.0);
+    __syncthreads();
+    if (lo == hi) {
+      if (t == 0) wout[hi] = H[(long long)hi * n + hi];
+      --hi;
+      stall = 0;
+      __syncthreads();
+      continue;
+    }
+    if (hi - lo + 1 <= nmin) {
+      const int r = single_shift_block(H, Z, wout, n, lo, hi, s_k);
+      if (r != 0) {
+        if (t == 0) info[b] = r;
+        return;
+      }
+      hi = lo - 1;
+      stall = 0;
+      __syncthreads();
+      continue;
+    }
+    if (++sweeps > max_sweeps) {
+      if (t == 0) info[b] = hi + 1;
+      return;
+    }
+    // shifts: eigenvalues of the trailing M x M block (exceptional shifts when stalled)
+    ++stall;
+    if (t == 0) {
+      const int b0 = hi - M + 1;
+      for (int r = 0; r < M; ++r)
+        for (int c = 0; c < M; ++c)
+          s_small[r * M + c] = (r > c + 1) ? mkc(0.0, 0.0) : H[(long long)(b0 + r) * n + b0 + c];
+      int bad = small_eigvals(s_small, M, M, s_sh);
+      if (bad || stall % 6 == 0) {
+        const double ex = 0.75 * cabs1(H[(long long)hi * n + hi - 1]);
+        for (int j = 0; j < M; ++j)
+          s_sh[j] = cadd(H[(long long)(hi - (j % 2)) * n + hi - (j % 2)],
+                         mkc(ex * (1.0 + 0.1 * j), 0.0));
+      }
+      s_flag = 0;
+    }
+    __syncthreads();
+    multishift_sweep<M>(H, Z, n, lo, hi, s_sh, W, Hw, rot_win, rot_all);
+  }
+}
+
+}  // namespace
+
+size_t wb_hqr_ms_rot_bytes(int n, int batch) {
+  return sizeof(Rot) * (size_t)(n + 3 * 8) * 8 * (size_t)batch;
+}
+
+// Multishift QR: H (Hessenberg, below-subdiagonal zero), Z (Q of the reduction) -> Schur T in H,
+// Schur vectors in Z, eigenvalues in w.
+hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws, int batch,
+                            hipStream_t stream) {
+  constexpr int M = 8;
+  static int W = 0;
+  if (W == 0) {
+    int dev = 0, smem = 65536;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const int cands[] = {64, 56, 48, 40, 32};
+    for (int c : cands) {
+      const size_t need = sizeof(cd) * (size_t)c * (c + 1) + sizeof(Rot) * (size_t)(c + 3 * M) * M +
+                          4096 + sizeof(int) * NT + sizeof(cd) * (M + M * M);
+      if (need <= (size_t)smem) { W = c; break; }
+    }
+    if (W == 0) W = 32;
+  }
+  // window + the window's reflectors (a window can hold up to W + 3M - 4 steps)
+  const size_t sh = sizeof(cd) * (size_t)W * (W + 1) + sizeof(Rot) * (size_t)(W + 3 * M) * M;
+  const int nmin = (W < 48) ? W : 48;
+  WB_TIMED_F(WB_T_HQR, stream, 72.0 * (double)n * n * n * batch,
+             hipLaunchKernelGGL(k_hqr_ms<M>, dim3(batch), dim3(NT), sh, stream, n, H, Z, w, info,
+                                (Rot*)rot_ws, W, nmin));
+  return hipGetLastError();
+}

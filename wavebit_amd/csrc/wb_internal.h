@@ -12,8 +12,11 @@ hipError_t wb_set_identity_launch(int n, BatchPtr B, int ldb, int batch, hipStre
 hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
                              int* info, int batch, hipStream_t stream);
 hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream);
-hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, int* info, int batch,
-                           hipStream_t stream);
+hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
+                           int batch, hipStream_t stream);
+size_t wb_hqr_ms_rot_bytes(int n, int batch);
+hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws, int batch,
+                            hipStream_t stream);
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
                                  cd* epy, cd* epz_i, void* ws, int* info_out, hipStream_t stream);

@@ -500,6 +500,7 @@ size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
   s += align_up(B * nn) * 6;          // M0..M5
   s += align_up(B * nn * 2);          // M6: S11 | S12 (row stride 2n)
   s += align_up(B * n * sizeof(cd)) * 3;   // w, q, tau
+  s += align_up(wb_hqr_ms_rot_bytes((int)n, (int)B));
   s += align_up(B * n * sizeof(int)) * 4;  // ipiv sets
   s += align_up(B * sizeof(int)) * 4;      // infos
   s += align_up((size_t)P * OV_COUNT * N * sizeof(cd));
@@ -525,6 +526,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   cd* w = (cd*)take((size_t)B * n * sizeof(cd));
   cd* q = (cd*)take((size_t)B * n * sizeof(cd));
   cd* tau = (cd*)take((size_t)B * n * sizeof(cd));
+  void* rotws = take(wb_hqr_ms_rot_bytes(n, B));
   int* ipiv[4];
   for (int i = 0; i < 4; ++i) ipiv[i] = (int*)take((size_t)B * n * sizeof(int));
   int* info[4];
@@ -569,7 +571,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1], nn), N, EPX, N, one, S(M[0], nn), n, B, st))) return e;
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1] + (long long)n * N, nn), N, EPY, N, one,
                              S(M[0] + N, nn), n, B, st))) return e;
-    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, info[0], B, st))) return e;
+    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st))) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
     // V GEMMs: top = -epy W_bot, bottom = epx W_top (into M4)
     if ((e = wb_zgemm_launch(N, n, N, mone, EPY, N, S(M[3] + (long long)N * n, nn), n, zero,
@@ -590,7 +592,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       WB_CHECK_LAUNCH();
       if ((e = wb_zgemm_launch(N, N, N, one, EPX, N, S(M[5], nn), N, zero, S(M[0], nn), N, B, st))) return e;
     }
-    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, info[0], B, st))) return e;
+    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st))) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
     cd* Bw = M[3];
     if (tm) {
