@@ -12,6 +12,10 @@ and the per-order efficiencies, in one launch sequence. ``value`` is points solv
 (one solve = one (problem, wavelength, angle, polarization) point). The plain form solves the
 same points one point per launch sequence; its rate and the ratio are reported beside it.
 
+The serving leg drives the continuous-batching server with a Poisson stream of requests at
+the same orders (1-4 layers, 1-8 wavelengths each, arriving while earlier ones are on the
+device) and reports its p50/p99 request latency and served solves/s.
+
 Also reported: the dominant kernel's roofline (algorithmic flops per launch / average launch
 time, from HIP events recorded on the launching stream during the timed steps) and the NumPy
 FP64 oracle timed on the host cores on a bounded sample of the same workload.
@@ -38,6 +42,8 @@ def main():
     ap.add_argument('--cpu-points', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--orders', type=int, default=9)
+    ap.add_argument('--serve-requests', type=int, default=48)
+    ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the points of a batch are '
@@ -152,6 +158,31 @@ def main():
                          f'the NumPy FP64 oracle, {cpu_dt:.1f} s'}
         oracle_err = max(errs)
 
+    # serving: a Poisson request stream through the continuous-batching server
+    serve_report = None
+    if args.serve_requests > 0:
+        from wavebit_amd.rcwa import serve
+        specs = workloads.serving_requests(args.serve_requests, seed=11, m_range=(M, M),
+                                           l_range=(1, 4), sizes=(64, 128), wl_range=(1, 8),
+                                           grad_fraction=0.0)
+        arrivals = serve.poisson_arrivals(len(specs), rate=args.serve_rate, seed=12)
+        reqs = []
+        for i, (s, t) in enumerate(zip(specs, arrivals)):
+            prob, num = workloads.to_problem(s)
+            reqs.append(serve.Request(problem=prob, numerics=num, arrival=float(t), id=i))
+        srv = serve.Server(device=dev, streams=3, max_points_per_launch=64)
+        results, rep = srv.serve(reqs, realtime=True)
+        failed = sum(1 for r in results.values() if isinstance(r, Exception))
+        serve_report = {'requests': rep['requests'], 'arrival_rate_per_s': args.serve_rate,
+                        'points': rep['points'], 'solves_per_s': round(rep['solves_per_s'], 4),
+                        'p50_latency_s': round(rep['p50_latency_s'], 4),
+                        'p99_latency_s': round(rep['p99_latency_s'], 4),
+                        'launches': rep['launches'],
+                        'mean_points_per_launch': round(rep['mean_points_per_launch'], 2),
+                        'failed_requests': failed,
+                        'stream': f'orders ({M},{M}), 1-4 layers of 64x64/128x128 rasters eps '
+                                  'U[1,16]+iU[0,1], 1-8 wavelengths in [400,1000] nm, Poisson arrivals'}
+
     fl_pt = workloads.flops_per_point((hu, hv), L)
     line = {
         'metric': 'RCWA solves/s @ (9,9) orders FP64 vs plain launch; p50/p99 serve latency',
@@ -168,6 +199,9 @@ def main():
         'batched_vs_plain': round(batched_rate / plain_rate, 3),
         'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
         'roofline': roofline, 'cpu_baseline': cpu,
+        'serve': serve_report,
+        'p50_latency_s': serve_report['p50_latency_s'] if serve_report else None,
+        'p99_latency_s': serve_report['p99_latency_s'] if serve_report else None,
         'oracle_max_abs_err_RT': oracle_err, 'flagged_points': n_flagged,
         'kernels_ms': {k: round(v['total_ms'] / args.steps, 3) for k, v in kernels.items()},
     }

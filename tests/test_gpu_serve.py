@@ -1,0 +1,46 @@
+"""The continuous-batching server on the GPU: every request's result is the same alone (through
+``rcwa.run``) and in the mix, per-order R/T within 1e-10; latencies are reported."""
+import numpy as np
+import pytest
+
+from wavebit_amd import rcwa, workloads
+from wavebit_amd.rcwa import serve
+
+pytestmark = pytest.mark.gpu
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    specs = workloads.serving_requests(12, seed=5, m_range=(2, 4), l_range=(1, 3), sizes=(16, 32),
+                                       wl_range=(1, 5), grad_fraction=0.0)
+    arrivals = serve.poisson_arrivals(len(specs), rate=200.0, seed=1)
+    reqs = []
+    for i, (s, t) in enumerate(zip(specs, arrivals)):
+        problem, numerics = workloads.to_problem(s)
+        reqs.append(serve.Request(problem=problem, numerics=numerics, arrival=float(t), id=i))
+    srv = serve.Server(streams=3, max_points_per_launch=8)
+    results, report = srv.serve(reqs)
+    assert report['requests'] == 12 and report['points'] == sum(len(s['wavelengths']) for s in specs)
+    assert report['p99_latency_s'] >= report['p50_latency_s'] > 0
+    assert report['mean_points_per_launch'] > 1.0          # points of different requests shared launches
+    for i, s in enumerate(specs):
+        problem, numerics = workloads.to_problem(s)
+        alone = rcwa.run(problem, numerics=numerics, backend='torch', device='cuda:0')
+        mixed = results[i]
+        assert not isinstance(mixed, Exception), mixed
+        assert np.abs(mixed.R - alone.R).max() < 1e-10
+        assert np.abs(mixed.T - alone.T).max() < 1e-10
+        assert np.abs(mixed.transmission.amplitude_ratio_s - alone.transmission.amplitude_ratio_s).max() < 1e-10
+
+
+def test_a_bad_request_fails_alone():
+    good = workloads.serving_requests(2, seed=2, m_range=(2, 2), l_range=(1, 1), sizes=(8,),
+                                      wl_range=(2, 2), grad_fraction=0.0)
+    p0, n0 = workloads.to_problem(good[0])
+    bad = dict(good[1])
+    bad['rasters'] = bad['rasters'].copy()
+    bad['rasters'][0, 0, 0] = np.nan
+    p1, n1 = workloads.to_problem(bad)
+    srv = serve.Server(streams=2)
+    results, _ = srv.serve([serve.Request(p0, n0, 0.0, 0), serve.Request(p1, n1, 0.0, 1)], realtime=False)
+    assert not isinstance(results[0], Exception)
+    assert isinstance(results[1], Exception)

@@ -1,0 +1,262 @@
+"""Continuous-batching RCWA server on one GPU.
+
+Requests (a ``Problem`` with its ``Numerics``) arrive over time. Each is resolved into plane-wave
+points; points wait in per-bucket queues keyed by (order window, layer count, solve path), and
+a launch packs queued points of one bucket -- from any number of requests -- into one
+``wb_solve`` launch sequence. Launches run on several HIP streams at once, so work admitted
+while others are on the device overlaps with them. Admission is by HBM footprint: a launch
+is started only while the workspaces of all launches in flight plus the device-resident
+convolution matrices fit the memory budget; a request's points are admitted as slots free
+up. Convolution matrices of each (raster stack, window) stay on the device in an LRU cache
+and are reused by every later request of the same design. A request completes when its
+last point has been solved; its latency is completion time minus arrival time.
+
+The reference has no serving loop (one solver per wavelength, ``_lifecycle.py:712-727``);
+this is the batching layer around the same per-point arithmetic, so a request's result is the
+same alone and in a mix.
+"""
+import collections
+import hashlib
+import heapq
+import itertools
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _native
+from . import _engine
+from ._lifecycle import Execution, _resolve, _runtime_checks
+from .results import pack
+from ..errors import NumericalError, ValidationError
+
+
+@dataclass
+class Request:
+    problem: object
+    numerics: object
+    arrival: float = 0.0              # seconds on the server's clock
+    id: int = -1
+
+
+@dataclass
+class _Work:
+    req: Request
+    resolved: object
+    n_points: int
+    results: dict = field(default_factory=dict)
+    done: int = 0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+
+
+class ConvLRU:
+    """Device-resident convolution matrices, least-recently-used eviction under a byte budget."""
+
+    def __init__(self, device, budget_bytes):
+        self.device = device
+        self.budget = budget_bytes
+        self.bytes = 0
+        self._d = collections.OrderedDict()
+        self.hits = 0
+        self.misses = 0
+
+    def get(self, rasters, hu, hv):
+        key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
+               rasters.shape, hu, hv)
+        hit = self._d.get(key)
+        if hit is not None:
+            self._d.move_to_end(key)
+            self.hits += 1
+            return hit
+        self.misses += 1
+        eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.device)
+        epx, epy, epz, info = _native.assemble_li(eps, hu, hv)
+        if int(info.abs().sum()) != 0:
+            raise NumericalError('a layer convolution matrix is singular', code='rcwa.singular',
+                                 path='stack.layers')
+        entry = (epx, epy, epz)
+        size = 3 * epx.numel() * 16
+        self._d[key] = entry
+        self.bytes += size
+        while self.bytes > self.budget and len(self._d) > 1:
+            _, old = self._d.popitem(last=False)
+            self.bytes -= 3 * old[0].numel() * 16
+        return entry
+
+
+class Server:
+    """Continuous batching with HBM-footprint admission and overlapping streams."""
+
+    def __init__(self, device='cuda:0', streams=3, memory_fraction=0.6, max_points_per_launch=64,
+                 conv_cache_fraction=0.1):
+        self.device = torch.device(device)
+        _native.context(self.device)
+        free, total = torch.cuda.mem_get_info(self.device)
+        self.budget = int(free * memory_fraction)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(max(1, streams))]
+        self.max_points = max_points_per_launch
+        self.conv = ConvLRU(self.device, int(free * conv_cache_fraction))
+        self.execution = Execution(backend='torch', device=str(self.device))
+        self.stats = collections.Counter()
+
+    # ------------------------------------------------------------------ request intake
+    def _admit(self, req, now):
+        r = _resolve(req.problem, req.numerics, self.execution)
+        checks = _runtime_checks(r)
+        bad = [k for k, ok in checks.items() if not ok]
+        if bad:
+            raise ValidationError(f'runtime guard {bad[0]} failed', code=f'prepared.{bad[0]}',
+                                  path='problem')
+        conv = self.conv.get(r.rasters, r.hu, r.hv)
+        work = _Work(req=req, resolved=r, n_points=len(r.wavelengths), t_admit=now)
+        work.checks = checks
+        pts = []
+        for i, wl in enumerate(r.wavelengths):
+            pts.append((work, i, _engine.Point(
+                stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
+                polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
+                period=r.period, thickness=tuple(float(t) for t in r.thickness)), conv))
+        key = (r.hu, r.hv, r.path, r.rasters.shape[0])
+        return work, key, pts
+
+    def _launch_bytes(self, key, P):
+        hu, hv, path, L = key
+        N = (2 * hu + 1) * (2 * hv + 1)
+        out = P * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8 + L * 4)
+        return _native.solve_workspace_bytes(P, L, hu, hv, _engine._MODE[path]) + out
+
+    def _launch(self, key, items, stream):
+        hu, hv, path, L = key
+        convs, index = [], {}
+        pts = []
+        for work, i, pt, conv in items:
+            cid = id(conv[0])
+            if cid not in index:
+                index[cid] = len(convs)
+                convs.append(conv)
+            pts.append(_engine.Point(stack=index[cid], wavelength=pt.wavelength, theta=pt.theta,
+                                     phi=pt.phi, polarization=pt.polarization, eps_top=pt.eps_top,
+                                     eps_bottom=pt.eps_bottom, period=pt.period,
+                                     thickness=pt.thickness))
+        bucket = _engine.Bucket(hu=hu, hv=hv, path=path, n_layers=L, points=pts)
+        with torch.cuda.stream(stream):
+            eff, amp, pe, flags, _ = _engine.run_bucket(bucket, convs, self.device)
+            done = torch.cuda.Event()
+            done.record(stream)
+        return dict(key=key, items=items, out=(eff, amp, pe, flags), event=done,
+                    bytes=self._launch_bytes(key, len(items)), stream=stream)
+
+    def _finish(self, launch, now, finished):
+        eff, amp, pe, flags = (t.cpu().numpy() for t in launch['out'])
+        hu, hv, path, L = launch['key']
+        for p, (work, i, _, _) in enumerate(launch['items']):
+            if np.any(flags[p] != 0):
+                work.failed = True
+            work.results[i] = _engine.unpack(eff, amp, pe, p, hu, hv, path)
+            work.done += 1
+            if work.done == work.n_points:
+                work.t_done = now
+                finished.append(work)
+
+    def _result(self, work):
+        r = work.resolved
+        if getattr(work, 'failed', False):
+            return NumericalError('A layer eigensolve did not converge or a factorization was singular.',
+                                  code='rcwa.singular', path='problem')
+        per = [work.results[i] for i in range(work.n_points)]
+        diag = {'solve_path': tuple(r.path for _ in r.wavelengths), 'served': True,
+                'latency_s': work.t_done - work.req.arrival}
+        prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 serve',
+                'length_units': work.req.problem.units.length}
+        return pack(per, r.hu, r.hv, r.wavelengths, work.checks, diag, prov)
+
+    # ------------------------------------------------------------------ the loop
+    def serve(self, requests, realtime=True):
+        """Run a request stream to completion. ``arrival`` times are seconds after the start
+        (``realtime=False`` admits everything at once). Returns (results by id, report)."""
+        reqs = sorted(requests, key=lambda q: q.arrival)
+        for i, q in enumerate(reqs):
+            if q.id < 0:
+                q.id = i
+        pending = collections.deque(reqs)
+        queues = collections.defaultdict(collections.deque)   # bucket key -> deque of items
+        order = collections.OrderedDict()                     # bucket key -> oldest arrival
+        inflight = []
+        results, latencies = {}, []
+        used = self.conv.bytes
+        free_streams = list(self.streams)
+        t0 = time.perf_counter()
+        solved_points = 0
+        while pending or any(queues.values()) or inflight:
+            now = time.perf_counter() - t0
+            # admit arrived requests
+            while pending and (not realtime or pending[0].arrival <= now):
+                q = pending.popleft()
+                try:
+                    work, key, items = self._admit(q, now)
+                except Exception as error:           # a bad request fails alone
+                    results[q.id] = error
+                    latencies.append(now - q.arrival)
+                    continue
+                queues[key].extend(items)
+                order.setdefault(key, q.arrival)
+            # retire finished launches
+            finished = []
+            still = []
+            for ln in inflight:
+                if ln['event'].query():
+                    self._finish(ln, time.perf_counter() - t0, finished)
+                    used -= ln['bytes']
+                    free_streams.append(ln['stream'])
+                    solved_points += len(ln['items'])
+                else:
+                    still.append(ln)
+            inflight = still
+            for work in finished:
+                results[work.req.id] = self._result(work)
+                latencies.append(work.t_done - work.req.arrival)
+            # start launches while streams and memory allow: oldest bucket first
+            started = False
+            for key in sorted([k for k in queues if queues[k]], key=lambda k: order.get(k, 0.0)):
+                if not free_streams:
+                    break
+                q = queues[key]
+                P = min(len(q), self.max_points)
+                while P > 1 and used + self._launch_bytes(key, P) > self.budget:
+                    P //= 2
+                if used + self._launch_bytes(key, P) > self.budget and inflight:
+                    continue
+                items = [q.popleft() for _ in range(P)]
+                if not q:
+                    order.pop(key, None)
+                stream = free_streams.pop()
+                ln = self._launch(key, items, stream)
+                used += ln['bytes']
+                inflight.append(ln)
+                self.stats['launches'] += 1
+                self.stats['launched_points'] += P
+                started = True
+            if not started and not finished:
+                if inflight:
+                    time.sleep(0.0005)
+                elif pending and realtime:
+                    time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        lat = np.array(latencies) if latencies else np.zeros(1)
+        report = {'requests': len(reqs), 'points': solved_points, 'wall_s': wall,
+                  'solves_per_s': solved_points / wall if wall > 0 else 0.0,
+                  'p50_latency_s': float(np.percentile(lat, 50)),
+                  'p99_latency_s': float(np.percentile(lat, 99)),
+                  'launches': self.stats['launches'],
+                  'mean_points_per_launch': self.stats['launched_points'] / max(1, self.stats['launches']),
+                  'conv_cache_hits': self.conv.hits, 'conv_cache_misses': self.conv.misses,
+                  'hbm_budget_bytes': self.budget}
+        return results, report
+
+
+def poisson_arrivals(count, rate, seed=0):
+    """Arrival times (s) of a Poisson process of ``rate`` requests per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, count))
