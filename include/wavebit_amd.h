@@ -24,6 +24,14 @@
  *                    the dense inv / solve / matmul calls of rcwa/_engine/scattering.py:116-200
  *                    (_ab_matrices, _s_layer, redheffer_star) and _common/backend.py:506-542
  *                    (linear_solve).
+ *   wb_solve_keep / wb_solve_adjoint / wb_assemble_li_adjoint
+ *                    the reverse pass PyTorch autograd takes through the same path
+ *                    (loss.backward() after Prepared.run, rcwa/_lifecycle.py:785-786), with
+ *                    the eigensolve's generator rule of rcwa/_engine/modal_gradient.py:161-251
+ *                    and the divided-difference exponential of :446-470 written by hand;
+ *   wb_zgemm_batched_ops / wb_zgetrs_batched_ops
+ *                    the conjugate-transposed products and solves of that reverse pass
+ *                    (torch matmul / linalg.solve backward).
  */
 #ifndef WAVEBIT_AMD_H
 #define WAVEBIT_AMD_H
@@ -44,7 +52,8 @@ typedef enum {
   WB_NONFINITE = 4,   /* reserved for callers that check outputs */
   WB_NOCONVERGE = 5,  /* reserved: eigensolver non-convergence is reported through flags/info */
   WB_FLAGGED = 6,     /* reserved */
-  WB_HIP = 7          /* a HIP runtime call failed; see wb_last_error */
+  WB_HIP = 7,         /* a HIP runtime call failed; see wb_last_error */
+  WB_UNSUPPORTED = 8  /* the combination asked for is not implemented (e.g. a 1D adjoint) */
 } wb_status;
 
 /* wb_solve mode */
@@ -122,6 +131,54 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
              const double* thick, const int* conv_idx, const double* epx, const double* epy,
              const double* epz_i, double* eff, double* amp, double* pe, int* flags,
              void* workspace, size_t workspace_bytes, void* stream);
+
+/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
+ * is stored k x m, B n x k). */
+int wb_zgemm_batched_ops(wb_ctx* ctx, int opa, int opb, int m, int n, int k, const double* alpha,
+                         const double* A, int lda, long long strideA, const double* B, int ldb,
+                         long long strideB, const double* beta, double* C, int ldc,
+                         long long strideC, int batch, void* stream);
+
+/* op 0: solve A X = B; op 1: solve A^H X = B; both from wb_zgetrf_batched's factors of A. */
+int wb_zgetrs_batched_ops(wb_ctx* ctx, int op, int n, int nrhs, const double* LU, int lda,
+                          long long strideA, const int* ipiv, double* B, int ldb, long long strideB,
+                          int batch, void* stream);
+
+/* Gradients (2D path, WB_MODE_2D).
+ * wb_solve_keep is wb_solve that also fills `checkpoint` (wb_solve_checkpoint_size bytes) with
+ * what the reverse pass needs: per (point, layer) the layer eigenvectors, the inverses of the
+ * rotated W and V, and the layer S11; per point the transmitted columns of the back-sweep.
+ * The workspace of that call then holds the rest (the per-layer S12, the cascade's LU factors
+ * and reflections) and must not be reused until wb_solve_adjoint has run.
+ * wb_solve_adjoint, given the cotangent eff_bar (P x 3 x 4 x N, the layout of eff) of the
+ * efficiencies of that solve, writes
+ *   d_epx, d_epy, d_epz   n_conv x N x N complex: the cotangents (d/dRe + i d/dIm of a real
+ *                         objective) of the conv matrices, summed over the (point, layer)
+ *                         pairs that use each slot
+ *   d_thick               P x L doubles: d/d thickness of each (point, layer).
+ * The eigenbasis is held fixed and the modal constants vary as a matrix in it (the generator
+ * rule), which gives the exact derivative of the efficiencies. */
+size_t wb_solve_checkpoint_size(int P, int L, int hu, int hv, int mode);
+int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                  const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                  const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                  void* workspace, size_t workspace_bytes, void* checkpoint,
+                  size_t checkpoint_bytes, void* stream);
+size_t wb_solve_adjoint_workspace_size(int P, int L, int hu, int hv, int mode);
+int wb_solve_adjoint(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                     const double* thick, const int* conv_idx, int n_conv, const double* epx,
+                     const double* epy, const double* epz_i, const void* solve_workspace,
+                     const void* checkpoint, const double* eff_bar, double* d_epx, double* d_epy,
+                     double* d_epz, double* d_thick, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+/* Cotangent of the raster eps (L, Ny, Nx) from those of wb_assemble_li's outputs; epx and epz_i
+ * are that call's outputs for the same eps. hu, hv <= 15. */
+size_t wb_assemble_li_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv);
+int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                           const double* epx, const double* epz_i, const double* d_epx,
+                           const double* d_epy, const double* d_epz, double* d_eps,
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 /* Per-kernel timing (no reference counterpart: measurement support). When enabled, the
  * launches of the kernels below are bracketed by hipEvent records on their stream;

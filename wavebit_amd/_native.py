@@ -24,7 +24,10 @@ PT_STRIDE = 16
 EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb_zgemm_batched',
            'wb_zgetrf_batched', 'wb_zgetrs_batched', 'wb_zgeev_workspace_size',
            'wb_zgeev_batched', 'wb_assemble_li_workspace_size', 'wb_assemble_li',
-           'wb_solve_workspace_size', 'wb_solve', 'wb_timing_enable', 'wb_timing_read')
+           'wb_solve_workspace_size', 'wb_solve', 'wb_zgemm_batched_ops', 'wb_zgetrs_batched_ops',
+           'wb_solve_checkpoint_size', 'wb_solve_keep', 'wb_solve_adjoint_workspace_size',
+           'wb_solve_adjoint', 'wb_assemble_li_adjoint_workspace_size', 'wb_assemble_li_adjoint',
+           'wb_timing_enable', 'wb_timing_read')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9}
@@ -69,6 +72,22 @@ def load_library():
             'wb_solve_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
+                                             c_int, c_ll, c_vp, c_int, c_ll, c_dp, c_vp, c_int,
+                                             c_ll, c_int, c_vp]),
+            'wb_zgetrs_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_ll, c_vp,
+                                              c_vp, c_int, c_ll, c_int, c_vp]),
+            'wb_solve_checkpoint_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_solve_keep': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                      c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp,
+                                      c_sz, c_vp]),
+            'wb_solve_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_solve_adjoint': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                         c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_assemble_li_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_assemble_li_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
+                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
             'wb_timing_enable': (c_int, [c_int]),
             'wb_timing_read': (c_int, [c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(c_ll), ctypes.POINTER(ctypes.c_double),
@@ -254,6 +273,123 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
                                _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
                                _stream(device)), 'solve')
     return eff, amp, pe, flags
+
+
+def zgemm_ops(A, B, opa=0, opb=0, C=None, alpha=1.0, beta=0.0):
+    """Batched ``alpha op(A) @ op(B) + beta C``, op 1 = conjugate transpose of the stored
+    (batch, rows, cols) operand."""
+    device = A.device
+    ctx = context(device)
+    _require(A, torch.complex128, device, 'A')
+    _require(B, torch.complex128, device, 'B')
+    batch = A.shape[0]
+    m, k = (A.shape[2], A.shape[1]) if opa else (A.shape[1], A.shape[2])
+    kb, n = (B.shape[2], B.shape[1]) if opb else (B.shape[1], B.shape[2])
+    if kb != k or B.shape[0] != batch:
+        raise ValueError(f'shape mismatch op{opa}{tuple(A.shape)} @ op{opb}{tuple(B.shape)}')
+    if C is None:
+        C = torch.zeros((batch, m, n), dtype=torch.complex128, device=device)
+        beta = 0.0
+    _require(C, torch.complex128, device, 'C')
+    al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+    ctx.check(ctx.lib.wb_zgemm_batched_ops(ctx.handle, opa, opb, m, n, k, al, _ptr(A), A.shape[2],
+                                           A.shape[1] * A.shape[2], _ptr(B), B.shape[2],
+                                           B.shape[1] * B.shape[2], be, _ptr(C), n, m * n, batch,
+                                           _stream(device)), 'zgemm_ops')
+    return C
+
+
+def zgetrs_ops(LU, ipiv, B, op=0):
+    """Solve ``A X = B`` (op 0) or ``A^H X = B`` (op 1) in place from ``zgetrf`` factors."""
+    device = LU.device
+    ctx = context(device)
+    _require(B, torch.complex128, device, 'B')
+    batch, n, _ = LU.shape
+    nrhs = B.shape[-1]
+    ctx.check(ctx.lib.wb_zgetrs_batched_ops(ctx.handle, op, n, nrhs, _ptr(LU), n, n * n, _ptr(ipiv),
+                                            _ptr(B), nrhs, n * nrhs, batch, _stream(device)),
+              'zgetrs_ops')
+    return B
+
+
+def checkpoint_bytes(P, L, hu, hv, mode=MODE_2D):
+    return int(load_library().wb_solve_checkpoint_size(P, L, hu, hv, mode))
+
+
+def solve_keep(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode=MODE_2D):
+    """``solve`` that also keeps the checkpoint of its reverse pass. Returns
+    (eff, amp, pe, flags, state); ``state`` holds the workspace and checkpoint tensors and is
+    consumed by ``solve_adjoint``."""
+    device = epx.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
+    ckbytes = ctx.lib.wb_solve_checkpoint_size(P, L, hu, hv, mode)
+    ws = _workspace(nbytes, device)
+    ck = _workspace(ckbytes, device)
+    ctx.check(ctx.lib.wb_solve_keep(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
+                                    _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(eff),
+                                    _ptr(amp), _ptr(pe), _ptr(flags), _ptr(ws), nbytes, _ptr(ck),
+                                    ckbytes, _stream(device)), 'solve_keep')
+    state = dict(ws=ws, ck=ck, pts=pts, thick=thick, conv_idx=conv_idx, epx=epx, epy=epy,
+                 epz_i=epz_i, hu=hu, hv=hv, mode=mode)
+    return eff, amp, pe, flags, state
+
+
+def solve_adjoint(state, eff_bar, n_conv=None):
+    """Reverse pass of ``solve_keep``: cotangent of eff (P, 3, 4, N) -> (d_epx, d_epy, d_epz,
+    d_thick); conv cotangents are summed per conv slot."""
+    epx = state['epx']
+    device = epx.device
+    ctx = context(device)
+    P, L = state['thick'].shape
+    hu, hv, mode = state['hu'], state['hv'], state['mode']
+    N = (2 * hu + 1) * (2 * hv + 1)
+    C = epx.shape[0] if n_conv is None else n_conv
+    eff_bar = eff_bar.to(device=device, dtype=torch.float64).contiguous()
+    if tuple(eff_bar.shape) != (P, 3, 4, N):
+        raise ValueError(f'eff_bar must have shape {(P, 3, 4, N)}, got {tuple(eff_bar.shape)}')
+    d = [torch.empty((C, N, N), dtype=torch.complex128, device=device) for _ in range(3)]
+    d_thick = torch.empty((P, L), dtype=torch.float64, device=device)
+    nbytes = ctx.lib.wb_solve_adjoint_workspace_size(P, L, hu, hv, mode)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve_adjoint(ctx.handle, P, L, hu, hv, mode, _ptr(state['pts']),
+                                       _ptr(state['thick']), _ptr(state['conv_idx']), C, _ptr(epx),
+                                       _ptr(state['epy']), _ptr(state['epz_i']), _ptr(state['ws']),
+                                       _ptr(state['ck']), _ptr(eff_bar), _ptr(d[0]), _ptr(d[1]),
+                                       _ptr(d[2]), _ptr(d_thick), _ptr(ws), nbytes,
+                                       _stream(device)), 'solve_adjoint')
+    return d[0], d[1], d[2], d_thick
+
+
+def assemble_li_adjoint(eps, hu, hv, epx, epz_i, d_epx, d_epy, d_epz):
+    """Cotangent of the raster stack eps (L, Ny, Nx) from those of ``assemble_li``'s outputs."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    for name, t in (('epx', epx), ('epz_i', epz_i), ('d_epx', d_epx), ('d_epy', d_epy),
+                    ('d_epz', d_epz)):
+        _require(t, torch.complex128, device, name)
+    L, Ny, Nx = eps.shape
+    d_eps = torch.empty_like(eps)
+    nbytes = ctx.lib.wb_assemble_li_adjoint_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_assemble_li_adjoint(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, _ptr(epx),
+                                             _ptr(epz_i), _ptr(d_epx), _ptr(d_epy), _ptr(d_epz),
+                                             _ptr(d_eps), _ptr(ws), nbytes, _stream(device)),
+              'assemble_li_adjoint')
+    return d_eps
 
 
 # ----------------------------------------------------------------------------- timing

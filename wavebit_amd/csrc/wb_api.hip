@@ -3,7 +3,7 @@
 #include <string>
 #include <cstring>
 
-#include "wb_internal.h"
+#include "wb_rcwa.h"
 #include "../../include/wavebit_amd.h"
 
 
@@ -170,8 +170,131 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
   a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
-  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve");
+}
+
+int wb_zgemm_batched_ops(wb_ctx* ctx, int opa, int opb, int m, int n, int k, const double* alpha,
+                         const double* A, int lda, long long strideA, const double* B, int ldb,
+                         long long strideB, const double* beta, double* C, int ldc,
+                         long long strideC, int batch, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (opa < 0 || opa > 1 || opb < 0 || opb > 1)
+    return fail(ctx, WB_BADARG, "wb_zgemm_batched_ops: op must be 0 (none) or 1 (conjugate transpose)");
+  if (m < 0 || n < 0 || k < 0 || batch < 0 || !alpha || !beta || lda < (opa ? m : k) ||
+      ldb < (opb ? k : n) || ldc < n)
+    return fail(ctx, WB_BADARG, "wb_zgemm_batched_ops: bad dimensions");
+  return hip_status(ctx,
+                    wb_zgemm_ops_launch(opa, opb, m, n, k, mkc(alpha[0], alpha[1]),
+                                        bp_strided(A, strideA), lda, bp_strided(B, strideB), ldb,
+                                        mkc(beta[0], beta[1]), bp_strided(C, strideC), ldc, batch,
+                                        (hipStream_t)stream),
+                    "wb_zgemm_batched_ops");
+}
+
+int wb_zgetrs_batched_ops(wb_ctx* ctx, int op, int n, int nrhs, const double* LU, int lda,
+                          long long strideA, const int* ipiv, double* B, int ldb, long long strideB,
+                          int batch, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (op < 0 || op > 1 || n < 0 || nrhs < 0 || lda < n || ldb < nrhs || batch < 0 || !ipiv)
+    return fail(ctx, WB_BADARG, "wb_zgetrs_batched_ops: bad arguments");
+  hipError_t e = op ? wb_getrs_conj_launch(n, nrhs, bp_strided(LU, strideA), lda, ipiv,
+                                           bp_strided(B, strideB), ldb, batch, (hipStream_t)stream)
+                    : wb_getrs_launch(n, nrhs, bp_strided(LU, strideA), lda, ipiv,
+                                      bp_strided(B, strideB), ldb, batch, (hipStream_t)stream);
+  return hip_status(ctx, e, "wb_zgetrs_batched_ops");
+}
+
+size_t wb_solve_checkpoint_size(int P, int L, int hu, int hv, int mode) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode != 0) return 0;
+  return wb_checkpoint_bytes(P, L, hu, hv, mode);
+}
+
+int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                  const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                  const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                  void* workspace, size_t workspace_bytes, void* checkpoint,
+                  size_t checkpoint_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (mode != 0)
+    return fail(ctx, WB_UNSUPPORTED, "wb_solve_keep: the adjoint is implemented for the 2D path (mode 0)");
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0)
+    return fail(ctx, WB_BADARG, "wb_solve_keep: bad dimensions");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe || !checkpoint)
+    return fail(ctx, WB_BADARG, "wb_solve_keep: null buffer");
+  if (workspace_bytes < wb_solve_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_keep: workspace too small");
+  if (checkpoint_bytes < wb_solve_checkpoint_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_keep: checkpoint too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = checkpoint;
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_keep");
+}
+
+size_t wb_solve_adjoint_workspace_size(int P, int L, int hu, int hv, int mode) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode != 0) return 0;
+  return wb_adjoint_workspace_bytes(P, L, hu, hv, mode);
+}
+
+int wb_solve_adjoint(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                     const double* thick, const int* conv_idx, int n_conv, const double* epx,
+                     const double* epy, const double* epz_i, const void* solve_workspace,
+                     const void* checkpoint, const double* eff_bar, double* d_epx, double* d_epy,
+                     double* d_epz, double* d_thick, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (mode != 0)
+    return fail(ctx, WB_UNSUPPORTED, "wb_solve_adjoint: implemented for the 2D path (mode 0)");
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || n_conv <= 0)
+    return fail(ctx, WB_BADARG, "wb_solve_adjoint: bad dimensions");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !solve_workspace || !checkpoint ||
+      !eff_bar || !d_epx || !d_epy || !d_epz || !d_thick)
+    return fail(ctx, WB_BADARG, "wb_solve_adjoint: null buffer");
+  if (workspace_bytes < wb_solve_adjoint_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_adjoint: workspace too small");
+  wb_solve_args f;
+  f.P = P; f.L = L; f.hu = hu; f.hv = hv; f.mode = mode;
+  f.pts = pts; f.thick = thick; f.conv_idx = conv_idx;
+  f.epx = (const cd*)epx; f.epy = (const cd*)epy; f.epz = (const cd*)epz_i;
+  f.eff = nullptr; f.amp = nullptr; f.pe = nullptr; f.flags = nullptr;
+  f.ck = const_cast<void*>(checkpoint);
+  wb_adjoint_args a;
+  a.fwd = &f; a.n_conv = n_conv; a.solve_ws = solve_workspace; a.ck = checkpoint;
+  a.eff_bar = eff_bar; a.d_epx = (cd*)d_epx; a.d_epy = (cd*)d_epy; a.d_epz = (cd*)d_epz;
+  a.d_thick = d_thick;
+  return hip_status(ctx, wb_adjoint_launch(a, workspace, (hipStream_t)stream), "wb_solve_adjoint");
+}
+
+size_t wb_assemble_li_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0) return 0;
+  return wb_assemble_adjoint_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                           const double* epx, const double* epz_i, const double* d_epx,
+                           const double* d_epy, const double* d_epz, double* d_eps,
+                           void* workspace, size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15)
+    return fail(ctx, WB_BADARG, "wb_assemble_li_adjoint: bad dimensions (need 0 <= hu, hv <= 15)");
+  if (!eps || !epx || !epz_i || !d_epx || !d_epy || !d_epz || !d_eps)
+    return fail(ctx, WB_BADARG, "wb_assemble_li_adjoint: null buffer");
+  if (workspace_bytes < wb_assemble_li_adjoint_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_li_adjoint: workspace too small");
+  return hip_status(ctx,
+                    wb_assemble_adjoint_launch((const cd*)eps, L, Ny, Nx, hu, hv, (const cd*)epx,
+                                               (const cd*)epz_i, (const cd*)d_epx, (const cd*)d_epy,
+                                               (const cd*)d_epz, (cd*)d_eps, workspace,
+                                               (hipStream_t)stream),
+                    "wb_assemble_li_adjoint");
 }
 
 }  // extern "C"

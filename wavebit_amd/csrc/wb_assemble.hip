@@ -145,6 +145,161 @@ __global__ void k_direct_bttb(const cd* c2d, int L, int hu, int hv, cd* out) {
   out[(long long)l * N * N + e] = c2d[((long long)l * (4 * hv + 1) + (n1 - n2 + 2 * hv)) * Ku + (m1 - m2 + 2 * hu)];
 }
 
+// ------------------------------------------------------------------------------ adjoint
+// The pixel series is linear in the samples: c_k = sum_j T[k][j] a_j with, for j >= 1,
+// T[k][j] = g_k w^{kj} / n and T[k][0] = -g_k S_k / n + [k == 0], g_k = sinc(k/n) e^{-i pi k/n},
+// w = e^{-2 pi i / n}, S_k = sum_{j>=1} w^{kj} = n - 1 if n | k else -1. Its adjoint:
+// abar_j = sum_k conj(T[k][j]) cbar_k, over the 4 half + 1 orders k = -2 half .. 2 half.
+__device__ inline cd pixel_series_adj(const cd* cbar, int cstride, int n, int j, int half) {
+  const int K = 4 * half + 1;
+  if (n <= 1) return (j == 0) ? cbar[(long long)(2 * half) * cstride] : mkc(0.0, 0.0);
+  double sr = 0.0, si = 0.0;
+  const double inv_n = 1.0 / (double)n;
+  for (int kk = 0; kk < K; ++kk) {
+    const int k = kk - 2 * half;
+    const cd cb = cbar[(long long)kk * cstride];
+    const double tt = (double)k / (double)n;
+    const double sinc = (k == 0) ? 1.0 : sinpi(tt) / (3.141592653589793 * tt);
+    double gs, gc;
+    sincospi(-tt, &gs, &gc);
+    const cd gconj = mkc(sinc * gc, -sinc * gs);
+    cd coef;
+    if (j >= 1) {
+      long long idx = ((long long)k * j) % n;
+      if (idx < 0) idx += n;
+      double s, c;
+      sincospi(2.0 * (double)idx / (double)n, &s, &c);
+      coef = cscale(cmul(gconj, mkc(c, s)), inv_n);
+    } else {
+      const double Sk = (k % n == 0) ? (double)(n - 1) : -1.0;
+      coef = cscale(gconj, -Sk * inv_n);
+      if (k == 0) coef = cadd(coef, mkc(1.0, 0.0));
+    }
+    sr += coef.x * cb.x - coef.y * cb.y;
+    si += coef.x * cb.y + coef.y * cb.x;
+  }
+  return mkc(sr, si);
+}
+
+// c2d_bar[l][kv][ku] = sum of zpre_bar over the entries the direct rule fills from c2d[kv][ku]
+__global__ void k_direct_adj(const cd* zpre_bar, int L, int hu, int hv, cd* c2d_bar) {
+  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv, Ku = 4 * hu + 1, Kv = 4 * hv + 1;
+  const int l = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Kv * Ku) return;
+  const int kv = e / Ku, ku = e % Ku;
+  const int dn = kv - 2 * hv, dm = ku - 2 * hu;
+  const cd* Z = zpre_bar + (long long)l * N * N;
+  cd acc = mkc(0.0, 0.0);
+  for (int n1 = 0; n1 < nv; ++n1) {
+    const int n2 = n1 - dn;
+    if (n2 < 0 || n2 >= nv) continue;
+    for (int m1 = 0; m1 < nu; ++m1) {
+      const int m2 = m1 - dm;
+      if (m2 < 0 || m2 >= nu) continue;
+      acc = cadd(acc, Z[(long long)(n1 * nu + m1) * N + (n2 * nu + m2)]);
+    }
+  }
+  c2d_bar[((long long)l * Kv + kv) * Ku + ku] = acc;
+}
+
+// inner_bar[slot][l][ju][e] from the block-Toeplitz outputs' cotangents (slot 0: pre_bar of
+// epx, slot 1: epy-bar): first the per-offset sums, then the u-series adjoint over columns.
+__global__ __launch_bounds__(64) void k_outer_adj(const cd* pre_bar, const cd* epy_bar, int L, int Nx,
+                                                  int hu, int hv, cd* inner_bar) {
+  const int e = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, Ku = 4 * hu + 1, N = nu * nv;
+  const int n1 = e / nv, n2 = e % nv;
+  const cd* ob = (slot == 0 ? pre_bar : epy_bar) + (long long)l * N * N;
+  __shared__ cd cu[61];
+  for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x) {
+    const int d = kk - 2 * hu;
+    cd acc = mkc(0.0, 0.0);
+    for (int m = 0; m < nu; ++m) {
+      const int mp = m - d;
+      if (mp < 0 || mp >= nu) continue;
+      acc = cadd(acc, ob[(long long)(n1 * nu + m) * N + (n2 * nu + mp)]);
+    }
+    cu[kk] = acc;
+  }
+  __syncthreads();
+  cd* out = inner_bar + (((long long)slot * L + l) * Nx) * nv * nv + e;
+  for (int ju = threadIdx.x; ju < Nx; ju += blockDim.x)
+    out[(long long)ju * nv * nv] = pixel_series_adj(cu, 1, Nx, ju, hu);
+}
+
+// Per column: Toeplitz-bar = -inner^H inner_bar inner^H, then its diagonals summed into the
+// series cotangent lines_bar[slot][l][ju][kk] (kk = r - c + 2hv).
+__global__ __launch_bounds__(64) void k_inner_adj(const cd* inner, const cd* inner_bar, int L, int Nx,
+                                                  int hv, cd* lines_bar) {
+  const int ju = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int nv = 2 * hv + 1, Kv = 4 * hv + 1;
+  const long long off = (((long long)slot * L + l) * Nx + ju) * nv * nv;
+  __shared__ cd Ai[31][32], Gb[31][32], Tm[31][32];
+  const int t = threadIdx.x;
+  for (int e = t; e < nv * nv; e += 64) {
+    Ai[e / nv][e % nv] = inner[off + e];
+    Gb[e / nv][e % nv] = inner_bar[off + e];
+  }
+  __syncthreads();
+  // Tm = Gb Ai^H
+  for (int e = t; e < nv * nv; e += 64) {
+    const int r = e / nv, c = e % nv;
+    cd acc = mkc(0.0, 0.0);
+    for (int k = 0; k < nv; ++k) acc = cadd(acc, cmul(Gb[r][k], cconj(Ai[c][k])));
+    Tm[r][c] = acc;
+  }
+  __syncthreads();
+  // Gb <- -Ai^H Tm
+  for (int e = t; e < nv * nv; e += 64) {
+    const int r = e / nv, c = e % nv;
+    cd acc = mkc(0.0, 0.0);
+    for (int k = 0; k < nv; ++k) acc = cadd(acc, cmul(cconj(Ai[k][r]), Tm[k][c]));
+    Gb[r][c] = cneg(acc);
+  }
+  __syncthreads();
+  cd* lb = lines_bar + (((long long)slot * L + l) * Nx + ju) * Kv;
+  for (int kk = t; kk < Kv; kk += 64) {
+    const int dd = kk - 2 * hv;   // r - c
+    cd acc = mkc(0.0, 0.0);
+    for (int r = 0; r < nv; ++r) {
+      const int c = r - dd;
+      if (c >= 0 && c < nv) acc = cadd(acc, Gb[r][c]);
+    }
+    lb[kk] = acc;
+  }
+}
+
+// lines_bar[0][l][ju][kv] += u-series adjoint of c2d_bar[l][kv][:]
+__global__ __launch_bounds__(64) void k_series2d_adj(const cd* c2d_bar, int L, int Nx, int hu, int hv,
+                                                     cd* lines_bar) {
+  const int kv = blockIdx.x, l = blockIdx.y;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const cd* cb = c2d_bar + ((long long)l * Kv + kv) * Ku;
+  for (int ju = threadIdx.x; ju < Nx; ju += blockDim.x) {
+    cd* x = lines_bar + ((long long)l * Nx + ju) * Kv + kv;
+    *x = cadd(*x, pixel_series_adj(cb, 1, Nx, ju, hu));
+  }
+}
+
+// eps_bar[l][jv][ju] = v-series adjoint of lines_bar[0] + conj(-1/eps^2) of lines_bar[1]
+__global__ __launch_bounds__(64) void k_series_v_adj(const cd* eps, const cd* lines_bar, int L, int Ny,
+                                                     int Nx, int hv, cd* eps_bar) {
+  const int ju = blockIdx.x, l = blockIdx.y;
+  const int Kv = 4 * hv + 1;
+  const cd* lb0 = lines_bar + ((long long)l * Nx + ju) * Kv;
+  const cd* lb1 = lines_bar + (((long long)L + l) * Nx + ju) * Kv;
+  for (int jv = threadIdx.x; jv < Ny; jv += blockDim.x) {
+    const long long o = ((long long)l * Ny + jv) * Nx + ju;
+    const cd a = eps[o];
+    const cd r = crecip(a);
+    const cd dr = cneg(cmul(r, r));          // d(1/a)/da
+    const cd g0 = pixel_series_adj(lb0, 1, Ny, jv, hv);
+    const cd g1 = pixel_series_adj(lb1, 1, Ny, jv, hv);
+    eps_bar[o] = cadd(g0, cmulc(dr, g1));
+  }
+}
+
 }  // namespace
 
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
@@ -194,4 +349,67 @@ hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, i
   e = wb_inverse_launch(N, bp_strided(epz_pre, NN), N, bp_strided(epz_i, NN), N, ipiv + (size_t)L * N,
                         info + L, L, stream);
   return e;
+}
+
+size_t wb_assemble_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  size_t b = 0;
+  b += sizeof(cd) * 2 * L * Nx * Kv * 2;        // lines, lines_bar
+  b += sizeof(cd) * 2 * L * Nx * nv * nv * 2;   // inner, inner_bar
+  b += sizeof(cd) * L * Kv * Ku;                // c2d_bar
+  b += sizeof(cd) * 3 * L * N * N;              // pre_bar, zpre_bar, GEMM temporary
+  return b + 1024;
+}
+
+// d_eps (L, Ny, Nx) from the cotangents of epx, epy, epz_i (L, N, N each); epx, epz_i are the
+// forward's outputs for the same eps.
+hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                      const cd* epx, const cd* epz_i, const cd* d_epx,
+                                      const cd* d_epy, const cd* d_epz, cd* d_eps, void* ws,
+                                      hipStream_t stream) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const long long NN = (long long)N * N;
+  char* p = (char*)ws;
+  cd* lines = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * Kv;
+  cd* lines_bar = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * Kv;
+  cd* inner = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * nv * nv;
+  cd* inner_bar = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * Nx * nv * nv;
+  cd* c2d_bar = (cd*)p; p += sizeof(cd) * (size_t)L * Kv * Ku;
+  cd* pre_bar = (cd*)p; p += sizeof(cd) * (size_t)L * NN;
+  cd* zpre_bar = (cd*)p; p += sizeof(cd) * (size_t)L * NN;
+  cd* tmp = (cd*)p; p += sizeof(cd) * (size_t)L * NN;
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  hipError_t e;
+  // recompute the column series and their Toeplitz inverses
+  hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
+  WB_CHECK_LAUNCH();
+  // outer inverses: pre_bar = -epx^H d_epx epx^H, zpre_bar = -epz^H d_epz epz^H
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, one, bp_strided(d_epx, NN), N, bp_strided(epx, NN), N,
+                               zero, bp_strided(tmp, NN), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mone, bp_strided(epx, NN), N, bp_strided(tmp, NN), N,
+                               zero, bp_strided(pre_bar, NN), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, one, bp_strided(d_epz, NN), N, bp_strided(epz_i, NN), N,
+                               zero, bp_strided(tmp, NN), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mone, bp_strided(epz_i, NN), N, bp_strided(tmp, NN), N,
+                               zero, bp_strided(zpre_bar, NN), N, L, stream))) return e;
+  hipLaunchKernelGGL(k_direct_adj, dim3((Kv * Ku + 255) / 256, L), dim3(256), 0, stream, zpre_bar, L,
+                     hu, hv, c2d_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_outer_adj, dim3(nv * nv, L, 2), dim3(64), 0, stream, pre_bar, d_epy, L, Nx, hu,
+                     hv, inner_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_inner_adj, dim3(Nx, L, 2), dim3(64), 0, stream, inner, inner_bar, L, Nx, hv,
+                     lines_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series2d_adj, dim3(Kv, L), dim3(64), 0, stream, c2d_bar, L, Nx, hu, hv,
+                     lines_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series_v_adj, dim3(Nx, L), dim3(64), 0, stream, eps, lines_bar, L, Ny, Nx, hv,
+                     d_eps);
+  WB_CHECK_LAUNCH();
+  return hipSuccess;
 }

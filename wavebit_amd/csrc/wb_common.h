@@ -2,9 +2,10 @@
 //
 // Complex numbers are interleaved double2 {re, im}; matrices are row-major with an explicit
 // leading dimension (row stride, in elements). Batches address their matrices through
-// BatchPtr: element b lives at base + (b / inner) * s_outer + (b % inner) * s_inner (or base + idx[b] *
-// s_outer with a gather index), which covers contiguous batches, shared operands (stride 0) and layer-major (point, layer) grids
-// without device pointer arrays.
+// BatchPtr: element b lives at base + (b / inner) * s_outer + (b % inner) * s_inner (or
+// base + idx[b * idx_stride] * s_outer with a gather index), which covers contiguous batches,
+// shared operands (stride 0), layer-major (point, layer) grids and one layer's column of such a
+// grid, without device pointer arrays.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -16,24 +17,27 @@ struct BatchPtr {
   long long s_outer;
   long long s_inner;
   int inner;
-  const int* idx;          // optional gather: element b is base + idx[b] * s_outer
+  const int* idx;          // optional gather: element b is base + idx[b * idx_stride] * s_outer
+  int idx_stride;
   __host__ __device__ inline double2* ptr(long long b) const {
-    if (idx) return base + (long long)idx[b] * s_outer;
+    if (idx) return base + (long long)idx[b * idx_stride] * s_outer;
     return base + (b / inner) * s_outer + (b % inner) * s_inner;
   }
 };
 
 static inline BatchPtr bp_strided(const void* base, long long stride) {
   BatchPtr p; p.base = (double2*)base; p.s_outer = stride; p.s_inner = 0; p.inner = 1; p.idx = nullptr;
+  p.idx_stride = 1;
   return p;
 }
 static inline BatchPtr bp_grid(const void* base, long long s_outer, long long s_inner, int inner) {
   BatchPtr p; p.base = (double2*)base; p.s_outer = s_outer; p.s_inner = s_inner; p.inner = inner;
-  p.idx = nullptr;
+  p.idx = nullptr; p.idx_stride = 1;
   return p;
 }
-static inline BatchPtr bp_gather(const void* base, long long stride, const int* idx) {
-  BatchPtr p = bp_strided(base, stride); p.idx = idx; return p;
+static inline BatchPtr bp_gather(const void* base, long long stride, const int* idx,
+                                 int idx_stride = 1) {
+  BatchPtr p = bp_strided(base, stride); p.idx = idx; p.idx_stride = idx_stride; return p;
 }
 static inline BatchPtr bp_offset(BatchPtr p, long long off) { p.base += off; return p; }
 

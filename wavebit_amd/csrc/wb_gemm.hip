@@ -1,6 +1,8 @@
 // Batched complex FP64 GEMM on CDNA4 matrix cores.
 //
-// C[b] = alpha * A[b] * B[b] + beta * C[b], A m x k, B k x n, C m x n, all row-major.
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], op(A) m x k, op(B) k x n, C m x n, all
+// row-major; op is the identity or the conjugate transpose (then A is stored k x m, B n x k,
+// and the tile loads walk the stored rows so global reads stay coalesced).
 // A complex product is four real v_mfma_f64_16x16x4f64 streams (re*re, -im*im, re*im, im*re)
 // into two accumulators. A 256-thread workgroup owns a 64 x 64 tile of C, its four waves a
 // 32 x 32 quadrant each (2 x 2 MFMA blocks); K is staged through LDS 16 at a time, split into
@@ -16,6 +18,7 @@ namespace {
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 2;
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+template <bool CA, bool CB>
 __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, BatchPtr A, int lda,
                                                BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc,
                                                int beta_zero) {
@@ -41,7 +44,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
   const int bk = t >> 4, bn = (t & 15) * 4;         // B: k row bk, cols bn..bn+3
 
   for (int k0 = 0; k0 < k; k0 += BK) {
-    {
+    if (!CA) {
       const int row = tm + ar;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -51,6 +54,19 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
         As_r[ak + q][ar] = v.x;
         As_i[ak + q][ar] = v.y;
       }
+    } else {
+      // stored k x m: thread reads 4 consecutive m of stored row kk
+      const int kk = k0 + (t >> 4), r0 = (t & 15) * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = tm + r0 + q;
+        cd v = mkc(0.0, 0.0);
+        if (row < m && kk < k) v = a[(long long)kk * lda + row];
+        As_r[t >> 4][r0 + q] = v.x;
+        As_i[t >> 4][r0 + q] = -v.y;
+      }
+    }
+    if (!CB) {
       const int kk = k0 + bk;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -59,6 +75,17 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
         if (kk < k && col < n) v = bb[(long long)kk * ldb + col];
         Bs_r[bk][bn + q] = v.x;
         Bs_i[bk][bn + q] = v.y;
+      }
+    } else {
+      // stored n x k: thread reads 4 consecutive k of stored row col
+      const int col = tn + (t >> 2), k1 = (t & 3) * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int kk = k0 + k1 + q;
+        cd v = mkc(0.0, 0.0);
+        if (kk < k && col < n) v = bb[(long long)col * ldb + kk];
+        Bs_r[k1 + q][t >> 2] = v.x;
+        Bs_i[k1 + q][t >> 2] = -v.y;
       }
     }
     __syncthreads();
@@ -105,13 +132,34 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
 
 }  // namespace
 
-hipError_t wb_zgemm_launch(int m, int n, int k, cd alpha, BatchPtr A, int lda, BatchPtr B, int ldb,
-                           cd beta, BatchPtr C, int ldc, int batch, hipStream_t stream) {
+hipError_t wb_zgemm_ops_launch(int opa, int opb, int m, int n, int k, cd alpha, BatchPtr A, int lda,
+                               BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc, int batch,
+                               hipStream_t stream) {
   if (m <= 0 || n <= 0 || batch <= 0) return hipSuccess;
   dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM, batch);
   int beta_zero = (beta.x == 0.0 && beta.y == 0.0) ? 1 : 0;
-  WB_TIMED_F(WB_T_ZGEMM, stream, 8.0 * m * n * (double)(k > 0 ? k : 0) * batch,
-           hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, stream, m, n, k, alpha, A, lda, B, ldb,
-                              beta, C, ldc, beta_zero));
+  const double fl = 8.0 * m * n * (double)(k > 0 ? k : 0) * batch;
+  if (!opa && !opb) {
+    WB_TIMED_F(WB_T_ZGEMM, stream, fl,
+               hipLaunchKernelGGL((k_zgemm<false, false>), grid, dim3(256), 0, stream, m, n, k, alpha,
+                                  A, lda, B, ldb, beta, C, ldc, beta_zero));
+  } else if (opa && !opb) {
+    WB_TIMED_F(WB_T_ZGEMM, stream, fl,
+               hipLaunchKernelGGL((k_zgemm<true, false>), grid, dim3(256), 0, stream, m, n, k, alpha,
+                                  A, lda, B, ldb, beta, C, ldc, beta_zero));
+  } else if (!opa && opb) {
+    WB_TIMED_F(WB_T_ZGEMM, stream, fl,
+               hipLaunchKernelGGL((k_zgemm<false, true>), grid, dim3(256), 0, stream, m, n, k, alpha,
+                                  A, lda, B, ldb, beta, C, ldc, beta_zero));
+  } else {
+    WB_TIMED_F(WB_T_ZGEMM, stream, fl,
+               hipLaunchKernelGGL((k_zgemm<true, true>), grid, dim3(256), 0, stream, m, n, k, alpha,
+                                  A, lda, B, ldb, beta, C, ldc, beta_zero));
+  }
   return hipGetLastError();
+}
+
+hipError_t wb_zgemm_launch(int m, int n, int k, cd alpha, BatchPtr A, int lda, BatchPtr B, int ldb,
+                           cd beta, BatchPtr C, int ldc, int batch, hipStream_t stream) {
+  return wb_zgemm_ops_launch(0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, stream);
 }

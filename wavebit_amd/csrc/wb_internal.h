@@ -8,6 +8,11 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
                            hipStream_t stream);
 hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
                            int ldb, int batch, hipStream_t stream);
+hipError_t wb_getrs_conj_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                                int ldb, int batch, hipStream_t stream);
+hipError_t wb_zgemm_ops_launch(int opa, int opb, int m, int n, int k, cd alpha, BatchPtr A, int lda,
+                               BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc, int batch,
+                               hipStream_t stream);
 hipError_t wb_set_identity_launch(int n, BatchPtr B, int ldb, int batch, hipStream_t stream);
 hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
                              int* info, int batch, hipStream_t stream);
@@ -20,6 +25,11 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws,
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
                                  cd* epy, cd* epz_i, void* ws, int* info_out, hipStream_t stream);
+size_t wb_assemble_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                      const cd* epx, const cd* epz_i, const cd* d_epx,
+                                      const cd* d_epy, const cd* d_epz, cd* d_eps, void* ws,
+                                      hipStream_t stream);
 
 struct wb_solve_args {
   int P, L, hu, hv, mode;
@@ -33,6 +43,7 @@ struct wb_solve_args {
   cd* amp;                  // P x 3 x 4 x N
   double* pe;               // P x 3
   int* flags;               // P x L: eig non-convergence / singular factorizations
+  void* ck;                 // optional checkpoint for the adjoint (2D path), or nullptr
 };
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

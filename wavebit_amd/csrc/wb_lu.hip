@@ -128,15 +128,17 @@ __global__ __launch_bounds__(64) void k_laswp_trsm(int n, int k0, int jb, BatchP
   }
 }
 
-// Row permutation of B by the LU pivots (sequential swaps, as LAPACK's zlaswp).
+// Row permutation of B by the LU pivots (sequential swaps, as LAPACK's zlaswp); reverse:
+// the swaps in the opposite order (the inverse permutation).
 __global__ __launch_bounds__(64) void k_laswp_rhs(int n, int nrhs, BatchPtr B, int ldb,
-                                                  const int* ipiv, int piv_stride) {
+                                                  const int* ipiv, int piv_stride, int reverse) {
   const long long b = blockIdx.y;
   cd* x = B.ptr(b);
   const int* piv = ipiv + b * (long long)piv_stride;
   const int col = blockIdx.x * 64 + threadIdx.x;
   if (col >= nrhs) return;
-  for (int j = 0; j < n; ++j) {
+  for (int i = 0; i < n; ++i) {
+    const int j = reverse ? n - 1 - i : i;
     const int p = piv[j];
     if (p != j) {
       cd v = x[(long long)j * ldb + col];
@@ -147,9 +149,11 @@ __global__ __launch_bounds__(64) void k_laswp_rhs(int n, int nrhs, BatchPtr B, i
 }
 
 // Solve the jb x jb diagonal block (rows r0..r0+jb-1) of a triangular factor against the
-// same rows of B, in place. lower: unit lower (L); else non-unit upper (U).
+// same rows of B, in place. KIND 0: unit lower L; 1: non-unit upper U; 2: non-unit lower U^H;
+// 3: unit upper L^H (kinds 2 and 3 read the packed LU factors transposed and conjugated).
+template <int KIND>
 __global__ __launch_bounds__(64) void k_trsm_diag(int r0, int jb, int nrhs, BatchPtr A, int lda,
-                                                  BatchPtr B, int ldb, int lower) {
+                                                  BatchPtr B, int ldb) {
   const long long b = blockIdx.y;
   const cd* a = A.ptr(b);
   cd* x = B.ptr(b);
@@ -157,7 +161,8 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int r0, int jb, int nrhs, Batc
   __shared__ cd T[WB_NB][WB_NB + 1];
   for (int e = t; e < jb * jb; e += 64) {
     const int r = e / jb, c = e % jb;
-    T[r][c] = a[(long long)(r0 + r) * lda + r0 + c];
+    if (KIND < 2) T[r][c] = a[(long long)(r0 + r) * lda + r0 + c];
+    else T[r][c] = cconj(a[(long long)(r0 + c) * lda + r0 + r]);
   }
   __syncthreads();
   const int col = blockIdx.x * 64 + t;
@@ -166,15 +171,15 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int r0, int jb, int nrhs, Batc
 #pragma unroll
   for (int r = 0; r < WB_NB; ++r)
     if (r < jb) v[r] = x[(long long)(r0 + r) * ldb + col];
-  if (lower) {
+  if (KIND == 0 || KIND == 2) {
 #pragma unroll
-    for (int r = 1; r < WB_NB; ++r) {
+    for (int r = 0; r < WB_NB; ++r) {
       if (r < jb) {
         cd s = v[r];
 #pragma unroll
         for (int q = 0; q < WB_NB; ++q)
           if (q < r) s = csub(s, cmul(T[r][q], v[q]));
-        v[r] = s;
+        v[r] = (KIND == 2) ? cdiv(s, T[r][r]) : s;
       }
     }
   } else {
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(64) void k_trsm_diag(int r0, int jb, int nrhs, Batc
 #pragma unroll
         for (int q = 0; q < WB_NB; ++q)
           if (q > r && q < jb) s = csub(s, cmul(T[r][q], v[q]));
-        v[r] = cdiv(s, T[r][r]);
+        v[r] = (KIND == 1) ? cdiv(s, T[r][r]) : s;
       }
     }
   }
@@ -249,12 +254,12 @@ hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipi
                            int ldb, int batch, hipStream_t stream) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
   dim3 g((nrhs + 63) / 64, batch);
-  hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n);
+  hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n, 0);
   WB_CHECK_LAUNCH();
   for (int r0 = 0; r0 < n; r0 += WB_NB) {
     const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
     WB_TIMED(WB_T_TRSM_DIAG, stream,
-             hipLaunchKernelGGL(k_trsm_diag, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb, 1));
+             hipLaunchKernelGGL(k_trsm_diag<0>, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb));
     WB_CHECK_LAUNCH();
     const int rest = n - r0 - jb;
     if (rest > 0) {
@@ -269,7 +274,7 @@ hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipi
   for (int r0 = last; r0 >= 0; r0 -= WB_NB) {
     const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
     WB_TIMED(WB_T_TRSM_DIAG, stream,
-             hipLaunchKernelGGL(k_trsm_diag, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb, 0));
+             hipLaunchKernelGGL(k_trsm_diag<1>, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb));
     WB_CHECK_LAUNCH();
     if (r0 > 0) {
       hipError_t e = wb_zgemm_launch(r0, nrhs, jb, mkc(-1.0, 0.0), offset(LU, (long long)r0), lda,
@@ -278,6 +283,48 @@ hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipi
       if (e != hipSuccess) return e;
     }
   }
+  return hipSuccess;
+}
+
+// Solve A^H X = B in place given the LU factors of A (A = P^T L U, so A^H = U^H L^H P):
+// forward substitution with U^H, backward with L^H, then the swaps in reverse order. The
+// off-diagonal updates are conjugate-transposed GEMMs reading the same packed factors.
+hipError_t wb_getrs_conj_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                                int ldb, int batch, hipStream_t stream) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  dim3 g((nrhs + 63) / 64, batch);
+  for (int r0 = 0; r0 < n; r0 += WB_NB) {
+    const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_trsm_diag<2>, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb));
+    WB_CHECK_LAUNCH();
+    const int rest = n - r0 - jb;
+    if (rest > 0) {
+      // B[r0+jb:] -= (U[r0:r0+jb, r0+jb:])^H B[r0:r0+jb]
+      hipError_t e = wb_zgemm_ops_launch(1, 0, rest, nrhs, jb, mkc(-1.0, 0.0),
+                                         offset(LU, (long long)r0 * lda + r0 + jb), lda,
+                                         offset(B, (long long)r0 * ldb), ldb, mkc(1.0, 0.0),
+                                         offset(B, (long long)(r0 + jb) * ldb), ldb, batch, stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  const int last = ((n - 1) / WB_NB) * WB_NB;
+  for (int r0 = last; r0 >= 0; r0 -= WB_NB) {
+    const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_trsm_diag<3>, g, dim3(64), 0, stream, r0, jb, nrhs, LU, lda, B, ldb));
+    WB_CHECK_LAUNCH();
+    if (r0 > 0) {
+      // B[0:r0] -= (L[r0:r0+jb, 0:r0])^H B[r0:r0+jb]
+      hipError_t e = wb_zgemm_ops_launch(1, 0, r0, nrhs, jb, mkc(-1.0, 0.0),
+                                         offset(LU, (long long)r0 * lda), lda,
+                                         offset(B, (long long)r0 * ldb), ldb, mkc(1.0, 0.0), B, ldb,
+                                         batch, stream);
+      if (e != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n, 1);
+  WB_CHECK_LAUNCH();
   return hipSuccess;
 }
 

@@ -1,0 +1,57 @@
+// Layout shared by the RCWA batch solve and its adjoint: the per-point parameter record, the
+// per-order vectors and diagonal port blocks kept per point, the workspace carve of a solve
+// and the checkpoint a solve keeps for its adjoint.
+#pragma once
+#include "wb_internal.h"
+#include "../../include/wavebit_amd.h"   // WB_FLAG_* bits
+
+// per-point parameter record (doubles), set by the host
+enum {
+  PT_WL = 0, PT_KX0 = 1, PT_KY0 = 2, PT_PHI = 3, PT_PX = 4, PT_PY = 5,
+  PT_ETOP = 6, PT_EBOT = 8, PT_AS = 10, PT_AP = 12, PT_STRIDE = 16
+};
+// per-order vectors kept per point: kx, ky, kz_kt, kz_kb, gap y, Fc, Fs  (7 x N complex)
+enum { OV_KX = 0, OV_KY, OV_KZT, OV_KZB, OV_Y, OV_FC, OV_FS, OV_COUNT };
+// per-point diagonal blocks (n each): ref11, ref12, ref21, ref22, trn11, trn12, trn21, trn22
+enum { DG_R11 = 0, DG_R12, DG_R21, DG_R22, DG_T11, DG_T12, DG_T21, DG_T22, DG_COUNT };
+
+// Buffers of one solve's workspace. Pair (point p, layer l) is batch element p * L + l.
+// After a solve: M0 holds LU(I - s11_l Gamma_l) per pair (pivots in ipiv[2], a P x n slice
+// per layer), M1 the solved s12 columns Xs_l, M2 Gamma_l Xs_l, M3 X Y X, M4 A, M5 B, M6 per
+// pair [Gamma_{l+1} | S12_l] (row stride 2n), G0 LU(I - t11 Gamma_L) per point (pivots in
+// ipiv[3]), vec[2] / vec[3] the reflected / transmitted basis columns.
+struct wb_solve_ws {
+  cd* M[6];
+  cd* M6;
+  cd *w, *q, *tau;
+  void* rotws;
+  int* ipiv[4];
+  int* info[4];
+  cd *ov, *dg, *G0;
+  cd* vec[4];
+  double* sig;
+};
+wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode);
+
+// Checkpoint kept by a solve for its adjoint (2D path): per pair the unrotated eigenvectors W,
+// Wi = (R W)^-1, Vi = (R' V)^-1 and the layer S11; per point the transmitted columns d_l,
+// l = 0..L, of the back-sweep (d_L = u at the top interface).
+struct wb_ck {
+  cd *W, *Wi, *Vi, *S11, *d;
+};
+size_t wb_checkpoint_bytes(int P, int L, int hu, int hv, int mode);
+wb_ck wb_ck_carve(void* ck, int P, int L, int hu, int hv, int mode);
+
+size_t wb_adjoint_workspace_bytes(int P, int L, int hu, int hv, int mode);
+struct wb_adjoint_args {
+  const wb_solve_args* fwd;   // the same arguments as the checkpointed solve
+  int n_conv;                 // number of conv slots the gather index addresses
+  const void* solve_ws;       // that solve's workspace, untouched since
+  const void* ck;             // that solve's checkpoint
+  const double* eff_bar;      // P x 3 x 4 x N cotangent of the efficiencies
+  cd* d_epx;                  // n_conv x N x N (overwritten)
+  cd* d_epy;
+  cd* d_epz;
+  double* d_thick;            // P x L (overwritten)
+};
+hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st);

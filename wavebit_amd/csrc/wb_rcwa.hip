@@ -21,19 +21,7 @@
 //                 transmitted wave is swept back through the stored LU factors as vectors: the
 //                 same S22/S12 columns the star cascade gives (scattering.py:191-200).
 //   efficiencies  scattering.py:907-994, solve_1d.py:1588-1610, observations/power.py.
-#include "wb_internal.h"
-
-
-// per-point parameter record (doubles), set by the host
-enum {
-  PT_WL = 0, PT_KX0 = 1, PT_KY0 = 2, PT_PHI = 3, PT_PX = 4, PT_PY = 5,
-  PT_ETOP = 6, PT_EBOT = 8, PT_AS = 10, PT_AP = 12, PT_STRIDE = 16
-};
-// per-order vectors kept per point: kx, ky, kz_kt, kz_kb, gap y, Fc, Fs  (7 x N complex)
-enum { OV_KX = 0, OV_KY, OV_KZT, OV_KZB, OV_Y, OV_FC, OV_FS, OV_COUNT };
-// per-point diagonal blocks (n each): ref11, ref12, ref21, ref22, trn11, trn12, trn21, trn22
-enum { WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2 };
-enum { DG_R11 = 0, DG_R12, DG_R21, DG_R22, DG_T11, DG_T12, DG_T21, DG_T22, DG_COUNT };
+#include "wb_rcwa.h"
 
 namespace {
 
@@ -511,6 +499,53 @@ size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
   return s + 4096;
 }
 
+wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode) {
+  const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
+  const size_t n = (mode == 0) ? 2 * N : N;
+  const size_t B = (size_t)P * L;
+  const size_t nn = n * n;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += align_up(bytes); return (void*)r; };
+  wb_solve_ws w;
+  for (int i = 0; i < 6; ++i) w.M[i] = (cd*)take(B * nn * sizeof(cd));
+  w.M6 = (cd*)take(B * nn * 2 * sizeof(cd));
+  w.w = (cd*)take(B * n * sizeof(cd));
+  w.q = (cd*)take(B * n * sizeof(cd));
+  w.tau = (cd*)take(B * n * sizeof(cd));
+  w.rotws = take(wb_hqr_ms_rot_bytes((int)n, (int)B));
+  for (int i = 0; i < 4; ++i) w.ipiv[i] = (int*)take(B * n * sizeof(int));
+  for (int i = 0; i < 4; ++i) w.info[i] = (int*)take(B * sizeof(int));
+  w.ov = (cd*)take((size_t)P * OV_COUNT * N * sizeof(cd));
+  w.dg = (cd*)take((size_t)P * DG_COUNT * n * sizeof(cd));
+  w.G0 = (cd*)take((size_t)P * nn * sizeof(cd));
+  for (int i = 0; i < 4; ++i) w.vec[i] = (cd*)take((size_t)P * n * 2 * sizeof(cd));
+  w.sig = (double*)take((size_t)P * sizeof(double));
+  return w;
+}
+
+size_t wb_checkpoint_bytes(int P, int L, int hu, int hv, int mode) {
+  const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
+  const size_t n = (mode == 0) ? 2 * N : N;
+  const size_t B = (size_t)P * L;
+  return 4 * align_up(B * n * n * sizeof(cd)) + align_up((size_t)P * (L + 1) * n * 2 * sizeof(cd)) +
+         256;
+}
+
+wb_ck wb_ck_carve(void* ck, int P, int L, int hu, int hv, int mode) {
+  const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
+  const size_t n = (mode == 0) ? 2 * N : N;
+  const size_t B = (size_t)P * L;
+  char* p = (char*)ck;
+  auto take = [&](size_t bytes) { char* r = p; p += align_up(bytes); return (cd*)r; };
+  wb_ck c;
+  c.W = take(B * n * n * sizeof(cd));
+  c.Wi = take(B * n * n * sizeof(cd));
+  c.Vi = take(B * n * n * sizeof(cd));
+  c.S11 = take(B * n * n * sizeof(cd));
+  c.d = take((size_t)P * (L + 1) * n * 2 * sizeof(cd));
+  return c;
+}
+
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   const int P = a.P, L = a.L, mode = a.mode;
   const int N = (2 * a.hu + 1) * (2 * a.hv + 1);
@@ -518,26 +553,27 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   const int B = P * L;
   const long long nn = (long long)n * n;
   const long long NN = (long long)N * N;
-  char* p = (char*)ws;
-  auto take = [&](size_t bytes) { char* r = p; p += align_up(bytes); return (void*)r; };
+  const wb_solve_ws W_ = wb_solve_carve(ws, P, L, a.hu, a.hv, mode);
   cd* M[6];
-  for (int i = 0; i < 6; ++i) M[i] = (cd*)take((size_t)B * nn * sizeof(cd));
-  cd* M6 = (cd*)take((size_t)B * nn * 2 * sizeof(cd));
-  cd* w = (cd*)take((size_t)B * n * sizeof(cd));
-  cd* q = (cd*)take((size_t)B * n * sizeof(cd));
-  cd* tau = (cd*)take((size_t)B * n * sizeof(cd));
-  void* rotws = take(wb_hqr_ms_rot_bytes(n, B));
+  for (int i = 0; i < 6; ++i) M[i] = W_.M[i];
+  cd* M6 = W_.M6;
+  cd* w = W_.w;
+  cd* q = W_.q;
+  cd* tau = W_.tau;
+  void* rotws = W_.rotws;
   int* ipiv[4];
-  for (int i = 0; i < 4; ++i) ipiv[i] = (int*)take((size_t)B * n * sizeof(int));
   int* info[4];
-  for (int i = 0; i < 4; ++i) info[i] = (int*)take((size_t)B * sizeof(int));
-  cd* ov = (cd*)take((size_t)P * OV_COUNT * N * sizeof(cd));
-  cd* dg = (cd*)take((size_t)P * DG_COUNT * n * sizeof(cd));
-  cd* G0 = (cd*)take((size_t)P * nn * sizeof(cd));
+  for (int i = 0; i < 4; ++i) { ipiv[i] = W_.ipiv[i]; info[i] = W_.info[i]; }
+  cd* ov = W_.ov;
+  cd* dg = W_.dg;
+  cd* G0 = W_.G0;
   const int nrhs = (mode == 0) ? 2 : 1;
   cd* vec[4];
-  for (int i = 0; i < 4; ++i) vec[i] = (cd*)take((size_t)P * n * 2 * sizeof(cd));
-  double* sig = (double*)take((size_t)P * sizeof(double));
+  for (int i = 0; i < 4; ++i) vec[i] = W_.vec[i];
+  double* sig = W_.sig;
+  const bool keep = (a.ck != nullptr) && mode == 0;
+  wb_ck CK;
+  if (keep) CK = wb_ck_carve(a.ck, P, L, a.hu, a.hv, mode);
   hipError_t e;
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
 
@@ -573,6 +609,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                              S(M[0] + N, nn), n, B, st))) return e;
     if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st))) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
+    if (keep) {
+      hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.W, nn), n);
+      WB_CHECK_LAUNCH();
+    }
     // V GEMMs: top = -epy W_bot, bottom = epx W_top (into M4)
     if ((e = wb_zgemm_launch(N, n, N, mone, EPY, N, S(M[3] + (long long)N * n, nn), n, zero,
                              S(M[4], nn), n, B, st))) return e;
@@ -619,10 +659,18 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
   if ((e = wb_getrs_launch(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, st))) return e;
+  if (keep) {
+    hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[0], nn), n, S(CK.Wi, nn), n);
+    WB_CHECK_LAUNCH();
+  }
   if ((e = wb_getrf_launch(n, S(bigV, nn), n, ipiv[1], info[2], B, st))) return e;
   FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[3], nn), n, B, st))) return e;
   if ((e = wb_getrs_launch(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, st))) return e;
+  if (keep) {
+    hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.Vi, nn), n);
+    WB_CHECK_LAUNCH();
+  }
   // A -> M4, B -> M5
   hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, M[0], M[3], ov, M[4], M[5]);
   WB_CHECK_LAUNCH();
@@ -655,6 +703,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_getrf_launch(n, S(M[0], nn), n, ipiv[1], info[2], B, st))) return e;
   FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
   if ((e = wb_getrs_launch(n, 2 * n, S(M[0], nn), n, ipiv[1], S(M6, n2), 2 * n, B, st))) return e;
+  if (keep) {
+    hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M6, n2), 2 * n, S(CK.S11, nn), n);
+    WB_CHECK_LAUNCH();
+  }
 
   // ---------------------------------------------------------------- cascade
   // pair (p, l) lives at index p * L + l; per-point views step by L pairs
@@ -707,6 +759,13 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_rhs_init, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[0]);
   WB_CHECK_LAUNCH();
   if ((e = wb_getrs_launch(n, nrhs, Tf, n, piv_f, S(vec[0], vs), nrhs, P, st))) return e;
+  // per-point transmitted columns d_l (l = 0..L) kept for the adjoint
+  const long long dstride = (long long)(L + 1) * vs;
+  if (keep) {
+    hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(vec[0], vs), nrhs,
+                       S(CK.d + (long long)L * vs, dstride), nrhs);
+    WB_CHECK_LAUNCH();
+  }
   // gu = G_L u
   if ((e = wb_zgemm_launch(n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero, S(vec[1], vs),
                            nrhs, P, st))) return e;
@@ -722,6 +781,11 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     if ((e = wb_zgemm_launch(n, nrhs, n, one, s12, 2 * n, S(dcur, vs), nrhs, zero, S(dnext, vs),
                              nrhs, P, st))) return e;
     if ((e = wb_getrs_launch(n, nrhs, Tm, n, piv_pl, S(dnext, vs), nrhs, P, st))) return e;
+    if (keep) {
+      hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(dnext, vs), nrhs,
+                         S(CK.d + (long long)l * vs, dstride), nrhs);
+      WB_CHECK_LAUNCH();
+    }
     cd* tmp = dcur; dcur = dnext; dnext = tmp;
   }
   hipLaunchKernelGGL(k_transmit, grid2(vs, P), dim3(256), 0, st, n, nrhs, dg, dcur, vec[3]);

@@ -84,11 +84,9 @@ class CheckReport:
 # --------------------------------------------------------------------------- admission
 
 def _host_array(value, path):
+    """A host complex128 snapshot of a raster; a live (requires_grad) tensor is read detached --
+    its graph is taken up by the differentiable run."""
     if hasattr(value, 'detach'):
-        if getattr(value, 'requires_grad', False):
-            raise UnsupportedFeatureError(
-                'Gradients through Prepared.run are provided by wavebit_amd.rcwa.grad, not by '
-                'autograd through this entry point.', code='rcwa.gradient', path=path)
         value = value.detach().cpu().numpy()
     try:
         return np.asarray(value, dtype=np.complex128)
@@ -96,11 +94,12 @@ def _host_array(value, path):
         raise ValidationError('expected numbers', code='rcwa.invalid', path=path) from None
 
 
-def _scalar(value, path, real=True):
+def _scalar(value, path, real=True, live_ok=False):
     if hasattr(value, 'detach'):
-        if getattr(value, 'requires_grad', False):
-            raise UnsupportedFeatureError('Autograd through Prepared.run is not implemented.',
-                                          code='rcwa.gradient', path=path)
+        if getattr(value, 'requires_grad', False) and not live_ok:
+            raise UnsupportedFeatureError(
+                'Gradients are implemented with respect to raster permittivities and layer '
+                'thicknesses; this value carries requires_grad.', code='rcwa.gradient', path=path)
         value = value.detach().cpu().item()
     z = complex(value)
     if real:
@@ -109,6 +108,16 @@ def _scalar(value, path, real=True):
                                           path=path)
         return z.real
     return z
+
+
+def _is_live(value):
+    return bool(getattr(value, 'requires_grad', False))
+
+
+def _live(problem):
+    """True when a layer's raster or thickness is a tensor that requires a gradient (reference
+    live contract, _lifecycle.py:785-786)."""
+    return any(_is_live(l.epsilon) or _is_live(l.thickness) for l in problem.stack.layers)
 
 
 def _material(m, path):
@@ -216,7 +225,7 @@ def _resolve(problem, numerics, execution):
     elif uniform:
         rasters = np.stack([np.full((1, 1), _material(l.background, f'stack.layers[{i}].background'),
                                     dtype=np.complex128) for i, l in enumerate(layers)])
-        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness')
+        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness', live_ok=True)
                               for i, l in enumerate(layers)])
     else:
         arrs = [_host_array(l.epsilon, f'stack.layers[{i}].epsilon') for i, l in enumerate(layers)]
@@ -228,7 +237,7 @@ def _resolve(problem, numerics, execution):
             raise ValidationError('Every raster layer of a stack must have the same (Ny, Nx) shape.',
                                   code='layer.epsilon_shape', path='stack.layers')
         rasters = np.stack(arrs)
-        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness')
+        thickness = np.array([_scalar(l.thickness, f'stack.layers[{i}].thickness', live_ok=True)
                               for i, l in enumerate(layers)])
     hu, hv = numerics.orders
     if rasters.shape[1] == 1 and hv != 0 and not uniform:
@@ -374,6 +383,8 @@ class Prepared:
             if not ok:
                 raise IncompatibleProblemError('The prepared runtime guard failed.',
                                                code=f'prepared.{name}', path='problem')
+        if _live(problem):
+            return _run_live(self, problem, r, checks)
         try:
             per_wl, flags = _solve_resolved(r, self._cache)
         except FloatingPointError as error:
@@ -396,6 +407,59 @@ class Prepared:
             raise NumericalError('The requested solve produced nonfinite observables; no implicit '
                                  'detuning was applied.', code='rcwa.nonfinite', path='result')
         return result
+
+
+def _run_live(prepared, problem, r, checks):
+    """Differentiable run: efficiencies as torch tensors carrying the graph to the layer rasters
+    and thicknesses (the hand-written reverse pass of ``adjoint.RasterSolve``)."""
+    import torch
+    from dataclasses import replace
+    from . import adjoint
+    if r.uniform or r.path != _engine.PATH_2D:
+        raise UnsupportedFeatureError(
+            'Gradients are implemented for raster stacks on the 2D coupled path; this problem '
+            f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
+    device = torch.device(prepared._cache.conv.device)
+    _engine._native.context(device)
+    layers = problem.stack.layers
+    eps_list = [l.epsilon if torch.is_tensor(l.epsilon) else torch.as_tensor(np.asarray(l.epsilon))
+                for l in layers]
+    cplx = any(e.is_complex() for e in eps_list)
+    dt = torch.complex128 if cplx else torch.float64
+    eps = torch.stack([e.to(dtype=dt) for e in eps_list])
+    thick = torch.stack([l.thickness.to(torch.float64).reshape(()) if torch.is_tensor(l.thickness)
+                         else torch.tensor(float(l.thickness), dtype=torch.float64) for l in layers])
+    points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
+                            polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
+                            period=r.period, thickness=tuple(float(t) for t in r.thickness))
+              for wl in r.wavelengths]
+    eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device)
+    if bool((flags != 0).any()):
+        raise NumericalError('A layer eigensolve did not converge or a scattering-matrix '
+                             'factorization was singular.', code='rcwa.singular', path='problem')
+    eff_h, amp_h, pe_h = (t.detach().cpu().numpy() for t in (eff, amp, pe))
+    per_wl = [_engine.unpack(eff_h, amp_h, pe_h, p, r.hu, r.hv, r.path) for p in range(len(points))]
+    prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 (differentiable)',
+            'length_units': problem.units.length, 'execution': prepared.execution,
+            'numerics': prepared.numerics}
+    diag = {'report': prepared.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
+            'gradient': 'hand-written adjoint (raster permittivity, layer thickness)'}
+    result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov)
+    if not result.validity.checks['numerical_finite']:
+        raise NumericalError('The requested solve produced nonfinite observables.',
+                             code='rcwa.nonfinite', path='result')
+    P = len(points)
+    nu, nv = 2 * r.hu + 1, 2 * r.hv + 1
+    e0 = eff[:, 0]
+    live = {}
+    for port, (s_i, p_i) in (('reflected', (0, 1)), ('transmitted', (2, 3))):
+        es = e0[:, s_i].reshape(P, nv, nu)
+        ep = e0[:, p_i].reshape(P, nv, nu)
+        live[port] = replace(result.outputs[port], efficiency=es + ep, efficiency_s=es,
+                             efficiency_p=ep)
+    outputs = dict(result.outputs)
+    outputs.update(live)
+    return replace(result, outputs=outputs)
 
 
 def _solve_resolved(r, cache):
