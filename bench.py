@@ -12,6 +12,9 @@ and the per-order efficiencies, in one launch sequence. ``value`` is points solv
 (one solve = one (problem, wavelength, angle, polarization) point). The plain form solves the
 same points one point per launch sequence; its rate and the ratio are reported beside it.
 
+The gradient leg differentiates the sum of T over the same 64 points with respect to every
+raster pixel and thickness (forward with checkpoint, then the hand-written reverse pass).
+
 The serving leg drives the continuous-batching server with a Poisson stream of requests at
 the same orders (1-4 layers, 1-8 wavelengths each, arriving while earlier ones are on the
 device) and reports its p50/p99 request latency and served solves/s.
@@ -39,9 +42,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--points', type=int, default=64)
     ap.add_argument('--plain-points', type=int, default=2)
-    ap.add_argument('--cpu-points', type=int, default=1)
+    ap.add_argument('--cpu-points', type=int, default=5)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--orders', type=int, default=9)
+    ap.add_argument('--grad-points', type=int, default=64,
+                    help='points of the gradient leg (0 skips it)')
     ap.add_argument('--serve-requests', type=int, default=48)
     ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
@@ -149,29 +154,76 @@ def main():
                     'flops_per_launch': flops_per_launch,
                     'share_of_step': round(k['total_ms'] / 1e3 / dt, 4)}
 
-    # CPU baseline (oracle, NumPy FP64) on a bounded sample; parity of the same points
+    # CPU baseline (oracle, NumPy FP64) on a bounded sample; parity of the same points.
+    # One untimed warm-up solve, then the median per-point time of the sample.
     cpu = None
     oracle_err = None
     if not args.no_cpu and args.cpu_points > 0:
         from oracle import rcwa_oracle as orc
         n_cpu = min(args.cpu_points, P)
-        effh = eff.cpu().numpy()
-        t0 = time.perf_counter()
-        errs = []
-        for i in range(n_cpu):
+        effh, amph, peh = eff.cpu().numpy(), amp.cpu().numpy(), pe.cpu().numpy()
+
+        def oracle_point(i):
             p = wl['points'][i]
-            ref = orc.solve_raster(wl['rasters'], wl['thickness'], wl['eps_top'], wl['eps_bottom'],
-                                   wl['period'], p['wavelength'], p['theta'], p['phi'],
-                                   p['polarization'], (hu, hv))
-            got = eng.unpack(effh, amp.cpu().numpy(), pe.cpu().numpy(), i, hu, hv, eng.PATH_2D)
+            return orc.solve_raster(wl['rasters'], wl['thickness'], wl['eps_top'], wl['eps_bottom'],
+                                    wl['period'], p['wavelength'], p['theta'], p['phi'],
+                                    p['polarization'], (hu, hv))
+        oracle_point(0)
+        times, errs = [], []
+        stride = max(1, P // n_cpu)
+        for i in range(0, stride * n_cpu, stride):
+            t0 = time.perf_counter()
+            ref = oracle_point(i)
+            times.append(time.perf_counter() - t0)
+            got = eng.unpack(effh, amph, peh, i, hu, hv, eng.PATH_2D)
             errs.append(max(float(np.abs(got['incident'][k] - ref['incident'][k]).max())
                             for k in ('R', 'T')))
-        cpu_dt = time.perf_counter() - t0
-        cpu = {'value': round(n_cpu / cpu_dt, 6), 'unit': 'solves/s', 'cores': os.cpu_count(),
+        med = float(np.median(times))
+        threads = os.cpu_count()
+        try:
+            from threadpoolctl import threadpool_info
+            blas = [t['num_threads'] for t in threadpool_info() if t.get('user_api') == 'blas']
+            if blas:
+                threads = max(blas)
+        except Exception:
+            pass
+        cpu = {'value': round(1.0 / med, 6), 'unit': 'solves/s', 'cores': threads,
                'kind': 'port',
-               'sample': f'{n_cpu} point(s) of the same workload (4 layers, orders (9,9)) through '
-                         f'the NumPy FP64 oracle, {cpu_dt:.1f} s'}
+               'sample': f'{n_cpu} points of the same workload (4 layers, orders (9,9), every '
+                         f'{stride}th wavelength) through the NumPy FP64 oracle after one warm-up; '
+                         f'median {med:.2f} s per point, {sum(times):.1f} s total'}
         oracle_err = max(errs)
+
+    # gradient: sum of T over the points, differentiated with respect to every raster pixel and
+    # thickness by the hand-written reverse pass (forward with checkpoint + adjoint + assembly
+    # adjoint), the same 4-layer (9,9) stack
+    grad = None
+    if args.grad_points > 0:
+        from wavebit_amd.rcwa import adjoint
+        Pg = min(args.grad_points, P)
+        pts_g = [eng.Point(stack=0, wavelength=p['wavelength'], theta=p['theta'], phi=p['phi'],
+                           polarization=p['polarization'], eps_top=wl['eps_top'],
+                           eps_bottom=wl['eps_bottom'], period=wl['period'],
+                           thickness=tuple(wl['thickness'])) for p in wl['points'][:Pg]]
+        eps_g = torch.tensor(wl['rasters'], device=dev, requires_grad=True)
+        thick_g = torch.tensor(wl['thickness'], device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eff_g, _, _, flags_g = adjoint.solve(eps_g, thick_g, pts_g, hu, hv, dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        (eff_g[:, 0, 2] + eff_g[:, 0, 3]).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grad = {'points': Pg, 'forward_with_checkpoint_s': round(t1 - t0, 4),
+                'backward_s': round(t2 - t1, 4),
+                'gradient_solves_per_s': round(Pg / (t2 - t0), 4),
+                'backward_over_forward': round((t2 - t1) / (t1 - t0), 3),
+                'grad_norm_eps': float(eps_g.grad.norm()), 'grad_thickness': thick_g.grad.tolist(),
+                'flagged_points': int((flags_g != 0).sum()),
+                'objective': 'sum of T over the points; d/d every pixel of 4 x 128 x 128 and the '
+                             '4 thicknesses'}
+        del eff_g, eps_g, thick_g
 
     # serving: a Poisson request stream through the continuous-batching server
     serve_report = None
@@ -214,6 +266,7 @@ def main():
         'batched_vs_plain': round(batched_rate / plain_rate, 3) if plain_rate else None,
         'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
         'roofline': roofline, 'cpu_baseline': cpu,
+        'gradient': grad,
         'serve': serve_report,
         'p50_latency_s': serve_report['p50_latency_s'] if serve_report else None,
         'p99_latency_s': serve_report['p99_latency_s'] if serve_report else None,
