@@ -35,6 +35,30 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+KERNEL_SYMBOLS = {'hqr': 'k_hqr_ms', 'gehrd': 'k_gehrd', 'unghr': 'k_unghr', 'trevc': 'k_trevc',
+                  'zgemm': 'k_zgemm', 'getrf_panel': 'k_getrf_panel'}
+
+
+def _traffic(name, orders, layers, points):
+    """HBM bytes per launch of kernel ``name`` from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE and WRITE_SIZE passes, calibrated by a known fill and scale-copy), or
+    None when no summary matches the workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        'rcwa_orders9_64pts_traffic.json')
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    w = data.get('workload', {})
+    if w.get('orders') != [orders, orders] or w.get('layers') != layers or w.get('points') != points:
+        return None, None
+    k = data.get('kernels', {}).get(KERNEL_SYMBOLS.get(name, name))
+    if not k:
+        return None, None
+    return float(k['bytes_per_launch']), 'profiles/rcwa_orders9_64pts_hbm_traffic.md'
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -147,9 +171,11 @@ def main():
         avg_s = k['total_ms'] / k['launches'] / 1e3
         flops_per_launch = k['flops'] / k['launches']
         achieved = flops_per_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+        traffic, traffic_src = _traffic(name, M, L, P)
         roofline = {'bound': 'mfma', 'kernel': name, 'achieved': round(achieved, 4),
                     'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                    'frac': round(achieved / PEAK_FP64_TFLOPS, 5), 'traffic': None,
+                    'frac': round(achieved / PEAK_FP64_TFLOPS, 5), 'traffic': traffic,
+                    'traffic_source': traffic_src,
                     'avg_launch_ms': round(k['total_ms'] / k['launches'], 4),
                     'flops_per_launch': flops_per_launch,
                     'share_of_step': round(k['total_ms'] / 1e3 / dt, 4)}
