@@ -23,6 +23,9 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int NTH = 1024;   // Hessenberg stages: 16 waves per matrix to hide global latency
+// From this order on the Hessenberg reduction and Q run blocked (panel kernel + GEMMs); the
+// scratch they need (n x HNB x 4 + panels x HNB^2) must fit the n x n buffer X.
+constexpr int WB_BLOCKED_HESSENBERG_MIN = 192;
 constexpr double ULP = DBL_EPSILON;            // 2^-52, LAPACK dlamch('P')
 constexpr double SAFMIN = DBL_MIN;
 
@@ -161,6 +164,158 @@ __global__ __launch_bounds__(NTH) void k_unghr(int n, cd* Ab, const cd* taub, cd
   }
 }
 
+// ----------------------------------------------------------------------------- blocked Hessenberg
+// Panel of HNB columns (LAPACK zlahr2's scheme, written for one workgroup per matrix): for
+// column c = k0 + i, apply the panel's earlier reflectors (from the right through
+// Y = A V T, from the left through V T^H V^H), generate reflector i from A[c+1:, c], and form
+// Y[k0+1:, i] = tau (A[k0+1:, c+1:] v - Y (V^H v)) -- the one matrix-vector product per column
+// that streams the trailing block (its columns are untouched within the panel). The trailing
+// updates A -= Y V^H and A -= V T^H V^H A are batched GEMMs on the matrix cores.
+constexpr int HNB = 32;
+
+// Y and T live in the per-matrix scratch X, whose batch stride is x_stride.
+__global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab, cd* Vb, cd* Yb,
+                                                   cd* Tb, long long x_stride) {
+  const long long b = blockIdx.x;
+  cd* A = Ab + b * (long long)n * n;
+  cd* V = Vb + b * (long long)n * n;            // column k0 + j holds reflector j (dense)
+  cd* Y = Yb + b * x_stride;                     // n x HNB
+  cd* Tg = Tb + b * x_stride;                    // this panel's T, HNB x HNB
+  __shared__ cd sT[HNB][HNB + 1];
+  __shared__ cd red[32][HNB + 1];
+  __shared__ cd sa[HNB], sw[HNB];
+  __shared__ double sred[NTH / 64];
+  __shared__ cd s_tau;
+  extern __shared__ cd sv[];                     // reflector i by absolute row, n entries
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int jj = t & 31, ch = t >> 5;            // (reflector index, row chunk) for reductions
+  for (int e = t; e < HNB * (HNB + 1); e += NTH) sT[e / (HNB + 1)][e % (HNB + 1)] = mkc(0.0, 0.0);
+  __syncthreads();
+  for (int i = 0; i < ib; ++i) {
+    const int c = k0 + i;
+    if (i > 0) {
+      // right: A[r][c] -= sum_j Y[r][j] conj(V[c][k0+j]), rows k0+1..n-1
+      if (t < i) sa[t] = cconj(V[(long long)c * n + k0 + t]);
+      __syncthreads();
+      for (int r = k0 + 1 + t; r < n; r += NTH) {
+        cd acc = mkc(0.0, 0.0);
+        for (int j = 0; j < i; ++j) acc = cadd(acc, cmul(Y[(long long)r * HNB + j], sa[j]));
+        A[(long long)r * n + c] = csub(A[(long long)r * n + c], acc);
+      }
+      __syncthreads();
+      // left: w = V^H A[:, c] (rows k0+1..), w = T^H w, A[:, c] -= V w
+      {
+        cd acc = mkc(0.0, 0.0);
+        if (jj < i)
+          for (int r = k0 + 1 + ch; r < n; r += 32)
+            acc = cadd(acc, cmulc(V[(long long)r * n + k0 + jj], A[(long long)r * n + c]));
+        red[ch][jj] = acc;
+      }
+      __syncthreads();
+      if (t < i) {
+        cd s = mkc(0.0, 0.0);
+        for (int q = 0; q < 32; ++q) s = cadd(s, red[q][t]);
+        sw[t] = s;
+      }
+      __syncthreads();
+      if (t < i) {
+        cd s = mkc(0.0, 0.0);
+        for (int l = 0; l <= t; ++l) s = cadd(s, cmulc(sT[l][t], sw[l]));
+        sa[t] = s;
+      }
+      __syncthreads();
+      for (int r = k0 + 1 + t; r < n; r += NTH) {
+        cd acc = mkc(0.0, 0.0);
+        for (int j = 0; j < i; ++j) acc = cadd(acc, cmul(V[(long long)r * n + k0 + j], sa[j]));
+        A[(long long)r * n + c] = csub(A[(long long)r * n + c], acc);
+      }
+      __syncthreads();
+    }
+    // reflector from x = A[c+1:, c]
+    double ss = 0.0;
+    for (int r = c + 2 + t; r < n; r += NTH) ss += cnorm2(A[(long long)r * n + c]);
+    const double xnorm2 = block_sum<NTH>(ss, sred);
+    const cd alpha = A[(long long)(c + 1) * n + c];
+    cd tk, scale;
+    double beta;
+    if (xnorm2 == 0.0 && alpha.y == 0.0) {
+      tk = mkc(0.0, 0.0);
+      beta = alpha.x;
+      scale = mkc(0.0, 0.0);
+    } else {
+      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
+      tk = mkc((beta - alpha.x) / beta, -alpha.y / beta);
+      scale = crecip(mkc(alpha.x - beta, alpha.y));
+    }
+    for (int r = c + 1 + t; r < n; r += NTH)
+      sv[r] = (r == c + 1) ? mkc(1.0, 0.0) : cmul(A[(long long)r * n + c], scale);
+    __syncthreads();
+    for (int r = t; r < n; r += NTH) {
+      V[(long long)r * n + k0 + i] = (r <= c) ? mkc(0.0, 0.0) : sv[r];
+      if (r == c + 1) A[(long long)r * n + c] = mkc(beta, 0.0);
+      else if (r > c + 1) A[(long long)r * n + c] = mkc(0.0, 0.0);
+    }
+    // Y[r][i] (unscaled) = A[r, c+1:] v, rows k0+1..n-1: a wave per row
+    for (int r = k0 + 1 + wave; r < n; r += NTH / 64) {
+      const cd* row = A + (long long)r * n;
+      double zr = 0.0, zi = 0.0;
+      for (int col = c + 1 + lane; col < n; col += 64) {
+        const cd p = cmul(row[col], sv[col]);
+        zr += p.x;
+        zi += p.y;
+      }
+      zr = wave_sum(zr);
+      zi = wave_sum(zi);
+      if (lane == 0) Y[(long long)r * HNB + i] = mkc(zr, zi);
+    }
+    // tv = V[:, 0:i]^H v over rows c+1..n-1
+    {
+      cd acc = mkc(0.0, 0.0);
+      if (jj < i)
+        for (int r = c + 1 + ch; r < n; r += 32)
+          acc = cadd(acc, cmulc(V[(long long)r * n + k0 + jj], sv[r]));
+      red[ch][jj] = acc;
+    }
+    if (t == 0) s_tau = tk;
+    __syncthreads();
+    if (t < i) {
+      cd s = mkc(0.0, 0.0);
+      for (int q = 0; q < 32; ++q) s = cadd(s, red[q][t]);
+      sw[t] = s;
+    }
+    __syncthreads();
+    // Y[:, i] = tau (y - Y[:, 0:i] tv) ; T[0:i, i] = -tau T[0:i, 0:i] tv ; T[i][i] = tau
+    for (int r = k0 + 1 + t; r < n; r += NTH) {
+      cd acc = Y[(long long)r * HNB + i];
+      for (int j = 0; j < i; ++j) acc = csub(acc, cmul(Y[(long long)r * HNB + j], sw[j]));
+      Y[(long long)r * HNB + i] = cmul(tk, acc);
+    }
+    if (t < i) {
+      cd s = mkc(0.0, 0.0);
+      for (int l = t; l < i; ++l) s = cadd(s, cmul(sT[t][l], sw[l]));
+      sT[t][i] = cneg(cmul(tk, s));
+    }
+    if (t == 0) sT[i][i] = tk;
+    __syncthreads();
+  }
+  for (int e = t; e < HNB * HNB; e += NTH) Tg[e] = sT[e / HNB][e % HNB];
+}
+
+// Zero a (rows x cols) block per batch element.
+__global__ void k_zero_block(int rows, int cols, cd* Xb, long long stride, int ld) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  Xb[b * stride + (e / cols) * ld + e % cols] = mkc(0.0, 0.0);
+}
+
+__global__ void k_identity(int n, cd* Qb) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  Qb[b * (long long)n * n + e] = mkc((e / n) == (e % n) ? 1.0 : 0.0, 0.0);
+}
+
 // ----------------------------------------------------------------------------- eigenvectors
 
 // X (n x n, upper triangular) with T X = X diag(T_kk): thread per eigenvector k, rows swept
@@ -228,6 +383,94 @@ __global__ __launch_bounds__(NT) void k_colnorm(int n, cd* Wb) {
 
 }  // namespace
 
+// Blocked reduction of every A to Hessenberg form: panels of HNB columns, each a panel kernel
+// followed by the GEMM updates of the rest of the matrix. V (n x n per matrix) receives the
+// reflectors as dense columns; X holds Y (n x HNB), the per-panel T factors and GEMM scratch.
+static hipError_t wb_gehrd_blocked(int n, cd* A, cd* V, cd* X, int batch, hipStream_t stream) {
+  const double n3 = (double)n * n * n;
+  WbStageTimer stage(WB_T_GEHRD, stream, 40.0 / 3.0 * n3 * batch);
+  const long long nn = (long long)n * n;
+  const int npan = (n - 2 + HNB - 1) / HNB;
+  cd* Y = X;                                             // n x HNB
+  cd* T = Y + (long long)n * HNB;                        // npan x HNB x HNB
+  cd* Wm = T + (long long)npan * HNB * HNB;              // HNB x n
+  cd* Wm2 = Wm + (long long)HNB * n;                     // HNB x n
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  auto S = [&](cd* p, long long stride) { return bp_strided(p, stride); };
+  hipError_t e;
+  hipLaunchKernelGGL(k_zero_block, dim3((unsigned)((nn + 255) / 256), batch), dim3(256), 0, stream,
+                     n, n, V, nn, n);
+  WB_CHECK_LAUNCH();
+  int p = 0;
+  for (int k0 = 0; k0 < n - 2; k0 += HNB, ++p) {
+    const int ib = (n - 2 - k0 < HNB) ? n - 2 - k0 : HNB;
+    cd* Tp = T + (long long)p * HNB * HNB;
+    hipLaunchKernelGGL(k_hrd_panel, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, k0, ib,
+                       A, V, Y, Tp, nn);
+    WB_CHECK_LAUNCH();
+    const int top = k0 + 1;                 // rows 0..k0
+    const int m1 = n - k0 - 1;              // rows k0+1..n-1
+    const int rest = n - k0 - ib;           // trailing columns k0+ib..n-1
+    // Y[0:top] = (A[0:top, k0+1:] V[k0+1:, panel]) T
+    if ((e = wb_zgemm_ops_launch(0, 0, top, ib, m1, one, S(A + k0 + 1, nn), n,
+                                 S(V + (long long)(k0 + 1) * n + k0, nn), n, zero, S(Wm, nn), ib,
+                                 batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(0, 0, top, ib, ib, one, S(Wm, nn), ib, S(Tp, nn), HNB, zero,
+                                 S(Y, nn), HNB, batch, stream))) return e;
+    // panel columns k0+1..k0+ib-1, rows 0..k0: A -= Y[0:top] V[k0+1:k0+ib, panel]^H
+    if (ib > 1 && (e = wb_zgemm_ops_launch(0, 1, top, ib - 1, ib, mone, S(Y, nn), HNB,
+                                           S(V + (long long)(k0 + 1) * n + k0, nn), n, one,
+                                           S(A + k0 + 1, nn), n, batch, stream))) return e;
+    if (rest <= 0) continue;
+    // right: A[:, k0+ib:] -= Y V[k0+ib:, panel]^H
+    if ((e = wb_zgemm_ops_launch(0, 1, n, rest, ib, mone, S(Y, nn), HNB,
+                                 S(V + (long long)(k0 + ib) * n + k0, nn), n, one,
+                                 S(A + k0 + ib, nn), n, batch, stream))) return e;
+    // left: A[k0+1:, k0+ib:] -= V T^H (V^H A[k0+1:, k0+ib:])
+    if ((e = wb_zgemm_ops_launch(1, 0, ib, rest, m1, one, S(V + (long long)(k0 + 1) * n + k0, nn), n,
+                                 S(A + (long long)(k0 + 1) * n + k0 + ib, nn), n, zero, S(Wm, nn), n,
+                                 batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(1, 0, ib, rest, ib, one, S(Tp, nn), HNB, S(Wm, nn), n, zero,
+                                 S(Wm2, nn), n, batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(0, 0, m1, rest, ib, mone, S(V + (long long)(k0 + 1) * n + k0, nn), n,
+                                 S(Wm2, nn), n, one, S(A + (long long)(k0 + 1) * n + k0 + ib, nn), n,
+                                 batch, stream))) return e;
+  }
+  return hipSuccess;
+}
+
+// Q = (I - V_0 T_0 V_0^H) (I - V_1 T_1 V_1^H) ... from the blocked reduction's V and T,
+// accumulated backward on the trailing block Q[k0+1:, k0+1:] of each panel.
+static hipError_t wb_unghr_blocked(int n, cd* V, cd* X, cd* Q, int batch, hipStream_t stream) {
+  const double n3 = (double)n * n * n;
+  WbStageTimer stage(WB_T_UNGHR, stream, 16.0 / 3.0 * n3 * batch);
+  const long long nn = (long long)n * n;
+  const int npan = (n - 2 + HNB - 1) / HNB;
+  cd* T = X + (long long)n * HNB;
+  cd* Wm = T + (long long)npan * HNB * HNB;
+  cd* Wm2 = Wm + (long long)HNB * n;
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  auto S = [&](cd* p, long long stride) { return bp_strided(p, stride); };
+  hipError_t e;
+  hipLaunchKernelGGL(k_identity, dim3((unsigned)((nn + 255) / 256), batch), dim3(256), 0, stream, n, Q);
+  WB_CHECK_LAUNCH();
+  for (int p = npan - 1; p >= 0; --p) {
+    const int k0 = p * HNB;
+    const int ib = (n - 2 - k0 < HNB) ? n - 2 - k0 : HNB;
+    const int m1 = n - k0 - 1;
+    cd* Tp = T + (long long)p * HNB * HNB;
+    cd* Vp = V + (long long)(k0 + 1) * n + k0;
+    cd* Qs = Q + (long long)(k0 + 1) * n + k0 + 1;
+    if ((e = wb_zgemm_ops_launch(1, 0, ib, m1, m1, one, S(Vp, nn), n, S(Qs, nn), n, zero, S(Wm, nn), n,
+                                 batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(0, 0, ib, m1, ib, one, S(Tp, nn), HNB, S(Wm, nn), n, zero, S(Wm2, nn),
+                                 n, batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(0, 0, m1, m1, ib, mone, S(Vp, nn), n, S(Wm2, nn), n, one, S(Qs, nn), n,
+                                 batch, stream))) return e;
+  }
+  return hipSuccess;
+}
+
 // A (batch of n x n) is destroyed. Outputs: w (batch x n) eigenvalues, W (batch x n x n)
 // eigenvectors in columns. Workspace: Z (batch x n x n), X (batch x n x n), tau (batch x n).
 hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
@@ -240,11 +483,19 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, vo
   const double n3 = (double)n * n * n;
   hipError_t z = wb_zero_int_launch(info, batch, stream);
   if (z != hipSuccess) return z;
-  WB_TIMED_F(WB_T_GEHRD, stream, 40.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(NTH), sh, stream, n, A, tau));
-  WB_CHECK_LAUNCH();
-  WB_TIMED_F(WB_T_UNGHR, stream, 16.0 / 3.0 * n3 * batch,
-           hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, A, tau, Z));
-  WB_CHECK_LAUNCH();
+  if (n >= WB_BLOCKED_HESSENBERG_MIN) {
+    // Blocked: V (dense reflectors) in W, Y / T / GEMM scratch in X, Q formed in Z.
+    hipError_t e = wb_gehrd_blocked(n, A, W, X, batch, stream);
+    if (e != hipSuccess) return e;
+    e = wb_unghr_blocked(n, W, X, Z, batch, stream);
+    if (e != hipSuccess) return e;
+  } else {
+    WB_TIMED_F(WB_T_GEHRD, stream, 40.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(NTH), sh, stream, n, A, tau));
+    WB_CHECK_LAUNCH();
+    WB_TIMED_F(WB_T_UNGHR, stream, 16.0 / 3.0 * n3 * batch,
+             hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, A, tau, Z));
+    WB_CHECK_LAUNCH();
+  }
   {
     hipError_t e2 = wb_hqr_ms_launch(n, A, Z, w, info, rot_ws, batch, stream);
     if (e2 != hipSuccess) return e2;

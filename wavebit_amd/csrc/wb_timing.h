@@ -10,14 +10,34 @@ enum {
 };
 
 extern bool wb_timing_on;
+extern int wb_timing_stage;   // > 0 while a multi-launch stage is being timed as one entry
 void wb_timer_begin(int id, hipStream_t s);
 void wb_timer_end(int id, hipStream_t s, double flops);
 
-// `flops` is the launch's algorithmic real FP64 flop count (0 where none is defined).
-#define WB_TIMED_F(id, stream, flops, ...)                     \
-  do {                                                         \
-    if (wb_timing_on) wb_timer_begin((id), (stream));          \
-    __VA_ARGS__;                                               \
-    if (wb_timing_on) wb_timer_end((id), (stream), (flops));   \
+// `flops` is the launch's algorithmic real FP64 flop count (0 where none is defined). Inside a
+// stage (WbStageTimer scope) the launches are not timed on their own.
+#define WB_TIMED_F(id, stream, flops, ...)                                           \
+  do {                                                                               \
+    const bool timed__ = wb_timing_on && wb_timing_stage == 0;                       \
+    if (timed__) wb_timer_begin((id), (stream));                                     \
+    __VA_ARGS__;                                                                     \
+    if (timed__) wb_timer_end((id), (stream), (flops));                              \
   } while (0)
+// A stage of several launches timed as one entry of `id` with `flops` algorithmic flops, for
+// the scope of the object (early returns included).
+struct WbStageTimer {
+  int id;
+  hipStream_t stream;
+  double flops;
+  bool timed;
+  WbStageTimer(int id_, hipStream_t s_, double flops_) : id(id_), stream(s_), flops(flops_) {
+    timed = wb_timing_on && wb_timing_stage == 0;
+    if (timed) wb_timer_begin(id, stream);
+    ++wb_timing_stage;
+  }
+  ~WbStageTimer() {
+    --wb_timing_stage;
+    if (timed) wb_timer_end(id, stream, flops);
+  }
+};
 #define WB_TIMED(id, stream, ...) WB_TIMED_F(id, stream, 0.0, __VA_ARGS__)

@@ -5,6 +5,7 @@
 #include "wb_timing.h"
 
 bool wb_timing_on = false;
+int wb_timing_stage = 0;
 
 namespace {
 struct Pending {
