@@ -21,7 +21,14 @@
 
 namespace {
 
-constexpr int NT = 256;
+#ifndef WB_HQR_NT
+#define WB_HQR_NT 512      // threads per matrix
+#endif
+#ifndef WB_HQR_M
+#define WB_HQR_M 8         // bulges (shifts) per sweep
+#endif
+constexpr int NT = WB_HQR_NT;
+constexpr int BG = (WB_HQR_M < 8) ? WB_HQR_M : 8;   // bulges per register band of the deferred updates
 constexpr double ULP = DBL_EPSILON;
 constexpr double SAFMIN = DBL_MIN;
 
@@ -234,9 +241,13 @@ __device__ int single_shift_block(cd* H, cd* Z, cd* wout, int n, int lo_lim, int
   return 0;
 }
 
-// Apply the steps [s0, s1) of a sweep's reflectors to one strip of B-register band.
-// left: row operations on a column strip (rows), else column operations on a row strip.
-// get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid range.
+// Apply the steps [s0, s1) of a group of G consecutive bulges' reflectors to one strip through
+// a B-register band. left: row operations on a column strip (rows), else column operations on
+// a row strip. get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid
+// range. `lo` is the group's entry position (lo - 3 g0 for the group starting at bulge g0) and
+// `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
+// only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
+// another in bulge order: the result equals the step-by-step order of the chase.
 template <int M, bool LEFT, typename Get, typename Put>
 __device__ inline void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
                                   int vlo, int vhi, Get get, Put put) {
@@ -274,14 +285,47 @@ __device__ inline void band_apply(int lo, int s0, int s1, const Rot* rots, int r
   }
 }
 
+// Reflector of bulge j at chase position k (k == lo: the bulge enters with shift sh).
+// Writes beta / 0 into the window for k > lo. Inactive positions give the identity.
+__device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, int k, cd sh) {
+  Rot r;
+  r.tau = mkc(0.0, 0.0);
+  r.u = mkc(0.0, 0.0);
+  if (k < lo || k > hi - 1) return r;
+  cd x1, x2;
+  if (k == lo) {
+    x1 = csub(Hw[(k - a) * ldw + (k - a)], sh);
+    x2 = Hw[(k + 1 - a) * ldw + (k - a)];
+    const double sc = cabs1(x1) + cabs1(x2);
+    if (sc != 0.0) { x1 = cscale(x1, 1.0 / sc); x2 = cscale(x2, 1.0 / sc); }
+  } else {
+    x1 = Hw[(k - a) * ldw + (k - 1 - a)];
+    x2 = Hw[(k + 1 - a) * ldw + (k - 1 - a)];
+  }
+  double beta;
+  refl2(x1, x2, &r.tau, &r.u, &beta);
+  if (k > lo) {
+    Hw[(k - a) * ldw + (k - 1 - a)] = mkc(beta, 0.0);
+    Hw[(k + 1 - a) * ldw + (k - 1 - a)] = mkc(0.0, 0.0);
+  }
+  return r;
+}
+
 // One multishift sweep over the active block [lo, hi] with shifts sh[0..M).
+// Per chase step two barriers: row operations of all bulges, then their column operations,
+// where the thread owning bulge j also finishes rows k+1, k+2 of its column operation and
+// forms bulge j's reflector for the next step from them (the only inputs it needs). The
+// current and next steps' reflectors live in an LDS ring; every step's reflectors are also
+// logged to rot_all for the deferred updates outside the window and of Z.
 template <int M>
 __device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* sh, int W,
-                                 cd* Hw, Rot* rot_win, Rot* rot_all) {
+                                 cd* Hw, Rot (*s_rot)[M], Rot* rot_win, Rot* rot_all) {
+  static_assert(M % BG == 0, "bulge count must be a multiple of the band group");
   const int t = threadIdx.x;
   const int T = (hi - 1 - lo) + 3 * (M - 1);       // last step
   const int ldw = W + 1;
   int s_begin = 0;
+  bool first = true;
   while (s_begin <= T) {
     int kmin = lo + s_begin - 3 * (M - 1);
     if (kmin < lo) kmin = lo;
@@ -292,60 +336,55 @@ __device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* 
     if (wend == hi + 1) s_end = T + 1;
     else s_end = wend - 3 - lo + 1;
     if (s_end > T + 1) s_end = T + 1;
-    // load window
     for (int e = t; e < ww * ww; e += NT) {
       const int r = e / ww, c = e % ww;
       Hw[r * ldw + c] = H[(long long)(a + r) * n + a + c];
     }
+    if (!first && t < M) rot_win[t] = s_rot[s_begin & 1][t];
     __syncthreads();
-    for (int s = s_begin; s < s_end; ++s) {
-      // reflectors
+    if (first) {
+      // reflectors of step 0 (only bulge 0 can be active)
       if (t < M) {
-        const int j = t;
-        const int k = lo + s - 3 * j;
-        Rot r;
-        r.tau = mkc(0.0, 0.0);
-        r.u = mkc(0.0, 0.0);
-        if (k >= lo && k <= hi - 1) {
-          cd x1, x2;
-          if (k == lo) {
-            x1 = csub(Hw[(k - a) * ldw + (k - a)], sh[j]);
-            x2 = Hw[(k + 1 - a) * ldw + (k - a)];
-            const double sc = cabs1(x1) + cabs1(x2);
-            if (sc != 0.0) { x1 = cscale(x1, 1.0 / sc); x2 = cscale(x2, 1.0 / sc); }
-          } else {
-            x1 = Hw[(k - a) * ldw + (k - 1 - a)];
-            x2 = Hw[(k + 1 - a) * ldw + (k - 1 - a)];
-          }
-          double beta;
-          refl2(x1, x2, &r.tau, &r.u, &beta);
-          if (k > lo) {
-            Hw[(k - a) * ldw + (k - 1 - a)] = mkc(beta, 0.0);
-            Hw[(k + 1 - a) * ldw + (k - 1 - a)] = mkc(0.0, 0.0);
-          }
-        }
-        rot_win[(s - s_begin) * M + j] = r;
-        rot_all[(long long)s * M + j] = r;
+        const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, lo + s_begin - 3 * t, sh[t]);
+        s_rot[s_begin & 1][t] = r;
+        rot_win[t] = r;
+        rot_all[(long long)s_begin * M + t] = r;
       }
+      first = false;
       __syncthreads();
+    }
+    for (int s = s_begin; s < s_end; ++s) {
+      const Rot* rs = s_rot[s & 1];
       // row operations: bulge j on rows k, k+1, columns k..wend-1
       for (int e = t; e < M * ww; e += NT) {
         const int j = e / ww, c = a + e % ww;
         const int k = lo + s - 3 * j;
         if (k < lo || k > hi - 1 || c < k) continue;
-        const Rot r = rot_win[(s - s_begin) * M + j];
-        rot_left(r, Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
+        rot_left(rs[j], Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
       }
       __syncthreads();
-      // column operations: bulge j on columns k, k+1, rows a..min(k+2, hi)
+      // column operations: bulge j on columns k, k+1, rows a..min(k+2, hi); rows k+1, k+2
+      // and the next reflector by thread j
       for (int e = t; e < M * ww; e += NT) {
         const int j = e / ww, q = a + e % ww;
         const int k = lo + s - 3 * j;
         if (k < lo || k > hi - 1) continue;
-        const int rmax = (k + 2 < hi) ? k + 2 : hi;
-        if (q > rmax) continue;
-        const Rot r = rot_win[(s - s_begin) * M + j];
-        rot_right(r, Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+        if (q > k || q > hi) continue;
+        rot_right(rs[j], Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+      }
+      if (t < M) {
+        const int j = t;
+        const int k = lo + s - 3 * j;
+        if (k >= lo && k <= hi - 1) {
+          for (int q = k + 1; q <= k + 2 && q <= hi; ++q)
+            rot_right(rs[j], Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+        }
+        if (s + 1 <= T) {
+          const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, k + 1, sh[j]);
+          s_rot[(s + 1) & 1][j] = r;
+          if (s + 1 < s_end) rot_win[(s + 1 - s_begin) * M + j] = r;
+          rot_all[(long long)(s + 1) * M + j] = r;
+        }
       }
       __syncthreads();
     }
@@ -354,26 +393,39 @@ __device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* 
       const int r = e / ww, c = e % ww;
       H[(long long)(a + r) * n + a + c] = Hw[r * ldw + c];
     }
+    const Rot* rw = rot_win;
     // deferred row operations on the columns right of the window (rows a..wend-1)
     for (int c = wend + t; c < n; c += NT) {
-      band_apply<M, true>(lo, s_begin, s_end, rot_win, M, a, wend,
-                          [&](int i) { return H[(long long)i * n + c]; },
-                          [&](int i, cd v) { H[(long long)i * n + c] = v; });
+      for (int g0 = 0; g0 < M; g0 += BG)
+        band_apply<BG, true>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
+                             [&](int i) { return H[(long long)i * n + c]; },
+                             [&](int i, cd v) { H[(long long)i * n + c] = v; });
     }
     // deferred column operations on the rows above the window (columns a..wend-1)
     for (int q = t; q < a; q += NT) {
       cd* row = H + (long long)q * n;
-      band_apply<M, false>(lo, s_begin, s_end, rot_win, M, a, wend,
-                           [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+      for (int g0 = 0; g0 < M; g0 += BG)
+        band_apply<BG, false>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
+                              [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
     }
     __syncthreads();
     s_begin = s_end;
   }
-  // Schur vectors: every row of Z gets the whole sweep's column operations
-  for (int q = t; q < n; q += NT) {
-    cd* row = Z + (long long)q * n;
-    band_apply<M, false>(lo, 0, T + 1, rot_all, M, lo, hi + 1,
-                         [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+  // Schur vectors: every row of Z gets the whole sweep's column operations, the sweep's
+  // reflectors staged through LDS (the window area) in chunks of steps
+  Rot* rz = (Rot*)Hw;
+  const int chunk = (int)((sizeof(cd) * (size_t)W * (W + 1)) / (sizeof(Rot) * M));
+  for (int c0 = 0; c0 <= T; c0 += chunk) {
+    const int c1 = (c0 + chunk < T + 1) ? c0 + chunk : T + 1;
+    __syncthreads();
+    for (int e = t; e < (c1 - c0) * M; e += NT) rz[e] = rot_all[(long long)c0 * M + e];
+    __syncthreads();
+    for (int q = t; q < n; q += NT) {
+      cd* row = Z + (long long)q * n;
+      for (int g0 = 0; g0 < M; g0 += BG)
+        band_apply<BG, false>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
+                              [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+    }
   }
   __syncthreads();
 }
@@ -389,10 +441,10 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, in
   extern __shared__ unsigned char smem[];
   cd* Hw = (cd*)smem;
   Rot* rot_win = (Rot*)(smem + sizeof(cd) * (size_t)W * (W + 1));
+  __shared__ Rot s_rot[2][M];
   __shared__ int s_k[NT];
   __shared__ cd s_sh[M];
   __shared__ cd s_small[M * M];
-  __shared__ int s_flag;
   const int t = threadIdx.x;
   int hi = n - 1;
   int lo = 0;
@@ -439,40 +491,40 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, in
           s_sh[j] = cadd(H[(long long)(hi - (j % 2)) * n + hi - (j % 2)],
                          mkc(ex * (1.0 + 0.1 * j), 0.0));
       }
-      s_flag = 0;
     }
     __syncthreads();
-    multishift_sweep<M>(H, Z, n, lo, hi, s_sh, W, Hw, rot_win, rot_all);
+    multishift_sweep<M>(H, Z, n, lo, hi, s_sh, W, Hw, s_rot, rot_win, rot_all);
   }
 }
 
 }  // namespace
 
 size_t wb_hqr_ms_rot_bytes(int n, int batch) {
-  return sizeof(Rot) * (size_t)(n + 3 * 8) * 8 * (size_t)batch;
+  return sizeof(Rot) * (size_t)(n + 3 * WB_HQR_M) * WB_HQR_M * (size_t)batch;
 }
 
 // Multishift QR: H (Hessenberg, below-subdiagonal zero), Z (Q of the reduction) -> Schur T in H,
 // Schur vectors in Z, eigenvalues in w.
 hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws, int batch,
                             hipStream_t stream) {
-  constexpr int M = 8;
+  constexpr int M = WB_HQR_M;
   static int W = 0;
   if (W == 0) {
     int dev = 0, smem = 65536;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    const int cands[] = {64, 56, 48, 40, 32};
-    for (int c : cands) {
-      const size_t need = sizeof(cd) * (size_t)c * (c + 1) + sizeof(Rot) * (size_t)(c + 3 * M) * M +
-                          4096 + sizeof(int) * NT + sizeof(cd) * (M + M * M);
-      if (need <= (size_t)smem) { W = c; break; }
+    // dynamic LDS: the window and its reflectors (a window holds at most W + 3M steps); the
+    // rest is the kernel's static LDS
+    const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * NT + sizeof(cd) * (M + M * M) + 2048;
+    for (int c = 128; c >= 32; c -= 8) {
+      if (c < 3 * M + 16) break;
+      const size_t need = sizeof(cd) * (size_t)c * (c + 1) + sizeof(Rot) * (size_t)(c + 3 * M) * M;
+      if (need + fixed <= (size_t)smem) { W = c; break; }
     }
-    if (W == 0) W = 32;
+    if (W == 0) W = 3 * M + 16;
   }
-  // window + the window's reflectors (a window can hold up to W + 3M - 4 steps)
   const size_t sh = sizeof(cd) * (size_t)W * (W + 1) + sizeof(Rot) * (size_t)(W + 3 * M) * M;
-  const int nmin = (W < 48) ? W : 48;
+  const int nmin = (3 * M + 8 > 48) ? 3 * M + 8 : 48;
   WB_TIMED_F(WB_T_HQR, stream, 72.0 * (double)n * n * n * batch,
              hipLaunchKernelGGL(k_hqr_ms<M>, dim3(batch), dim3(NT), sh, stream, n, H, Z, w, info,
                                 (Rot*)rot_ws, W, nmin));
