@@ -357,6 +357,58 @@ __global__ __launch_bounds__(NT) void k_trevc(int n, const cd* Tb, cd* Xb) {
   }
 }
 
+// Blocked eigenvectors of the triangular Schur factor. Column k of X solves
+// (T - T_kk I) x = 0 with x_k = 1 and x_j = 0 for j > k. Row blocks of TB rows are solved from
+// the bottom: the coupling of block I to the rows already solved, S = T[I, i1:] X[i1:, i1:], is
+// one batched GEMM on the matrix cores; k_trevc_block then finishes each column's TB x TB
+// shifted triangular solve in registers (thread per column, T_II in LDS). Divisors smaller
+// than max(ulp |T_kk|, smlnum) are replaced by that bound, as in LAPACK's ztrevc.
+constexpr int TB = 32;
+
+__global__ __launch_bounds__(256) void k_trevc_block(int n, int i0, const cd* Tb, cd* Xb,
+                                                     const cd* Sb, long long s_stride) {
+  const long long b = blockIdx.y;
+  const cd* T = Tb + b * (long long)n * n;
+  cd* X = Xb + b * (long long)n * n;
+  const cd* S = Sb + b * s_stride;                  // TB x (n - i1), ld n
+  const int i1 = (i0 + TB < n) ? i0 + TB : n;
+  const int nbk = i1 - i0;
+  __shared__ cd Tii[TB][TB + 1];
+  for (int e = threadIdx.x; e < TB * TB; e += 256) {
+    const int r = e / TB, c = e % TB;
+    Tii[r][c] = (r < nbk && c < nbk) ? T[(long long)(i0 + r) * n + i0 + c] : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+  const int k = i0 + blockIdx.x * 256 + threadIdx.x;   // columns k >= i0
+  if (k >= n) return;
+  const cd lam = T[(long long)k * n + k];
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  const double smin = fmax(ULP * cabs1(lam), smlnum);
+  cd x[TB];
+#pragma unroll
+  for (int jj = TB - 1; jj >= 0; --jj) {
+    const int j = i0 + jj;
+    cd val = mkc(0.0, 0.0);
+    if (jj < nbk && j <= k) {
+      if (j == k) {
+        val = mkc(1.0, 0.0);
+      } else {
+        cd acc = (k >= i1) ? S[(long long)jj * n + (k - i1)] : mkc(0.0, 0.0);
+#pragma unroll
+        for (int ll = jj + 1; ll < TB; ++ll)
+          if (ll < nbk && i0 + ll <= k) acc = cadd(acc, cmul(Tii[jj][ll], x[ll]));
+        cd d = csub(Tii[jj][jj], lam);
+        if (cabs1(d) < smin) d = mkc(smin, 0.0);
+        val = cneg(cdiv(acc, d));
+      }
+    }
+    x[jj] = val;
+  }
+#pragma unroll
+  for (int jj = 0; jj < TB; ++jj)
+    if (jj < nbk && i0 + jj <= k) X[(long long)(i0 + jj) * n + k] = x[jj];
+}
+
 // Unit 2-norm columns with the largest-modulus component made real and positive.
 __global__ __launch_bounds__(NT) void k_colnorm(int n, cd* Wb) {
   const long long b = blockIdx.x;
@@ -471,6 +523,31 @@ static hipError_t wb_unghr_blocked(int n, cd* V, cd* X, cd* Q, int batch, hipStr
   return hipSuccess;
 }
 
+// Blocked triangular eigenvectors X of the Schur factor T (both n x n per matrix); S (n x n per
+// matrix, TB rows used) is scratch for the block couplings.
+static hipError_t wb_trevc_blocked(int n, const cd* T, cd* X, cd* S, int batch, hipStream_t stream) {
+  const double n3 = (double)n * n * n;
+  WbStageTimer stage(WB_T_TREVC, stream, 4.0 / 3.0 * n3 * batch);
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(k_identity, dim3((unsigned)((nn + 255) / 256), batch), dim3(256), 0, stream, n, X);
+  WB_CHECK_LAUNCH();
+  for (int i0 = ((n - 1) / TB) * TB; i0 >= 0; i0 -= TB) {
+    const int i1 = (i0 + TB < n) ? i0 + TB : n;
+    if (i1 < n) {
+      hipError_t e = wb_zgemm_ops_launch(0, 0, i1 - i0, n - i1, n - i1, mkc(1.0, 0.0),
+                                         bp_strided(T + (long long)i0 * n + i1, nn), n,
+                                         bp_strided(X + (long long)i1 * n + i1, nn), n, mkc(0.0, 0.0),
+                                         bp_strided(S, nn), n, batch, stream);
+      if (e != hipSuccess) return e;
+    }
+    const int cols = n - i0;
+    hipLaunchKernelGGL(k_trevc_block, dim3((cols + 255) / 256, batch), dim3(256), 0, stream, n, i0, T,
+                       X, S, nn);
+    WB_CHECK_LAUNCH();
+  }
+  return hipSuccess;
+}
+
 // A (batch of n x n) is destroyed. Outputs: w (batch x n) eigenvalues, W (batch x n x n)
 // eigenvectors in columns. Workspace: Z (batch x n x n), X (batch x n x n), tau (batch x n).
 hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
@@ -500,8 +577,13 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, vo
     hipError_t e2 = wb_hqr_ms_launch(n, A, Z, w, info, rot_ws, batch, stream);
     if (e2 != hipSuccess) return e2;
   }
-  WB_TIMED_F(WB_T_TREVC, stream, 4.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(NT), 0, stream, n, A, X));
-  WB_CHECK_LAUNCH();
+  if (n >= WB_BLOCKED_HESSENBERG_MIN) {
+    hipError_t e1 = wb_trevc_blocked(n, A, X, W, batch, stream);
+    if (e1 != hipSuccess) return e1;
+  } else {
+    WB_TIMED_F(WB_T_TREVC, stream, 4.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(NT), 0, stream, n, A, X));
+    WB_CHECK_LAUNCH();
+  }
   hipError_t e = wb_zgemm_launch(n, n, n, mkc(1.0, 0.0), bp_strided(Z, (long long)n * n), n,
                                  bp_strided(X, (long long)n * n), n, mkc(0.0, 0.0),
                                  bp_strided(W, (long long)n * n), n, batch, stream);

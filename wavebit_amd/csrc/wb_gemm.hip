@@ -6,7 +6,8 @@
 // A complex product is four real v_mfma_f64_16x16x4f64 streams (re*re, -im*im, re*im, im*re)
 // into two accumulators. A 256-thread workgroup owns a 64 x 64 tile of C, its four waves a
 // 32 x 32 quadrant each (2 x 2 MFMA blocks); K is staged through LDS 16 at a time, split into
-// real and imaginary planes so every MFMA operand is one 8-byte LDS read.
+// real and imaginary planes so every MFMA operand is one 8-byte LDS read, in two LDS stages: the
+// next K tile is fetched into registers while the current one feeds the matrix cores.
 //
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][kk] in lane i + 16*kk, B[kk][j] in
 // lane j + 16*kk, D[i][j] in lane j + 16*(i%4), register i/4.
@@ -30,8 +31,10 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
 
-  __shared__ double As_r[BK][BM + PAD], As_i[BK][BM + PAD];
-  __shared__ double Bs_r[BK][BN + PAD], Bs_i[BK][BN + PAD];
+  // two LDS stages: the next K tile is fetched into registers while the current one feeds the
+  // matrix cores, then written to the other stage; one barrier per K tile
+  __shared__ double As_r[2][BK][BM + PAD], As_i[2][BK][BM + PAD];
+  __shared__ double Bs_r[2][BK][BN + PAD], Bs_i[2][BK][BN + PAD];
 
   v4d acc_r[2][2], acc_i[2][2];
 #pragma unroll
@@ -39,66 +42,65 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
 #pragma unroll
     for (int j = 0; j < 2; ++j) { acc_r[i][j] = (v4d){0, 0, 0, 0}; acc_i[i][j] = (v4d){0, 0, 0, 0}; }
 
-  // global -> LDS assignment
-  const int ar = t >> 2, ak = (t & 3) * 4;          // A: row ar, k ak..ak+3
-  const int bk = t >> 4, bn = (t & 15) * 4;         // B: k row bk, cols bn..bn+3
+  // global -> register assignment (4 elements of A and 4 of B per thread per K tile)
+  const int ar = t >> 2, ak = (t & 3) * 4;          // A as stored: row ar, k ak..ak+3
+  const int ak2 = t >> 4, ar2 = (t & 15) * 4;       // A^H: stored row (k) ak2, m ar2..ar2+3
+  const int bk = t >> 4, bn = (t & 15) * 4;         // B as stored: k row bk, cols bn..bn+3
+  const int bn2 = t >> 2, bk2 = (t & 3) * 4;        // B^H: stored row (n) bn2, k bk2..bk2+3
+  cd ra[4], rb[4];
 
-  for (int k0 = 0; k0 < k; k0 += BK) {
-    if (!CA) {
-      const int row = tm + ar;
+  auto fetch = [&](int k0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k0 + ak + q;
-        cd v = mkc(0.0, 0.0);
+    for (int q = 0; q < 4; ++q) {
+      cd v = mkc(0.0, 0.0);
+      if (!CA) {
+        const int row = tm + ar, kk = k0 + ak + q;
         if (row < m && kk < k) v = a[(long long)row * lda + kk];
-        As_r[ak + q][ar] = v.x;
-        As_i[ak + q][ar] = v.y;
-      }
-    } else {
-      // stored k x m: thread reads 4 consecutive m of stored row kk
-      const int kk = k0 + (t >> 4), r0 = (t & 15) * 4;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = tm + r0 + q;
-        cd v = mkc(0.0, 0.0);
+      } else {
+        const int kk = k0 + ak2, row = tm + ar2 + q;
         if (row < m && kk < k) v = a[(long long)kk * lda + row];
-        As_r[t >> 4][r0 + q] = v.x;
-        As_i[t >> 4][r0 + q] = -v.y;
       }
+      ra[q] = v;
+      cd w = mkc(0.0, 0.0);
+      if (!CB) {
+        const int kk = k0 + bk, col = tn + bn + q;
+        if (kk < k && col < n) w = bb[(long long)kk * ldb + col];
+      } else {
+        const int col = tn + bn2, kk = k0 + bk2 + q;
+        if (kk < k && col < n) w = bb[(long long)col * ldb + kk];
+      }
+      rb[q] = w;
     }
-    if (!CB) {
-      const int kk = k0 + bk;
+  };
+  auto stash = [&](int st) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int col = tn + bn + q;
-        cd v = mkc(0.0, 0.0);
-        if (kk < k && col < n) v = bb[(long long)kk * ldb + col];
-        Bs_r[bk][bn + q] = v.x;
-        Bs_i[bk][bn + q] = v.y;
-      }
-    } else {
-      // stored n x k: thread reads 4 consecutive k of stored row col
-      const int col = tn + (t >> 2), k1 = (t & 3) * 4;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int kk = k0 + k1 + q;
-        cd v = mkc(0.0, 0.0);
-        if (kk < k && col < n) v = bb[(long long)col * ldb + kk];
-        Bs_r[k1 + q][t >> 2] = v.x;
-        Bs_i[k1 + q][t >> 2] = -v.y;
-      }
+    for (int q = 0; q < 4; ++q) {
+      if (!CA) { As_r[st][ak + q][ar] = ra[q].x; As_i[st][ak + q][ar] = ra[q].y; }
+      else { As_r[st][ak2][ar2 + q] = ra[q].x; As_i[st][ak2][ar2 + q] = -ra[q].y; }
+      if (!CB) { Bs_r[st][bk][bn + q] = rb[q].x; Bs_i[st][bk][bn + q] = rb[q].y; }
+      else { Bs_r[st][bk2 + q][bn2] = rb[q].x; Bs_i[st][bk2 + q][bn2] = -rb[q].y; }
     }
-    __syncthreads();
+  };
+
+  const int nk = (k + BK - 1) / BK;
+  if (nk > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int st = kt & 1;
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
       const int kr = s * 4 + (lane >> 4);
       double ar_[2], ai_[2], br_[2], bi_[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        ar_[i] = As_r[kr][wm + i * 16 + (lane & 15)];
-        ai_[i] = As_i[kr][wm + i * 16 + (lane & 15)];
-        br_[i] = Bs_r[kr][wn + i * 16 + (lane & 15)];
-        bi_[i] = Bs_i[kr][wn + i * 16 + (lane & 15)];
+        ar_[i] = As_r[st][kr][wm + i * 16 + (lane & 15)];
+        ai_[i] = As_i[st][kr][wm + i * 16 + (lane & 15)];
+        br_[i] = Bs_r[st][kr][wn + i * 16 + (lane & 15)];
+        bi_[i] = Bs_i[st][kr][wn + i * 16 + (lane & 15)];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
           acc_i[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai_[i], br_[j], acc_i[i][j], 0, 0, 0);
         }
     }
+    if (kt + 1 < nk) stash(st ^ 1);
     __syncthreads();
   }
 
