@@ -491,20 +491,21 @@ static hipError_t wb_gehrd_blocked(int n, cd* A, cd* V, cd* X, int batch, hipStr
   return hipSuccess;
 }
 
-// Q = (I - V_0 T_0 V_0^H) (I - V_1 T_1 V_1^H) ... from the blocked reduction's V and T,
-// accumulated backward on the trailing block Q[k0+1:, k0+1:] of each panel.
-static hipError_t wb_unghr_blocked(int n, cd* V, cd* X, cd* Q, int batch, hipStream_t stream) {
+// Q^H for Q = (I - V_0 T_0 V_0^H) (I - V_1 T_1 V_1^H) ... from the blocked reduction's V and T:
+// N = I, then for the panels from the last to the first N := N (I - V_p T_p^H V_p^H) on the
+// trailing block N[k0+1:, k0+1:] (the only part of N that is not yet the identity).
+static hipError_t wb_unghr_blocked(int n, cd* V, cd* X, cd* Qh, int batch, hipStream_t stream) {
   const double n3 = (double)n * n * n;
   WbStageTimer stage(WB_T_UNGHR, stream, 16.0 / 3.0 * n3 * batch);
   const long long nn = (long long)n * n;
   const int npan = (n - 2 + HNB - 1) / HNB;
   cd* T = X + (long long)n * HNB;
-  cd* Wm = T + (long long)npan * HNB * HNB;
-  cd* Wm2 = Wm + (long long)HNB * n;
+  cd* Wm = T + (long long)npan * HNB * HNB;             // n x HNB
+  cd* Wm2 = Wm + (long long)HNB * n;                     // n x HNB
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
   auto S = [&](cd* p, long long stride) { return bp_strided(p, stride); };
   hipError_t e;
-  hipLaunchKernelGGL(k_identity, dim3((unsigned)((nn + 255) / 256), batch), dim3(256), 0, stream, n, Q);
+  hipLaunchKernelGGL(k_identity, dim3((unsigned)((nn + 255) / 256), batch), dim3(256), 0, stream, n, Qh);
   WB_CHECK_LAUNCH();
   for (int p = npan - 1; p >= 0; --p) {
     const int k0 = p * HNB;
@@ -512,12 +513,12 @@ static hipError_t wb_unghr_blocked(int n, cd* V, cd* X, cd* Q, int batch, hipStr
     const int m1 = n - k0 - 1;
     cd* Tp = T + (long long)p * HNB * HNB;
     cd* Vp = V + (long long)(k0 + 1) * n + k0;
-    cd* Qs = Q + (long long)(k0 + 1) * n + k0 + 1;
-    if ((e = wb_zgemm_ops_launch(1, 0, ib, m1, m1, one, S(Vp, nn), n, S(Qs, nn), n, zero, S(Wm, nn), n,
+    cd* Ns = Qh + (long long)(k0 + 1) * n + k0 + 1;
+    if ((e = wb_zgemm_ops_launch(0, 0, m1, ib, m1, one, S(Ns, nn), n, S(Vp, nn), n, zero, S(Wm, nn), HNB,
                                  batch, stream))) return e;
-    if ((e = wb_zgemm_ops_launch(0, 0, ib, m1, ib, one, S(Tp, nn), HNB, S(Wm, nn), n, zero, S(Wm2, nn),
-                                 n, batch, stream))) return e;
-    if ((e = wb_zgemm_ops_launch(0, 0, m1, m1, ib, mone, S(Vp, nn), n, S(Wm2, nn), n, one, S(Qs, nn), n,
+    if ((e = wb_zgemm_ops_launch(0, 1, m1, ib, ib, one, S(Wm, nn), HNB, S(Tp, nn), HNB, zero, S(Wm2, nn),
+                                 HNB, batch, stream))) return e;
+    if ((e = wb_zgemm_ops_launch(0, 1, m1, m1, ib, mone, S(Wm2, nn), HNB, S(Vp, nn), n, one, S(Ns, nn), n,
                                  batch, stream))) return e;
   }
   return hipSuccess;
@@ -560,8 +561,9 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, vo
   const double n3 = (double)n * n * n;
   hipError_t z = wb_zero_int_launch(info, batch, stream);
   if (z != hipSuccess) return z;
-  if (n >= WB_BLOCKED_HESSENBERG_MIN) {
-    // Blocked: V (dense reflectors) in W, Y / T / GEMM scratch in X, Q formed in Z.
+  const bool blocked = n >= WB_BLOCKED_HESSENBERG_MIN;
+  if (blocked) {
+    // Blocked: V (dense reflectors) in W, Y / T / GEMM scratch in X, Q^H formed in Z.
     hipError_t e = wb_gehrd_blocked(n, A, W, X, batch, stream);
     if (e != hipSuccess) return e;
     e = wb_unghr_blocked(n, W, X, Z, batch, stream);
@@ -574,19 +576,20 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, vo
     WB_CHECK_LAUNCH();
   }
   {
-    hipError_t e2 = wb_hqr_ms_launch(n, A, Z, w, info, rot_ws, batch, stream);
+    hipError_t e2 = wb_hqr_ms_launch(n, A, Z, blocked ? 1 : 0, w, info, rot_ws, batch, stream);
     if (e2 != hipSuccess) return e2;
   }
-  if (n >= WB_BLOCKED_HESSENBERG_MIN) {
+  if (blocked) {
     hipError_t e1 = wb_trevc_blocked(n, A, X, W, batch, stream);
     if (e1 != hipSuccess) return e1;
   } else {
     WB_TIMED_F(WB_T_TREVC, stream, 4.0 / 3.0 * n3 * batch, hipLaunchKernelGGL(k_trevc, dim3(batch), dim3(NT), 0, stream, n, A, X));
     WB_CHECK_LAUNCH();
   }
-  hipError_t e = wb_zgemm_launch(n, n, n, mkc(1.0, 0.0), bp_strided(Z, (long long)n * n), n,
-                                 bp_strided(X, (long long)n * n), n, mkc(0.0, 0.0),
-                                 bp_strided(W, (long long)n * n), n, batch, stream);
+  // W = Z X (the QR keeps the Schur vectors as Z^H on the blocked path)
+  hipError_t e = wb_zgemm_ops_launch(blocked ? 1 : 0, 0, n, n, n, mkc(1.0, 0.0),
+                                     bp_strided(Z, (long long)n * n), n, bp_strided(X, (long long)n * n),
+                                     n, mkc(0.0, 0.0), bp_strided(W, (long long)n * n), n, batch, stream);
   if (e != hipSuccess) return e;
   WB_TIMED(WB_T_COLNORM, stream, hipLaunchKernelGGL(k_colnorm, dim3(batch), dim3(NT), 0, stream, n, W));
   return hipGetLastError();

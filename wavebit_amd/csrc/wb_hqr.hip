@@ -18,6 +18,7 @@
 #include "wb_internal.h"
 #include "wb_timing.h"
 #include <float.h>
+#include <type_traits>
 
 namespace {
 
@@ -29,6 +30,10 @@ namespace {
 #endif
 constexpr int NT = WB_HQR_NT;
 constexpr int BG = (WB_HQR_M < 8) ? WB_HQR_M : 8;   // bulges per register band of the deferred updates
+#ifndef WB_HQR_STRIPS
+#define WB_HQR_STRIPS 1   // strips per thread in the deferred updates
+#endif
+constexpr int HS = WB_HQR_STRIPS;
 constexpr double ULP = DBL_EPSILON;
 constexpr double SAFMIN = DBL_MIN;
 
@@ -163,7 +168,7 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
 
 // Single-shift iteration on the active block [lo_lim, hi] until all its eigenvalues converge
 // (the zlahqr scheme, rotations applied straight to global H and Z). Returns 0 on success.
-__device__ int single_shift_block(cd* H, cd* Z, cd* wout, int n, int lo_lim, int hi, int* s_k) {
+__device__ int single_shift_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo_lim, int hi, int* s_k) {
   const int t = threadIdx.x;
 #define HH(r, c) H[(long long)(r) * n + (c)]
   int i = hi;
@@ -223,9 +228,14 @@ __device__ int single_shift_block(cd* H, cd* Z, cd* wout, int n, int lo_lim, int
           __syncthreads();
           const int rmax = (k + 2 < i) ? k + 2 : i;
           for (int q = t; q <= rmax; q += NT) rot_right(r, HH(q, k), HH(q, k + 1));
-          for (int q = t; q < n; q += NT) {
-            cd* zr = Z + (long long)q * n + k;
-            rot_right(r, zr[0], zr[1]);
+          if (zh) {
+            for (int q = t; q < n; q += NT)
+              rot_left(r, Z[(long long)k * n + q], Z[(long long)(k + 1) * n + q]);
+          } else {
+            for (int q = t; q < n; q += NT) {
+              cd* zr = Z + (long long)q * n + k;
+              rot_right(r, zr[0], zr[1]);
+            }
           }
         }
         __syncthreads();
@@ -248,41 +258,65 @@ __device__ int single_shift_block(cd* H, cd* Z, cd* wout, int n, int lo_lim, int
 // `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
 // only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
 // another in bulge order: the result equals the step-by-step order of the chase.
-template <int M, bool LEFT, typename Get, typename Put>
+template <int M, bool LEFT, int S, typename Get, typename Put>
 __device__ inline void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
                                   int vlo, int vhi, Get get, Put put) {
+  // S independent strips share each step's reflectors and interleave their dependency chains
+  // (one wave per SIMD leaves nothing else to hide FP64 latency behind).
   constexpr int B = 3 * (M - 1) + 2;
-  cd band[B];
+  cd band[S][B];
   const int base0 = lo + s0 - 3 * (M - 1);
 #pragma unroll
   for (int i = 0; i < B - 1; ++i) {
     const int idx = base0 + i;
-    band[i] = (idx >= vlo && idx < vhi) ? get(idx) : mkc(0.0, 0.0);
+    const bool ok = idx >= vlo && idx < vhi;
+#pragma unroll
+    for (int k = 0; k < S; ++k) band[k][i] = ok ? get(k, idx) : mkc(0.0, 0.0);
   }
   for (int s = s0; s < s1; ++s) {
     const int base = lo + s - 3 * (M - 1);
     const int top = base + B - 1;
-    band[B - 1] = (top >= vlo && top < vhi) ? get(top) : mkc(0.0, 0.0);
+    const bool ok_top = top >= vlo && top < vhi;
+#pragma unroll
+    for (int k = 0; k < S; ++k) band[k][B - 1] = ok_top ? get(k, top) : mkc(0.0, 0.0);
     const Rot* rs = rots + (long long)(s - s0) * rot_stride;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
       const Rot r = rs[j];
-      constexpr int dummy = 0;
-      (void)dummy;
       const int slot = 3 * (M - 1 - j);
-      if (LEFT) rot_left(r, band[slot], band[slot + 1]);
-      else rot_right(r, band[slot], band[slot + 1]);
-    }
-    if (base >= vlo && base < vhi) put(base, band[0]);
 #pragma unroll
-    for (int i = 0; i < B - 1; ++i) band[i] = band[i + 1];
+      for (int k = 0; k < S; ++k) {
+        if (LEFT) rot_left(r, band[k][slot], band[k][slot + 1]);
+        else rot_right(r, band[k][slot], band[k][slot + 1]);
+      }
+    }
+    if (base >= vlo && base < vhi) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) put(k, base, band[k][0]);
+    }
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i)
+#pragma unroll
+      for (int k = 0; k < S; ++k) band[k][i] = band[k][i + 1];
   }
   const int base_end = lo + s1 - 3 * (M - 1);
 #pragma unroll
   for (int i = 0; i < B - 1; ++i) {
     const int idx = base_end + i;
-    if (idx >= vlo && idx < vhi) put(idx, band[i]);
+    if (idx >= vlo && idx < vhi) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) put(k, idx, band[k][i]);
+    }
   }
+}
+
+// Calls f(x, S) for this thread's strips x = first, first + NT, ... < end: SB strips at a time
+// (x, x + NT, ..., x + (SB-1) NT), then the remainder one at a time.
+template <int SB, typename F>
+__device__ inline void for_strips(int first, int end, F f) {
+  int x = first;
+  for (; x + (SB - 1) * NT < end; x += SB * NT) f(x, std::integral_constant<int, SB>());
+  for (; x < end; x += NT) f(x, std::integral_constant<int, 1>());
 }
 
 // Reflector of bulge j at chase position k (k == lo: the bulge enters with shift sh).
@@ -318,7 +352,7 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
 // current and next steps' reflectors live in an LDS ring; every step's reflectors are also
 // logged to rot_all for the deferred updates outside the window and of Z.
 template <int M>
-__device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* sh, int W,
+__device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, const cd* sh, int W,
                                  cd* Hw, Rot (*s_rot)[M], Rot* rot_win, Rot* rot_all) {
   static_assert(M % BG == 0, "bulge count must be a multiple of the band group");
   const int t = threadIdx.x;
@@ -395,19 +429,21 @@ __device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* 
     }
     const Rot* rw = rot_win;
     // deferred row operations on the columns right of the window (rows a..wend-1)
-    for (int c = wend + t; c < n; c += NT) {
+    for_strips<HS>(wend + t, n, [&](int c, auto sc) {
+      constexpr int SS = decltype(sc)::value;
       for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, true>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
-                             [&](int i) { return H[(long long)i * n + c]; },
-                             [&](int i, cd v) { H[(long long)i * n + c] = v; });
-    }
+        band_apply<BG, true, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
+                                 [&](int k, int i) { return H[(long long)i * n + c + k * NT]; },
+                                 [&](int k, int i, cd v) { H[(long long)i * n + c + k * NT] = v; });
+    });
     // deferred column operations on the rows above the window (columns a..wend-1)
-    for (int q = t; q < a; q += NT) {
-      cd* row = H + (long long)q * n;
+    for_strips<HS>(t, a, [&](int q, auto sc) {
+      constexpr int SS = decltype(sc)::value;
       for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, false>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
-                              [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
-    }
+        band_apply<BG, false, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
+                                  [&](int k, int i) { return H[(long long)(q + k * NT) * n + i]; },
+                                  [&](int k, int i, cd v) { H[(long long)(q + k * NT) * n + i] = v; });
+    });
     __syncthreads();
     s_begin = s_end;
   }
@@ -420,19 +456,31 @@ __device__ void multishift_sweep(cd* H, cd* Z, int n, int lo, int hi, const cd* 
     __syncthreads();
     for (int e = t; e < (c1 - c0) * M; e += NT) rz[e] = rot_all[(long long)c0 * M + e];
     __syncthreads();
-    for (int q = t; q < n; q += NT) {
-      cd* row = Z + (long long)q * n;
-      for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, false>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
-                              [&](int i) { return row[i]; }, [&](int i, cd v) { row[i] = v; });
+    if (zh) {
+      // Z^H: row operations on its column strips, consecutive lanes on consecutive columns
+      for_strips<HS>(t, n, [&](int q, auto sc) {
+        constexpr int SS = decltype(sc)::value;
+        for (int g0 = 0; g0 < M; g0 += BG)
+          band_apply<BG, true, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
+                                   [&](int k, int i) { return Z[(long long)i * n + q + k * NT]; },
+                                   [&](int k, int i, cd v) { Z[(long long)i * n + q + k * NT] = v; });
+      });
+    } else {
+      for_strips<HS>(t, n, [&](int q, auto sc) {
+        constexpr int SS = decltype(sc)::value;
+        for (int g0 = 0; g0 < M; g0 += BG)
+          band_apply<BG, false, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
+                                    [&](int k, int i) { return Z[(long long)(q + k * NT) * n + i]; },
+                                    [&](int k, int i, cd v) { Z[(long long)(q + k * NT) * n + i] = v; });
+      });
     }
   }
   __syncthreads();
 }
 
 template <int M>
-__global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, int* info, Rot* rws,
-                                               int W, int nmin) {
+__global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd* wb, int* info,
+                                               Rot* rws, int W, int nmin) {
   const long long b = blockIdx.x;
   cd* H = Hb + b * (long long)n * n;
   cd* Z = Zb + b * (long long)n * n;
@@ -463,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, in
       continue;
     }
     if (hi - lo + 1 <= nmin) {
-      const int r = single_shift_block(H, Z, wout, n, lo, hi, s_k);
+      const int r = single_shift_block(H, Z, zh, wout, n, lo, hi, s_k);
       if (r != 0) {
         if (t == 0) info[b] = r;
         return;
@@ -493,7 +541,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, cd* wb, in
       }
     }
     __syncthreads();
-    multishift_sweep<M>(H, Z, n, lo, hi, s_sh, W, Hw, s_rot, rot_win, rot_all);
+    multishift_sweep<M>(H, Z, zh, n, lo, hi, s_sh, W, Hw, s_rot, rot_win, rot_all);
   }
 }
 
@@ -503,9 +551,9 @@ size_t wb_hqr_ms_rot_bytes(int n, int batch) {
   return sizeof(Rot) * (size_t)(n + 3 * WB_HQR_M) * WB_HQR_M * (size_t)batch;
 }
 
-// Multishift QR: H (Hessenberg, below-subdiagonal zero), Z (Q of the reduction) -> Schur T in H,
-// Schur vectors in Z, eigenvalues in w.
-hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws, int batch,
+// Multishift QR: H (Hessenberg, below-subdiagonal zero), Z (Q of the reduction, or Q^H when zh)
+// -> Schur T in H, Schur vectors in Z (as Z^H when zh), eigenvalues in w.
+hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void* rot_ws, int batch,
                             hipStream_t stream) {
   constexpr int M = WB_HQR_M;
   static int W = 0;
@@ -526,7 +574,7 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws,
   const size_t sh = sizeof(cd) * (size_t)W * (W + 1) + sizeof(Rot) * (size_t)(W + 3 * M) * M;
   const int nmin = (3 * M + 8 > 48) ? 3 * M + 8 : 48;
   WB_TIMED_F(WB_T_HQR, stream, 72.0 * (double)n * n * n * batch,
-             hipLaunchKernelGGL(k_hqr_ms<M>, dim3(batch), dim3(NT), sh, stream, n, H, Z, w, info,
+             hipLaunchKernelGGL(k_hqr_ms<M>, dim3(batch), dim3(NT), sh, stream, n, H, Z, zh, w, info,
                                 (Rot*)rot_ws, W, nmin));
   return hipGetLastError();
 }

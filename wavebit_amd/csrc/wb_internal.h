@@ -20,7 +20,8 @@ hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream);
 hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
                            int batch, hipStream_t stream);
 size_t wb_hqr_ms_rot_bytes(int n, int batch);
-hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, cd* w, int* info, void* rot_ws, int batch,
+// zh: Z is held conjugate-transposed (Z^H) when nonzero.
+hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void* rot_ws, int batch,
                             hipStream_t stream);
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
