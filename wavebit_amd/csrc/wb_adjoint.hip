@@ -336,6 +336,7 @@ struct AdjWs {
   cd *E3;
   cd *crb, *ctb, *db, *gub, *ub, *tv, *Zs;
   int *pivA, *pivM, *infoA, *infoM;
+  void* luws;
 };
 
 AdjWs adj_carve(void* ws, int P, int L, int n, int N, bool count_only, size_t* bytes) {
@@ -357,6 +358,7 @@ AdjWs adj_carve(void* ws, int P, int L, int n, int N, bool count_only, size_t* b
   a.pivM = (int*)take((size_t)P * n * sizeof(int));
   a.infoA = (int*)take((size_t)P * sizeof(int));
   a.infoM = (int*)take((size_t)P * sizeof(int));
+  a.luws = take(wb_getrs_ws_bytes(n, 2 * n, P));
   (void)count_only;
   if (bytes) *bytes = used + 256;
   return a;
@@ -468,7 +470,7 @@ hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st)
     if ((e = wb_zgemm_ops_launch(1, 0, n, n, n, one, S12, 2 * n, TT(T.Gb), n, zero, TT(T.Hb), n, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(0, 1, n, n, n, one, TT(T.Hb), n, Xs, n, zero, TT(T.Gn), n, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(1, 0, n, n, n, one, Gam, ldgam, TT(T.Hb), n, zero, TT(T.Xsb), n, P, st))) return e;
-    if ((e = wb_getrs_conj_launch(n, n, LUMx, n, piv, TT(T.Xsb), n, P, st))) return e;
+    if ((e = wb_getrs_blocked(n, n, LUMx, n, piv, TT(T.Xsb), n, P, 1, T.luws, st))) return e;
     hipLaunchKernelGGL(k_add, grid2(nn, P), dim3(256), 0, st, n, TT(T.Xsb), n, SBr, 2 * n);
     WB_CHECK_LAUNCH();
     if ((e = wb_zgemm_ops_launch(0, 1, n, n, n, mone, TT(T.Xsb), n, Xs, n, zero, TT(T.Mxb), n, P, st))) return e;
@@ -480,7 +482,7 @@ hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st)
     // ---- layer S-matrix: [S11-bar | S12-bar] -> A-bar, B-bar, X-bar
     hipLaunchKernelGGL(k_copy2, grid2(nn, P), dim3(256), 0, st, n, n, Am, n, TT(T.Atmp), n);
     WB_CHECK_LAUNCH();
-    if ((e = wb_inverse_launch(n, TT(T.Atmp), n, TT(T.Ai), n, T.pivA, T.infoA, P, st))) return e;
+    if ((e = wb_inverse_blocked(n, TT(T.Atmp), n, TT(T.Ai), n, T.pivA, T.infoA, P, T.luws, st))) return e;
     if ((e = wb_zgemm_ops_launch(0, 0, n, n, n, one, Bm, n, TT(T.Ai), n, zero, TT(T.Y), n, P, st))) return e;
     hipLaunchKernelGGL(k_copy2, grid2(nn, P), dim3(256), 0, st, n, n, Am, n, TT(T.Mlu), n);
     WB_CHECK_LAUNCH();
@@ -490,7 +492,7 @@ hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st)
     WB_CHECK_LAUNCH();
     if ((e = wb_zgemm_ops_launch(0, 0, n, n, n, mone, TT(T.Y), n, Bm, n, one, TT(T.F), n, P, st))) return e;
     // [E1-bar | E2-bar] = M^-H [S11-bar | S12-bar]
-    if ((e = wb_getrs_conj_launch(n, 2 * n, TT(T.Mlu), n, T.pivM, SB, 2 * n, P, st))) return e;
+    if ((e = wb_getrs_blocked(n, 2 * n, TT(T.Mlu), n, T.pivM, SB, 2 * n, P, 1, T.luws, st))) return e;
     // M-bar = -(E1b S11^H + E2b S12^H)
     if ((e = wb_zgemm_ops_launch(0, 1, n, n, n, mone, SB, 2 * n, S11, n, zero, TT(T.Mb), n, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(0, 1, n, n, n, mone, SBr, 2 * n, S12, 2 * n, one, TT(T.Mb), n, P, st))) return e;

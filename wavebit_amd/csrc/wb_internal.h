@@ -14,6 +14,12 @@ hipError_t wb_zgemm_ops_launch(int opa, int opb, int m, int n, int k, cd alpha, 
                                BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc, int batch,
                                hipStream_t stream);
 hipError_t wb_set_identity_launch(int n, BatchPtr B, int ldb, int batch, hipStream_t stream);
+// Many-right-hand-side solves blocked by 64 rows on GEMMs (scratch: wb_getrs_ws_bytes).
+size_t wb_getrs_ws_bytes(int n, int nrhs, int batch);
+hipError_t wb_getrs_blocked(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                            int ldb, int batch, int conj, void* ws, hipStream_t stream);
+hipError_t wb_inverse_blocked(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv, int* info,
+                              int batch, void* ws, hipStream_t stream);
 hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
                              int* info, int batch, hipStream_t stream);
 hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream);

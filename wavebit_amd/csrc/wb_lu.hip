@@ -328,6 +328,159 @@ hipError_t wb_getrs_conj_launch(int n, int nrhs, BatchPtr LU, int lda, const int
   return hipSuccess;
 }
 
+// ----------------------------------------------------------------------------- blocked solves
+// Solves with many right-hand sides run in blocks of LB = 64 rows: the 64 x 64 diagonal blocks
+// of L and U are inverted once (k_tri_inv, in LDS), so every step of both sweeps is a k = 64
+// GEMM on the matrix cores -- the diagonal block's solve (X_b = inv(T_bb) B_b, through a
+// scratch block) and the update of the rows still to be solved. A^H solves use the same
+// inverses conjugate-transposed.
+namespace {
+
+constexpr int LB = 64;
+
+// inv(unit lower L_bb) (kind 0) or inv(upper U_bb) (kind 1) of diagonal block bb, LB x LB
+// row-major per (matrix, block, kind); thread per column.
+__global__ __launch_bounds__(LB) void k_tri_inv(int n, BatchPtr LU, int lda, cd* inv, int nblk) {
+  const int bb = blockIdx.x, kind = blockIdx.z;
+  const long long b = blockIdx.y;
+  const cd* a = LU.ptr(b);
+  const int r0 = bb * LB;
+  const int jb = (n - r0 < LB) ? n - r0 : LB;
+  __shared__ cd T[LB][LB + 1];
+  __shared__ cd X[LB][LB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < LB * LB; e += LB) {
+    const int r = e / LB, c = e % LB;
+    T[r][c] = (r < jb && c < jb) ? a[(long long)(r0 + r) * lda + r0 + c] : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+  const int j = t;   // column of the inverse
+  if (kind == 0) {
+    for (int i = 0; i < LB; ++i) {
+      cd x = mkc(i == j ? 1.0 : 0.0, 0.0);
+      if (i < jb && j < jb) {
+        for (int l = j; l < i; ++l) x = csub(x, cmul(T[i][l], X[l][j]));
+      } else {
+        x = mkc(0.0, 0.0);
+      }
+      X[i][j] = x;
+    }
+  } else {
+    for (int i = LB - 1; i >= 0; --i) {
+      cd x = mkc(i == j ? 1.0 : 0.0, 0.0);
+      if (i < jb && j < jb) {
+        for (int l = i + 1; l <= j; ++l) x = csub(x, cmul(T[i][l], X[l][j]));
+        x = cdiv(x, T[i][i]);
+      } else {
+        x = mkc(0.0, 0.0);
+      }
+      X[i][j] = x;
+    }
+  }
+  __syncthreads();
+  cd* out = inv + ((b * nblk + bb) * 2 + kind) * (long long)LB * LB;
+  for (int e = t; e < LB * LB; e += LB) out[e] = X[e / LB][e % LB];
+}
+
+__global__ void k_copy_rows(int rows, int cols, const cd* src, long long s_stride, int lds, BatchPtr dst,
+                            int ldd) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e / cols), c = (int)(e % cols);
+  dst.ptr(b)[(long long)r * ldd + c] = src[b * s_stride + (long long)r * lds + c];
+}
+
+}  // namespace
+
+size_t wb_getrs_ws_bytes(int n, int nrhs, int batch) {
+  const size_t nblk = (size_t)(n + LB - 1) / LB;
+  return sizeof(cd) * (size_t)batch * (nblk * 2 * LB * LB + (size_t)LB * nrhs) + 256;
+}
+
+// A X = B (conj = 0) or A^H X = B (conj = 1) in place, from wb_getrf_launch's factors of A,
+// with scratch ws of wb_getrs_ws_bytes(n, nrhs, batch) bytes.
+hipError_t wb_getrs_blocked(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
+                            int ldb, int batch, int conj, void* ws, hipStream_t stream) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  const int nblk = (n + LB - 1) / LB;
+  cd* inv = (cd*)ws;
+  cd* tmp = inv + (size_t)batch * nblk * 2 * LB * LB;
+  const long long inv_stride = (long long)nblk * 2 * LB * LB;
+  const long long tmp_stride = (long long)LB * nrhs;
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  hipError_t e;
+  dim3 g((nrhs + 63) / 64, batch);
+  if (!conj) {
+    hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n, 0);
+    WB_CHECK_LAUNCH();
+  }
+  WB_TIMED(WB_T_TRSM_DIAG, stream,
+           hipLaunchKernelGGL(k_tri_inv, dim3(nblk, batch, 2), dim3(LB), 0, stream, n, LU, lda, inv, nblk));
+  WB_CHECK_LAUNCH();
+  auto INV = [&](int bb, int kind) { return bp_strided(inv + ((long long)bb * 2 + kind) * LB * LB, inv_stride); };
+  auto diag_solve = [&](int bb, int kind, int opa) -> hipError_t {
+    const int r0 = bb * LB, jb = (n - r0 < LB) ? n - r0 : LB;
+    hipError_t e2 = wb_zgemm_ops_launch(opa, 0, jb, nrhs, jb, one, INV(bb, kind), LB, offset(B, (long long)r0 * ldb),
+                                        ldb, zero, bp_strided(tmp, tmp_stride), nrhs, batch, stream);
+    if (e2 != hipSuccess) return e2;
+    const long long tot = (long long)jb * nrhs;
+    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256), batch), dim3(256), 0, stream, jb, nrhs,
+                       tmp, tmp_stride, nrhs, offset(B, (long long)r0 * ldb), ldb);
+    return hipGetLastError();
+  };
+  if (!conj) {
+    for (int bb = 0; bb < nblk; ++bb) {           // L: forward
+      const int r0 = bb * LB, jb = (n - r0 < LB) ? n - r0 : LB;
+      if ((e = diag_solve(bb, 0, 0))) return e;
+      const int rest = n - r0 - jb;
+      if (rest > 0 && (e = wb_zgemm_ops_launch(0, 0, rest, nrhs, jb, mone, offset(LU, (long long)(r0 + jb) * lda + r0), lda,
+                                               offset(B, (long long)r0 * ldb), ldb, one,
+                                               offset(B, (long long)(r0 + jb) * ldb), ldb, batch, stream))) return e;
+    }
+    for (int bb = nblk - 1; bb >= 0; --bb) {      // U: backward
+      const int r0 = bb * LB;
+      if ((e = diag_solve(bb, 1, 0))) return e;
+      if (r0 > 0) {
+        const int jb = (n - r0 < LB) ? n - r0 : LB;
+        if ((e = wb_zgemm_ops_launch(0, 0, r0, nrhs, jb, mone, offset(LU, (long long)r0), lda,
+                                     offset(B, (long long)r0 * ldb), ldb, one, B, ldb, batch, stream))) return e;
+      }
+    }
+  } else {
+    for (int bb = 0; bb < nblk; ++bb) {           // U^H: forward
+      const int r0 = bb * LB, jb = (n - r0 < LB) ? n - r0 : LB;
+      if ((e = diag_solve(bb, 1, 1))) return e;
+      const int rest = n - r0 - jb;
+      if (rest > 0 && (e = wb_zgemm_ops_launch(1, 0, rest, nrhs, jb, mone, offset(LU, (long long)r0 * lda + r0 + jb), lda,
+                                               offset(B, (long long)r0 * ldb), ldb, one,
+                                               offset(B, (long long)(r0 + jb) * ldb), ldb, batch, stream))) return e;
+    }
+    for (int bb = nblk - 1; bb >= 0; --bb) {      // L^H: backward
+      const int r0 = bb * LB;
+      if ((e = diag_solve(bb, 0, 1))) return e;
+      if (r0 > 0) {
+        const int jb = (n - r0 < LB) ? n - r0 : LB;
+        if ((e = wb_zgemm_ops_launch(1, 0, r0, nrhs, jb, mone, offset(LU, (long long)r0 * lda), lda,
+                                     offset(B, (long long)r0 * ldb), ldb, one, B, ldb, batch, stream))) return e;
+      }
+    }
+    hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n, 1);
+    WB_CHECK_LAUNCH();
+  }
+  return hipSuccess;
+}
+
+// inv(A) with the blocked solve: A is overwritten by its LU factors, Ainv receives the inverse.
+hipError_t wb_inverse_blocked(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv, int* info,
+                              int batch, void* ws, hipStream_t stream) {
+  hipError_t e = wb_getrf_launch(n, A, lda, ipiv, info, batch, stream);
+  if (e != hipSuccess) return e;
+  e = wb_set_identity_launch(n, Ainv, ldi, batch, stream);
+  if (e != hipSuccess) return e;
+  return wb_getrs_blocked(n, n, A, lda, ipiv, Ainv, ldi, batch, 0, ws, stream);
+}
+
 hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_int, dim3((count + 255) / 256), dim3(256), 0, stream, x, count);

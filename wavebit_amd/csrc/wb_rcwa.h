@@ -30,6 +30,7 @@ struct wb_solve_ws {
   cd *ov, *dg, *G0;
   cd* vec[4];
   double* sig;
+  void* luws;              // scratch of the blocked many-right-hand-side solves
 };
 wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode);
 

@@ -496,6 +496,7 @@ size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
   s += align_up((size_t)P * nn);           // Gamma_0
   s += align_up((size_t)P * n * 2 * sizeof(cd)) * 4;  // vectors
   s += align_up((size_t)P * sizeof(double));
+  s += align_up(wb_getrs_ws_bytes((int)n, 2 * (int)n, (int)B));
   return s + 4096;
 }
 
@@ -520,6 +521,7 @@ wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode) {
   w.G0 = (cd*)take((size_t)P * nn * sizeof(cd));
   for (int i = 0; i < 4; ++i) w.vec[i] = (cd*)take((size_t)P * n * 2 * sizeof(cd));
   w.sig = (double*)take((size_t)P * sizeof(double));
+  w.luws = take(wb_getrs_ws_bytes((int)n, 2 * (int)n, (int)B));
   return w;
 }
 
@@ -642,7 +644,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       if ((e = wb_getrf_launch(N, S(M[5], nn), N, ipiv[3], info[3], B, st))) return e;
       hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, S(M[3], nn), N, S(M[4], nn), N);
       WB_CHECK_LAUNCH();
-      if ((e = wb_getrs_launch(N, N, S(M[5], nn), N, ipiv[3], S(M[4], nn), N, B, st))) return e;
+      if ((e = wb_getrs_blocked(N, N, S(M[5], nn), N, ipiv[3], S(M[4], nn), N, B, 0, W_.luws, st))) return e;
       Bw = M[4];
     }
     // W stays in M3; V into M2
@@ -658,7 +660,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_getrf_launch(n, S(bigW, nn), n, ipiv[0], info[1], B, st))) return e;
   FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
-  if ((e = wb_getrs_launch(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, st))) return e;
+  if ((e = wb_getrs_blocked(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, 0, W_.luws, st))) return e;
   if (keep) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[0], nn), n, S(CK.Wi, nn), n);
     WB_CHECK_LAUNCH();
@@ -666,7 +668,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_getrf_launch(n, S(bigV, nn), n, ipiv[1], info[2], B, st))) return e;
   FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[3], nn), n, B, st))) return e;
-  if ((e = wb_getrs_launch(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, st))) return e;
+  if ((e = wb_getrs_blocked(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, 0, W_.luws, st))) return e;
   if (keep) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.Vi, nn), n);
     WB_CHECK_LAUNCH();
@@ -680,7 +682,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_getrf_launch(n, S(M[1], nn), n, ipiv[0], info[1], B, st))) return e;
   FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
-  if ((e = wb_getrs_launch(n, n, S(M[1], nn), n, ipiv[0], S(M[0], nn), n, B, st))) return e;
+  if ((e = wb_getrs_blocked(n, n, S(M[1], nn), n, ipiv[0], S(M[0], nn), n, B, 0, W_.luws, st))) return e;
   // Y = B Ainv -> M2 ; XYX -> M3
   if ((e = wb_zgemm_launch(n, n, n, one, S(M[5], nn), n, S(M[0], nn), n, zero, S(M[2], nn), n, B, st))) return e;
   hipLaunchKernelGGL(k_xyx, grid2(nn, B), dim3(256), 0, st, n, L, M[2], q, a.pts, a.thick, M[3]);
@@ -702,7 +704,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   // [S11 | S12] = Mm^-1 [E1 | E2]
   if ((e = wb_getrf_launch(n, S(M[0], nn), n, ipiv[1], info[2], B, st))) return e;
   FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
-  if ((e = wb_getrs_launch(n, 2 * n, S(M[0], nn), n, ipiv[1], S(M6, n2), 2 * n, B, st))) return e;
+  if ((e = wb_getrs_blocked(n, 2 * n, S(M[0], nn), n, ipiv[1], S(M6, n2), 2 * n, B, 0, W_.luws, st))) return e;
   if (keep) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M6, n2), 2 * n, S(CK.S11, nn), n);
     WB_CHECK_LAUNCH();
@@ -732,7 +734,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     // Xs = Mx^-1 s12
     hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, s12, 2 * n, Xs, n);
     WB_CHECK_LAUNCH();
-    if ((e = wb_getrs_launch(n, n, Tm, n, piv_pl, Xs, n, P, st))) return e;
+    if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, Xs, n, P, 0, W_.luws, st))) return e;
     // G2 = G Xs ; s11 += s12 G2  (s11 becomes Gamma_{l+1})
     if ((e = wb_zgemm_launch(n, n, n, one, Gam, ldg, Xs, n, zero, G2, n, P, st))) return e;
     if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, G2, n, one, s11, 2 * n, P, st))) return e;
