@@ -29,12 +29,30 @@ namespace {
 #define WB_HQR_M 8         // bulges (shifts) per sweep
 #endif
 constexpr int NT = WB_HQR_NT;
+static_assert(NT % 64 == 0, "whole waves");
 constexpr int BG = (WB_HQR_M < 8) ? WB_HQR_M : 8;   // bulges per register band of the deferred updates
 #ifndef WB_HQR_STRIPS
 #define WB_HQR_STRIPS 1   // strips per thread in the deferred updates
 #endif
 constexpr int HS = WB_HQR_STRIPS;
+
 constexpr double ULP = DBL_EPSILON;
+
+// The sweep and the band updates are separate (non-inlined) functions: these assumptions tell
+// the compiler which pointers are LDS or global, so their accesses use LDS / global
+// instructions rather than flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ASSUME_LDS(p) __builtin_assume(__builtin_amdgcn_is_shared((const void*)(p)))
+#define ASSUME_GLOBAL(p)                                              \
+  do {                                                                \
+    __builtin_assume(!__builtin_amdgcn_is_shared((const void*)(p)));  \
+    __builtin_assume(!__builtin_amdgcn_is_private((const void*)(p))); \
+  } while (0)
+#else
+#define ASSUME_LDS(p) ((void)0)
+#define ASSUME_GLOBAL(p) ((void)0)
+#endif
+
 constexpr double SAFMIN = DBL_MIN;
 
 struct Rot {
@@ -55,17 +73,31 @@ __device__ inline void refl2(cd x1, cd x2, cd* tau, cd* u, double* beta) {
   *u = cdiv(x2, mkc(x1.x - b, x1.y));
 }
 
-// left application (rows k, k+1): [a0; a1] <- H^H [a0; a1]
+// left application (rows k, k+1): [a0; a1] <- H^H [a0; a1], i.e. s = conj(tau) (a0 + conj(u) a1),
+// a0 -= s, a1 -= s u (14 FP64 operations, fused)
 __device__ inline void rot_left(const Rot& r, cd& a0, cd& a1) {
-  const cd s = cmul(cconj(r.tau), cadd(a0, cmulc(r.u, a1)));
-  a0 = csub(a0, s);
-  a1 = csub(a1, cmul(s, r.u));
+  const cd u = r.u, tau = r.tau;
+  const double tx = fma(u.x, a1.x, fma(u.y, a1.y, a0.x));
+  const double ty = fma(u.x, a1.y, fma(-u.y, a1.x, a0.y));
+  const double sx = fma(tau.x, tx, tau.y * ty);
+  const double sy = fma(tau.x, ty, -(tau.y * tx));
+  a0.x -= sx;
+  a0.y -= sy;
+  a1.x = fma(-sx, u.x, fma(sy, u.y, a1.x));
+  a1.y = fma(-sx, u.y, fma(-sy, u.x, a1.y));
 }
-// right application (columns k, k+1): [a0, a1] <- [a0, a1] H
+// right application (columns k, k+1): [a0, a1] <- [a0, a1] H, i.e. s = tau (a0 + u a1),
+// a0 -= s, a1 -= s conj(u)
 __device__ inline void rot_right(const Rot& r, cd& a0, cd& a1) {
-  const cd s = cmul(r.tau, cadd(a0, cmul(r.u, a1)));
-  a0 = csub(a0, s);
-  a1 = csub(a1, cmul(s, cconj(r.u)));
+  const cd u = r.u, tau = r.tau;
+  const double tx = fma(u.x, a1.x, fma(-u.y, a1.y, a0.x));
+  const double ty = fma(u.x, a1.y, fma(u.y, a1.x, a0.y));
+  const double sx = fma(tau.x, tx, -(tau.y * ty));
+  const double sy = fma(tau.x, ty, tau.y * tx);
+  a0.x -= sx;
+  a0.y -= sy;
+  a1.x = fma(-sx, u.x, fma(-sy, u.y, a1.x));
+  a1.y = fma(-sy, u.x, fma(sx, u.y, a1.y));
 }
 
 // Eigenvalues of a small upper Hessenberg matrix (row-major, ld) by single-shift QR, in
@@ -258,56 +290,153 @@ __device__ int single_shift_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo_
 // `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
 // only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
 // another in bulge order: the result equals the step-by-step order of the chase.
-template <int M, bool LEFT, int S, typename Get, typename Put>
-__device__ inline void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
-                                  int vlo, int vhi, Get get, Put put) {
-  // S independent strips share each step's reflectors and interleave their dependency chains
-  // (one wave per SIMD leaves nothing else to hide FP64 latency behind).
+// Compiler fence on values: the asm "reads and writes" x, so everything computing x is
+// emitted before it and everything using x after it (no instruction is generated).
+__device__ inline void pin(cd& x) { asm volatile("" : "+v"(x.x), "+v"(x.y)); }
+
+// Strip element access through a buffer resource over one matrix: element i of strip k sits at
+// byte offset soff0 + k sstride + i estride. Offsets outside the valid range are replaced by
+// BUF_OOB, beyond the resource: such loads return zero and such stores are dropped by the
+// hardware, so the band needs no branches around its loads and stores (and the waits for its
+// loads do not depend on which stores were issued).
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+constexpr unsigned BUF_OOB = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t strip_rsrc(const cd* base, int nbytes) {
+  const unsigned long long a = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  const int nrec = __builtin_amdgcn_readfirstlane(nbytes);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(((unsigned long long)hi << 32) | lo), (short)0, nrec,
+                                           0x00020000);
+}
+__device__ inline cd buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  union { v4u u; double d[2]; } c;
+  c.u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return mkc(c.d[0], c.d[1]);
+}
+__device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, cd v) {
+  union { v4u u; double d[2]; } c;
+  c.d[0] = v.x;
+  c.d[1] = v.y;
+  __builtin_amdgcn_raw_buffer_store_b128(c.u, r, off, 0, 0);
+}
+
+
+template <int M, bool LEFT, int S>
+__device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
+                                                     int vlo, int vhi, const cd* base, int nbytes, int estride,
+                                                     int soff0, int sstride) {
+  ASSUME_LDS(rots);
+  // Steps run in groups of U = 3 with compile-time band slots (bulge j of step s + u acts on
+  // slots 3 (M - 1 - j) + u, + u + 1), so the band shifts by three registers once per group
+  // rather than by one every step, and the group's three entering elements are loaded one
+  // group ahead, overlapping their memory latency with the rotations of the current group.
+  // Within a step the bulges are independent; they are applied BC at a time, the reflectors
+  // of the next BC loaded while the current ones are applied, and pinned so that no more
+  // than two chunks of reflectors are held in registers.
   constexpr int B = 3 * (M - 1) + 2;
-  cd band[S][B];
+  constexpr int U = 3;
+  constexpr int BC = (M % 4 == 0) ? 4 : M;
+  constexpr int CPS = M / BC;          // chunks per step
+  constexpr int NCH = U * CPS;         // chunks per group
+  const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc(base, nbytes);
+  cd band[S][B + U - 1];
+  cd pre[S][U];
   const int base0 = lo + s0 - 3 * (M - 1);
+  const int top_last = base0 + (s1 - 1 - s0) + B - 1;   // last element the pass reads
+  const int hi_read = (top_last + 1 < vhi) ? top_last + 1 : vhi;
+  auto off = [&](int k, int idx, int lim) -> unsigned {
+    return (idx >= vlo && idx < lim) ? (unsigned)(soff0 + k * sstride + idx * estride) : BUF_OOB;
+  };
+  auto rotate = [&](const Rot& r, cd& x0, cd& x1) {
+    if (LEFT) rot_left(r, x0, x1);
+    else rot_right(r, x0, x1);
+  };
 #pragma unroll
-  for (int i = 0; i < B - 1; ++i) {
-    const int idx = base0 + i;
-    const bool ok = idx >= vlo && idx < vhi;
+  for (int i = 0; i < B - 1; ++i)
 #pragma unroll
-    for (int k = 0; k < S; ++k) band[k][i] = ok ? get(k, idx) : mkc(0.0, 0.0);
-  }
-  for (int s = s0; s < s1; ++s) {
+    for (int k = 0; k < S; ++k) band[k][i] = buf_load(rsrc, off(k, base0 + i, hi_read));
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int k = 0; k < S; ++k) pre[k][u] = buf_load(rsrc, off(k, base0 + B - 1 + u, hi_read));
+  int s = s0;
+  int dep = 0;   // always 0; threaded through the pins to order the reflector loads
+  const int ng = (s1 - s0) / U;
+  for (int g = 0; g < ng; ++g, s += U) {
     const int base = lo + s - 3 * (M - 1);
-    const int top = base + B - 1;
-    const bool ok_top = top >= vlo && top < vhi;
 #pragma unroll
-    for (int k = 0; k < S; ++k) band[k][B - 1] = ok_top ? get(k, top) : mkc(0.0, 0.0);
-    const Rot* rs = rots + (long long)(s - s0) * rot_stride;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const Rot r = rs[j];
-      const int slot = 3 * (M - 1 - j);
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int k = 0; k < S; ++k) {
-        if (LEFT) rot_left(r, band[k][slot], band[k][slot + 1]);
-        else rot_right(r, band[k][slot], band[k][slot + 1]);
+        band[k][B - 1 + u] = pre[k][u];
+        pre[k][u] = buf_load(rsrc, off(k, base + U + B - 1 + u, hi_read));
       }
-    }
-    if (base >= vlo && base < vhi) {
+    const Rot* rs = rots + (long long)(s - s0) * rot_stride;
+    Rot cur[BC], nxt[BC];
 #pragma unroll
-      for (int k = 0; k < S; ++k) put(k, base, band[k][0]);
+    for (int jj = 0; jj < BC; ++jj) cur[jj] = rs[jj];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int u = c / CPS, j0 = (c % CPS) * BC;
+      if (c + 1 < NCH) {
+        const int un = (c + 1) / CPS, jn = ((c + 1) % CPS) * BC;
+#pragma unroll
+        for (int jj = 0; jj < BC; ++jj) nxt[jj] = rs[(long long)un * rot_stride + jn + jj + dep];
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) {
+        const int slot = 3 * (M - 1 - (j0 + jj)) + u;
+#pragma unroll
+        for (int k = 0; k < S; ++k) rotate(cur[jj], band[k][slot], band[k][slot + 1]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) {
+        const int slot = 3 * (M - 1 - (j0 + jj)) + u;
+#pragma unroll
+        for (int k = 0; k < S; ++k) { pin(band[k][slot]); pin(band[k][slot + 1]); }
+      }
+      asm volatile("" : "+v"(dep));
+      if (c % CPS == CPS - 1) {
+        // slot u is final once step u is done
+#pragma unroll
+        for (int k = 0; k < S; ++k) buf_store(rsrc, off(k, base + u, vhi), band[k][u]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) cur[jj] = nxt[jj];
     }
 #pragma unroll
     for (int i = 0; i < B - 1; ++i)
 #pragma unroll
+      for (int k = 0; k < S; ++k) band[k][i] = band[k][i + U];
+  }
+  // the remaining (at most U - 1) steps one at a time; their entering elements are in pre
+#pragma unroll
+  for (int u = 0; u < U - 1; ++u) {
+    if (s >= s1) break;
+    const int base = lo + s - 3 * (M - 1);
+#pragma unroll
+    for (int k = 0; k < S; ++k) band[k][B - 1] = pre[k][u];
+    const Rot* rs = rots + (long long)(s - s0) * rot_stride;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const Rot r = rs[j];
+#pragma unroll
+      for (int k = 0; k < S; ++k) rotate(r, band[k][3 * (M - 1 - j)], band[k][3 * (M - 1 - j) + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) buf_store(rsrc, off(k, base, vhi), band[k][0]);
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i)
+#pragma unroll
       for (int k = 0; k < S; ++k) band[k][i] = band[k][i + 1];
+    ++s;
   }
   const int base_end = lo + s1 - 3 * (M - 1);
 #pragma unroll
-  for (int i = 0; i < B - 1; ++i) {
-    const int idx = base_end + i;
-    if (idx >= vlo && idx < vhi) {
+  for (int i = 0; i < B - 1; ++i)
 #pragma unroll
-      for (int k = 0; k < S; ++k) put(k, idx, band[k][i]);
-    }
-  }
+    for (int k = 0; k < S; ++k) buf_store(rsrc, off(k, base_end + i, vhi), band[k][i]);
 }
 
 // Calls f(x, S) for this thread's strips x = first, first + NT, ... < end: SB strips at a time
@@ -354,6 +483,14 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
 template <int M>
 __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, const cd* sh, int W,
                                  cd* Hw, Rot (*s_rot)[M], Rot* rot_win, Rot* rot_all) {
+  const int nbytes = 16 * n * n;   // one matrix (H or Z): byte offsets fit in 32 bits
+  ASSUME_LDS(Hw);
+  ASSUME_LDS(s_rot);
+  ASSUME_LDS(rot_win);
+  ASSUME_LDS(sh);
+  ASSUME_GLOBAL(H);
+  ASSUME_GLOBAL(Z);
+  ASSUME_GLOBAL(rot_all);
   static_assert(M % BG == 0, "bulge count must be a multiple of the band group");
   const int t = threadIdx.x;
   const int T = (hi - 1 - lo) + 3 * (M - 1);       // last step
@@ -389,22 +526,23 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
     }
     for (int s = s_begin; s < s_end; ++s) {
       const Rot* rs = s_rot[s & 1];
-      // row operations: bulge j on rows k, k+1, columns k..wend-1
-      for (int e = t; e < M * ww; e += NT) {
-        const int j = e / ww, c = a + e % ww;
-        const int k = lo + s - 3 * j;
-        if (k < lo || k > hi - 1 || c < k) continue;
-        rot_left(rs[j], Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
-      }
-      __syncthreads();
-      // column operations: bulge j on columns k, k+1, rows a..min(k+2, hi); rows k+1, k+2
-      // and the next reflector by thread j
-      for (int e = t; e < M * ww; e += NT) {
-        const int j = e / ww, q = a + e % ww;
+      // row operations: bulge j (wave j mod the wave count) on rows k, k+1, columns k..wend-1
+      for (int j = t >> 6; j < M; j += NT / 64) {
         const int k = lo + s - 3 * j;
         if (k < lo || k > hi - 1) continue;
-        if (q > k || q > hi) continue;
-        rot_right(rs[j], Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+        const Rot r = rs[j];
+        for (int c = k + (t & 63); c < wend; c += 64)
+          rot_left(r, Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
+      }
+      __syncthreads();
+      // column operations: bulge j on columns k, k+1, rows a..k; rows k+1, k+2 and the next
+      // reflector by thread j
+      for (int j = t >> 6; j < M; j += NT / 64) {
+        const int k = lo + s - 3 * j;
+        if (k < lo || k > hi - 1) continue;
+        const Rot r = rs[j];
+        for (int q = a + (t & 63); q <= k; q += 64)
+          rot_right(r, Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
       }
       if (t < M) {
         const int j = t;
@@ -432,17 +570,15 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
     for_strips<HS>(wend + t, n, [&](int c, auto sc) {
       constexpr int SS = decltype(sc)::value;
       for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, true, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
-                                 [&](int k, int i) { return H[(long long)i * n + c + k * NT]; },
-                                 [&](int k, int i, cd v) { H[(long long)i * n + c + k * NT] = v; });
+        band_apply<BG, true, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend, H, nbytes, 16 * n,
+                                 16 * c, 16 * NT);
     });
     // deferred column operations on the rows above the window (columns a..wend-1)
     for_strips<HS>(t, a, [&](int q, auto sc) {
       constexpr int SS = decltype(sc)::value;
       for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, false, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend,
-                                  [&](int k, int i) { return H[(long long)(q + k * NT) * n + i]; },
-                                  [&](int k, int i, cd v) { H[(long long)(q + k * NT) * n + i] = v; });
+        band_apply<BG, false, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend, H, nbytes, 16,
+                                  16 * n * q, 16 * n * NT);
     });
     __syncthreads();
     s_begin = s_end;
@@ -461,17 +597,15 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
       for_strips<HS>(t, n, [&](int q, auto sc) {
         constexpr int SS = decltype(sc)::value;
         for (int g0 = 0; g0 < M; g0 += BG)
-          band_apply<BG, true, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
-                                   [&](int k, int i) { return Z[(long long)i * n + q + k * NT]; },
-                                   [&](int k, int i, cd v) { Z[(long long)i * n + q + k * NT] = v; });
+          band_apply<BG, true, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1, Z, nbytes, 16 * n, 16 * q,
+                                   16 * NT);
       });
     } else {
       for_strips<HS>(t, n, [&](int q, auto sc) {
         constexpr int SS = decltype(sc)::value;
         for (int g0 = 0; g0 < M; g0 += BG)
-          band_apply<BG, false, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1,
-                                    [&](int k, int i) { return Z[(long long)(q + k * NT) * n + i]; },
-                                    [&](int k, int i, cd v) { Z[(long long)(q + k * NT) * n + i] = v; });
+          band_apply<BG, false, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1, Z, nbytes, 16,
+                                    16 * n * q, 16 * n * NT);
       });
     }
   }
@@ -482,10 +616,10 @@ template <int M>
 __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd* wb, int* info,
                                                Rot* rws, int W, int nmin) {
   const long long b = blockIdx.x;
-  cd* H = Hb + b * (long long)n * n;
-  cd* Z = Zb + b * (long long)n * n;
+  cd* H = (cd*)(Hb + b * (long long)n * n);
+  cd* Z = (cd*)(Zb + b * (long long)n * n);
   cd* wout = wb + b * (long long)n;
-  Rot* rot_all = rws + b * (long long)(n + 3 * M) * M;
+  Rot* rot_all = (Rot*)(rws + b * (long long)(n + 3 * M) * M);
   extern __shared__ unsigned char smem[];
   cd* Hw = (cd*)smem;
   Rot* rot_win = (Rot*)(smem + sizeof(cd) * (size_t)W * (W + 1));
