@@ -339,8 +339,10 @@ namespace {
 constexpr int LB = 64;
 
 // inv(unit lower L_bb) (kind 0) or inv(upper U_bb) (kind 1) of diagonal block bb, LB x LB
-// row-major per (matrix, block, kind); thread per column.
-__global__ __launch_bounds__(LB) void k_tri_inv(int n, BatchPtr LU, int lda, cd* inv, int nblk) {
+// row-major per (matrix, block, kind). Four lanes per column of the inverse: row i of the
+// inverse is one dot product per column over the rows already found, split four ways and
+// summed across the lane quad; rows go top-down (kind 0) or bottom-up (kind 1).
+__global__ __launch_bounds__(4 * LB) void k_tri_inv(int n, BatchPtr LU, int lda, cd* inv, int nblk) {
   const int bb = blockIdx.x, kind = blockIdx.z;
   const long long b = blockIdx.y;
   const cd* a = LU.ptr(b);
@@ -349,37 +351,43 @@ __global__ __launch_bounds__(LB) void k_tri_inv(int n, BatchPtr LU, int lda, cd*
   __shared__ cd T[LB][LB + 1];
   __shared__ cd X[LB][LB + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < LB * LB; e += LB) {
+  for (int e = t; e < LB * LB; e += 4 * LB) {
     const int r = e / LB, c = e % LB;
     T[r][c] = (r < jb && c < jb) ? a[(long long)(r0 + r) * lda + r0 + c] : mkc(0.0, 0.0);
   }
   __syncthreads();
-  const int j = t;   // column of the inverse
+  const int j = t >> 2, p = t & 3;   // column of the inverse, quarter of the dot product
+  const bool live = j < jb;
   if (kind == 0) {
     for (int i = 0; i < LB; ++i) {
-      cd x = mkc(i == j ? 1.0 : 0.0, 0.0);
-      if (i < jb && j < jb) {
-        for (int l = j; l < i; ++l) x = csub(x, cmul(T[i][l], X[l][j]));
-      } else {
-        x = mkc(0.0, 0.0);
+      cd acc = mkc(0.0, 0.0);
+      if (live && i < jb)
+        for (int l = j + p; l < i; l += 4) acc = cadd(acc, cmul(T[i][l], X[l][j]));
+      acc.x += __shfl_xor(acc.x, 1); acc.y += __shfl_xor(acc.y, 1);
+      acc.x += __shfl_xor(acc.x, 2); acc.y += __shfl_xor(acc.y, 2);
+      if (p == 0) {
+        cd x = mkc(0.0, 0.0);
+        if (live && i < jb) x = (i == j) ? mkc(1.0, 0.0) : (i > j ? cneg(acc) : mkc(0.0, 0.0));
+        X[i][j] = x;
       }
-      X[i][j] = x;
     }
   } else {
     for (int i = LB - 1; i >= 0; --i) {
-      cd x = mkc(i == j ? 1.0 : 0.0, 0.0);
-      if (i < jb && j < jb) {
-        for (int l = i + 1; l <= j; ++l) x = csub(x, cmul(T[i][l], X[l][j]));
-        x = cdiv(x, T[i][i]);
-      } else {
-        x = mkc(0.0, 0.0);
+      cd acc = mkc(0.0, 0.0);
+      if (live && i < jb)
+        for (int l = i + 1 + p; l <= j; l += 4) acc = cadd(acc, cmul(T[i][l], X[l][j]));
+      acc.x += __shfl_xor(acc.x, 1); acc.y += __shfl_xor(acc.y, 1);
+      acc.x += __shfl_xor(acc.x, 2); acc.y += __shfl_xor(acc.y, 2);
+      if (p == 0) {
+        cd x = mkc(0.0, 0.0);
+        if (live && i < jb && i <= j) x = cdiv(csub(mkc(i == j ? 1.0 : 0.0, 0.0), acc), T[i][i]);
+        X[i][j] = x;
       }
-      X[i][j] = x;
     }
   }
   __syncthreads();
   cd* out = inv + ((b * nblk + bb) * 2 + kind) * (long long)LB * LB;
-  for (int e = t; e < LB * LB; e += LB) out[e] = X[e / LB][e % LB];
+  for (int e = t; e < LB * LB; e += 4 * LB) out[e] = X[e / LB][e % LB];
 }
 
 __global__ void k_copy_rows(int rows, int cols, const cd* src, long long s_stride, int lds, BatchPtr dst,
@@ -416,7 +424,7 @@ hipError_t wb_getrs_blocked(int n, int nrhs, BatchPtr LU, int lda, const int* ip
     WB_CHECK_LAUNCH();
   }
   WB_TIMED(WB_T_TRSM_DIAG, stream,
-           hipLaunchKernelGGL(k_tri_inv, dim3(nblk, batch, 2), dim3(LB), 0, stream, n, LU, lda, inv, nblk));
+           hipLaunchKernelGGL(k_tri_inv, dim3(nblk, batch, 2), dim3(4 * LB), 0, stream, n, LU, lda, inv, nblk));
   WB_CHECK_LAUNCH();
   auto INV = [&](int bb, int kind) { return bp_strided(inv + ((long long)bb * 2 + kind) * LB * LB, inv_stride); };
   auto diag_solve = [&](int bb, int kind, int opa) -> hipError_t {

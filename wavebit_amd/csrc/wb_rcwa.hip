@@ -246,18 +246,38 @@ __global__ void k_ab(int N, int n, int L, int mode, const cd* Wi, const cd* Vi, 
 }
 
 // XYX[r][c] = X_r Y[r][c] X_c with X = exp(-k0 q d).
-__global__ void k_xyx(int n, int L, const cd* Y, const cd* q, const double* pts, const double* thick,
-                      cd* XYX) {
+// Layer propagation diagonal x_r = exp(-k0 d q_r) per (pair, mode).
+__global__ void k_xvec(int n, int L, const cd* q, const double* pts, const double* thick, cd* xv) {
   const long long b = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
   const int p = (int)(b / L);
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)n * n) return;
-  const int r = (int)(e / n), c = (int)(e % n);
   const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
-  const double d = thick[b];
-  const cd xr = cexp_(cscale(q[b * n + r], -k0 * d));
-  const cd xc = cexp_(cscale(q[b * n + c], -k0 * d));
-  XYX[b * (long long)n * n + e] = cmul(cmul(xr, Y[b * (long long)n * n + e]), xc);
+  xv[b * n + r] = cexp_(cscale(q[b * n + r], -k0 * thick[b]));
+}
+
+// Tiled conjugate transpose: dst = src^H (n x n per batch element), 32 x 32 tiles through LDS
+// so both the reads and the writes are coalesced. With xv, dst = X src^H X (X = diag(xv)).
+__global__ __launch_bounds__(256) void k_ctrans(int n, const cd* src, cd* dst, const cd* xv) {
+  __shared__ cd tile[32][33];
+  const long long b = blockIdx.z;
+  const cd* S = src + b * (long long)n * n;
+  cd* D = dst + b * (long long)n * n;
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;   // tile of dst
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {                         // src rows c0 + k, columns r0 + tx
+    const int sr = c0 + k, sc = r0 + tx;
+    if (sr < n && sc < n) tile[k][tx] = S[(long long)sr * n + sc];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {                         // dst row r0 + k, column c0 + tx
+    const int r = r0 + k, c = c0 + tx;
+    if (r < n && c < n) {
+      cd v = cconj(tile[tx][k]);
+      if (xv) v = cmul(cmul(xv[b * n + r], v), xv[b * n + c]);
+      D[(long long)r * n + c] = v;
+    }
+  }
 }
 
 // dst (ld_dst) = src (ld_src), n x ncols per batch element
@@ -270,17 +290,13 @@ __global__ void k_copy(int rows, int cols, BatchPtr src, int lds, BatchPtr dst, 
 }
 
 // E2 rows scaled by X = exp(-k0 q d): E2 lives at columns n..2n-1 of a row stride 2n buffer.
-__global__ void k_scale_rows_x(int n, int L, cd* E, const cd* q, const double* pts,
-                               const double* thick) {
+__global__ void k_scale_rows_x(int n, cd* E, const cd* xv) {
   const long long b = blockIdx.y;
-  const int p = (int)(b / L);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)n * n) return;
   const int r = (int)(e / n), c = (int)(e % n);
-  const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
-  const cd xr = cexp_(cscale(q[b * n + r], -k0 * thick[b]));
   cd* x = E + b * (long long)n * 2 * n + (long long)r * 2 * n + n + c;
-  *x = cmul(xr, *x);
+  *x = cmul(xv[b * n + r], *x);
 }
 
 // Gamma_0: full matrix with the diagonal of S_ref's 22 block.
@@ -676,16 +692,19 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   // A -> M4, B -> M5
   hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, M[0], M[3], ov, M[4], M[5]);
   WB_CHECK_LAUNCH();
-  // Ainv: LU of a copy of A in M1, inverse in M0
+  // Y = B A^-1 from A^H Y^H = B^H: LU of a copy of A in M1, Y^H -> M2 ; XYX -> M3
   hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M[1], nn), n);
   WB_CHECK_LAUNCH();
   if ((e = wb_getrf_launch(n, S(M[1], nn), n, ipiv[0], info[1], B, st))) return e;
   FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
-  if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
-  if ((e = wb_getrs_blocked(n, n, S(M[1], nn), n, ipiv[0], S(M[0], nn), n, B, 0, W_.luws, st))) return e;
-  // Y = B Ainv -> M2 ; XYX -> M3
-  if ((e = wb_zgemm_launch(n, n, n, one, S(M[5], nn), n, S(M[0], nn), n, zero, S(M[2], nn), n, B, st))) return e;
-  hipLaunchKernelGGL(k_xyx, grid2(nn, B), dim3(256), 0, st, n, L, M[2], q, a.pts, a.thick, M[3]);
+  const dim3 tgrid((n + 31) / 32, (n + 31) / 32, B);
+  hipLaunchKernelGGL(k_ctrans, tgrid, dim3(256), 0, st, n, M[5], M[2], (const cd*)nullptr);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_getrs_blocked(n, n, S(M[1], nn), n, ipiv[0], S(M[2], nn), n, B, 1, W_.luws, st))) return e;
+  cd* xv = tau;   // the Householder scalars of the eigensolve are no longer needed
+  hipLaunchKernelGGL(k_xvec, dim3((n + 255) / 256, B), dim3(256), 0, st, n, L, q, a.pts, a.thick, xv);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ctrans, tgrid, dim3(256), 0, st, n, M[2], M[3], (const cd*)xv);
   WB_CHECK_LAUNCH();
   // Mm = A - XYX B -> M0
   hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M[0], nn), n);
@@ -698,8 +717,9 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_zgemm_launch(n, n, n, one, S(M[3], nn), n, S(M[4], nn), n, mone, S(M6, n2), 2 * n, B, st))) return e;
   hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M6 + n, n2), 2 * n);
   WB_CHECK_LAUNCH();
-  if ((e = wb_zgemm_launch(n, n, n, mone, S(M[2], nn), n, S(M[5], nn), n, one, S(M6 + n, n2), 2 * n, B, st))) return e;
-  hipLaunchKernelGGL(k_scale_rows_x, grid2(nn, B), dim3(256), 0, st, n, L, M6, q, a.pts, a.thick);
+  if ((e = wb_zgemm_ops_launch(1, 0, n, n, n, mone, S(M[2], nn), n, S(M[5], nn), n, one, S(M6 + n, n2), 2 * n,
+                               B, st))) return e;
+  hipLaunchKernelGGL(k_scale_rows_x, grid2(nn, B), dim3(256), 0, st, n, M6, xv);
   WB_CHECK_LAUNCH();
   // [S11 | S12] = Mm^-1 [E1 | E2]
   if ((e = wb_getrf_launch(n, S(M[0], nn), n, ipiv[1], info[2], B, st))) return e;
