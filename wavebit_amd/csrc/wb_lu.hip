@@ -12,7 +12,6 @@
 #include "wb_internal.h"
 #include "wb_timing.h"
 
-
 #define WB_NB 32
 
 namespace {
@@ -75,6 +74,106 @@ __global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, Batc
       for (int c = j + 1 - k0; c < jb; ++c) row[k0 + c] = csub(row[k0 + c], cmul(l, s_prow[c]));
     }
     __syncthreads();
+  }
+}
+
+// The same panel factorization with the panel held in registers: one thread per row (n <= PNT),
+// its WB_NB panel entries in registers, so a column step is a wave-level argmax (first maximum
+// wins), one barrier to combine the waves, one to exchange the pivot row through LDS (double
+// buffered), and a rank-1 update of each thread's own row, with no global traffic inside the
+// panel.
+constexpr int PNT = 768;
+constexpr int PNW = PNT / 64;
+struct PanelCtx {
+  int n, k0, jb, r, lane, wv, t;
+  bool own;
+  int* piv;
+  int* info;
+  long long b;
+  double* s_v;
+  int* s_i;
+  cd (*s_prow)[WB_NB];
+  cd (*s_jrow)[WB_NB];
+};
+// Column C of the register-resident panel (compile-time C keeps x[] in registers).
+template <int C>
+__device__ __forceinline__ void panel_col(cd (&x)[WB_NB], const PanelCtx& P) {
+  if (C < P.jb) {
+    const int j = P.k0 + C, r = P.r;
+    const bool cand = P.own && r >= j;
+    double v = cand ? cabs1(x[C]) : -1.0;
+    int vi = cand ? r : P.n;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(v, off);
+      const int i2 = __shfl_xor(vi, off);
+      if (v2 > v || (v2 == v && i2 < vi)) { v = v2; vi = i2; }
+    }
+    if (P.lane == 0) { P.s_v[P.wv] = v; P.s_i[P.wv] = vi; }
+    __syncthreads();
+    double pv = P.s_v[0];
+    int p = P.s_i[0];
+#pragma unroll
+    for (int w = 1; w < PNW; ++w) {
+      const double v2 = P.s_v[w];
+      const int i2 = P.s_i[w];
+      if (v2 > pv || (v2 == pv && i2 < p)) { pv = v2; p = i2; }
+    }
+    constexpr int buf = C & 1;
+    if (r == p) {
+#pragma unroll
+      for (int cc = 0; cc < WB_NB; ++cc) P.s_prow[buf][cc] = x[cc];
+    }
+    if (r == j && p != j) {
+#pragma unroll
+      for (int cc = 0; cc < WB_NB; ++cc) P.s_jrow[buf][cc] = x[cc];
+    }
+    __syncthreads();
+    if (r == j) {
+#pragma unroll
+      for (int cc = 0; cc < WB_NB; ++cc) x[cc] = P.s_prow[buf][cc];
+    } else if (r == p) {
+#pragma unroll
+      for (int cc = 0; cc < WB_NB; ++cc) x[cc] = P.s_jrow[buf][cc];
+    }
+    if (P.t == 0) {
+      P.piv[j] = p;
+      if (pv == 0.0 && P.info[P.b] == 0) P.info[P.b] = j + 1;
+    }
+    if (P.own && r > j) {
+      const cd l = (pv == 0.0) ? x[C] : cmul(x[C], crecip(P.s_prow[buf][C]));
+      x[C] = l;
+#pragma unroll
+      for (int cc = C + 1; cc < WB_NB; ++cc) x[cc] = csub(x[cc], cmul(l, P.s_prow[buf][cc]));
+    }
+  }
+  if constexpr (C + 1 < WB_NB) panel_col<C + 1>(x, P);
+}
+
+__global__ __launch_bounds__(PNT) void k_getrf_panel_reg(int n, int k0, int jb, BatchPtr A, int lda,
+                                                         int* ipiv, int* info) {
+  __shared__ double s_v[PNW];
+  __shared__ int s_i[PNW];
+  __shared__ cd s_prow[2][WB_NB];
+  __shared__ cd s_jrow[2][WB_NB];
+  PanelCtx P;
+  P.b = blockIdx.x;
+  cd* a = A.ptr(P.b);
+  P.n = n; P.k0 = k0; P.jb = jb;
+  P.t = threadIdx.x; P.lane = P.t & 63; P.wv = P.t >> 6;
+  P.r = P.t;
+  P.own = P.r >= k0 && P.r < n;
+  P.piv = ipiv + P.b * (long long)n;
+  P.info = info;
+  P.s_v = s_v; P.s_i = s_i; P.s_prow = s_prow; P.s_jrow = s_jrow;
+  cd x[WB_NB];
+#pragma unroll
+  for (int c = 0; c < WB_NB; ++c) x[c] = (P.own && c < jb) ? a[(long long)P.r * lda + k0 + c] : mkc(0.0, 0.0);
+  panel_col<0>(x, P);
+  if (P.own) {
+#pragma unroll
+    for (int c = 0; c < WB_NB; ++c)
+      if (c < jb) a[(long long)P.r * lda + k0 + c] = x[c];
   }
 }
 
@@ -225,9 +324,15 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
   WB_CHECK_LAUNCH();
   for (int k0 = 0; k0 < n; k0 += WB_NB) {
     const int jb = (n - k0 < WB_NB) ? n - k0 : WB_NB;
-    WB_TIMED(WB_T_GETRF_PANEL, stream,
-             hipLaunchKernelGGL(k_getrf_panel, dim3(batch), dim3(256), 0, stream, n, k0, jb, A, lda,
-                                ipiv, info));
+    if (n <= PNT) {
+      WB_TIMED(WB_T_GETRF_PANEL, stream,
+               hipLaunchKernelGGL(k_getrf_panel_reg, dim3(batch), dim3(PNT), 0, stream, n, k0, jb, A, lda,
+                                  ipiv, info));
+    } else {
+      WB_TIMED(WB_T_GETRF_PANEL, stream,
+               hipLaunchKernelGGL(k_getrf_panel, dim3(batch), dim3(256), 0, stream, n, k0, jb, A, lda,
+                                  ipiv, info));
+    }
     WB_CHECK_LAUNCH();
     const int outside = n - jb;
     if (outside > 0) {
