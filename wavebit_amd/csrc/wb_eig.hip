@@ -172,9 +172,51 @@ __global__ __launch_bounds__(NTH) void k_unghr(int n, cd* Ab, const cd* taub, cd
 // that streams the trailing block (its columns are untouched within the panel). The trailing
 // updates A -= Y V^H and A -= V T^H V^H A are batched GEMMs on the matrix cores.
 constexpr int HNB = 32;
+constexpr int HNT = 512;     // panel threads: 8 waves, each with a full row of GEMV loads in flight
+
+// sum_{j < m} x[j] w[j] over a contiguous row segment x (m <= HNB) in unrolled chunks of 8 with
+// the loads unconditional (x has HNB entries) and the terms j >= m selected away, so a chunk's
+// loads are in flight together.
+__device__ inline cd row_dot(const cd* x, const cd* w, int m) {
+  cd acc0 = mkc(0.0, 0.0), acc1 = mkc(0.0, 0.0);
+  for (int j0 = 0; j0 < m; j0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      const int j = j0 + u;
+      const cd p0 = cmul(x[j], w[j]), p1 = cmul(x[j + 1], w[j + 1]);
+      if (j < m) acc0 = cadd(acc0, p0);
+      if (j + 1 < m) acc1 = cadd(acc1, p1);
+    }
+  }
+  return cadd(acc0, acc1);
+}
+// sum over rows r = r0, r0 + st, ... < rend of conj(v[r n]) a[r n] (column entries with row
+// stride n), four rows in flight.
+__device__ inline cd col_dotc(const cd* v, const cd* a, int n, int r0, int rend, int st) {
+  cd acc[4] = {mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0)};
+  int r = r0;
+  for (; r + 3 * st < rend; r += 4 * st) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      acc[u] = cadd(acc[u], cmulc(v[(long long)(r + st * u) * n], a[(long long)(r + st * u) * n]));
+  }
+  for (; r < rend; r += st) acc[0] = cadd(acc[0], cmulc(v[(long long)r * n], a[(long long)r * n]));
+  return cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3]));
+}
+// the same with the second factor a vector in LDS indexed by row
+__device__ inline cd col_dotc_lds(const cd* v, const cd* s, int n, int r0, int rend, int st) {
+  cd acc[4] = {mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0), mkc(0.0, 0.0)};
+  int r = r0;
+  for (; r + 3 * st < rend; r += 4 * st) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = cadd(acc[u], cmulc(v[(long long)(r + st * u) * n], s[r + st * u]));
+  }
+  for (; r < rend; r += st) acc[0] = cadd(acc[0], cmulc(v[(long long)r * n], s[r]));
+  return cadd(cadd(acc[0], acc[1]), cadd(acc[2], acc[3]));
+}
 
 // Y and T live in the per-matrix scratch X, whose batch stride is x_stride.
-__global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab, cd* Vb, cd* Yb,
+__global__ __launch_bounds__(HNT) void k_hrd_panel(int n, int k0, int ib, cd* Ab, cd* Vb, cd* Yb,
                                                    cd* Tb, long long x_stride) {
   const long long b = blockIdx.x;
   cd* A = Ab + b * (long long)n * n;
@@ -182,14 +224,14 @@ __global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab
   cd* Y = Yb + b * x_stride;                     // n x HNB
   cd* Tg = Tb + b * x_stride;                    // this panel's T, HNB x HNB
   __shared__ cd sT[HNB][HNB + 1];
-  __shared__ cd red[32][HNB + 1];
+  __shared__ cd red[HNT / 32][HNB + 1];
   __shared__ cd sa[HNB], sw[HNB];
-  __shared__ double sred[NTH / 64];
+  __shared__ double sred[HNT / 64];
   __shared__ cd s_tau;
   extern __shared__ cd sv[];                     // reflector i by absolute row, n entries
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int jj = t & 31, ch = t >> 5;            // (reflector index, row chunk) for reductions
-  for (int e = t; e < HNB * (HNB + 1); e += NTH) sT[e / (HNB + 1)][e % (HNB + 1)] = mkc(0.0, 0.0);
+  for (int e = t; e < HNB * (HNB + 1); e += HNT) sT[e / (HNB + 1)][e % (HNB + 1)] = mkc(0.0, 0.0);
   __syncthreads();
   for (int i = 0; i < ib; ++i) {
     const int c = k0 + i;
@@ -197,24 +239,19 @@ __global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab
       // right: A[r][c] -= sum_j Y[r][j] conj(V[c][k0+j]), rows k0+1..n-1
       if (t < i) sa[t] = cconj(V[(long long)c * n + k0 + t]);
       __syncthreads();
-      for (int r = k0 + 1 + t; r < n; r += NTH) {
-        cd acc = mkc(0.0, 0.0);
-        for (int j = 0; j < i; ++j) acc = cadd(acc, cmul(Y[(long long)r * HNB + j], sa[j]));
-        A[(long long)r * n + c] = csub(A[(long long)r * n + c], acc);
-      }
+      for (int r = k0 + 1 + t; r < n; r += HNT)
+        A[(long long)r * n + c] = csub(A[(long long)r * n + c], row_dot(Y + (long long)r * HNB, sa, i));
       __syncthreads();
       // left: w = V^H A[:, c] (rows k0+1..), w = T^H w, A[:, c] -= V w
       {
         cd acc = mkc(0.0, 0.0);
-        if (jj < i)
-          for (int r = k0 + 1 + ch; r < n; r += 32)
-            acc = cadd(acc, cmulc(V[(long long)r * n + k0 + jj], A[(long long)r * n + c]));
+        if (jj < i) acc = col_dotc(V + k0 + jj, A + c, n, k0 + 1 + ch, n, HNT / 32);
         red[ch][jj] = acc;
       }
       __syncthreads();
       if (t < i) {
         cd s = mkc(0.0, 0.0);
-        for (int q = 0; q < 32; ++q) s = cadd(s, red[q][t]);
+        for (int q = 0; q < HNT / 32; ++q) s = cadd(s, red[q][t]);
         sw[t] = s;
       }
       __syncthreads();
@@ -224,17 +261,14 @@ __global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab
         sa[t] = s;
       }
       __syncthreads();
-      for (int r = k0 + 1 + t; r < n; r += NTH) {
-        cd acc = mkc(0.0, 0.0);
-        for (int j = 0; j < i; ++j) acc = cadd(acc, cmul(V[(long long)r * n + k0 + j], sa[j]));
-        A[(long long)r * n + c] = csub(A[(long long)r * n + c], acc);
-      }
+      for (int r = k0 + 1 + t; r < n; r += HNT)
+        A[(long long)r * n + c] = csub(A[(long long)r * n + c], row_dot(V + (long long)r * n + k0, sa, i));
       __syncthreads();
     }
     // reflector from x = A[c+1:, c]
     double ss = 0.0;
-    for (int r = c + 2 + t; r < n; r += NTH) ss += cnorm2(A[(long long)r * n + c]);
-    const double xnorm2 = block_sum<NTH>(ss, sred);
+    for (int r = c + 2 + t; r < n; r += HNT) ss += cnorm2(A[(long long)r * n + c]);
+    const double xnorm2 = block_sum<HNT>(ss, sred);
     const cd alpha = A[(long long)(c + 1) * n + c];
     cd tk, scale;
     double beta;
@@ -247,47 +281,61 @@ __global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab
       tk = mkc((beta - alpha.x) / beta, -alpha.y / beta);
       scale = crecip(mkc(alpha.x - beta, alpha.y));
     }
-    for (int r = c + 1 + t; r < n; r += NTH)
+    for (int r = c + 1 + t; r < n; r += HNT)
       sv[r] = (r == c + 1) ? mkc(1.0, 0.0) : cmul(A[(long long)r * n + c], scale);
     __syncthreads();
-    for (int r = t; r < n; r += NTH) {
+    for (int r = t; r < n; r += HNT) {
       V[(long long)r * n + k0 + i] = (r <= c) ? mkc(0.0, 0.0) : sv[r];
       if (r == c + 1) A[(long long)r * n + c] = mkc(beta, 0.0);
       else if (r > c + 1) A[(long long)r * n + c] = mkc(0.0, 0.0);
     }
-    // Y[r][i] (unscaled) = A[r, c+1:] v, rows k0+1..n-1: a wave per row
-    for (int r = k0 + 1 + wave; r < n; r += NTH / 64) {
-      const cd* row = A + (long long)r * n;
-      double zr = 0.0, zi = 0.0;
-      for (int col = c + 1 + lane; col < n; col += 64) {
-        const cd p = cmul(row[col], sv[col]);
-        zr += p.x;
-        zi += p.y;
+    // Y[r][i] (unscaled) = A[r, c+1:] v, rows k0+1..n-1: a wave per row, all of a lane's loads
+    // of the row (up to GC) issued at once (the GEMV is the HBM-bound part of the reduction:
+    // the trailing matrix is streamed once per column).
+    {
+      constexpr int GC = 12;
+      const int cb = c + 1 + lane;
+      for (int r = k0 + 1 + wave; r < n; r += HNT / 64) {
+        const cd* row = A + (long long)r * n;
+        cd a[GC];
+#pragma unroll
+        for (int u = 0; u < GC; ++u) a[u] = (cb + 64 * u < n) ? row[cb + 64 * u] : mkc(0.0, 0.0);
+        double zr0 = 0.0, zi0 = 0.0, zr1 = 0.0, zi1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < GC; u += 2) {
+          const cd v0 = (cb + 64 * u < n) ? sv[cb + 64 * u] : mkc(0.0, 0.0);
+          const cd v1 = (cb + 64 * (u + 1) < n) ? sv[cb + 64 * (u + 1)] : mkc(0.0, 0.0);
+          const cd p0 = cmul(a[u], v0), p1 = cmul(a[u + 1], v1);
+          zr0 += p0.x; zi0 += p0.y;
+          zr1 += p1.x; zi1 += p1.y;
+        }
+        for (int col = cb + 64 * GC; col < n; col += 64) {
+          const cd p = cmul(row[col], sv[col]);
+          zr0 += p.x;
+          zi0 += p.y;
+        }
+        const double sr = wave_sum(zr0 + zr1);
+        const double si = wave_sum(zi0 + zi1);
+        if (lane == 0) Y[(long long)r * HNB + i] = mkc(sr, si);
       }
-      zr = wave_sum(zr);
-      zi = wave_sum(zi);
-      if (lane == 0) Y[(long long)r * HNB + i] = mkc(zr, zi);
     }
     // tv = V[:, 0:i]^H v over rows c+1..n-1
     {
       cd acc = mkc(0.0, 0.0);
-      if (jj < i)
-        for (int r = c + 1 + ch; r < n; r += 32)
-          acc = cadd(acc, cmulc(V[(long long)r * n + k0 + jj], sv[r]));
+      if (jj < i) acc = col_dotc_lds(V + k0 + jj, sv, n, c + 1 + ch, n, HNT / 32);
       red[ch][jj] = acc;
     }
     if (t == 0) s_tau = tk;
     __syncthreads();
     if (t < i) {
       cd s = mkc(0.0, 0.0);
-      for (int q = 0; q < 32; ++q) s = cadd(s, red[q][t]);
+      for (int q = 0; q < HNT / 32; ++q) s = cadd(s, red[q][t]);
       sw[t] = s;
     }
     __syncthreads();
     // Y[:, i] = tau (y - Y[:, 0:i] tv) ; T[0:i, i] = -tau T[0:i, 0:i] tv ; T[i][i] = tau
-    for (int r = k0 + 1 + t; r < n; r += NTH) {
-      cd acc = Y[(long long)r * HNB + i];
-      for (int j = 0; j < i; ++j) acc = csub(acc, cmul(Y[(long long)r * HNB + j], sw[j]));
+    for (int r = k0 + 1 + t; r < n; r += HNT) {
+      const cd acc = csub(Y[(long long)r * HNB + i], row_dot(Y + (long long)r * HNB, sw, i));
       Y[(long long)r * HNB + i] = cmul(tk, acc);
     }
     if (t < i) {
@@ -298,7 +346,7 @@ __global__ __launch_bounds__(NTH) void k_hrd_panel(int n, int k0, int ib, cd* Ab
     if (t == 0) sT[i][i] = tk;
     __syncthreads();
   }
-  for (int e = t; e < HNB * HNB; e += NTH) Tg[e] = sT[e / HNB][e % HNB];
+  for (int e = t; e < HNB * HNB; e += HNT) Tg[e] = sT[e / HNB][e % HNB];
 }
 
 // Zero a (rows x cols) block per batch element.
@@ -457,7 +505,7 @@ static hipError_t wb_gehrd_blocked(int n, cd* A, cd* V, cd* X, int batch, hipStr
   for (int k0 = 0; k0 < n - 2; k0 += HNB, ++p) {
     const int ib = (n - 2 - k0 < HNB) ? n - 2 - k0 : HNB;
     cd* Tp = T + (long long)p * HNB * HNB;
-    hipLaunchKernelGGL(k_hrd_panel, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, k0, ib,
+    hipLaunchKernelGGL(k_hrd_panel, dim3(batch), dim3(HNT), sizeof(cd) * (size_t)n, stream, n, k0, ib,
                        A, V, Y, Tp, nn);
     WB_CHECK_LAUNCH();
     const int top = k0 + 1;                 // rows 0..k0
