@@ -73,6 +73,10 @@ def main():
                     help='points of the gradient leg (0 skips it)')
     ap.add_argument('--serve-requests', type=int, default=48)
     ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
+    ap.add_argument('--autotune', type=int, default=1,
+                    help='tune the GEMM tile shapes of the benchmark steps before timing (0: defaults)')
+    ap.add_argument('--small-runs', type=int, default=50,
+                    help='runs of the 1D orders-(10,0) grating, eager vs captured graph (0 skips)')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='run one device copy of this size first (HBM counter calibration)')
     args = ap.parse_args()
@@ -128,6 +132,30 @@ def main():
     for _ in range(args.warmup):
         step(pts_d, thick_d, idx_d, ws)
     torch.cuda.synchronize()
+    # GEMM tile shapes: record the shapes of one batched step and one plain (one-point) step,
+    # then time every tile configuration on each and keep the fastest (results are bitwise the
+    # same for every configuration)
+    tuning = None
+    ws1 = None
+    if args.autotune:
+        _native.gemm_tuning_record(True)
+        step(pts_d, thick_d, idx_d, ws)
+        if args.plain_points > 0:
+            ws1 = torch.empty(_native.solve_workspace_bytes(1, L, hu, hv, _native.MODE_2D),
+                              dtype=torch.uint8, device=dev)
+            step(pts_d[:1].contiguous(), thick_d[:1].contiguous(), idx_d[:1].contiguous(), ws1)
+        _native.gemm_tuning_record(False)
+        t_tune = time.perf_counter()
+        n_tuned = _native.gemm_autotune(dev, reps=2)
+        t_tune = time.perf_counter() - t_tune
+        cfg_names = _native.gemm_tuning_configs()
+        counts = {}
+        for ln in _native.gemm_tuning_table().splitlines():
+            nm_ = cfg_names[int(ln.split()[-1])]
+            counts[nm_] = counts.get(nm_, 0) + 1
+        tuning = {'shapes_tuned': n_tuned, 'configs_chosen': counts, 'tuning_s': round(t_tune, 2)}
+        step(pts_d, thick_d, idx_d, ws)
+        torch.cuda.synchronize()
     names = list(_native.TIMER_IDS)
     _native.timing_enable(True)
     for n_ in names:
@@ -152,8 +180,9 @@ def main():
     plain_rate = None
     if args.plain_points > 0:
         plain_n = args.plain_points
-        ws1 = torch.empty(_native.solve_workspace_bytes(1, L, hu, hv, _native.MODE_2D),
-                          dtype=torch.uint8, device=dev)
+        if ws1 is None:
+            ws1 = torch.empty(_native.solve_workspace_bytes(1, L, hu, hv, _native.MODE_2D),
+                              dtype=torch.uint8, device=dev)
         step(pts_d[:1].contiguous(), thick_d[:1].contiguous(), idx_d[:1].contiguous(), ws1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -276,6 +305,64 @@ def main():
                         'stream': f'orders ({M},{M}), 1-4 layers of 64x64/128x128 rasters eps '
                                   'U[1,16]+iU[0,1], 1-8 wavelengths in [400,1000] nm, Poisson arrivals'}
 
+    # the launch-bound end: the 1D binary grating at orders (10,0) through Prepared.run, eager
+    # launch sequence vs the captured HIP graph it replays from its second run on
+    small = None
+    if args.small_runs > 0:
+        import wavebit_amd
+        from wavebit_amd import rcwa
+        g1 = workloads.binary_grating_1d()
+        p1 = g1['points'][0]
+
+        def prob1(wl_nm):
+            layers = [rcwa.Layer(thickness=float(t), epsilon=e) for t, e in zip(g1['thickness'], g1['rasters'])]
+            return rcwa.Problem(
+                stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(g1['eps_top']),
+                                 bottom=wavebit_amd.Material(g1['eps_bottom']), period=g1['period']),
+                excitation=rcwa.Excitation(wavelength=wl_nm, sources=(rcwa.PlaneWave(
+                    polarization='s', theta=p1['theta'], phi=None),)),
+                units=wavebit_amd.Units(length='nm'))
+        num1 = rcwa.Numerics(orders=g1['orders'])
+        timings = {}
+        for mode in ('eager', 'graph'):
+            prep = rcwa.prepare(prob1(p1['wavelength']), numerics=num1, backend='torch', device='cuda:0')
+            if mode == 'eager':
+                prep._cache.graph = None
+            for _ in range(3):
+                prep.run(prob1(p1['wavelength']))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(args.small_runs):
+                res1 = prep.run(prob1(p1['wavelength'] + 0.01 * i))
+            torch.cuda.synchronize()
+            timings[mode] = (time.perf_counter() - t0) / args.small_runs
+        # the device solve alone (engine level: inputs already on the device, one sync per solve)
+        e_conv = eng.ConvCache(dev).get('g1', torch.from_numpy(np.ascontiguousarray(g1['rasters'])).to(dev),
+                                        g1['orders'][0], g1['orders'][1])
+        bucket = eng.Bucket(hu=g1['orders'][0], hv=g1['orders'][1], path=eng.PATH_TE, n_layers=1)
+        bucket.points.append(eng.Point(stack=0, wavelength=p1['wavelength'], theta=p1['theta'], phi=None,
+                                       polarization='s', eps_top=g1['eps_top'], eps_bottom=g1['eps_bottom'],
+                                       period=g1['period'], thickness=tuple(g1['thickness'])))
+        dev_t = {}
+        for mode in ('eager', 'graph'):
+            slot = None if mode == 'eager' else [None, None]
+            for _ in range(3):
+                eng.run_bucket(bucket, [e_conv], dev, graph_slot=slot)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.small_runs):
+                eng.run_bucket(bucket, [e_conv], dev, graph_slot=slot)
+                torch.cuda.synchronize()
+            dev_t[mode] = (time.perf_counter() - t0) / args.small_runs
+        small = {'workload': '1D binary grating, period 700 nm, one 300 nm layer, orders (10,0) = 21 '
+                             'harmonics, one wavelength per run',
+                 'runs': args.small_runs,
+                 'lifecycle_eager_ms': round(timings['eager'] * 1e3, 4),
+                 'lifecycle_graph_ms': round(timings['graph'] * 1e3, 4),
+                 'solve_eager_ms': round(dev_t['eager'] * 1e3, 4),
+                 'solve_graph_ms': round(dev_t['graph'] * 1e3, 4),
+                 'R_sum': float(res1.R.sum()), 'T_sum': float(res1.T.sum())}
+
     fl_pt = workloads.flops_per_point((hu, hv), L)
     line = {
         'metric': 'RCWA solves/s @ (9,9) orders FP64 vs plain launch; p50/p99 serve latency',
@@ -294,6 +381,8 @@ def main():
         'roofline': roofline, 'cpu_baseline': cpu,
         'gradient': grad,
         'serve': serve_report,
+        'launch_bound': small,
+        'gemm_autotune': tuning,
         'p50_latency_s': serve_report['p50_latency_s'] if serve_report else None,
         'p99_latency_s': serve_report['p99_latency_s'] if serve_report else None,
         'oracle_max_abs_err_RT': oracle_err, 'flagged_points': n_flagged,

@@ -191,6 +191,33 @@ enum {
 int wb_timing_enable(int on);
 int wb_timing_read(int id, double* total_ms, long long* count, double* flops, int reset);
 
+/* GEMM tile-shape autotuning (no reference counterpart: the reference calls BLAS, whose tile
+ * choice is internal). Every batched complex GEMM of the engine picks one of
+ * wb_gemm_tuning_configs() tile configurations per shape (op(A), op(B), m, n, k, batch); all
+ * configurations accumulate in the same order and give bitwise identical results, so tuning
+ * changes speed only. Untuned shapes use configuration 0 (64 x 64).
+ *   wb_gemm_tuning_record(1)   subsequent launches record their shapes (0 stops recording);
+ *   wb_gemm_autotune_size()    scratch bytes the recorded, untuned shapes need;
+ *   wb_gemm_autotune(ctx, ...) times every configuration on each recorded, untuned shape on the
+ *                              caller's scratch (zeroed) and keeps the fastest; it waits for the
+ *                              stream, so call it outside any solve or graph capture. Returns
+ *                              the number of shapes tuned (>= 0) or a negative wb_status;
+ *   wb_gemm_tuning_table(out, cap) the table as text lines "opa opb m n k batch config",
+ *                              returns the bytes needed (out may be NULL);
+ *   wb_gemm_tuning_load(text)  merges such lines, returns how many were taken;
+ *   wb_gemm_tuning_set(...)    forces one shape's configuration (config < 0 removes it);
+ *   wb_gemm_tuning_clear()     forgets the table and the recorded shapes.
+ * The table is per process and shared by all contexts. */
+int wb_gemm_tuning_configs(void);
+const char* wb_gemm_tuning_config_name(int config);
+int wb_gemm_tuning_record(int on);
+size_t wb_gemm_autotune_size(void);
+int wb_gemm_autotune(wb_ctx* ctx, int reps, void* scratch, size_t scratch_bytes, void* stream);
+size_t wb_gemm_tuning_table(char* out, size_t cap);
+int wb_gemm_tuning_load(const char* text);
+int wb_gemm_tuning_set(int opa, int opb, int m, int n, int k, int batch, int config);
+void wb_gemm_tuning_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

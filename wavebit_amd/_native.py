@@ -27,7 +27,10 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_workspace_size', 'wb_solve', 'wb_zgemm_batched_ops', 'wb_zgetrs_batched_ops',
            'wb_solve_checkpoint_size', 'wb_solve_keep', 'wb_solve_adjoint_workspace_size',
            'wb_solve_adjoint', 'wb_assemble_li_adjoint_workspace_size', 'wb_assemble_li_adjoint',
-           'wb_timing_enable', 'wb_timing_read')
+           'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
+           'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
+           'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
+           'wb_gemm_tuning_clear')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9}
@@ -89,6 +92,15 @@ def load_library():
             'wb_assemble_li_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
                                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
             'wb_timing_enable': (c_int, [c_int]),
+            'wb_gemm_tuning_configs': (c_int, []),
+            'wb_gemm_tuning_config_name': (ctypes.c_char_p, [c_int]),
+            'wb_gemm_tuning_record': (c_int, [c_int]),
+            'wb_gemm_autotune_size': (c_sz, []),
+            'wb_gemm_autotune': (c_int, [c_vp, c_int, c_vp, c_sz, c_vp]),
+            'wb_gemm_tuning_table': (c_sz, [ctypes.c_char_p, c_sz]),
+            'wb_gemm_tuning_load': (c_int, [ctypes.c_char_p]),
+            'wb_gemm_tuning_set': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
+            'wb_gemm_tuning_clear': (None, []),
             'wb_timing_read': (c_int, [c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(c_ll), ctypes.POINTER(ctypes.c_double),
                                        c_int]),
@@ -393,6 +405,58 @@ def assemble_li_adjoint(eps, hu, hv, epx, epz_i, d_epx, d_epy, d_epz):
 
 
 # ----------------------------------------------------------------------------- timing
+
+
+# ----------------------------------------------------------------------------- GEMM autotuning
+
+def gemm_tuning_configs():
+    """Names of the GEMM tile configurations the autotuner chooses from (index = config id)."""
+    lib = load_library()
+    return [lib.wb_gemm_tuning_config_name(i).decode() for i in range(lib.wb_gemm_tuning_configs())]
+
+
+def gemm_tuning_record(on):
+    """Record (or stop recording) the shapes of subsequent GEMM launches."""
+    load_library().wb_gemm_tuning_record(1 if on else 0)
+
+
+def gemm_autotune(device, reps=3):
+    """Time every tile configuration on each recorded, untuned GEMM shape and keep the fastest.
+    Waits for the device; returns the number of shapes tuned."""
+    ctx = context(device)
+    nbytes = ctx.lib.wb_gemm_autotune_size()
+    scratch = _workspace(nbytes, device) if nbytes > 0 else None
+    r = ctx.lib.wb_gemm_autotune(ctx.handle, int(reps), _ptr(scratch), nbytes, _stream(device))
+    if r < 0:
+        ctx.check(-r, 'gemm_autotune')
+    torch.cuda.synchronize(device)
+    return int(r)
+
+
+def gemm_tuning_table():
+    """The tuning table as text lines 'opa opb m n k batch config'."""
+    lib = load_library()
+    n = lib.wb_gemm_tuning_table(None, 0)
+    buf = ctypes.create_string_buffer(int(n))
+    lib.wb_gemm_tuning_table(buf, n)
+    return buf.value.decode()
+
+
+def gemm_tuning_load(text):
+    """Merge table lines (as produced by ``gemm_tuning_table``); returns how many were taken."""
+    return int(load_library().wb_gemm_tuning_load(text.encode()))
+
+
+def gemm_tuning_set(opa, opb, m, n, k, batch, config):
+    """Force one GEMM shape's tile configuration (config < 0 removes the entry)."""
+    r = load_library().wb_gemm_tuning_set(opa, opb, m, n, k, batch, config)
+    if r != 0:
+        raise ValueError(f'gemm_tuning_set: bad arguments ({opa}, {opb}, {m}, {n}, {k}, {batch}, {config})')
+
+
+def gemm_tuning_clear():
+    """Forget the tuning table and the recorded shapes."""
+    load_library().wb_gemm_tuning_clear()
 
 def timing_enable(on=True):
     load_library().wb_timing_enable(1 if on else 0)

@@ -297,4 +297,35 @@ int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx
                     "wb_assemble_li_adjoint");
 }
 
+int wb_gemm_tuning_configs(void) { return wb_gemm_config_count(); }
+const char* wb_gemm_tuning_config_name(int config) { return wb_gemm_config_name(config); }
+int wb_gemm_tuning_record(int on) {
+  wb_gemm_record(on);
+  return WB_OK;
+}
+size_t wb_gemm_autotune_size(void) { return wb_gemm_autotune_bytes(); }
+int wb_gemm_autotune(wb_ctx* ctx, int reps, void* scratch, size_t scratch_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return -s;
+  if (reps < 1) return -fail(ctx, WB_BADARG, "wb_gemm_autotune: reps must be >= 1");
+  const size_t need = wb_gemm_autotune_bytes();
+  if (need > 0 && (!scratch || scratch_bytes < need))
+    return -fail(ctx, WB_OOM, "wb_gemm_autotune: scratch smaller than wb_gemm_autotune_size()");
+  hipError_t e = hipSuccess;
+  const int tuned = wb_gemm_autotune_recorded(reps, scratch, scratch_bytes, (hipStream_t)stream, &e);
+  if (tuned < 0) return -hip_status(ctx, e, "wb_gemm_autotune");
+  ctx->error.clear();
+  return tuned;
+}
+size_t wb_gemm_tuning_table(char* out, size_t cap) { return wb_gemm_table_text(out, cap); }
+int wb_gemm_tuning_load(const char* text) { return wb_gemm_table_load(text); }
+int wb_gemm_tuning_set(int opa, int opb, int m, int n, int k, int batch, int config) {
+  if (opa < 0 || opa > 1 || opb < 0 || opb > 1 || m < 0 || n < 0 || k < 0 || batch < 0 ||
+      config >= wb_gemm_config_count())
+    return WB_BADARG;
+  wb_gemm_table_set(opa, opb, m, n, k, batch, config);
+  return WB_OK;
+}
+void wb_gemm_tuning_clear(void) { wb_gemm_table_clear(); }
+
 }  // extern "C"

@@ -1,5 +1,5 @@
 // Windowed small-bulge multishift QR for the complex Hessenberg eigenproblem, with Schur
-// vectors (one 256-thread workgroup per matrix).
+// vectors (one 512-thread workgroup per matrix).
 //
 // A multishift sweep chases M single-shift bulges (2-element reflectors) down the active block
 // [lo, hi] in a tightly packed chain: bulge j enters at step 3j and at step s acts at position
@@ -13,8 +13,8 @@
 // always touches band slots 3(M-1-j) and 3(M-1-j)+1 of the band [lo+s-3(M-1), lo+s+1], so a
 // strip of H or Z streams through B = 3(M-1)+2 registers with compile-time indices, each element
 // read and written once per window (H right of and above the window) or once per sweep (Z).
-// Shifts are the eigenvalues of the trailing M x M block; small active blocks finish with the
-// single-shift iteration.
+// Shifts are the eigenvalues of the trailing M x M block; active blocks of at most nmin rows
+// finish with a single-shift iteration held in LDS by one wave (small_block).
 #include "wb_internal.h"
 #include "wb_timing.h"
 #include <float.h>
@@ -198,37 +198,65 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
   return r;
 }
 
-// Single-shift iteration on the active block [lo_lim, hi] until all its eigenvalues converge
-// (the zlahqr scheme, rotations applied straight to global H and Z). Returns 0 on success.
-__device__ int single_shift_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo_lim, int hi, int* s_k) {
-  const int t = threadIdx.x;
-#define HH(r, c) H[(long long)(r) * n + (c)]
-  int i = hi;
+// Active blocks of at most nmin rows. The block's single-shift QR (the zlahqr scheme) runs in
+// LDS on one wave with no barriers, every reflector also applied to U (the block's accumulated
+// unitary); the rows of H above the block, its columns right of the block and the Schur vectors
+// then get U in three small products with all threads. Returns 0, or i + 1 if eigenvalue i of
+// the block did not converge.
+__device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
+  const int lane = threadIdx.x & 63;
+  const double smlnum = SAFMIN * ((double)n / ULP);
+  int i = m - 1;
   int kdefl = 0;
-  const int itmax = 30 * ((hi - lo_lim + 1) > 10 ? (hi - lo_lim + 1) : 10);
-  while (i >= lo_lim) {
-    int l = lo_lim;
+  const int itmax = 30 * (m > 10 ? m : 10);
+#define TT(r, c) T[(r) * ld + (c)]
+  while (i >= 0) {
+    int l = 0;
     bool converged = false;
     for (int its = 0; its <= itmax; ++its) {
-      l = find_split(H, n, l, i, s_k);
-      if (t == 0 && l > 0) HH(l, l - 1) = mkc(0.0, 0.0);
+      // largest k in (l, i] with a negligible subdiagonal (Ahues-Tisseur), lane per k
+      const int k = l + 1 + lane;
+      bool small = false;
+      if (k <= i) {
+        const cd hk = TT(k, k - 1);
+        small = cabs1(hk) <= smlnum;
+        if (!small) {
+          double tst = cabs1(TT(k - 1, k - 1)) + cabs1(TT(k, k));
+          if (tst == 0.0) {
+            if (k - 2 >= 0) tst += fabs(TT(k - 1, k - 2).x);
+            if (k + 1 <= m - 1) tst += fabs(TT(k + 1, k).x);
+          }
+          if (cabs1(hk) <= ULP * tst) {
+            const double h1 = cabs1(hk), h2 = cabs1(TT(k - 1, k));
+            const double ab = fmax(h1, h2), ba = fmin(h1, h2);
+            const double g1 = cabs1(TT(k, k));
+            const double g2 = cabs1(csub(TT(k - 1, k - 1), TT(k, k)));
+            const double aa = fmax(g1, g2), bb = fmin(g1, g2);
+            const double sc = aa + ab;
+            small = ba * (ab / sc) <= fmax(smlnum, ULP * (bb * (aa / sc)));
+          }
+        }
+      }
+      const unsigned long long mask = __ballot(small);
+      l = mask ? l + 1 + (63 - __clzll(mask)) : l;
+      if (l > 0 && lane == 0) TT(l, l - 1) = mkc(0.0, 0.0);
       if (l >= i) { converged = true; break; }
       ++kdefl;
       cd sh;
       if (kdefl % 20 == 0) {
-        sh = cadd(mkc(0.75 * fabs(HH(i, i - 1).x), 0.0), HH(i, i));
+        sh = cadd(mkc(0.75 * fabs(TT(i, i - 1).x), 0.0), TT(i, i));
       } else if (kdefl % 10 == 0) {
-        sh = cadd(mkc(0.75 * fabs(HH(l + 1, l).x), 0.0), HH(l, l));
+        sh = cadd(mkc(0.75 * fabs(TT(l + 1, l).x), 0.0), TT(l, l));
       } else {
-        sh = HH(i, i);
-        const cd u = cmul(csqrt_p(HH(i - 1, i)), csqrt_p(HH(i, i - 1)));
-        double s = cabs1(u);
-        if (s != 0.0) {
-          const cd x = cscale(csub(HH(i - 1, i - 1), sh), 0.5);
+        sh = TT(i, i);
+        const cd u = cmul(csqrt_p(TT(i - 1, i)), csqrt_p(TT(i, i - 1)));
+        double su = cabs1(u);
+        if (su != 0.0) {
+          const cd x = cscale(csub(TT(i - 1, i - 1), sh), 0.5);
           const double sx = cabs1(x);
-          s = fmax(s, sx);
-          const cd xs = cscale(x, 1.0 / s), us = cscale(u, 1.0 / s);
-          cd y = cscale(csqrt_p(cadd(cmul(xs, xs), cmul(us, us))), s);
+          su = fmax(su, sx);
+          const cd xs = cscale(x, 1.0 / su), us = cscale(u, 1.0 / su);
+          cd y = cscale(csqrt_p(cadd(cmul(xs, xs), cmul(us, us))), su);
           if (sx > 0.0) {
             const cd xn = cscale(x, 1.0 / sx);
             if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
@@ -236,60 +264,134 @@ __device__ int single_shift_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo_
           sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
         }
       }
-      for (int k = l; k < i; ++k) {
+      for (int kk = l; kk < i; ++kk) {
         cd v1, v2;
-        if (k == l) {
-          v1 = csub(HH(l, l), sh);
-          v2 = HH(l + 1, l);
-          const double s = cabs1(v1) + cabs1(v2);
-          if (s != 0.0) { v1 = cscale(v1, 1.0 / s); v2 = cscale(v2, 1.0 / s); }
+        if (kk == l) {
+          v1 = csub(TT(l, l), sh);
+          v2 = TT(l + 1, l);
+          const double sc = cabs1(v1) + cabs1(v2);
+          if (sc != 0.0) { v1 = cscale(v1, 1.0 / sc); v2 = cscale(v2, 1.0 / sc); }
         } else {
-          v1 = HH(k, k - 1);
-          v2 = HH(k + 1, k - 1);
+          v1 = TT(kk, kk - 1);
+          v2 = TT(kk + 1, kk - 1);
         }
         Rot r;
         double beta;
         refl2(v1, v2, &r.tau, &r.u, &beta);
-        __syncthreads();
-        if (k > l && t == 0) {
-          HH(k, k - 1) = mkc(beta, 0.0);
-          HH(k + 1, k - 1) = mkc(0.0, 0.0);
+        if (kk > l && lane == 0) {
+          TT(kk, kk - 1) = mkc(beta, 0.0);
+          TT(kk + 1, kk - 1) = mkc(0.0, 0.0);
         }
-        if (r.tau.x != 0.0 || r.tau.y != 0.0) {
-          for (int j = k + t; j < n; j += NT) rot_left(r, HH(k, j), HH(k + 1, j));
-          __syncthreads();
-          const int rmax = (k + 2 < i) ? k + 2 : i;
-          for (int q = t; q <= rmax; q += NT) rot_right(r, HH(q, k), HH(q, k + 1));
-          if (zh) {
-            for (int q = t; q < n; q += NT)
-              rot_left(r, Z[(long long)k * n + q], Z[(long long)(k + 1) * n + q]);
-          } else {
-            for (int q = t; q < n; q += NT) {
-              cd* zr = Z + (long long)q * n + k;
-              rot_right(r, zr[0], zr[1]);
-            }
-          }
-        }
-        __syncthreads();
+        // row operation on columns kk..m-1, then the column operation on rows 0..min(kk+2, i)
+        // and on U (LDS accesses of one wave stay in program order)
+        if (lane >= kk && lane < m) rot_left(r, TT(kk, lane), TT(kk + 1, lane));
+        const int rmax = (kk + 2 < i) ? kk + 2 : i;
+        if (lane <= rmax) rot_right(r, TT(lane, kk), TT(lane, kk + 1));
+        if (lane < m) rot_right(r, U[lane * ld + kk], U[lane * ld + kk + 1]);
       }
     }
     if (!converged) return i + 1;
-    if (t == 0) wout[i] = HH(i, i);
+    if (lane == 0) w[i] = TT(i, i);
     kdefl = 0;
     i = l - 1;
-    __syncthreads();
   }
-#undef HH
+#undef TT
   return 0;
 }
 
-// Apply the steps [s0, s1) of a group of G consecutive bulges' reflectors to one strip through
-// a B-register band. left: row operations on a column strip (rows), else column operations on
-// a row strip. get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid
-// range. `lo` is the group's entry position (lo - 3 g0 for the group starting at bulge g0) and
-// `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
-// only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
-// another in bulge order: the result equals the step-by-step order of the chase.
+// A (rows x m, row stride lda) := A U, U m x m in LDS (ldu): a wave per group of four rows,
+// lane j computes column j; the group's outputs are written after a barrier.
+__device__ void rows_times_u(cd* A, int lda, int rows, int m, const cd* U, int ldu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int R = 4;
+  for (int q0 = 0; q0 < rows; q0 += R * (NT / 64)) {
+    cd acc[R];
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      acc[rr] = mkc(0.0, 0.0);
+      const int q = q0 + wave * R + rr;
+      if (q < rows && lane < m) {
+        const cd* a = A + (long long)q * lda;
+        for (int i = 0; i < m; ++i) acc[rr] = cadd(acc[rr], cmul(a[i], U[i * ldu + lane]));
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      const int q = q0 + wave * R + rr;
+      if (q < rows && lane < m) A[(long long)q * lda + lane] = acc[rr];
+    }
+  }
+  __syncthreads();
+}
+
+// B (m x cols, row stride ldb) := U^H B: lanes on 64 consecutive columns, wave w computes the
+// output rows j = w, w + 8, ...; written after a barrier.
+__device__ void uh_times_cols(cd* B, int ldb, int m, int cols, const cd* U, int ldu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NW = NT / 64;
+  constexpr int JM = (48 + NW - 1) / NW;   // output rows per wave (m <= 48)
+  for (int c0 = 0; c0 < cols; c0 += 64) {
+    const int c = c0 + lane;
+    cd acc[JM];
+#pragma unroll
+    for (int jj = 0; jj < JM; ++jj) acc[jj] = mkc(0.0, 0.0);
+    if (c < cols) {
+      for (int i = 0; i < m; ++i) {
+        const cd b = B[(long long)i * ldb + c];
+#pragma unroll
+        for (int jj = 0; jj < JM; ++jj) {
+          const int j = wave + NW * jj;
+          if (j < m) acc[jj] = cadd(acc[jj], cmulc(U[i * ldu + j], b));
+        }
+      }
+    }
+    __syncthreads();
+    if (c < cols) {
+#pragma unroll
+      for (int jj = 0; jj < JM; ++jj) {
+        const int j = wave + NW * jj;
+        if (j < m) B[(long long)j * ldb + c] = acc[jj];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+static_assert(3 * WB_HQR_M + 8 <= 48, "small blocks (at most nmin = 48 rows) must fit one wave and the LDS window");
+__device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo, int hi, cd* lds,
+                                                     int* s_info) {
+  ASSUME_LDS(lds);
+  ASSUME_LDS(s_info);
+  const int t = threadIdx.x;
+  const int m = hi - lo + 1;
+  const int ld = (m % 2) ? m : m + 1;      // odd row stride: column accesses spread over banks
+  cd* T = lds;
+  cd* U = lds + m * ld;
+  for (int e = t; e < m * m; e += NT) {
+    const int r = e / m, c = e % m;
+    T[r * ld + c] = H[(long long)(lo + r) * n + lo + c];
+    U[r * ld + c] = mkc(r == c ? 1.0 : 0.0, 0.0);
+  }
+  if (t == 0) *s_info = 0;
+  __syncthreads();
+  if (t < 64) {
+    const int info = wave_qr(T, U, m, ld, n, wout + lo);
+    if (t == 0) *s_info = info ? lo + info : 0;
+  }
+  __syncthreads();
+  const int info = *s_info;
+  for (int e = t; e < m * m; e += NT) {
+    const int r = e / m, c = e % m;
+    H[(long long)(lo + r) * n + lo + c] = T[r * ld + c];
+  }
+  rows_times_u(H + lo, n, lo, m, U, ld);                              // rows above the block
+  uh_times_cols(H + (long long)lo * n + hi + 1, n, m, n - hi - 1, U, ld);   // right of the block
+  if (zh) uh_times_cols(Z + (long long)lo * n, n, m, n, U, ld);        // Z^H rows lo..hi
+  else rows_times_u(Z + lo, n, n, m, U, ld);                          // Z columns lo..hi
+  return info;
+}
+
 // Compiler fence on values: the asm "reads and writes" x, so everything computing x is
 // emitted before it and everything using x after it (no instruction is generated).
 __device__ inline void pin(cd& x) { asm volatile("" : "+v"(x.x), "+v"(x.y)); }
@@ -322,6 +424,13 @@ __device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, cd v) {
 }
 
 
+// Apply the steps [s0, s1) of a group of G consecutive bulges' reflectors to one strip through
+// a B-register band. left: row operations on a column strip (rows), else column operations on
+// a row strip. get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid
+// range. `lo` is the group's entry position (lo - 3 g0 for the group starting at bulge g0) and
+// `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
+// only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
+// another in bulge order: the result equals the step-by-step order of the chase.
 template <int M, bool LEFT, int S>
 __device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, const Rot* rots, int rot_stride,
                                                      int vlo, int vhi, const cd* base, int nbytes, int estride,
@@ -645,7 +754,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       continue;
     }
     if (hi - lo + 1 <= nmin) {
-      const int r = single_shift_block(H, Z, zh, wout, n, lo, hi, s_k);
+      const int r = small_block(H, Z, zh, wout, n, lo, hi, Hw, &s_k[0]);
       if (r != 0) {
         if (t == 0) info[b] = r;
         return;

@@ -10,6 +10,16 @@ hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipi
                            int ldb, int batch, hipStream_t stream);
 hipError_t wb_getrs_conj_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
                                 int ldb, int batch, hipStream_t stream);
+// GEMM tile-shape autotuner (wb_gemm.hip)
+void wb_gemm_record(int on);
+size_t wb_gemm_autotune_bytes();
+int wb_gemm_autotune_recorded(int reps, void* scratch, size_t bytes, hipStream_t stream, hipError_t* err);
+size_t wb_gemm_table_text(char* out, size_t cap);
+int wb_gemm_table_load(const char* text);
+void wb_gemm_table_clear();
+void wb_gemm_table_set(int opa, int opb, int m, int n, int k, int batch, int cfg);
+const char* wb_gemm_config_name(int cfg);
+int wb_gemm_config_count();
 hipError_t wb_zgemm_ops_launch(int opa, int opb, int m, int n, int k, cd alpha, BatchPtr A, int lda,
                                BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc, int batch,
                                hipStream_t stream);

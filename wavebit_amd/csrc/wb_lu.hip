@@ -83,7 +83,6 @@ __global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, Batc
 // buffered), and a rank-1 update of each thread's own row, with no global traffic inside the
 // panel.
 constexpr int PNT = 768;
-constexpr int PNW = PNT / 64;
 struct PanelCtx {
   int n, k0, jb, r, lane, wv, t;
   bool own;
@@ -96,7 +95,7 @@ struct PanelCtx {
   cd (*s_jrow)[WB_NB];
 };
 // Column C of the register-resident panel (compile-time C keeps x[] in registers).
-template <int C>
+template <int C, int PNW>
 __device__ __forceinline__ void panel_col(cd (&x)[WB_NB], const PanelCtx& P) {
   if (C < P.jb) {
     const int j = P.k0 + C, r = P.r;
@@ -147,11 +146,14 @@ __device__ __forceinline__ void panel_col(cd (&x)[WB_NB], const PanelCtx& P) {
       for (int cc = C + 1; cc < WB_NB; ++cc) x[cc] = csub(x[cc], cmul(l, P.s_prow[buf][cc]));
     }
   }
-  if constexpr (C + 1 < WB_NB) panel_col<C + 1>(x, P);
+  if constexpr (C + 1 < WB_NB) panel_col<C + 1, PNW>(x, P);
 }
 
-__global__ __launch_bounds__(PNT) void k_getrf_panel_reg(int n, int k0, int jb, BatchPtr A, int lda,
+// NTP threads (one per row): 768, or one wave for n <= 64, where the barriers cost nothing.
+template <int NTP>
+__global__ __launch_bounds__(NTP) void k_getrf_panel_reg(int n, int k0, int jb, BatchPtr A, int lda,
                                                          int* ipiv, int* info) {
+  constexpr int PNW = NTP / 64;
   __shared__ double s_v[PNW];
   __shared__ int s_i[PNW];
   __shared__ cd s_prow[2][WB_NB];
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(PNT) void k_getrf_panel_reg(int n, int k0, int jb, 
   cd x[WB_NB];
 #pragma unroll
   for (int c = 0; c < WB_NB; ++c) x[c] = (P.own && c < jb) ? a[(long long)P.r * lda + k0 + c] : mkc(0.0, 0.0);
-  panel_col<0>(x, P);
+  panel_col<0, PNW>(x, P);
   if (P.own) {
 #pragma unroll
     for (int c = 0; c < WB_NB; ++c)
@@ -324,10 +326,14 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
   WB_CHECK_LAUNCH();
   for (int k0 = 0; k0 < n; k0 += WB_NB) {
     const int jb = (n - k0 < WB_NB) ? n - k0 : WB_NB;
-    if (n <= PNT) {
+    if (n <= 64) {
       WB_TIMED(WB_T_GETRF_PANEL, stream,
-               hipLaunchKernelGGL(k_getrf_panel_reg, dim3(batch), dim3(PNT), 0, stream, n, k0, jb, A, lda,
+               hipLaunchKernelGGL(k_getrf_panel_reg<64>, dim3(batch), dim3(64), 0, stream, n, k0, jb, A, lda,
                                   ipiv, info));
+    } else if (n <= PNT) {
+      WB_TIMED(WB_T_GETRF_PANEL, stream,
+               hipLaunchKernelGGL(k_getrf_panel_reg<PNT>, dim3(batch), dim3(PNT), 0, stream, n, k0, jb, A,
+                                  lda, ipiv, info));
     } else {
       WB_TIMED(WB_T_GETRF_PANEL, stream,
                hipLaunchKernelGGL(k_getrf_panel, dim3(batch), dim3(256), 0, stream, n, k0, jb, A, lda,

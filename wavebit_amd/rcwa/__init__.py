@@ -5,7 +5,8 @@ from .sources import PlaneWave
 from .numerics import Numerics
 from .results import DiffractionOrders, Result
 from ._lifecycle import CheckReport, Execution, Prepared, check, prepare, run
+from .tuning import autotune     # extension: GEMM tile-shape autotuning of a problem's solve
 
 __all__ = ['Excitation', 'Layer', 'Outputs', 'Problem', 'Region', 'Stack', 'PlaneWave', 'Numerics',
            'DiffractionOrders', 'Result', 'CheckReport', 'Execution', 'Prepared', 'check',
-           'prepare', 'run']
+           'prepare', 'run', 'autotune']

@@ -138,10 +138,13 @@ def point_record(pt, path):
             a_s.real, a_s.imag, a_p.real, a_p.imag, 0.0, 0.0]
 
 
-def run_bucket(bucket, conv, device, workspace=None):
-    """Solve every point of a bucket; returns host numpy arrays (eff, amp, pe, flags).
+def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
+    """Solve every point of a bucket; returns device tensors (eff, amp, pe, flags) and N.
 
     ``conv`` is a list of (epx, epy, epz_i) per stack, each (L, N, N) on the device.
+    ``graph_slot`` (a two-element list [graph, last shapes], or None) holds a
+    ``graphs.SolveGraph`` reused while the shapes match: launch-bound sizes run eagerly the first
+    time, are captured the second time the same shapes come, and are replayed afterwards.
     """
     P, L = len(bucket.points), bucket.n_layers
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
@@ -155,8 +158,23 @@ def run_bucket(bucket, conv, device, workspace=None):
     pts_d = torch.from_numpy(rec).to(device)
     thick_d = torch.from_numpy(thick).to(device)
     idx_d = torch.from_numpy(idx).to(device)
+    mode = _MODE[bucket.path]
+    if graph_slot is not None:
+        from . import graphs
+        g = graph_slot[0]
+        if g is not None and g.matches(pts_d, thick_d, epx, bucket.hu, bucket.hv, mode):
+            eff, amp, pe, flags = g.run(pts_d, thick_d, idx_d, epx, epy, epz)
+            return eff, amp, pe, flags, N
+        key = (tuple(pts_d.shape), tuple(thick_d.shape), tuple(epx.shape), bucket.hu, bucket.hv, mode)
+        if graph_slot[1] == key and graphs.worth_capturing(P, L, bucket.hu, bucket.hv, mode):
+            # the second run of the same shapes: capture, then replay from now on
+            g = graphs.SolveGraph(pts_d, thick_d, idx_d, epx, epy, epz, bucket.hu, bucket.hv, mode)
+            graph_slot[0] = g
+            eff, amp, pe, flags = g.run(pts_d, thick_d, idx_d, epx, epy, epz)
+            return eff, amp, pe, flags, N
+        graph_slot[1] = key
     eff, amp, pe, flags = _native.solve(pts_d, thick_d, idx_d, epx, epy, epz, bucket.hu,
-                                        bucket.hv, _MODE[bucket.path], workspace)
+                                        bucket.hv, mode, workspace)
     return eff, amp, pe, flags, N
 
 

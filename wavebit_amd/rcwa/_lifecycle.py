@@ -336,6 +336,7 @@ class _Cache:
 
     def __init__(self, device):
         self.conv = _engine.ConvCache(device)
+        self.graph = [None, None]    # captured solve of this preparation (launch-bound sizes)
 
     def get(self, rasters, hu, hv):
         import torch
@@ -474,7 +475,7 @@ def _solve_resolved(r, cache):
             stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
             period=r.period, thickness=tuple(float(t) for t in r.thickness)))
-    eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device)
+    eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
     eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
     per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
     return per_wl, flags
@@ -503,7 +504,8 @@ def prepare(problem, *, numerics, backend='numpy', device='cpu', precision='doub
         'live': ('raster values', 'half-space materials', 'wavelength values', 'incident angles',
                  'polarization', 'layer thickness'),
         'cache_policy': 'convolution matrices stay on the device per distinct raster stack; '
-                        'modes and propagation are rebuilt every run'})
+                        'modes and propagation are rebuilt every run; launch-bound sizes replay '
+                        'a captured HIP graph of the solve'})
     return Prepared(numerics=numerics, execution=execution, report=report, contract=contract,
                     _structure=_structure_key(r), _cache=_Cache(device))
 
