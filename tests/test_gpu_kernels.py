@@ -25,7 +25,8 @@ def test_zgemm_matches_numpy(m, n, k, batch):
     assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max()) * max(1, k) ** 0.5
 
 
-@pytest.mark.parametrize('n,nrhs,batch', [(1, 1, 1), (5, 3, 2), (33, 40, 3), (97, 97, 2), (450, 8, 2)])
+@pytest.mark.parametrize('n,nrhs,batch', [(1, 1, 1), (5, 3, 2), (33, 40, 3), (97, 97, 2), (450, 8, 2),
+                                          (1000, 5, 1)])
 def test_lu_solve_matches_numpy(n, nrhs, batch):
     rng = np.random.default_rng(n)
     A = crand(rng, batch, n, n)
@@ -63,7 +64,7 @@ def _check_eig(A, w, W):
 
 
 @pytest.mark.parametrize('n,batch', [(1, 1), (2, 2), (3, 1), (10, 3), (50, 2), (98, 2), (257, 1),
-                                     (450, 2), (722, 1)])
+                                     (450, 2), (722, 1), (1250, 1)])
 def test_zgeev_random(n, batch):
     rng = np.random.default_rng(100 + n)
     A = crand(rng, batch, n, n)

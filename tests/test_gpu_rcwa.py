@@ -52,6 +52,8 @@ CASES = [
     (1, 5, 5, (2, 2), 0.0, 0.0, (1.0, 0.0), 1.0, 1.0),
     (2, 6, 6, (2, 3), 0.3, 0.4, (0.4, 0.3 + 0.2j), 1.0, 2.25 + 0.1j),
     (3, 12, 10, (3, 3), 0.5, 2.0, 'p', 1.44, 4.0),
+    # n = 1250 > 768: the global-memory LU panel, 20 blocks of the blocked solves, windowed QR
+    (1, 16, 16, (12, 12), 0.2, 0.3, 'p', 1.0, 2.25),
 ]
 
 
