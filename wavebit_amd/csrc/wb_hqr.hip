@@ -10,11 +10,15 @@
 //
 // The chase runs in a W x W diagonal window held in LDS. Everything outside the window is
 // deferred and applied from the recorded reflectors by a register "band": in step s, bulge j
-// always touches band slots 3(M-1-j) and 3(M-1-j)+1 of the band [lo+s-3(M-1), lo+s+1], so a
-// strip of H or Z streams through B = 3(M-1)+2 registers with compile-time indices, each element
-// read and written once per window (H right of and above the window) or once per sweep (Z).
-// Shifts are the eigenvalues of the trailing M x M block; active blocks of at most nmin rows
-// finish with a single-shift iteration held in LDS by one wave (small_block).
+// always touches band slots 3(G-1-j) and 3(G-1-j)+1 of its group's band [lo+s-3(G-1), lo+s+1]
+// (G = 8 bulges per group), so a strip of H or Z streams through B = 3(G-1)+2 registers with
+// compile-time indices. With M = 16 the two groups of a strip run on the two half-waves in
+// lockstep (band_apply_pair), so each element is read and written once per window (H right of
+// and above the window) or once per sweep (Z) for all 16 bulges. The window's reflectors are
+// written to global memory during the chase and staged back into the window's LDS for the
+// deferred updates. Shifts are the eigenvalues of the trailing M x M block (one wave's QR
+// iteration, wave_qr); active blocks of at most nmin rows finish with a single-shift iteration
+// held in LDS by one wave (small_block).
 #include "wb_internal.h"
 #include "wb_timing.h"
 #include <float.h>
@@ -26,15 +30,11 @@ namespace {
 #define WB_HQR_NT 512      // threads per matrix
 #endif
 #ifndef WB_HQR_M
-#define WB_HQR_M 8         // bulges (shifts) per sweep
+#define WB_HQR_M 16        // bulges (shifts) per sweep: 8 or 16
 #endif
 constexpr int NT = WB_HQR_NT;
 static_assert(NT % 64 == 0, "whole waves");
-constexpr int BG = (WB_HQR_M < 8) ? WB_HQR_M : 8;   // bulges per register band of the deferred updates
-#ifndef WB_HQR_STRIPS
-#define WB_HQR_STRIPS 1   // strips per thread in the deferred updates
-#endif
-constexpr int HS = WB_HQR_STRIPS;
+
 
 constexpr double ULP = DBL_EPSILON;
 
@@ -98,67 +98,6 @@ __device__ inline void rot_right(const Rot& r, cd& a0, cd& a1) {
   a0.y -= sy;
   a1.x = fma(-sx, u.x, fma(-sy, u.y, a1.x));
   a1.y = fma(-sy, u.x, fma(sx, u.y, a1.y));
-}
-
-// Eigenvalues of a small upper Hessenberg matrix (row-major, ld) by single-shift QR, in
-// place; one thread. Returns 0 on success.
-__device__ int small_eigvals(cd* A, int m, int ld, cd* out) {
-  const double smlnum = SAFMIN * ((double)m / ULP);
-  int i = m - 1;
-  int kdefl = 0;
-  while (i >= 0) {
-    int l = 0;
-    bool conv = false;
-    for (int its = 0; its <= 30 * 10; ++its) {
-      int k;
-      for (k = i; k > l; --k) {
-        const cd hk = A[k * ld + k - 1];
-        if (cabs1(hk) <= smlnum) break;
-        double tst = cabs1(A[(k - 1) * ld + k - 1]) + cabs1(A[k * ld + k]);
-        if (cabs1(hk) <= ULP * tst) break;
-      }
-      l = k;
-      if (l > 0) A[l * ld + l - 1] = mkc(0.0, 0.0);
-      if (l >= i) { conv = true; break; }
-      ++kdefl;
-      cd sh = A[i * ld + i];
-      if (kdefl % 10 == 0) {
-        sh = cadd(sh, mkc(0.75 * cabs1(A[i * ld + i - 1]), 0.0));
-      } else {
-        const cd u = cmul(csqrt_p(A[(i - 1) * ld + i]), csqrt_p(A[i * ld + i - 1]));
-        double s = cabs1(u);
-        if (s != 0.0) {
-          const cd x = cscale(csub(A[(i - 1) * ld + i - 1], sh), 0.5);
-          const double sx = cabs1(x);
-          s = fmax(s, sx);
-          cd y = cscale(csqrt_p(cadd(cmul(cscale(x, 1.0 / s), cscale(x, 1.0 / s)),
-                                     cmul(cscale(u, 1.0 / s), cscale(u, 1.0 / s)))), s);
-          if (sx > 0.0) {
-            const cd xn = cscale(x, 1.0 / sx);
-            if (xn.x * y.x + xn.y * y.y < 0.0) y = cneg(y);
-          }
-          sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
-        }
-      }
-      for (int kk = l; kk < i; ++kk) {
-        cd x1, x2;
-        if (kk == l) { x1 = csub(A[l * ld + l], sh); x2 = A[(l + 1) * ld + l]; }
-        else { x1 = A[kk * ld + kk - 1]; x2 = A[(kk + 1) * ld + kk - 1]; }
-        Rot r;
-        double beta;
-        refl2(x1, x2, &r.tau, &r.u, &beta);
-        if (kk > l) { A[kk * ld + kk - 1] = mkc(beta, 0.0); A[(kk + 1) * ld + kk - 1] = mkc(0.0, 0.0); }
-        for (int j = kk; j <= i; ++j) rot_left(r, A[kk * ld + j], A[(kk + 1) * ld + j]);
-        const int rmax = (kk + 2 < i) ? kk + 2 : i;
-        for (int q = l; q <= rmax; ++q) rot_right(r, A[q * ld + kk], A[q * ld + kk + 1]);
-      }
-    }
-    if (!conv) return 1;
-    out[i] = A[i * ld + i];
-    kdefl = 0;
-    i = l - 1;
-  }
-  return 0;
 }
 
 // Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo.
@@ -287,7 +226,7 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
         if (lane >= kk && lane < m) rot_left(r, TT(kk, lane), TT(kk + 1, lane));
         const int rmax = (kk + 2 < i) ? kk + 2 : i;
         if (lane <= rmax) rot_right(r, TT(lane, kk), TT(lane, kk + 1));
-        if (lane < m) rot_right(r, U[lane * ld + kk], U[lane * ld + kk + 1]);
+        if (U && lane < m) rot_right(r, U[lane * ld + kk], U[lane * ld + kk + 1]);
       }
     }
     if (!converged) return i + 1;
@@ -330,7 +269,7 @@ __device__ void rows_times_u(cd* A, int lda, int rows, int m, const cd* U, int l
 __device__ void uh_times_cols(cd* B, int ldb, int m, int cols, const cd* U, int ldu) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int NW = NT / 64;
-  constexpr int JM = (48 + NW - 1) / NW;   // output rows per wave (m <= 48)
+  constexpr int JM = (64 + NW - 1) / NW;   // output rows per wave (m <= 64)
   for (int c0 = 0; c0 < cols; c0 += 64) {
     const int c = c0 + lane;
     cd acc[JM];
@@ -358,7 +297,7 @@ __device__ void uh_times_cols(cd* B, int ldb, int m, int cols, const cd* U, int 
   __syncthreads();
 }
 
-static_assert(3 * WB_HQR_M + 8 <= 48, "small blocks (at most nmin = 48 rows) must fit one wave and the LDS window");
+static_assert(3 * WB_HQR_M + 8 <= 64, "small blocks (at most nmin rows) must fit one wave");
 __device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo, int hi, cd* lds,
                                                      int* s_info) {
   ASSUME_LDS(lds);
@@ -548,13 +487,141 @@ __device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, con
     for (int k = 0; k < S; ++k) buf_store(rsrc, off(k, base_end + i, vhi), band[k][i]);
 }
 
-// Calls f(x, S) for this thread's strips x = first, first + NT, ... < end: SB strips at a time
-// (x, x + NT, ..., x + (SB-1) NT), then the remainder one at a time.
-template <int SB, typename F>
-__device__ inline void for_strips(int first, int end, F f) {
-  int x = first;
-  for (; x + (SB - 1) * NT < end; x += SB * NT) f(x, std::integral_constant<int, SB>());
-  for (; x < end; x += NT) f(x, std::integral_constant<int, 1>());
+// Two bulge groups of 8 on one strip, one group per half-wave: lane l (group 0, bulges 0..7)
+// and lane l + 32 (group 1, bulges 8..15) hold the two bands of strip l, in lockstep. Group 1's
+// band trails group 0's by 24 positions, so the element that leaves group 0's band at step s
+// enters group 1's at step s + 2: it crosses to the other half-wave (a lane swap) and waits in
+// a two-element queue. The result equals the two groups' passes one after the other (group 1
+// only meets elements group 0 no longer touches), with each strip element read and written
+// once per pass for all 16 bulges. `active` is false for lanes past the last strip (their
+// loads and stores go to BUF_OOB).
+// Value select of two complex numbers (a conditional on the values, not on their addresses, so
+// the register arrays they come from stay in registers).
+__device__ inline cd csel(bool c, cd a, cd b) { return mkc(c ? a.x : b.x, c ? a.y : b.y); }
+
+template <bool LEFT>
+__device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1, const Rot* rots, int rot_stride,
+                                                          int vlo, int vhi, const cd* base, int nbytes,
+                                                          int estride, int soff0, bool active) {
+  ASSUME_LDS(rots);
+  constexpr int G = 8;                    // bulges per group
+  constexpr int B = 3 * (G - 1) + 2;
+  constexpr int U = 3;
+  constexpr int BC = 4;
+  constexpr int CPS = G / BC;
+  constexpr int NCH = U * CPS;
+  const int lane = threadIdx.x & 63;
+  const bool g1 = lane >= 32;
+  const int lo_g = g1 ? lo - 3 * G : lo;
+  const Rot* rots_g = rots + (g1 ? G : 0);
+  const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc(base, nbytes);
+  cd band[B + U - 1];
+  cd pre[U];
+  cd carried[2];
+  const int base0 = lo_g + s0 - 3 * (G - 1);
+  const int top_last = base0 + (s1 - 1 - s0) + B - 1;   // last element group 0 reads
+  const int hi_read = (top_last + 1 < vhi) ? top_last + 1 : vhi;
+  auto off = [&](int idx, int lim) -> unsigned {
+    return (active && idx >= vlo && idx < lim) ? (unsigned)(soff0 + idx * estride) : BUF_OOB;
+  };
+  auto rotate = [&](const Rot& r, cd& x0, cd& x1) {
+    if (LEFT) rot_left(r, x0, x1);
+    else rot_right(r, x0, x1);
+  };
+  auto swap_halves = [&](cd x) {
+    cd y;
+    y.x = __shfl_xor(x.x, 32);
+    y.y = __shfl_xor(x.y, 32);
+    return y;
+  };
+  // group 0 reads its band and the entering elements from memory; group 1 its band and the two
+  // elements in flight between the groups at the start of the pass
+#pragma unroll
+  for (int i = 0; i < B - 1; ++i) band[i] = buf_load(rsrc, off(base0 + i, g1 ? vhi : hi_read));
+#pragma unroll
+  for (int u = 0; u < U; ++u) pre[u] = buf_load(rsrc, g1 ? BUF_OOB : off(base0 + B - 1 + u, hi_read));
+#pragma unroll
+  for (int q = 0; q < 2; ++q) carried[q] = buf_load(rsrc, g1 ? off(base0 + B - 1 + q, vhi) : BUF_OOB);
+#pragma unroll
+  for (int i = 0; i < B - 1; ++i) pin(band[i]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) pin(pre[u]);
+  pin(carried[0]);
+  pin(carried[1]);
+  int s = s0;
+  int dep = 0;
+  const int ng = (s1 - s0) / U;
+  for (int g = 0; g < ng; ++g, s += U) {
+    const int base = lo_g + s - 3 * (G - 1);
+    band[B - 1] = csel(g1, carried[0], pre[0]);
+    band[B] = csel(g1, carried[1], pre[1]);
+    band[B + 1] = pre[2];                      // group 1: replaced after step 0
+#pragma unroll
+    for (int u = 0; u < U; ++u) pre[u] = buf_load(rsrc, g1 ? BUF_OOB : off(base + U + B - 1 + u, hi_read));
+    const Rot* rs = rots_g + (long long)(s - s0) * rot_stride;
+    Rot cur[BC], nxt[BC];
+#pragma unroll
+    for (int jj = 0; jj < BC; ++jj) cur[jj] = rs[jj];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int u = c / CPS, j0 = (c % CPS) * BC;
+      if (c + 1 < NCH) {
+        const int un = (c + 1) / CPS, jn = ((c + 1) % CPS) * BC;
+#pragma unroll
+        for (int jj = 0; jj < BC; ++jj) nxt[jj] = rs[(long long)un * rot_stride + jn + jj + dep];
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) {
+        const int slot = 3 * (G - 1 - (j0 + jj)) + u;
+        rotate(cur[jj], band[slot], band[slot + 1]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) {
+        const int slot = 3 * (G - 1 - (j0 + jj)) + u;
+        pin(band[slot]);
+        pin(band[slot + 1]);
+      }
+      asm volatile("" : "+v"(dep));
+      if (c % CPS == CPS - 1) {
+        // slot u is final: group 0 hands it to group 1, group 1 writes it back
+        const cd x = swap_halves(band[u]);
+        if (u == 0) band[B + 1] = csel(g1, x, band[B + 1]);
+        if (u == 1) carried[0] = x;
+        if (u == 2) carried[1] = x;
+        buf_store(rsrc, g1 ? off(base + u, vhi) : BUF_OOB, band[u]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < BC; ++jj) cur[jj] = nxt[jj];
+    }
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i) band[i] = band[i + U];
+  }
+  // the remaining (at most U - 1) steps one at a time
+#pragma unroll
+  for (int u = 0; u < U - 1; ++u) {
+    if (s >= s1) break;
+    const int base = lo_g + s - 3 * (G - 1);
+    band[B - 1] = csel(g1, carried[0], pre[u]);
+    const Rot* rs = rots_g + (long long)(s - s0) * rot_stride;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const Rot r = rs[j];
+      rotate(r, band[3 * (G - 1 - j)], band[3 * (G - 1 - j) + 1]);
+    }
+    const cd x = swap_halves(band[0]);
+    carried[0] = carried[1];
+    carried[1] = x;
+    buf_store(rsrc, g1 ? off(base, vhi) : BUF_OOB, band[0]);
+#pragma unroll
+    for (int i = 0; i < B - 1; ++i) band[i] = band[i + 1];
+    ++s;
+  }
+  // flush: each group its band; group 1 also the two elements in flight
+  const int base_end = lo_g + s1 - 3 * (G - 1);
+#pragma unroll
+  for (int i = 0; i < B - 1; ++i) buf_store(rsrc, off(base_end + i, vhi), band[i]);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) buf_store(rsrc, g1 ? off(base_end + B - 1 + q, vhi) : BUF_OOB, carried[q]);
 }
 
 // Reflector of bulge j at chase position k (k == lo: the bulge enters with shift sh).
@@ -589,18 +656,34 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
 // forms bulge j's reflector for the next step from them (the only inputs it needs). The
 // current and next steps' reflectors live in an LDS ring; every step's reflectors are also
 // logged to rot_all for the deferred updates outside the window and of Z.
+// Deferred updates of one window's steps [s0, s1) (reflectors rw in LDS, M per step) on the
+// strips [first, end) of a matrix: M = 8 one band per thread, M = 16 a band pair per two lanes.
+template <int M, bool LEFT>
+__device__ inline void deferred(int lo, int s0, int s1, const Rot* rw, int vlo, int vhi, const cd* base,
+                                int nbytes, int estride, int sstride, int first, int end) {
+  const int t = threadIdx.x;
+  if constexpr (M == 16) {
+    for (int xb = first + (t >> 6) * 32; xb < end; xb += NT / 2) {
+      const int x = xb + (t & 31);
+      band_apply_pair<LEFT>(lo, s0, s1, rw, M, vlo, vhi, base, nbytes, estride, sstride * x, x < end);
+    }
+  } else {
+    static_assert(M == 8, "8 or 16 bulges per sweep");
+    for (int x = first + t; x < end; x += NT)
+      band_apply<M, LEFT, 1>(lo, s0, s1, rw, M, vlo, vhi, base, nbytes, estride, sstride * x, 0);
+  }
+}
+
 template <int M>
 __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, const cd* sh, int W,
-                                 cd* Hw, Rot (*s_rot)[M], Rot* rot_win, Rot* rot_all) {
+                                 cd* Hw, Rot (*s_rot)[M], Rot* rot_all) {
   const int nbytes = 16 * n * n;   // one matrix (H or Z): byte offsets fit in 32 bits
   ASSUME_LDS(Hw);
   ASSUME_LDS(s_rot);
-  ASSUME_LDS(rot_win);
   ASSUME_LDS(sh);
   ASSUME_GLOBAL(H);
   ASSUME_GLOBAL(Z);
   ASSUME_GLOBAL(rot_all);
-  static_assert(M % BG == 0, "bulge count must be a multiple of the band group");
   const int t = threadIdx.x;
   const int T = (hi - 1 - lo) + 3 * (M - 1);       // last step
   const int ldw = W + 1;
@@ -620,14 +703,12 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
       const int r = e / ww, c = e % ww;
       Hw[r * ldw + c] = H[(long long)(a + r) * n + a + c];
     }
-    if (!first && t < M) rot_win[t] = s_rot[s_begin & 1][t];
     __syncthreads();
     if (first) {
       // reflectors of step 0 (only bulge 0 can be active)
       if (t < M) {
         const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, lo + s_begin - 3 * t, sh[t]);
         s_rot[s_begin & 1][t] = r;
-        rot_win[t] = r;
         rot_all[(long long)s_begin * M + t] = r;
       }
       first = false;
@@ -663,7 +744,6 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
         if (s + 1 <= T) {
           const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, k + 1, sh[j]);
           s_rot[(s + 1) & 1][j] = r;
-          if (s + 1 < s_end) rot_win[(s + 1 - s_begin) * M + j] = r;
           rot_all[(long long)(s + 1) * M + j] = r;
         }
       }
@@ -674,21 +754,15 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
       const int r = e / ww, c = e % ww;
       H[(long long)(a + r) * n + a + c] = Hw[r * ldw + c];
     }
-    const Rot* rw = rot_win;
-    // deferred row operations on the columns right of the window (rows a..wend-1)
-    for_strips<HS>(wend + t, n, [&](int c, auto sc) {
-      constexpr int SS = decltype(sc)::value;
-      for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, true, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend, H, nbytes, 16 * n,
-                                 16 * c, 16 * NT);
-    });
-    // deferred column operations on the rows above the window (columns a..wend-1)
-    for_strips<HS>(t, a, [&](int q, auto sc) {
-      constexpr int SS = decltype(sc)::value;
-      for (int g0 = 0; g0 < M; g0 += BG)
-        band_apply<BG, false, SS>(lo - 3 * g0, s_begin, s_end, rw + g0, M, a, wend, H, nbytes, 16,
-                                  16 * n * q, 16 * n * NT);
-    });
+    // the window's reflectors, from rot_all into the LDS the window no longer needs
+    __syncthreads();
+    Rot* rw = (Rot*)Hw;
+    for (int e = t; e < (s_end - s_begin) * M; e += NT) rw[e] = rot_all[(long long)s_begin * M + e];
+    __syncthreads();
+    // deferred row operations on the columns right of the window (rows a..wend-1), then column
+    // operations on the rows above it (columns a..wend-1)
+    deferred<M, true>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16 * n, 16, wend, n);
+    deferred<M, false>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16, 16 * n, 0, a);
     __syncthreads();
     s_begin = s_end;
   }
@@ -701,22 +775,10 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
     __syncthreads();
     for (int e = t; e < (c1 - c0) * M; e += NT) rz[e] = rot_all[(long long)c0 * M + e];
     __syncthreads();
-    if (zh) {
-      // Z^H: row operations on its column strips, consecutive lanes on consecutive columns
-      for_strips<HS>(t, n, [&](int q, auto sc) {
-        constexpr int SS = decltype(sc)::value;
-        for (int g0 = 0; g0 < M; g0 += BG)
-          band_apply<BG, true, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1, Z, nbytes, 16 * n, 16 * q,
-                                   16 * NT);
-      });
-    } else {
-      for_strips<HS>(t, n, [&](int q, auto sc) {
-        constexpr int SS = decltype(sc)::value;
-        for (int g0 = 0; g0 < M; g0 += BG)
-          band_apply<BG, false, SS>(lo - 3 * g0, c0, c1, rz + g0, M, lo, hi + 1, Z, nbytes, 16,
-                                    16 * n * q, 16 * n * NT);
-      });
-    }
+    // Z^H: row operations on its column strips (consecutive lanes on consecutive columns);
+    // Z: column operations on its row strips
+    if (zh) deferred<M, true>(lo, c0, c1, rz, lo, hi + 1, Z, nbytes, 16 * n, 16, 0, n);
+    else deferred<M, false>(lo, c0, c1, rz, lo, hi + 1, Z, nbytes, 16, 16 * n, 0, n);
   }
   __syncthreads();
 }
@@ -731,7 +793,6 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
   Rot* rot_all = (Rot*)(rws + b * (long long)(n + 3 * M) * M);
   extern __shared__ unsigned char smem[];
   cd* Hw = (cd*)smem;
-  Rot* rot_win = (Rot*)(smem + sizeof(cd) * (size_t)W * (W + 1));
   __shared__ Rot s_rot[2][M];
   __shared__ int s_k[NT];
   __shared__ cd s_sh[M];
@@ -768,27 +829,46 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       if (t == 0) info[b] = hi + 1;
       return;
     }
-    // shifts: eigenvalues of the trailing M x M block (exceptional shifts when stalled)
+    // shifts: eigenvalues of the trailing M x M block (one wave; exceptional shifts when
+    // stalled or when that small iteration fails)
     ++stall;
-    if (t == 0) {
+    {
       const int b0 = hi - M + 1;
-      for (int r = 0; r < M; ++r)
-        for (int c = 0; c < M; ++c)
-          s_small[r * M + c] = (r > c + 1) ? mkc(0.0, 0.0) : H[(long long)(b0 + r) * n + b0 + c];
-      int bad = small_eigvals(s_small, M, M, s_sh);
-      if (bad || stall % 6 == 0) {
-        const double ex = 0.75 * cabs1(H[(long long)hi * n + hi - 1]);
-        for (int j = 0; j < M; ++j)
-          s_sh[j] = cadd(H[(long long)(hi - (j % 2)) * n + hi - (j % 2)],
-                         mkc(ex * (1.0 + 0.1 * j), 0.0));
+      for (int e = t; e < M * M; e += NT) {
+        const int r = e / M, c = e % M;
+        s_small[e] = (r > c + 1) ? mkc(0.0, 0.0) : H[(long long)(b0 + r) * n + b0 + c];
+      }
+      __syncthreads();
+      if (t < 64) {
+        const int bad = wave_qr(s_small, nullptr, M, M, M, s_sh);
+        if (t == 0 && (bad || stall % 6 == 0)) {
+          const double ex = 0.75 * cabs1(H[(long long)hi * n + hi - 1]);
+          for (int j = 0; j < M; ++j)
+            s_sh[j] = cadd(H[(long long)(hi - (j % 2)) * n + hi - (j % 2)], mkc(ex * (1.0 + 0.1 * j), 0.0));
+        }
       }
     }
     __syncthreads();
-    multishift_sweep<M>(H, Z, zh, n, lo, hi, s_sh, W, Hw, s_rot, rot_win, rot_all);
+    multishift_sweep<M>(H, Z, zh, n, lo, hi, s_sh, W, Hw, s_rot, rot_all);
   }
 }
 
 }  // namespace
+
+// smallest active block handled by the multishift sweep - 1 (blocks of at most nmin rows go to
+// small_block)
+static int hqr_nmin() { return (3 * WB_HQR_M + 8 > 48) ? 3 * WB_HQR_M + 8 : 48; }
+static size_t hqr_lds_bytes(int W) {
+  constexpr int M = WB_HQR_M;
+  const size_t window = sizeof(cd) * (size_t)W * (W + 1);
+  const size_t rots = sizeof(Rot) * (size_t)(W + 3 * M) * M;
+  const int m = hqr_nmin(), ld = (m % 2) ? m : m + 1;
+  const size_t small = 2 * sizeof(cd) * (size_t)m * ld;
+  size_t need = window;
+  if (rots > need) need = rots;
+  if (small > need) need = small;
+  return need;
+}
 
 size_t wb_hqr_ms_rot_bytes(int n, int batch) {
   return sizeof(Rot) * (size_t)(n + 3 * WB_HQR_M) * WB_HQR_M * (size_t)batch;
@@ -804,18 +884,18 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void*
     int dev = 0, smem = 65536;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    // dynamic LDS: the window and its reflectors (a window holds at most W + 3M steps); the
-    // rest is the kernel's static LDS
+    // dynamic LDS: the W x W window; after the window is stored the same space holds its
+    // reflectors (at most W + 3M steps of M), and the small-block iteration uses it for T and
+    // U. The rest is the kernel's static LDS.
     const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * NT + sizeof(cd) * (M + M * M) + 2048;
     for (int c = 128; c >= 32; c -= 8) {
       if (c < 3 * M + 16) break;
-      const size_t need = sizeof(cd) * (size_t)c * (c + 1) + sizeof(Rot) * (size_t)(c + 3 * M) * M;
-      if (need + fixed <= (size_t)smem) { W = c; break; }
+      if (hqr_lds_bytes(c) + fixed <= (size_t)smem) { W = c; break; }
     }
     if (W == 0) W = 3 * M + 16;
   }
-  const size_t sh = sizeof(cd) * (size_t)W * (W + 1) + sizeof(Rot) * (size_t)(W + 3 * M) * M;
-  const int nmin = (3 * M + 8 > 48) ? 3 * M + 8 : 48;
+  const size_t sh = hqr_lds_bytes(W);
+  const int nmin = hqr_nmin();
   WB_TIMED_F(WB_T_HQR, stream, 72.0 * (double)n * n * n * batch,
              hipLaunchKernelGGL(k_hqr_ms<M>, dim3(batch), dim3(NT), sh, stream, n, H, Z, zh, w, info,
                                 (Rot*)rot_ws, W, nmin));
