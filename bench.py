@@ -176,6 +176,32 @@ def main():
     n_flagged = int((flags != 0).sum())
     batched_rate = P * args.steps / dt
 
+    # per-stage rates from one more (untimed-for-value) step with whole stages bracketed by HIP
+    # events: the layer eigensolves and the S-matrix work (Omega^2, V, layer S, cascade) against
+    # the FP64 peak, the assembly gathers (raster -> Toeplitz blocks) against the HBM peak
+    _native.timing_enable(True, stages=True)
+    for n_ in names:
+        _native.timing_read(n_, reset=True)
+    step(pts_d, thick_d, idx_d, ws)
+    torch.cuda.synchronize()
+    _native.timing_enable(False)
+    stages = {}
+    for key, nm in (('eig', 'stage_eig'), ('smatrix', 'stage_smat')):
+        ms, cnt, fl = _native.timing_read(nm, reset=True)
+        if cnt:
+            tf = fl / (ms * 1e-3) / 1e12
+            stages[key] = {'ms': round(ms, 3), 'flops': fl, 'achieved_tflops': round(tf, 3),
+                           'peak_tflops': PEAK_FP64_TFLOPS, 'frac': round(tf / PEAK_FP64_TFLOPS, 4)}
+    ms, cnt, _ = _native.timing_read('assemble', reset=True)
+    if cnt:
+        Ny, Nx = wl['rasters'].shape[1:]
+        # algorithmic bytes: the complex rasters read once, three N x N complex blocks written
+        asm_bytes = L * Ny * Nx * 16 + 3 * L * N * N * 16
+        gbs = asm_bytes / (ms * 1e-3) / 1e9
+        stages['assembly_gather'] = {'ms': round(ms, 4), 'bytes': asm_bytes,
+                                     'achieved_gbs': round(gbs, 1), 'peak_gbs': PEAK_HBM_GBS,
+                                     'frac': round(gbs / PEAK_HBM_GBS, 4)}
+
     # plain: one point per launch sequence (--plain-points 0 skips it, e.g. under a profiler)
     plain_rate = None
     if args.plain_points > 0:
@@ -378,7 +404,7 @@ def main():
         'plain_solves_per_s': round(plain_rate, 4) if plain_rate else None,
         'batched_vs_plain': round(batched_rate / plain_rate, 3) if plain_rate else None,
         'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
-        'roofline': roofline, 'cpu_baseline': cpu,
+        'roofline': roofline, 'cpu_baseline': cpu, 'stages': stages,
         'gradient': grad,
         'serve': serve_report,
         'launch_bound': small,

@@ -180,13 +180,17 @@ int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx
                            const double* d_epy, const double* d_epz, double* d_eps,
                            void* workspace, size_t workspace_bytes, void* stream);
 
-/* Per-kernel timing (no reference counterpart: measurement support). When enabled, the
- * launches of the kernels below are bracketed by hipEvent records on their stream;
+/* Per-kernel and per-stage timing (no reference counterpart: measurement support).
+ * wb_timing_enable(1) brackets the launches of the kernels below by hipEvent records on their
+ * stream; wb_timing_enable(2) instead times whole stages of a solve: WB_T_STAGE_EIG (the layer
+ * eigensolves, 100 n^3 flops each), WB_T_STAGE_SMAT (the rest of wb_solve: Omega^2, V, the
+ * layer S-matrices and the cascade, 84 n^3 per layer-point + 96 (L+1) n^3 per point) and
+ * WB_T_ASSEMBLE (the assembly gathers, timed in both modes); 0 turns timing off.
  * wb_timing_read returns the summed milliseconds, launch count and algorithmic real FP64 flops
  * (GEMM 8mnk; eigensolver stages from EIG(n) = 100 n^3) since the last reset. */
 enum {
   WB_T_ZGEMM = 0, WB_T_GETRF_PANEL, WB_T_LASWP_TRSM, WB_T_TRSM_DIAG, WB_T_GEHRD, WB_T_UNGHR,
-  WB_T_HQR, WB_T_TREVC, WB_T_COLNORM, WB_T_ASSEMBLE, WB_T_COUNT
+  WB_T_HQR, WB_T_TREVC, WB_T_COLNORM, WB_T_ASSEMBLE, WB_T_STAGE_EIG, WB_T_STAGE_SMAT, WB_T_COUNT
 };
 int wb_timing_enable(int on);
 int wb_timing_read(int id, double* total_ms, long long* count, double* flops, int reset);

@@ -18,8 +18,11 @@ def _eps_stack(rng, L, Ny, Nx, lossy=True):
     return e.astype(np.complex128)
 
 
+# (1, 2100, 3) and (1, 3, 2100): series longer than the LDS-staged kernels take (2048 samples)
+# run on the global-memory kernels
 @pytest.mark.parametrize('L,Ny,Nx,hu,hv', [(1, 1, 4, 10, 0), (2, 5, 6, 2, 2), (3, 16, 12, 3, 4),
-                                           (1, 64, 64, 7, 7), (2, 9, 1, 2, 3)])
+                                           (1, 64, 64, 7, 7), (2, 9, 1, 2, 3), (1, 2100, 3, 2, 2),
+                                           (1, 3, 2100, 2, 1)])
 def test_assembly_matches_oracle(L, Ny, Nx, hu, hv):
     rng = np.random.default_rng(L * 100 + Ny)
     eps = _eps_stack(rng, L, Ny, Nx)
@@ -73,3 +76,39 @@ def test_solve_matches_oracle(case):
         for k in ('r_s', 'r_p', 't_s', 't_p'):
             assert np.abs(got[v][k] - want[v][k]).max() < 1e-9, (v, k)
         assert abs(got[v]['power_entering'] - want[v]['power_entering']) < 1e-9
+
+
+def test_stage_timing_splits_the_solve():
+    """Per-stage timing brackets the eigensolves and the S-matrix work of one solve with the
+    documented algorithmic flops; per-kernel timing leaves the stage entries empty."""
+    rng = np.random.default_rng(5)
+    L, hu, hv, P = 2, 2, 2, 3
+    eps = _eps_stack(rng, L, 8, 8)
+    conv = _native.assemble_li(torch.from_numpy(eps).to(DEV), hu, hv)[:3]
+    b = eng.Bucket(hu=hu, hv=hv, path=eng.PATH_2D, n_layers=L)
+    for i in range(P):
+        b.points.append(eng.Point(stack=0, wavelength=600.0 + 10 * i, theta=0.2, phi=0.3,
+                                  polarization='p', eps_top=1.0, eps_bottom=2.25,
+                                  period=(500.0, 500.0), thickness=(100.0, 150.0)))
+    names = list(_native.TIMER_IDS)
+    try:
+        for stages in (False, True):
+            _native.timing_enable(True, stages=stages)
+            for nm in names:
+                _native.timing_read(nm, reset=True)
+            eng.run_bucket(b, [conv], DEV)
+            torch.cuda.synchronize()
+            _native.timing_enable(False)
+            got = {nm: _native.timing_read(nm, reset=True) for nm in names}
+            if not stages:
+                assert got['stage_eig'][1] == 0 and got['stage_smat'][1] == 0
+                assert got['zgemm'][1] > 0 and got['hqr'][1] > 0
+            else:
+                n = 2 * (2 * hu + 1) * (2 * hv + 1)
+                assert got['stage_eig'][1] == 1 and got['stage_eig'][0] > 0
+                assert got['stage_eig'][2] == pytest.approx(100.0 * n ** 3 * P * L)
+                assert got['stage_smat'][1] == 2 and got['stage_smat'][0] > 0
+                assert got['stage_smat'][2] == pytest.approx((84.0 * P * L + 96.0 * (L + 1) * P) * n ** 3)
+                assert all(got[nm][1] == 0 for nm in names if not nm.startswith('stage_'))
+    finally:
+        _native.timing_enable(False)

@@ -33,7 +33,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_gemm_tuning_clear')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
-             'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9}
+             'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
+             'stage_smat': 11}
 
 
 class EngineUnavailable(RuntimeError):
@@ -406,6 +407,22 @@ def assemble_li_adjoint(eps, hu, hv, epx, epz_i, d_epx, d_epy, d_epz):
 
 # ----------------------------------------------------------------------------- timing
 
+def timing_enable(on=True, stages=False):
+    """Per-kernel timing (``on``), or per-stage timing (``stages``: eigensolve, S-matrices,
+    assembly gathers), of subsequent launches; ``on=False`` turns timing off."""
+    load_library().wb_timing_enable((2 if stages else 1) if on else 0)
+
+
+def timing_read(name, reset=True):
+    """(total milliseconds, launch count, algorithmic flops) of kernel ``name`` since reset."""
+    lib = load_library()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_longlong()
+    fl = ctypes.c_double()
+    lib.wb_timing_read(TIMER_IDS[name], ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(fl),
+                       1 if reset else 0)
+    return ms.value, cnt.value, fl.value
+
 
 # ----------------------------------------------------------------------------- GEMM autotuning
 
@@ -457,17 +474,3 @@ def gemm_tuning_set(opa, opb, m, n, k, batch, config):
 def gemm_tuning_clear():
     """Forget the tuning table and the recorded shapes."""
     load_library().wb_gemm_tuning_clear()
-
-def timing_enable(on=True):
-    load_library().wb_timing_enable(1 if on else 0)
-
-
-def timing_read(name, reset=True):
-    """(total milliseconds, launch count, algorithmic flops) of kernel ``name`` since reset."""
-    lib = load_library()
-    ms = ctypes.c_double()
-    cnt = ctypes.c_longlong()
-    fl = ctypes.c_double()
-    lib.wb_timing_read(TIMER_IDS[name], ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(fl),
-                       1 if reset else 0)
-    return ms.value, cnt.value, fl.value

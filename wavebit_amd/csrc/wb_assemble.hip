@@ -13,9 +13,13 @@
 //                  block-Toeplitz layout out[(n,m),(n',m')] = A_{m-m'}[n,n'].
 //   k_direct_bttb  the 2D direct rule M[(n,m),(n',m')] = c(n-n', m-m').
 // Then epx = inv(T_u([T_v(eps)]^-1)) and epz_i = inv(M) on the batched LU path.
-// The raster is read once per slot by coalesced column walks; the N x N outputs are written
-// once. Twiddles use exact integer phase reduction (k*j mod n) and sincospi.
+// Each series workgroup first stages its n samples and the n twiddles e^{-2 pi i idx/n}
+// (sincospi of the exactly reduced phase idx = k*j mod n) in LDS, so the per-order sums read
+// LDS only and every twiddle is evaluated once per workgroup instead of once per (order,
+// sample). The outer kernel writes its (2hu+1)^2 Toeplitz sub-block with whole-row runs.
+// Series longer than SERIES_LDS_MAX samples take the global-memory kernels (same arithmetic).
 #include "wb_internal.h"
+#include "wb_timing.h"
 
 
 namespace {
@@ -50,6 +54,99 @@ __device__ inline cd pixel_series(const cd* vals, long long stride, int n, int k
   return out;
 }
 
+// ------------------------------------------------------------------ LDS-staged series
+constexpr int SERIES_LDS_MAX = 2048;   // samples per staged series: 2 x 32 KiB of LDS
+
+// Stage samples s[j] = g[j * stride] (their reciprocals if recip) and twiddles
+// tw[idx] = e^{-2 pi i idx / n} in LDS; the caller synchronizes before reading them.
+__device__ inline void stage_series(const cd* g, long long stride, int n, bool recip, cd* s, cd* tw) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    cd a = g[(long long)j * stride];
+    s[j] = recip ? crecip(a) : a;
+    double sn, c;
+    sincospi(-2.0 * (double)j / (double)n, &sn, &c);
+    tw[j] = mkc(c, sn);
+  }
+}
+
+// pixel_series over staged samples and twiddles: the same terms in the same order (the phase
+// index k*j mod n is advanced by k mod n per sample).
+__device__ inline cd pixel_series_lds(const cd* s, const cd* tw, int n, int k) {
+  const cd base = s[0];
+  double sr = 0.0, si = 0.0;
+  const double inv_n = 1.0 / (double)n;
+  const int kmod = ((k % n) + n) % n;
+  int idx = 0;
+  for (int j = 1; j < n; ++j) {
+    idx += kmod;
+    if (idx >= n) idx -= n;
+    const cd d = cscale(csub(s[j], base), inv_n);
+    const cd w = tw[idx];
+    sr += d.x * w.x - d.y * w.y;
+    si += d.x * w.y + d.y * w.x;
+  }
+  cd out = mkc(sr, si);
+  if (n > 1) {
+    const double tt = (double)k / (double)n;
+    const double sinc = (k == 0) ? 1.0 : sinpi(tt) / (3.141592653589793 * tt);
+    double sn, c;
+    sincospi(-tt, &sn, &c);
+    out = cscale(cmul(out, mkc(c, sn)), sinc);
+  } else {
+    out = mkc(0.0, 0.0);
+  }
+  if (k == 0) out = cadd(out, base);
+  return out;
+}
+
+// k_series_v with the column staged in LDS (dynamic LDS: 2 Ny complex)
+__global__ __launch_bounds__(64) void k_series_v_lds(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines) {
+  extern __shared__ cd sm[];
+  const int ju = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int Kv = 4 * hv + 1;
+  stage_series(eps + (long long)l * Ny * Nx + ju, Nx, Ny, slot == 1, sm, sm + Ny);
+  __syncthreads();
+  for (int kk = threadIdx.x; kk < Kv; kk += blockDim.x)
+    lines[(((long long)slot * L + l) * Nx + ju) * Kv + kk] = pixel_series_lds(sm, sm + Ny, Ny, kk - 2 * hv);
+}
+
+// k_series_2d with the series staged in LDS (dynamic LDS: 2 Nx complex)
+__global__ __launch_bounds__(64) void k_series_2d_lds(const cd* lines, int L, int Nx, int hu, int hv, cd* c2d) {
+  extern __shared__ cd sm[];
+  const int kv = blockIdx.x, l = blockIdx.y;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  stage_series(lines + ((long long)l * Nx) * Kv + kv, Kv, Nx, false, sm, sm + Nx);
+  __syncthreads();
+  for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x)
+    c2d[((long long)l * Kv + kv) * Ku + kk] = pixel_series_lds(sm, sm + Nx, Nx, kk - 2 * hu);
+}
+
+// k_outer_bttb with the series staged in LDS: the Ku coefficients of inner entry e = (n1, n2)
+// go to LDS, then the workgroup writes the nu x nu Toeplitz sub-block (n1, n2) of the output,
+// element (m, m') = coefficient m - m', consecutive lanes along each output row.
+// Dynamic LDS: 2 Nx + Ku complex.
+__global__ __launch_bounds__(256) void k_outer_bttb_lds(const cd* inner, int L, int Nx, int hu, int hv,
+                                                        cd* outA, cd* outB) {
+  extern __shared__ cd sm[];
+  const int e = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, Ku = 4 * hu + 1;
+  const int N = nu * nv;
+  const int n1 = e / nv, n2 = e % nv;
+  cd* coef = sm + 2 * Nx;
+  stage_series(inner + (((long long)slot * L + l) * Nx) * nv * nv + e, (long long)nv * nv, Nx, false,
+               sm, sm + Nx);
+  __syncthreads();
+  for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x)
+    coef[kk] = pixel_series_lds(sm, sm + Nx, Nx, kk - 2 * hu);
+  __syncthreads();
+  cd* out = (slot == 0 ? outA : outB) + (long long)l * N * N + (long long)(n1 * nu) * N + n2 * nu;
+  for (int t = threadIdx.x; t < nu * nu; t += blockDim.x) {
+    const int m = t / nu, mp = t % nu;
+    out[(long long)m * N + mp] = coef[m - mp + 2 * hu];
+  }
+}
+
+// ------------------------------------------------------------------ global-memory series
 // lines[slot][l][ju][kv], slot 0: eps, slot 1: 1/eps
 __global__ void k_series_v(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines) {
   const int ju = blockIdx.x, l = blockIdx.y, slot = blockIdx.z;
@@ -300,6 +397,16 @@ __global__ __launch_bounds__(64) void k_series_v_adj(const cd* eps, const cd* li
   }
 }
 
+// Column series of eps and 1/eps (lines), staged in LDS up to SERIES_LDS_MAX rows.
+hipError_t series_v_launch(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines, hipStream_t stream) {
+  if (Ny <= SERIES_LDS_MAX)
+    hipLaunchKernelGGL(k_series_v_lds, dim3(Nx, L, 2), dim3(64), 2 * Ny * sizeof(cd), stream, eps, L, Ny,
+                       Nx, hv, lines);
+  else
+    hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
@@ -325,25 +432,36 @@ hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, i
   cd* pre = (cd*)p; p += sizeof(cd) * 2 * (size_t)L * N * N;     // [0]: epx pre, [1]: epz pre
   int* ipiv = (int*)p; p += sizeof(int) * 2 * (size_t)L * N;
   int* info = info_out;
-  hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
-  WB_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_series_2d, dim3(Kv, L), dim3(64), 0, stream, lines, L, Nx, hu, hv, c2d);
-  WB_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
-  WB_CHECK_LAUNCH();
-  // slot 0 (eps) -> epx pre-inversion, slot 1 (1/eps) -> epy directly
-  hipLaunchKernelGGL(k_outer_bttb, dim3(nv * nv, L, 2), dim3(64), 0, stream, inner, L, Nx, hu, hv,
-                     pre, epy);
-  WB_CHECK_LAUNCH();
+  hipError_t e;
   cd* epz_pre = pre + (size_t)L * N * N;
   const long long NN = (long long)N * N;
-  hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), dim3(256), 0, stream, c2d,
-                     L, hu, hv, epz_pre);
-  WB_CHECK_LAUNCH();
+  {
+    // the gathers (raster -> epx pre-inversion, epy, epz pre-inversion): one timed stage
+    WbStageTimer timer(WB_T_ASSEMBLE, stream, 0.0, WB_STAGE_BOTH);
+    if ((e = series_v_launch(eps, L, Ny, Nx, hv, lines, stream)) != hipSuccess) return e;
+    if (Nx <= SERIES_LDS_MAX)
+      hipLaunchKernelGGL(k_series_2d_lds, dim3(Kv, L), dim3(64), 2 * Nx * sizeof(cd), stream, lines, L, Nx,
+                         hu, hv, c2d);
+    else
+      hipLaunchKernelGGL(k_series_2d, dim3(Kv, L), dim3(64), 0, stream, lines, L, Nx, hu, hv, c2d);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
+    WB_CHECK_LAUNCH();
+    // slot 0 (eps) -> epx pre-inversion, slot 1 (1/eps) -> epy directly
+    if ((2 * Nx + Ku) * sizeof(cd) <= 65536)
+      hipLaunchKernelGGL(k_outer_bttb_lds, dim3(nv * nv, L, 2), dim3(256), (2 * Nx + Ku) * sizeof(cd), stream,
+                         inner, L, Nx, hu, hv, pre, epy);
+    else
+      hipLaunchKernelGGL(k_outer_bttb, dim3(nv * nv, L, 2), dim3(64), 0, stream, inner, L, Nx, hu, hv,
+                         pre, epy);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), dim3(256), 0, stream, c2d,
+                       L, hu, hv, epz_pre);
+    WB_CHECK_LAUNCH();
+  }
   // epx = inv(pre), epz_i = inv(epz_pre): one batched inversion of 2L matrices
   // output slots are separate buffers: use a grid BatchPtr over (slot, layer)
   BatchPtr A = bp_strided(pre, NN);
-  hipError_t e;
   e = wb_inverse_launch(N, A, N, bp_strided(epx, NN), N, ipiv, info, L, stream);
   if (e != hipSuccess) return e;
   e = wb_inverse_launch(N, bp_strided(epz_pre, NN), N, bp_strided(epz_i, NN), N, ipiv + (size_t)L * N,
@@ -383,8 +501,7 @@ hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int 
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
   hipError_t e;
   // recompute the column series and their Toeplitz inverses
-  hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
-  WB_CHECK_LAUNCH();
+  if ((e = series_v_launch(eps, L, Ny, Nx, hv, lines, stream)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
   WB_CHECK_LAUNCH();
   // outer inverses: pre_bar = -epx^H d_epx epx^H, zpre_bar = -epz^H d_epz epz^H

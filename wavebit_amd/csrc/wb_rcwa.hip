@@ -22,6 +22,7 @@
 //                 same S22/S12 columns the star cascade gives (scattering.py:191-200).
 //   efficiencies  scattering.py:907-994, solve_1d.py:1588-1610, observations/power.py.
 #include "wb_rcwa.h"
+#include "wb_timing.h"
 
 namespace {
 
@@ -594,6 +595,22 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if (keep) CK = wb_ck_carve(a.ck, P, L, a.hu, a.hv, mode);
   hipError_t e;
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  // per-stage timing: the layer eigensolves vs everything else of the solve (algorithmic flops:
+  // EIG(n) = 100 n^3 per layer-point; Omega^2 + V + A, B and S_layer = 84 n^3 per layer-point
+  // and the cascade 96 (L+1) n^3 per point)
+  const double n3 = (double)n * n * n;
+  WbStageTimer smat(WB_T_STAGE_SMAT, st, (84.0 * B + 96.0 * (L + 1) * (double)P) * n3, WB_STAGE_COARSE);
+  auto eig = [&]() -> hipError_t {
+    smat.stop();
+    smat.flops = 0.0;
+    hipError_t r;
+    {
+      WbStageTimer t(WB_T_STAGE_EIG, st, 100.0 * n3 * B, WB_STAGE_COARSE);
+      r = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st);
+    }
+    smat.start();
+    return r;
+  };
 
 #define FLAG(count, spread, inf, bit)                                                          \
   do {                                                                                       \
@@ -625,7 +642,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1], nn), N, EPX, N, one, S(M[0], nn), n, B, st))) return e;
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1] + (long long)n * N, nn), N, EPY, N, one,
                              S(M[0] + N, nn), n, B, st))) return e;
-    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st))) return e;
+    if ((e = eig())) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
     if (keep) {
       hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.W, nn), n);
@@ -650,7 +667,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       WB_CHECK_LAUNCH();
       if ((e = wb_zgemm_launch(N, N, N, one, EPX, N, S(M[5], nn), N, zero, S(M[0], nn), N, B, st))) return e;
     }
-    if ((e = wb_zgeev_launch(n, M[0], w, M[3], M[1], M[2], tau, rotws, info[0], B, st))) return e;
+    if ((e = eig())) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
     cd* Bw = M[3];
     if (tm) {

@@ -4,12 +4,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-enum {
-  WB_T_ZGEMM = 0, WB_T_GETRF_PANEL, WB_T_LASWP_TRSM, WB_T_TRSM_DIAG, WB_T_GEHRD, WB_T_UNGHR,
-  WB_T_HQR, WB_T_TREVC, WB_T_COLNORM, WB_T_ASSEMBLE, WB_T_COUNT
-};
+#include "../../include/wavebit_amd.h"   // the timer ids WB_T_*
+
+// Two timing modes: per-kernel (wb_timing_enable(1): WB_TIMED_F launches and fine stages) and
+// per-stage (wb_timing_enable(2): the coarse stages of a solve, eigensolve vs S-matrices).
+// A stage timer of the active mode suppresses the timers of the launches inside it.
+enum { WB_STAGE_FINE = 0, WB_STAGE_COARSE = 1, WB_STAGE_BOTH = 2 };
 
 extern bool wb_timing_on;
+extern bool wb_timing_coarse;
 extern int wb_timing_stage;   // > 0 while a multi-launch stage is being timed as one entry
 void wb_timer_begin(int id, hipStream_t s);
 void wb_timer_end(int id, hipStream_t s, double flops);
@@ -18,26 +21,34 @@ void wb_timer_end(int id, hipStream_t s, double flops);
 // stage (WbStageTimer scope) the launches are not timed on their own.
 #define WB_TIMED_F(id, stream, flops, ...)                                           \
   do {                                                                               \
-    const bool timed__ = wb_timing_on && wb_timing_stage == 0;                       \
+    const bool timed__ = wb_timing_on && !wb_timing_coarse && wb_timing_stage == 0;  \
     if (timed__) wb_timer_begin((id), (stream));                                     \
     __VA_ARGS__;                                                                     \
     if (timed__) wb_timer_end((id), (stream), (flops));                              \
   } while (0)
 // A stage of several launches timed as one entry of `id` with `flops` algorithmic flops, for
-// the scope of the object (early returns included).
+// the scope of the object (early returns included), in the timing mode(s) `level` names.
 struct WbStageTimer {
-  int id;
+  int id, level;
   hipStream_t stream;
   double flops;
-  bool timed;
-  WbStageTimer(int id_, hipStream_t s_, double flops_) : id(id_), stream(s_), flops(flops_) {
-    timed = wb_timing_on && wb_timing_stage == 0;
-    if (timed) wb_timer_begin(id, stream);
-    ++wb_timing_stage;
+  bool active = false, timed = false;
+  WbStageTimer(int id_, hipStream_t s_, double flops_, int level_ = WB_STAGE_FINE)
+      : id(id_), level(level_), stream(s_), flops(flops_) {
+    start();
   }
-  ~WbStageTimer() {
-    --wb_timing_stage;
+  ~WbStageTimer() { stop(); }
+  void start() {
+    active = level == WB_STAGE_BOTH || (level == WB_STAGE_COARSE) == wb_timing_coarse;
+    timed = active && wb_timing_on && wb_timing_stage == 0;
+    if (timed) wb_timer_begin(id, stream);
+    if (active) ++wb_timing_stage;
+  }
+  // End the stage early (the destructor then does nothing); start() opens it again.
+  void stop() {
+    if (active) --wb_timing_stage;
     if (timed) wb_timer_end(id, stream, flops);
+    active = timed = false;
   }
 };
 #define WB_TIMED(id, stream, ...) WB_TIMED_F(id, stream, 0.0, __VA_ARGS__)

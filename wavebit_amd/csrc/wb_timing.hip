@@ -5,6 +5,7 @@
 #include "wb_timing.h"
 
 bool wb_timing_on = false;
+bool wb_timing_coarse = false;
 int wb_timing_stage = 0;
 
 namespace {
@@ -52,9 +53,12 @@ void wb_timer_end(int id, hipStream_t s, double flops) {
 
 extern "C" {
 
+// on: 0 off, 1 per kernel, 2 per stage (eigensolve, S-matrices, assembly gathers)
 int wb_timing_enable(int on) {
+  if (on < 0 || on > 2) return 1;
   std::lock_guard<std::mutex> lk(g_mu);
   wb_timing_on = on != 0;
+  wb_timing_coarse = on == 2;
   return 0;
 }
 
