@@ -4,11 +4,14 @@
 // row-major; op is the identity or the conjugate transpose (then A is stored k x m, B n x k,
 // and the tile loads walk the stored rows so global reads stay coalesced).
 // A complex product is four real v_mfma_f64_16x16x4f64 streams (re*re, -im*im, re*im, im*re)
-// into two accumulators. A 256-thread workgroup owns a BM x BN tile of C (64 x 64 by default:
-// four waves of 2 x 2 MFMA blocks); K is staged through LDS 16 at a time, split into real and
-// imaginary planes so every MFMA operand is one 8-byte LDS read, in two LDS stages: the next K
-// tile is fetched into registers while the current one feeds the matrix cores. The tile shape
-// is picked per GEMM shape by the autotuner below.
+// into two accumulators. A workgroup of 4 or 8 waves owns a BM x BN tile of C (64 x 64 by
+// default: four waves of 2 x 2 MFMA blocks); K is staged through LDS 8 at a time (16 for the
+// skinny tile), split into real and imaginary planes so every MFMA operand is one 8-byte LDS
+// read, in two LDS stages: the next K tile is fetched into registers while the current one
+// feeds the matrix cores. The short K tile keeps the LDS stages at 33 KiB and the registers
+// under 128 per lane, so four waves share each SIMD and one wave's barrier wait is covered by
+// the others' MFMAs (measured: 60 TF/s on 256 x 722^3 against 51 TF/s with 16-deep tiles and
+// two waves per SIMD). The tile shape is picked per GEMM shape by the autotuner below.
 //
 // MFMA f64 16x16x4 lane layout (measured on gfx950): A[i][kk] in lane i + 16*kk, B[kk][j] in
 // lane j + 16*kk, D[i][j] in lane j + 16*(i%4), register i/4.
@@ -27,21 +30,23 @@ namespace {
 constexpr int PAD = 2;
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Tile configuration: BM x BN of C per 256-thread workgroup, K staged BK at a time, the four
-// waves laid out WGM (along M) x 4 / WGM (along N), each owning a block of 16 x 16 MFMA tiles.
-// Every configuration accumulates each C entry over k in the same order (k tiles of 16, MFMA
-// steps of 4), so they give bitwise identical results and the choice is a pure speed question.
-template <int BM, int BN, int BK, int WGM, bool CA, bool CB>
-__global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, BatchPtr A, int lda,
-                                               BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc,
-                                               int beta_zero) {
-  constexpr int WGN = 4 / WGM;
+// Tile configuration: BM x BN of C per workgroup of NW waves, K staged BK at a time, the waves
+// laid out WGM (along M) x NW / WGM (along N), each owning a block of 16 x 16 MFMA tiles; MINB
+// workgroups per CU bound the registers. Every configuration accumulates each C entry over k
+// in the same order (MFMA steps of 4 in increasing k), so they give bitwise identical results
+// and the choice is a pure speed question.
+template <int BM, int BN, int BK, int WGM, int NW, int MINB, bool CA, bool CB>
+__global__ __launch_bounds__(NW * 64, MINB) void k_zgemm(int m, int n, int k, cd alpha, BatchPtr A, int lda,
+                                                         BatchPtr B, int ldb, cd beta, BatchPtr C, int ldc,
+                                                         int beta_zero) {
+  constexpr int NT = NW * 64;
+  constexpr int WGN = NW / WGM;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int EA = BM * BK / 256, EB = BK * BN / 256;    // elements per thread per K tile
+  constexpr int EA = BM * BK / NT, EB = BK * BN / NT;      // elements per thread per K tile
   constexpr int VA = EA < 4 ? EA : 4, VB = EB < 4 ? EB : 4; // consecutive elements per load group
   constexpr int GA = EA / VA, GB = EB / VB;
-  static_assert(WGM * WGN == 4 && FM * 16 == WM && FN * 16 == WN && BK % 4 == 0, "tile shape");
+  static_assert(WGM * WGN == NW && FM * 16 == WM && FN * 16 == WN && BK % 4 == 0, "tile shape");
   static_assert(EA >= 1 && EB >= 1 && GA * VA == EA && GB * VB == EB, "tile loads");
   const long long b = blockIdx.z;
   const cd* __restrict__ a = A.ptr(b);
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int g = 0; g < GA; ++g) {
-      const int f = t + 256 * g;
+      const int f = t + NT * g;
 #pragma unroll
       for (int q = 0; q < VA; ++q) {
         cd v = mkc(0.0, 0.0);
@@ -82,7 +87,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
     }
 #pragma unroll
     for (int g = 0; g < GB; ++g) {
-      const int f = t + 256 * g;
+      const int f = t + NT * g;
 #pragma unroll
       for (int q = 0; q < VB; ++q) {
         cd w = mkc(0.0, 0.0);
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
   auto stash = [&](int st) {
 #pragma unroll
     for (int g = 0; g < GA; ++g) {
-      const int f = t + 256 * g;
+      const int f = t + NT * g;
 #pragma unroll
       for (int q = 0; q < VA; ++q) {
         if (!CA) {
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
     }
 #pragma unroll
     for (int g = 0; g < GB; ++g) {
-      const int f = t + 256 * g;
+      const int f = t + NT * g;
 #pragma unroll
       for (int q = 0; q < VB; ++q) {
         if (!CB) {
@@ -182,27 +187,27 @@ __global__ __launch_bounds__(256) void k_zgemm(int m, int n, int k, cd alpha, Ba
       }
 }
 
-// The candidate configurations: (BM, BN, BK, WGM).
-struct GemmCfg { int bm, bn; const char* name; };
+// The candidate configurations.
+struct GemmCfg { int bm, bn, nw; const char* name; };
 constexpr int NCFG = 4;
-constexpr GemmCfg CFGS[NCFG] = {{64, 64, "64x64"}, {128, 64, "128x64"}, {64, 128, "64x128"},
-                                {128, 16, "128x16"}};
+constexpr GemmCfg CFGS[NCFG] = {{64, 64, 4, "64x64"}, {128, 64, 8, "128x64"}, {64, 128, 8, "64x128"},
+                                {128, 16, 4, "128x16"}};
 
 template <int CFG, bool CA, bool CB>
 void launch_cfg_ops(dim3 grid, hipStream_t s, int m, int n, int k, cd alpha, BatchPtr A, int lda, BatchPtr B,
                     int ldb, cd beta, BatchPtr C, int ldc, int bz) {
   if constexpr (CFG == 0)
-    hipLaunchKernelGGL((k_zgemm<64, 64, 16, 2, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, bz);
+    hipLaunchKernelGGL((k_zgemm<64, 64, 8, 2, 4, 3, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc, bz);
   else if constexpr (CFG == 1)
-    hipLaunchKernelGGL((k_zgemm<128, 64, 16, 2, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, bz);
+    hipLaunchKernelGGL((k_zgemm<128, 64, 8, 4, 8, 2, CA, CB>), grid, dim3(512), 0, s, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc, bz);
   else if constexpr (CFG == 2)
-    hipLaunchKernelGGL((k_zgemm<64, 128, 16, 2, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, bz);
+    hipLaunchKernelGGL((k_zgemm<64, 128, 8, 2, 8, 2, CA, CB>), grid, dim3(512), 0, s, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc, bz);
   else
-    hipLaunchKernelGGL((k_zgemm<128, 16, 16, 4, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, bz);
+    hipLaunchKernelGGL((k_zgemm<128, 16, 16, 4, 4, 1, CA, CB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc, bz);
 }
 
 template <int CFG>
