@@ -499,11 +499,19 @@ __device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, con
 // the register arrays they come from stay in registers).
 __device__ inline cd csel(bool c, cd a, cd b) { return mkc(c ? a.x : b.x, c ? a.y : b.y); }
 
-template <bool LEFT>
-__device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1, const Rot* rots, int rot_stride,
+//
+// STAGED: the pass spans more steps than LDS holds. `rots` is then an LDS stage of `chunk`
+// steps (a multiple of U) that the workgroup refills from `rglob` (all steps, M reflectors
+// each, from step 0) whenever the pass crosses a chunk boundary, so each strip element is still
+// read and written once for all steps [s0, s1). Every thread of the workgroup must make the
+// same calls (the refills synchronize the workgroup).
+template <bool LEFT, bool STAGED>
+__device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1, Rot* rots, int rot_stride,
                                                           int vlo, int vhi, const cd* base, int nbytes,
-                                                          int estride, int soff0, bool active) {
+                                                          int estride, int soff0, bool active,
+                                                          const Rot* rglob, int chunk) {
   ASSUME_LDS(rots);
+  ASSUME_GLOBAL(rglob);
   constexpr int G = 8;                    // bulges per group
   constexpr int B = 3 * (G - 1) + 2;
   constexpr int U = 3;
@@ -515,6 +523,22 @@ __device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1
   const int lo_g = g1 ? lo - 3 * G : lo;
   const Rot* rots_g = rots + (g1 ? G : 0);
   const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc(base, nbytes);
+  int cbase = s0;   // first step held by the stage
+  auto restage = [&](int sb) {
+    __syncthreads();
+    const int ns = (s1 - sb < chunk) ? s1 - sb : chunk;
+    for (int e = threadIdx.x; e < ns * rot_stride; e += NT) rots[e] = rglob[(long long)sb * rot_stride + e];
+    __syncthreads();
+    cbase = sb;
+  };
+  if constexpr (STAGED) {
+    restage(s0);
+    // a wave with no strip in range only takes part in the refills
+    if (__ballot(active) == 0) {
+      for (int sb = s0 + chunk; sb < s1; sb += chunk) restage(sb);
+      return;
+    }
+  }
   cd band[B + U - 1];
   cd pre[U];
   cd carried[2];
@@ -552,13 +576,16 @@ __device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1
   int dep = 0;
   const int ng = (s1 - s0) / U;
   for (int g = 0; g < ng; ++g, s += U) {
+    if constexpr (STAGED) {
+      if (s - cbase >= chunk) restage(s);
+    }
     const int base = lo_g + s - 3 * (G - 1);
     band[B - 1] = csel(g1, carried[0], pre[0]);
     band[B] = csel(g1, carried[1], pre[1]);
     band[B + 1] = pre[2];                      // group 1: replaced after step 0
 #pragma unroll
     for (int u = 0; u < U; ++u) pre[u] = buf_load(rsrc, g1 ? BUF_OOB : off(base + U + B - 1 + u, hi_read));
-    const Rot* rs = rots_g + (long long)(s - s0) * rot_stride;
+    const Rot* rs = rots_g + (long long)(s - cbase) * rot_stride;
     Rot cur[BC], nxt[BC];
 #pragma unroll
     for (int jj = 0; jj < BC; ++jj) cur[jj] = rs[jj];
@@ -600,9 +627,12 @@ __device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1
 #pragma unroll
   for (int u = 0; u < U - 1; ++u) {
     if (s >= s1) break;
+    if constexpr (STAGED) {
+      if (s - cbase >= chunk) restage(s);
+    }
     const int base = lo_g + s - 3 * (G - 1);
     band[B - 1] = csel(g1, carried[0], pre[u]);
-    const Rot* rs = rots_g + (long long)(s - s0) * rot_stride;
+    const Rot* rs = rots_g + (long long)(s - cbase) * rot_stride;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const Rot r = rs[j];
@@ -659,18 +689,50 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
 // Deferred updates of one window's steps [s0, s1) (reflectors rw in LDS, M per step) on the
 // strips [first, end) of a matrix: M = 8 one band per thread, M = 16 a band pair per two lanes.
 template <int M, bool LEFT>
-__device__ inline void deferred(int lo, int s0, int s1, const Rot* rw, int vlo, int vhi, const cd* base,
+__device__ inline void deferred(int lo, int s0, int s1, Rot* rw, int vlo, int vhi, const cd* base,
                                 int nbytes, int estride, int sstride, int first, int end) {
   const int t = threadIdx.x;
   if constexpr (M == 16) {
     for (int xb = first + (t >> 6) * 32; xb < end; xb += NT / 2) {
       const int x = xb + (t & 31);
-      band_apply_pair<LEFT>(lo, s0, s1, rw, M, vlo, vhi, base, nbytes, estride, sstride * x, x < end);
+      band_apply_pair<LEFT, false>(lo, s0, s1, rw, M, vlo, vhi, base, nbytes, estride, sstride * x, x < end,
+                                   nullptr, 0);
     }
   } else {
     static_assert(M == 8, "8 or 16 bulges per sweep");
     for (int x = first + t; x < end; x += NT)
       band_apply<M, LEFT, 1>(lo, s0, s1, rw, M, vlo, vhi, base, nbytes, estride, sstride * x, 0);
+  }
+}
+
+// Deferred updates of a whole sweep's steps [0, nsteps) on the strips [first, end), the
+// reflectors read from rot_all (step-major, M per step) through the LDS stage `rz` of `cap`
+// steps. M = 16: one pass per strip (the stage is refilled inside the pass, every thread takes
+// part); M = 8: one pass per stage-full of steps.
+template <int M, bool LEFT>
+__device__ inline void deferred_sweep(int lo, int nsteps, const Rot* rot_all, Rot* rz, int cap, int vlo,
+                                      int vhi, const cd* base, int nbytes, int estride, int sstride,
+                                      int first, int end) {
+  const int t = threadIdx.x;
+  if (end <= first) return;
+  if constexpr (M == 16) {
+    const int chunk = (cap / 3) * 3;
+    const int iters = (end - first + NT / 2 - 1) / (NT / 2);
+    for (int it = 0; it < iters; ++it) {
+      const int x = first + it * (NT / 2) + (t >> 6) * 32 + (t & 31);
+      band_apply_pair<LEFT, true>(lo, 0, nsteps, rz, M, vlo, vhi, base, nbytes, estride, sstride * x, x < end,
+                                  rot_all, chunk);
+    }
+    __syncthreads();
+  } else {
+    for (int c0 = 0; c0 < nsteps; c0 += cap) {
+      const int c1 = (c0 + cap < nsteps) ? c0 + cap : nsteps;
+      __syncthreads();
+      for (int e = t; e < (c1 - c0) * M; e += NT) rz[e] = rot_all[(long long)c0 * M + e];
+      __syncthreads();
+      deferred<M, LEFT>(lo, c0, c1, rz, vlo, vhi, base, nbytes, estride, sstride, first, end);
+    }
+    __syncthreads();
   }
 }
 
@@ -759,27 +821,23 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
     Rot* rw = (Rot*)Hw;
     for (int e = t; e < (s_end - s_begin) * M; e += NT) rw[e] = rot_all[(long long)s_begin * M + e];
     __syncthreads();
-    // deferred row operations on the columns right of the window (rows a..wend-1), then column
-    // operations on the rows above it (columns a..wend-1)
-    deferred<M, true>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16 * n, 16, wend, n);
-    deferred<M, false>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16, 16 * n, 0, a);
+    // deferred row operations on the active columns right of the window (rows a..wend-1), then
+    // column operations on the active rows above it (columns a..wend-1): later windows read them
+    deferred<M, true>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16 * n, 16, wend, hi + 1);
+    deferred<M, false>(lo, s_begin, s_end, rw, a, wend, H, nbytes, 16, 16 * n, lo, a);
     __syncthreads();
     s_begin = s_end;
   }
-  // Schur vectors: every row of Z gets the whole sweep's column operations, the sweep's
-  // reflectors staged through LDS (the window area) in chunks of steps
+  // The strips no window reads get the whole sweep in one pass each: the columns right of the
+  // active block (row operations on rows lo..hi), the rows above it (column operations on
+  // columns lo..hi) and the Schur vectors (every row of Z: column operations; Z^H: row
+  // operations on its column strips)
   Rot* rz = (Rot*)Hw;
-  const int chunk = (int)((sizeof(cd) * (size_t)W * (W + 1)) / (sizeof(Rot) * M));
-  for (int c0 = 0; c0 <= T; c0 += chunk) {
-    const int c1 = (c0 + chunk < T + 1) ? c0 + chunk : T + 1;
-    __syncthreads();
-    for (int e = t; e < (c1 - c0) * M; e += NT) rz[e] = rot_all[(long long)c0 * M + e];
-    __syncthreads();
-    // Z^H: row operations on its column strips (consecutive lanes on consecutive columns);
-    // Z: column operations on its row strips
-    if (zh) deferred<M, true>(lo, c0, c1, rz, lo, hi + 1, Z, nbytes, 16 * n, 16, 0, n);
-    else deferred<M, false>(lo, c0, c1, rz, lo, hi + 1, Z, nbytes, 16, 16 * n, 0, n);
-  }
+  const int cap = (int)((sizeof(cd) * (size_t)W * (W + 1)) / (sizeof(Rot) * M));
+  deferred_sweep<M, true>(lo, T + 1, rot_all, rz, cap, lo, hi + 1, H, nbytes, 16 * n, 16, hi + 1, n);
+  deferred_sweep<M, false>(lo, T + 1, rot_all, rz, cap, lo, hi + 1, H, nbytes, 16, 16 * n, 0, lo);
+  if (zh) deferred_sweep<M, true>(lo, T + 1, rot_all, rz, cap, lo, hi + 1, Z, nbytes, 16 * n, 16, 0, n);
+  else deferred_sweep<M, false>(lo, T + 1, rot_all, rz, cap, lo, hi + 1, Z, nbytes, 16, 16 * n, 0, n);
   __syncthreads();
 }
 
