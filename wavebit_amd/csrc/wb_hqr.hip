@@ -1,20 +1,20 @@
 // Windowed small-bulge multishift QR for the complex Hessenberg eigenproblem, with Schur
 // vectors (one 512-thread workgroup per matrix).
 //
-// A multishift sweep chases M single-shift bulges (2-element reflectors) down the active block
+// A multishift sweep chases M single-shift bulges (plane rotations) down the active block
 // [lo, hi] in a tightly packed chain: bulge j enters at step 3j and at step s acts at position
-// k = lo + s - 3j. Spacing 3 keeps the bulges' reflector inputs disjoint, so each step computes
-// all M reflectors, then applies all M row operations, then all M column operations, and the
+// k = lo + s - 3j. Spacing 3 keeps the bulges' rotation inputs disjoint, so each step computes
+// all M rotations, then applies all M row operations, then all M column operations, and the
 // result equals M consecutive single-shift sweeps (the implicit-Q argument behind
 // Braman-Byers-Mathias small-bulge multishift QR; Golub & Van Loan 7.5.7).
 //
 // The chase runs in a W x W diagonal window held in LDS. Everything outside the window is
-// deferred and applied from the recorded reflectors by a register "band": in step s, bulge j
+// deferred and applied from the recorded rotations by a register "band": in step s, bulge j
 // always touches band slots 3(G-1-j) and 3(G-1-j)+1 of its group's band [lo+s-3(G-1), lo+s+1]
 // (G = 8 bulges per group), so a strip of H or Z streams through B = 3(G-1)+2 registers with
 // compile-time indices. With M = 16 the two groups of a strip run on the two half-waves in
 // lockstep (band_apply_pair), so each element is read and written once per window (H right of
-// and above the window) or once per sweep (Z) for all 16 bulges. The window's reflectors are
+// and above the window) or once per sweep (Z) for all 16 bulges. The window's rotations are
 // written to global memory during the chase and staged back into the window's LDS for the
 // deferred updates. Shifts are the eigenvalues of the trailing M x M block (one wave's QR
 // iteration, wave_qr); active blocks of at most nmin rows finish with a single-shift iteration
@@ -55,49 +55,64 @@ constexpr double ULP = DBL_EPSILON;
 
 constexpr double SAFMIN = DBL_MIN;
 
+// Plane rotation Q = [c s; -conj(s) c] (c real, c^2 + |s|^2 = 1) with Q [f; g] = [r; 0]. A QR
+// step applies it as the similarity H <- Q H Q^H (rows k, k+1 from the left, columns k, k+1
+// from the right) and Z <- Z Q^H. Three doubles: a rotation costs 12 FP64 operations per pair
+// of entries (a 2-element complex reflector with complex tau costs 14), and the subdiagonal
+// entries it produces are complex in general.
 struct Rot {
-  cd tau;
-  cd u;
+  double c;
+  double sx, sy;
 };
 
-// 2-element reflector H = I - tau [1;u][1;u]^H with H^H [x1; x2] = [beta; 0].
-__device__ inline void refl2(cd x1, cd x2, cd* tau, cd* u, double* beta) {
-  const double x2n = cnorm2(x2);
-  if (x2n == 0.0 && x1.y == 0.0) {
-    *tau = mkc(0.0, 0.0); *u = mkc(0.0, 0.0); *beta = x1.x;
-    return;
+// Q [f; g] = [r; 0] (zlartg): c = |f| / d, s = (f / |f|) conj(g) / d, r = (f / |f|) d with
+// d = sqrt(|f|^2 + |g|^2), computed on f and g scaled by max(|f|_1, |g|_1).
+__device__ inline Rot givens(cd f, cd g, cd* r) {
+  Rot q;
+  if (g.x == 0.0 && g.y == 0.0) {
+    q.c = 1.0; q.sx = 0.0; q.sy = 0.0;
+    *r = f;
+    return q;
   }
-  const double b = -copysign(sqrt(x1.x * x1.x + x1.y * x1.y + x2n), x1.x);
-  *beta = b;
-  *tau = mkc((b - x1.x) / b, -x1.y / b);
-  *u = cdiv(x2, mkc(x1.x - b, x1.y));
+  if (f.x == 0.0 && f.y == 0.0) {
+    const double gn = cabs(g);
+    q.c = 0.0; q.sx = g.x / gn; q.sy = -g.y / gn;
+    *r = mkc(gn, 0.0);
+    return q;
+  }
+  const double scale = fmax(cabs1(f), cabs1(g));
+  const cd fs = cscale(f, 1.0 / scale), gs = cscale(g, 1.0 / scale);
+  const double f2 = cnorm2(fs), g2 = cnorm2(gs);
+  const double fa = sqrt(f2), d = sqrt(f2 + g2);
+  const cd ph = cscale(fs, 1.0 / fa);          // f / |f|
+  q.c = fa / d;
+  const double id = 1.0 / d;
+  q.sx = (ph.x * gs.x + ph.y * gs.y) * id;     // ph conj(gs) / d
+  q.sy = (ph.y * gs.x - ph.x * gs.y) * id;
+  *r = cscale(ph, d * scale);
+  return q;
 }
 
-// left application (rows k, k+1): [a0; a1] <- H^H [a0; a1], i.e. s = conj(tau) (a0 + conj(u) a1),
-// a0 -= s, a1 -= s u (14 FP64 operations, fused)
-__device__ inline void rot_left(const Rot& r, cd& a0, cd& a1) {
-  const cd u = r.u, tau = r.tau;
-  const double tx = fma(u.x, a1.x, fma(u.y, a1.y, a0.x));
-  const double ty = fma(u.x, a1.y, fma(-u.y, a1.x, a0.y));
-  const double sx = fma(tau.x, tx, tau.y * ty);
-  const double sy = fma(tau.x, ty, -(tau.y * tx));
-  a0.x -= sx;
-  a0.y -= sy;
-  a1.x = fma(-sx, u.x, fma(sy, u.y, a1.x));
-  a1.y = fma(-sx, u.y, fma(-sy, u.x, a1.y));
+// left application (rows k, k+1): [a0; a1] <- Q [a0; a1]: a0 = c a0 + s a1, a1 = c a1 - conj(s) a0
+__device__ inline void rot_left(const Rot& q, cd& a0, cd& a1) {
+  const double c = q.c, sx = q.sx, sy = q.sy;
+  const double b0x = fma(sx, a1.x, fma(-sy, a1.y, c * a0.x));
+  const double b0y = fma(sx, a1.y, fma(sy, a1.x, c * a0.y));
+  const double b1x = fma(-sx, a0.x, fma(-sy, a0.y, c * a1.x));
+  const double b1y = fma(-sx, a0.y, fma(sy, a0.x, c * a1.y));
+  a0 = mkc(b0x, b0y);
+  a1 = mkc(b1x, b1y);
 }
-// right application (columns k, k+1): [a0, a1] <- [a0, a1] H, i.e. s = tau (a0 + u a1),
-// a0 -= s, a1 -= s conj(u)
-__device__ inline void rot_right(const Rot& r, cd& a0, cd& a1) {
-  const cd u = r.u, tau = r.tau;
-  const double tx = fma(u.x, a1.x, fma(-u.y, a1.y, a0.x));
-  const double ty = fma(u.x, a1.y, fma(u.y, a1.x, a0.y));
-  const double sx = fma(tau.x, tx, -(tau.y * ty));
-  const double sy = fma(tau.x, ty, tau.y * tx);
-  a0.x -= sx;
-  a0.y -= sy;
-  a1.x = fma(-sx, u.x, fma(-sy, u.y, a1.x));
-  a1.y = fma(-sy, u.x, fma(sx, u.y, a1.y));
+// right application (columns k, k+1): [a0, a1] <- [a0, a1] Q^H: a0 = c a0 + conj(s) a1,
+// a1 = c a1 - s a0
+__device__ inline void rot_right(const Rot& q, cd& a0, cd& a1) {
+  const double c = q.c, sx = q.sx, sy = q.sy;
+  const double b0x = fma(sx, a1.x, fma(sy, a1.y, c * a0.x));
+  const double b0y = fma(sx, a1.y, fma(-sy, a1.x, c * a0.y));
+  const double b1x = fma(-sx, a0.x, fma(sy, a0.y, c * a1.x));
+  const double b1y = fma(-sx, a0.y, fma(-sy, a0.x, c * a1.y));
+  a0 = mkc(b0x, b0y);
+  a1 = mkc(b1x, b1y);
 }
 
 // Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo.
@@ -111,8 +126,8 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
     if (!small) {
       double tst = cabs1(H[(long long)(k - 1) * n + k - 1]) + cabs1(H[(long long)k * n + k]);
       if (tst == 0.0) {
-        if (k - 2 >= 0) tst += fabs(H[(long long)(k - 1) * n + k - 2].x);
-        if (k + 1 <= n - 1) tst += fabs(H[(long long)(k + 1) * n + k].x);
+        if (k - 2 >= 0) tst += cabs1(H[(long long)(k - 1) * n + k - 2]);
+        if (k + 1 <= n - 1) tst += cabs1(H[(long long)(k + 1) * n + k]);
       }
       if (cabs1(hk) <= ULP * tst) {
         const double h1 = cabs1(hk), h2 = cabs1(H[(long long)(k - 1) * n + k]);
@@ -138,7 +153,7 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
 }
 
 // Active blocks of at most nmin rows. The block's single-shift QR (the zlahqr scheme) runs in
-// LDS on one wave with no barriers, every reflector also applied to U (the block's accumulated
+// LDS on one wave with no barriers, every rotation also applied to U (the block's accumulated
 // unitary); the rows of H above the block, its columns right of the block and the Schur vectors
 // then get U in three small products with all threads. Returns 0, or i + 1 if eigenvalue i of
 // the block did not converge.
@@ -162,8 +177,8 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
         if (!small) {
           double tst = cabs1(TT(k - 1, k - 1)) + cabs1(TT(k, k));
           if (tst == 0.0) {
-            if (k - 2 >= 0) tst += fabs(TT(k - 1, k - 2).x);
-            if (k + 1 <= m - 1) tst += fabs(TT(k + 1, k).x);
+            if (k - 2 >= 0) tst += cabs1(TT(k - 1, k - 2));
+            if (k + 1 <= m - 1) tst += cabs1(TT(k + 1, k));
           }
           if (cabs1(hk) <= ULP * tst) {
             const double h1 = cabs1(hk), h2 = cabs1(TT(k - 1, k));
@@ -183,9 +198,9 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
       ++kdefl;
       cd sh;
       if (kdefl % 20 == 0) {
-        sh = cadd(mkc(0.75 * fabs(TT(i, i - 1).x), 0.0), TT(i, i));
+        sh = cadd(mkc(0.75 * cabs1(TT(i, i - 1)), 0.0), TT(i, i));
       } else if (kdefl % 10 == 0) {
-        sh = cadd(mkc(0.75 * fabs(TT(l + 1, l).x), 0.0), TT(l, l));
+        sh = cadd(mkc(0.75 * cabs1(TT(l + 1, l)), 0.0), TT(l, l));
       } else {
         sh = TT(i, i);
         const cd u = cmul(csqrt_p(TT(i - 1, i)), csqrt_p(TT(i, i - 1)));
@@ -214,11 +229,10 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
           v1 = TT(kk, kk - 1);
           v2 = TT(kk + 1, kk - 1);
         }
-        Rot r;
-        double beta;
-        refl2(v1, v2, &r.tau, &r.u, &beta);
+        cd rr;
+        const Rot r = givens(v1, v2, &rr);
         if (kk > l && lane == 0) {
-          TT(kk, kk - 1) = mkc(beta, 0.0);
+          TT(kk, kk - 1) = rr;
           TT(kk + 1, kk - 1) = mkc(0.0, 0.0);
         }
         // row operation on columns kk..m-1, then the column operation on rows 0..min(kk+2, i)
@@ -363,11 +377,11 @@ __device__ inline void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, cd v) {
 }
 
 
-// Apply the steps [s0, s1) of a group of G consecutive bulges' reflectors to one strip through
+// Apply the steps [s0, s1) of a group of G consecutive bulges' rotations to one strip through
 // a B-register band. left: row operations on a column strip (rows), else column operations on
 // a row strip. get(i)/put(i) address strip element i (absolute index); [vlo, vhi) is the valid
 // range. `lo` is the group's entry position (lo - 3 g0 for the group starting at bulge g0) and
-// `rots` points at the group's first bulge in rows of rot_stride reflectors per step. Bulge j
+// `rots` points at the group's first bulge in rows of rot_stride rotations per step. Bulge j
 // only conflicts with earlier rotations of bulge j - 1, so whole groups may be applied one after
 // another in bulge order: the result equals the step-by-step order of the chase.
 template <int M, bool LEFT, int S>
@@ -379,9 +393,9 @@ __device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, con
   // slots 3 (M - 1 - j) + u, + u + 1), so the band shifts by three registers once per group
   // rather than by one every step, and the group's three entering elements are loaded one
   // group ahead, overlapping their memory latency with the rotations of the current group.
-  // Within a step the bulges are independent; they are applied BC at a time, the reflectors
+  // Within a step the bulges are independent; they are applied BC at a time, the rotations
   // of the next BC loaded while the current ones are applied, and pinned so that no more
-  // than two chunks of reflectors are held in registers.
+  // than two chunks of rotations are held in registers.
   constexpr int B = 3 * (M - 1) + 2;
   constexpr int U = 3;
   constexpr int BC = (M % 4 == 0) ? 4 : M;
@@ -409,7 +423,7 @@ __device__ __attribute__((noinline)) void band_apply(int lo, int s0, int s1, con
 #pragma unroll
     for (int k = 0; k < S; ++k) pre[k][u] = buf_load(rsrc, off(k, base0 + B - 1 + u, hi_read));
   int s = s0;
-  int dep = 0;   // always 0; threaded through the pins to order the reflector loads
+  int dep = 0;   // always 0; threaded through the pins to order the rotation loads
   const int ng = (s1 - s0) / U;
   for (int g = 0; g < ng; ++g, s += U) {
     const int base = lo + s - 3 * (M - 1);
@@ -501,7 +515,7 @@ __device__ inline cd csel(bool c, cd a, cd b) { return mkc(c ? a.x : b.x, c ? a.
 
 //
 // STAGED: the pass spans more steps than LDS holds. `rots` is then an LDS stage of `chunk`
-// steps (a multiple of U) that the workgroup refills from `rglob` (all steps, M reflectors
+// steps (a multiple of U) that the workgroup refills from `rglob` (all steps, M rotations
 // each, from step 0) whenever the pass crosses a chunk boundary, so each strip element is still
 // read and written once for all steps [s0, s1). Every thread of the workgroup must make the
 // same calls (the refills synchronize the workgroup).
@@ -655,11 +669,12 @@ __device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1
 }
 
 // Reflector of bulge j at chase position k (k == lo: the bulge enters with shift sh).
-// Writes beta / 0 into the window for k > lo. Inactive positions give the identity.
-__device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, int k, cd sh) {
+// Writes r / 0 into the window for k > lo. Inactive positions give the identity.
+__device__ inline Rot bulge_rotation(cd* Hw, int ldw, int a, int lo, int hi, int k, cd sh) {
   Rot r;
-  r.tau = mkc(0.0, 0.0);
-  r.u = mkc(0.0, 0.0);
+  r.c = 1.0;
+  r.sx = 0.0;
+  r.sy = 0.0;
   if (k < lo || k > hi - 1) return r;
   cd x1, x2;
   if (k == lo) {
@@ -671,10 +686,10 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
     x1 = Hw[(k - a) * ldw + (k - 1 - a)];
     x2 = Hw[(k + 1 - a) * ldw + (k - 1 - a)];
   }
-  double beta;
-  refl2(x1, x2, &r.tau, &r.u, &beta);
+  cd rr;
+  r = givens(x1, x2, &rr);
   if (k > lo) {
-    Hw[(k - a) * ldw + (k - 1 - a)] = mkc(beta, 0.0);
+    Hw[(k - a) * ldw + (k - 1 - a)] = rr;
     Hw[(k + 1 - a) * ldw + (k - 1 - a)] = mkc(0.0, 0.0);
   }
   return r;
@@ -683,10 +698,10 @@ __device__ inline Rot bulge_reflector(cd* Hw, int ldw, int a, int lo, int hi, in
 // One multishift sweep over the active block [lo, hi] with shifts sh[0..M).
 // Per chase step two barriers: row operations of all bulges, then their column operations,
 // where the thread owning bulge j also finishes rows k+1, k+2 of its column operation and
-// forms bulge j's reflector for the next step from them (the only inputs it needs). The
-// current and next steps' reflectors live in an LDS ring; every step's reflectors are also
+// forms bulge j's rotation for the next step from them (the only inputs it needs). The
+// current and next steps' rotations live in an LDS ring; every step's rotations are also
 // logged to rot_all for the deferred updates outside the window and of Z.
-// Deferred updates of one window's steps [s0, s1) (reflectors rw in LDS, M per step) on the
+// Deferred updates of one window's steps [s0, s1) (rotations rw in LDS, M per step) on the
 // strips [first, end) of a matrix: M = 8 one band per thread, M = 16 a band pair per two lanes.
 template <int M, bool LEFT>
 __device__ inline void deferred(int lo, int s0, int s1, Rot* rw, int vlo, int vhi, const cd* base,
@@ -706,7 +721,7 @@ __device__ inline void deferred(int lo, int s0, int s1, Rot* rw, int vlo, int vh
 }
 
 // Deferred updates of a whole sweep's steps [0, nsteps) on the strips [first, end), the
-// reflectors read from rot_all (step-major, M per step) through the LDS stage `rz` of `cap`
+// rotations read from rot_all (step-major, M per step) through the LDS stage `rz` of `cap`
 // steps. M = 16: one pass per strip (the stage is refilled inside the pass, every thread takes
 // part); M = 8: one pass per stage-full of steps.
 template <int M, bool LEFT>
@@ -767,9 +782,9 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
     }
     __syncthreads();
     if (first) {
-      // reflectors of step 0 (only bulge 0 can be active)
+      // rotations of step 0 (only bulge 0 can be active)
       if (t < M) {
-        const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, lo + s_begin - 3 * t, sh[t]);
+        const Rot r = bulge_rotation(Hw, ldw, a, lo, hi, lo + s_begin - 3 * t, sh[t]);
         s_rot[s_begin & 1][t] = r;
         rot_all[(long long)s_begin * M + t] = r;
       }
@@ -788,7 +803,7 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
       }
       __syncthreads();
       // column operations: bulge j on columns k, k+1, rows a..k; rows k+1, k+2 and the next
-      // reflector by thread j
+      // rotation by thread j
       for (int j = t >> 6; j < M; j += NT / 64) {
         const int k = lo + s - 3 * j;
         if (k < lo || k > hi - 1) continue;
@@ -804,7 +819,7 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
             rot_right(rs[j], Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
         }
         if (s + 1 <= T) {
-          const Rot r = bulge_reflector(Hw, ldw, a, lo, hi, k + 1, sh[j]);
+          const Rot r = bulge_rotation(Hw, ldw, a, lo, hi, k + 1, sh[j]);
           s_rot[(s + 1) & 1][j] = r;
           rot_all[(long long)(s + 1) * M + j] = r;
         }
@@ -816,7 +831,7 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
       const int r = e / ww, c = e % ww;
       H[(long long)(a + r) * n + a + c] = Hw[r * ldw + c];
     }
-    // the window's reflectors, from rot_all into the LDS the window no longer needs
+    // the window's rotations, from rot_all into the LDS the window no longer needs
     __syncthreads();
     Rot* rw = (Rot*)Hw;
     for (int e = t; e < (s_end - s_begin) * M; e += NT) rw[e] = rot_all[(long long)s_begin * M + e];
@@ -943,7 +958,7 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void*
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     // dynamic LDS: the W x W window; after the window is stored the same space holds its
-    // reflectors (at most W + 3M steps of M), and the small-block iteration uses it for T and
+    // rotations (at most W + 3M steps of M), and the small-block iteration uses it for T and
     // U. The rest is the kernel's static LDS.
     const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * NT + sizeof(cd) * (M + M * M) + 2048;
     for (int c = 128; c >= 32; c -= 8) {
