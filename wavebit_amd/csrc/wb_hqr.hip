@@ -696,9 +696,10 @@ __device__ inline Rot bulge_rotation(cd* Hw, int ldw, int a, int lo, int hi, int
 }
 
 // One multishift sweep over the active block [lo, hi] with shifts sh[0..M).
-// Per chase step two barriers: row operations of all bulges, then their column operations,
-// where the thread owning bulge j also finishes rows k+1, k+2 of its column operation and
-// forms bulge j's rotation for the next step from them (the only inputs it needs). The
+// Per chase step two barriers: row operations of all bulges, then their column operations
+// on waves 1.., while lane j of wave 0 finishes rows k+1, k+2 of bulge j's column operation in
+// registers and forms bulge j's rotation for the next step from them (the only inputs it
+// needs), the step's serial chain. The
 // current and next steps' rotations live in an LDS ring; every step's rotations are also
 // logged to rot_all for the deferred updates outside the window and of Z.
 // Deferred updates of one window's steps [s0, s1) (rotations rw in LDS, M per step) on the
@@ -802,26 +803,51 @@ __device__ void multishift_sweep(cd* H, cd* Z, int zh, int n, int lo, int hi, co
           rot_left(r, Hw[(k - a) * ldw + (c - a)], Hw[(k + 1 - a) * ldw + (c - a)]);
       }
       __syncthreads();
-      // column operations: bulge j on columns k, k+1, rows a..k; rows k+1, k+2 and the next
-      // rotation by thread j
-      for (int j = t >> 6; j < M; j += NT / 64) {
-        const int k = lo + s - 3 * j;
-        if (k < lo || k > hi - 1) continue;
-        const Rot r = rs[j];
-        for (int q = a + (t & 63); q <= k; q += 64)
-          rot_right(r, Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
-      }
-      if (t < M) {
+      // column operations: bulge j on columns k, k+1, rows a..k (waves 1.. ); rows k+1, k+2
+      // and the next rotation by lane j of wave 0, the serial part of the step
+      if (t >= 64) {
+        for (int j = (t >> 6) - 1; j < M; j += NT / 64 - 1) {
+          const int k = lo + s - 3 * j;
+          if (k < lo || k > hi - 1) continue;
+          const Rot r = rs[j];
+          for (int q = a + (t & 63); q <= k; q += 64)
+            rot_right(r, Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+        }
+      } else if (t < M) {
         const int j = t;
         const int k = lo + s - 3 * j;
+        Rot nr;
+        nr.c = 1.0;
+        nr.sx = 0.0;
+        nr.sy = 0.0;
         if (k >= lo && k <= hi - 1) {
-          for (int q = k + 1; q <= k + 2 && q <= hi; ++q)
-            rot_right(rs[j], Hw[(q - a) * ldw + (k - a)], Hw[(q - a) * ldw + (k + 1 - a)]);
+          // rows k+1 and k+2 of columns k, k+1 stay in registers: rotated, then the entries
+          // (k+1, k) and (k+2, k) give the next rotation (position k+1) directly
+          const Rot r = rs[j];
+          cd* p1 = &Hw[(k + 1 - a) * ldw + (k - a)];
+          cd a10 = p1[0], a11 = p1[1];
+          rot_right(r, a10, a11);
+          p1[1] = a11;
+          if (k + 2 <= hi) {
+            cd* p2 = p1 + ldw;
+            cd a20 = p2[0], a21 = p2[1];
+            rot_right(r, a20, a21);
+            p2[1] = a21;
+            if (s + 1 <= T) {
+              cd rr;
+              nr = givens(a10, a20, &rr);
+              a10 = rr;
+              a20 = mkc(0.0, 0.0);
+            }
+            p2[0] = a20;
+          }
+          p1[0] = a10;
+        } else if (s + 1 <= T) {
+          nr = bulge_rotation(Hw, ldw, a, lo, hi, k + 1, sh[j]);   // entering or inactive
         }
         if (s + 1 <= T) {
-          const Rot r = bulge_rotation(Hw, ldw, a, lo, hi, k + 1, sh[j]);
-          s_rot[(s + 1) & 1][j] = r;
-          rot_all[(long long)(s + 1) * M + j] = r;
+          s_rot[(s + 1) & 1][j] = nr;
+          rot_all[(long long)(s + 1) * M + j] = nr;
         }
       }
       __syncthreads();
