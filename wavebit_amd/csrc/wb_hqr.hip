@@ -152,6 +152,17 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
   return r;
 }
 
+// Lane `src` (wave-uniform) of a complex value, to every lane of the wave.
+__device__ inline cd readlane_c(cd x, int src) {
+  const unsigned long long rb = __double_as_longlong(x.x), ib = __double_as_longlong(x.y);
+  const int r0 = __builtin_amdgcn_readlane((int)(rb & 0xffffffffull), src);
+  const int r1 = __builtin_amdgcn_readlane((int)(rb >> 32), src);
+  const int i0 = __builtin_amdgcn_readlane((int)(ib & 0xffffffffull), src);
+  const int i1 = __builtin_amdgcn_readlane((int)(ib >> 32), src);
+  return mkc(__longlong_as_double(((unsigned long long)(unsigned)r1 << 32) | (unsigned)r0),
+             __longlong_as_double(((unsigned long long)(unsigned)i1 << 32) | (unsigned)i0));
+}
+
 // Active blocks of at most nmin rows. The block's single-shift QR (the zlahqr scheme) runs in
 // LDS on one wave with no barriers, every rotation also applied to U (the block's accumulated
 // unitary); the rows of H above the block, its columns right of the block and the Schur vectors
@@ -218,17 +229,12 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
           sh = csub(sh, cmul(u, cdiv(u, cadd(x, y))));
         }
       }
+      cd v1 = csub(TT(l, l), sh), v2 = TT(l + 1, l);
+      {
+        const double sc = cabs1(v1) + cabs1(v2);
+        if (sc != 0.0) { v1 = cscale(v1, 1.0 / sc); v2 = cscale(v2, 1.0 / sc); }
+      }
       for (int kk = l; kk < i; ++kk) {
-        cd v1, v2;
-        if (kk == l) {
-          v1 = csub(TT(l, l), sh);
-          v2 = TT(l + 1, l);
-          const double sc = cabs1(v1) + cabs1(v2);
-          if (sc != 0.0) { v1 = cscale(v1, 1.0 / sc); v2 = cscale(v2, 1.0 / sc); }
-        } else {
-          v1 = TT(kk, kk - 1);
-          v2 = TT(kk + 1, kk - 1);
-        }
         cd rr;
         const Rot r = givens(v1, v2, &rr);
         if (kk > l && lane == 0) {
@@ -239,8 +245,21 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
         // and on U (LDS accesses of one wave stay in program order)
         if (lane >= kk && lane < m) rot_left(r, TT(kk, lane), TT(kk + 1, lane));
         const int rmax = (kk + 2 < i) ? kk + 2 : i;
-        if (lane <= rmax) rot_right(r, TT(lane, kk), TT(lane, kk + 1));
+        cd col = mkc(0.0, 0.0);
+        if (lane <= rmax) {
+          cd x0 = TT(lane, kk), x1 = TT(lane, kk + 1);
+          rot_right(r, x0, x1);
+          TT(lane, kk) = x0;
+          TT(lane, kk + 1) = x1;
+          col = x0;
+        }
         if (U && lane < m) rot_right(r, U[lane * ld + kk], U[lane * ld + kk + 1]);
+        // the next rotation's inputs (rows kk+1, kk+2 of column kk) straight from the lanes that
+        // just computed them
+        if (kk + 1 < i) {
+          v1 = readlane_c(col, kk + 1);
+          v2 = readlane_c(col, kk + 2);
+        }
       }
     }
     if (!converged) return i + 1;
