@@ -32,6 +32,9 @@ namespace {
 #ifndef WB_HQR_M
 #define WB_HQR_M 16        // bulges (shifts) per sweep: 8 or 16
 #endif
+#ifndef WB_HQR_BC
+#define WB_HQR_BC 2        // rotations per register chunk of the band updates
+#endif
 constexpr int NT = WB_HQR_NT;
 static_assert(NT % 64 == 0, "whole waves");
 
@@ -141,13 +144,16 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
     }
     if (small && k > found) found = k;
   }
-  s_k[t] = found;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (t < s && s_k[t + s] > s_k[t]) s_k[t] = s_k[t + s];
-    __syncthreads();
+  // maximum over the workgroup: within each wave, then over the waves
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(found, off);
+    found = (o > found) ? o : found;
   }
-  const int r = s_k[0];
+  if ((t & 63) == 0) s_k[t >> 6] = found;
+  __syncthreads();
+  int r = s_k[0];
+  for (int w = 1; w < NT / 64; ++w) r = (s_k[w] > r) ? s_k[w] : r;
   __syncthreads();
   return r;
 }
@@ -548,7 +554,7 @@ __device__ __attribute__((noinline)) void band_apply_pair(int lo, int s0, int s1
   constexpr int G = 8;                    // bulges per group
   constexpr int B = 3 * (G - 1) + 2;
   constexpr int U = 3;
-  constexpr int BC = 4;
+  constexpr int BC = WB_HQR_BC;
   constexpr int CPS = G / BC;
   constexpr int NCH = U * CPS;
   const int lane = threadIdx.x & 63;
@@ -912,7 +918,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
   extern __shared__ unsigned char smem[];
   cd* Hw = (cd*)smem;
   __shared__ Rot s_rot[2][M];
-  __shared__ int s_k[NT];
+  __shared__ int s_k[NT / 64];
   __shared__ cd s_sh[M];
   __shared__ cd s_small[M * M];
   const int t = threadIdx.x;
@@ -1005,7 +1011,7 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void*
     // dynamic LDS: the W x W window; after the window is stored the same space holds its
     // rotations (at most W + 3M steps of M), and the small-block iteration uses it for T and
     // U. The rest is the kernel's static LDS.
-    const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * NT + sizeof(cd) * (M + M * M) + 2048;
+    const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * (NT / 64) + sizeof(cd) * (M + M * M) + 2048;
     for (int c = 128; c >= 32; c -= 8) {
       if (c < 3 * M + 16) break;
       if (hqr_lds_bytes(c) + fixed <= (size_t)smem) { W = c; break; }
