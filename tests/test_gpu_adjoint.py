@@ -37,7 +37,7 @@ def test_zgemm_ops_matches_numpy(opa, opb, m, n, k):
     assert np.abs(Cd.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize('n,nrhs', [(37, 3), (100, 2), (129, 129)])
+@pytest.mark.parametrize('n,nrhs', [(37, 3), (100, 2), (129, 129), (722, 2), (33, 1), (64, 4)])
 def test_getrs_conjugate_transpose(n, nrhs):
     from wavebit_amd import _native
     rng = np.random.default_rng(n + nrhs)

@@ -25,8 +25,10 @@ def test_zgemm_matches_numpy(m, n, k, batch):
     assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max()) * max(1, k) ** 0.5
 
 
+# nrhs <= 4 (and n x nrhs within 64 KiB of LDS) takes the one-workgroup-per-matrix solve
 @pytest.mark.parametrize('n,nrhs,batch', [(1, 1, 1), (5, 3, 2), (33, 40, 3), (97, 97, 2), (450, 8, 2),
-                                          (1000, 5, 1)])
+                                          (1000, 5, 1), (722, 2, 3), (722, 1, 2), (31, 4, 2), (64, 4, 1),
+                                          (1000, 4, 1)])
 def test_lu_solve_matches_numpy(n, nrhs, batch):
     rng = np.random.default_rng(n)
     A = crand(rng, batch, n, n)

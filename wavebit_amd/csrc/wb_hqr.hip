@@ -32,6 +32,12 @@ namespace {
 #ifndef WB_HQR_M
 #define WB_HQR_M 16        // bulges (shifts) per sweep: 8 or 16
 #endif
+#ifndef WB_HQR_NMIN
+#define WB_HQR_NMIN ((3 * WB_HQR_M + 8 > 48) ? 3 * WB_HQR_M + 8 : 48)   // largest small block
+#endif
+#ifndef WB_HQR_WSTEP
+#define WB_HQR_WSTEP 8     // window sizes tried, largest first
+#endif
 #ifndef WB_HQR_BC
 #define WB_HQR_BC 2        // rotations per register chunk of the band updates
 #endif
@@ -336,7 +342,7 @@ __device__ void uh_times_cols(cd* B, int ldb, int m, int cols, const cd* U, int 
   __syncthreads();
 }
 
-static_assert(3 * WB_HQR_M + 8 <= 64, "small blocks (at most nmin rows) must fit one wave");
+static_assert(WB_HQR_NMIN <= 64 && WB_HQR_NMIN >= WB_HQR_M + 2, "small blocks (at most nmin rows) fit one wave");
 __device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo, int hi, cd* lds,
                                                      int* s_info) {
   ASSUME_LDS(lds);
@@ -981,7 +987,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
 
 // smallest active block handled by the multishift sweep - 1 (blocks of at most nmin rows go to
 // small_block)
-static int hqr_nmin() { return (3 * WB_HQR_M + 8 > 48) ? 3 * WB_HQR_M + 8 : 48; }
+static int hqr_nmin() { return WB_HQR_NMIN; }
 static size_t hqr_lds_bytes(int W) {
   constexpr int M = WB_HQR_M;
   const size_t window = sizeof(cd) * (size_t)W * (W + 1);
@@ -1012,7 +1018,7 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void*
     // rotations (at most W + 3M steps of M), and the small-block iteration uses it for T and
     // U. The rest is the kernel's static LDS.
     const size_t fixed = sizeof(Rot) * 2 * M + sizeof(int) * (NT / 64) + sizeof(cd) * (M + M * M) + 2048;
-    for (int c = 128; c >= 32; c -= 8) {
+    for (int c = 128; c >= 32; c -= WB_HQR_WSTEP) {
       if (c < 3 * M + 16) break;
       if (hqr_lds_bytes(c) + fixed <= (size_t)smem) { W = c; break; }
     }

@@ -316,6 +316,164 @@ __global__ void k_set_identity(int n, BatchPtr B, int ldb) {
 
 inline BatchPtr offset(BatchPtr p, long long off) { p.base += off; return p; }
 
+// ----------------------------------------------------------------------------- few-column solves
+// A X = B (CONJ = false) or A^H X = B (CONJ = true) from the packed LU factors for at most
+// FEW_NR right-hand sides, the whole solve in one 512-thread workgroup per matrix: B and the
+// pivots staged in LDS (dynamic: n x nrhs complex + n ints), the row swaps applied there by one
+// lane per column, then both triangular sweeps in 32-row blocks -- the block's diagonal
+// triangle (staged in LDS) solved by wave 0 with one row per lane and the solved values
+// broadcast by readlane, the rows beyond the block updated by all threads -- and the result
+// written back once. Replaces 4 n / 32 small launches and a swap loop over global memory.
+constexpr int FEW_NR = 4;
+constexpr int FEW_B = 32;
+constexpr int FEW_NT = 512;
+
+__device__ inline cd few_readlane(cd x, int src) {
+  const unsigned long long rb = __double_as_longlong(x.x), ib = __double_as_longlong(x.y);
+  const int r0 = __builtin_amdgcn_readlane((int)(rb & 0xffffffffull), src);
+  const int r1 = __builtin_amdgcn_readlane((int)(rb >> 32), src);
+  const int i0 = __builtin_amdgcn_readlane((int)(ib & 0xffffffffull), src);
+  const int i1 = __builtin_amdgcn_readlane((int)(ib >> 32), src);
+  return mkc(__longlong_as_double(((unsigned long long)(unsigned)r1 << 32) | (unsigned)r0),
+             __longlong_as_double(((unsigned long long)(unsigned)i1 << 32) | (unsigned)i0));
+}
+
+template <bool CONJ>
+__global__ __launch_bounds__(FEW_NT) void k_getrs_few(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv,
+                                                     BatchPtr B, int ldb) {
+  extern __shared__ unsigned char few_smem[];
+  cd* __restrict__ sx = (cd*)few_smem;                               // n x nrhs
+  int* __restrict__ sp = (int*)(few_smem + sizeof(cd) * (size_t)n * nrhs);   // n pivots
+  __shared__ cd T[2][FEW_B][FEW_B + 1];                              // diagonal blocks, double buffered
+  const long long b = blockIdx.x;
+  const cd* a = LU.ptr(b);
+  cd* x = B.ptr(b);
+  const int* piv = ipiv + b * (long long)n;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int e = t; e < n * nrhs; e += FEW_NT) {
+    const int r = e / nrhs, c = e % nrhs;
+    sx[e] = x[(long long)r * ldb + c];
+  }
+  for (int e = t; e < n; e += FEW_NT) sp[e] = piv[e];
+  __syncthreads();
+  auto swaps = [&](bool reverse) {
+    if (t < nrhs) {
+      for (int i = 0; i < n; ++i) {
+        const int j = reverse ? n - 1 - i : i;
+        const int q = sp[j];
+        if (q != j) {
+          const cd u = sx[j * nrhs + t];
+          sx[j * nrhs + t] = sx[q * nrhs + t];
+          sx[q * nrhs + t] = u;
+        }
+      }
+    }
+    __syncthreads();
+  };
+  // factor entries: the lower sweep uses L (!CONJ, unit) or U^H (CONJ, non-unit), the upper
+  // sweep U (!CONJ, non-unit) or L^H (CONJ, unit); (r, c) in the swept triangle's coordinates
+  auto fac = [&](int r, int c) -> cd {
+    return CONJ ? cconj(a[(long long)c * lda + r]) : a[(long long)r * lda + c];
+  };
+  constexpr bool LOWER_UNIT = !CONJ, UPPER_UNIT = CONJ;
+  auto block_len = [&](int blk) { return (n - blk < FEW_B) ? n - blk : FEW_B; };
+  // the block is stored transposed (T[c][r] = entry (r, c)): in the solve, lane r reads entry
+  // (r, j) of column j, consecutive lanes on consecutive LDS addresses
+  auto load_block = [&](int buf, int blk) {
+    const int jb = block_len(blk);
+    for (int e = t; e < jb * jb; e += FEW_NT) {
+      const int r = e / jb, c = e % jb;
+      T[buf][c][r] = fac(blk + r, blk + c);
+    }
+  };
+  // row i -= (factor row i, columns blk..blk+jb-1) x (solved rows blk..blk+jb-1): the block's
+  // factor entries are loaded together (a fixed-length unrolled loop), then accumulated
+  auto update_row = [&](int i, int blk, int jb) {
+    cd m[FEW_B];
+#pragma unroll
+    for (int k = 0; k < FEW_B; ++k) m[k] = (k < jb) ? fac(i, blk + k) : mkc(0.0, 0.0);
+    cd acc[FEW_NR];
+#pragma unroll
+    for (int c = 0; c < FEW_NR; ++c) acc[c] = (c < nrhs) ? sx[i * nrhs + c] : mkc(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < FEW_B; ++k) {
+      if (k < jb) {
+#pragma unroll
+        for (int c = 0; c < FEW_NR; ++c)
+          if (c < nrhs) acc[c] = csub(acc[c], cmul(m[k], sx[(blk + k) * nrhs + c]));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < FEW_NR; ++c)
+      if (c < nrhs) sx[i * nrhs + c] = acc[c];
+  };
+  // wave 0: the block's triangle, one row per lane (lower: rows in increasing order)
+  auto solve_block = [&](int buf, int blk, bool lower, bool unit) {
+    const int jb = block_len(blk);
+    cd xi[FEW_NR];
+#pragma unroll
+    for (int c = 0; c < FEW_NR; ++c) xi[c] = (lane < jb && c < nrhs) ? sx[(blk + lane) * nrhs + c] : mkc(0.0, 0.0);
+    // non-unit diagonal: every lane forms its own row's reciprocal before the serial loop
+    const cd rd = (!unit && lane < jb) ? crecip(T[buf][lane][lane]) : mkc(1.0, 0.0);
+    for (int s = 0; s < jb; ++s) {
+      const int j = lower ? s : jb - 1 - s;
+      if (!unit && lane == j) {
+#pragma unroll
+        for (int c = 0; c < FEW_NR; ++c) xi[c] = cmul(xi[c], rd);
+      }
+      const bool below = lower ? (lane > j && lane < jb) : (lane < j);
+      const cd tij = below ? T[buf][j][lane] : mkc(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < FEW_NR; ++c) {
+        if (c < nrhs) {
+          const cd xj = few_readlane(xi[c], j);
+          if (below) xi[c] = csub(xi[c], cmul(tij, xj));
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < FEW_NR; ++c)
+      if (lane < jb && c < nrhs) sx[(blk + lane) * nrhs + c] = xi[c];
+  };
+  if (!CONJ) swaps(false);
+  // lower sweep: per block, wave 0 solves the triangle, then every thread updates the rows
+  // below it while the next block's triangle is fetched into the other buffer
+  const int nb = (n + FEW_B - 1) / FEW_B;
+  load_block(0, 0);
+  __syncthreads();
+  for (int q = 0; q < nb; ++q) {
+    const int blk = q * FEW_B, jb = block_len(blk), buf = q & 1;
+    if (wv == 0) solve_block(buf, blk, true, LOWER_UNIT);
+    __syncthreads();
+    if (q + 1 < nb) load_block(buf ^ 1, blk + FEW_B);
+    for (int i = blk + jb + t; i < n; i += FEW_NT) update_row(i, blk, jb);
+    __syncthreads();
+  }
+  // upper sweep, blocks from the last
+  load_block(0, (nb - 1) * FEW_B);
+  __syncthreads();
+  for (int q = nb - 1, it = 0; q >= 0; --q, ++it) {
+    const int blk = q * FEW_B, jb = block_len(blk), buf = it & 1;
+    if (wv == 0) solve_block(buf, blk, false, UPPER_UNIT);
+    __syncthreads();
+    if (q > 0) load_block(buf ^ 1, blk - FEW_B);
+    for (int i = t; i < blk; i += FEW_NT) update_row(i, blk, jb);
+    __syncthreads();
+  }
+  if (CONJ) swaps(true);
+  for (int e = t; e < n * nrhs; e += FEW_NT) {
+    const int r = e / nrhs, c = e % nrhs;
+    x[(long long)r * ldb + c] = sx[e];
+  }
+}
+
+// dynamic LDS of k_getrs_few, or 0 when the solve does not fit it
+inline size_t getrs_few_lds(int n, int nrhs) {
+  if (nrhs < 1 || nrhs > FEW_NR) return 0;
+  const size_t bytes = sizeof(cd) * (size_t)n * nrhs + sizeof(int) * (size_t)n;
+  return bytes <= 65536 ? bytes : 0;
+}
+
 }  // namespace
 
 hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int batch,
@@ -364,6 +522,12 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
 hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
                            int ldb, int batch, hipStream_t stream) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  if (const size_t sh = getrs_few_lds(n, nrhs)) {
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_getrs_few<false>, dim3(batch), dim3(FEW_NT), sh, stream, n, nrhs, LU, lda, ipiv,
+                                B, ldb));
+    return hipGetLastError();
+  }
   dim3 g((nrhs + 63) / 64, batch);
   hipLaunchKernelGGL(k_laswp_rhs, g, dim3(64), 0, stream, n, nrhs, B, ldb, ipiv, n, 0);
   WB_CHECK_LAUNCH();
@@ -403,6 +567,12 @@ hipError_t wb_getrs_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipi
 hipError_t wb_getrs_conj_launch(int n, int nrhs, BatchPtr LU, int lda, const int* ipiv, BatchPtr B,
                                 int ldb, int batch, hipStream_t stream) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  if (const size_t sh = getrs_few_lds(n, nrhs)) {
+    WB_TIMED(WB_T_TRSM_DIAG, stream,
+             hipLaunchKernelGGL(k_getrs_few<true>, dim3(batch), dim3(FEW_NT), sh, stream, n, nrhs, LU, lda, ipiv,
+                                B, ldb));
+    return hipGetLastError();
+  }
   dim3 g((nrhs + 63) / 64, batch);
   for (int r0 = 0; r0 < n; r0 += WB_NB) {
     const int jb = (n - r0 < WB_NB) ? n - r0 : WB_NB;
