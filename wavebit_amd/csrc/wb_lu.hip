@@ -671,20 +671,13 @@ __global__ __launch_bounds__(4 * LB) void k_tri_inv(int n, BatchPtr LU, int lda,
   for (int e = t; e < LB * LB; e += 4 * LB) out[e] = X[e / LB][e % LB];
 }
 
-__global__ void k_copy_rows(int rows, int cols, const cd* src, long long s_stride, int lds, BatchPtr dst,
-                            int ldd) {
-  const long long b = blockIdx.y;
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long long)rows * cols) return;
-  const int r = (int)(e / cols), c = (int)(e % cols);
-  dst.ptr(b)[(long long)r * ldd + c] = src[b * s_stride + (long long)r * lds + c];
-}
 
 }  // namespace
 
 size_t wb_getrs_ws_bytes(int n, int nrhs, int batch) {
   const size_t nblk = (size_t)(n + LB - 1) / LB;
-  return sizeof(cd) * (size_t)batch * (nblk * 2 * LB * LB + (size_t)LB * nrhs) + 256;
+  (void)nrhs;
+  return sizeof(cd) * (size_t)batch * nblk * 2 * LB * LB + 256;
 }
 
 // A X = B (conj = 0) or A^H X = B (conj = 1) in place, from wb_getrf_launch's factors of A,
@@ -694,9 +687,7 @@ hipError_t wb_getrs_blocked(int n, int nrhs, BatchPtr LU, int lda, const int* ip
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
   const int nblk = (n + LB - 1) / LB;
   cd* inv = (cd*)ws;
-  cd* tmp = inv + (size_t)batch * nblk * 2 * LB * LB;
   const long long inv_stride = (long long)nblk * 2 * LB * LB;
-  const long long tmp_stride = (long long)LB * nrhs;
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
   hipError_t e;
   dim3 g((nrhs + 63) / 64, batch);
@@ -708,15 +699,14 @@ hipError_t wb_getrs_blocked(int n, int nrhs, BatchPtr LU, int lda, const int* ip
            hipLaunchKernelGGL(k_tri_inv, dim3(nblk, batch, 2), dim3(4 * LB), 0, stream, n, LU, lda, inv, nblk));
   WB_CHECK_LAUNCH();
   auto INV = [&](int bb, int kind) { return bp_strided(inv + ((long long)bb * 2 + kind) * LB * LB, inv_stride); };
+  // B_b := inv(T_bb) B_b in place: every GEMM tile configuration covers all jb <= 64 rows of
+  // an output tile in one workgroup, which reads its whole column slice of B_b (the K loop)
+  // before its epilogue writes that slice, and no other workgroup touches those columns
+  static_assert(LB <= 64, "in-place diagonal-block products need one row tile per column slice");
   auto diag_solve = [&](int bb, int kind, int opa) -> hipError_t {
     const int r0 = bb * LB, jb = (n - r0 < LB) ? n - r0 : LB;
-    hipError_t e2 = wb_zgemm_ops_launch(opa, 0, jb, nrhs, jb, one, INV(bb, kind), LB, offset(B, (long long)r0 * ldb),
-                                        ldb, zero, bp_strided(tmp, tmp_stride), nrhs, batch, stream);
-    if (e2 != hipSuccess) return e2;
-    const long long tot = (long long)jb * nrhs;
-    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((tot + 255) / 256), batch), dim3(256), 0, stream, jb, nrhs,
-                       tmp, tmp_stride, nrhs, offset(B, (long long)r0 * ldb), ldb);
-    return hipGetLastError();
+    return wb_zgemm_ops_launch(opa, 0, jb, nrhs, jb, one, INV(bb, kind), LB, offset(B, (long long)r0 * ldb), ldb,
+                               zero, offset(B, (long long)r0 * ldb), ldb, batch, stream);
   };
   if (!conj) {
     for (int bb = 0; bb < nblk; ++bb) {           // L: forward
