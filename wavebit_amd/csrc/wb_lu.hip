@@ -1,11 +1,13 @@
 // Batched complex FP64 LU with partial pivoting and the triangular solves built on it.
 //
-// Right-looking blocked factorization, one launch sequence per panel of WB_NB columns:
-//   k_getrf_panel   one workgroup per matrix factors the tall panel (pivot search, row swap,
+// Right-looking blocked factorization in steps of 2 WB_NB columns, each factored as two
+// WB_NB-column panels:
+//   k_getrf_panel   one workgroup per matrix factors a tall panel (pivot search, row swap,
 //                   scale, rank-1 update inside the panel);
-//   k_laswp_trsm    applies the panel's row swaps to every column outside it and solves
+//   k_laswp_trsm    applies a panel's row swaps to every column outside it and/or solves
 //                   L11 U12 = A12 for the columns to its right (64 columns per workgroup);
-//   zgemm           A22 -= L21 U12 on the matrix cores.
+//   zgemm           the right half brought up to date with the left half between the two
+//                   panels, then A22 -= L21 U12 with k = 2 WB_NB on the matrix cores.
 // Solves (getrs) reuse the same split: row permutation, then per diagonal block a small
 // triangular kernel followed by a GEMM update of the rows still to be solved. The pivots are
 // chosen by |re| + |im| with the first maximum winning, as LAPACK's izamax does.
@@ -181,18 +183,36 @@ __global__ __launch_bounds__(NTP) void k_getrf_panel_reg(int n, int k0, int jb, 
 
 // Swaps for panel rows k0..k0+jb-1 applied to every column outside the panel; then the unit
 // lower triangular solve for the columns right of the panel. 64 columns per workgroup.
+// mode 0: both; 1: the swaps only; 2: the triangular solve only (the grid then covers just the
+// columns right of the panel).
 __global__ __launch_bounds__(64) void k_laswp_trsm(int n, int k0, int jb, BatchPtr A, int lda,
-                                                   const int* ipiv) {
+                                                   const int* ipiv, int mode) {
   const long long b = blockIdx.y;
   cd* a = A.ptr(b);
   const int* piv = ipiv + b * (long long)n;
   const int t = threadIdx.x;
   __shared__ cd L[WB_NB][WB_NB + 1];
-  const int outside = n - jb;
   const int ci = blockIdx.x * 64 + t;
   int col = -1;
-  if (ci < outside) col = (ci < k0) ? ci : ci + jb;
-  if (col >= 0) {
+  if (mode == 2) {
+    if (ci < n - k0 - jb) col = k0 + jb + ci;
+  } else if (ci < n - jb) {
+    col = (ci < k0) ? ci : ci + jb;
+  }
+  if (mode == 1) {
+    if (col >= 0) {
+      for (int j = k0; j < k0 + jb; ++j) {
+        const int p = piv[j];
+        if (p != j) {
+          cd x = a[(long long)j * lda + col];
+          a[(long long)j * lda + col] = a[(long long)p * lda + col];
+          a[(long long)p * lda + col] = x;
+        }
+      }
+    }
+    return;
+  }
+  if (col >= 0 && mode == 0) {
     for (int j = k0; j < k0 + jb; ++j) {
       const int p = piv[j];
       if (p != j) {
@@ -482,8 +502,12 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
   // a fill kernel rather than a memset node, so the sequence stays safe under graph capture
   hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
   WB_CHECK_LAUNCH();
-  for (int k0 = 0; k0 < n; k0 += WB_NB) {
-    const int jb = (n - k0 < WB_NB) ? n - k0 : WB_NB;
+  // Right-looking, 64 columns per step: the 64-column block is factored as two 32-column
+  // register panels (the right half brought up to date with the left half in between), and
+  // the trailing matrix then gets one k = 64 update, so it is read and written once per 64
+  // columns. The swaps of each half reach every column outside that half, as in LAPACK.
+  const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
+  auto panel = [&](int k0, int jb) -> hipError_t {
     if (n <= 64) {
       WB_TIMED(WB_T_GETRF_PANEL, stream,
                hipLaunchKernelGGL(k_getrf_panel_reg<64>, dim3(batch), dim3(64), 0, stream, n, k0, jb, A, lda,
@@ -497,23 +521,39 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
                hipLaunchKernelGGL(k_getrf_panel, dim3(batch), dim3(256), 0, stream, n, k0, jb, A, lda,
                                   ipiv, info));
     }
-    WB_CHECK_LAUNCH();
-    const int outside = n - jb;
-    if (outside > 0) {
-      WB_TIMED(WB_T_LASWP_TRSM, stream,
-               hipLaunchKernelGGL(k_laswp_trsm, dim3((outside + 63) / 64, batch), dim3(64), 0, stream,
-                                  n, k0, jb, A, lda, ipiv));
-      WB_CHECK_LAUNCH();
+    return hipGetLastError();
+  };
+  auto swap_trsm = [&](int k0, int jb, int mode) -> hipError_t {
+    const int cols = (mode == 2) ? n - k0 - jb : n - jb;
+    if (cols <= 0) return hipSuccess;
+    WB_TIMED(WB_T_LASWP_TRSM, stream,
+             hipLaunchKernelGGL(k_laswp_trsm, dim3((cols + 63) / 64, batch), dim3(64), 0, stream, n, k0, jb, A,
+                                lda, ipiv, mode));
+    return hipGetLastError();
+  };
+  // C[r0.., c0..] (m x nc) -= A[r0.., a0..] (m x kk) * A[a0.., c0..] (kk x nc)
+  auto update = [&](int r0, int c0, int m, int nc, int a0, int kk) -> hipError_t {
+    if (m <= 0 || nc <= 0 || kk <= 0) return hipSuccess;
+    return wb_zgemm_launch(m, nc, kk, mone, offset(A, (long long)r0 * lda + a0), lda,
+                           offset(A, (long long)a0 * lda + c0), lda, one, offset(A, (long long)r0 * lda + c0),
+                           lda, batch, stream);
+  };
+  hipError_t e;
+  for (int k0 = 0; k0 < n; k0 += 2 * WB_NB) {
+    const int jb = (n - k0 < 2 * WB_NB) ? n - k0 : 2 * WB_NB;
+    const int h = (jb < WB_NB) ? jb : WB_NB;
+    const int k1 = k0 + h, jr = jb - h;            // right half: columns k1 .. k1 + jr - 1
+    if ((e = panel(k0, h))) return e;
+    // left half's swaps on every other column; its U rows for every column right of it
+    if ((e = swap_trsm(k0, h, 0))) return e;
+    if (jr > 0) {
+      if ((e = update(k1, k1, n - k1, jr, k0, h))) return e;          // right half <- left half
+      if ((e = panel(k1, jr))) return e;
+      if ((e = swap_trsm(k1, jr, 1))) return e;                       // right half's swaps
+      if ((e = update(k1, k0 + jb, jr, n - k0 - jb, k0, h))) return e;  // its rows of the trailing columns
+      if ((e = swap_trsm(k1, jr, 2))) return e;                       // their U rows
     }
-    const int rest = n - k0 - jb;
-    if (rest > 0) {
-      hipError_t e = wb_zgemm_launch(rest, rest, jb, mkc(-1.0, 0.0),
-                                     offset(A, (long long)(k0 + jb) * lda + k0), lda,
-                                     offset(A, (long long)k0 * lda + k0 + jb), lda, mkc(1.0, 0.0),
-                                     offset(A, (long long)(k0 + jb) * lda + k0 + jb), lda, batch,
-                                     stream);
-      if (e != hipSuccess) return e;
-    }
+    if ((e = update(k0 + jb, k0 + jb, n - k0 - jb, n - k0 - jb, k0, jb))) return e;   // trailing, k = jb
   }
   return hipSuccess;
 }
