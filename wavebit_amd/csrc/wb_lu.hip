@@ -1,13 +1,13 @@
 // Batched complex FP64 LU with partial pivoting and the triangular solves built on it.
 //
-// Right-looking blocked factorization in steps of 2 WB_NB columns, each factored as two
-// WB_NB-column panels:
+// Right-looking blocked factorization in steps of WB_NP panels of WB_NB columns:
 //   k_getrf_panel   one workgroup per matrix factors a tall panel (pivot search, row swap,
 //                   scale, rank-1 update inside the panel);
 //   k_laswp_trsm    applies a panel's row swaps to every column outside it and/or solves
 //                   L11 U12 = A12 for the columns to its right (64 columns per workgroup);
-//   zgemm           the right half brought up to date with the left half between the two
-//                   panels, then A22 -= L21 U12 with k = 2 WB_NB on the matrix cores.
+//   zgemm           the step's later panels and the panel rows of the trailing columns kept
+//                   up to date inside the step, then A22 -= L21 U12 with k = WB_NP WB_NB on
+//                   the matrix cores.
 // Solves (getrs) reuse the same split: row permutation, then per diagonal block a small
 // triangular kernel followed by a GEMM update of the rows still to be solved. The pivots are
 // chosen by |re| + |im| with the first maximum winning, as LAPACK's izamax does.
@@ -15,6 +15,9 @@
 #include "wb_timing.h"
 
 #define WB_NB 32
+#ifndef WB_NP
+#define WB_NP 4    // panels per factorization step (the trailing update has k = WB_NP WB_NB)
+#endif
 
 namespace {
 
@@ -502,10 +505,11 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
   // a fill kernel rather than a memset node, so the sequence stays safe under graph capture
   hipLaunchKernelGGL(k_zero_int, dim3((batch + 255) / 256), dim3(256), 0, stream, info, batch);
   WB_CHECK_LAUNCH();
-  // Right-looking, 64 columns per step: the 64-column block is factored as two 32-column
-  // register panels (the right half brought up to date with the left half in between), and
-  // the trailing matrix then gets one k = 64 update, so it is read and written once per 64
-  // columns. The swaps of each half reach every column outside that half, as in LAPACK.
+  // Right-looking, WB_NP panels of WB_NB columns per step: inside the step each panel is
+  // factored, its swaps reach every column outside it (as in LAPACK), its rows of the trailing
+  // columns are brought up to date with the step's earlier panels before its U rows are
+  // solved, and the step's later panels get its update; the trailing matrix then gets one
+  // k = WB_NP WB_NB update, so it is read and written once per step.
   const cd one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0);
   auto panel = [&](int k0, int jb) -> hipError_t {
     if (n <= 64) {
@@ -539,21 +543,22 @@ hipError_t wb_getrf_launch(int n, BatchPtr A, int lda, int* ipiv, int* info, int
                            lda, batch, stream);
   };
   hipError_t e;
-  for (int k0 = 0; k0 < n; k0 += 2 * WB_NB) {
-    const int jb = (n - k0 < 2 * WB_NB) ? n - k0 : 2 * WB_NB;
-    const int h = (jb < WB_NB) ? jb : WB_NB;
-    const int k1 = k0 + h, jr = jb - h;            // right half: columns k1 .. k1 + jr - 1
-    if ((e = panel(k0, h))) return e;
-    // left half's swaps on every other column; its U rows for every column right of it
-    if ((e = swap_trsm(k0, h, 0))) return e;
-    if (jr > 0) {
-      if ((e = update(k1, k1, n - k1, jr, k0, h))) return e;          // right half <- left half
-      if ((e = panel(k1, jr))) return e;
-      if ((e = swap_trsm(k1, jr, 1))) return e;                       // right half's swaps
-      if ((e = update(k1, k0 + jb, jr, n - k0 - jb, k0, h))) return e;  // its rows of the trailing columns
-      if ((e = swap_trsm(k1, jr, 2))) return e;                       // their U rows
+  constexpr int SB = WB_NP * WB_NB;                 // columns per step
+  for (int k0 = 0; k0 < n; k0 += SB) {
+    const int jb = (n - k0 < SB) ? n - k0 : SB;
+    const int kend = k0 + jb;
+    for (int cq = k0; cq < kend; cq += WB_NB) {
+      const int wq = (kend - cq < WB_NB) ? kend - cq : WB_NB;
+      const int cr = cq + wq;                       // first column right of this panel
+      if ((e = panel(cq, wq))) return e;            // its columns are up to date (below)
+      if ((e = swap_trsm(cq, wq, 1))) return e;     // its swaps, every other column
+      // its rows of the trailing columns: the earlier panels of this step, then its U rows
+      if ((e = update(cq, kend, wq, n - kend, k0, cq - k0))) return e;
+      if ((e = swap_trsm(cq, wq, 2))) return e;
+      // the later panels of this step: every row below this panel (right-looking in the step)
+      if ((e = update(cr, cr, n - cr, kend - cr, cq, wq))) return e;
     }
-    if ((e = update(k0 + jb, k0 + jb, n - k0 - jb, n - k0 - jb, k0, jb))) return e;   // trailing, k = jb
+    if ((e = update(kend, kend, n - kend, n - kend, k0, jb))) return e;   // trailing, k = jb
   }
   return hipSuccess;
 }
