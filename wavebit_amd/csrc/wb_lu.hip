@@ -83,13 +83,15 @@ __global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, Batc
 }
 
 // The same panel factorization with the panel held in registers: one thread per row (n <= PNT),
-// its WB_NB panel entries in registers, so a column step is a wave-level argmax (first maximum
-// wins), one barrier to combine the waves, one to exchange the pivot row through LDS (double
-// buffered), and a rank-1 update of each thread's own row, with no global traffic inside the
-// panel.
+// its WB_NB panel entries in registers. Rows are never moved inside the panel: every thread
+// keeps the logical position of the row it holds (pos), a row swap only exchanges two
+// positions, and the rows are stored to their final positions at the end. A column step is a
+// wave-level argmax over the candidate positions (first maximum wins), one barrier to combine
+// the waves, one to broadcast the pivot row through LDS (double buffered), and a rank-1 update
+// of each thread's own row, with no global traffic inside the panel.
 constexpr int PNT = 768;
 struct PanelCtx {
-  int n, k0, jb, r, lane, wv, t;
+  int n, k0, jb, lane, wv, t;
   bool own;
   int* piv;
   int* info;
@@ -97,16 +99,24 @@ struct PanelCtx {
   double* s_v;
   int* s_i;
   cd (*s_prow)[WB_NB];
-  cd (*s_jrow)[WB_NB];
 };
+// 1 / b with one real division: b scaled by a power of two so |b|^2 neither overflows nor
+// underflows, conj(b') / |b'|^2, scaled back.
+__device__ inline cd crecip_scaled(cd b) {
+  const double m = fmax(fabs(b.x), fabs(b.y));
+  const int e = ilogb(m);
+  const double bx = ldexp(b.x, -e), by = ldexp(b.y, -e);
+  const double inv = 1.0 / (bx * bx + by * by);
+  return mkc(ldexp(bx * inv, -e), ldexp(-by * inv, -e));
+}
 // Column C of the register-resident panel (compile-time C keeps x[] in registers).
 template <int C, int PNW>
-__device__ __forceinline__ void panel_col(cd (&x)[WB_NB], const PanelCtx& P) {
+__device__ __forceinline__ void panel_col(cd (&x)[WB_NB], int& pos, const PanelCtx& P) {
   if (C < P.jb) {
-    const int j = P.k0 + C, r = P.r;
-    const bool cand = P.own && r >= j;
+    const int j = P.k0 + C;
+    const bool cand = P.own && pos >= j;
     double v = cand ? cabs1(x[C]) : -1.0;
-    int vi = cand ? r : P.n;
+    int vi = cand ? pos : P.n;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double v2 = __shfl_xor(v, off);
@@ -124,34 +134,27 @@ __device__ __forceinline__ void panel_col(cd (&x)[WB_NB], const PanelCtx& P) {
       if (v2 > pv || (v2 == pv && i2 < p)) { pv = v2; p = i2; }
     }
     constexpr int buf = C & 1;
-    if (r == p) {
+    const bool is_pivot = P.own && pos == p;
+    if (is_pivot) {
 #pragma unroll
       for (int cc = 0; cc < WB_NB; ++cc) P.s_prow[buf][cc] = x[cc];
     }
-    if (r == j && p != j) {
-#pragma unroll
-      for (int cc = 0; cc < WB_NB; ++cc) P.s_jrow[buf][cc] = x[cc];
-    }
-    __syncthreads();
-    if (r == j) {
-#pragma unroll
-      for (int cc = 0; cc < WB_NB; ++cc) x[cc] = P.s_prow[buf][cc];
-    } else if (r == p) {
-#pragma unroll
-      for (int cc = 0; cc < WB_NB; ++cc) x[cc] = P.s_jrow[buf][cc];
-    }
+    // the swap: the pivot row takes position j, the row at j takes the pivot's position
+    if (P.own && pos == j) pos = p;
+    if (is_pivot) pos = j;
     if (P.t == 0) {
       P.piv[j] = p;
       if (pv == 0.0 && P.info[P.b] == 0) P.info[P.b] = j + 1;
     }
-    if (P.own && r > j) {
-      const cd l = (pv == 0.0) ? x[C] : cmul(x[C], crecip(P.s_prow[buf][C]));
+    __syncthreads();
+    if (P.own && pos > j) {
+      const cd l = (pv == 0.0) ? x[C] : cmul(x[C], crecip_scaled(P.s_prow[buf][C]));
       x[C] = l;
 #pragma unroll
       for (int cc = C + 1; cc < WB_NB; ++cc) x[cc] = csub(x[cc], cmul(l, P.s_prow[buf][cc]));
     }
   }
-  if constexpr (C + 1 < WB_NB) panel_col<C + 1, PNW>(x, P);
+  if constexpr (C + 1 < WB_NB) panel_col<C + 1, PNW>(x, pos, P);
 }
 
 // NTP threads (one per row): 768, or one wave for n <= 64, where the barriers cost nothing.
@@ -162,25 +165,25 @@ __global__ __launch_bounds__(NTP) void k_getrf_panel_reg(int n, int k0, int jb, 
   __shared__ double s_v[PNW];
   __shared__ int s_i[PNW];
   __shared__ cd s_prow[2][WB_NB];
-  __shared__ cd s_jrow[2][WB_NB];
   PanelCtx P;
   P.b = blockIdx.x;
   cd* a = A.ptr(P.b);
   P.n = n; P.k0 = k0; P.jb = jb;
   P.t = threadIdx.x; P.lane = P.t & 63; P.wv = P.t >> 6;
-  P.r = P.t;
-  P.own = P.r >= k0 && P.r < n;
+  const int r = P.t;
+  P.own = r >= k0 && r < n;
   P.piv = ipiv + P.b * (long long)n;
   P.info = info;
-  P.s_v = s_v; P.s_i = s_i; P.s_prow = s_prow; P.s_jrow = s_jrow;
+  P.s_v = s_v; P.s_i = s_i; P.s_prow = s_prow;
   cd x[WB_NB];
 #pragma unroll
-  for (int c = 0; c < WB_NB; ++c) x[c] = (P.own && c < jb) ? a[(long long)P.r * lda + k0 + c] : mkc(0.0, 0.0);
-  panel_col<0, PNW>(x, P);
+  for (int c = 0; c < WB_NB; ++c) x[c] = (P.own && c < jb) ? a[(long long)r * lda + k0 + c] : mkc(0.0, 0.0);
+  int pos = r;
+  panel_col<0, PNW>(x, pos, P);
   if (P.own) {
 #pragma unroll
     for (int c = 0; c < WB_NB; ++c)
-      if (c < jb) a[(long long)P.r * lda + k0 + c] = x[c];
+      if (c < jb) a[(long long)pos * lda + k0 + c] = x[c];
   }
 }
 
