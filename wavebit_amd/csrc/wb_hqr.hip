@@ -75,7 +75,9 @@ struct Rot {
 };
 
 // Q [f; g] = [r; 0] (zlartg): c = |f| / d, s = (f / |f|) conj(g) / d, r = (f / |f|) d with
-// d = sqrt(|f|^2 + |g|^2), computed on f and g scaled by max(|f|_1, |g|_1).
+// d = sqrt(|f|^2 + |g|^2). f and g are scaled by a power of two (exactly) so the larger
+// component lies in [1, 2); then with q = 1 / (|f| d): c = |f|^2 q, s = f conj(g) q and
+// r = f d^2 q, one division and one square root on the rotation's serial path.
 __device__ inline Rot givens(cd f, cd g, cd* r) {
   Rot q;
   if (g.x == 0.0 && g.y == 0.0) {
@@ -89,16 +91,15 @@ __device__ inline Rot givens(cd f, cd g, cd* r) {
     *r = mkc(gn, 0.0);
     return q;
   }
-  const double scale = fmax(cabs1(f), cabs1(g));
-  const cd fs = cscale(f, 1.0 / scale), gs = cscale(g, 1.0 / scale);
-  const double f2 = cnorm2(fs), g2 = cnorm2(gs);
-  const double fa = sqrt(f2), d = sqrt(f2 + g2);
-  const cd ph = cscale(fs, 1.0 / fa);          // f / |f|
-  q.c = fa / d;
-  const double id = 1.0 / d;
-  q.sx = (ph.x * gs.x + ph.y * gs.y) * id;     // ph conj(gs) / d
-  q.sy = (ph.y * gs.x - ph.x * gs.y) * id;
-  *r = cscale(ph, d * scale);
+  const int e = ilogb(fmax(fmax(fabs(f.x), fabs(f.y)), fmax(fabs(g.x), fabs(g.y))));
+  const cd fs = mkc(ldexp(f.x, -e), ldexp(f.y, -e)), gs = mkc(ldexp(g.x, -e), ldexp(g.y, -e));
+  const double f2 = cnorm2(fs), d2 = f2 + cnorm2(gs);
+  const double iq = 1.0 / sqrt(f2 * d2);        // 1 / (|f| d)
+  q.c = f2 * iq;
+  q.sx = (fs.x * gs.x + fs.y * gs.y) * iq;      // f conj(g) q
+  q.sy = (fs.y * gs.x - fs.x * gs.y) * iq;
+  const double t = d2 * iq;                      // d / |f|
+  *r = mkc(ldexp(fs.x * t, e), ldexp(fs.y * t, e));
   return q;
 }
 
