@@ -17,7 +17,8 @@ raster pixel and thickness (forward with checkpoint, then the hand-written rever
 
 The serving leg drives the continuous-batching server with a Poisson stream of requests at
 the same orders (1-4 layers, 1-8 wavelengths each, arriving while earlier ones are on the
-device) and reports its p50/p99 request latency and served solves/s.
+device; 30% also ask for the raster gradient) and reports its p50/p99 request latency and
+served solves/s.
 
 Also reported: the dominant kernel's roofline (algorithmic flops per launch / average launch
 time, from HIP events recorded on the launching stream during the timed steps) and the NumPy
@@ -73,6 +74,8 @@ def main():
                     help='points of the gradient leg (0 skips it)')
     ap.add_argument('--serve-requests', type=int, default=48)
     ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
+    ap.add_argument('--serve-grad-fraction', type=float, default=0.3,
+                    help='fraction of served requests that also ask for the raster gradient')
     ap.add_argument('--autotune', type=int, default=1,
                     help='tune the GEMM tile shapes of the benchmark steps before timing (0: defaults)')
     ap.add_argument('--small-runs', type=int, default=50,
@@ -312,12 +315,13 @@ def main():
         from wavebit_amd.rcwa import serve
         specs = workloads.serving_requests(args.serve_requests, seed=11, m_range=(M, M),
                                            l_range=(1, 4), sizes=(64, 128), wl_range=(1, 8),
-                                           grad_fraction=0.0)
+                                           grad_fraction=args.serve_grad_fraction)
         arrivals = serve.poisson_arrivals(len(specs), rate=args.serve_rate, seed=12)
         reqs = []
         for i, (s, t) in enumerate(zip(specs, arrivals)):
             prob, num = workloads.to_problem(s)
-            reqs.append(serve.Request(problem=prob, numerics=num, arrival=float(t), id=i))
+            reqs.append(serve.Request(problem=prob, numerics=num, arrival=float(t), id=i,
+                                      gradient=bool(s['gradient'])))
         srv = serve.Server(device=dev, streams=3, max_points_per_launch=64)
         results, rep = srv.serve(reqs, realtime=True)
         failed = sum(1 for r in results.values() if isinstance(r, Exception))
@@ -326,10 +330,12 @@ def main():
                         'p50_latency_s': round(rep['p50_latency_s'], 4),
                         'p99_latency_s': round(rep['p99_latency_s'], 4),
                         'launches': rep['launches'],
+                        'gradient_requests': rep['gradient_requests'],
                         'mean_points_per_launch': round(rep['mean_points_per_launch'], 2),
                         'failed_requests': failed,
                         'stream': f'orders ({M},{M}), 1-4 layers of 64x64/128x128 rasters eps '
-                                  'U[1,16]+iU[0,1], 1-8 wavelengths in [400,1000] nm, Poisson arrivals'}
+                                  'U[1,16]+iU[0,1], 1-8 wavelengths in [400,1000] nm, Poisson arrivals, '
+                                  f'{args.serve_grad_fraction:.0%} also asking for the raster gradient'}
 
     # the launch-bound end: the 1D binary grating at orders (10,0) through Prepared.run, eager
     # launch sequence vs the captured HIP graph it replays from its second run on

@@ -44,3 +44,46 @@ def test_a_bad_request_fails_alone():
     results, _ = srv.serve([serve.Request(p0, n0, 0.0, 0), serve.Request(p1, n1, 0.0, 1)], realtime=False)
     assert not isinstance(results[0], Exception)
     assert isinstance(results[1], Exception)
+
+
+def test_gradient_requests_match_the_differentiable_run_alone():
+    """Requests that also ask for d(sum of T)/d(rasters, thicknesses) are served through the
+    reverse pass; result and gradient equal a differentiable ``rcwa.run`` of the same request
+    alone, while forward-only requests share launches around them."""
+    import torch
+    import wavebit_amd
+    specs = workloads.serving_requests(6, seed=9, m_range=(2, 3), l_range=(1, 2), sizes=(16,),
+                                       wl_range=(1, 3), grad_fraction=0.0)
+    for i in (1, 4):
+        specs[i]['gradient'] = True
+    reqs = []
+    for i, s in enumerate(specs):
+        problem, numerics = workloads.to_problem(s)
+        reqs.append(serve.Request(problem=problem, numerics=numerics, arrival=0.0, id=i,
+                                  gradient=s['gradient']))
+    srv = serve.Server(streams=3, max_points_per_launch=8)
+    results, report = srv.serve(reqs, realtime=False)
+    assert report['gradient_requests'] == 2
+    for i, s in enumerate(specs):
+        res = results[i]
+        assert not isinstance(res, Exception), res
+        problem, numerics = workloads.to_problem(s)
+        alone = rcwa.run(problem, numerics=numerics, backend='torch', device='cuda:0')
+        assert np.abs(res.R - alone.R).max() < 1e-10
+        assert np.abs(res.T - alone.T).max() < 1e-10
+        if not s['gradient']:
+            assert 'gradient' not in res.diagnostics
+            continue
+        # the same gradient through the public differentiable lifecycle
+        eps = torch.tensor(s['rasters'], dtype=torch.complex128, requires_grad=True)
+        thick = torch.tensor(s['thickness'], dtype=torch.float64, requires_grad=True)
+        layers = [rcwa.Layer(thickness=thick[l], epsilon=eps[l]) for l in range(eps.shape[0])]
+        live = rcwa.Problem(stack=rcwa.Stack(layers=layers, top=problem.stack.top, bottom=problem.stack.bottom,
+                                             period=problem.stack.period),
+                            excitation=problem.excitation, units=problem.units)
+        out = rcwa.run(live, numerics=numerics, backend='torch', device='cuda:0')
+        out.T.sum().backward()
+        g = res.diagnostics['gradient']
+        ge, gt = eps.grad.numpy(), thick.grad.numpy()
+        assert np.abs(g['d_epsilon'] - ge).max() <= 1e-9 * max(1e-12, np.abs(ge).max())
+        assert np.abs(g['d_thickness'] - gt).max() <= 1e-9 * max(1e-12, np.abs(gt).max())

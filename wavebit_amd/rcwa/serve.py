@@ -11,6 +11,13 @@ up. Convolution matrices of each (raster stack, window) stay on the device in an
 and are reused by every later request of the same design. A request completes when its
 last point has been solved; its latency is completion time minus arrival time.
 
+A request may also ask for the gradient of its total transmitted efficiency (the sum of T over
+its wavelengths and orders) with respect to every raster pixel and layer thickness. Such a
+request runs as a launch of its own through the hand-written reverse pass (forward with
+per-layer checkpoints, then the adjoint), on a server stream and under the same HBM
+admission, while the forward-only launches already on the device keep running; its result
+carries ``diagnostics['gradient']``.
+
 The reference has no serving loop (one solver per wavelength, ``_lifecycle.py:712-727``);
 this is the batching layer around the same per-point arithmetic, so a request's result is the
 same alone and in a mix.
@@ -29,7 +36,7 @@ from .. import _native
 from . import _engine
 from ._lifecycle import Execution, _resolve, _runtime_checks
 from .results import pack
-from ..errors import NumericalError, ValidationError
+from ..errors import NumericalError, UnsupportedFeatureError, ValidationError
 
 
 @dataclass
@@ -38,6 +45,7 @@ class Request:
     numerics: object
     arrival: float = 0.0              # seconds on the server's clock
     id: int = -1
+    gradient: bool = False            # also d(sum of T)/d(rasters, thicknesses)
 
 
 @dataclass
@@ -127,6 +135,51 @@ class Server:
         out = P * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8 + L * 4)
         return _native.solve_workspace_bytes(P, L, hu, hv, _engine._MODE[path]) + out
 
+    def _gradient_bytes(self, key, P):
+        hu, hv, path, L = key
+        mode = _engine._MODE[path]
+        return (2 * _native.solve_workspace_bytes(P, L, hu, hv, mode)
+                + _native.checkpoint_bytes(P, L, hu, hv, mode))
+
+    def _gradient(self, work, stream):
+        """Forward with checkpoints and the reverse pass for one request, on ``stream``: fills
+        the request's per-wavelength results and returns d(sum of T)/d(eps), d/d(thickness)."""
+        from . import adjoint
+        r = work.resolved
+        if r.uniform or r.path != _engine.PATH_2D:
+            raise UnsupportedFeatureError(
+                'Gradients are implemented for raster stacks on the 2D coupled path; this request '
+                f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
+        points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
+                                polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
+                                period=r.period, thickness=tuple(float(t) for t in r.thickness))
+                  for wl in r.wavelengths]
+        with torch.cuda.stream(stream):
+            eps = torch.tensor(np.ascontiguousarray(r.rasters), dtype=torch.complex128, device=self.device,
+                               requires_grad=True)
+            thick = torch.tensor(np.asarray(r.thickness, dtype=np.float64), device=self.device,
+                                 requires_grad=True)
+            eff, amp, pe, flags = adjoint.solve(eps, thick, points, r.hu, r.hv, self.device)
+            (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
+            d_eps, d_t = eps.grad.detach().cpu().numpy(), thick.grad.detach().cpu().numpy()
+            eff_h, amp_h, pe_h, fl = (t.detach().cpu().numpy() for t in (eff, amp, pe, flags))
+        for i in range(work.n_points):
+            if np.any(fl[i] != 0):
+                work.failed = True
+            work.results[i] = _engine.unpack(eff_h, amp_h, pe_h, i, r.hu, r.hv, r.path)
+        work.done = work.n_points
+        return d_eps, d_t
+
+    def _gradient_result(self, work, grads):
+        from dataclasses import replace
+        result = self._result(work)
+        if isinstance(result, Exception):
+            return result
+        diag = dict(result.diagnostics)
+        diag['gradient'] = {'objective': 'sum of T over the wavelengths and orders',
+                            'd_epsilon': grads[0], 'd_thickness': grads[1]}
+        return replace(result, diagnostics=diag)
+
     def _launch(self, key, items, stream):
         hu, hv, path, L = key
         convs, index = [], {}
@@ -181,6 +234,7 @@ class Server:
             if q.id < 0:
                 q.id = i
         pending = collections.deque(reqs)
+        grads = collections.deque()                           # admitted gradient requests
         queues = collections.defaultdict(collections.deque)   # bucket key -> deque of items
         order = collections.OrderedDict()                     # bucket key -> oldest arrival
         inflight = []
@@ -189,7 +243,7 @@ class Server:
         free_streams = list(self.streams)
         t0 = time.perf_counter()
         solved_points = 0
-        while pending or any(queues.values()) or inflight:
+        while pending or grads or any(queues.values()) or inflight:
             now = time.perf_counter() - t0
             # admit arrived requests
             while pending and (not realtime or pending[0].arrival <= now):
@@ -199,6 +253,9 @@ class Server:
                 except Exception as error:           # a bad request fails alone
                     results[q.id] = error
                     latencies.append(now - q.arrival)
+                    continue
+                if q.gradient:
+                    grads.append((work, key))
                     continue
                 queues[key].extend(items)
                 order.setdefault(key, q.arrival)
@@ -217,8 +274,30 @@ class Server:
             for work in finished:
                 results[work.req.id] = self._result(work)
                 latencies.append(work.t_done - work.req.arrival)
-            # start launches while streams and memory allow: oldest bucket first
+            # a gradient request runs as a launch of its own when a stream and its memory are free
+            # (the host waits for it; the forward launches already in flight keep running)
             started = False
+            if grads and free_streams:
+                work, key = grads[0]
+                need = self._gradient_bytes(key, work.n_points)
+                if used + need <= self.budget or not inflight:
+                    grads.popleft()
+                    stream = free_streams.pop()
+                    try:
+                        g = self._gradient(work, stream)
+                        work.t_done = time.perf_counter() - t0
+                        res = self._gradient_result(work, g)
+                    except Exception as error:       # fails alone
+                        work.t_done = time.perf_counter() - t0
+                        res = error
+                    free_streams.append(stream)
+                    results[work.req.id] = res
+                    latencies.append(work.t_done - work.req.arrival)
+                    if not isinstance(res, Exception):
+                        solved_points += work.n_points
+                    self.stats['gradient_requests'] += 1
+                    started = True
+            # start launches while streams and memory allow: oldest bucket first
             for key in sorted([k for k in queues if queues[k]], key=lambda k: order.get(k, 0.0)):
                 if not free_streams:
                     break
@@ -251,6 +330,7 @@ class Server:
                   'p99_latency_s': float(np.percentile(lat, 99)),
                   'launches': self.stats['launches'],
                   'mean_points_per_launch': self.stats['launched_points'] / max(1, self.stats['launches']),
+                  'gradient_requests': self.stats['gradient_requests'],
                   'conv_cache_hits': self.conv.hits, 'conv_cache_misses': self.conv.misses,
                   'hbm_budget_bytes': self.budget}
         return results, report
