@@ -20,6 +20,11 @@ the same orders (1-4 layers, 1-8 wavelengths each, arriving while earlier ones a
 device; 30% also ask for the raster gradient) and reports its p50/p99 request latency and
 served solves/s.
 
+Two more workloads run as legs of their own: the crossed grating (3 layers of 256x256 binary
+rasters at orders (7,7), 512 points over wavelength, angle and polarization in one launch
+sequence, with every point's energy balance) and, on request (--uneven-problems), a batch of
+problems of different order windows and layer counts served at once, size-bucketed.
+
 Also reported: the dominant kernel's roofline (algorithmic flops per launch / average launch
 time, from HIP events recorded on the launching stream during the timed steps) and the NumPy
 FP64 oracle timed on the host cores on a bounded sample of the same workload.
@@ -74,6 +79,10 @@ def main():
                     help='points of the gradient leg (0 skips it)')
     ap.add_argument('--serve-requests', type=int, default=48)
     ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
+    ap.add_argument('--crossed-points', type=int, default=512,
+                    help='points of the crossed-grating leg (orders (7,7), 3 layers of 256x256; 0 skips)')
+    ap.add_argument('--uneven-problems', type=int, default=0,
+                    help='problems of the uneven-batch leg (first of the seeded 2048; 0 skips)')
     ap.add_argument('--serve-grad-fraction', type=float, default=0.3,
                     help='fraction of served requests that also ask for the raster gradient')
     ap.add_argument('--autotune', type=int, default=1,
@@ -395,6 +404,60 @@ def main():
                  'solve_graph_ms': round(dev_t['graph'] * 1e3, 4),
                  'R_sum': float(res1.R.sum()), 'T_sum': float(res1.T.sum())}
 
+    # the crossed grating: 3 layers of 256x256 binary rasters, orders (7,7) = 225 harmonics,
+    # points over wavelength, angle and polarization in one batched launch sequence (assembly
+    # included); lossless, so every point's R + T - 1 is reported
+    crossed = None
+    if args.crossed_points > 0:
+        w2 = workloads.crossed_grating_2d(points=args.crossed_points)
+        h2 = w2['orders']
+        r2 = torch.from_numpy(w2['rasters']).to(dev)
+
+        def crossed_step(points):
+            b2 = eng.Bucket(hu=h2[0], hv=h2[1], path=eng.PATH_2D, n_layers=r2.shape[0])
+            for p in points:
+                b2.points.append(eng.Point(stack=0, wavelength=p['wavelength'], theta=p['theta'],
+                                           phi=p['phi'], polarization=p['polarization'],
+                                           eps_top=w2['eps_top'], eps_bottom=w2['eps_bottom'],
+                                           period=w2['period'], thickness=tuple(w2['thickness'])))
+            conv2 = [_native.assemble_li(r2, h2[0], h2[1])[:3]]
+            return eng.run_bucket(b2, conv2, dev)
+        crossed_step(w2['points'][:8])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o2 = crossed_step(w2['points'])
+        torch.cuda.synchronize()
+        dt2 = time.perf_counter() - t0
+        e2, a2, p2, f2 = (t.cpu().numpy() for t in o2[:4])
+        err2 = max(abs(float(u['R'].sum() + u['T'].sum()) - 1.0)
+                   for u in (eng.unpack(e2, a2, p2, i, h2[0], h2[1], eng.PATH_2D)['incident']
+                             for i in range(len(w2['points']))))
+        crossed = {'workload': '3 layers of 256x256 binary {1, 12.25} rasters, period 1000 nm, orders '
+                               '(7,7) = 225 harmonics (n=450), wavelength U[400,800] nm, theta U[0,30] '
+                               'deg, phi U[0,360) deg, s/p, one batched launch sequence',
+                   'points': len(w2['points']), 's': round(dt2, 4),
+                   'solves_per_s': round(len(w2['points']) / dt2, 3),
+                   'max_abs_R_plus_T_minus_1': err2, 'flagged_points': int((f2 != 0).sum())}
+
+    # the uneven batch: problems of different order windows and layer counts served at once,
+    # size-bucketed by (orders, layers), one launch sequence per bucket on three streams
+    uneven = None
+    if args.uneven_problems > 0:
+        from wavebit_amd.rcwa import serve
+        specs3 = workloads.uneven_problems()[:args.uneven_problems]
+        reqs3 = []
+        for i, s3 in enumerate(specs3):
+            prob3, num3 = workloads.to_problem(s3)
+            reqs3.append(serve.Request(problem=prob3, numerics=num3, arrival=0.0, id=i))
+        srv3 = serve.Server(device=dev, streams=3, max_points_per_launch=64)
+        res3, rep3 = srv3.serve(reqs3, realtime=False)
+        uneven = {'workload': 'orders per axis U{3..12}, 1-8 layers of 128x128 two-material rasters '
+                              '(eps\' U[1,16], eps\'\' U[0,2]), substrate U[1.5,4], one point each, '
+                              'the first problems of the seeded 2048-problem set',
+                  'problems': len(specs3), 'points': rep3['points'], 's': round(rep3['wall_s'], 3),
+                  'solves_per_s': round(rep3['solves_per_s'], 3), 'launches': rep3['launches'],
+                  'failed': sum(1 for v in res3.values() if isinstance(v, Exception))}
+
     fl_pt = workloads.flops_per_point((hu, hv), L)
     line = {
         'metric': 'RCWA solves/s @ (9,9) orders FP64 vs plain launch; p50/p99 serve latency',
@@ -414,6 +477,8 @@ def main():
         'gradient': grad,
         'serve': serve_report,
         'launch_bound': small,
+        'crossed_grating': crossed,
+        'uneven_batch': uneven,
         'gemm_autotune': tuning,
         'p50_latency_s': serve_report['p50_latency_s'] if serve_report else None,
         'p99_latency_s': serve_report['p99_latency_s'] if serve_report else None,
