@@ -15,14 +15,15 @@ A request may also ask for the gradient of its total transmitted efficiency (the
 its wavelengths and orders) with respect to every raster pixel and layer thickness. Such a
 request runs as a launch of its own through the hand-written reverse pass (forward with
 per-layer checkpoints, then the adjoint), on a server stream and under the same HBM
-admission, while the forward-only launches already on the device keep running; its result
-carries ``diagnostics['gradient']``.
+admission, driven by a worker thread so the loop keeps admitting and launching forward work
+meanwhile; its result carries ``diagnostics['gradient']``.
 
 The reference has no serving loop (one solver per wavelength, ``_lifecycle.py:712-727``);
 this is the batching layer around the same per-point arithmetic, so a request's result is the
 same alone and in a mix.
 """
 import collections
+import concurrent.futures
 import hashlib
 import heapq
 import itertools
@@ -108,6 +109,7 @@ class Server:
         self.conv = ConvLRU(self.device, int(free * conv_cache_fraction))
         self.execution = Execution(backend='torch', device=str(self.device))
         self.stats = collections.Counter()
+        self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
     # ------------------------------------------------------------------ request intake
     def _admit(self, req, now):
@@ -243,7 +245,8 @@ class Server:
         free_streams = list(self.streams)
         t0 = time.perf_counter()
         solved_points = 0
-        while pending or grads or any(queues.values()) or inflight:
+        grad_job = None
+        while pending or grads or grad_job is not None or any(queues.values()) or inflight:
             now = time.perf_counter() - t0
             # admit arrived requests
             while pending and (not realtime or pending[0].arrival <= now):
@@ -274,28 +277,37 @@ class Server:
             for work in finished:
                 results[work.req.id] = self._result(work)
                 latencies.append(work.t_done - work.req.arrival)
-            # a gradient request runs as a launch of its own when a stream and its memory are free
-            # (the host waits for it; the forward launches already in flight keep running)
+            # gradient requests: retire a finished one, start the next when a stream and its
+            # memory are free (one at a time, on the worker thread)
             started = False
-            if grads and free_streams:
+            if grad_job is not None and grad_job[0].done():
+                fut, work, stream, need = grad_job
+                try:
+                    res = self._gradient_result(work, fut.result())
+                except Exception as error:           # fails alone
+                    res = error
+                work.t_done = getattr(work, 't_grad_done', time.perf_counter() - t0)
+                results[work.req.id] = res
+                latencies.append(work.t_done - work.req.arrival)
+                if not isinstance(res, Exception):
+                    solved_points += work.n_points
+                free_streams.append(stream)
+                used -= need
+                grad_job = None
+                self.stats['gradient_requests'] += 1
+            if grads and grad_job is None and free_streams:
                 work, key = grads[0]
                 need = self._gradient_bytes(key, work.n_points)
                 if used + need <= self.budget or not inflight:
                     grads.popleft()
                     stream = free_streams.pop()
-                    try:
-                        g = self._gradient(work, stream)
-                        work.t_done = time.perf_counter() - t0
-                        res = self._gradient_result(work, g)
-                    except Exception as error:       # fails alone
-                        work.t_done = time.perf_counter() - t0
-                        res = error
-                    free_streams.append(stream)
-                    results[work.req.id] = res
-                    latencies.append(work.t_done - work.req.arrival)
-                    if not isinstance(res, Exception):
-                        solved_points += work.n_points
-                    self.stats['gradient_requests'] += 1
+                    used += need
+
+                    def job(work=work, stream=stream):
+                        out = self._gradient(work, stream)
+                        work.t_grad_done = time.perf_counter() - t0
+                        return out
+                    grad_job = (self._pool.submit(job), work, stream, need)
                     started = True
             # start launches while streams and memory allow: oldest bucket first
             for key in sorted([k for k in queues if queues[k]], key=lambda k: order.get(k, 0.0)):
@@ -318,7 +330,7 @@ class Server:
                 self.stats['launched_points'] += P
                 started = True
             if not started and not finished:
-                if inflight:
+                if inflight or grad_job is not None:
                     time.sleep(0.0005)
                 elif pending and realtime:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
