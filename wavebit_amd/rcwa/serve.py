@@ -111,6 +111,18 @@ class Server:
         self.stats = collections.Counter()
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
 
+    def close(self):
+        """Stop the gradient worker thread (a later ``serve`` call starts a new one)."""
+        self._pool.shutdown(wait=True)
+        self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self._pool.shutdown(wait=True)
+        return False
+
     # ------------------------------------------------------------------ request intake
     def _admit(self, req, now):
         r = _resolve(req.problem, req.numerics, self.execution)
