@@ -15,6 +15,10 @@
  * seams its raster RCWA path calls (paths relative to the reference's src/wavebit/):
  *   wb_assemble_li   rcwa/_engine/factorization.py:915-940  to_conv_mat_raster_sampled
  *                    (li_conv_matrices, :658-671), one call for a whole layer stack;
+ *   wb_assemble_laurent
+ *                    rcwa/_engine/factorization.py:1016-1034 to_conv_mat_raster_laurent
+ *                    (laurent_conv_matrices, :955-1000; _laurent_conv, :944-952), the
+ *                    Numerics(factorization='laurent') opt-in, one call for a layer stack;
  *   wb_solve         rcwa/_engine/scattering.py:1386-1456 smm_2d and :704-902 smm_1d as the
  *                    engine calls them (rcwa/_engine/engine.py:3283-3316, 3336-3434), for a
  *                    batch of plane-wave points instead of one wavelength per call
@@ -24,7 +28,7 @@
  *                    the dense inv / solve / matmul calls of rcwa/_engine/scattering.py:116-200
  *                    (_ab_matrices, _s_layer, redheffer_star) and _common/backend.py:506-542
  *                    (linear_solve).
- *   wb_solve_keep / wb_solve_adjoint / wb_assemble_li_adjoint
+ *   wb_solve_keep / wb_solve_adjoint / wb_assemble_li_adjoint / wb_assemble_laurent_adjoint
  *                    the reverse pass PyTorch autograd takes through the same path
  *                    (loss.backward() after Prepared.run, rcwa/_lifecycle.py:785-786), with
  *                    the eigensolve's generator rule of rcwa/_engine/modal_gradient.py:161-251
@@ -114,6 +118,14 @@ int wb_assemble_li(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu
                    double* epx, double* epy, double* epz_i, void* workspace,
                    size_t workspace_bytes, int* info, void* stream);
 
+/* Laurent-rule convolution matrices (sampled coefficients) of L raster layers eps (L, Ny, Nx):
+ * epx = epy = the direct rule [[eps]] (equal to the bit), epz_i = [[eps]]^-1 (the same matrix
+ * wb_assemble_li returns in that slot). hu, hv <= 15. info: L ints (inversion per layer). */
+size_t wb_assemble_laurent_workspace_size(int L, int Ny, int Nx, int hu, int hv);
+int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                        double* epx, double* epy, double* epz_i, void* workspace,
+                        size_t workspace_bytes, int* info, void* stream);
+
 /* Batch solve of P plane-wave points, each through L layers (bottom to top), top incidence.
  *   pts       P x WB_PT_STRIDE doubles (layout above)
  *   thick     P x L layer thicknesses
@@ -179,6 +191,14 @@ int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx
                            const double* epx, const double* epz_i, const double* d_epx,
                            const double* d_epy, const double* d_epz, double* d_eps,
                            void* workspace, size_t workspace_bytes, void* stream);
+
+/* Cotangent of the raster eps (L, Ny, Nx) from those of wb_assemble_laurent's outputs; epz_i is
+ * that call's output for the same eps. hu, hv <= 15. */
+size_t wb_assemble_laurent_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv);
+int wb_assemble_laurent_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                                const double* epz_i, const double* d_epx, const double* d_epy,
+                                const double* d_epz, double* d_eps, void* workspace,
+                                size_t workspace_bytes, void* stream);
 
 /* Per-kernel and per-stage timing (no reference counterpart: measurement support).
  * wb_timing_enable(1) brackets the launches of the kernels below by hipEvent records on their

@@ -17,6 +17,10 @@ module-level ``PINNED`` table and checked by ``tests/test_oracle.py`` against:
   -- 1e-9;
 * the reference's legacy per-order goldens (``tests/fixtures/legacy_rcwa/golden_legacy.npz``) --
   efficiencies, amplitudes and ``epz_i`` at 1e-12;
+* the reference's fmmax ``Formulation.FFT`` goldens for Laurent's rule
+  (``tests/references/golden_fmmax_laurent.npz``, the two rectangular cells), extrapolated over
+  the sampling as ``tests/references/test_fmmax_laurent_golden.py:83-89`` does -- per order,
+  within the extrapolation's own bound;
 * closed forms: exact pixel integrals of a staircase, uniform layer = eps*I, Fresnel/Airy for a
   uniform slab, R+T=1 for lossless stacks, mode residuals.
 """
@@ -33,6 +37,8 @@ _HALF_PI_F32_BELOW = float(np.nextafter(np.float32(_HALF_PI), np.float32(0)))
 PINNED = {
     'raster_series': 'closed-form pixel integrals (tests/test_oracle.py::test_series_is_exact_pixel_integral)',
     'li_conv_matrices': 'legacy golden epz_i 1e-12; uniform -> eps*I; y-invariant -> 1D Li rule',
+    'laurent_conv_matrices': 'fmmax FFT goldens per order (2 cells); inv(legacy golden epz_i) 1e-12; '
+                             'uniform -> eps*I; y-invariant TE equals Li',
     'boundary_kz': 'Fresnel slab closed form; RETICOLO goldens',
     'mode_q': 'uniform-layer closed form q**2 = kx**2+ky**2-eps; mode residuals',
     'order_azimuth': 'legacy golden amplitudes (signed) 1e-12',
@@ -108,6 +114,17 @@ def li_conv_matrices(eps, hu, hv):
     epy = nested_rule_v(1.0 / eps, hu, hv, invert_outer=False)
     epz_i = np.linalg.inv(conv_matrix(eps, hu, hv))
     return epx, epy, epz_i
+
+
+def laurent_conv_matrices(eps, hu, hv):
+    """``(epx, epy, epz_i)`` for one raster layer under Laurent's rule, sampled coefficients
+    (``factorization.py:944-1000``): the direct rule ``[[eps]]`` -- one matrix -- in both in-plane
+    slots and its inverse in the z slot, which is Li's z slot too."""
+    direct = conv_matrix(np.asarray(eps, dtype=np.complex128), hu, hv)
+    return direct, direct, np.linalg.inv(direct)
+
+
+CONV_RULES = {'li': li_conv_matrices, 'laurent': laurent_conv_matrices}
 
 
 # --------------------------------------------------------------------------- conventions
@@ -294,7 +311,7 @@ def _centre_parity(hu, hv):
 
 
 def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, theta, hu,
-           phi=None):
+           phi=None, factorization='li'):
     """Pure 1D TE/TM solve of a one-row raster stack, top incidence.
 
     Restates ``scattering.py:704-902`` as the engine calls it (``engine.py:3283-3316``): kernel
@@ -322,7 +339,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     S_trn = s_transmission(I, V_kb, I, V_gap)
     S = S_ref
     for li in range(len(thickness)):
-        epx, epy, epz_i = li_conv_matrices(eps_layers[li], hu, 0)
+        epx, epy, epz_i = CONV_RULES[factorization](eps_layers[li], hu, 0)
         W, V, q = layer_modes_1d(tetm, kx, epx, epy, epz_i)
         X = np.exp(-k0 * thickness[li] * q)
         A, B = ab_matrices(W, V, I, V_gap)
@@ -458,14 +475,15 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
 
 
 def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, theta, phi,
-                 polarization, orders):
+                 polarization, orders, factorization='li'):
     """One plane-wave point through a raster stack, routed the way the reference routes it.
 
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
     is ``'s'``/``'p'`` or a Jones pair. Routing (``engine.py:3092-3181``): a one-row raster with
     a polarization word and ``phi is None`` takes the pure 1D path; everything else 2D (the 1D
-    conical layout is out of scope).
+    conical layout is out of scope). ``factorization``: ``'li'`` (the default) or ``'laurent'``.
     """
+    rule = CONV_RULES[factorization]
     eps_layers = np.asarray(eps_layers, dtype=np.complex128)
     hu, hv = orders
     one_row = eps_layers.shape[1] == 1
@@ -473,7 +491,7 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         if one_row and phi is None and hv == 0:
             tetm = 0 if polarization == 's' else 1
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
-                         theta, hu, phi)
+                         theta, hu, phi, factorization)
             inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
                    if k not in ('kx', 'power_entering')}
             inc['power_entering'] = out['power_entering']
@@ -484,7 +502,7 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         pol = pol / np.sqrt(np.sum(np.abs(pol) ** 2))
     if one_row and hv != 0:
         raise ValueError('a one-row raster has no y orders')
-    conv = [li_conv_matrices(e, hu, hv) for e in eps_layers]
+    conv = [rule(e, hu, hv) for e in eps_layers]
     out = smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv)
     out['path'] = '2d'
     return out

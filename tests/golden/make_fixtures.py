@@ -12,6 +12,10 @@ Sources (paths relative to the reference root):
 * ``tests/fixtures/legacy_rcwa/golden_legacy.npz`` -- legacy per-order efficiencies, boundary
   amplitudes, ``epz_i`` and ``q`` per layer; inputs at
   ``tests/rcwa/test_golden_legacy_parity.py:250-264`` (layers listed top-to-bottom there).
+* ``tests/references/golden_fmmax_laurent.npz`` -- fmmax 1.7.1 ``Formulation.FFT`` (Laurent's
+  rule) totals and per-order efficiencies at four samplings per pixel, double-precision-phase
+  flavour, for the two rectangular cells (the hexagonal one needs an oblique lattice); the
+  scene table is ``tests/references/_fmmax_laurent_cases.py:44-99``.
 """
 import json
 import sys
@@ -27,6 +31,8 @@ RETICOLO_CASES = ['te_normal', 'te_oblique', 'tm_normal', 'tm_oblique', 'te_quar
 
 LEGACY_CASES = ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique', 'te_2layer',
                 'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer']
+
+FMMAX_LAURENT_CASES = ['rod_eps12_rect', 'hole_metal_rect']
 
 # tests/references/test_reticolo_li_nesting.py:34-39 -- RETICOLO V10 total T, (TE, TM).
 LI_NESTING_FIXED = {'3': [0.9822061471, 0.8913451356], '5': [0.9816211625, 0.8830613359]}
@@ -50,10 +56,17 @@ def main():
             keep[k] = lg[k]
     np.savez_compressed(HERE / 'legacy_subset.npz', **keep)
 
+    fl = np.load(REF / 'tests/references/golden_fmmax_laurent.npz', allow_pickle=False)
+    laurent = {k: fl[k] for k in fl.files
+               if k == '__levels' or (k.split('__')[0] in FMMAX_LAURENT_CASES
+                                      and (k.endswith('__orders') or '__phase64__' in k))}
+    np.savez_compressed(HERE / 'fmmax_laurent.npz', **laurent)
+
     (HERE / 'li_nesting_fixed.json').write_text(json.dumps(
         {'source': 'tests/references/test_reticolo_li_nesting.py:34-39 (RETICOLO V10, nesting fixed)',
          'tolerance': 1e-9, 'total_T_te_tm': LI_NESTING_FIXED}, indent=1))
-    print('wrote', len(out), 'reticolo arrays,', len(keep), 'legacy arrays')
+    print('wrote', len(out), 'reticolo arrays,', len(keep), 'legacy arrays,', len(laurent),
+          'fmmax Laurent arrays')
 
 
 if __name__ == '__main__':

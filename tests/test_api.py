@@ -84,6 +84,32 @@ def test_non_default_numerics_are_unsupported(numerics):
     assert e.value.code == 'rcwa.unsupported'
 
 
+def test_laurent_is_accepted_and_reported():
+    numerics = rcwa.Numerics(orders=(2, 2), factorization='laurent')
+    rep = rcwa.check(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert rep.ok
+    assert rep.resolved['factorization'] == 'laurent' and rep.resolved['li_nesting'] is None
+    rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda')
+    assert rep.resolved['factorization'] == 'li' and rep.resolved['li_nesting'] == 'fixed'
+
+
+@pytest.mark.parametrize('nesting', ['fixed', 'symmetric'])
+def test_an_explicit_nesting_is_refused_under_laurent(nesting):
+    """Reference engine.py:5887-5899: Laurent's rule nests nothing, so any explicit value is
+    refused rather than ignored."""
+    numerics = rcwa.Numerics(orders=(1, 1), factorization='laurent', li_nesting=nesting)
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert e.value.code == 'rcwa.unsupported' and 'li_nesting' in str(e.value)
+
+
+def test_laurent_workspace_queries_need_no_gpu():
+    lib = _native.load_library()
+    assert 0 < lib.wb_assemble_laurent_workspace_size(2, 8, 8, 3, 3) < lib.wb_assemble_li_workspace_size(2, 8, 8, 3, 3)
+    assert lib.wb_assemble_laurent_workspace_size(0, 8, 8, 3, 3) == 0
+    assert lib.wb_assemble_laurent_adjoint_workspace_size(2, 8, 8, 3, 3) > 0
+
+
 def test_declaration_errors_carry_codes():
     with pytest.raises(ValidationError) as e:
         rcwa.Numerics(orders=(1, -1))

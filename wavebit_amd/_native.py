@@ -27,6 +27,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_workspace_size', 'wb_solve', 'wb_zgemm_batched_ops', 'wb_zgetrs_batched_ops',
            'wb_solve_checkpoint_size', 'wb_solve_keep', 'wb_solve_adjoint_workspace_size',
            'wb_solve_adjoint', 'wb_assemble_li_adjoint_workspace_size', 'wb_assemble_li_adjoint',
+           'wb_assemble_laurent_workspace_size', 'wb_assemble_laurent',
+           'wb_assemble_laurent_adjoint_workspace_size', 'wb_assemble_laurent_adjoint',
            'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
@@ -92,6 +94,14 @@ def load_library():
             'wb_assemble_li_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_assemble_li_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
                                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_assemble_laurent_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_assemble_laurent': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
+                                            c_vp, c_vp, c_vp, c_sz, c_vp, c_vp]),
+            'wb_assemble_laurent_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int,
+                                                                  c_int]),
+            'wb_assemble_laurent_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int,
+                                                    c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                                    c_vp]),
             'wb_timing_enable': (c_int, [c_int]),
             'wb_gemm_tuning_configs': (c_int, []),
             'wb_gemm_tuning_config_name': (ctypes.c_char_p, [c_int]),
@@ -259,6 +269,33 @@ def assemble_li(eps, hu, hv):
     return out[0], out[1], out[2], info
 
 
+def assemble_laurent(eps, hu, hv):
+    """Laurent convolution matrices of a raster stack eps (L, Ny, Nx) -> (epx, epy, epz_i, info):
+    epx and epy are the same direct-rule matrix, epz_i its inverse; info holds L ints."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    L, Ny, Nx = eps.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    out = [torch.empty((L, N, N), dtype=torch.complex128, device=device) for _ in range(3)]
+    info = torch.empty((L,), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_assemble_laurent_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_assemble_laurent(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, _ptr(out[0]),
+                                          _ptr(out[1]), _ptr(out[2]), _ptr(ws), nbytes, _ptr(info),
+                                          _stream(device)), 'assemble_laurent')
+    return out[0], out[1], out[2], info
+
+
+def assemble(eps, hu, hv, factorization='li'):
+    """The convolution matrices of ``factorization`` ('li' or 'laurent') for a raster stack."""
+    if factorization == 'laurent':
+        return assemble_laurent(eps, hu, hv)
+    if factorization == 'li':
+        return assemble_li(eps, hu, hv)
+    raise ValueError(f'no assembly for factorization={factorization!r}')
+
+
 def solve_workspace_bytes(P, L, hu, hv, mode):
     return int(load_library().wb_solve_workspace_size(P, L, hu, hv, mode))
 
@@ -402,6 +439,25 @@ def assemble_li_adjoint(eps, hu, hv, epx, epz_i, d_epx, d_epy, d_epz):
                                              _ptr(epz_i), _ptr(d_epx), _ptr(d_epy), _ptr(d_epz),
                                              _ptr(d_eps), _ptr(ws), nbytes, _stream(device)),
               'assemble_li_adjoint')
+    return d_eps
+
+
+def assemble_laurent_adjoint(eps, hu, hv, epz_i, d_epx, d_epy, d_epz):
+    """Cotangent of the raster stack eps (L, Ny, Nx) from those of ``assemble_laurent``'s
+    outputs."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    for name, t in (('epz_i', epz_i), ('d_epx', d_epx), ('d_epy', d_epy), ('d_epz', d_epz)):
+        _require(t, torch.complex128, device, name)
+    L, Ny, Nx = eps.shape
+    d_eps = torch.empty_like(eps)
+    nbytes = ctx.lib.wb_assemble_laurent_adjoint_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_assemble_laurent_adjoint(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv,
+                                                  _ptr(epz_i), _ptr(d_epx), _ptr(d_epy),
+                                                  _ptr(d_epz), _ptr(d_eps), _ptr(ws), nbytes,
+                                                  _stream(device)), 'assemble_laurent_adjoint')
     return d_eps
 
 

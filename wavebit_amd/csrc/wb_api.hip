@@ -149,6 +149,28 @@ int wb_assemble_li(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu
                     "wb_assemble_li");
 }
 
+size_t wb_assemble_laurent_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0) return 0;
+  return wb_assemble_laurent_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                        double* epx, double* epy, double* epz_i, void* workspace,
+                        size_t workspace_bytes, int* info, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15 || !info)
+    return fail(ctx, WB_BADARG, "wb_assemble_laurent: bad dimensions (need L, Ny, Nx >= 1, 0 <= hu, hv <= 15)");
+  if (!eps || !epx || !epy || !epz_i)
+    return fail(ctx, WB_BADARG, "wb_assemble_laurent: null buffer");
+  if (workspace_bytes < wb_assemble_laurent_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_laurent: workspace too small");
+  return hip_status(ctx,
+                    wb_assemble_laurent_launch((const cd*)eps, L, Ny, Nx, hu, hv, (cd*)epx, (cd*)epy,
+                                               (cd*)epz_i, workspace, info, (hipStream_t)stream),
+                    "wb_assemble_laurent");
+}
+
 size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode) {
   if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
   return wb_solve_workspace_bytes(P, L, hu, hv, mode);
@@ -295,6 +317,31 @@ int wb_assemble_li_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx
                                                (const cd*)d_epz, (cd*)d_eps, workspace,
                                                (hipStream_t)stream),
                     "wb_assemble_li_adjoint");
+}
+
+size_t wb_assemble_laurent_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0) return 0;
+  return wb_assemble_laurent_adjoint_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_laurent_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                                const double* epz_i, const double* d_epx, const double* d_epy,
+                                const double* d_epz, double* d_eps, void* workspace,
+                                size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15)
+    return fail(ctx, WB_BADARG, "wb_assemble_laurent_adjoint: bad dimensions (need 0 <= hu, hv <= 15)");
+  if (!eps || !epz_i || !d_epx || !d_epy || !d_epz || !d_eps)
+    return fail(ctx, WB_BADARG, "wb_assemble_laurent_adjoint: null buffer");
+  if (workspace_bytes < wb_assemble_laurent_adjoint_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_laurent_adjoint: workspace too small");
+  return hip_status(ctx,
+                    wb_assemble_laurent_adjoint_launch((const cd*)eps, L, Ny, Nx, hu, hv, (const cd*)epz_i,
+                                                       (const cd*)d_epx, (const cd*)d_epy,
+                                                       (const cd*)d_epz, (cd*)d_eps, workspace,
+                                                       (hipStream_t)stream),
+                    "wb_assemble_laurent_adjoint");
 }
 
 int wb_gemm_tuning_configs(void) { return wb_gemm_config_count(); }

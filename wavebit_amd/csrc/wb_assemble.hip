@@ -13,6 +13,9 @@
 //                  block-Toeplitz layout out[(n,m),(n',m')] = A_{m-m'}[n,n'].
 //   k_direct_bttb  the 2D direct rule M[(n,m),(n',m')] = c(n-n', m-m').
 // Then epx = inv(T_u([T_v(eps)]^-1)) and epz_i = inv(M) on the batched LU path.
+// Laurent's rule (reference factorization.py:944-1034) is the direct rule in every slot:
+// epx = epy = M and epz_i = inv(M), from the column series of eps alone (k_series_v,
+// k_series_2d, k_direct_bttb); its M is the same matrix, from the same arithmetic, as Li's.
 // Each series workgroup first stages its n samples and the n twiddles e^{-2 pi i idx/n}
 // (sincospi of the exactly reduced phase idx = k*j mod n) in LDS, so the per-order sums read
 // LDS only and every twiddle is evaluated once per workgroup instead of once per (order,
@@ -232,14 +235,19 @@ __global__ void k_outer_bttb(const cd* inner, int L, int Nx, int hu, int hv, cd*
   }
 }
 
-__global__ void k_direct_bttb(const cd* c2d, int L, int hu, int hv, cd* out) {
+// out, and out2/out3 when given, receive the same matrix.
+__global__ void k_direct_bttb(const cd* c2d, int L, int hu, int hv, cd* out, cd* out2, cd* out3) {
   const int l = blockIdx.y;
   const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv, Ku = 4 * hu + 1;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)N * N) return;
   const int r = (int)(e / N), c = (int)(e % N);
   const int n1 = r / nu, m1 = r % nu, n2 = c / nu, m2 = c % nu;
-  out[(long long)l * N * N + e] = c2d[((long long)l * (4 * hv + 1) + (n1 - n2 + 2 * hv)) * Ku + (m1 - m2 + 2 * hu)];
+  const cd v = c2d[((long long)l * (4 * hv + 1) + (n1 - n2 + 2 * hv)) * Ku + (m1 - m2 + 2 * hu)];
+  const long long o = (long long)l * N * N + e;
+  out[o] = v;
+  if (out2) out2[o] = v;
+  if (out3) out3[o] = v;
 }
 
 // ------------------------------------------------------------------------------ adjoint
@@ -367,44 +375,68 @@ __global__ __launch_bounds__(64) void k_inner_adj(const cd* inner, const cd* inn
   }
 }
 
-// lines_bar[0][l][ju][kv] += u-series adjoint of c2d_bar[l][kv][:]
+// lines_bar[0][l][ju][kv] += u-series adjoint of c2d_bar[l][kv][:] (= when !accumulate)
 __global__ __launch_bounds__(64) void k_series2d_adj(const cd* c2d_bar, int L, int Nx, int hu, int hv,
-                                                     cd* lines_bar) {
+                                                     cd* lines_bar, int accumulate) {
   const int kv = blockIdx.x, l = blockIdx.y;
   const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
   const cd* cb = c2d_bar + ((long long)l * Kv + kv) * Ku;
   for (int ju = threadIdx.x; ju < Nx; ju += blockDim.x) {
     cd* x = lines_bar + ((long long)l * Nx + ju) * Kv + kv;
-    *x = cadd(*x, pixel_series_adj(cb, 1, Nx, ju, hu));
+    const cd g = pixel_series_adj(cb, 1, Nx, ju, hu);
+    *x = accumulate ? cadd(*x, g) : g;
   }
 }
 
 // eps_bar[l][jv][ju] = v-series adjoint of lines_bar[0] + conj(-1/eps^2) of lines_bar[1]
+// (the second term only when recip: Laurent's rule has no 1/eps series)
 __global__ __launch_bounds__(64) void k_series_v_adj(const cd* eps, const cd* lines_bar, int L, int Ny,
-                                                     int Nx, int hv, cd* eps_bar) {
+                                                     int Nx, int hv, cd* eps_bar, int recip) {
   const int ju = blockIdx.x, l = blockIdx.y;
   const int Kv = 4 * hv + 1;
   const cd* lb0 = lines_bar + ((long long)l * Nx + ju) * Kv;
   const cd* lb1 = lines_bar + (((long long)L + l) * Nx + ju) * Kv;
   for (int jv = threadIdx.x; jv < Ny; jv += blockDim.x) {
     const long long o = ((long long)l * Ny + jv) * Nx + ju;
+    const cd g0 = pixel_series_adj(lb0, 1, Ny, jv, hv);
+    if (!recip) {
+      eps_bar[o] = g0;
+      continue;
+    }
     const cd a = eps[o];
     const cd r = crecip(a);
     const cd dr = cneg(cmul(r, r));          // d(1/a)/da
-    const cd g0 = pixel_series_adj(lb0, 1, Ny, jv, hv);
     const cd g1 = pixel_series_adj(lb1, 1, Ny, jv, hv);
     eps_bar[o] = cadd(g0, cmulc(dr, g1));
   }
 }
 
-// Column series of eps and 1/eps (lines), staged in LDS up to SERIES_LDS_MAX rows.
-hipError_t series_v_launch(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines, hipStream_t stream) {
+// Column series of eps and (slots == 2) 1/eps (lines), staged in LDS up to SERIES_LDS_MAX rows.
+hipError_t series_v_launch(const cd* eps, int L, int Ny, int Nx, int hv, cd* lines, hipStream_t stream,
+                           int slots = 2) {
   if (Ny <= SERIES_LDS_MAX)
-    hipLaunchKernelGGL(k_series_v_lds, dim3(Nx, L, 2), dim3(64), 2 * Ny * sizeof(cd), stream, eps, L, Ny,
-                       Nx, hv, lines);
+    hipLaunchKernelGGL(k_series_v_lds, dim3(Nx, L, slots), dim3(64), 2 * Ny * sizeof(cd), stream, eps, L,
+                       Ny, Nx, hv, lines);
   else
-    hipLaunchKernelGGL(k_series_v, dim3(Nx, L, 2), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
+    hipLaunchKernelGGL(k_series_v, dim3(Nx, L, slots), dim3(64), 0, stream, eps, L, Ny, Nx, hv, lines);
   return hipGetLastError();
+}
+
+// The direct 2D coefficients c2d (L, Kv, Ku) of eps from its column series (slot 0 of lines).
+hipError_t series_2d_launch(const cd* lines, int L, int Nx, int hu, int hv, cd* c2d, hipStream_t stream) {
+  const int Kv = 4 * hv + 1;
+  if (Nx <= SERIES_LDS_MAX)
+    hipLaunchKernelGGL(k_series_2d_lds, dim3(Kv, L), dim3(64), 2 * Nx * sizeof(cd), stream, lines, L, Nx,
+                       hu, hv, c2d);
+  else
+    hipLaunchKernelGGL(k_series_2d, dim3(Kv, L), dim3(64), 0, stream, lines, L, Nx, hu, hv, c2d);
+  return hipGetLastError();
+}
+
+// z[i] += x[i] + y[i] over n complex entries
+__global__ void k_add2(cd* z, const cd* x, const cd* y, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = cadd(z[i], cadd(x[i], y[i]));
 }
 
 }  // namespace
@@ -439,12 +471,7 @@ hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, i
     // the gathers (raster -> epx pre-inversion, epy, epz pre-inversion): one timed stage
     WbStageTimer timer(WB_T_ASSEMBLE, stream, 0.0, WB_STAGE_BOTH);
     if ((e = series_v_launch(eps, L, Ny, Nx, hv, lines, stream)) != hipSuccess) return e;
-    if (Nx <= SERIES_LDS_MAX)
-      hipLaunchKernelGGL(k_series_2d_lds, dim3(Kv, L), dim3(64), 2 * Nx * sizeof(cd), stream, lines, L, Nx,
-                         hu, hv, c2d);
-    else
-      hipLaunchKernelGGL(k_series_2d, dim3(Kv, L), dim3(64), 0, stream, lines, L, Nx, hu, hv, c2d);
-    WB_CHECK_LAUNCH();
+    if ((e = series_2d_launch(lines, L, Nx, hu, hv, c2d, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_inner_inv, dim3(Nx, L, 2), dim3(64), 0, stream, lines, L, Nx, hv, inner);
     WB_CHECK_LAUNCH();
     // slot 0 (eps) -> epx pre-inversion, slot 1 (1/eps) -> epy directly
@@ -456,7 +483,7 @@ hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, i
                          pre, epy);
     WB_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), dim3(256), 0, stream, c2d,
-                       L, hu, hv, epz_pre);
+                       L, hu, hv, epz_pre, (cd*)nullptr, (cd*)nullptr);
     WB_CHECK_LAUNCH();
   }
   // epx = inv(pre), epz_i = inv(epz_pre): one batched inversion of 2L matrices
@@ -523,10 +550,89 @@ hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int 
                      lines_bar);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_series2d_adj, dim3(Kv, L), dim3(64), 0, stream, c2d_bar, L, Nx, hu, hv,
-                     lines_bar);
+                     lines_bar, 1);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_series_v_adj, dim3(Nx, L), dim3(64), 0, stream, eps, lines_bar, L, Ny, Nx, hv,
-                     d_eps);
+                     d_eps, 1);
+  WB_CHECK_LAUNCH();
+  return hipSuccess;
+}
+
+// ------------------------------------------------------------------------ Laurent's rule
+size_t wb_assemble_laurent_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  size_t b = 0;
+  b += sizeof(cd) * L * Nx * Kv;   // column series of eps
+  b += sizeof(cd) * L * Kv * Ku;   // c2d
+  b += sizeof(cd) * L * N * N;     // the direct rule, factored in place by the inversion
+  b += sizeof(int) * L * N;        // pivots
+  return b + 1024;
+}
+
+hipError_t wb_assemble_laurent_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
+                                      cd* epy, cd* epz_i, void* ws, int* info, hipStream_t stream) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  char* p = (char*)ws;
+  cd* lines = (cd*)p; p += sizeof(cd) * (size_t)L * Nx * Kv;
+  cd* c2d = (cd*)p; p += sizeof(cd) * (size_t)L * Kv * Ku;
+  cd* pre = (cd*)p; p += sizeof(cd) * (size_t)L * N * N;
+  int* ipiv = (int*)p;
+  const long long NN = (long long)N * N;
+  hipError_t e;
+  {
+    WbStageTimer timer(WB_T_ASSEMBLE, stream, 0.0, WB_STAGE_BOTH);
+    if ((e = series_v_launch(eps, L, Ny, Nx, hv, lines, stream, 1)) != hipSuccess) return e;
+    if ((e = series_2d_launch(lines, L, Nx, hu, hv, c2d, stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), dim3(256), 0, stream, c2d,
+                       L, hu, hv, epx, epy, pre);
+    WB_CHECK_LAUNCH();
+  }
+  return wb_inverse_launch(N, bp_strided(pre, NN), N, bp_strided(epz_i, NN), N, ipiv, info, L, stream);
+}
+
+size_t wb_assemble_laurent_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  size_t b = 0;
+  b += sizeof(cd) * L * Nx * Kv;   // lines_bar
+  b += sizeof(cd) * L * Kv * Ku;   // c2d_bar
+  b += sizeof(cd) * 2 * L * N * N; // cotangent of the direct rule, GEMM temporary
+  return b + 1024;
+}
+
+// d_eps from the cotangents of epx = epy = M and epz_i = inv(M):
+// M_bar = d_epx + d_epy - epz_i^H d_epz epz_i^H, then the direct rule's and the series' adjoints.
+hipError_t wb_assemble_laurent_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                              const cd* epz_i, const cd* d_epx, const cd* d_epy,
+                                              const cd* d_epz, cd* d_eps, void* ws, hipStream_t stream) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const long long NN = (long long)N * N;
+  char* p = (char*)ws;
+  cd* lines_bar = (cd*)p; p += sizeof(cd) * (size_t)L * Nx * Kv;
+  cd* c2d_bar = (cd*)p; p += sizeof(cd) * (size_t)L * Kv * Ku;
+  cd* zbar = (cd*)p; p += sizeof(cd) * (size_t)L * NN;
+  cd* tmp = (cd*)p;
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
+  hipError_t e;
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, one, bp_strided(d_epz, NN), N, bp_strided(epz_i, NN), N,
+                               zero, bp_strided(tmp, NN), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mone, bp_strided(epz_i, NN), N, bp_strided(tmp, NN), N,
+                               zero, bp_strided(zbar, NN), N, L, stream))) return e;
+  const long long tot = (long long)L * NN;
+  hipLaunchKernelGGL(k_add2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, zbar, d_epx, d_epy,
+                     tot);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_direct_adj, dim3((Kv * Ku + 255) / 256, L), dim3(256), 0, stream, zbar, L, hu, hv,
+                     c2d_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series2d_adj, dim3(Kv, L), dim3(64), 0, stream, c2d_bar, L, Nx, hu, hv, lines_bar,
+                     0);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series_v_adj, dim3(Nx, L), dim3(64), 0, stream, eps, lines_bar, L, Ny, Nx, hv, d_eps,
+                     0);
   WB_CHECK_LAUNCH();
   return hipSuccess;
 }

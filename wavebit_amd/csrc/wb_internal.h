@@ -42,6 +42,13 @@ hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void*
 size_t wb_assemble_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_li_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
                                  cd* epy, cd* epz_i, void* ws, int* info_out, hipStream_t stream);
+size_t wb_assemble_laurent_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+hipError_t wb_assemble_laurent_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv, cd* epx,
+                                      cd* epy, cd* epz_i, void* ws, int* info, hipStream_t stream);
+size_t wb_assemble_laurent_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+hipError_t wb_assemble_laurent_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                              const cd* epz_i, const cd* d_epx, const cd* d_epy,
+                                              const cd* d_epz, cd* d_eps, void* ws, hipStream_t stream);
 size_t wb_assemble_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
                                       const cd* epx, const cd* epz_i, const cd* d_epx,

@@ -110,11 +110,11 @@ class ConvCache:
         self.device = device
         self._store = {}
 
-    def get(self, key, eps, hu, hv):
+    def get(self, key, eps, hu, hv, factorization='li'):
         hit = self._store.get(key)
         if hit is not None:
             return hit
-        epx, epy, epz_i, info = _native.assemble_li(eps, hu, hv)
+        epx, epy, epz_i, info = _native.assemble(eps, hu, hv, factorization)
         if int(info.abs().sum()) != 0:
             raise FloatingPointError('a layer\'s convolution matrix is singular')
         self._store[key] = (epx, epy, epz_i)

@@ -157,6 +157,7 @@ class _Resolved:
     hu: int
     hv: int
     uniform: bool
+    factorization: str = 'li'    # 'li' (the default) or 'laurent'
 
 
 def _resolve(problem, numerics, execution):
@@ -170,8 +171,17 @@ def _resolve(problem, numerics, execution):
     if bad is not None:
         raise UnsupportedFeatureError(
             f'numerics.{bad[0]}={bad[1]!r} is not implemented by this engine (it implements the '
-            'raster defaults: SMM, sampled coefficients, Li rules with fixed nesting).',
+            'raster defaults: SMM, sampled coefficients, Li rules with fixed nesting, and the '
+            "opt-in factorization='laurent').",
             code='rcwa.unsupported', path=f'numerics.{bad[0]}')
+    factorization = numerics.factorization or 'li'
+    if factorization == 'laurent' and numerics.li_nesting is not None:
+        # reference engine.py:5887-5899: an explicit nesting has no reader under Laurent's rule
+        raise UnsupportedFeatureError(
+            f"li_nesting={numerics.li_nesting!r} has no reader here: factorization='laurent' "
+            "nests no rules -- it takes the direct rule for every product, so there is no "
+            "inverse for an order to matter to. It selects the order Li's rules are nested in, "
+            "so it needs factorization='li'.", code='rcwa.unsupported', path='numerics.li_nesting')
     if problem.outputs.fields:
         raise UnsupportedFeatureError('Field reconstruction is not implemented by this engine.',
                                       code='rcwa.unsupported', path='outputs.fields')
@@ -259,7 +269,7 @@ def _resolve(problem, numerics, execution):
     ws = _wavelengths(problem)
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
-                     hu=hu, hv=hv, uniform=uniform)
+                     hu=hu, hv=hv, uniform=uniform, factorization=factorization)
 
 
 def _structure_key(r):
@@ -307,8 +317,10 @@ def _report(problem, numerics, execution, r):
         resolved=MappingProxyType({'order_window': ((0, r.hu), (0, r.hv)),
                                    'wavelength_count': int(r.wavelengths.size),
                                    'layer_count': len(problem.stack.layers),
-                                   'fourier_coefficients': 'sampled', 'factorization': 'li',
-                                   'li_nesting': 'fixed', 'algorithm': 'smm',
+                                   'fourier_coefficients': 'sampled',
+                                   'factorization': r.factorization,
+                                   'li_nesting': 'fixed' if r.factorization == 'li' else None,
+                                   'algorithm': 'smm',
                                    'formulation': 'second_order', 'solve_path': r.path,
                                    'backend': execution.backend, 'device': execution.device,
                                    'precision': execution.precision, 'engine': 'wavebit_amd/hip-gfx950',
@@ -338,12 +350,12 @@ class _Cache:
         self.conv = _engine.ConvCache(device)
         self.graph = [None, None]    # captured solve of this preparation (launch-bound sizes)
 
-    def get(self, rasters, hu, hv):
+    def get(self, rasters, hu, hv, factorization='li'):
         import torch
         key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
-               rasters.shape, hu, hv)
+               rasters.shape, hu, hv, factorization)
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.conv.device)
-        return self.conv.get(key, eps, hu, hv)
+        return self.conv.get(key, eps, hu, hv, factorization)
 
 
 @dataclass(frozen=True)
@@ -434,7 +446,8 @@ def _run_live(prepared, problem, r, checks):
                             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                             period=r.period, thickness=tuple(float(t) for t in r.thickness))
               for wl in r.wavelengths]
-    eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device)
+    eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device,
+                                        factorization=r.factorization)
     if bool((flags != 0).any()):
         raise NumericalError('A layer eigensolve did not converge or a scattering-matrix '
                              'factorization was singular.', code='rcwa.singular', path='problem')
@@ -468,7 +481,7 @@ def _solve_resolved(r, cache):
     import torch
     device = torch.device(cache.conv.device)
     _engine._native.context(device)          # raises EngineUnavailable without a GPU or library
-    conv = cache.get(r.rasters, r.hu, r.hv)
+    conv = cache.get(r.rasters, r.hu, r.hv, r.factorization)
     bucket = _engine.Bucket(hu=r.hu, hv=r.hv, path=r.path, n_layers=r.rasters.shape[0])
     for wl in r.wavelengths:
         bucket.points.append(_engine.Point(

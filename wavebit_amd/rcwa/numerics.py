@@ -56,7 +56,7 @@ class Numerics:
     def unsupported(self):
         """The first non-default choice this engine does not implement, as (field, value)."""
         for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled')),
-                         ('factorization', (None, 'li')), ('formulation', ('auto', 'second_order')),
+                         ('factorization', (None, 'li', 'laurent')), ('formulation', ('auto', 'second_order')),
                          ('li_nesting', (None, 'fixed')), ('detuning', (None,))):
             if getattr(self, name) not in ok:
                 return name, getattr(self, name)

@@ -71,9 +71,9 @@ class ConvLRU:
         self.hits = 0
         self.misses = 0
 
-    def get(self, rasters, hu, hv):
+    def get(self, rasters, hu, hv, factorization='li'):
         key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
-               rasters.shape, hu, hv)
+               rasters.shape, hu, hv, factorization)
         hit = self._d.get(key)
         if hit is not None:
             self._d.move_to_end(key)
@@ -81,7 +81,7 @@ class ConvLRU:
             return hit
         self.misses += 1
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.device)
-        epx, epy, epz, info = _native.assemble_li(eps, hu, hv)
+        epx, epy, epz, info = _native.assemble(eps, hu, hv, factorization)
         if int(info.abs().sum()) != 0:
             raise NumericalError('a layer convolution matrix is singular', code='rcwa.singular',
                                  path='stack.layers')
@@ -131,7 +131,7 @@ class Server:
         if bad:
             raise ValidationError(f'runtime guard {bad[0]} failed', code=f'prepared.{bad[0]}',
                                   path='problem')
-        conv = self.conv.get(r.rasters, r.hu, r.hv)
+        conv = self.conv.get(r.rasters, r.hu, r.hv, r.factorization)
         work = _Work(req=req, resolved=r, n_points=len(r.wavelengths), t_admit=now)
         work.checks = checks
         pts = []
@@ -173,7 +173,8 @@ class Server:
                                requires_grad=True)
             thick = torch.tensor(np.asarray(r.thickness, dtype=np.float64), device=self.device,
                                  requires_grad=True)
-            eff, amp, pe, flags = adjoint.solve(eps, thick, points, r.hu, r.hv, self.device)
+            eff, amp, pe, flags = adjoint.solve(eps, thick, points, r.hu, r.hv, self.device,
+                                                factorization=r.factorization)
             (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
             d_eps, d_t = eps.grad.detach().cpu().numpy(), thick.grad.detach().cpu().numpy()
             eff_h, amp_h, pe_h, fl = (t.detach().cpu().numpy() for t in (eff, amp, pe, flags))
