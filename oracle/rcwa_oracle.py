@@ -37,6 +37,10 @@ _HALF_PI_F32_BELOW = float(np.nextafter(np.float32(_HALF_PI), np.float32(0)))
 PINNED = {
     'raster_series': 'closed-form pixel integrals (tests/test_oracle.py::test_series_is_exact_pixel_integral)',
     'li_conv_matrices': 'legacy golden epz_i 1e-12; uniform -> eps*I; y-invariant -> 1D Li rule',
+    'segment_series': 'closed-form integrals of a staircase; equals raster_series on pixel edges',
+    'compression_indices': 'the compressed layer on its edges has the raster\'s own series',
+    'li_conv_matrices_integrated': 'legacy golden 2d_cfs (integrated) 1e-12; equals the sampled rule',
+    'laurent_conv_matrices_integrated': 'equals the sampled Laurent rule; fmmax goldens through it',
     'laurent_conv_matrices': 'fmmax FFT goldens per order (2 cells); inv(legacy golden epz_i) 1e-12; '
                              'uniform -> eps*I; y-invariant TE equals Li',
     'boundary_kz': 'Fresnel slab closed form; RETICOLO goldens',
@@ -75,6 +79,41 @@ def raster_series(a, half):
     return out
 
 
+def segment_series(a, edges, half):
+    """Fourier coefficients along the last axis of a profile constant on segments whose
+    right-hand edges are ``edges`` (increasing, the last closing the period), orders
+    ``-2*half..2*half``: the closed-form segment integral in its jump form (``factorization.py:
+    443-500``, ``_segment_bounds`` / ``_segment_series``):
+    ``c_k = sum_j (a_j - a_{j-1}) exp(-2 pi i k left_j) / (2 pi i k)``, ``c_0 = sum_j a_j width_j``.
+    """
+    a = np.asarray(a, dtype=np.complex128)
+    right = np.asarray(edges, dtype=np.float64).reshape(-1)
+    right = right / right[-1]
+    left = np.concatenate([[0.0], right[:-1]])
+    width = right - left
+    orders = np.arange(-2 * half, 2 * half + 1)
+    with np.errstate(divide='ignore'):
+        scale = np.where(orders == 0, 0.0, 1.0 / (2j * _PI * np.where(orders == 0, 1, orders)))
+    weights = np.exp(-2j * _PI * (left[:, None] * orders[None, :]))
+    jumps = a - np.roll(a, 1, axis=-1)
+    ac = jumps @ (weights * scale[None, :])
+    dc = (a @ width)[..., None]
+    return np.where(orders == 0, dc, ac)
+
+
+def compression_indices(cell):
+    """Where a raster's runs of equal columns, then of equal rows (on the column-reduced raster),
+    end, and their right-hand edges on the unit cell (``factorization.py:3005-3047``)."""
+    cell = np.asarray(cell)
+    n_row, n_col = cell.shape
+    col_idx = np.flatnonzero(np.any(cell[:, :-1] != cell[:, 1:], axis=0)).tolist() + [n_col - 1]
+    reduced = cell[:, col_idx]
+    row_idx = np.flatnonzero(np.any(reduced[:-1, :] != reduced[1:, :], axis=1)).tolist() + [n_row - 1]
+    x = (np.asarray(col_idx, dtype=np.float64) + 1.0) / n_col
+    y = (np.asarray(row_idx, dtype=np.float64) + 1.0) / n_row
+    return row_idx, col_idx, x, y
+
+
 def toeplitz_positions(half):
     """Gather index of order ``r - c`` in a ``4*half+1`` vector (``factorization.py:276-286,
     358-366``): row-major over an ``(2*half+1)**2`` matrix."""
@@ -83,25 +122,26 @@ def toeplitz_positions(half):
     return (r[:, None] - r[None, :] + 2 * half).reshape(-1)
 
 
-def conv_matrix(grid, hu, hv):
+def conv_matrix(grid, hu, hv, series_u=raster_series, series_v=raster_series):
     """Direct rule both ways, BTTB layout ``M[(n,m),(n',m')] = c(n-n', m-m')``, v first
-    (``factorization.py:504-520``)."""
+    (``factorization.py:504-520``). ``series_u`` / ``series_v``: ``f(a, half)`` along the last
+    axis (the pixel series by default)."""
     nu, nv = 2 * hu + 1, 2 * hv + 1
-    by_column = raster_series(grid.T, hv)            # [j_u, k_v]
-    coeffs = raster_series(by_column.T, hu)           # [k_v, k_u]
+    by_column = series_v(grid.T, hv)                  # [j_u, k_v]
+    coeffs = series_u(by_column.T, hu)                # [k_v, k_u]
     blocks = coeffs[toeplitz_positions(hv)][:, toeplitz_positions(hu)]
     out = blocks.reshape(nv, nv, nu, nu).transpose(0, 2, 1, 3)
     return out.reshape(nv * nu, nv * nu)
 
 
-def nested_rule_v(grid, hu, hv, invert_outer):
+def nested_rule_v(grid, hu, hv, invert_outer, series_u=raster_series, series_v=raster_series):
     """Li's nested operator with ``v`` inside (``factorization.py:523-567``, ``inner='v'``):
     ``T_u([T_v(g)]^-1)``, optionally inverted."""
     nu, nv = 2 * hu + 1, 2 * hv + 1
-    lines = raster_series(grid.T, hv)                # [j_u, k_v]
+    lines = series_v(grid.T, hv)                      # [j_u, k_v]
     inner = np.linalg.inv(lines[:, toeplitz_positions(hv)].reshape(lines.shape[0], nv, nv))
     entries = inner.reshape(lines.shape[0], nv * nv).T
-    outer = raster_series(entries, hu)[:, toeplitz_positions(hu)]
+    outer = series_u(entries, hu)[:, toeplitz_positions(hu)]
     out = outer.reshape(nv, nv, nu, nu).transpose(0, 2, 1, 3).reshape(nv * nu, nv * nu)
     return np.linalg.inv(out) if invert_outer else out
 
@@ -124,7 +164,39 @@ def laurent_conv_matrices(eps, hu, hv):
     return direct, direct, np.linalg.inv(direct)
 
 
-CONV_RULES = {'li': li_conv_matrices, 'laurent': laurent_conv_matrices}
+def _integrated_parts(eps):
+    """The compressed layer and its segment series pair (``factorization.py:651-656, 3135-3156``)."""
+    eps = np.asarray(eps, dtype=np.complex128)
+    row_idx, col_idx, x, y = compression_indices(eps)
+    comp = eps[row_idx][:, col_idx]
+    return comp, (lambda a, h: segment_series(a, x, h)), (lambda a, h: segment_series(a, y, h))
+
+
+def li_conv_matrices_integrated(eps, hu, hv):
+    """Li's fixed nesting with integrated (closed-form segment) coefficients on the compressed
+    raster (``factorization.py:674-686, 3181-3206``)."""
+    comp, su, sv = _integrated_parts(eps)
+    epx = nested_rule_v(comp, hu, hv, True, su, sv)
+    epy = nested_rule_v(1.0 / comp, hu, hv, False, su, sv)
+    epz_i = np.linalg.inv(conv_matrix(comp, hu, hv, su, sv))
+    return epx, epy, epz_i
+
+
+def laurent_conv_matrices_integrated(eps, hu, hv):
+    """Laurent's rule with integrated coefficients (``factorization.py:1003-1013, 3209-3232``)."""
+    comp, su, sv = _integrated_parts(eps)
+    direct = conv_matrix(comp, hu, hv, su, sv)
+    return direct, direct, np.linalg.inv(direct)
+
+
+CONV_RULES = {'li': li_conv_matrices, 'laurent': laurent_conv_matrices,
+              ('li', 'integrated'): li_conv_matrices_integrated,
+              ('laurent', 'integrated'): laurent_conv_matrices_integrated}
+
+
+def conv_rule(factorization='li', coefficients='sampled'):
+    """The oracle's builder for one (factorization, fourier_coefficients) pair."""
+    return CONV_RULES[factorization if coefficients in (None, 'sampled') else (factorization, coefficients)]
 
 
 # --------------------------------------------------------------------------- conventions
@@ -311,7 +383,7 @@ def _centre_parity(hu, hv):
 
 
 def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, theta, hu,
-           phi=None, factorization='li'):
+           phi=None, factorization='li', coefficients='sampled'):
     """Pure 1D TE/TM solve of a one-row raster stack, top incidence.
 
     Restates ``scattering.py:704-902`` as the engine calls it (``engine.py:3283-3316``): kernel
@@ -339,7 +411,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     S_trn = s_transmission(I, V_kb, I, V_gap)
     S = S_ref
     for li in range(len(thickness)):
-        epx, epy, epz_i = CONV_RULES[factorization](eps_layers[li], hu, 0)
+        epx, epy, epz_i = conv_rule(factorization, coefficients)(eps_layers[li], hu, 0)
         W, V, q = layer_modes_1d(tetm, kx, epx, epy, epz_i)
         X = np.exp(-k0 * thickness[li] * q)
         A, B = ab_matrices(W, V, I, V_gap)
@@ -475,15 +547,16 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
 
 
 def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, theta, phi,
-                 polarization, orders, factorization='li'):
+                 polarization, orders, factorization='li', coefficients='sampled'):
     """One plane-wave point through a raster stack, routed the way the reference routes it.
 
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
     is ``'s'``/``'p'`` or a Jones pair. Routing (``engine.py:3092-3181``): a one-row raster with
     a polarization word and ``phi is None`` takes the pure 1D path; everything else 2D (the 1D
-    conical layout is out of scope). ``factorization``: ``'li'`` (the default) or ``'laurent'``.
+    conical layout is out of scope). ``factorization``: ``'li'`` (the default) or ``'laurent'``;
+    ``coefficients``: ``'sampled'`` (the default) or ``'integrated'``.
     """
-    rule = CONV_RULES[factorization]
+    rule = conv_rule(factorization, coefficients)
     eps_layers = np.asarray(eps_layers, dtype=np.complex128)
     hu, hv = orders
     one_row = eps_layers.shape[1] == 1
@@ -491,7 +564,7 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         if one_row and phi is None and hv == 0:
             tetm = 0 if polarization == 's' else 1
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
-                         theta, hu, phi, factorization)
+                         theta, hu, phi, factorization, coefficients)
             inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
                    if k not in ('kx', 'power_entering')}
             inc['power_entering'] = out['power_entering']

@@ -30,7 +30,7 @@ RETICOLO_CASES = ['te_normal', 'te_oblique', 'tm_normal', 'tm_oblique', 'te_quar
                   'tm_quarter', 'te_two_layer', 'tm_two_layer', 'te_high_order', 'tm_high_order']
 
 LEGACY_CASES = ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique', 'te_2layer',
-                'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer']
+                'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer', '2d_cfs']
 
 FMMAX_LAURENT_CASES = ['rod_eps12_rect', 'hole_metal_rect']
 

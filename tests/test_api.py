@@ -77,7 +77,7 @@ def test_cpu_backends_are_refused_not_rerouted(kwargs, code):
 @pytest.mark.parametrize('numerics', [rcwa.Numerics(orders=(1, 1), algorithm='etm'),
                                       rcwa.Numerics(orders=(1, 1), factorization='nvf'),
                                       rcwa.Numerics(orders=(1, 1), li_nesting='symmetric'),
-                                      rcwa.Numerics(orders=(1, 1), fourier_coefficients='integrated')])
+                                      rcwa.Numerics(orders=(1, 1), formulation='first_order')])
 def test_non_default_numerics_are_unsupported(numerics):
     with pytest.raises(UnsupportedFeatureError) as e:
         rcwa.prepare(_problem(), numerics=numerics, backend='torch', device='cuda')
@@ -91,6 +91,12 @@ def test_laurent_is_accepted_and_reported():
     assert rep.resolved['factorization'] == 'laurent' and rep.resolved['li_nesting'] is None
     rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda')
     assert rep.resolved['factorization'] == 'li' and rep.resolved['li_nesting'] == 'fixed'
+
+
+def test_integrated_coefficients_are_accepted_and_reported():
+    numerics = rcwa.Numerics(orders=(2, 2), fourier_coefficients='integrated')
+    rep = rcwa.check(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert rep.ok and rep.resolved['fourier_coefficients'] == 'integrated'
 
 
 @pytest.mark.parametrize('nesting', ['fixed', 'symmetric'])

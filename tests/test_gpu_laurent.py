@@ -1,4 +1,5 @@
-"""Laurent's rule (``Numerics(factorization='laurent')``) on the GPU.
+"""Laurent's rule (``Numerics(factorization='laurent')``) and integrated coefficients
+(``fourier_coefficients='integrated'``) on the GPU.
 
 * Assembly against the oracle (``oracle.rcwa_oracle.laurent_conv_matrices``) at 1e-12 relative;
   the two in-plane slots equal to the bit, and the z slot equal to the bit to Li's.
@@ -11,6 +12,8 @@
   oracle (1e-6 relative, step 1e-6).
 * Serving: an 'li' and a 'laurent' request on the same raster share no cached matrices; each
   equals its solve alone.
+* Integrated coefficients under both rules: per order within 1e-10 of the oracle's closed-form
+  segment series on a compressible raster, and equal to the sampled run.
 """
 import numpy as np
 import pytest
@@ -205,3 +208,25 @@ def test_serving_keeps_li_and_laurent_apart():
         assert np.abs(np.asarray(results[i].R) - np.asarray(alone[i].R)).max() < 1e-10
         assert np.abs(np.asarray(results[i].T) - np.asarray(alone[i].T)).max() < 1e-10
     assert np.abs(np.asarray(alone[0].T) - np.asarray(alone[2].T)).max() > 1e-6
+
+
+@pytest.mark.parametrize('rule', ['li', 'laurent'])
+def test_integrated_coefficients_match_the_oracle_segment_series(rule):
+    rng = np.random.default_rng(17)
+    raster = np.where(rng.uniform(size=(2, 12, 12)) < 0.5, 2.25, 9.0 + 0.2j).astype(np.complex128)
+    raster[:, :, :4] = 1.0
+    thick, wls, pol = [0.2, 0.11], [0.8, 0.95], (0.6, 0.8j)
+    runs = {}
+    for coef in ('sampled', 'integrated'):
+        problem = _problem(raster, (0.9, 0.8), pol, wavelength=wls, theta=0.2, phi=0.4, thick=thick,
+                           bottom=2.25, units='um')
+        numerics = rcwa.Numerics(orders=(3, 3), factorization=rule, fourier_coefficients=coef)
+        rep = rcwa.check(problem, numerics=numerics, backend='torch', device='cuda:0')
+        assert rep.ok and rep.resolved['fourier_coefficients'] == coef
+        runs[coef] = rcwa.run(problem, numerics=numerics, backend='torch', device='cuda:0')
+    np.testing.assert_array_equal(np.asarray(runs['integrated'].R), np.asarray(runs['sampled'].R))
+    for i, wl in enumerate(wls):
+        want = orc.solve_raster(raster, thick, 1.0, 2.25, (0.9, 0.8), wl, 0.2, 0.4, pol, (3, 3), rule,
+                                'integrated')['incident']
+        assert np.abs(np.asarray(runs['integrated'].R)[i] - want['R']).max() < 1e-10
+        assert np.abs(np.asarray(runs['integrated'].T)[i] - want['T']).max() < 1e-10

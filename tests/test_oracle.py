@@ -49,6 +49,7 @@ LEGACY = {
     '2d_normal': dict(psi=0., phi=0., eps=_rand((1, 5, 5), 0), d=[0.2], top=1., bot=1., theta=0., M=(2, 2), period=(1., 1.), wl=0.9),
     '2d_oblique': dict(psi=0.9, phi=0.4, eps=_rand((1, 6, 6), 1), d=[0.3], top=1., bot=2.25, theta=0.3, M=(2, 2), period=(1., 1.2), wl=0.8),
     '2d_2layer': dict(psi=1.1, phi=0.5, eps=_rand((2, 4, 4), 2), d=[0.15, 0.22], top=1.44, bot=1., theta=0.2, M=(2, 2), period=(0.9, 0.9), wl=0.7),
+    '2d_cfs': dict(psi=0.7, phi=0.3, eps=_rand((1, 5, 5), 4), d=[0.2], top=2.25, bot=1., theta=0.2, M=(2, 2), period=(1., 1.), wl=0.9, coef='integrated'),
 }
 
 
@@ -151,7 +152,7 @@ def test_oracle_matches_reticolo_1d(name):
 def test_oracle_matches_legacy_golden(name):
     c, eps, d, pol = legacy_case(name)
     res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'],
-                           pol, c['M'])
+                           pol, c['M'], coefficients=c.get('coef', 'sampled'))
     hu, hv = c['M']
     kx = (orc.incident_kxy(c['top'], c['theta'], c['phi'])[0]
           + np.arange(-hu, hu + 1) * c['wl'] / c['period'][0])
@@ -166,14 +167,15 @@ def test_oracle_matches_legacy_golden(name):
             assert np.abs(np.reshape(want, got[k].shape) - got[k]).max() < 1e-12, (sub, k)
 
 
-@pytest.mark.parametrize('name', ['2d_normal', '2d_oblique', '2d_2layer'])
+@pytest.mark.parametrize('name', ['2d_normal', '2d_oblique', '2d_2layer', '2d_cfs'])
 def test_oracle_epz_i_and_q_match_legacy(name):
     g = np.load(GOLDEN / 'legacy_subset.npz')
     c, eps, d, pol = legacy_case(name)
     hu, hv = c['M']
+    coef = c.get('coef', 'sampled')
     res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'],
-                           pol, c['M'])
-    conv = [orc.li_conv_matrices(e, hu, hv) for e in eps]
+                           pol, c['M'], coefficients=coef)
+    conv = [orc.conv_rule('li', coef)(e, hu, hv) for e in eps]
     kx, ky = orc.order_kxy_2d(c['wl'], c['period'], c['top'], c['theta'], c['phi'], hu, hv)
     kxf = np.broadcast_to(kx[None, :], (2 * hv + 1, 2 * hu + 1)).reshape(-1)
     kyf = np.broadcast_to(ky[:, None], (2 * hv + 1, 2 * hu + 1)).reshape(-1)
@@ -256,3 +258,47 @@ def test_modes_satisfy_both_maxwell_factors():
     assert np.abs(O2 @ W - W * q ** 2).max() < 1e-11 * np.abs(O2).max()
     assert np.abs(OR @ W - V * q).max() < 1e-11 * np.abs(OR).max()
     assert np.all(q.real >= 0)
+
+
+# ------------------------------------------------------------- integrated (segment) coefficients
+
+def test_segment_series_is_the_closed_form_integral():
+    """Three segments on [0, 0.2), [0.2, 0.7), [0.7, 1): c_k against the integral written out."""
+    a = np.array([2.0, 5.0 - 1j, 1.5])
+    edges = np.array([0.2, 0.7, 1.0])
+    got = orc.segment_series(a, edges, 3)
+    left = np.array([0.0, 0.2, 0.7])
+    for i, k in enumerate(range(-6, 7)):
+        if k == 0:
+            want = np.sum(a * (edges - left))
+        else:
+            want = np.sum(a * (np.exp(-2j * np.pi * k * left) - np.exp(-2j * np.pi * k * edges))
+                          / (2j * np.pi * k))
+        assert abs(got[i] - want) < 1e-15 * 10
+
+
+def test_segment_series_on_pixel_edges_is_the_pixel_series():
+    rng = np.random.default_rng(9)
+    a = rng.uniform(1, 9, (3, 12)) + 1j * rng.uniform(0, 1, (3, 12))
+    got = orc.segment_series(a, np.arange(1, 13) / 12.0, 7)
+    assert np.abs(got - orc.raster_series(a, 7)).max() < 1e-13
+
+
+@pytest.mark.parametrize('rule', ['li', 'laurent'])
+@pytest.mark.parametrize('shape,hu,hv', [((16, 16), 4, 3), ((1, 40), 8, 0), ((9, 7), 2, 4)])
+def test_integrated_rules_equal_the_sampled_ones_on_a_raster(rule, shape, hu, hv):
+    """A raster is a staircase; both series integrate it exactly, the integrated one after
+    compressing runs of equal columns and rows, so the two builds agree to round-off."""
+    rng = np.random.default_rng(shape[0] * 31 + hu)
+    eps = np.where(rng.uniform(size=shape) < 0.5, 2.25, 12.25 + 0.3j).astype(np.complex128)
+    eps[:, : shape[1] // 3] = 1.0                                   # wide runs for the compression
+    row_idx, col_idx, x, y = orc.compression_indices(eps)
+    assert len(col_idx) < shape[1]
+    for got, want in zip(orc.conv_rule(rule, 'integrated')(eps, hu, hv), orc.conv_rule(rule)(eps, hu, hv)):
+        assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_integrated_uniform_layer_is_exactly_eps_times_identity():
+    eps = np.full((6, 5), 3.5 + 0.25j)
+    for m in orc.li_conv_matrices_integrated(eps, 2, 2)[:2]:
+        np.testing.assert_array_equal(m, (3.5 + 0.25j) * np.eye(25))

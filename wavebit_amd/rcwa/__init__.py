@@ -1,5 +1,6 @@
 """RCWA on one MI355X: the reference's ``wavebit.rcwa`` lifecycle and declarations over a
-HIP engine (raster layers, sampled Li coefficients, SMM, complex FP64)."""
+HIP engine (raster layers; Li's or Laurent's rule, sampled or integrated coefficients; SMM;
+complex FP64)."""
 from .problem import Excitation, Layer, Outputs, Problem, Region, Stack
 from .sources import PlaneWave
 from .numerics import Numerics

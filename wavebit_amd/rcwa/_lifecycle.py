@@ -158,6 +158,7 @@ class _Resolved:
     hv: int
     uniform: bool
     factorization: str = 'li'    # 'li' (the default) or 'laurent'
+    coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
 
 
 def _resolve(problem, numerics, execution):
@@ -171,8 +172,8 @@ def _resolve(problem, numerics, execution):
     if bad is not None:
         raise UnsupportedFeatureError(
             f'numerics.{bad[0]}={bad[1]!r} is not implemented by this engine (it implements the '
-            'raster defaults: SMM, sampled coefficients, Li rules with fixed nesting, and the '
-            "opt-in factorization='laurent').",
+            'raster defaults: SMM, Li rules with fixed nesting, sampled or integrated '
+            "coefficients, and the opt-in factorization='laurent').",
             code='rcwa.unsupported', path=f'numerics.{bad[0]}')
     factorization = numerics.factorization or 'li'
     if factorization == 'laurent' and numerics.li_nesting is not None:
@@ -269,7 +270,8 @@ def _resolve(problem, numerics, execution):
     ws = _wavelengths(problem)
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
-                     hu=hu, hv=hv, uniform=uniform, factorization=factorization)
+                     hu=hu, hv=hv, uniform=uniform, factorization=factorization,
+                     coefficients=numerics.fourier_coefficients or 'sampled')
 
 
 def _structure_key(r):
@@ -317,7 +319,7 @@ def _report(problem, numerics, execution, r):
         resolved=MappingProxyType({'order_window': ((0, r.hu), (0, r.hv)),
                                    'wavelength_count': int(r.wavelengths.size),
                                    'layer_count': len(problem.stack.layers),
-                                   'fourier_coefficients': 'sampled',
+                                   'fourier_coefficients': r.coefficients,
                                    'factorization': r.factorization,
                                    'li_nesting': 'fixed' if r.factorization == 'li' else None,
                                    'algorithm': 'smm',

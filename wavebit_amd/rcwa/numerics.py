@@ -2,8 +2,10 @@
 
 ``orders`` is the retained Fourier window ``(Mu, Mv)``. The engine implements the reference's
 defaults for raster layers -- SMM, sampled coefficients, Li's rules with fixed nesting, the
-second-order formulation -- and refuses other choices at admission with
-``UnsupportedFeatureError``.
+second-order formulation -- plus Laurent's rule (``factorization='laurent'``) and integrated
+coefficients (``fourier_coefficients='integrated'``: on a raster the closed-form segment
+integral of the same staircase, which the pixel series already evaluates exactly), and refuses
+other choices at admission with ``UnsupportedFeatureError``.
 """
 from dataclasses import dataclass
 
@@ -55,7 +57,7 @@ class Numerics:
 
     def unsupported(self):
         """The first non-default choice this engine does not implement, as (field, value)."""
-        for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled')),
+        for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled', 'integrated')),
                          ('factorization', (None, 'li', 'laurent')), ('formulation', ('auto', 'second_order')),
                          ('li_nesting', (None, 'fixed')), ('detuning', (None,))):
             if getattr(self, name) not in ok:
