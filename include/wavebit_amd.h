@@ -23,6 +23,11 @@
  *                    engine calls them (rcwa/_engine/engine.py:3283-3316, 3336-3434), for a
  *                    batch of plane-wave points instead of one wavelength per call
  *                    (rcwa/_lifecycle.py:712-727);
+ *   wb_solve_scattering
+ *                    the same solve with Outputs(scattering=True): the stack's ScatteringMatrix
+ *                    blocks as rcwa/_engine/scattering.py:607-656 (_scattering_matrix,
+ *                    _coupled_ports_1d) and rcwa/_engine/engine.py:2995-3009 (cell-centre signs)
+ *                    publish them, read by rcwa/results.py:136-141 (Result.scattering);
  *   wb_zgeev_batched _common/backend.py:1767-1812 eig (numpy.linalg.eig / torch.linalg.eig);
  *   wb_zgetrf_batched / wb_zgetrs_batched / wb_zgemm_batched
  *                    the dense inv / solve / matmul calls of rcwa/_engine/scattering.py:116-200
@@ -146,6 +151,19 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
 
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
  * is stored k x m, B n x k). */
+/* wb_solve plus the stack's scattering matrix per point: smat is P x 4 x n x n complex,
+ * n = 2N on the 2D path (s amplitudes over p port variables) and N on the 1D paths, blocks in
+ * the order (S11, S12, S21, S22), incidence-relative for top incidence: S11 reflects back into
+ * the top medium, S21 transmits top to bottom, S12 bottom to top, S22 reflects into the bottom
+ * medium. Every port amplitude is referenced at the cell centre; the p port variable is the p
+ * wave's magnetic field (S11 e = -r_p, S21 e = t_p n_bot / n_top on the specular column e).
+ * The workspace is wb_solve_workspace_size's. */
+int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                        const double* thick, const int* conv_idx, const double* epx,
+                        const double* epy, const double* epz_i, double* eff, double* amp, double* pe,
+                        int* flags, double* smat, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
 int wb_zgemm_batched_ops(wb_ctx* ctx, int opa, int opb, int m, int n, int k, const double* alpha,
                          const double* A, int lda, long long strideA, const double* B, int ldb,
                          long long strideB, const double* beta, double* C, int ldc,

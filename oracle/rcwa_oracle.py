@@ -314,6 +314,15 @@ def s_transmission(W_bot, V_bot, W_gap, V_gap):
     return B @ A_i, 0.5 * (A - B @ A_i @ B), 2 * A_i, -A_i @ B
 
 
+def published_scattering(S, sign, flip=(1.0, 1.0, 1.0, 1.0)):
+    """The stack's scattering matrix as the reference publishes it for top incidence: the kernel's
+    blocks (start port = scene bottom) flipped per block (``flip``: the 1D TM reflections,
+    ``scattering.py:627-656``), reversed into incidence-relative order (``:607-624``) and signed
+    per port amplitude by ``sign`` (frame sign x cell-centre sign, ``engine.py:2995-3009``)."""
+    blocks = [f * (sign[:, None] * b * sign[None, :]) for f, b in zip(flip, S)][::-1]
+    return dict(zip(('S11', 'S12', 'S21', 'S22'), blocks))
+
+
 def redheffer_star(I, S_a, S_b):
     """``scattering.py:191-200``."""
     a11, a12, a21, a22 = S_a
@@ -417,6 +426,10 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
         A, B = ab_matrices(W, V, I, V_gap)
         S = redheffer_star(I, S, s_layer(A, B, X))
     S = redheffer_star(I, S, S_trn)
+    scattering = published_scattering(S, sigma * _centre_parity(hu, 0).reshape(-1),
+                                      (-1.0, 1.0, 1.0, -1.0) if tetm == 1 else (1.0, 1.0, 1.0, 1.0))
+    scattering.update(kx=kx, ky=None, n_ref=np.sqrt(np.complex128(eps_top)),
+                      n_trn=np.sqrt(np.complex128(eps_bot)), channel='tm' if tetm == 1 else 'te')
     delta = np.zeros(N, dtype=np.complex128)
     delta[N // 2] = 1.0
     delta = delta * sigma[N // 2]
@@ -445,6 +458,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     out['T'] = out['T_s'] + out['T_p']
     # power entering the lit face (observations/power.py:37-48 net_power, summed / incoming)
     out['power_entering'] = float(np.sum(((delta + R) * np.conj(adm_ref * (delta - R))).real) / incoming)
+    out['scattering'] = scattering
     return out
 
 
@@ -498,6 +512,10 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
         if return_modes:
             modes.append((W, V, q))
     S = redheffer_star(bI, S, S_trn)
+    par = _centre_parity(hu, hv).reshape(-1)
+    scattering = published_scattering(S, np.concatenate([par, par]))
+    scattering.update(kx=kx, ky=ky, n_ref=np.sqrt(np.complex128(eps_top)),
+                      n_trn=np.sqrt(np.complex128(eps_bot)), channel='sp')
     # backside read: reflection S22, transmission S12
     kz_ref, kz_trn = kz_kb, kz_kt
     n_ref, n_trn = n_kb, n_kt
@@ -541,6 +559,7 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
         sub['power_entering'] = flux.sum() / den
         out[name] = sub
     out['kx'], out['ky'] = kx, ky
+    out['scattering'] = scattering
     if return_modes:
         out['modes'] = modes
     return out
@@ -566,9 +585,10 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
                          theta, hu, phi, factorization, coefficients)
             inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
-                   if k not in ('kx', 'power_entering')}
+                   if k not in ('kx', 'power_entering', 'scattering')}
             inc['power_entering'] = out['power_entering']
-            return {'path': '1d_te' if tetm == 0 else '1d_tm', 'incident': inc}
+            return {'path': '1d_te' if tetm == 0 else '1d_tm', 'incident': inc,
+                    'scattering': out['scattering']}
         pol = (1.0, 0.0) if polarization == 's' else (0.0, 1.0)
     else:
         pol = np.asarray(polarization, dtype=np.complex128)

@@ -1,0 +1,119 @@
+"""The stack's scattering matrix (``Outputs(scattering=True)``) on the GPU.
+
+* ``wb_solve_scattering`` blocks against the oracle's published blocks (every block, every
+  column) within 1e-10 of the block scale: 2D (Jones polarization, lossy layers, oblique
+  incidence), 1D TE and TM; its efficiencies and amplitudes equal to the plain solve's.
+* Through ``rcwa.run``: one ``ScatteringMatrix`` per wavelength with its ports; the specular
+  columns are the reported amplitudes; a stack cut in two and joined by ``cascade`` is the whole
+  stack's matrix; pieces whose ports differ are refused.
+* A differentiable run and the server refuse the output instead of dropping it.
+"""
+import numpy as np
+import pytest
+import torch
+
+import wavebit_amd
+from oracle import rcwa_oracle as orc
+from wavebit_amd import _native, rcwa
+from wavebit_amd.errors import UnsupportedFeatureError
+from wavebit_amd.rcwa import _engine as eng
+from wavebit_amd.rcwa import serve
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device('cuda:0')
+KEYS = ('S11', 'S12', 'S21', 'S22')
+
+
+def _device_blocks(eps, d, top, bot, period, wl, theta, phi, pol, M):
+    hu, hv = M
+    path = eng.route(eps.shape[1], pol, phi, hv)
+    b = eng.Bucket(hu=hu, hv=hv, path=path, n_layers=eps.shape[0])
+    b.points.append(eng.Point(stack=0, wavelength=wl, theta=theta, phi=phi, polarization=pol, eps_top=top,
+                              eps_bottom=bot, period=period, thickness=tuple(d)))
+    conv = [_native.assemble_li(torch.from_numpy(eps).to(DEV), hu, hv)[:3]]
+    eff, amp, pe, flags, N, smat = eng.run_bucket_scattering(b, conv, DEV)
+    assert int(flags.abs().sum()) == 0
+    eff0, amp0, pe0, flags0, _ = eng.run_bucket(b, conv, DEV)
+    v = 3 if path == eng.PATH_2D else 1          # the 1D paths fill incidence variant 0 only
+    assert torch.equal(eff[:, :v], eff0[:, :v]) and torch.equal(amp[:, :v], amp0[:, :v])
+    return dict(zip(KEYS, smat[0].cpu().numpy())), path
+
+
+CASES = [
+    # (L, Ny, Nx, orders, theta, phi, pol, top, bot)
+    (2, 6, 6, (2, 3), 0.3, 0.4, (0.4, 0.3 + 0.2j), 1.0, 2.25 + 0.1j),
+    (3, 12, 10, (3, 3), 0.5, 2.0, 'p', 1.44, 4.0),
+    (3, 16, 16, (6, 6), 0.2, 0.3, 's', 1.0, 2.25),
+    (2, 1, 8, (6, 0), 0.3, None, 's', 1.0, 2.25),
+    (2, 1, 8, (6, 0), 0.3, None, 'p', 1.0, 2.25),
+]
+
+
+@pytest.mark.parametrize('case', CASES)
+def test_scattering_blocks_match_the_oracle(case):
+    L, Ny, Nx, M, theta, phi, pol, top, bot = case
+    rng = np.random.default_rng(L * 7 + Nx)
+    eps = (rng.uniform(1, 9, (L, Ny, Nx)) + 1j * rng.uniform(0, 0.4, (L, Ny, Nx))).astype(np.complex128)
+    d = list(rng.uniform(0.1, 0.3, L))
+    got, path = _device_blocks(eps, d, top, bot, (0.8, 0.7), 0.9, theta, phi, pol, M)
+    want = orc.solve_raster(eps, d, top, bot, (0.8, 0.7), 0.9, theta, phi, pol, M)['scattering']
+    assert want['channel'] == {'2d': 'sp', '1d_te': 'te', '1d_tm': 'tm'}[path]
+    for k in KEYS:
+        scale = max(1.0, np.abs(want[k]).max())
+        assert np.abs(got[k] - want[k]).max() < 1e-10 * scale, (k, np.abs(got[k] - want[k]).max())
+
+
+def _problem(eps, d, top, bot, wls, theta=0.25, phi=0.35, pol=(0.6, 0.8j), scattering=True):
+    layers = [rcwa.Layer(thickness=d[i], epsilon=eps[i]) for i in range(eps.shape[0])]
+    return rcwa.Problem(
+        stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(top), bottom=wavebit_amd.Material(bot),
+                         period=(0.75, 0.7)),
+        excitation=rcwa.Excitation(wavelength=wls, sources=(rcwa.PlaneWave(polarization=pol, theta=theta,
+                                                                           phi=phi),)),
+        units=wavebit_amd.Units(length='um'), outputs=rcwa.Outputs(scattering=scattering))
+
+
+def test_run_keeps_the_scattering_matrix_and_it_cascades():
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(1.5, 9.0, (3, 10, 10)) + 0j
+    d = [0.12, 0.2, 0.15]
+    wls = [0.8, 0.93]
+    numerics = rcwa.Numerics(orders=(3, 3))
+    run = lambda e, t, top, bot: rcwa.run(_problem(e, t, top, bot, wls), numerics=numerics,   # noqa: E731
+                                          backend='torch', device='cuda:0')
+    whole = run(eps, d, 1.0, 2.25)
+    assert len(whole.scattering) == len(wls)
+    N = 49
+    for i, m in enumerate(whole.scattering):
+        assert m.entry_port.channel == 'sp' and m.S11.shape == (2 * N, 2 * N)
+        assert abs(m.entry_port.refractive_index - 1.0) < 1e-15 and abs(m.exit_port.refractive_index - 1.5) < 1e-15
+        # the top-lit specular s column is the reported s amplitudes of pure s incidence
+        s_inc = whole.diagnostics['incidence_variants']['s_inc']
+        assert np.abs(m.S11[:N, N // 2] - s_inc['r_s'][i].reshape(-1)).max() < 1e-12
+        assert np.abs(m.S21[:N, N // 2] - s_inc['t_s'][i].reshape(-1)).max() < 1e-12
+    upper = run(eps[2:], d[2:], 1.0, 1.0)
+    lower = run(eps[:2], d[:2], 1.0, 2.25)
+    for i in range(len(wls)):
+        joined = upper.scattering[i].cascade(lower.scattering[i])
+        for k in KEYS:
+            w = getattr(whole.scattering[i], k)
+            assert np.abs(getattr(joined, k) - w).max() < 1e-10 * max(1.0, np.abs(w).max()), k
+    with pytest.raises(ValueError, match='different half-spaces'):
+        lower.scattering[0].cascade(upper.scattering[0])
+    plain = rcwa.run(_problem(eps, d, 1.0, 2.25, wls, scattering=False), numerics=numerics,
+                     backend='torch', device='cuda:0')
+    with pytest.raises(UnsupportedFeatureError):
+        plain.scattering
+    assert np.array_equal(plain.R, whole.R)
+
+
+def test_gradient_runs_and_the_server_refuse_the_scattering_output():
+    eps = torch.full((1, 4, 4), 4.0, dtype=torch.float64, requires_grad=True)
+    problem = _problem(eps, [0.2], 1.0, 2.25, [0.8])
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.run(problem, numerics=rcwa.Numerics(orders=(1, 1)), backend='torch', device='cuda:0')
+    assert e.value.code == 'rcwa.gradient'
+    concrete = _problem(np.full((1, 4, 4), 4.0), [0.2], 1.0, 2.25, [0.8])
+    with serve.Server(streams=1) as srv:
+        results, _ = srv.serve([serve.Request(concrete, rcwa.Numerics(orders=(1, 1)), 0.0, 0)], realtime=False)
+    assert isinstance(results[0], UnsupportedFeatureError)

@@ -302,3 +302,67 @@ def test_integrated_uniform_layer_is_exactly_eps_times_identity():
     eps = np.full((6, 5), 3.5 + 0.25j)
     for m in orc.li_conv_matrices_integrated(eps, 2, 2)[:2]:
         np.testing.assert_array_equal(m, (3.5 + 0.25j) * np.eye(25))
+
+
+# ------------------------------------------------------------- the stack's scattering matrix
+
+def _star(Sa, Sb):
+    return dict(zip(('S11', 'S12', 'S21', 'S22'),
+                    orc.redheffer_star(np.eye(Sa['S11'].shape[0]), [Sa[k] for k in ('S11', 'S12', 'S21', 'S22')],
+                                       [Sb[k] for k in ('S11', 'S12', 'S21', 'S22')])))
+
+
+@pytest.mark.parametrize('name', ['2d_normal', '2d_oblique', '2d_2layer', '2d_cfs'])
+def test_scattering_specular_columns_are_the_pinned_amplitudes(name):
+    """The published blocks read on the specular column are the amplitudes the legacy golden pins:
+    S11 e_s = r_s, S11 e_p = -r_p (p port variable), S21 e_s = t_s, S21 e_p = t_p n_top / n_bot
+    (reference scattering.py:524-536, in the stated units) -- the cross terms as well."""
+    c, eps, d, pol = legacy_case(name)
+    res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'], pol,
+                           c['M'], coefficients=c.get('coef', 'sampled'))
+    S = res['scattering']
+    N = S['S11'].shape[0] // 2
+    c0 = N // 2
+    n_ref, n_trn = np.sqrt(complex(c['top'])), np.sqrt(complex(c['bot']))
+    mi = -1j * n_ref
+    s, p = res['s_inc'], res['p_inc']
+    pairs = [(S['S11'][:N, c0], s['r_s']), (-S['S11'][N:, c0] / mi, s['r_p']),
+             (S['S21'][:N, c0], s['t_s']), (S['S21'][N:, c0] / (-1j * n_trn), s['t_p']),
+             (S['S11'][:N, N + c0] * mi, p['r_s']), (-S['S11'][N:, N + c0], p['r_p']),
+             (S['S21'][:N, N + c0] * mi, p['t_s']), (S['S21'][N:, N + c0] * n_ref / n_trn, p['t_p'])]
+    for got, want in pairs:
+        assert np.abs(got - np.asarray(want).reshape(-1)).max() < 1e-12
+
+
+@pytest.mark.parametrize('name', ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique'])
+def test_scattering_1d_specular_column_is_the_pinned_amplitude(name):
+    c, eps, d, pol = legacy_case(name)
+    res = orc.solve_raster(eps, d, c['top'], c['bot'], c['period'], c['wl'], c['theta'], c['phi'], pol, c['M'])
+    S, inc = res['scattering'], res['incident']
+    c0 = S['S11'].shape[0] // 2
+    n_ref, n_trn = np.sqrt(complex(c['top'])), np.sqrt(complex(c['bot']))
+    if c['pol'] == 's':
+        assert np.abs(S['S11'][:, c0] - inc['r_s'].reshape(-1)).max() < 1e-12
+        assert np.abs(S['S21'][:, c0] - inc['t_s'].reshape(-1)).max() < 1e-12
+    else:
+        assert np.abs(-S['S11'][:, c0] - inc['r_p'].reshape(-1)).max() < 1e-12
+        assert np.abs(S['S21'][:, c0] * n_ref / n_trn - inc['t_p'].reshape(-1)).max() < 1e-12
+
+
+@pytest.mark.parametrize('pol,M,phi,shape', [((0.6, 0.8j), (2, 2), 0.4, (3, 5, 6)), ('s', (5, 0), None, (3, 1, 8)),
+                                             ('p', (5, 0), None, (3, 1, 8))])
+def test_scattering_of_a_stack_is_the_cascade_of_its_pieces(pol, M, phi, shape):
+    """Cut a 3-layer stack after its top layer: the whole stack's matrix is the top piece's
+    followed by the bottom piece's, the joint half-space being the top medium (reference
+    ScatteringMatrix.cascade, scattering.py:550-604). Every block and column is compared."""
+    rng = np.random.default_rng(41)
+    eps = rng.uniform(1.5, 9.0, shape) + 1j * rng.uniform(0.0, 0.3, shape)
+    d = [0.12, 0.2, 0.15]               # bottom-to-top
+    top, bot, theta, wl, period = 1.0, 2.25, 0.25, 0.83, (0.7, 0.65)
+    whole = orc.solve_raster(eps, d, top, bot, period, wl, theta, phi, pol, M)['scattering']
+    upper = orc.solve_raster(eps[2:], d[2:], top, top, period, wl, theta, phi, pol, M)['scattering']
+    lower = orc.solve_raster(eps[:2], d[:2], top, bot, period, wl, theta, phi, pol, M)['scattering']
+    joined = _star(upper, lower)
+    scale = max(np.abs(whole[k]).max() for k in joined)
+    for k in joined:
+        assert np.abs(joined[k] - whole[k]).max() < 1e-12 * scale, k

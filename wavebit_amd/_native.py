@@ -29,7 +29,7 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_adjoint', 'wb_assemble_li_adjoint_workspace_size', 'wb_assemble_li_adjoint',
            'wb_assemble_laurent_workspace_size', 'wb_assemble_laurent',
            'wb_assemble_laurent_adjoint_workspace_size', 'wb_assemble_laurent_adjoint',
-           'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
+           'wb_solve_scattering', 'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
            'wb_gemm_tuning_clear')
@@ -102,6 +102,9 @@ def load_library():
             'wb_assemble_laurent_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int,
                                                     c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
                                                     c_vp]),
+            'wb_solve_scattering': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp,
+                                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                            c_vp, c_sz, c_vp]),
             'wb_timing_enable': (c_int, [c_int]),
             'wb_gemm_tuning_configs': (c_int, []),
             'wb_gemm_tuning_config_name': (ctypes.c_char_p, [c_int]),
@@ -323,6 +326,33 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
                                _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
                                _stream(device)), 'solve')
     return eff, amp, pe, flags
+
+
+def solve_scattering(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode):
+    """``solve`` plus the stack's scattering matrix; see ``wb_solve_scattering``.
+    Returns (eff, amp, pe, flags, smat) with smat (P, 4, n, n): S11, S12, S21, S22."""
+    device = epx.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    n = 2 * N if mode == MODE_2D else N
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    smat = torch.empty((P, 4, n, n), dtype=torch.complex128, device=device)
+    nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve_scattering(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
+                                          _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                          _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
+                                          _ptr(ws), nbytes, _stream(device)), 'solve_scattering')
+    return eff, amp, pe, flags, smat
 
 
 def zgemm_ops(A, B, opa=0, opb=0, C=None, alpha=1.0, beta=0.0):

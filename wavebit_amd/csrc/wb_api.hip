@@ -196,6 +196,27 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve");
 }
 
+int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                        const double* thick, const int* conv_idx, const double* epx,
+                        const double* epy, const double* epz_i, double* eff, double* amp, double* pe,
+                        int* flags, double* smat, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_solve_scattering: bad dimensions");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe || !smat)
+    return fail(ctx, WB_BADARG, "wb_solve_scattering: null buffer");
+  if (workspace_bytes < wb_solve_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_scattering: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_scattering");
+}
+
 int wb_zgemm_batched_ops(wb_ctx* ctx, int opa, int opb, int m, int n, int k, const double* alpha,
                          const double* A, int lda, long long strideA, const double* B, int ldb,
                          long long strideB, const double* beta, double* C, int ldc,

@@ -21,6 +21,14 @@
 //                 transmitted wave is swept back through the stored LU factors as vectors: the
 //                 same S22/S12 columns the star cascade gives (scattering.py:191-200).
 //   efficiencies  scattering.py:907-994, solve_1d.py:1588-1610, observations/power.py.
+//   scattering    (when asked) the stack's four blocks between the two half-spaces, every order:
+//                 the top-lit pair from the same recursion with an identity right-hand side,
+//                 the bottom-lit pair from a second recursion started at the top port (the
+//                 gap-embedded layer S-matrices are symmetric, so the stack read from the other
+//                 end is the same algebra with the ports' blocks swapped); then published as
+//                 scattering.py:607-624 (_scattering_matrix: incidence-relative block order),
+//                 :627-656 (_coupled_ports_1d: D S D, TM reflections negated) and the cell-centre
+//                 sign C S C of rcwa/_engine/engine.py:2995-3009.
 #include "wb_rcwa.h"
 #include "wb_timing.h"
 
@@ -319,6 +327,51 @@ __global__ void k_i_minus(int n, BatchPtr T, int ldt, const cd* dg, int dg_block
   cd v = *x;
   if (dg) v = cmul(dg[b * DG_COUNT * n + dg_block * n + r], v);
   *x = csub(r == c ? mkc(1.0, 0.0) : mkc(0.0, 0.0), v);
+}
+
+// out (per point, n x n) = diag(dg[block])
+__global__ void k_diag_full(int n, const cd* dg, int block, cd* out) {
+  const long long p = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  out[p * (long long)n * n + e] = (r == c) ? dg[p * DG_COUNT * n + block * n + r] : mkc(0.0, 0.0);
+}
+
+// dst = diag(dg[left]) src (+ diag(dg[diag]) when diag >= 0); per point n x n, dst row stride n.
+__global__ void k_diag_combine(int n, const cd* dg, int left, int diag, const cd* src, BatchPtr dst) {
+  const long long p = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const cd* d = dg + p * DG_COUNT * n;
+  cd v = cmul(d[left * n + r], src[p * (long long)n * n + e]);
+  if (diag >= 0 && r == c) v = cadd(v, d[diag * n + r]);
+  dst.ptr(p)[e] = v;
+}
+
+// Published scattering blocks: entry (i, j) of every block times s_i s_j, s_i the cell-centre
+// sign (-1)^(m+n) of port amplitude i's order (and, on the 1D paths, its frame sign); the two
+// reflection blocks of the 1D TM path negated besides. smat: P x 4 x n x n.
+__global__ void k_publish_smat(int hu, int hv, int mode, const cd* ov, cd* smat) {
+  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv;
+  const int n = (mode == 0) ? 2 * N : N;
+  const long long p = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 4LL * n * n) return;
+  const int blk = (int)(e / ((long long)n * n));
+  const long long rc = e % ((long long)n * n);
+  const int r = (int)(rc / n) % N, c = (int)(rc % n) % N;
+  const cd* o = ov + p * OV_COUNT * N;
+  auto sign = [&](int i) {
+    double s = (((i % nu - hu) + (i / nu - hv)) & 1) ? -1.0 : 1.0;
+    if (mode != 0 && o[OV_KX * N + i].x < 0.0) s = -s;
+    return s;
+  };
+  double s = sign(r) * sign(c);
+  if (mode == 2 && (blk == 0 || blk == 3)) s = -s;
+  cd* x = smat + p * 4LL * n * n + e;
+  *x = cscale(*x, s);
 }
 
 // Right-hand side for the final interface: rhs = diag(t12) e_j for the basis incidences.
@@ -748,6 +801,12 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   }
 
   // ---------------------------------------------------------------- cascade
+  // the bottom-lit recursion of the scattering output needs every layer's S11, which the
+  // cascade below overwrites: keep a copy (M3 is free once the layer S-matrices are formed)
+  if (a.smat) {
+    hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M6, n2), 2 * n, S(M[3], nn), n);
+    WB_CHECK_LAUNCH();
+  }
   // pair (p, l) lives at index p * L + l; per-point views step by L pairs
   hipLaunchKernelGGL(k_gamma0, grid2(nn, P), dim3(256), 0, st, n, dg, G0);
   WB_CHECK_LAUNCH();
@@ -832,5 +891,91 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_efficiency, dim3(P), dim3(256), 0, st, a.hu, a.hv, mode, a.pts, ov, vec[2],
                      vec[3], a.eff, a.amp, a.pe);
   WB_CHECK_LAUNCH();
+  if (!a.smat) return hipSuccess;
+
+  // ---------------------------------------------------------------- scattering output
+  // kernel frame: start port = scene bottom. Published (incidence-relative, top lit):
+  //   S11 = kernel S22 = t22 + t21 G_L U,  U = (I - t11 G_L)^-1 t12   (block 0)
+  //   S21 = kernel S12 = r12 d_0, d_L = U, d_l = (I - s11_l G_l)^-1 s12_l d_{l+1}   (block 2)
+  // and from the recursion started at the top port (ports' blocks swapped):
+  //   S22 = kernel S11 = r11 + r12 G'_0 U',  U' = (I - r22 G'_0)^-1 r21   (block 3)
+  //   S12 = kernel S21 = t21 d'_{L}   (block 1)
+  {
+    BatchPtr blk[4];
+    for (int b = 0; b < 4; ++b) blk[b] = bp_strided(a.smat + (long long)b * nn, 4 * nn);
+    BatchPtr GamL = bp_strided(M6 + (long long)(L - 1) * n2, (long long)L * n2);
+    // top-lit pair
+    hipLaunchKernelGGL(k_diag_full, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_T12, M[1]);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_getrs_blocked(n, n, S(G0, nn), n, piv_f, S(M[1], nn), n, P, 0, W_.luws, st))) return e;
+    if ((e = wb_zgemm_launch(n, n, n, one, GamL, 2 * n, S(M[1], nn), n, zero, S(M[4], nn), n, P, st))) return e;
+    hipLaunchKernelGGL(k_diag_combine, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_T21, (int)DG_T22,
+                       M[4], blk[0]);
+    WB_CHECK_LAUNCH();
+    cd* dc = M[1];
+    cd* dn = M[2];
+    for (int l = L - 1; l >= 0; --l) {
+      BatchPtr s12 = bp_offset(bp_strided(M6 + (long long)l * n2, (long long)L * n2), n);
+      BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);
+      int* piv_pl = ipiv[2] + (long long)l * P * n;
+      if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, S(dc, nn), n, zero, S(dn, nn), n, P, st))) return e;
+      if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, S(dn, nn), n, P, 0, W_.luws, st))) return e;
+      cd* t = dc; dc = dn; dn = t;
+    }
+    hipLaunchKernelGGL(k_diag_combine, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_R12, -1, dc, blk[2]);
+    WB_CHECK_LAUNCH();
+
+    // bottom-lit pair: recursion from the top port down, layers top to bottom
+    hipLaunchKernelGGL(k_diag_full, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_T11, G0);
+    WB_CHECK_LAUNCH();
+    BatchPtr Gp = bp_strided(G0, nn);
+    for (int l = L - 1; l >= 0; --l) {
+      BatchPtr s11c = bp_strided(M[3] + (long long)l * nn, (long long)L * nn);
+      BatchPtr s12 = bp_offset(bp_strided(M6 + (long long)l * n2, (long long)L * n2), n);
+      BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);
+      BatchPtr Xs = bp_strided(M[1] + (long long)l * nn, (long long)L * nn);
+      BatchPtr G2 = bp_strided(M[2] + (long long)l * nn, (long long)L * nn);
+      int* piv_pl = ipiv[2] + (long long)l * P * n;
+      if ((e = wb_zgemm_launch(n, n, n, one, s11c, n, Gp, n, zero, Tm, n, P, st))) return e;
+      hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tm, n, nullptr, 0);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_getrf_launch(n, Tm, n, piv_pl, info[3], P, st))) return e;
+      FLAG(B, L, info[3], WB_FLAG_SINGULAR);
+      hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, s12, 2 * n, Xs, n);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, Xs, n, P, 0, W_.luws, st))) return e;
+      if ((e = wb_zgemm_launch(n, n, n, one, Gp, n, Xs, n, zero, G2, n, P, st))) return e;
+      if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, G2, n, one, s11c, n, P, st))) return e;
+      Gp = s11c;
+    }
+    // U' = (I - r22 G')^-1 r21 ; S22 = r11 + r12 G' U'
+    hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, Gp, n, S(G0, nn), n);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, S(G0, nn), n, dg, (int)DG_R22);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_getrf_launch(n, S(G0, nn), n, piv_f, info[3], P, st))) return e;
+    FLAG(B, L, info[3], WB_FLAG_SINGULAR);
+    hipLaunchKernelGGL(k_diag_full, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_R21, M[4]);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_getrs_blocked(n, n, S(G0, nn), n, piv_f, S(M[4], nn), n, P, 0, W_.luws, st))) return e;
+    if ((e = wb_zgemm_launch(n, n, n, one, Gp, n, S(M[4], nn), n, zero, S(M[1], nn), n, P, st))) return e;
+    hipLaunchKernelGGL(k_diag_combine, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_R12, (int)DG_R11,
+                       M[1], blk[3]);
+    WB_CHECK_LAUNCH();
+    dc = M[4];
+    dn = M[5];
+    for (int l = 0; l < L; ++l) {
+      BatchPtr s12 = bp_offset(bp_strided(M6 + (long long)l * n2, (long long)L * n2), n);
+      BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);
+      int* piv_pl = ipiv[2] + (long long)l * P * n;
+      if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, S(dc, nn), n, zero, S(dn, nn), n, P, st))) return e;
+      if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, S(dn, nn), n, P, 0, W_.luws, st))) return e;
+      cd* t = dc; dc = dn; dn = t;
+    }
+    hipLaunchKernelGGL(k_diag_combine, grid2(nn, P), dim3(256), 0, st, n, dg, (int)DG_T21, -1, dc, blk[1]);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_publish_smat, grid2(4 * nn, P), dim3(256), 0, st, a.hu, a.hv, mode, ov, a.smat);
+    WB_CHECK_LAUNCH();
+  }
   return hipSuccess;
 }

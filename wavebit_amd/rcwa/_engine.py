@@ -178,6 +178,23 @@ def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
     return eff, amp, pe, flags, N
 
 
+def run_bucket_scattering(bucket, conv, device):
+    """``run_bucket`` that also returns the stack's published scattering blocks per point:
+    (eff, amp, pe, flags, N, smat) with smat (P, 4, n, n) on the device."""
+    P, L = len(bucket.points), bucket.n_layers
+    N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
+    epx = torch.cat([c[0] for c in conv]).contiguous()
+    epy = torch.cat([c[1] for c in conv]).contiguous()
+    epz = torch.cat([c[2] for c in conv]).contiguous()
+    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
+    eff, amp, pe, flags, smat = _native.solve_scattering(
+        torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device),
+        torch.from_numpy(idx).to(device), epx, epy, epz, bucket.hu, bucket.hv, _MODE[bucket.path])
+    return eff, amp, pe, flags, N, smat
+
+
 def unpack(eff, amp, pe, p, hu, hv, path):
     """One point's outputs as ``{'incident', 's_inc', 'p_inc'}`` dicts of order-grid arrays."""
     nv, nu = 2 * hv + 1, 2 * hu + 1

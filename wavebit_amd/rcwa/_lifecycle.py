@@ -159,6 +159,7 @@ class _Resolved:
     uniform: bool
     factorization: str = 'li'    # 'li' (the default) or 'laurent'
     coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
+    scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
 
 
 def _resolve(problem, numerics, execution):
@@ -186,9 +187,6 @@ def _resolve(problem, numerics, execution):
     if problem.outputs.fields:
         raise UnsupportedFeatureError('Field reconstruction is not implemented by this engine.',
                                       code='rcwa.unsupported', path='outputs.fields')
-    if problem.outputs.scattering:
-        raise UnsupportedFeatureError('Scattering-matrix output is not implemented by this engine.',
-                                      code='rcwa.unsupported', path='outputs.scattering')
     if not problem.outputs.diffraction:
         raise UnsupportedFeatureError('Outputs(diffraction=False) leaves nothing this engine computes.',
                                       code='rcwa.unsupported', path='outputs.diffraction')
@@ -271,7 +269,8 @@ def _resolve(problem, numerics, execution):
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
                      hu=hu, hv=hv, uniform=uniform, factorization=factorization,
-                     coefficients=numerics.fourier_coefficients or 'sampled')
+                     coefficients=numerics.fourier_coefficients or 'sampled',
+                     scattering=bool(problem.outputs.scattering))
 
 
 def _structure_key(r):
@@ -401,7 +400,7 @@ class Prepared:
         if _live(problem):
             return _run_live(self, problem, r, checks)
         try:
-            per_wl, flags = _solve_resolved(r, self._cache)
+            per_wl, flags, smats = _solve_resolved(r, self._cache)
         except FloatingPointError as error:
             raise NumericalError(str(error), code='rcwa.singular', path='problem') from error
         except ValueError as error:
@@ -417,7 +416,7 @@ class Prepared:
         diag = {'report': self.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
                 'effective_wavelengths': tuple(float(w) for w in r.wavelengths),
                 'effective_theta': tuple(r.source.theta for _ in r.wavelengths)}
-        result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov)
+        result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats)
         if not result.validity.checks['numerical_finite']:
             raise NumericalError('The requested solve produced nonfinite observables; no implicit '
                                  'detuning was applied.', code='rcwa.nonfinite', path='result')
@@ -434,6 +433,11 @@ def _run_live(prepared, problem, r, checks):
         raise UnsupportedFeatureError(
             'Gradients are implemented for raster stacks on the 2D coupled path; this problem '
             f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
+    if r.scattering:
+        raise UnsupportedFeatureError(
+            'Gradients are implemented for the diffraction outputs; a differentiable run cannot '
+            'also keep the scattering matrix. Solve it with concrete values.',
+            code='rcwa.gradient', path='outputs.scattering')
     device = torch.device(prepared._cache.conv.device)
     _engine._native.context(device)
     layers = problem.stack.layers
@@ -490,10 +494,35 @@ def _solve_resolved(r, cache):
             stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
             period=r.period, thickness=tuple(float(t) for t in r.thickness)))
-    eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
+    smats = None
+    if r.scattering:
+        eff, amp, pe, flags, _, smat = _engine.run_bucket_scattering(bucket, [conv], device)
+        smats = _scattering_matrices(r, smat.cpu().numpy())
+    else:
+        eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
     eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
     per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
-    return per_wl, flags
+    return per_wl, flags, smats
+
+
+def _scattering_matrices(r, smat):
+    """One ``ScatteringMatrix`` per wavelength from the device blocks (P, 4, n, n), with the ports
+    the reference attaches (``scattering.py:607-624``): the retained orders' kx/k0 (and ky/k0 on
+    the coupled path), the top medium's index at the entry, the bottom medium's at the exit."""
+    from .scattering import Port, ScatteringMatrix
+    channel = {_engine.PATH_2D: 'sp', _engine.PATH_TE: 'te', _engine.PATH_TM: 'tm'}[r.path]
+    kx0, ky0 = _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
+    n_top, n_bot = np.sqrt(complex(r.eps_top)), np.sqrt(complex(r.eps_bottom))
+    out = []
+    for p, wl in enumerate(r.wavelengths):
+        kx = kx0 + np.arange(-r.hu, r.hu + 1) * (float(wl) / r.period[0])
+        ky = (ky0 + np.arange(-r.hv, r.hv + 1) * (float(wl) / r.period[1])
+              if channel == 'sp' else None)
+        S = smat[p]
+        out.append(ScatteringMatrix(S[0].copy(), S[1].copy(), S[2].copy(), S[3].copy(),
+                                    entry_port=Port(kx, ky, n_top, 1.0, channel),
+                                    exit_port=Port(kx, ky, n_bot, 1.0, channel)))
+    return out
 
 
 def prepare(problem, *, numerics, backend='numpy', device='cpu', precision='double'):

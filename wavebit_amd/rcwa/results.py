@@ -43,6 +43,7 @@ class Result:
     validity: object
     diagnostics: object
     provenance: object
+    _scattering: object = None       # tuple of ScatteringMatrix, one per wavelength
 
     def __post_init__(self):
         for name in ('outputs', 'diagnostics', 'provenance'):
@@ -76,15 +77,18 @@ class Result:
 
     @property
     def scattering(self):
-        raise UnsupportedFeatureError('Scattering-matrix output is not implemented by this engine.',
-                                      code='outputs.unavailable', path='outputs.scattering')
+        """The stack's ``ScatteringMatrix`` per wavelength (reference ``results.py:136-141``)."""
+        if self._scattering is None:
+            raise UnsupportedFeatureError('Request Outputs(scattering=True) with the SMM algorithm before running.',
+                                          code='outputs.unavailable', path='outputs.scattering')
+        return self._scattering
 
 
-def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance):
+def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, scattering=None):
     """Stack per-wavelength solve outputs (dicts from the engine) into a ``Result``.
 
     NaN-masks every masked output when a validity check failed (reference
-    ``results.py:441-464`` ``admission_scale``).
+    ``results.py:441-464`` ``admission_scale``), the scattering blocks included.
     """
     orders_u = np.arange(-hu, hu + 1)
     orders_v = np.arange(-hv, hv + 1)
@@ -116,7 +120,10 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance):
                            for k in ('R', 'T', 'r_s', 'r_p', 't_s', 't_p')}
     diagnostics = dict(diagnostics)
     diagnostics['incidence_variants'] = variants
-    return Result(outputs=outputs, wavelengths=ws,
+    if scattering is not None:
+        scattering = tuple(m._replace(S11=m.S11 * scale, S12=m.S12 * scale, S21=m.S21 * scale,
+                                      S22=m.S22 * scale) for m in scattering)
+    return Result(outputs=outputs, wavelengths=ws, _scattering=scattering,
                   validity=Validity(checks={**checks, 'numerical_finite': combined},
                                     convergence={'status': 'not assessed',
                                                  'requires': 'independent refinement study'}),

@@ -126,6 +126,10 @@ class Server:
     # ------------------------------------------------------------------ request intake
     def _admit(self, req, now):
         r = _resolve(req.problem, req.numerics, self.execution)
+        if r.scattering:
+            raise UnsupportedFeatureError(
+                'The server returns diffraction outputs; solve a scattering-matrix request with '
+                'rcwa.run.', code='rcwa.unsupported', path='outputs.scattering')
         checks = _runtime_checks(r)
         bad = [k for k, ok in checks.items() if not ok]
         if bad:
