@@ -185,7 +185,7 @@ def main():
         if cnt:
             kernels[n_] = dict(total_ms=ms, launches=cnt, flops=fl)
     eff, amp, pe, flags = out
-    n_flagged = int((flags != 0).sum())
+    n_flagged = int(((flags & 3) != 0).sum())          # failed eigensolves or factorizations
     batched_rate = P * args.steps / dt
 
     # per-stage rates from one more (untimed-for-value) step with whole stages bracketed by HIP
@@ -313,7 +313,7 @@ def main():
                 'gradient_solves_per_s': round(Pg / (t2 - t0), 4),
                 'backward_over_forward': round((t2 - t1) / (t1 - t0), 3),
                 'grad_norm_eps': float(eps_g.grad.norm()), 'grad_thickness': thick_g.grad.tolist(),
-                'flagged_points': int((flags_g != 0).sum()),
+                'flagged_points': int(((flags_g & 3) != 0).sum()),
                 'objective': 'sum of T over the points; d/d every pixel of 4 x 128 x 128 and the '
                              '4 thicknesses'}
         del eff_g, eps_g, thick_g
@@ -437,7 +437,8 @@ def main():
                                'deg, phi U[0,360) deg, s/p, one batched launch sequence',
                    'points': len(w2['points']), 's': round(dt2, 4),
                    'solves_per_s': round(len(w2['points']) / dt2, 3),
-                   'max_abs_R_plus_T_minus_1': err2, 'flagged_points': int((f2 != 0).sum())}
+                   'max_abs_R_plus_T_minus_1': err2, 'flagged_points': int(((f2 & 3) != 0).sum()),
+                   'grazing_or_defective_layer_solves': int(((f2 & 12) != 0).sum())}
 
     # the uneven batch: problems of different order windows and layer counts served at once,
     # size-bucketed by (orders, layers), one launch sequence per bucket on three streams

@@ -84,7 +84,12 @@ enum {
 };
 
 /* flag bits written per (point, layer) by wb_solve */
-enum { WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2 };
+/* WB_FLAG_EP: a 2D layer's eigensolve returned a near-defective mode pair (two modes that are
+ * each other's nearest eigenvalue with unit eigenvectors at a sine below 1e-2, roots above
+ * 1e-9; rebuilt by the reference under exceptional_points='rebuild', rcwa/_engine/
+ * solve_2d.py:583-660). WB_FLAG_GRAZING: a 2D layer has a grazing mode, |w| below 1e-9 of the
+ * largest (always rebuilt by the reference, :2463-2485). This engine reports both instead. */
+enum { WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2, WB_FLAG_EP = 4, WB_FLAG_GRAZING = 8 };
 
 int wb_version(void);
 int wb_ctx_create(int device, wb_ctx** out);

@@ -183,3 +183,21 @@ def test_incident_kxy_follows_the_reference_rules():
     # grazing is held just below pi/2 (single-precision bound)
     kx, _ = eng.incident_kxy(1.0, float(np.float32(np.pi / 2)), 0.0)
     assert 0.999999 < kx < 1.0
+
+
+@pytest.mark.parametrize('bits,ep,error,notice', [
+    (0, 'rebuild', None, False), (1, 'rebuild', 'rcwa.singular', False), (2, 'off', 'rcwa.singular', False),
+    (8, 'off', None, True), (8, 'rebuild', None, True), (4, 'rebuild', None, True), (4, 'off', None, False)])
+def test_solve_flags_map_to_errors_and_notices(bits, ep, error, notice):
+    """Failed eigensolves / factorizations raise; grazing modes (always rebuilt by the reference)
+    and near-defective pairs (rebuilt under exceptional_points='rebuild') are kept, counted and
+    announced instead."""
+    from wavebit_amd.rcwa._lifecycle import flag_error, flag_notice
+    flags = np.zeros((3, 2), dtype=np.int32)
+    flags[1, 1] = bits
+    numerics = rcwa.Numerics(orders=(1, 1), exceptional_points=ep)
+    got = flag_error(flags, numerics)
+    assert (got is None) if error is None else (got.code == error)
+    msg, counts = flag_notice(flags, numerics)
+    assert (msg is not None) == notice
+    assert counts == {'grazing_layer_solves': int(bits == 8), 'defective_pair_layer_solves': int(bits == 4)}

@@ -79,7 +79,7 @@ def test_solve_adjoint_matches_central_differences_of_the_forward():
     rng = np.random.default_rng(7)
     eb = rng.normal(size=(P, 3, 4, N))
     eff, amp, pe, flags, state = _native.solve_keep(rec, th, idx, *conv, hu, hv)
-    assert int(flags.abs().sum()) == 0
+    assert int((flags & 3).sum()) == 0          # no failed eigensolve or factorization
     d_conv = _native.solve_adjoint(state, torch.from_numpy(eb).to(DEV))
     d = [t.cpu().numpy() for t in d_conv[:3]]
     d_thick = d_conv[3].cpu().numpy()
@@ -230,7 +230,7 @@ def _fd_totals(eps, thick, points, hu, hv, perturb):
         th = torch.tensor(np.array(rows_th), dtype=torch.float64, device=DEV)
         idx = torch.tensor(rows_idx, dtype=torch.int32, device=DEV)
         e_fd, _, _, fl = _native.solve(pts, th, idx, epx, epy, epz, hu, hv, _native.MODE_2D)
-        assert int(fl.abs().sum()) == 0
+        assert int((fl & 3).sum()) == 0
         tot = (e_fd[:, 0, 2] + e_fd[:, 0, 3]).sum(dim=1).cpu().numpy().reshape(-1, K).sum(axis=1)
         out.extend(tot.tolist())
         del epx, epy, epz, e_fd
@@ -270,7 +270,7 @@ def test_continuous_stack_orders_9_gradient_matches_the_oracle_and_differences()
     eps_t = torch.tensor(eps, requires_grad=True)
     thick_t = torch.tensor(thick, requires_grad=True)
     eff, amp, pe, flags = adjoint.solve(eps_t, thick_t, points, hu, hv, DEV)
-    assert int(flags.abs().sum()) == 0
+    assert int((flags & 3).sum()) == 0          # no failed eigensolve or factorization
     (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
     g_eps = eps_t.grad.numpy()          # complex leaves: d/dRe + i d/dIm
     g_t = thick_t.grad.numpy()

@@ -35,7 +35,7 @@ def test_crossed_grating_512_points_conserve_energy_and_match_the_oracle():
                                   thickness=tuple(w['thickness'])))
     conv = [_native.assemble_li(torch.from_numpy(w['rasters']).to(DEV), hu, hv)[:3]]
     eff, amp, pe, flags, N = eng.run_bucket(b, conv, DEV)
-    assert int(flags.abs().sum()) == 0
+    assert int((flags & 3).sum()) == 0          # no failed eigensolve or factorization
     eff_h, amp_h, pe_h = eff.cpu().numpy(), amp.cpu().numpy(), pe.cpu().numpy()
     worst = 0.0
     for i in range(len(w['points'])):

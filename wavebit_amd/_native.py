@@ -18,7 +18,7 @@ _ctxs = {}
 
 WB_OK, WB_BADARG, WB_OOM, WB_SINGULAR, WB_NONFINITE, WB_NOCONVERGE, WB_FLAGGED, WB_HIP = range(8)
 MODE_2D, MODE_1D_TE, MODE_1D_TM = 0, 1, 2
-FLAG_NOCONVERGE, FLAG_SINGULAR = 1, 2
+FLAG_NOCONVERGE, FLAG_SINGULAR, FLAG_EP, FLAG_GRAZING = 1, 2, 4, 8
 PT_STRIDE = 16
 
 EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb_zgemm_batched',

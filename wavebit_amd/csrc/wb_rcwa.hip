@@ -374,6 +374,66 @@ __global__ void k_publish_smat(int hu, int hv, int mode, const cd* ov, cd* smat)
   *x = cscale(*x, s);
 }
 
+// Exceptional-point screen of one 2D layer eigensolve per workgroup (reference
+// solve_2d.py:583-627 _defective_pairs, :2463-2465 the grazing gate): mode i and its nearest
+// eigenvalue nb(i) form a pair when each is the other's nearest, their unit eigenvectors (columns
+// of W) meet at a sine below 1e-2 and both roots exceed 1e-9 (|w| > 1e-18); a grazing layer has
+// min |w| < 1e-9 max |w|. They set WB_FLAG_EP / WB_FLAG_GRAZING on the pair (b = p * L + l).
+constexpr int EP_NMAX = 2048;   // every 2D order window up to (15, 15): n <= 1922
+__global__ __launch_bounds__(256) void k_ep_screen(int n, const cd* w, const cd* W, int* flags) {
+  __shared__ cd sw[EP_NMAX];
+  __shared__ int snb[EP_NMAX];
+  __shared__ double red_min[256], red_max[256];
+  __shared__ int hit, grazing;
+  const long long b = blockIdx.x;
+  const cd* wb = w + b * n;
+  const cd* Wb = W + b * (long long)n * n;
+  const int t = threadIdx.x;
+  if (t == 0) { hit = 0; grazing = 0; }
+  double mn = 1e300, mx = 0.0;
+  for (int i = t; i < n; i += 256) {
+    sw[i] = wb[i];
+    const double a = sqrt(cnorm2(wb[i]));
+    mn = fmin(mn, a);
+    mx = fmax(mx, a);
+  }
+  red_min[t] = mn;
+  red_max[t] = mx;
+  __syncthreads();
+  for (int i = t; i < n; i += 256) {
+    double best = 1e300;
+    int arg = i;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double d = cnorm2(csub(sw[i], sw[j]));
+      if (d < best) { best = d; arg = j; }
+    }
+    snb[i] = arg;
+  }
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red_min[t] = fmin(red_min[t], red_min[t + s]);
+      red_max[t] = fmax(red_max[t], red_max[t + s]);
+    }
+    __syncthreads();
+  }
+  if (t == 0 && red_min[0] < 1e-9 * red_max[0]) grazing = 1;
+  for (int i = t; i < n; i += 256) {
+    const int j = snb[i];
+    if (j <= i || snb[j] != i) continue;
+    if (cnorm2(sw[i]) <= 1e-36 || cnorm2(sw[j]) <= 1e-36) continue;
+    double re = 0.0, im = 0.0;
+    for (int r = 0; r < n; ++r) {
+      const cd a = Wb[(long long)r * n + i], c = Wb[(long long)r * n + j];
+      re += a.x * c.x + a.y * c.y;   // conj(a) c
+      im += a.x * c.y - a.y * c.x;
+    }
+    if (1.0 - (re * re + im * im) < 1e-4) hit = 1;
+  }
+  __syncthreads();
+  if (t == 0) flags[b] |= (hit ? WB_FLAG_EP : 0) | (grazing ? WB_FLAG_GRAZING : 0);
+}
+
 // Right-hand side for the final interface: rhs = diag(t12) e_j for the basis incidences.
 // 2D: two columns (s and p unit at the specular order); 1D: one column (frame sign applied).
 __global__ void k_rhs_init(int N, int n, int nrhs, const cd* dg, const double* sigma_c, cd* rhs) {
@@ -697,6 +757,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                              S(M[0] + N, nn), n, B, st))) return e;
     if ((e = eig())) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
+    if (a.flags && n <= EP_NMAX) {
+      hipLaunchKernelGGL(k_ep_screen, dim3(B), dim3(256), 0, st, n, w, M[3], a.flags);
+      WB_CHECK_LAUNCH();
+    }
     if (keep) {
       hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.W, nn), n);
       WB_CHECK_LAUNCH();

@@ -405,17 +405,22 @@ class Prepared:
             raise NumericalError(str(error), code='rcwa.singular', path='problem') from error
         except ValueError as error:
             raise ValidationError(str(error), code='rcwa.invalid', path='problem') from error
-        if np.any(flags != 0):
-            raise NumericalError(
-                'A layer eigensolve did not converge or a scattering-matrix factorization was '
-                'singular.', code='rcwa.singular', path='problem')
+        error = flag_error(flags, self.numerics)
+        if error is not None:
+            raise error
+        notice, ep_counts = flag_notice(flags, self.numerics)
+        if notice is not None:
+            import warnings
+            from .diagnostics import ExceptionalPointWarning
+            warnings.warn(notice, ExceptionalPointWarning, stacklevel=2)
         prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950', 'length_units': problem.units.length,
                 'execution': self.execution, 'numerics': self.numerics,
                 'coordinate_frame': '+z up; bottom interface z=0; layers bottom-to-top',
                 'time_convention': 'exp(-i*omega*t)'}
         diag = {'report': self.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
                 'effective_wavelengths': tuple(float(w) for w in r.wavelengths),
-                'effective_theta': tuple(r.source.theta for _ in r.wavelengths)}
+                'effective_theta': tuple(r.source.theta for _ in r.wavelengths),
+                'exceptional_points': ep_counts}
         result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats)
         if not result.validity.checks['numerical_finite']:
             raise NumericalError('The requested solve produced nonfinite observables; no implicit '
@@ -454,9 +459,15 @@ def _run_live(prepared, problem, r, checks):
               for wl in r.wavelengths]
     eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device,
                                         factorization=r.factorization)
-    if bool((flags != 0).any()):
-        raise NumericalError('A layer eigensolve did not converge or a scattering-matrix '
-                             'factorization was singular.', code='rcwa.singular', path='problem')
+    fl_h = flags.cpu().numpy()
+    error = flag_error(fl_h, prepared.numerics)
+    if error is not None:
+        raise error
+    notice, _ = flag_notice(fl_h, prepared.numerics)
+    if notice is not None:
+        import warnings
+        from .diagnostics import ExceptionalPointWarning
+        warnings.warn(notice, ExceptionalPointWarning, stacklevel=3)
     eff_h, amp_h, pe_h = (t.detach().cpu().numpy() for t in (eff, amp, pe))
     per_wl = [_engine.unpack(eff_h, amp_h, pe_h, p, r.hu, r.hv, r.path) for p in range(len(points))]
     prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 (differentiable)',
@@ -480,6 +491,44 @@ def _run_live(prepared, problem, r, checks):
     outputs = dict(result.outputs)
     outputs.update(live)
     return replace(result, outputs=outputs)
+
+
+def flag_error(flags, numerics):
+    """The error a solve's per-pair flags (``WB_FLAG_*``) call for, or None: non-convergence and
+    singular factorizations are ``NumericalError('rcwa.singular')``."""
+    fl = _engine._native
+    if np.any(np.asarray(flags) & (fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR)):
+        return NumericalError('A layer eigensolve did not converge or a scattering-matrix factorization '
+                              'was singular.', code='rcwa.singular', path='problem')
+    return None
+
+
+def flag_notice(flags, numerics):
+    """``(message, counts)`` for the modes the reference would rebuild, or ``(None, counts)``.
+
+    A grazing mode (|w| below 1e-9 of the largest) the reference always re-derives from the
+    coupled first-order problem (reference ``solve_2d.py:2463-2485``); a near-defective pair it
+    rebuilds as a Schur pair under ``exceptional_points='rebuild'`` (``:583-660, 2491-2575``).
+    This engine keeps the modes as computed, counts the flagged layer solves in the result's
+    diagnostics and, for a ``Prepared.run``, warns (``ExceptionalPointWarning``).
+    ``exceptional_points='off'`` takes the pairs as computed, as the reference does.
+    """
+    fl = _engine._native
+    flags = np.asarray(flags)
+    counts = {'grazing_layer_solves': int(np.count_nonzero(flags & fl.FLAG_GRAZING)),
+              'defective_pair_layer_solves': int(np.count_nonzero(flags & fl.FLAG_EP))}
+    parts = []
+    if counts['grazing_layer_solves']:
+        parts.append(f"{counts['grazing_layer_solves']} layer solve(s) hold a grazing mode, which the "
+                     'reference re-derives from the coupled first-order problem')
+    if counts['defective_pair_layer_solves'] and numerics.exceptional_points == 'rebuild':
+        parts.append(f"{counts['defective_pair_layer_solves']} layer solve(s) hold a near-defective "
+                     "mode pair, which the reference rebuilds as a Schur pair under "
+                     "exceptional_points='rebuild'")
+    if not parts:
+        return None, counts
+    return ('; '.join(parts) + '. This engine keeps the modes as computed; results near such a '
+            'point can differ from the reference beyond round-off.'), counts
 
 
 def _solve_resolved(r, cache):

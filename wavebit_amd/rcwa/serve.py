@@ -35,7 +35,7 @@ import torch
 
 from .. import _native
 from . import _engine
-from ._lifecycle import Execution, _resolve, _runtime_checks
+from ._lifecycle import Execution, _resolve, _runtime_checks, flag_error, flag_notice
 from .results import pack
 from ..errors import NumericalError, UnsupportedFeatureError, ValidationError
 
@@ -58,6 +58,7 @@ class _Work:
     done: int = 0
     t_admit: float = 0.0
     t_done: float = 0.0
+    flags: int = 0               # WB_FLAG_* bits over the request's points and layers
 
 
 class ConvLRU:
@@ -183,8 +184,7 @@ class Server:
             d_eps, d_t = eps.grad.detach().cpu().numpy(), thick.grad.detach().cpu().numpy()
             eff_h, amp_h, pe_h, fl = (t.detach().cpu().numpy() for t in (eff, amp, pe, flags))
         for i in range(work.n_points):
-            if np.any(fl[i] != 0):
-                work.failed = True
+            work.flags |= int(np.bitwise_or.reduce(fl[i]))
             work.results[i] = _engine.unpack(eff_h, amp_h, pe_h, i, r.hu, r.hv, r.path)
         work.done = work.n_points
         return d_eps, d_t
@@ -224,8 +224,7 @@ class Server:
         eff, amp, pe, flags = (t.cpu().numpy() for t in launch['out'])
         hu, hv, path, L = launch['key']
         for p, (work, i, _, _) in enumerate(launch['items']):
-            if np.any(flags[p] != 0):
-                work.failed = True
+            work.flags |= int(np.bitwise_or.reduce(flags[p]))
             work.results[i] = _engine.unpack(eff, amp, pe, p, hu, hv, path)
             work.done += 1
             if work.done == work.n_points:
@@ -234,12 +233,13 @@ class Server:
 
     def _result(self, work):
         r = work.resolved
-        if getattr(work, 'failed', False):
-            return NumericalError('A layer eigensolve did not converge or a factorization was singular.',
-                                  code='rcwa.singular', path='problem')
+        error = flag_error(work.flags, work.req.numerics)
+        if error is not None:
+            return error
         per = [work.results[i] for i in range(work.n_points)]
         diag = {'solve_path': tuple(r.path for _ in r.wavelengths), 'served': True,
-                'latency_s': work.t_done - work.req.arrival}
+                'latency_s': work.t_done - work.req.arrival,
+                'exceptional_points': flag_notice(work.flags, work.req.numerics)[1]}
         prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 serve',
                 'length_units': work.req.problem.units.length}
         return pack(per, r.hu, r.hv, r.wavelengths, work.checks, diag, prov)
