@@ -87,3 +87,27 @@ def test_gradient_requests_match_the_differentiable_run_alone():
         ge, gt = eps.grad.numpy(), thick.grad.numpy()
         assert np.abs(g['d_epsilon'] - ge).max() <= 1e-9 * max(1e-12, np.abs(ge).max())
         assert np.abs(g['d_thickness'] - gt).max() <= 1e-9 * max(1e-12, np.abs(gt).max())
+
+
+def test_a_job_document_runs_and_serves_like_its_problem():
+    """A ``wavebit.job/v2`` document through ``jobs.run`` and through the server equals
+    ``rcwa.run`` of the problem it carries."""
+    import wavebit_amd
+    from wavebit_amd import jobs
+    from wavebit_amd.rcwa._lifecycle import Execution
+    rng = np.random.default_rng(3)
+    problem = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.2, epsilon=rng.uniform(1, 9, (16, 16)))],
+                         top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25), period=(0.8, 0.7)),
+        excitation=rcwa.Excitation(wavelength=[0.75, 0.9],
+                                   sources=(rcwa.PlaneWave(polarization='s', theta=0.2, phi=0.4),)),
+        units=wavebit_amd.Units(length='um'))
+    numerics = rcwa.Numerics(orders=(3, 3))
+    text = jobs.Job(solver='rcwa', problem=problem, numerics=numerics,
+                    execution=Execution(backend='torch', device='cuda:0')).to_json()
+    alone = rcwa.run(problem, numerics=numerics, backend='torch', device='cuda:0')
+    by_job = jobs.run(jobs.Job.from_json(text))
+    np.testing.assert_array_equal(np.asarray(by_job.R), np.asarray(alone.R))
+    with serve.Server(streams=1) as srv:
+        results, _ = srv.serve([serve.Request.from_job(text, arrival=0.0, id=0)], realtime=False)
+    assert np.abs(np.asarray(results[0].T) - np.asarray(alone.T)).max() < 1e-12

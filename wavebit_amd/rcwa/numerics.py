@@ -33,9 +33,20 @@ class Numerics:
     fourier_coefficients: object = None
     factorization: object = None
     formulation: str = 'auto'
+    raster_normal_method: object = None
+    nvf_projector: object = None
+    shape_nvf_grid_size: object = None
     li_nesting: object = None
     detuning: object = None
+    topology_check: str = 'device'
+    half_space_merge_tolerance: float = 0.0
     exceptional_points: str = 'rebuild'
+    wall_resolution: object = None
+    shape_normal_method: object = None
+    slicing: object = None
+    # written to a job only off their default (reference numerics.py:104-105)
+    _FIELDS_OMITTED_AT_DEFAULT = ('half_space_merge_tolerance', 'exceptional_points',
+                                  'wall_resolution', 'shape_normal_method', 'slicing')
 
     def __post_init__(self):
         object.__setattr__(self, 'orders', order_pair(self.orders))
@@ -43,7 +54,11 @@ class Numerics:
                    'fourier_coefficients': (None, 'sampled', 'integrated'),
                    'factorization': (None, 'li', 'nvf', 'laurent'),
                    'formulation': ('auto', 'first_order', 'second_order'),
+                   'raster_normal_method': (None, 'distance', 'contour', 'spectral', 'smooth'),
+                   'nvf_projector': (None, 'linear', 'circular'),
                    'li_nesting': (None, 'fixed', 'symmetric'),
+                   'shape_normal_method': (None, 'per_shape', 'interfaces'),
+                   'topology_check': ('device', 'host'),
                    'exceptional_points': ('rebuild', 'off')}
         for name, allowed in choices.items():
             if getattr(self, name) not in allowed:
@@ -59,7 +74,11 @@ class Numerics:
         """The first non-default choice this engine does not implement, as (field, value)."""
         for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled', 'integrated')),
                          ('factorization', (None, 'li', 'laurent')), ('formulation', ('auto', 'second_order')),
-                         ('li_nesting', (None, 'fixed')), ('detuning', (None,))):
+                         ('li_nesting', (None, 'fixed')), ('detuning', (None,)),
+                         ('raster_normal_method', (None,)), ('nvf_projector', (None,)),
+                         ('shape_nvf_grid_size', (None,)), ('half_space_merge_tolerance', (0.0,)),
+                         ('wall_resolution', (None,)), ('shape_normal_method', (None,)),
+                         ('slicing', (None,))):
             if getattr(self, name) not in ok:
                 return name, getattr(self, name)
         return None

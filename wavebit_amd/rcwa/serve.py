@@ -48,6 +48,15 @@ class Request:
     id: int = -1
     gradient: bool = False            # also d(sum of T)/d(rasters, thicknesses)
 
+    @classmethod
+    def from_job(cls, text, arrival=0.0, id=-1, gradient=False):
+        """A request from a ``wavebit.job/v2`` document (``wavebit_amd.jobs.Job``); the job's
+        execution must name this server's kind of device."""
+        from ..jobs import Job
+        job = text if isinstance(text, Job) else Job.from_json(text)
+        job.execution.validate()
+        return cls(problem=job.problem, numerics=job.numerics, arrival=arrival, id=id, gradient=gradient)
+
 
 @dataclass
 class _Work:
