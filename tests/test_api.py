@@ -201,3 +201,12 @@ def test_solve_flags_map_to_errors_and_notices(bits, ep, error, notice):
     msg, counts = flag_notice(flags, numerics)
     assert (msg is not None) == notice
     assert counts == {'grazing_layer_solves': int(bits == 8), 'defective_pair_layer_solves': int(bits == 4)}
+
+
+def test_detuning_is_accepted_and_validated():
+    rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(1, 1), detuning=1e-9), backend='torch',
+                     device='cuda')
+    assert rep.ok
+    with pytest.raises(ValidationError) as e:
+        rcwa.Numerics(orders=(1, 1), detuning=-1.0)
+    assert e.value.code == 'numerics.detuning'

@@ -139,3 +139,28 @@ def test_a_grazing_mode_is_flagged_counted_and_announced():
     res = rcwa.run(fine, numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda:0')
     assert res.diagnostics['exceptional_points'] == {'grazing_layer_solves': 0, 'defective_pair_layer_solves': 0}
     assert abs(float(np.sum(res.R) + np.sum(res.T)) - 1.0) < 1e-10
+
+
+def test_detuning_moves_a_grazing_point_off_the_rayleigh_point():
+    """Numerics(detuning=1e-6) re-solves the grazing point of the test above at
+    wavelength x (1 + 1e-6): the mode no longer grazes, the result reports the effective
+    wavelength, warns AutoDetuneWarning, and the other point is untouched."""
+    from wavebit_amd.rcwa.diagnostics import AutoDetuneWarning
+    raster = np.ones((1, 4, 4))
+    wls = [1.0 + 1e-12, 0.9]
+    base = _problem(raster, [0.2], 1.0, 2.25, wls, theta=0.0, phi=0.0, pol='p', scattering=False)
+    problem = rcwa.Problem(stack=rcwa.Stack(layers=base.stack.layers, top=base.stack.top,
+                                            bottom=base.stack.bottom, period=(1.0, 1.0)),
+                           excitation=base.excitation, units=base.units)
+    with pytest.warns(AutoDetuneWarning):
+        res = rcwa.run(problem, numerics=rcwa.Numerics(orders=(2, 2), detuning=1e-6), backend='torch',
+                       device='cuda:0')
+    assert res.diagnostics['exceptional_points']['grazing_layer_solves'] == 0
+    eff_wl = res.diagnostics['effective_wavelengths']
+    assert eff_wl[0] == (1.0 + 1e-12) * (1 + 1e-6) and eff_wl[1] == 0.9
+    assert np.array_equal(np.asarray(res.wavelengths), np.asarray(eff_wl))
+    moved = _problem(raster, [0.2], 1.0, 2.25, [eff_wl[0], 0.9], theta=0.0, phi=0.0, pol='p', scattering=False)
+    moved = rcwa.Problem(stack=problem.stack, excitation=moved.excitation, units=moved.units)
+    ref = rcwa.run(moved, numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda:0')
+    np.testing.assert_allclose(np.asarray(res.R), np.asarray(ref.R), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.asarray(res.T), np.asarray(ref.T), rtol=0, atol=1e-12)

@@ -160,6 +160,7 @@ class _Resolved:
     factorization: str = 'li'    # 'li' (the default) or 'laurent'
     coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
     scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
+    thetas: tuple = None             # per-point effective theta after a detuning ladder
 
 
 def _resolve(problem, numerics, execution):
@@ -351,6 +352,13 @@ class _Cache:
         self.conv = _engine.ConvCache(device)
         self.graph = [None, None]    # captured solve of this preparation (launch-bound sizes)
 
+    def sharing_conv(self):
+        """A cache with the same convolution matrices and no captured solve (for the points a
+        detuning ladder solves again)."""
+        other = _Cache.__new__(_Cache)
+        other.conv, other.graph = self.conv, [None, None]
+        return other
+
     def get(self, rasters, hu, hv, factorization='li'):
         import torch
         key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
@@ -401,6 +409,7 @@ class Prepared:
             return _run_live(self, problem, r, checks)
         try:
             per_wl, flags, smats = _solve_resolved(r, self._cache)
+            r, per_wl, flags, smats = _detune(r, self._cache, per_wl, flags, smats, self.numerics)
         except FloatingPointError as error:
             raise NumericalError(str(error), code='rcwa.singular', path='problem') from error
         except ValueError as error:
@@ -419,7 +428,7 @@ class Prepared:
                 'time_convention': 'exp(-i*omega*t)'}
         diag = {'report': self.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
                 'effective_wavelengths': tuple(float(w) for w in r.wavelengths),
-                'effective_theta': tuple(r.source.theta for _ in r.wavelengths),
+                'effective_theta': r.thetas or tuple(r.source.theta for _ in r.wavelengths),
                 'exceptional_points': ep_counts}
         result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats)
         if not result.validity.checks['numerical_finite']:
@@ -438,6 +447,10 @@ def _run_live(prepared, problem, r, checks):
         raise UnsupportedFeatureError(
             'Gradients are implemented for raster stacks on the 2D coupled path; this problem '
             f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
+    if prepared.numerics.detuning:
+        raise UnsupportedFeatureError(
+            'A differentiable run solves the requested point; detuning is applied to concrete '
+            'runs only.', code='rcwa.gradient', path='numerics.detuning')
     if r.scattering:
         raise UnsupportedFeatureError(
             'Gradients are implemented for the diffraction outputs; a differentiable run cannot '
@@ -529,6 +542,76 @@ def flag_notice(flags, numerics):
         return None, counts
     return ('; '.join(parts) + '. This engine keeps the modes as computed; results near such a '
             'point can differ from the reference beyond round-off.'), counts
+
+
+def _detune(r, cache, per_wl, flags, smats, numerics):
+    """``Numerics(detuning=step)``: the reference's Rayleigh-point ladder (reference
+    ``engine.py:6382-6483``). A point whose solve holds a grazing mode or failed is solved again
+    with its wavelength times ``1 + step``; one still flagged also gets theta times ``1 + step``
+    (theta = 0 has no second rung). The knobs multiply, each point keeps the first rung that
+    clears it (or the last that solved), and the moved points are reported:
+    ``AutoDetuneWarning``, the effective wavelengths as the result's wavelengths and the
+    effective angles in its diagnostics; a condition that outlives the ladder warns
+    ``RayleighPointWarning``. Returns ``(r, per_wl, flags, smats)`` with ``r`` carrying the
+    effective wavelengths and a per-point ``thetas``."""
+    import warnings
+    from dataclasses import replace
+    from .diagnostics import AutoDetuneWarning, RayleighPointWarning
+    step = numerics.detuning
+    fl = _engine._native
+    retry_bits = fl.FLAG_GRAZING | fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR
+    if not step:
+        return r, per_wl, flags, smats
+    flags = np.array(flags)
+    bad = [p for p in range(len(per_wl)) if np.any(flags[p] & retry_bits)]
+    if not bad:
+        return r, per_wl, flags, smats
+    per_wl, smats = list(per_wl), (list(smats) if smats is not None else None)
+    wls = np.array(r.wavelengths, dtype=np.float64)
+    theta0 = r.source.theta
+    thetas = [theta0 for _ in per_wl]
+    knobs = ['wavelength'] + (['theta'] if theta0 not in (None, 0, 0.0) else [])
+    moved = {}
+    for knob in knobs:
+        if not bad:
+            break
+        trial_wl = wls[bad] * (1 + step)
+        trial_theta = theta0 * (1 + step) if knob == 'theta' else None
+        sub = replace(r, wavelengths=trial_wl,
+                      source=replace(r.source, theta=trial_theta) if trial_theta is not None else r.source)
+        sub_per, sub_flags, sub_smats = _solve_resolved(sub, cache.sharing_conv())
+        still = []
+        for j, p in enumerate(bad):
+            if np.any(sub_flags[j] & (fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR)):
+                still.append(p)            # this rung did not take; the point keeps its last answer
+                continue
+            wls[p] = trial_wl[j]
+            per_wl[p] = sub_per[j]
+            flags[p] = sub_flags[j]
+            if smats is not None:
+                smats[p] = sub_smats[j]
+            if trial_theta is not None:
+                thetas[p] = trial_theta
+            moved.setdefault(p, []).append(knob)
+            if np.any(sub_flags[j] & fl.FLAG_GRAZING):
+                still.append(p)
+        bad = still
+    if moved:
+        where = ', '.join(f'{len(v)} point(s) by {"+".join(k)}' for k, v in
+                          _group_moves(moved).items())
+        warnings.warn(f'detuning={step:g} moved {where} off the requested point; the result '
+                      'reports the effective wavelengths and angles.', AutoDetuneWarning, stacklevel=3)
+    if bad:
+        warnings.warn(f'{len(bad)} point(s) still hold a layer mode on its light line after the '
+                      'detuning ladder.', RayleighPointWarning, stacklevel=3)
+    return replace(r, wavelengths=wls, thetas=tuple(thetas)), per_wl, flags, smats
+
+
+def _group_moves(moved):
+    out = {}
+    for p, knobs in moved.items():
+        out.setdefault(tuple(knobs), []).append(p)
+    return out
 
 
 def _solve_resolved(r, cache):

@@ -74,7 +74,7 @@ class Numerics:
         """The first non-default choice this engine does not implement, as (field, value)."""
         for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled', 'integrated')),
                          ('factorization', (None, 'li', 'laurent')), ('formulation', ('auto', 'second_order')),
-                         ('li_nesting', (None, 'fixed')), ('detuning', (None,)),
+                         ('li_nesting', (None, 'fixed')),
                          ('raster_normal_method', (None,)), ('nvf_projector', (None,)),
                          ('shape_nvf_grid_size', (None,)), ('half_space_merge_tolerance', (0.0,)),
                          ('wall_resolution', (None,)), ('shape_normal_method', (None,)),

@@ -140,6 +140,10 @@ class Server:
             raise UnsupportedFeatureError(
                 'The server returns diffraction outputs; solve a scattering-matrix request with '
                 'rcwa.run.', code='rcwa.unsupported', path='outputs.scattering')
+        if req.numerics.detuning:
+            raise UnsupportedFeatureError(
+                'The server solves the requested points; run a request with detuning through '
+                'rcwa.run.', code='rcwa.unsupported', path='numerics.detuning')
         checks = _runtime_checks(r)
         bad = [k for k, ok in checks.items() if not ok]
         if bad:
