@@ -463,8 +463,18 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
 
 
 def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv):
-    """Separable ``kx``, ``ky`` (``engine.py:953-993``)."""
+    """Separable ``kx``, ``ky`` (``engine.py:953-993``); on a skewed cell, ``period`` a 2 x 2
+    lattice whose columns are a1 and a2, the two (2hv+1, 2hu+1) grids of ``lattice.py:899-944``."""
     k_inc_x, k_inc_y = incident_kxy(eps_top, theta, phi)
+    A = np.asarray(period, dtype=np.float64)
+    if A.shape == (2, 2):
+        a1x, a2x, a1y, a2y = A[0, 0], A[0, 1], A[1, 0], A[1, 1]
+        lam = wavelength / (a1x * a2y - a2x * a1y)
+        m = np.arange(-hu, hu + 1, dtype=np.float64)[None, :]
+        n = np.arange(-hv, hv + 1, dtype=np.float64)[:, None]
+        kx = k_inc_x + lam * (a2y * m - a1y * n)
+        ky = k_inc_y + lam * (-a2x * m + a1x * n)
+        return kx.astype(np.complex128), ky.astype(np.complex128)
     kx = (k_inc_x + np.arange(-hu, hu + 1) * (wavelength / period[0])).astype(np.complex128)
     ky = (k_inc_y + np.arange(-hv, hv + 1) * (wavelength / period[1])).astype(np.complex128)
     return kx, ky
@@ -479,8 +489,11 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
     kx, ky = order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv)
     nu, nv = 2 * hu + 1, 2 * hv + 1
     N = nu * nv
-    kx_g = np.broadcast_to(kx[None, :], (nv, nu))
-    ky_g = np.broadcast_to(ky[:, None], (nv, nu))
+    if kx.ndim == 2:                    # skewed cell: the grids themselves
+        kx_g, ky_g = kx, ky
+    else:
+        kx_g = np.broadcast_to(kx[None, :], (nv, nu))
+        ky_g = np.broadcast_to(ky[:, None], (nv, nu))
     kxf, kyf = kx_g.reshape(-1).copy(), ky_g.reshape(-1).copy()
     k0 = 2 * _PI / wavelength
     varphi = order_azimuth(ky_g, kx_g, phi)
@@ -572,8 +585,9 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
     is ``'s'``/``'p'`` or a Jones pair. Routing (``engine.py:3092-3181``): a one-row raster with
     a polarization word and ``phi is None`` takes the pure 1D path; everything else 2D (the 1D
-    conical layout is out of scope). ``factorization``: ``'li'`` (the default) or ``'laurent'``;
-    ``coefficients``: ``'sampled'`` (the default) or ``'integrated'``.
+    conical layout is out of scope). ``period``: ``(Lx, Ly)``, or a 2 x 2 lattice whose columns
+    are the lattice vectors (a skewed cell; Laurent's rule). ``factorization``: ``'li'`` (the
+    default) or ``'laurent'``; ``coefficients``: ``'sampled'`` (the default) or ``'integrated'``.
     """
     rule = conv_rule(factorization, coefficients)
     eps_layers = np.asarray(eps_layers, dtype=np.complex128)

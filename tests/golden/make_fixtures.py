@@ -14,8 +14,8 @@ Sources (paths relative to the reference root):
   ``tests/rcwa/test_golden_legacy_parity.py:250-264`` (layers listed top-to-bottom there).
 * ``tests/references/golden_fmmax_laurent.npz`` -- fmmax 1.7.1 ``Formulation.FFT`` (Laurent's
   rule) totals and per-order efficiencies at four samplings per pixel, double-precision-phase
-  flavour, for the two rectangular cells (the hexagonal one needs an oblique lattice); the
-  scene table is ``tests/references/_fmmax_laurent_cases.py:44-99``.
+  flavour, for the two rectangular cells and the hexagonal one; the scene table is
+  ``tests/references/_fmmax_laurent_cases.py:39-71``.
 """
 import json
 import sys
@@ -32,7 +32,7 @@ RETICOLO_CASES = ['te_normal', 'te_oblique', 'tm_normal', 'tm_oblique', 'te_quar
 LEGACY_CASES = ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique', 'te_2layer',
                 'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer', '2d_cfs']
 
-FMMAX_LAURENT_CASES = ['rod_eps12_rect', 'hole_metal_rect']
+FMMAX_LAURENT_CASES = ['rod_eps12_rect', 'hole_metal_rect', 'cross_eps6_hex']
 
 # tests/references/test_reticolo_li_nesting.py:34-39 -- RETICOLO V10 total T, (TE, TM).
 LI_NESTING_FIXED = {'3': [0.9822061471, 0.8913451356], '5': [0.9816211625, 0.8830613359]}

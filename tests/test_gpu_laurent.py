@@ -57,9 +57,10 @@ def _problem(raster, period, pol, wavelength=WL, theta=np.deg2rad(THETA_DEG), ph
              top=1.0, bottom=1.0, units='nm'):
     layers = [rcwa.Layer(thickness=thick if np.ndim(thick) == 0 else thick[i], epsilon=raster[i])
               for i in range(raster.shape[0])]
+    cell = dict(lattice=period) if np.shape(period) == (2, 2) else dict(period=period)
     return rcwa.Problem(
         stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(top), bottom=wavebit_amd.Material(bottom),
-                         period=period),
+                         **cell),
         excitation=rcwa.Excitation(wavelength=wavelength,
                                    sources=(rcwa.PlaneWave(polarization=pol, theta=theta, phi=phi),)),
         units=wavebit_amd.Units(length=units))
@@ -230,3 +231,37 @@ def test_integrated_coefficients_match_the_oracle_segment_series(rule):
                                 'integrated')['incident']
         assert np.abs(np.asarray(runs['integrated'].R)[i] - want['R']).max() < 1e-10
         assert np.abs(np.asarray(runs['integrated'].T)[i] - want['T']).max() < 1e-10
+
+
+def test_a_skewed_cell_needs_laurent_and_an_axis_aligned_lattice_is_the_period():
+    """Li's rules are refused on the hexagonal cell (they run along the cell's axes); a lattice
+    that is diag(Lx, Ly) is the rectangular cell, to the bit."""
+    from wavebit_amd.errors import UnsupportedFeatureError
+    period, raster = CASES['cross_eps6_hex']
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.run(_problem(raster[None], period, 's'), numerics=rcwa.Numerics(orders=(3, 3)), backend='torch',
+                 device='cuda:0')
+    assert e.value.path == 'numerics.factorization'
+    rng = np.random.default_rng(2)
+    eps = rng.uniform(1, 9, (1, 10, 12))
+    numerics = rcwa.Numerics(orders=(3, 2))
+    a = rcwa.run(_problem(eps, (0.7, 0.6), 'p', wavelength=0.9, units='um'), numerics=numerics, backend='torch',
+                 device='cuda:0')
+    b = rcwa.run(_problem(eps, np.diag([0.7, 0.6]), 'p', wavelength=0.9, units='um'), numerics=numerics,
+                 backend='torch', device='cuda:0')
+    assert np.array_equal(np.asarray(a.R), np.asarray(b.R)) and np.array_equal(np.asarray(a.T), np.asarray(b.T))
+
+
+def test_skewed_cell_lossy_stack_matches_the_oracle():
+    """Two lossy layers on a sheared cell, oblique incidence with a Jones polarization."""
+    rng = np.random.default_rng(8)
+    eps = (rng.uniform(1, 9, (2, 8, 10)) + 1j * rng.uniform(0, 0.5, (2, 8, 10))).astype(np.complex128)
+    cell = np.array([[0.8, 0.25], [0.0, 0.7]])
+    problem = _problem(eps, cell, (0.6, 0.8j), wavelength=0.9, theta=0.3, phi=0.7, thick=[0.2, 0.15],
+                       bottom=2.25, units='um')
+    res = rcwa.run(problem, numerics=rcwa.Numerics(orders=(3, 3), factorization='laurent'), backend='torch',
+                   device='cuda:0')
+    want = orc.solve_raster(eps, [0.2, 0.15], 1.0, 2.25, cell, 0.9, 0.3, 0.7, (0.6, 0.8j), (3, 3),
+                            'laurent')['incident']
+    assert np.abs(np.asarray(res.R)[0] - want['R']).max() < 1e-10
+    assert np.abs(np.asarray(res.T)[0] - want['T']).max() < 1e-10

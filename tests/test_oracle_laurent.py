@@ -22,7 +22,7 @@ from oracle import rcwa_oracle as orc
 GOLDEN = Path(__file__).resolve().parent / 'golden'
 ROUND_OFF = 1e-12
 
-# tests/references/_fmmax_laurent_cases.py:39-71 (reference): the scene table, rectangular cells.
+# tests/references/_fmmax_laurent_cases.py:39-71 (reference): the scene table.
 WL, THETA_DEG, THICK, ORDER, PIXELS = 500.0, 11.0, 240.0, 7, 16
 
 
@@ -32,10 +32,19 @@ def _rod(eps_rod, eps_bg, rows, cols):
     return a
 
 
-CASES = {   # name -> (period (x, y) nm, raster [v, u])
+def _cross(eps_in, eps_bg):
+    a = np.full((PIXELS, PIXELS), eps_bg, dtype=complex)
+    a[3:13, 5:8] = eps_in          # a tall arm
+    a[6:9, 2:14] = eps_in          # a wide arm, off the tall one's centre
+    return a
+
+
+CASES = {   # name -> (period (x, y) nm or a 2 x 2 lattice with the lattice vectors as columns, raster [v, u])
     'rod_eps12_rect': ((700.0, 560.0), _rod(12.25, 1.0, (4, 11), (2, 12))),
     'hole_metal_rect': ((600.0, 600.0), _rod(1.0, -16.49 + 1.06j, (5, 12), (3, 9))),
+    'cross_eps6_hex': (np.array([[640.0, 320.0], [0.0, 320.0 * np.sqrt(3.0)]]), _cross(6.0, 1.0)),
 }
+RECT = [name for name, (cell, _) in CASES.items() if np.shape(cell) == (2,)]
 
 
 def golden():
@@ -98,14 +107,15 @@ def test_laurent_orders_match_fmmax_and_the_mirror_misses(name):
     assert mirrored > 1e3 * tol, (mirrored, tol)
 
 
-@pytest.mark.parametrize('name', list(CASES))
+@pytest.mark.parametrize('name', RECT)
 def test_the_golden_tells_li_from_laurent(name):
     want, tol = extrapolated(golden(), name, 'totals')
     assert np.abs(totals(solve(name, 'li')) - want).max() > 1e6 * tol
 
 
-def test_laurent_conserves_energy_on_the_lossless_cell():
-    got = totals(solve('rod_eps12_rect'))
+@pytest.mark.parametrize('name', ['rod_eps12_rect', 'cross_eps6_hex'])
+def test_laurent_conserves_energy_on_the_lossless_cells(name):
+    got = totals(solve(name))
     assert abs(got[0] + got[1] - 1.0) < 1e-12 and abs(got[2] + got[3] - 1.0) < 1e-12
 
 

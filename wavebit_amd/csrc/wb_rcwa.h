@@ -8,8 +8,10 @@
 // per-point parameter record (doubles), set by the host
 enum {
   PT_WL = 0, PT_KX0 = 1, PT_KY0 = 2, PT_PHI = 3, PT_PX = 4, PT_PY = 5,
-  PT_ETOP = 6, PT_EBOT = 8, PT_AS = 10, PT_AP = 12, PT_STRIDE = 16
+  PT_ETOP = 6, PT_EBOT = 8, PT_AS = 10, PT_AP = 12, PT_A1Y = 14, PT_A2X = 15, PT_STRIDE = 16
 };
+// The cell: PT_PX / PT_PY are a1x / a2y of the lattice vectors a1 = (a1x, a1y), a2 = (a2x, a2y)
+// (the periods of a rectangular cell), PT_A1Y / PT_A2X its skew entries (zero when rectangular).
 // per-order vectors kept per point: kx, ky, kz_kt, kz_kb, gap y, Fc, Fs  (7 x N complex)
 enum { OV_KX = 0, OV_KY, OV_KZT, OV_KZB, OV_Y, OV_FC, OV_FS, OV_COUNT };
 // per-point diagonal blocks (n each): ref11, ref12, ref21, ref22, trn11, trn12, trn21, trn22

@@ -80,8 +80,18 @@ __global__ void k_point_setup(int P, int hu, int hv, int mode, const double* pts
   const double* pt = pts + (long long)p * PT_STRIDE;
   const int mi = r % nu - hu, ni = r / nu - hv;
   const double wl = pt[PT_WL];
-  const double kx = pt[PT_KX0] + (double)mi * (wl / pt[PT_PX]);
-  const double ky = (mode == 0) ? pt[PT_KY0] + (double)ni * (wl / pt[PT_PY]) : 0.0;
+  // order (m, n) sits at k_inc + m b1 + n b2 (reference lattice.py:899-944): on a rectangular
+  // cell kx0 + m wl/Lx and ky0 + n wl/Ly; on a skewed one the closed form over det(a1, a2)
+  const double a1x = pt[PT_PX], a2y = pt[PT_PY], a1y = pt[PT_A1Y], a2x = pt[PT_A2X];
+  double kx, ky;
+  if (a1y == 0.0 && a2x == 0.0) {
+    kx = pt[PT_KX0] + (double)mi * (wl / a1x);
+    ky = (mode == 0) ? pt[PT_KY0] + (double)ni * (wl / a2y) : 0.0;
+  } else {
+    const double lam = wl / (a1x * a2y - a2x * a1y);
+    kx = pt[PT_KX0] + lam * (a2y * (double)mi - a1y * (double)ni);
+    ky = pt[PT_KY0] + lam * (-a2x * (double)mi + a1x * (double)ni);
+  }
   const cd kpar2 = mkc(kx * kx + ky * ky, 0.0);
   // kernel top = scene bottom (substrate), kernel bottom = scene top (superstrate)
   const cd e_kt = pt_c(pt, PT_EBOT), e_kb = pt_c(pt, PT_ETOP);

@@ -133,9 +133,24 @@ def point_record(pt, path):
     a_s, a_p = jones(pt.polarization) if path == PATH_2D else (1.0 + 0j, 0.0 + 0j)
     eps_t, eps_b = complex(pt.eps_top), complex(pt.eps_bottom)
     py = pt.period[1] if len(pt.period) > 1 else 1.0
+    a1y, a2x = (float(pt.period[2]), float(pt.period[3])) if len(pt.period) == 4 else (0.0, 0.0)
     return [pt.wavelength, kx0, ky0, 0.0 if pt.phi is None else float(pt.phi),
             float(pt.period[0]), float(py), eps_t.real, eps_t.imag, eps_b.real, eps_b.imag,
-            a_s.real, a_s.imag, a_p.real, a_p.imag, 0.0, 0.0]
+            a_s.real, a_s.imag, a_p.real, a_p.imag, a1y, a2x]
+
+
+def order_kxy(kx0, ky0, wavelength, period, hu, hv):
+    """kx/k0, ky/k0 of the retained orders: vectors (2hu+1,), (2hv+1,) on a rectangular cell;
+    on a skewed one (``period`` = (a1x, a2y, a1y, a2x)) two (2hv+1, 2hu+1) grids (reference
+    ``lattice.py:899-944``)."""
+    m = np.arange(-hu, hu + 1, dtype=np.float64)
+    n = np.arange(-hv, hv + 1, dtype=np.float64)
+    if len(period) != 4 or (period[2] == 0.0 and period[3] == 0.0):
+        return kx0 + m * (wavelength / period[0]), ky0 + n * (wavelength / period[1])
+    a1x, a2y, a1y, a2x = (float(v) for v in period)
+    lam = wavelength / (a1x * a2y - a2x * a1y)
+    return (kx0 + lam * (a2y * m[None, :] - a1y * n[:, None]),
+            ky0 + lam * (-a2x * m[None, :] + a1x * n[:, None]))
 
 
 def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):

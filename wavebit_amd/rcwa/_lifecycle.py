@@ -193,12 +193,20 @@ def _resolve(problem, numerics, execution):
                                       code='rcwa.unsupported', path='outputs.diffraction')
     stack = problem.stack
     if stack.lattice is not None:
-        raise UnsupportedFeatureError('Oblique lattices are not implemented; give period=(Lx, Ly).',
-                                      code='rcwa.unsupported', path='stack.lattice')
-    period = tuple(_scalar(v, 'stack.period') for v in stack.period)
-    if len(period) != 2 or not all(p > 0 for p in period):
-        raise ValidationError('period must be two positive lengths.', code='stack.period',
-                              path='stack.period')
+        period = _cell(stack.lattice)
+    else:
+        period = tuple(_scalar(v, 'stack.period') for v in stack.period)
+        if len(period) != 2 or not all(p > 0 for p in period):
+            raise ValidationError('period must be two positive lengths.', code='stack.period',
+                                  path='stack.period')
+    skewed = len(period) == 4
+    if skewed and (numerics.factorization or 'li') != 'laurent':
+        # reference: Li's inverse rules run along the lattice axes and are refused on a skewed
+        # cell; Laurent's rule names no axis (factorization.py:991-994)
+        raise UnsupportedFeatureError(
+            "factorization='li' runs its inverse rules along the cell's axes, which a skewed "
+            "lattice does not have: pass factorization='laurent' on this cell.",
+            code='rcwa.unsupported', path='numerics.factorization')
     sources = problem.excitation.sources
     if len(sources) != 1:
         raise UnsupportedFeatureError('One incident plane wave per problem is implemented.',
@@ -266,12 +274,38 @@ def _resolve(problem, numerics, execution):
                                   path='excitation.sources.polarization')
     n_rows = rasters.shape[1] if not uniform else 2
     path = _engine.route(n_rows, pol, phi, hv)
+    if skewed and path != _engine.PATH_2D:
+        raise UnsupportedFeatureError('A skewed lattice is solved on the 2D coupled path: give a '
+                                      'Jones polarization or phi, or more than one raster row.',
+                                      code='rcwa.unsupported', path='stack.lattice')
     ws = _wavelengths(problem)
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
                      hu=hu, hv=hv, uniform=uniform, factorization=factorization,
                      coefficients=numerics.fourier_coefficients or 'sampled',
                      scattering=bool(problem.outputs.scattering))
+
+
+def _cell(lattice):
+    """A lattice (2 x 2, columns a1 and a2, or an object with ``u`` / ``v``) as ``(a1x, a2y)``
+    when axis-aligned -- the rectangular cell, bit-identical to ``period`` -- else
+    ``(a1x, a2y, a1y, a2x)``."""
+    try:
+        if hasattr(lattice, 'u') and hasattr(lattice, 'v'):
+            A = np.stack([np.asarray(lattice.u, dtype=np.float64), np.asarray(lattice.v, dtype=np.float64)], axis=1)
+        else:
+            A = np.asarray(lattice, dtype=np.float64)
+    except (TypeError, ValueError):
+        raise ValidationError('lattice must be a 2 x 2 basis whose columns are the lattice vectors.',
+                              code='stack.lattice', path='stack.lattice') from None
+    if A.shape != (2, 2) or not np.all(np.isfinite(A)):
+        raise ValidationError('lattice must be a finite 2 x 2 basis whose columns are the lattice vectors.',
+                              code='stack.lattice', path='stack.lattice')
+    a1x, a1y, a2x, a2y = float(A[0, 0]), float(A[1, 0]), float(A[0, 1]), float(A[1, 1])
+    if a1x <= 0 or a2y <= 0 or a1x * a2y - a2x * a1y <= 0:
+        raise ValidationError('lattice vectors must form a right-handed basis with a1x, a2y > 0.',
+                              code='stack.lattice', path='stack.lattice')
+    return (a1x, a2y) if a1y == 0.0 and a2x == 0.0 else (a1x, a2y, a1y, a2x)
 
 
 def _structure_key(r):
@@ -647,9 +681,8 @@ def _scattering_matrices(r, smat):
     n_top, n_bot = np.sqrt(complex(r.eps_top)), np.sqrt(complex(r.eps_bottom))
     out = []
     for p, wl in enumerate(r.wavelengths):
-        kx = kx0 + np.arange(-r.hu, r.hu + 1) * (float(wl) / r.period[0])
-        ky = (ky0 + np.arange(-r.hv, r.hv + 1) * (float(wl) / r.period[1])
-              if channel == 'sp' else None)
+        kx, ky = _engine.order_kxy(kx0, ky0, float(wl), r.period, r.hu, r.hv)
+        ky = ky if channel == 'sp' else None
         S = smat[p]
         out.append(ScatteringMatrix(S[0].copy(), S[1].copy(), S[2].copy(), S[3].copy(),
                                     entry_port=Port(kx, ky, n_top, 1.0, channel),
