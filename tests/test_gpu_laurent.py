@@ -265,3 +265,36 @@ def test_skewed_cell_lossy_stack_matches_the_oracle():
                             'laurent')['incident']
     assert np.abs(np.asarray(res.R)[0] - want['R']).max() < 1e-10
     assert np.abs(np.asarray(res.T)[0] - want['T']).max() < 1e-10
+
+
+def test_gradients_on_a_skewed_cell_match_central_differences_of_the_oracle():
+    rng = np.random.default_rng(19)
+    L, Ny, Nx, M = 1, 6, 8, (2, 2)
+    eps = rng.uniform(1, 9, (L, Ny, Nx))
+    cell = np.array([[0.8, 0.25], [0.0, 0.7]])
+    eps_t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    thick_t = torch.tensor([0.2], dtype=torch.float64, requires_grad=True)
+    problem = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=thick_t[0], epsilon=eps_t[0])], top=wavebit_amd.Material(1.0),
+                         bottom=wavebit_amd.Material(2.25), lattice=cell),
+        excitation=rcwa.Excitation(wavelength=[0.9], sources=(rcwa.PlaneWave(polarization='p', theta=0.2, phi=0.3),)),
+        units=wavebit_amd.Units(length='um'))
+    res = rcwa.run(problem, numerics=rcwa.Numerics(orders=M, factorization='laurent'), backend='torch',
+                   device='cuda:0')
+    loss = res.T.sum()
+    loss.backward()
+    g = eps_t.grad.numpy()
+
+    def total(e):
+        return float(orc.solve_raster(e.astype(np.complex128), [0.2], 1.0, 2.25, cell, 0.9, 0.2, 0.3, 'p', M,
+                                      'laurent')['incident']['T'].sum())
+
+    assert abs(float(loss.detach()) - total(eps)) < 1e-10
+    h = 1e-6
+    for _ in range(4):
+        y, x = int(rng.integers(0, Ny)), int(rng.integers(0, Nx))
+        ep, em = eps.copy(), eps.copy()
+        ep[0, y, x] += h
+        em[0, y, x] -= h
+        fd = (total(ep) - total(em)) / (2 * h)
+        assert abs(g[0, y, x] - fd) <= 1e-6 * abs(fd) + 5e-9, (y, x, g[0, y, x], fd)
