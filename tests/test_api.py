@@ -210,3 +210,40 @@ def test_detuning_is_accepted_and_validated():
     with pytest.raises(ValidationError) as e:
         rcwa.Numerics(orders=(1, 1), detuning=-1.0)
     assert e.value.code == 'numerics.detuning'
+
+
+def test_detuning_ladder_moves_wavelength_then_angle_once_each(monkeypatch):
+    """The ladder on the host (reference engine.py:6382-6483), with the device solve replaced
+    by a stand-in that grazes at the requested point and at the nudged wavelength alone:
+    the point ends at wavelength x (1+step) and theta x (1+step) -- each knob applied once."""
+    import warnings
+    from dataclasses import replace
+    from wavebit_amd.rcwa import _lifecycle
+    from wavebit_amd.rcwa._lifecycle import _detune, _resolve, Execution
+    from wavebit_amd.rcwa.diagnostics import AutoDetuneWarning
+    step, theta = 1e-6, 0.2
+    p = _problem(wavelength=[0.8, 0.9], theta=theta)
+    r = _resolve(p, rcwa.Numerics(orders=(1, 1), detuning=step), Execution(backend='torch', device='cuda'))
+    calls = []
+
+    def fake_solve(sub, cache):
+        calls.append((tuple(sub.wavelengths), sub.source.theta))
+        moved_theta = sub.source.theta != theta
+        flags = np.array([[0 if moved_theta else _native.FLAG_GRAZING] for _ in sub.wavelengths])
+        return [('solved', float(w), sub.source.theta) for w in sub.wavelengths], flags, None
+
+    class Cache:
+        def sharing_conv(self):
+            return self
+    monkeypatch.setattr(_lifecycle, '_solve_resolved', fake_solve)
+    flags = np.array([[_native.FLAG_GRAZING], [0]])
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter('always')
+        r2, per, fl, _ = _detune(r, Cache(), ['first0', 'first1'], flags, None,
+                                 rcwa.Numerics(orders=(1, 1), detuning=step))
+    assert any(issubclass(w.category, AutoDetuneWarning) for w in caught)
+    assert calls == [((0.8 * (1 + step),), theta), ((0.8 * (1 + step),), theta * (1 + step))]
+    assert tuple(r2.wavelengths) == (0.8 * (1 + step), 0.9)
+    assert r2.thetas == (theta * (1 + step), theta)
+    assert per[1] == 'first1' and per[0] == ('solved', 0.8 * (1 + step), theta * (1 + step))
+    assert int(fl[0, 0]) == 0

@@ -601,15 +601,18 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
     if not bad:
         return r, per_wl, flags, smats
     per_wl, smats = list(per_wl), (list(smats) if smats is not None else None)
-    wls = np.array(r.wavelengths, dtype=np.float64)
+    wl0 = np.array(r.wavelengths, dtype=np.float64)
+    wls = wl0.copy()
     theta0 = r.source.theta
     thetas = [theta0 for _ in per_wl]
     knobs = ['wavelength'] + (['theta'] if theta0 not in (None, 0, 0.0) else [])
+    wl_applied = set()                 # points whose wavelength rung took: the knob stays applied
     moved = {}
     for knob in knobs:
         if not bad:
             break
-        trial_wl = wls[bad] * (1 + step)
+        trial_wl = np.array([wl0[p] * (1 + step) if (knob == 'wavelength' or p in wl_applied) else wl0[p]
+                             for p in bad])
         trial_theta = theta0 * (1 + step) if knob == 'theta' else None
         sub = replace(r, wavelengths=trial_wl,
                       source=replace(r.source, theta=trial_theta) if trial_theta is not None else r.source)
@@ -627,6 +630,8 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
             if trial_theta is not None:
                 thetas[p] = trial_theta
             moved.setdefault(p, []).append(knob)
+            if knob == 'wavelength':
+                wl_applied.add(p)
             if np.any(sub_flags[j] & fl.FLAG_GRAZING):
                 still.append(p)
         bad = still
