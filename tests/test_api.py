@@ -247,3 +247,9 @@ def test_detuning_ladder_moves_wavelength_then_angle_once_each(monkeypatch):
     assert r2.thetas == (theta * (1 + step), theta)
     assert per[1] == 'first1' and per[0] == ('solved', 0.8 * (1 + step), theta * (1 + step))
     assert int(fl[0, 0]) == 0
+
+
+def test_y_orders_above_fifteen_are_refused():
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(_problem(), numerics=rcwa.Numerics(orders=(1, 16)), backend='torch', device='cuda')
+    assert e.value.code == 'rcwa.unsupported' and e.value.path == 'numerics.orders'

@@ -90,3 +90,27 @@ def test_prepared_reuses_device_conv_matrices_and_later_runs_do_not_alias():
     r3 = prep.run(_problem(w, [500.0, 600.0], 's', 0.1, 0.2))
     assert len(prep._cache.conv) == 2            # same raster: cached matrices reused
     assert np.array_equal(r3.R, keep)
+
+
+@pytest.mark.parametrize('pol', ['s', 'p'])
+def test_a_stack_with_no_layers_is_the_bare_interface(pol):
+    """No layers: the two half-spaces meet at z = 0 and the specular order carries Fresnel's
+    reflectance; nothing else is diffracted."""
+    import wavebit_amd
+    from wavebit_amd import rcwa
+    theta, n1, n2 = 0.4, 1.0, 1.5
+    problem = rcwa.Problem(
+        stack=rcwa.Stack(layers=[], top=wavebit_amd.Material(n1 ** 2), bottom=wavebit_amd.Material(n2 ** 2),
+                         period=(0.7, 0.7)),
+        excitation=rcwa.Excitation(wavelength=0.9, sources=(rcwa.PlaneWave(polarization=pol, theta=theta, phi=0.0),)),
+        units=wavebit_amd.Units(length='um'))
+    res = rcwa.run(problem, numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda:0')
+    c1 = np.cos(theta)
+    c2 = np.sqrt(1 - (n1 * np.sin(theta) / n2) ** 2)
+    r = ((n1 * c1 - n2 * c2) / (n1 * c1 + n2 * c2) if pol == 's'
+         else (n2 * c1 - n1 * c2) / (n2 * c1 + n1 * c2))
+    R = np.asarray(res.R)[0]
+    T = np.asarray(res.T)[0]
+    assert abs(R[2, 2] - r ** 2) < 1e-12
+    assert abs(T[2, 2] - (1 - r ** 2)) < 1e-12
+    assert abs(R.sum() - R[2, 2]) < 1e-14 and abs(T.sum() - T[2, 2]) < 1e-14
