@@ -114,3 +114,28 @@ def test_a_stack_with_no_layers_is_the_bare_interface(pol):
     assert abs(R[2, 2] - r ** 2) < 1e-12
     assert abs(T[2, 2] - (1 - r ** 2)) < 1e-12
     assert abs(R.sum() - R[2, 2]) < 1e-14 and abs(T.sum() - T[2, 2]) < 1e-14
+
+
+def test_absorption_is_power_entering_less_transmission():
+    """Lossless layers absorb nothing; a lossy layer absorbs 1 - R - T (the top medium is
+    lossless, so the power entering is 1 - R)."""
+    import wavebit_amd
+    from wavebit_amd import rcwa
+    rng = np.random.default_rng(4)
+    for loss in (0.0, 0.4):
+        eps = rng.uniform(1, 9, (2, 8, 8)) + 1j * loss * rng.uniform(0, 1, (2, 8, 8))
+        problem = rcwa.Problem(
+            stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.2, epsilon=e) for e in eps],
+                             top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25), period=(0.8, 0.8)),
+            excitation=rcwa.Excitation(wavelength=[0.85, 0.95],
+                                       sources=(rcwa.PlaneWave(polarization=(0.6, 0.8), theta=0.3, phi=0.2),)),
+            units=wavebit_amd.Units(length='um'))
+        res = rcwa.run(problem, numerics=rcwa.Numerics(orders=(3, 3)), backend='torch', device='cuda:0')
+        R, T = np.asarray(res.R).sum(axis=(1, 2)), np.asarray(res.T).sum(axis=(1, 2))
+        a = np.asarray(res.absorption)
+        assert a.shape == (2,)
+        assert np.abs(a - (1.0 - R - T)).max() < 1e-12
+        if loss == 0.0:
+            assert np.abs(a).max() < 1e-11
+        else:
+            assert a.min() > 1e-3

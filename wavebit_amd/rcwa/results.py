@@ -76,6 +76,16 @@ class Result:
         return self.outputs['transmitted']
 
     @property
+    def absorption(self):
+        """Net power absorbed in the finite layers over the incident power, per wavelength:
+        ``power_entering - T.sum()`` (reference ``results.py:266-272``,
+        ``observations/diffraction.py:92-104``); half-space absorption excluded, not clipped."""
+        if 'absorption' not in self.outputs:
+            raise UnsupportedFeatureError('Absorption is unavailable for this observation geometry.',
+                                          code='outputs.unavailable', path='outputs.absorption')
+        return self.outputs['absorption']
+
+    @property
     def scattering(self):
         """The stack's ``ScatteringMatrix`` per wavelength (reference ``results.py:136-141``)."""
         if self._scattering is None:
@@ -113,6 +123,7 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, s
                        'incident_port': 'top', 'polarization_frame': "each order's outgoing s/p frame",
                        'time_convention': 'exp(-i*omega*t)'})
     outputs['power_entering'] = np.array([float(d['incident']['power_entering']) for d in per_wavelength]) * scale
+    outputs['absorption'] = outputs['power_entering'] - outputs['transmitted'].efficiency.sum(axis=(1, 2))
     variants = {}
     for v in ('s_inc', 'p_inc'):
         if all(v in d for d in per_wavelength):
