@@ -253,3 +253,23 @@ def test_y_orders_above_fifteen_are_refused():
     with pytest.raises(UnsupportedFeatureError) as e:
         rcwa.prepare(_problem(), numerics=rcwa.Numerics(orders=(1, 16)), backend='torch', device='cuda')
     assert e.value.code == 'rcwa.unsupported' and e.value.path == 'numerics.orders'
+
+
+def test_diffraction_orders_are_read_by_order_label_and_wavelength_index():
+    from wavebit_amd.rcwa.results import DiffractionOrders
+    eff = np.arange(2 * 3 * 5, dtype=float).reshape(2, 3, 5)
+    amp = eff * (1 + 1j)
+    d = DiffractionOrders(wavelengths=np.array([0.8, 0.9]), orders_u=np.arange(-2, 3), orders_v=np.arange(-1, 2),
+                          amplitude_ratio_s=amp, amplitude_ratio_p=-amp, efficiency=eff, efficiency_s=eff / 2,
+                          efficiency_p=eff / 2)
+    o = d.order(1, -1)
+    assert o.order_u == 1 and o.order_v == -1
+    np.testing.assert_array_equal(o.efficiency, eff[:, 0, 3])
+    np.testing.assert_array_equal(o.amplitude_ratio_p, -amp[:, 0, 3])
+    assert d.order(0).efficiency.tolist() == eff[:, 1, 2].tolist()
+    with pytest.raises(KeyError):
+        d.order(3, 0)
+    w = d.at_wavelength_index(1)
+    assert w.wavelengths.tolist() == [0.9] and w.efficiency.shape == (1, 3, 5)
+    with pytest.raises(IndexError):
+        d.at_wavelength_index(2)

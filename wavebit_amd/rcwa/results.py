@@ -7,12 +7,32 @@ amplitude ratios in each order's own s/p frame, centre-referenced, as the refere
 them. Every array is an independent host (NumPy) snapshot: a later solve never changes an
 earlier result.
 """
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from types import MappingProxyType
 
 import numpy as np
 
 from ..errors import UnsupportedFeatureError
+
+
+_ORDER_ARRAYS = ('amplitude_ratio_s', 'amplitude_ratio_p', 'efficiency', 'efficiency_s', 'efficiency_p',
+                 'k_parallel')
+
+
+@dataclass(frozen=True)
+class DiffractionOrder:
+    """One order ``(order_u, order_v)`` of a ``DiffractionOrders`` table: each array at that
+    order, over the wavelength axis."""
+    wavelengths: object
+    order_u: int
+    order_v: int
+    amplitude_ratio_s: object
+    amplitude_ratio_p: object
+    efficiency: object
+    efficiency_s: object
+    efficiency_p: object
+    k_parallel: object = None
+    reference: dict = field(default_factory=dict)
 
 
 @dataclass(frozen=True)
@@ -27,6 +47,29 @@ class DiffractionOrders:
     efficiency_p: object
     k_parallel: object = None
     reference: dict = field(default_factory=dict)
+
+    def order(self, m, n=0):
+        """Order ``(m, n)`` as one record (reference ``observations/contracts.py:257-264``):
+        indexed by the order labels, not the array position; ``KeyError`` when not retained."""
+        try:
+            i = [int(v) for v in self.orders_u].index(int(m))
+            j = [int(v) for v in self.orders_v].index(int(n))
+        except ValueError as exc:
+            raise KeyError((m, n)) from exc
+        data = {name: None if getattr(self, name) is None else getattr(self, name)[:, j, i]
+                for name in _ORDER_ARRAYS}
+        return DiffractionOrder(wavelengths=self.wavelengths, order_u=int(m), order_v=int(n),
+                                reference=self.reference, **data)
+
+    def at_wavelength_index(self, index):
+        """These orders at the wavelength in position ``index``, as a length-1 axis (reference
+        ``observations/contracts.py:266-283``)."""
+        i = int(index)
+        if not 0 <= i < len(self.wavelengths):
+            raise IndexError(i)
+        data = {name: None if getattr(self, name) is None else getattr(self, name)[i:i + 1]
+                for name in _ORDER_ARRAYS}
+        return replace(self, wavelengths=self.wavelengths[i:i + 1], **data)
 
 
 @dataclass(frozen=True)
