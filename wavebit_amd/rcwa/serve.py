@@ -50,11 +50,11 @@ class Request:
 
     @classmethod
     def from_job(cls, text, arrival=0.0, id=-1, gradient=False):
-        """A request from a ``wavebit.job/v2`` document (``wavebit_amd.jobs.Job``); the job's
-        execution must name this server's kind of device."""
+        """A request from a ``wavebit.job/v2`` document (``wavebit_amd.jobs.Job``). The server
+        runs every request on its own device, so the job's execution settings (the reference's
+        default is ``backend='numpy'``) are not consulted."""
         from ..jobs import Job
         job = text if isinstance(text, Job) else Job.from_json(text)
-        job.execution.validate()
         return cls(problem=job.problem, numerics=job.numerics, arrival=arrival, id=id, gradient=gradient)
 
 
