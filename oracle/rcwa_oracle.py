@@ -314,12 +314,13 @@ def s_transmission(W_bot, V_bot, W_gap, V_gap):
     return B @ A_i, 0.5 * (A - B @ A_i @ B), 2 * A_i, -A_i @ B
 
 
-def published_scattering(S, sign, flip=(1.0, 1.0, 1.0, 1.0)):
+def published_scattering(S, sign, flip=(1.0, 1.0, 1.0, 1.0), reverse=True):
     """The stack's scattering matrix as the reference publishes it for top incidence: the kernel's
     blocks (start port = scene bottom) flipped per block (``flip``: the 1D TM reflections,
     ``scattering.py:627-656``), reversed into incidence-relative order (``:607-624``) and signed
     per port amplitude by ``sign`` (frame sign x cell-centre sign, ``engine.py:2995-3009``)."""
-    blocks = [f * (sign[:, None] * b * sign[None, :]) for f, b in zip(flip, S)][::-1]
+    blocks = [f * (sign[:, None] * b * sign[None, :]) for f, b in zip(flip, S)]
+    blocks = blocks[::-1] if reverse else blocks     # bottom incidence: the kernel's own order
     return dict(zip(('S11', 'S12', 'S21', 'S22'), blocks))
 
 
@@ -392,7 +393,7 @@ def _centre_parity(hu, hv):
 
 
 def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, theta, hu,
-           phi=None, factorization='li', coefficients='sampled'):
+           phi=None, factorization='li', coefficients='sampled', side='top'):
     """Pure 1D TE/TM solve of a one-row raster stack, top incidence.
 
     Restates ``scattering.py:704-902`` as the engine calls it (``engine.py:3283-3316``): kernel
@@ -400,7 +401,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     ``eps_layers`` is ``(L, 1, Nx)`` bottom-to-top; returns per-order arrays of shape
     ``(2*hu+1,)``.
     """
-    k_inc_x, _ = incident_kxy(eps_top, theta, phi)
+    k_inc_x, _ = incident_kxy(eps_top if side == 'top' else eps_bot, theta, phi)
     kx = (k_inc_x + np.arange(-hu, hu + 1) * (wavelength / period_x)).astype(np.complex128)
     N = kx.shape[0]
     I = np.eye(N, dtype=np.complex128)
@@ -426,19 +427,23 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
         A, B = ab_matrices(W, V, I, V_gap)
         S = redheffer_star(I, S, s_layer(A, B, X))
     S = redheffer_star(I, S, S_trn)
+    top = side == 'top'
     scattering = published_scattering(S, sigma * _centre_parity(hu, 0).reshape(-1),
-                                      (-1.0, 1.0, 1.0, -1.0) if tetm == 1 else (1.0, 1.0, 1.0, 1.0))
-    scattering.update(kx=kx, ky=None, n_ref=np.sqrt(np.complex128(eps_top)),
-                      n_trn=np.sqrt(np.complex128(eps_bot)), channel='tm' if tetm == 1 else 'te')
+                                      (-1.0, 1.0, 1.0, -1.0) if tetm == 1 else (1.0, 1.0, 1.0, 1.0), top)
+    n_in, n_out = np.sqrt(np.complex128(eps_top)), np.sqrt(np.complex128(eps_bot))
+    scattering.update(kx=kx, ky=None, n_ref=n_in if top else n_out, n_trn=n_out if top else n_in,
+                      channel='tm' if tetm == 1 else 'te')
     delta = np.zeros(N, dtype=np.complex128)
     delta[N // 2] = 1.0
     delta = delta * sigma[N // 2]
-    R = S[3] @ delta
-    Tt = S[1] @ delta
-    # backside: ref = kernel bottom (scene top), trn = kernel top (scene bottom)
-    kz_ref, kz_trn = kz_kb, kz_kt
-    n_ref, n_trn = n_kb, n_kt
-    e_ref, e_trn = e_kb, e_kt
+    # top incidence reads the kernel's backside (S22 / S12, ref = kernel bottom = scene top);
+    # bottom incidence its front side (S11 / S21, ref = kernel top = scene bottom)
+    # (scattering.py:943-949, the backside flag)
+    R = (S[3] if top else S[0]) @ delta
+    Tt = (S[1] if top else S[2]) @ delta
+    kz_ref, kz_trn = (kz_kb, kz_kt) if top else (kz_kt, kz_kb)
+    n_ref, n_trn = (n_kb, n_kt) if top else (n_kt, n_kb)
+    e_ref, e_trn = (e_kb, e_kt) if top else (e_kt, e_kb)
     adm_ref = kz_ref if tetm == 0 else kz_ref / e_ref
     adm_trn = kz_trn if tetm == 0 else kz_trn / e_trn
     incoming = adm_ref[N // 2].real
@@ -481,12 +486,13 @@ def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv):
 
 
 def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
-           return_modes=False):
+           return_modes=False, side='top'):
     """Coupled s/p 2D solve, top incidence (``scattering.py:1185-1456``, ``:907-994``, as called
     by ``engine.py:3402-3420``). ``conv`` is a list of ``(epx, epy, epz_i)`` bottom-to-top;
     ``pol`` the unit pair ``(a_s, a_p)``. Returns the three incidence variants
     (``incident``, ``s_inc``, ``p_inc``), each with per-order arrays ``(2hv+1, 2hu+1)``."""
-    kx, ky = order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv)
+    top = side == 'top'
+    kx, ky = order_kxy_2d(wavelength, period, eps_top if top else eps_bot, theta, phi, hu, hv)
     nu, nv = 2 * hu + 1, 2 * hv + 1
     N = nu * nv
     if kx.ndim == 2:                    # skewed cell: the grids themselves
@@ -526,13 +532,15 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
             modes.append((W, V, q))
     S = redheffer_star(bI, S, S_trn)
     par = _centre_parity(hu, hv).reshape(-1)
-    scattering = published_scattering(S, np.concatenate([par, par]))
-    scattering.update(kx=kx, ky=ky, n_ref=np.sqrt(np.complex128(eps_top)),
-                      n_trn=np.sqrt(np.complex128(eps_bot)), channel='sp')
-    # backside read: reflection S22, transmission S12
-    kz_ref, kz_trn = kz_kb, kz_kt
-    n_ref, n_trn = n_kb, n_kt
-    e_ref, e_trn = e_kb, e_kt
+    scattering = published_scattering(S, np.concatenate([par, par]), reverse=top)
+    n_in, n_out = np.sqrt(np.complex128(eps_top)), np.sqrt(np.complex128(eps_bot))
+    scattering.update(kx=kx, ky=ky, n_ref=n_in if top else n_out, n_trn=n_out if top else n_in,
+                      channel='sp')
+    # top incidence: the backside read (reflection S22, transmission S12); bottom: the front
+    # side (S11, S21) -- scattering.py:943-949
+    kz_ref, kz_trn = (kz_kb, kz_kt) if top else (kz_kt, kz_kb)
+    n_ref, n_trn = (n_kb, n_kt) if top else (n_kt, n_kb)
+    e_ref, e_trn = (e_kb, e_kt) if top else (e_kt, e_kb)
     delta = np.zeros(N, dtype=np.complex128)
     delta[N // 2] = 1.0
 
@@ -543,8 +551,8 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
     cols = [incident(a_s, a_p), incident(np.sin(np.complex128(_PI / 2)), np.cos(np.complex128(_PI / 2))),
             incident(np.sin(np.complex128(0.0)), np.cos(np.complex128(0.0)))]
     c_inc = np.stack(cols, axis=1)
-    c_ref = S[3] @ c_inc
-    c_trn = S[1] @ c_inc
+    c_ref = (S[3] if top else S[0]) @ c_inc
+    c_trn = (S[1] if top else S[2]) @ c_inc
     s_power = kz_ref[N // 2].real
     p_power = abs(n_ref) ** 2 * (kz_ref[N // 2] / e_ref).real
     powers = [abs(a_s) ** 2 * s_power + abs(a_p) ** 2 * p_power, s_power, p_power]
@@ -579,7 +587,7 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
 
 
 def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, theta, phi,
-                 polarization, orders, factorization='li', coefficients='sampled'):
+                 polarization, orders, factorization='li', coefficients='sampled', side='top'):
     """One plane-wave point through a raster stack, routed the way the reference routes it.
 
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
@@ -597,7 +605,7 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         if one_row and phi is None and hv == 0:
             tetm = 0 if polarization == 's' else 1
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
-                         theta, hu, phi, factorization, coefficients)
+                         theta, hu, phi, factorization, coefficients, side)
             inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
                    if k not in ('kx', 'power_entering', 'scattering')}
             inc['power_entering'] = out['power_entering']
@@ -610,6 +618,7 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
     if one_row and hv != 0:
         raise ValueError('a one-row raster has no y orders')
     conv = [rule(e, hu, hv) for e in eps_layers]
-    out = smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv)
+    out = smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
+                 side=side)
     out['path'] = '2d'
     return out

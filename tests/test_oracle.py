@@ -366,3 +366,31 @@ def test_scattering_of_a_stack_is_the_cascade_of_its_pieces(pol, M, phi, shape):
     scale = max(np.abs(whole[k]).max() for k in joined)
     for k in joined:
         assert np.abs(joined[k] - whole[k]).max() < 1e-12 * scale, k
+
+
+# ------------------------------------------------------------------------------ bottom incidence
+
+@pytest.mark.parametrize('pol,M,phi,shape', [((0.6, 0.8j), (2, 2), 0.4, (2, 5, 6)), ('s', (5, 0), None, (2, 1, 8)),
+                                             ('p', (5, 0), None, (2, 1, 8))])
+def test_bottom_incidence_is_top_incidence_on_the_mirrored_stack(pol, M, phi, shape):
+    """The front-side read of the kernel cascade (reference scattering.py:943-949, backside off)
+    equals the backside read of the stack mirrored in z -- half-spaces swapped, layers reversed:
+    every order's efficiencies and amplitudes, and the published scattering blocks."""
+    rng = np.random.default_rng(23)
+    eps = rng.uniform(1.5, 9.0, shape) + 1j * rng.uniform(0.0, 0.3, shape)
+    d = [0.12, 0.2]
+    top, bot, theta, wl, period = 1.44, 2.25 + 0.05j, 0.25, 0.83, (0.7, 0.65)
+    up = orc.solve_raster(eps, d, top, bot, period, wl, theta, phi, pol, M, side='bottom')
+    mirror = orc.solve_raster(eps[::-1], d[::-1], bot, top, period, wl, theta, phi, pol, M)
+    for k in ('R', 'T', 'r_s', 'r_p', 't_s', 't_p'):
+        assert np.abs(np.asarray(up['incident'][k]) - np.asarray(mirror['incident'][k])).max() < 1e-12, k
+    for k in ('S11', 'S12', 'S21', 'S22'):
+        assert np.abs(up['scattering'][k] - mirror['scattering'][k]).max() < 1e-12, k
+
+
+def test_bottom_incidence_conserves_energy_on_a_lossless_stack():
+    rng = np.random.default_rng(29)
+    eps = rng.uniform(1.5, 9.0, (2, 6, 6))
+    res = orc.solve_raster(eps, [0.15, 0.2], 1.0, 2.25, (0.8, 0.75), 0.9, 0.2, 0.5, (0.8, 0.6j), (2, 2),
+                           side='bottom')['incident']
+    assert abs(res['R'].sum() + res['T'].sum() - 1.0) < 1e-12

@@ -161,6 +161,7 @@ class _Resolved:
     coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
     scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
     thetas: tuple = None             # per-point effective theta after a detuning ladder
+    side: str = 'top'                # the lit half-space; 'bottom' is solved on the mirrored stack
 
 
 def _resolve(problem, numerics, execution):
@@ -212,9 +213,6 @@ def _resolve(problem, numerics, execution):
         raise UnsupportedFeatureError('One incident plane wave per problem is implemented.',
                                       code='rcwa.unsupported', path='excitation.sources')
     src = sources[0]
-    if src.side != 'top':
-        raise UnsupportedFeatureError('Bottom incidence is not implemented by this engine.',
-                                      code='rcwa.unsupported', path='excitation.sources.side')
     if src.kx is not None or src.ky is not None:
         raise UnsupportedFeatureError('Direct kx/ky incidence is not implemented; use theta/phi.',
                                       code='rcwa.unsupported', path='excitation.sources')
@@ -272,6 +270,13 @@ def _resolve(problem, numerics, execution):
             raise ValidationError('Jones polarization must have exactly two components.',
                                   code='source.polarization_shape',
                                   path='excitation.sources.polarization')
+    if src.side == 'bottom':
+        # Bottom incidence reads the kernel cascade's front side (reference scattering.py:943-949
+        # with the backside flag off). The gap-embedded layer S-matrices are symmetric, so that
+        # is the top-incidence read of the stack mirrored in z: the half-spaces swapped and the
+        # layers in the opposite order, every order's kx / ky unchanged.
+        rasters, thickness = rasters[::-1].copy(), thickness[::-1].copy()
+        eps_top, eps_bottom = eps_bottom, eps_top
     n_rows = rasters.shape[1] if not uniform else 2
     path = _engine.route(n_rows, pol, phi, hv)
     if skewed and path != _engine.PATH_2D:
@@ -283,7 +288,7 @@ def _resolve(problem, numerics, execution):
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
                      hu=hu, hv=hv, uniform=uniform, factorization=factorization,
                      coefficients=numerics.fourier_coefficients or 'sampled',
-                     scattering=bool(problem.outputs.scattering))
+                     scattering=bool(problem.outputs.scattering), side=src.side)
 
 
 def _cell(lattice):
@@ -464,7 +469,8 @@ class Prepared:
                 'effective_wavelengths': tuple(float(w) for w in r.wavelengths),
                 'effective_theta': r.thetas or tuple(r.source.theta for _ in r.wavelengths),
                 'exceptional_points': ep_counts}
-        result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats)
+        result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats,
+                      side=r.side)
         if not result.validity.checks['numerical_finite']:
             raise NumericalError('The requested solve produced nonfinite observables; no implicit '
                                  'detuning was applied.', code='rcwa.nonfinite', path='result')
@@ -500,6 +506,8 @@ def _run_live(prepared, problem, r, checks):
     eps = torch.stack([e.to(dtype=dt) for e in eps_list])
     thick = torch.stack([l.thickness.to(torch.float64).reshape(()) if torch.is_tensor(l.thickness)
                          else torch.tensor(float(l.thickness), dtype=torch.float64) for l in layers])
+    if r.side == 'bottom':                   # the mirrored stack (see _resolve); flip is differentiable
+        eps, thick = torch.flip(eps, [0]), torch.flip(thick, [0])
     points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                             period=r.period, thickness=tuple(float(t) for t in r.thickness))
@@ -522,7 +530,7 @@ def _run_live(prepared, problem, r, checks):
             'numerics': prepared.numerics}
     diag = {'report': prepared.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
             'gradient': 'hand-written adjoint (raster permittivity, layer thickness)'}
-    result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov)
+    result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, side=r.side)
     if not result.validity.checks['numerical_finite']:
         raise NumericalError('The requested solve produced nonfinite observables.',
                              code='rcwa.nonfinite', path='result')

@@ -137,7 +137,8 @@ class Result:
         return self._scattering
 
 
-def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, scattering=None):
+def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, scattering=None,
+         side='top'):
     """Stack per-wavelength solve outputs (dicts from the engine) into a ``Result``.
 
     NaN-masks every masked output when a validity check failed (reference
@@ -154,6 +155,7 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, s
     combined = bool(finite and all(bool(v) for v in checks.values()))
     scale = 1.0 if combined else float('nan')
     outputs = {}
+    other = 'bottom' if side == 'top' else 'top'
     for name, prefix, eff in (('reflected', 'r', 'R'), ('transmitted', 't', 'T')):
         outputs[name] = DiffractionOrders(
             wavelengths=ws, orders_u=orders_u, orders_v=orders_v,
@@ -162,8 +164,8 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, s
             efficiency=stack(eff) * scale,
             efficiency_s=stack(eff + '_s') * scale,
             efficiency_p=stack(eff + '_p') * scale,
-            reference={'port': 'top' if name == 'reflected' else 'bottom', 'wave': name,
-                       'incident_port': 'top', 'polarization_frame': "each order's outgoing s/p frame",
+            reference={'port': side if (name == 'reflected') else other, 'wave': name,
+                       'incident_port': side, 'polarization_frame': "each order's outgoing s/p frame",
                        'time_convention': 'exp(-i*omega*t)'})
     outputs['power_entering'] = np.array([float(d['incident']['power_entering']) for d in per_wavelength]) * scale
     outputs['absorption'] = outputs['power_entering'] - outputs['transmitted'].efficiency.sum(axis=(1, 2))

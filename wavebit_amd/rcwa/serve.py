@@ -195,6 +195,8 @@ class Server:
                                                 factorization=r.factorization)
             (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
             d_eps, d_t = eps.grad.detach().cpu().numpy(), thick.grad.detach().cpu().numpy()
+            if r.side == 'bottom':               # solved on the mirrored stack: back to the declared order
+                d_eps, d_t = d_eps[::-1].copy(), d_t[::-1].copy()
             eff_h, amp_h, pe_h, fl = (t.detach().cpu().numpy() for t in (eff, amp, pe, flags))
         for i in range(work.n_points):
             work.flags |= int(np.bitwise_or.reduce(fl[i]))
@@ -255,7 +257,7 @@ class Server:
                 'exceptional_points': flag_notice(work.flags, work.req.numerics)[1]}
         prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 serve',
                 'length_units': work.req.problem.units.length}
-        return pack(per, r.hu, r.hv, r.wavelengths, work.checks, diag, prov)
+        return pack(per, r.hu, r.hv, r.wavelengths, work.checks, diag, prov, side=r.side)
 
     # ------------------------------------------------------------------ the loop
     def serve(self, requests, realtime=True):
