@@ -393,7 +393,7 @@ def _centre_parity(hu, hv):
 
 
 def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, theta, hu,
-           phi=None, factorization='li', coefficients='sampled', side='top'):
+           phi=None, factorization='li', coefficients='sampled', side='top', k_direct=None):
     """Pure 1D TE/TM solve of a one-row raster stack, top incidence.
 
     Restates ``scattering.py:704-902`` as the engine calls it (``engine.py:3283-3316``): kernel
@@ -401,7 +401,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     ``eps_layers`` is ``(L, 1, Nx)`` bottom-to-top; returns per-order arrays of shape
     ``(2*hu+1,)``.
     """
-    k_inc_x, _ = incident_kxy(eps_top if side == 'top' else eps_bot, theta, phi)
+    k_inc_x, _ = incident_k(eps_top if side == 'top' else eps_bot, theta, phi, wavelength, k_direct)
     kx = (k_inc_x + np.arange(-hu, hu + 1) * (wavelength / period_x)).astype(np.complex128)
     N = kx.shape[0]
     I = np.eye(N, dtype=np.complex128)
@@ -467,10 +467,17 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     return out
 
 
-def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv):
+def incident_k(eps_inc, theta, phi, wavelength, k_direct=None):
+    """``incident_kxy``, or direct physical wavenumbers over k0 (``engine.py:2755-2758``)."""
+    if k_direct is not None:
+        return k_direct[0] * wavelength / (2 * _PI), k_direct[1] * wavelength / (2 * _PI)
+    return incident_kxy(eps_inc, theta, phi)
+
+
+def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv, k_direct=None):
     """Separable ``kx``, ``ky`` (``engine.py:953-993``); on a skewed cell, ``period`` a 2 x 2
     lattice whose columns are a1 and a2, the two (2hv+1, 2hu+1) grids of ``lattice.py:899-944``."""
-    k_inc_x, k_inc_y = incident_kxy(eps_top, theta, phi)
+    k_inc_x, k_inc_y = incident_k(eps_top, theta, phi, wavelength, k_direct)
     A = np.asarray(period, dtype=np.float64)
     if A.shape == (2, 2):
         a1x, a2x, a1y, a2y = A[0, 0], A[0, 1], A[1, 0], A[1, 1]
@@ -486,13 +493,13 @@ def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv):
 
 
 def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
-           return_modes=False, side='top'):
+           return_modes=False, side='top', k_direct=None):
     """Coupled s/p 2D solve, top incidence (``scattering.py:1185-1456``, ``:907-994``, as called
     by ``engine.py:3402-3420``). ``conv`` is a list of ``(epx, epy, epz_i)`` bottom-to-top;
     ``pol`` the unit pair ``(a_s, a_p)``. Returns the three incidence variants
     (``incident``, ``s_inc``, ``p_inc``), each with per-order arrays ``(2hv+1, 2hu+1)``."""
     top = side == 'top'
-    kx, ky = order_kxy_2d(wavelength, period, eps_top if top else eps_bot, theta, phi, hu, hv)
+    kx, ky = order_kxy_2d(wavelength, period, eps_top if top else eps_bot, theta, phi, hu, hv, k_direct)
     nu, nv = 2 * hu + 1, 2 * hv + 1
     N = nu * nv
     if kx.ndim == 2:                    # skewed cell: the grids themselves
@@ -587,7 +594,8 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
 
 
 def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, theta, phi,
-                 polarization, orders, factorization='li', coefficients='sampled', side='top'):
+                 polarization, orders, factorization='li', coefficients='sampled', side='top',
+                 k_direct=None):
     """One plane-wave point through a raster stack, routed the way the reference routes it.
 
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
@@ -602,10 +610,10 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
     hu, hv = orders
     one_row = eps_layers.shape[1] == 1
     if isinstance(polarization, str):
-        if one_row and phi is None and hv == 0:
+        if one_row and phi is None and hv == 0 and (k_direct is None or k_direct[1] == 0):
             tetm = 0 if polarization == 's' else 1
             out = smm_1d(tetm, wavelength, period[0], eps_layers, thickness, eps_top, eps_bot,
-                         theta, hu, phi, factorization, coefficients, side)
+                         theta, hu, phi, factorization, coefficients, side, k_direct)
             inc = {k: np.asarray(v).reshape(1, -1) for k, v in out.items()
                    if k not in ('kx', 'power_entering', 'scattering')}
             inc['power_entering'] = out['power_entering']
@@ -619,6 +627,6 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         raise ValueError('a one-row raster has no y orders')
     conv = [rule(e, hu, hv) for e in eps_layers]
     out = smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
-                 side=side)
+                 side=side, k_direct=k_direct)
     out['path'] = '2d'
     return out

@@ -140,3 +140,27 @@ def test_bottom_incidence_matches_the_oracle(pol, M, phi, Ny):
            't_p': res.transmission.amplitude_ratio_p[0]}
     for k, v in got.items():
         assert np.abs(np.asarray(v).reshape(-1) - np.asarray(want[k]).reshape(-1)).max() < 1e-10, k
+
+
+@pytest.mark.parametrize('k,pol,Ny,M', [((3.1, 1.2), (0.6, 0.8j), 6, (3, 2)), ((2.5, None), 'p', 1, (6, 0))])
+def test_direct_wavenumber_incidence_matches_the_oracle(k, pol, Ny, M):
+    """PlaneWave(kx=, ky=) in rad/um over two wavelengths: the same physical in-plane wavevector
+    is a different angle at each, per order within 1e-10 of the oracle; a one-row raster with
+    ky unset takes the pure 1D path."""
+    import wavebit_amd
+    from wavebit_amd import rcwa
+    rng = np.random.default_rng(41)
+    eps = (rng.uniform(1.5, 9.0, (2, Ny, 8)) + 1j * rng.uniform(0.0, 0.3, (2, Ny, 8))).astype(np.complex128)
+    d, wls = [0.12, 0.2], [0.83, 0.95]
+    problem = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=d[i], epsilon=eps[i]) for i in range(2)],
+                         top=wavebit_amd.Material(1.44), bottom=wavebit_amd.Material(2.25), period=(0.7, 0.65)),
+        excitation=rcwa.Excitation(wavelength=wls, sources=(rcwa.PlaneWave(polarization=pol, kx=k[0], ky=k[1]),)),
+        units=wavebit_amd.Units(length='um'))
+    res = rcwa.run(problem, numerics=rcwa.Numerics(orders=M), backend='torch', device='cuda:0')
+    assert res.diagnostics['solve_path'][0] == ('2d' if Ny > 1 else '1d_tm')
+    kd = (k[0], 0.0 if k[1] is None else k[1])
+    for i, wl in enumerate(wls):
+        want = orc.solve_raster(eps, d, 1.44, 2.25, (0.7, 0.65), wl, None, None, pol, M, k_direct=kd)['incident']
+        assert np.abs(np.asarray(res.R)[i].reshape(-1) - np.asarray(want['R']).reshape(-1)).max() < 1e-10
+        assert np.abs(np.asarray(res.T)[i].reshape(-1) - np.asarray(want['T']).reshape(-1)).max() < 1e-10

@@ -394,3 +394,21 @@ def test_bottom_incidence_conserves_energy_on_a_lossless_stack():
     res = orc.solve_raster(eps, [0.15, 0.2], 1.0, 2.25, (0.8, 0.75), 0.9, 0.2, 0.5, (0.8, 0.6j), (2, 2),
                            side='bottom')['incident']
     assert abs(res['R'].sum() + res['T'].sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize('pol,M,phi,shape', [((0.6, 0.8j), (2, 2), 0.4, (2, 5, 6)), ('p', (5, 0), None, (2, 1, 8))])
+def test_direct_wavenumbers_are_the_angles_they_spell(pol, M, phi, shape):
+    """PlaneWave(kx=, ky=) in radians per length unit is k0 times the angles' kx/k0, ky/k0
+    (reference engine.py:2755-2758): the same solve to round-off."""
+    rng = np.random.default_rng(37)
+    eps = rng.uniform(1.5, 9.0, shape) + 1j * rng.uniform(0.0, 0.3, shape)
+    top, theta, wl = 1.44, 0.3, 0.83
+    n = np.sqrt(top)
+    k0 = 2 * np.pi / wl
+    p = 0.0 if phi is None else phi
+    k = (n * np.sin(theta) * np.cos(p) * k0, n * np.sin(theta) * np.sin(p) * k0)
+    a = orc.solve_raster(eps, [0.12, 0.2], top, 2.25, (0.7, 0.65), wl, theta, phi, pol, M)['incident']
+    b = orc.solve_raster(eps, [0.12, 0.2], top, 2.25, (0.7, 0.65), wl, None, None if phi is None else None, pol, M,
+                         k_direct=k)['incident']
+    for key in ('R', 'T'):
+        assert np.abs(np.asarray(a[key]) - np.asarray(b[key])).max() < 1e-12, key

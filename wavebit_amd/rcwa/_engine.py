@@ -90,6 +90,16 @@ class Point:
     eps_bottom: complex
     period: tuple
     thickness: tuple
+    k_direct: object = None  # (kx, ky) in radians per length unit: PlaneWave(kx=, ky=)
+
+
+def incident_k(eps_inc, theta, phi, wavelength, k_direct=None):
+    """Incident in-plane wavevector / k0: from the angles (``incident_kxy``), or from direct
+    physical wavenumbers divided by k0 = 2 pi / wavelength (reference engine.py:2755-2758)."""
+    if k_direct is not None:
+        scale = float(wavelength) / (2 * _PI)
+        return k_direct[0] * scale, k_direct[1] * scale
+    return incident_kxy(eps_inc, theta, phi)
 
 
 @dataclass
@@ -129,7 +139,7 @@ class ConvCache:
 
 def point_record(pt, path):
     """The WB_PT_* record of one point."""
-    kx0, ky0 = incident_kxy(pt.eps_top, pt.theta, pt.phi)
+    kx0, ky0 = incident_k(pt.eps_top, pt.theta, pt.phi, pt.wavelength, pt.k_direct)
     a_s, a_p = jones(pt.polarization) if path == PATH_2D else (1.0 + 0j, 0.0 + 0j)
     eps_t, eps_b = complex(pt.eps_top), complex(pt.eps_bottom)
     py = pt.period[1] if len(pt.period) > 1 else 1.0

@@ -162,6 +162,7 @@ class _Resolved:
     scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
     thetas: tuple = None             # per-point effective theta after a detuning ladder
     side: str = 'top'                # the lit half-space; 'bottom' is solved on the mirrored stack
+    k_direct: tuple = None           # PlaneWave(kx=, ky=): radians per length unit
 
 
 def _resolve(problem, numerics, execution):
@@ -213,9 +214,11 @@ def _resolve(problem, numerics, execution):
         raise UnsupportedFeatureError('One incident plane wave per problem is implemented.',
                                       code='rcwa.unsupported', path='excitation.sources')
     src = sources[0]
+    k_direct = None
     if src.kx is not None or src.ky is not None:
-        raise UnsupportedFeatureError('Direct kx/ky incidence is not implemented; use theta/phi.',
-                                      code='rcwa.unsupported', path='excitation.sources')
+        k_direct = tuple(0.0 if v is None else _scalar(v, 'excitation.sources.kx' if i == 0 else
+                                                      'excitation.sources.ky')
+                         for i, v in enumerate((src.kx, src.ky)))
     theta = None if src.theta is None else _scalar(src.theta, 'excitation.sources.theta')
     phi = None if src.phi is None else _scalar(src.phi, 'excitation.sources.phi')
     layers = stack.layers
@@ -278,7 +281,9 @@ def _resolve(problem, numerics, execution):
         rasters, thickness = rasters[::-1].copy(), thickness[::-1].copy()
         eps_top, eps_bottom = eps_bottom, eps_top
     n_rows = rasters.shape[1] if not uniform else 2
-    path = _engine.route(n_rows, pol, phi, hv)
+    # a direct ky off zero is conical: route it as an azimuth would be
+    route_phi = 0.0 if (k_direct is not None and k_direct[1] != 0.0) else phi
+    path = _engine.route(n_rows, pol, route_phi, hv)
     if skewed and path != _engine.PATH_2D:
         raise UnsupportedFeatureError('A skewed lattice is solved on the 2D coupled path: give a '
                                       'Jones polarization or phi, or more than one raster row.',
@@ -288,7 +293,7 @@ def _resolve(problem, numerics, execution):
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
                      hu=hu, hv=hv, uniform=uniform, factorization=factorization,
                      coefficients=numerics.fourier_coefficients or 'sampled',
-                     scattering=bool(problem.outputs.scattering), side=src.side)
+                     scattering=bool(problem.outputs.scattering), side=src.side, k_direct=k_direct)
 
 
 def _cell(lattice):
@@ -376,7 +381,7 @@ def check(problem, *, numerics, backend='numpy', device='cpu', precision='double
     execution = Execution(backend=backend, device=device, precision=precision)
     try:
         r = _resolve(problem, numerics, execution)
-        _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
+        _engine.incident_k(r.eps_top, r.source.theta, r.source.phi, r.wavelengths[0], r.k_direct)
         return _report(problem, numerics, execution, r)
     except WavebitError as error:
         return CheckReport(issues=(error.issue,))
@@ -510,7 +515,7 @@ def _run_live(prepared, problem, r, checks):
         eps, thick = torch.flip(eps, [0]), torch.flip(thick, [0])
     points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
-                            period=r.period, thickness=tuple(float(t) for t in r.thickness))
+                            period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
               for wl in r.wavelengths]
     eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device,
                                         factorization=r.factorization)
@@ -672,7 +677,7 @@ def _solve_resolved(r, cache):
         bucket.points.append(_engine.Point(
             stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
-            period=r.period, thickness=tuple(float(t) for t in r.thickness)))
+            period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct))
     smats = None
     if r.scattering:
         eff, amp, pe, flags, _, smat = _engine.run_bucket_scattering(bucket, [conv], device)
@@ -690,10 +695,10 @@ def _scattering_matrices(r, smat):
     the coupled path), the top medium's index at the entry, the bottom medium's at the exit."""
     from .scattering import Port, ScatteringMatrix
     channel = {_engine.PATH_2D: 'sp', _engine.PATH_TE: 'te', _engine.PATH_TM: 'tm'}[r.path]
-    kx0, ky0 = _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
     n_top, n_bot = np.sqrt(complex(r.eps_top)), np.sqrt(complex(r.eps_bottom))
     out = []
     for p, wl in enumerate(r.wavelengths):
+        kx0, ky0 = _engine.incident_k(r.eps_top, r.source.theta, r.source.phi, wl, r.k_direct)
         kx, ky = _engine.order_kxy(kx0, ky0, float(wl), r.period, r.hu, r.hv)
         ky = ky if channel == 'sp' else None
         S = smat[p]
@@ -708,7 +713,7 @@ def prepare(problem, *, numerics, backend='numpy', device='cpu', precision='doub
     execution = Execution(backend=backend, device=device, precision=precision)
     try:
         r = _resolve(problem, numerics, execution)
-        _engine.incident_kxy(r.eps_top, r.source.theta, r.source.phi)
+        _engine.incident_k(r.eps_top, r.source.theta, r.source.phi, r.wavelengths[0], r.k_direct)
     except WavebitError:
         raise
     except NotImplementedError as error:

@@ -157,7 +157,7 @@ class Server:
             pts.append((work, i, _engine.Point(
                 stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                 polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
-                period=r.period, thickness=tuple(float(t) for t in r.thickness)), conv))
+                period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct), conv))
         key = (r.hu, r.hv, r.path, r.rasters.shape[0])
         return work, key, pts
 
@@ -184,7 +184,7 @@ class Server:
                 f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
         points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                                 polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
-                                period=r.period, thickness=tuple(float(t) for t in r.thickness))
+                                period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
                   for wl in r.wavelengths]
         with torch.cuda.stream(stream):
             eps = torch.tensor(np.ascontiguousarray(r.rasters), dtype=torch.complex128, device=self.device,
