@@ -111,3 +111,29 @@ def test_a_job_document_runs_and_serves_like_its_problem():
     with serve.Server(streams=1) as srv:
         results, _ = srv.serve([serve.Request.from_job(text, arrival=0.0, id=0)], realtime=False)
     assert np.abs(np.asarray(results[0].T) - np.asarray(alone.T)).max() < 1e-12
+
+
+def test_bottom_lit_and_direct_wavenumber_requests_serve_like_their_runs_alone():
+    import wavebit_amd
+    rng = np.random.default_rng(12)
+    eps = rng.uniform(1, 9, (2, 12, 12))
+    sources = [rcwa.PlaneWave(polarization='s', theta=0.2, phi=0.3),
+               rcwa.PlaneWave(polarization=(0.6, 0.8j), theta=0.25, phi=0.1, side='bottom'),
+               rcwa.PlaneWave(polarization='p', kx=2.0, ky=1.0),
+               rcwa.PlaneWave(polarization='p', kx=1.5, ky=-0.5, side='bottom')]
+    reqs, alone = [], []
+    numerics = rcwa.Numerics(orders=(3, 3))
+    for i, src in enumerate(sources):
+        problem = rcwa.Problem(
+            stack=rcwa.Stack(layers=[rcwa.Layer(thickness=t, epsilon=e) for t, e in zip((0.2, 0.12), eps)],
+                             top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25), period=(0.8, 0.7)),
+            excitation=rcwa.Excitation(wavelength=[0.8, 0.9], sources=(src,)), units=wavebit_amd.Units(length='um'))
+        reqs.append(serve.Request(problem=problem, numerics=numerics, arrival=0.0, id=i))
+        alone.append(rcwa.run(problem, numerics=numerics, backend='torch', device='cuda:0'))
+    with serve.Server(streams=2, max_points_per_launch=8) as srv:
+        results, report = srv.serve(reqs, realtime=False)
+    for i in range(len(reqs)):
+        assert not isinstance(results[i], Exception), results[i]
+        assert results[i].reflection.reference['incident_port'] == sources[i].side
+        assert np.abs(np.asarray(results[i].R) - np.asarray(alone[i].R)).max() < 1e-10
+        assert np.abs(np.asarray(results[i].T) - np.asarray(alone[i].T)).max() < 1e-10

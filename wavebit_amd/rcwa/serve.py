@@ -23,6 +23,7 @@ this is the batching layer around the same per-point arithmetic, so a request's 
 same alone and in a mix.
 """
 import collections
+import dataclasses
 import concurrent.futures
 import hashlib
 import heapq
@@ -223,10 +224,7 @@ class Server:
             if cid not in index:
                 index[cid] = len(convs)
                 convs.append(conv)
-            pts.append(_engine.Point(stack=index[cid], wavelength=pt.wavelength, theta=pt.theta,
-                                     phi=pt.phi, polarization=pt.polarization, eps_top=pt.eps_top,
-                                     eps_bottom=pt.eps_bottom, period=pt.period,
-                                     thickness=pt.thickness))
+            pts.append(dataclasses.replace(pt, stack=index[cid]))   # every other field as admitted
         bucket = _engine.Bucket(hu=hu, hv=hv, path=path, n_layers=L, points=pts)
         with torch.cuda.stream(stream):
             eff, amp, pe, flags, _ = _engine.run_bucket(bucket, convs, self.device)
