@@ -154,6 +154,20 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
              const double* epz_i, double* eff, double* amp, double* pe, int* flags,
              void* workspace, size_t workspace_bytes, void* stream);
 
+/* wb_solve without a workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart
+ * in the reference (which allocates through NumPy/PyTorch on every call). The solve runs on a
+ * device array of WB_SOLVE_NOALLOC_BYTES that the library holds on each device it runs on until
+ * the process ends; the call allocates nothing. Problems whose wb_solve_workspace_size exceeds
+ * WB_SOLVE_NOALLOC_BYTES are refused with WB_OOM. Every call on one device shares that device's
+ * array: calls that overlap on one device (two streams or two threads) race, so serialize them.
+ * Arguments and outputs are wb_solve's. */
+#define WB_SOLVE_NOALLOC_BYTES (64u << 20)
+int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                     const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     void* stream);
+
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
  * is stored k x m, B n x k). */
 /* wb_solve plus the stack's scattering matrix per point: smat is P x 4 x n x n complex,

@@ -32,7 +32,7 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_scattering', 'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
-           'wb_gemm_tuning_clear')
+           'wb_gemm_tuning_clear', 'wb_solve_noalloc')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -78,6 +78,8 @@ def load_library():
             'wb_solve_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
                                              c_int, c_ll, c_vp, c_int, c_ll, c_dp, c_vp, c_int,
                                              c_ll, c_int, c_vp]),
@@ -325,6 +327,39 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
                                _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(eff),
                                _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
                                _stream(device)), 'solve')
+    return eff, amp, pe, flags
+
+
+SOLVE_NOALLOC_BYTES = 64 << 20   # WB_SOLVE_NOALLOC_BYTES
+
+
+def solve_noalloc(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, out=None):
+    """``solve`` through ``wb_solve_noalloc``: the engine runs on its own fixed device array, and
+    with ``out`` (the ``(eff, amp, pe, flags)`` tensors of an earlier call of the same shapes)
+    nothing is allocated on the device for the call. Returns (eff, amp, pe, flags)."""
+    device = epx.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    if out is None:
+        out = (torch.empty((P, 3, 4, N), dtype=torch.float64, device=device),
+               torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device),
+               torch.empty((P, 3), dtype=torch.float64, device=device),
+               torch.empty((P, L), dtype=torch.int32, device=device))
+    eff, amp, pe, flags = out
+    for name, t, shape in (('eff', eff, (P, 3, 4, N)), ('amp', amp, (P, 3, 4, N)),
+                           ('pe', pe, (P, 3)), ('flags', flags, (P, L))):
+        if tuple(t.shape) != shape:
+            raise ValueError(f'{name}: expected shape {shape}, got {tuple(t.shape)}')
+    ctx.check(ctx.lib.wb_solve_noalloc(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
+                                       _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                       _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags),
+                                       _stream(device)), 'solve_noalloc')
     return eff, amp, pe, flags
 
 

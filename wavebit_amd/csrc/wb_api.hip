@@ -48,6 +48,14 @@ int wb_ctx_create(int device, wb_ctx** out) {
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return WB_BADARG;
+  // the allocation-free solve's device array: its address is taken here, once per device, with
+  // that device current
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) return WB_HIP;
+  if (hipSetDevice(device) != hipSuccess) return WB_HIP;
+  const hipError_t attached = wb_noalloc_attach(device);
+  hipSetDevice(prev);
+  if (attached != hipSuccess) return WB_HIP;
   wb_ctx* c = new (std::nothrow) wb_ctx;
   if (!c) return WB_OOM;
   c->device = device;
@@ -194,6 +202,32 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve");
+}
+
+int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                     const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_solve_noalloc: bad dimensions");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe)
+    return fail(ctx, WB_BADARG, "wb_solve_noalloc: null buffer");
+  const size_t need = wb_solve_workspace_size(P, L, hu, hv, mode);
+  if (need > (size_t)WB_SOLVE_NOALLOC_BYTES)
+    return fail(ctx, WB_OOM,
+                "wb_solve_noalloc: the problem needs " + std::to_string(need) +
+                    " bytes of scratch, above the fixed " + std::to_string((size_t)WB_SOLVE_NOALLOC_BYTES) +
+                    " of the allocation-free solve (fewer points per call, or wb_solve)");
+  void* scratch = wb_noalloc_scratch(ctx->device);
+  if (!scratch) return fail(ctx, WB_HIP, "wb_solve_noalloc: the device array was not attached");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr;
+  return hip_status(ctx, wb_solve_launch(a, scratch, (hipStream_t)stream), "wb_solve_noalloc");
 }
 
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,

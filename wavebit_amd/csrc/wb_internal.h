@@ -33,6 +33,10 @@ hipError_t wb_inverse_blocked(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi
 hipError_t wb_inverse_launch(int n, BatchPtr A, int lda, BatchPtr Ainv, int ldi, int* ipiv,
                              int* info, int batch, hipStream_t stream);
 hipError_t wb_zero_int_launch(int* x, int count, hipStream_t stream);
+// The allocation-free solve's device array (wb_noalloc.hip): its address on `device`, taken once
+// per device at context creation with that device current.
+hipError_t wb_noalloc_attach(int device);
+void* wb_noalloc_scratch(int device);
 hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
                            int batch, hipStream_t stream);
 size_t wb_hqr_ms_rot_bytes(int n, int batch);
