@@ -83,13 +83,23 @@ enum {
   WB_PT_STRIDE = 16
 };
 
-/* flag bits written per (point, layer) by wb_solve */
-/* WB_FLAG_EP: a 2D layer's eigensolve returned a near-defective mode pair (two modes that are
+/* flag bits written per (point, layer) by wb_solve
+ * WB_FLAG_EP: a 2D layer's eigensolve returned a near-defective mode pair (two modes that are
  * each other's nearest eigenvalue with unit eigenvectors at a sine below 1e-2, roots above
  * 1e-9; rebuilt by the reference under exceptional_points='rebuild', rcwa/_engine/
- * solve_2d.py:583-660). WB_FLAG_GRAZING: a 2D layer has a grazing mode, |w| below 1e-9 of the
- * largest (always rebuilt by the reference, :2463-2485). This engine reports both instead. */
-enum { WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2, WB_FLAG_EP = 4, WB_FLAG_GRAZING = 8 };
+ * solve_2d.py:583-660). WB_FLAG_EP_LOUD: such a pair at a sine below sqrt(2^-53 / 1e-10) =
+ * 1.05e-3, where the unrebuilt modes may be off by more than 1e-10 (the reference measures the
+ * error as K u / s^2 with K <= 1e-2, :222-229; this bound takes K = 1): a caller holding the
+ * reference's tolerance refuses the point. Above that sine the modes as computed are within it.
+ * WB_FLAG_GRAZING: a 2D layer has a grazing mode, |w| below 1e-9 of the largest (always rebuilt
+ * by the reference, :2463-2485): wb_solve_rebuild solves such points again with the cluster
+ * re-derived, and then sets WB_FLAG_REBUILT (rebuilt, pair flags cleared) or WB_FLAG_DEFECTIVE
+ * (the cluster is numerically defective: no mode basis exists there; refuse the point). A grazing
+ * layer with neither kept its modes (cluster above 8 modes, or its two products disagree). */
+enum {
+  WB_FLAG_NOCONVERGE = 1, WB_FLAG_SINGULAR = 2, WB_FLAG_EP = 4, WB_FLAG_GRAZING = 8,
+  WB_FLAG_REBUILT = 16, WB_FLAG_DEFECTIVE = 32, WB_FLAG_EP_LOUD = 64
+};
 
 int wb_version(void);
 int wb_ctx_create(int device, wb_ctx** out);
@@ -167,6 +177,19 @@ int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const 
                      const double* thick, const int* conv_idx, const double* epx, const double* epy,
                      const double* epz_i, double* eff, double* amp, double* pe, int* flags,
                      void* stream);
+
+/* wb_solve (2D path, WB_MODE_2D) with every grazing layer's mode cluster re-derived from the
+ * coupled first-order problem, as the reference always does (rcwa/_engine/solve_2d.py:118-228
+ * _rebuild_grazing_cluster, entered at :2463-2485). Meant for the points whose wb_solve flags
+ * carry WB_FLAG_GRAZING (a rare condition: an order on a layer's light line); the extra work is
+ * one more n x n eigensolve per (point, layer) pair of the call. smat is NULL or the output of
+ * wb_solve_scattering. Layers up to n = 2048 and clusters up to 8 modes are rebuilt. The
+ * workspace is wb_solve_rebuild_workspace_size's. */
+size_t wb_solve_rebuild_workspace_size(int P, int L, int hu, int hv);
+int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pts,
+                     const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     double* smat, void* workspace, size_t workspace_bytes, void* stream);
 
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
  * is stored k x m, B n x k). */

@@ -355,6 +355,143 @@ def layer_modes_2d(kx, ky, epx, epy, epz_i):
     return W, V, q, O2, OR
 
 
+# --------------------------------------------------------------------------- Rayleigh / EP set
+
+GRAZE_REL = 1e-9        # solve_2d.py:44-56 (_COUPLED_REL): a grazing layer, min|w| < 1e-9 max|w|
+DEFECTIVE_REL = 1e-10   # solve_2d.py:58-69 (_DEFECTIVE_REL): cluster rates below 1e-10 sqrt(max|w|)
+EP_SINE = 1e-2          # solve_2d.py:222-231 (_EP_SINE): the near-defective pair gate
+EP_ROOT_FLOOR = 1e-9    # solve_2d.py:233-235 (_EP_ROOT_FLOOR)
+# The sine below which the unrebuilt pair may be off by more than 1e-10: the reference measures
+# the product construction's error as K u / s**2 with K <= 1e-2 (solve_2d.py:222-229); this
+# bound takes K = 1, so u / s**2 > 1e-10 -> s < sqrt(2**-53 / 1e-10).
+EP_LOUD_SINE = math.sqrt(2.0 ** -53 / 1e-10)
+
+
+class DefectiveGrazeError(FloatingPointError):
+    """A grazing cluster whose coupled rates split below working precision: no exponential mode
+    basis represents the layer (solve_2d.py:190-201). Move off the condition."""
+
+
+class DefectivePairError(FloatingPointError):
+    """A near-defective mode pair whose unrebuilt construction may be off by more than 1e-10."""
+
+
+def omega_left(kx, ky, epz_i):
+    """``Omega_L = [[-Cc, B], [-D, E]]`` (solve_2d.py:2377-2392, 2474-2476), non-magnetic."""
+    N = kx.shape[0]
+    I = np.eye(N, dtype=np.complex128)
+    B = (kx[:, None] * epz_i) * kx[None, :] - I
+    D = (ky[:, None] * epz_i) * ky[None, :] - I
+    Cc = (kx[:, None] * epz_i) * ky[None, :]
+    E = (ky[:, None] * epz_i) * kx[None, :]
+    return np.block([[-Cc, B], [-D, E]])
+
+
+def _small(w, rel):
+    a = np.abs(w)
+    return np.flatnonzero(a < rel * a.max())
+
+
+def rebuild_grazing_cluster(w, W, V, q, OL, OR):
+    """Re-derive the grazing cluster of a layer from the coupled first-order problem
+    (solve_2d.py:118-228, ``_rebuild_grazing_cluster``; non-magnetic SMM root).
+
+    The cluster columns J (|w| below GRAZE_REL of the largest) span S_E; the same cluster of the
+    swapped product ``OR @ OL`` spans S_H. With orthonormal bases Q_E, Q_H the factors restricted
+    to them, ``R_r = Q_H^H OR Q_E`` and ``L_r = Q_E^H OL Q_H``, give the k x k first-order problem
+    ``[[0, L_r], [R_r, 0]] g = mu g``; k of its 2k eigenvectors (the +-mu twins share a field
+    pattern) are picked greedily so that both their E parts and their H parts stay independent,
+    each branch-fixed to ``mode_q(mu**2)``. Returns ``(W, V, q)`` with the cluster replaced,
+    ``None`` when the two products disagree on the cluster size (the caller keeps the
+    construction), or raises DefectiveGrazeError when a picked rate is below DEFECTIVE_REL of
+    ``sqrt(max|w|)``.
+    """
+    J = _small(w, GRAZE_REL)
+    wh, Vh = np.linalg.eig(OR @ OL)
+    Jh = _small(wh, GRAZE_REL)
+    if J.size != Jh.size:
+        return None
+    k = J.size
+    Q_E = np.linalg.qr(W[:, J])[0]
+    Q_H = np.linalg.qr(Vh[:, Jh])[0]
+    R_r = Q_H.conj().T @ OR @ Q_E
+    L_r = Q_E.conj().T @ OL @ Q_H
+    zero = np.zeros((k, k), dtype=np.complex128)
+    mu, g = np.linalg.eig(np.block([[zero, L_r], [R_r, zero]]))
+    g = g / np.linalg.norm(g, axis=0)
+    # greedy pick: the candidate with the most E and H left after projecting out the picked ones
+    rest_e, rest_h = g[:k].copy(), g[k:].copy()
+    picked = []
+    for _ in range(k):
+        score = np.sum(np.abs(rest_e) ** 2, axis=0) + np.sum(np.abs(rest_h) ** 2, axis=0)
+        score[picked] = -1.0
+        j = int(np.argmax(score))
+        picked.append(j)
+        for rest in (rest_e, rest_h):
+            u = rest[:, j]
+            nu = np.linalg.norm(u)
+            if nu > 1e-150:
+                u = u / nu
+                rest -= np.outer(u, u.conj() @ rest)
+    mu_p = mu[picked]
+    if np.abs(mu_p).min() < DEFECTIVE_REL * np.sqrt(np.abs(w).max()):
+        raise DefectiveGrazeError('a grazing mode cluster is numerically defective (its coupled '
+                                  'rates split below 1e-10 of the layer scale); move off the '
+                                  'condition by a relative step of the wavelength or angle')
+    q_p = mode_q(mu_p ** 2)
+    sign = np.where(np.abs(q_p - mu_p) > np.abs(q_p + mu_p), -1.0, 1.0)   # took the -q twin
+    W, V, q = W.copy(), V.copy(), q.copy()
+    W[:, J] = Q_E @ g[:k, picked]
+    V[:, J] = Q_H @ (g[k:, picked] * sign[None, :])
+    q[J] = q_p
+    return W, V, q
+
+
+def pair_sines(w, W):
+    """Sines between the unit eigenvectors of the near-defective pair candidates of a layer:
+    modes that are each other's nearest eigenvalue with both roots above EP_ROOT_FLOOR
+    (solve_2d.py:583-627, ``_defective_pairs``, the gate without its separation rule). One
+    entry per pair; empty when there is none."""
+    d = np.abs(w[:, None] - w[None, :])
+    np.fill_diagonal(d, np.inf)
+    nb = np.argmin(d, axis=1)
+    idx = np.arange(w.size)
+    roots = np.sqrt(np.abs(w))
+    keep = (nb[nb] == idx) & (idx < nb) & (np.minimum(roots, roots[nb]) > EP_ROOT_FLOOR)
+    Wn = W / np.linalg.norm(W, axis=0)
+    a, b = idx[keep], nb[keep]
+    cos = np.abs(np.sum(np.conj(Wn[:, a]) * Wn[:, b], axis=0))
+    return np.sqrt(np.maximum(0.0, 1.0 - cos ** 2))
+
+
+def layer_modes_2d_screened(kx, ky, epx, epy, epz_i, exceptional_points='rebuild'):
+    """``layer_modes_2d`` with the reference's Rayleigh/EP handling (solve_2d.py:2450-2581):
+    a grazing layer's cluster is rebuilt (``rebuild_grazing_cluster``: ``'rebuilt'``, or
+    ``'kept'`` when the products disagree on the cluster); otherwise, under
+    ``exceptional_points='rebuild'``, a pair whose sine is below EP_LOUD_SINE raises
+    DefectivePairError (the reference rebuilds it; this restatement keeps pairs above that sine,
+    whose construction is within 1e-10). Returns ``(W, V, q, status)``, status one of
+    ``'plain'``, ``'rebuilt'``, ``'kept'``, ``'pair'`` (a gated pair above the loud sine)."""
+    with np.errstate(divide='ignore', invalid='ignore'):
+        W, V, q, _, OR = layer_modes_2d(kx, ky, epx, epy, epz_i)
+    w = q ** 2
+    if _small(w, GRAZE_REL).size:
+        rebuilt = rebuild_grazing_cluster(w, W, V, q, omega_left(kx, ky, epz_i), OR)
+        if rebuilt is not None:
+            return (*rebuilt, 'rebuilt')
+        status = 'kept'
+    else:
+        status = 'plain'
+    if exceptional_points == 'rebuild':
+        s = pair_sines(w, W)
+        if s.size and s.min() < EP_LOUD_SINE:
+            raise DefectivePairError(f'a near-defective mode pair (eigenvector sine {s.min():.2e}) '
+                                     'whose unrebuilt construction may be off by more than 1e-10')
+        if status == 'plain' and s.size and s.min() < EP_SINE:
+            status = 'pair'
+    return W, V, q, status
+
+
 def sp_modes_2d(W, V, Fc, Fs, N):
     """Rotation into the s/p boundary basis (``solve_2d.py:2584-2616``)."""
     c, s = Fc[:, None], Fs[:, None]
@@ -493,7 +630,7 @@ def order_kxy_2d(wavelength, period, eps_top, theta, phi, hu, hv, k_direct=None)
 
 
 def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
-           return_modes=False, side='top', k_direct=None):
+           return_modes=False, side='top', k_direct=None, exceptional_points='rebuild'):
     """Coupled s/p 2D solve, top incidence (``scattering.py:1185-1456``, ``:907-994``, as called
     by ``engine.py:3402-3420``). ``conv`` is a list of ``(epx, epy, epz_i)`` bottom-to-top;
     ``pol`` the unit pair ``(a_s, a_p)``. Returns the three incidence variants
@@ -529,8 +666,10 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
     S = s_reflection(W_kt, V_kt, W_gap, V_gap)
     S_trn = s_transmission(W_kb, V_kb, W_gap, V_gap)
     modes = []
+    status = []
     for li, (epx, epy, epz_i) in enumerate(conv):
-        W, V, q, _, _ = layer_modes_2d(kxf, kyf, epx, epy, epz_i)
+        W, V, q, st = layer_modes_2d_screened(kxf, kyf, epx, epy, epz_i, exceptional_points)
+        status.append(st)
         big_W, big_V = sp_modes_2d(W, V, Fc, Fs, N)
         X = np.exp(-k0 * thickness[li] * q)
         A, B = ab_matrices(big_W, big_V, W_gap, V_gap)
@@ -588,6 +727,7 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
         out[name] = sub
     out['kx'], out['ky'] = kx, ky
     out['scattering'] = scattering
+    out['layer_status'] = status
     if return_modes:
         out['modes'] = modes
     return out
@@ -595,7 +735,7 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
 
 def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, theta, phi,
                  polarization, orders, factorization='li', coefficients='sampled', side='top',
-                 k_direct=None):
+                 k_direct=None, exceptional_points='rebuild'):
     """One plane-wave point through a raster stack, routed the way the reference routes it.
 
     ``eps_layers``: ``(L, Ny, Nx)`` complex, bottom-to-top (``problem.py:1-5``). ``polarization``
@@ -627,6 +767,6 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
         raise ValueError('a one-row raster has no y orders')
     conv = [rule(e, hu, hv) for e in eps_layers]
     out = smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv,
-                 side=side, k_direct=k_direct)
+                 side=side, k_direct=k_direct, exceptional_points=exceptional_points)
     out['path'] = '2d'
     return out

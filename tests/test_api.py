@@ -187,11 +187,17 @@ def test_incident_kxy_follows_the_reference_rules():
 
 @pytest.mark.parametrize('bits,ep,error,notice', [
     (0, 'rebuild', None, False), (1, 'rebuild', 'rcwa.singular', False), (2, 'off', 'rcwa.singular', False),
-    (8, 'off', None, True), (8, 'rebuild', None, True), (4, 'rebuild', None, True), (4, 'off', None, False)])
+    # grazing: kept (no rebuild took) warns; rebuilt is quiet; a defective cluster refuses
+    (8, 'off', None, True), (8, 'rebuild', None, True), (8 | 16, 'rebuild', None, False),
+    (8 | 32, 'rebuild', 'rcwa.singular', False), (8 | 32, 'off', 'rcwa.singular', False),
+    # pairs: counted; below the loud sine they refuse under 'rebuild' and stand under 'off'
+    (4, 'rebuild', None, False), (4, 'off', None, False),
+    (4 | 64, 'rebuild', 'rcwa.singular', False), (4 | 64, 'off', None, False)])
 def test_solve_flags_map_to_errors_and_notices(bits, ep, error, notice):
-    """Failed eigensolves / factorizations raise; grazing modes (always rebuilt by the reference)
-    and near-defective pairs (rebuilt under exceptional_points='rebuild') are kept, counted and
-    announced instead."""
+    """The correct-or-loud contract of the Rayleigh/EP set: failed eigensolves and factorizations,
+    defective grazing clusters and (under exceptional_points='rebuild') pairs whose modes may miss
+    1e-10 raise; a rebuilt grazing layer and a pair above the loud sine stand quietly; a grazing
+    cluster the rebuild could not restate is kept and announced."""
     from wavebit_amd.rcwa._lifecycle import flag_error, flag_notice
     flags = np.zeros((3, 2), dtype=np.int32)
     flags[1, 1] = bits
@@ -200,7 +206,10 @@ def test_solve_flags_map_to_errors_and_notices(bits, ep, error, notice):
     assert (got is None) if error is None else (got.code == error)
     msg, counts = flag_notice(flags, numerics)
     assert (msg is not None) == notice
-    assert counts == {'grazing_layer_solves': int(bits == 8), 'defective_pair_layer_solves': int(bits == 4)}
+    assert counts == {'grazing_layer_solves': int(bool(bits & 8)),
+                      'grazing_rebuilt_layer_solves': int(bool(bits & 16)),
+                      'grazing_kept_layer_solves': int(bits & (8 | 16 | 32) == 8),
+                      'defective_pair_layer_solves': int(bool(bits & 4))}
 
 
 def test_detuning_is_accepted_and_validated():

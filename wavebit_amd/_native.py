@@ -19,6 +19,8 @@ _ctxs = {}
 WB_OK, WB_BADARG, WB_OOM, WB_SINGULAR, WB_NONFINITE, WB_NOCONVERGE, WB_FLAGGED, WB_HIP = range(8)
 MODE_2D, MODE_1D_TE, MODE_1D_TM = 0, 1, 2
 FLAG_NOCONVERGE, FLAG_SINGULAR, FLAG_EP, FLAG_GRAZING = 1, 2, 4, 8
+FLAG_REBUILT, FLAG_DEFECTIVE, FLAG_EP_LOUD = 16, 32, 64
+GRAZE_NMAX = 2048            # largest mode order wb_solve_rebuild takes
 PT_STRIDE = 16
 
 EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb_zgemm_batched',
@@ -32,7 +34,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_scattering', 'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
-           'wb_gemm_tuning_clear', 'wb_solve_noalloc')
+           'wb_gemm_tuning_clear', 'wb_solve_noalloc', 'wb_solve_rebuild_workspace_size',
+           'wb_solve_rebuild')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -78,6 +81,9 @@ def load_library():
             'wb_solve_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_solve_rebuild_workspace_size': (c_sz, [c_int, c_int, c_int, c_int]),
+            'wb_solve_rebuild': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
             'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
@@ -328,6 +334,33 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
                                _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
                                _stream(device)), 'solve')
     return eff, amp, pe, flags
+
+
+def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=False):
+    """``solve`` of 2D points with every grazing layer's mode cluster rebuilt; see
+    ``wb_solve_rebuild``. Returns (eff, amp, pe, flags) and, with ``scattering``, smat."""
+    device = epx.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    n = 2 * N
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    smat = torch.empty((P, 4, n, n), dtype=torch.complex128, device=device) if scattering else None
+    nbytes = ctx.lib.wb_solve_rebuild_workspace_size(P, L, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve_rebuild(ctx.handle, P, L, hu, hv, _ptr(pts), _ptr(thick),
+                                       _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                       _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
+                                       _ptr(ws), nbytes, _stream(device)), 'solve_rebuild')
+    return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
 
 
 SOLVE_NOALLOC_BYTES = 64 << 20   # WB_SOLVE_NOALLOC_BYTES

@@ -230,6 +230,35 @@ int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const 
   return hip_status(ctx, wb_solve_launch(a, scratch, (hipStream_t)stream), "wb_solve_noalloc");
 }
 
+size_t wb_solve_rebuild_workspace_size(int P, int L, int hu, int hv) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0) return 0;
+  const int n = 2 * (2 * hu + 1) * (2 * hv + 1);
+  return align_up(wb_solve_workspace_bytes(P, L, hu, hv, WB_MODE_2D)) + wb_graze_bytes(P * L, n);
+}
+
+int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pts,
+                     const double* thick, const int* conv_idx, const double* epx, const double* epy,
+                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     double* smat, void* workspace, size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0)
+    return fail(ctx, WB_BADARG, "wb_solve_rebuild: bad dimensions");
+  if (2 * (2 * hu + 1) * (2 * hv + 1) > WB_GRAZE_NMAX)
+    return fail(ctx, WB_UNSUPPORTED, "wb_solve_rebuild: mode order above 2048");
+  if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe || !flags)
+    return fail(ctx, WB_BADARG, "wb_solve_rebuild: null buffer");
+  if (workspace_bytes < wb_solve_rebuild_workspace_size(P, L, hu, hv))
+    return fail(ctx, WB_OOM, "wb_solve_rebuild: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = WB_MODE_2D;
+  a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
+  a.graze = (char*)workspace + align_up(wb_solve_workspace_bytes(P, L, hu, hv, WB_MODE_2D));
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_rebuild");
+}
+
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                         const double* thick, const int* conv_idx, const double* epx,
                         const double* epy, const double* epz_i, double* eff, double* amp, double* pe,

@@ -125,8 +125,12 @@ __device__ inline void rot_right(const Rot& q, cd& a0, cd& a1) {
   a1 = mkc(b1x, b1y);
 }
 
-// Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo.
-__device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
+// Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo. With
+// `relaxed` (the iteration has stalled) the conventional test |h(k,k-1)| <= ulp (|h(k-1,k-1)| +
+// |h(k,k)|) alone decides: still a normwise backward-stable deflation, and the one that lets an
+// exactly degenerate pair coupled at round-off (a uniform layer's two polarizations of one order)
+// split, where the Ahues-Tisseur refinement waits for a separation QR cannot produce.
+__device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k, bool relaxed = false) {
   const int t = threadIdx.x;
   const double smlnum = SAFMIN * ((double)n / ULP);
   int found = lo;
@@ -146,7 +150,7 @@ __device__ int find_split(const cd* H, int n, int lo, int hi, int* s_k) {
         const double g2 = cabs1(csub(H[(long long)(k - 1) * n + k - 1], H[(long long)k * n + k]));
         const double aa = fmax(g1, g2), bb = fmin(g1, g2);
         const double s = aa + ab;
-        small = ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
+        small = relaxed || ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)));
       }
     }
     if (small && k > found) found = k;
@@ -211,7 +215,8 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
             const double g2 = cabs1(csub(TT(k - 1, k - 1), TT(k, k)));
             const double aa = fmax(g1, g2), bb = fmin(g1, g2);
             const double sc = aa + ab;
-            small = ba * (ab / sc) <= fmax(smlnum, ULP * (bb * (aa / sc)));
+            // after 10 iterations without a deflation the conventional test alone (find_split)
+            small = its >= 10 || ba * (ab / sc) <= fmax(smlnum, ULP * (bb * (aa / sc)));
           }
         }
       }
@@ -935,7 +940,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
   int sweeps = 0;
   const int max_sweeps = 40 * (n / M + 10);
   while (hi >= 0) {
-    lo = find_split(H, n, 0, hi, s_k);
+    lo = find_split(H, n, 0, hi, s_k, stall >= 3);
     if (lo > 0 && t == 0) H[(long long)lo * n + lo - 1] = mkc(0.0, 0.0);
     __syncthreads();
     if (lo == hi) {

@@ -73,6 +73,8 @@ struct wb_solve_args {
   int* flags;               // P x L: eig non-convergence / singular factorizations
   void* ck = nullptr;       // optional checkpoint for the adjoint (2D path), or nullptr
   cd* smat = nullptr;       // optional P x 4 x n x n stack scattering matrix (S11, S12, S21, S22)
+  void* graze = nullptr;    // 2D: scratch of the grazing-cluster rebuild (wb_graze_bytes), or
+                            // nullptr for no rebuild
 };
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

@@ -36,6 +36,7 @@ __global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, Batc
     int bi = n;
     for (int i = j + t; i < n; i += 256) {
       double v = cabs1(a[(long long)i * lda + j]);
+      if (v != v) v = INFINITY;            // a NaN entry is a candidate like any other
       if (v > best) { best = v; bi = i; }
     }
     s_val[t] = best;
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(256) void k_getrf_panel(int n, int k0, int jb, Batc
       }
       __syncthreads();
     }
+    // p lies in [j, n): every thread with a row in range offers it, and NaN counts as the largest
     const int p = s_idx[0];
     const double pv = s_val[0];
     __syncthreads();
@@ -116,6 +118,7 @@ __device__ __forceinline__ void panel_col(cd (&x)[WB_NB], int& pos, const PanelC
     const int j = P.k0 + C;
     const bool cand = P.own && pos >= j;
     double v = cand ? cabs1(x[C]) : -1.0;
+    if (v != v) v = INFINITY;              // NaN is the largest candidate, so the pivot stays in range
     int vi = cand ? pos : P.n;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

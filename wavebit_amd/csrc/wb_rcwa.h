@@ -5,6 +5,14 @@
 #include "wb_internal.h"
 #include "../../include/wavebit_amd.h"   // WB_FLAG_* bits
 
+// A layer mode's q from its eigenvalue w = q^2: the root with Re(q) >= 0, Im(q) <= 0 on the flat
+// tie |Re(q)| <= 1e-12 (reference _common/conventions.py:186-250, mode_q; the SMM root).
+__device__ inline cd mode_q(cd w) {
+  const cd v = csqrt_p(w);
+  const bool flip = (fabs(v.x) > 1e-12) ? (v.x < 0.0) : (v.y > 0.0);
+  return flip ? cneg(v) : v;
+}
+
 // per-point parameter record (doubles), set by the host
 enum {
   PT_WL = 0, PT_KX0 = 1, PT_KY0 = 2, PT_PHI = 3, PT_PX = 4, PT_PY = 5,
@@ -44,6 +52,29 @@ struct wb_ck {
 };
 size_t wb_checkpoint_bytes(int P, int L, int hu, int hv, int mode);
 wb_ck wb_ck_carve(void* ck, int P, int L, int hu, int hv, int mode);
+
+// Grazing-cluster rebuild (wb_graze.hip): scratch of a batch of B pairs at mode order n, the
+// stage run after the layer eigensolve and the patch run after the s/p rotation.
+constexpr int WB_GRAZE_KMAX = 8;        // largest grazing cluster rebuilt (modes)
+constexpr int WB_GRAZE_NMAX = 2048;     // largest mode order n = 2N rebuilt
+enum { WB_GRAZE_NONE = 0, WB_GRAZE_KEPT, WB_GRAZE_ACTIVE, WB_GRAZE_REBUILT, WB_GRAZE_DEFECTIVE };
+struct wb_graze_ws {
+  int *jn, *status, *einfo, *sinfo, *jidx;
+  double *wmax, *pad;
+  cd *QE, *QH, *TE, *TH, *WC, *VC;          // n x K per pair
+  cd *Rr, *Lr, *GE, *GH;                    // K x K per pair
+  cd *Ms, *gv, *Zs, *Xs;                    // 2K x 2K per pair
+  cd *mu, *taus, *qc;
+  void* rot;
+};
+size_t wb_graze_bytes(int B, int n);
+wb_graze_ws wb_graze_carve(void* ws, int B, int n);
+hipError_t wb_graze_stage(int N, int L, int B, const cd* w, const cd* W, BatchPtr epx,
+                          BatchPtr epy, BatchPtr epz, const cd* ov, cd* olr, cd* H, cd* Z, cd* X,
+                          cd* Vh, cd* wh, cd* tau, void* rotws, const wb_graze_ws& g,
+                          hipStream_t st);
+hipError_t wb_graze_patch(int N, int L, int B, const cd* ov, cd* bigW, cd* bigV, cd* q,
+                          const wb_graze_ws& g, int* flags, hipStream_t st);
 
 size_t wb_adjoint_workspace_bytes(int P, int L, int hu, int hv, int mode);
 struct wb_adjoint_args {

@@ -36,18 +36,11 @@ namespace {
 
 constexpr double PI = 3.141592653589793;
 constexpr double HALF_PI = 1.5707963267948966;
-constexpr double FLAT = 1e-12;
 
 __device__ inline cd boundary_kz(cd eps, cd kpar2) {
   cd v = csqrt_p(csub(eps, kpar2));
   if (v.x + v.y < 0.0) v = cneg(v);
   return v;
-}
-
-__device__ inline cd mode_q(cd w) {
-  cd v = csqrt_p(w);
-  const bool flip = (fabs(v.x) > FLAT) ? (v.x < 0.0) : (v.y > 0.0);
-  return flip ? cneg(v) : v;
 }
 
 // Real-angle order azimuth (kx, ky real): atan with the half-turn on Re(kx) < 0, the
@@ -388,18 +381,22 @@ __global__ void k_publish_smat(int hu, int hv, int mode, const cd* ov, cd* smat)
 // solve_2d.py:583-627 _defective_pairs, :2463-2465 the grazing gate): mode i and its nearest
 // eigenvalue nb(i) form a pair when each is the other's nearest, their unit eigenvectors (columns
 // of W) meet at a sine below 1e-2 and both roots exceed 1e-9 (|w| > 1e-18); a grazing layer has
-// min |w| < 1e-9 max |w|. They set WB_FLAG_EP / WB_FLAG_GRAZING on the pair (b = p * L + l).
+// min |w| < 1e-9 max |w|. They set WB_FLAG_EP / WB_FLAG_GRAZING on the pair (b = p * L + l), and
+// WB_FLAG_EP_LOUD for a pair whose sine is below sqrt(EP_LOUD_SIN2).
 constexpr int EP_NMAX = 2048;   // every 2D order window up to (15, 15): n <= 1922
+// sine^2 below which a pair's unrebuilt modes may be off by more than 1e-10: u / s^2 > 1e-10,
+// u = 2^-53 (the reference's measured K u / s^2 with K <= 1e-2, bounded here with K = 1)
+constexpr double EP_LOUD_SIN2 = 1.1102230246251565e-16 / 1e-10;
 __global__ __launch_bounds__(256) void k_ep_screen(int n, const cd* w, const cd* W, int* flags) {
   __shared__ cd sw[EP_NMAX];
   __shared__ int snb[EP_NMAX];
   __shared__ double red_min[256], red_max[256];
-  __shared__ int hit, grazing;
+  __shared__ int hit, loud, grazing;
   const long long b = blockIdx.x;
   const cd* wb = w + b * n;
   const cd* Wb = W + b * (long long)n * n;
   const int t = threadIdx.x;
-  if (t == 0) { hit = 0; grazing = 0; }
+  if (t == 0) { hit = 0; loud = 0; grazing = 0; }
   double mn = 1e300, mx = 0.0;
   for (int i = t; i < n; i += 256) {
     sw[i] = wb[i];
@@ -438,10 +435,13 @@ __global__ __launch_bounds__(256) void k_ep_screen(int n, const cd* w, const cd*
       re += a.x * c.x + a.y * c.y;   // conj(a) c
       im += a.x * c.y - a.y * c.x;
     }
-    if (1.0 - (re * re + im * im) < 1e-4) hit = 1;
+    const double sin2 = 1.0 - (re * re + im * im);
+    if (sin2 < 1e-4) hit = 1;
+    if (sin2 < EP_LOUD_SIN2) loud = 1;
   }
   __syncthreads();
-  if (t == 0) flags[b] |= (hit ? WB_FLAG_EP : 0) | (grazing ? WB_FLAG_GRAZING : 0);
+  if (t == 0)
+    flags[b] |= (hit ? WB_FLAG_EP : 0) | (loud ? WB_FLAG_EP_LOUD : 0) | (grazing ? WB_FLAG_GRAZING : 0);
 }
 
 // Right-hand side for the final interface: rhs = diag(t12) e_j for the basis incidences.
@@ -714,6 +714,9 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   for (int i = 0; i < 4; ++i) vec[i] = W_.vec[i];
   double* sig = W_.sig;
   const bool keep = (a.ck != nullptr) && mode == 0;
+  const bool graze = (a.graze != nullptr) && mode == 0;
+  wb_graze_ws G{};
+  if (graze) G = wb_graze_carve(a.graze, B, n);
   wb_ck CK;
   if (keep) CK = wb_ck_carve(a.ck, P, L, a.hu, a.hv, mode);
   hipError_t e;
@@ -775,6 +778,13 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.W, nn), n);
       WB_CHECK_LAUNCH();
     }
+    // grazing-cluster rebuild: scratch in M0 (H), M1/M2 (eigensolver), M5 (eigenvectors of H), M6
+    // (Omega_L | Omega_R), q (eigenvalues of H), tau and the rotation staging, all free until the
+    // V GEMMs and the s/p rotation below
+    if (graze) {
+      if ((e = wb_graze_stage(N, L, B, w, M[3], EPX, EPY, EPZ, ov, M6, M[0], M[1], M[2], M[5], q, tau,
+                              rotws, G, st))) return e;
+    }
     // V GEMMs: top = -epy W_bot, bottom = epx W_top (into M4)
     if ((e = wb_zgemm_launch(N, n, N, mone, EPY, N, S(M[3] + (long long)N * n, nn), n, zero,
                              S(M[4], nn), n, B, st))) return e;
@@ -783,6 +793,9 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     hipLaunchKernelGGL(k_modes_post_2d, grid2((long long)N * n, B), dim3(256), 0, st, N, L, M[3],
                        M[4], w, ov, M[1], M[2], q);
     WB_CHECK_LAUNCH();
+    if (graze) {
+      if ((e = wb_graze_patch(N, L, B, ov, M[1], M[2], q, G, a.flags, st))) return e;
+    }
     bigW = M[1]; bigV = M[2];
   } else {
     const int tm = (mode == 2);

@@ -163,6 +163,44 @@ def order_kxy(kx0, ky0, wavelength, period, hu, hv):
             ky0 + lam * (-a2x * m[None, :] + a1x * n[:, None]))
 
 
+def bucket_inputs(bucket, conv, device):
+    """The device inputs of a bucket's solve: (pts, thick, conv_idx, epx, epy, epz_i)."""
+    P, L = len(bucket.points), bucket.n_layers
+    # concatenate the stacks' matrices: stack s layer l -> s * L + l
+    epx = torch.cat([c[0] for c in conv]).contiguous()
+    epy = torch.cat([c[1] for c in conv]).contiguous()
+    epz = torch.cat([c[2] for c in conv]).contiguous()
+    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
+    return (torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device),
+            torch.from_numpy(idx).to(device), epx, epy, epz)
+
+
+def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
+    """Solve again the points of a 2D bucket with a grazing layer (``FLAG_GRAZING``), every such
+    layer's mode cluster re-derived from the coupled first-order problem (``wb_solve_rebuild``;
+    reference ``solve_2d.py:2463-2485``, which always does so). ``eff, amp, pe, flags`` (and
+    ``smat``) are the bucket's host arrays, updated in place for those points. Returns the
+    number of points solved again."""
+    if bucket.path != PATH_2D:
+        return 0
+    bad = np.flatnonzero(np.any(np.asarray(flags) & _native.FLAG_GRAZING, axis=1))
+    N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
+    if not bad.size or 2 * N > _native.GRAZE_NMAX:
+        return 0
+    sub = Bucket(hu=bucket.hu, hv=bucket.hv, path=bucket.path, n_layers=bucket.n_layers,
+                 points=[bucket.points[i] for i in bad])
+    out = _native.solve_rebuild(*bucket_inputs(sub, conv, device), bucket.hu, bucket.hv,
+                                scattering=smat is not None)
+    host = [t.cpu().numpy() for t in out]
+    for dst, src in zip((eff, amp, pe, flags), host[:4]):
+        dst[bad] = src
+    if smat is not None:
+        smat[bad] = host[4]
+    return int(bad.size)
+
+
 def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
     """Solve every point of a bucket; returns device tensors (eff, amp, pe, flags) and N.
 
@@ -173,16 +211,7 @@ def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
     """
     P, L = len(bucket.points), bucket.n_layers
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
-    # concatenate the stacks' matrices: stack s layer l -> s * L + l
-    epx = torch.cat([c[0] for c in conv]).contiguous()
-    epy = torch.cat([c[1] for c in conv]).contiguous()
-    epz = torch.cat([c[2] for c in conv]).contiguous()
-    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
-    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
-    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
-    pts_d = torch.from_numpy(rec).to(device)
-    thick_d = torch.from_numpy(thick).to(device)
-    idx_d = torch.from_numpy(idx).to(device)
+    pts_d, thick_d, idx_d, epx, epy, epz = bucket_inputs(bucket, conv, device)
     mode = _MODE[bucket.path]
     if graph_slot is not None:
         from . import graphs
@@ -206,17 +235,9 @@ def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
 def run_bucket_scattering(bucket, conv, device):
     """``run_bucket`` that also returns the stack's published scattering blocks per point:
     (eff, amp, pe, flags, N, smat) with smat (P, 4, n, n) on the device."""
-    P, L = len(bucket.points), bucket.n_layers
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
-    epx = torch.cat([c[0] for c in conv]).contiguous()
-    epy = torch.cat([c[1] for c in conv]).contiguous()
-    epz = torch.cat([c[2] for c in conv]).contiguous()
-    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
-    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
-    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
     eff, amp, pe, flags, smat = _native.solve_scattering(
-        torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device),
-        torch.from_numpy(idx).to(device), epx, epy, epz, bucket.hu, bucket.hv, _MODE[bucket.path])
+        *bucket_inputs(bucket, conv, device), bucket.hu, bucket.hv, _MODE[bucket.path])
     return eff, amp, pe, flags, N, smat
 
 

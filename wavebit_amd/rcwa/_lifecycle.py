@@ -553,42 +553,77 @@ def _run_live(prepared, problem, r, checks):
     return replace(result, outputs=outputs)
 
 
-def flag_error(flags, numerics):
-    """The error a solve's per-pair flags (``WB_FLAG_*``) call for, or None: non-convergence and
-    singular factorizations are ``NumericalError('rcwa.singular')``."""
+def _fatal_bits(numerics):
+    """Flag bits that leave no valid answer under ``numerics``: a failed eigensolve or
+    factorization, a numerically defective grazing cluster (reference ``solve_2d.py:190-201``
+    raises there) and, under ``exceptional_points='rebuild'``, a near-defective pair whose modes
+    as computed may be off by more than 1e-10 (the reference would rebuild it)."""
     fl = _engine._native
-    if np.any(np.asarray(flags) & (fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR)):
+    bits = fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR | fl.FLAG_DEFECTIVE
+    if numerics.exceptional_points == 'rebuild':
+        bits |= fl.FLAG_EP_LOUD
+    return bits
+
+
+def _kept_grazing(flags):
+    """Grazing layers whose cluster was not rebuilt (kept as computed, as the reference keeps
+    a cluster it cannot restate)."""
+    fl = _engine._native
+    flags = np.asarray(flags)
+    return ((flags & fl.FLAG_GRAZING) != 0) & ((flags & (fl.FLAG_REBUILT | fl.FLAG_DEFECTIVE)) == 0)
+
+
+def flag_error(flags, numerics):
+    """The error a solve's per-pair flags (``WB_FLAG_*``) call for, or None. Every case is
+    ``NumericalError('rcwa.singular')``, the code the reference gives a linear-algebra failure
+    and its defective-cluster refusal (``_lifecycle.py:718-719`` of the reference)."""
+    fl = _engine._native
+    flags = np.asarray(flags)
+    if np.any(flags & (fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR)):
         return NumericalError('A layer eigensolve did not converge or a scattering-matrix factorization '
                               'was singular.', code='rcwa.singular', path='problem')
+    if np.any(flags & fl.FLAG_DEFECTIVE):
+        return NumericalError('A 2D layer eigenmode cluster is numerically defective at a grazing '
+                              'condition (an order on the layer light line): the coupled mode rates '
+                              'split by less than 1e-10 of the layer modal scale, and no exponential '
+                              'mode basis represents the layer at working precision. Move off the '
+                              'condition by a relative step of the wavelength or angle; '
+                              'Numerics(detuning=step) does that retry.',
+                              code='rcwa.singular', path='problem')
+    if numerics.exceptional_points == 'rebuild' and np.any(flags & fl.FLAG_EP_LOUD):
+        return NumericalError('A 2D layer holds a near-defective mode pair (eigenvectors at a sine '
+                              'below 1.05e-3, an exceptional point of the layer operator) whose '
+                              'modes as computed may be off by more than 1e-10; this engine does not '
+                              'rebuild such a pair. Move off the point by a relative step of the '
+                              "wavelength or angle (Numerics(detuning=step)), or take the modes as "
+                              "computed with Numerics(exceptional_points='off').",
+                              code='rcwa.singular', path='problem')
     return None
 
 
 def flag_notice(flags, numerics):
-    """``(message, counts)`` for the modes the reference would rebuild, or ``(None, counts)``.
+    """``(message, counts)`` for the layer solves that keep modes the reference would rebuild,
+    or ``(None, counts)``.
 
-    A grazing mode (|w| below 1e-9 of the largest) the reference always re-derives from the
-    coupled first-order problem (reference ``solve_2d.py:2463-2485``); a near-defective pair it
-    rebuilds as a Schur pair under ``exceptional_points='rebuild'`` (``:583-660, 2491-2575``).
-    This engine keeps the modes as computed, counts the flagged layer solves in the result's
-    diagnostics and, for a ``Prepared.run``, warns (``ExceptionalPointWarning``).
-    ``exceptional_points='off'`` takes the pairs as computed, as the reference does.
-    """
+    A grazing layer (|w| below 1e-9 of the largest) has its cluster re-derived from the coupled
+    first-order problem (``wb_solve_rebuild``; reference ``solve_2d.py:2463-2485``); a cluster
+    the rebuild cannot restate (above 8 modes, or products that disagree on its size) is kept as
+    computed with a warning, as the reference keeps it. Near-defective pairs above the loud sine
+    are counted and taken as computed: their error is below 1e-10 by the bound of the pair
+    screen."""
     fl = _engine._native
     flags = np.asarray(flags)
+    kept = _kept_grazing(flags)
     counts = {'grazing_layer_solves': int(np.count_nonzero(flags & fl.FLAG_GRAZING)),
+              'grazing_rebuilt_layer_solves': int(np.count_nonzero(flags & fl.FLAG_REBUILT)),
+              'grazing_kept_layer_solves': int(np.count_nonzero(kept)),
               'defective_pair_layer_solves': int(np.count_nonzero(flags & fl.FLAG_EP))}
-    parts = []
-    if counts['grazing_layer_solves']:
-        parts.append(f"{counts['grazing_layer_solves']} layer solve(s) hold a grazing mode, which the "
-                     'reference re-derives from the coupled first-order problem')
-    if counts['defective_pair_layer_solves'] and numerics.exceptional_points == 'rebuild':
-        parts.append(f"{counts['defective_pair_layer_solves']} layer solve(s) hold a near-defective "
-                     "mode pair, which the reference rebuilds as a Schur pair under "
-                     "exceptional_points='rebuild'")
-    if not parts:
+    if not counts['grazing_kept_layer_solves']:
         return None, counts
-    return ('; '.join(parts) + '. This engine keeps the modes as computed; results near such a '
-            'point can differ from the reference beyond round-off.'), counts
+    return (f"{counts['grazing_kept_layer_solves']} layer solve(s) hold a grazing mode cluster the "
+            'rebuild could not restate (more than 8 modes, or the two operator products disagree '
+            'on its size); their modes are kept as computed and results near such a point can '
+            'differ from the reference beyond round-off.'), counts
 
 
 def _detune(r, cache, per_wl, flags, smats, numerics):
@@ -605,12 +640,15 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
     from dataclasses import replace
     from .diagnostics import AutoDetuneWarning, RayleighPointWarning
     step = numerics.detuning
-    fl = _engine._native
-    retry_bits = fl.FLAG_GRAZING | fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR
+    fatal = _fatal_bits(numerics)
     if not step:
         return r, per_wl, flags, smats
     flags = np.array(flags)
-    bad = [p for p in range(len(per_wl)) if np.any(flags[p] & retry_bits)]
+
+    def retry(row):
+        return bool(np.any(row & fatal) or np.any(_kept_grazing(row)))
+
+    bad = [p for p in range(len(per_wl)) if retry(flags[p])]
     if not bad:
         return r, per_wl, flags, smats
     per_wl, smats = list(per_wl), (list(smats) if smats is not None else None)
@@ -632,7 +670,7 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
         sub_per, sub_flags, sub_smats = _solve_resolved(sub, cache.sharing_conv())
         still = []
         for j, p in enumerate(bad):
-            if np.any(sub_flags[j] & (fl.FLAG_NOCONVERGE | fl.FLAG_SINGULAR)):
+            if np.any(sub_flags[j] & fatal):
                 still.append(p)            # this rung did not take; the point keeps its last answer
                 continue
             wls[p] = trial_wl[j]
@@ -645,7 +683,7 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
             moved.setdefault(p, []).append(knob)
             if knob == 'wavelength':
                 wl_applied.add(p)
-            if np.any(sub_flags[j] & fl.FLAG_GRAZING):
+            if retry(sub_flags[j]):
                 still.append(p)
         bad = still
     if moved:
@@ -679,12 +717,16 @@ def _solve_resolved(r, cache):
             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
             period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct))
     smats = None
+    smat = None
     if r.scattering:
         eff, amp, pe, flags, _, smat = _engine.run_bucket_scattering(bucket, [conv], device)
-        smats = _scattering_matrices(r, smat.cpu().numpy())
+        smat = smat.cpu().numpy()
     else:
         eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
     eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
+    _engine.rebuild_grazing(bucket, [conv], device, eff, amp, pe, flags, smat)
+    if smat is not None:
+        smats = _scattering_matrices(r, smat)
     per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
     return per_wl, flags, smats
 

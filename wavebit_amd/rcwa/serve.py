@@ -230,12 +230,18 @@ class Server:
             eff, amp, pe, flags, _ = _engine.run_bucket(bucket, convs, self.device)
             done = torch.cuda.Event()
             done.record(stream)
-        return dict(key=key, items=items, out=(eff, amp, pe, flags), event=done,
+        return dict(key=key, items=items, out=(eff, amp, pe, flags), event=done, bucket=bucket,
+                    convs=convs,
                     bytes=self._launch_bytes(key, len(items)), stream=stream)
 
     def _finish(self, launch, now, finished):
         eff, amp, pe, flags = (t.cpu().numpy() for t in launch['out'])
         hu, hv, path, L = launch['key']
+        # points with a grazing layer are solved again with the cluster rebuilt, on the launch's
+        # stream (its outputs are complete: the launch's event has been reached)
+        with torch.cuda.stream(launch['stream']):
+            _engine.rebuild_grazing(launch['bucket'], launch['convs'], self.device, eff, amp, pe,
+                                    flags)
         for p, (work, i, _, _) in enumerate(launch['items']):
             work.flags |= int(np.bitwise_or.reduce(flags[p]))
             work.results[i] = _engine.unpack(eff, amp, pe, p, hu, hv, path)
