@@ -181,13 +181,14 @@ def test_oracle_epz_i_and_q_match_legacy(name):
     kyf = np.broadcast_to(ky[:, None], (2 * hv + 1, 2 * hu + 1)).reshape(-1)
     L = len(d)
     for i in range(L):
-        # golden layers are in the legacy (top-to-bottom) kernel order: layer i of the golden is
-        # our layer L-1-i? The legacy kernel listed layers in its own order; compare as sets.
-        want_epz = [g[f'{name}__layer{j}__epz_i'] for j in range(L)]
-        assert min(np.abs(w - conv[i][2]).max() for w in want_epz) < 1e-12
+        # the golden's layer i is our layer i: legacy_case reverses the fixture's top-to-bottom
+        # structure list into our bottom-to-top order, and the fixture indexes its per-layer
+        # records in the order the reference's kernel received them, which is that same order
+        # (on 2d_2layer the other layer's epz_i differs by 7e-2, so a swap fails here)
+        assert np.abs(g[f'{name}__layer{i}__epz_i'] - conv[i][2]).max() < 1e-12
         W, V, q, _, _ = orc.layer_modes_2d(kxf, kyf, *conv[i])
-        want_q = [np.sort(np.abs(g[f'{name}__layer{j}__q'])) for j in range(L)]
-        assert min(np.abs(w - np.sort(np.abs(q))).max() for w in want_q) < 1e-10
+        want_q = np.sort(np.abs(g[f'{name}__layer{i}__q']))
+        assert np.abs(want_q - np.sort(np.abs(q))).max() < 1e-10
     assert res['path'] == '2d'
 
 

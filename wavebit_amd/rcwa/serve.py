@@ -277,7 +277,9 @@ class Server:
         order = collections.OrderedDict()                     # bucket key -> oldest arrival
         inflight = []
         results, latencies = {}, []
-        used = self.conv.bytes
+        # device bytes held by launches in flight; the resident conv matrices are charged as
+        # they stand at each admission (the cache grows as requests bring new rasters)
+        used = 0
         free_streams = list(self.streams)
         t0 = time.perf_counter()
         solved_points = 0
@@ -334,7 +336,7 @@ class Server:
             if grads and grad_job is None and free_streams:
                 work, key = grads[0]
                 need = self._gradient_bytes(key, work.n_points)
-                if used + need <= self.budget or not inflight:
+                if self.conv.bytes + used + need <= self.budget or not inflight:
                     grads.popleft()
                     stream = free_streams.pop()
                     used += need
@@ -351,9 +353,9 @@ class Server:
                     break
                 q = queues[key]
                 P = min(len(q), self.max_points)
-                while P > 1 and used + self._launch_bytes(key, P) > self.budget:
+                while P > 1 and self.conv.bytes + used + self._launch_bytes(key, P) > self.budget:
                     P //= 2
-                if used + self._launch_bytes(key, P) > self.budget and inflight:
+                if self.conv.bytes + used + self._launch_bytes(key, P) > self.budget and inflight:
                     continue
                 items = [q.popleft() for _ in range(P)]
                 if not q:
