@@ -65,6 +65,66 @@ def _traffic(name, orders, layers, points):
     return float(k['bytes_per_launch']), 'profiles/rcwa_orders9_64pts_hbm_traffic.md'
 
 
+def _blas():
+    """(vendor string, threads) of the BLAS NumPy calls on this host."""
+    try:
+        from threadpoolctl import threadpool_info
+        info = [t for t in threadpool_info() if t.get('user_api') == 'blas']
+    except Exception:
+        info = []
+    if not info:
+        return 'unknown', os.cpu_count()
+    t = info[0]
+    return f"{t.get('internal_api')} {t.get('version')} ({t.get('threading_layer')})", int(t['num_threads'])
+
+
+def _cpu_baseline(fn, calls, unit_per_call, sample):
+    """The CPU baseline of one workload: one untimed warm-up call, then the median of ``calls``
+    timed calls (time.perf_counter); ``fn(i)`` runs call i and returns the number of solves."""
+    fn(0)
+    times, solves = [], []
+    for i in range(calls):
+        t0 = time.perf_counter()
+        solves.append(fn(i))
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    vendor, threads = _blas()
+    return {'value': round(float(np.median(solves)) / med, 6), 'unit': unit_per_call,
+            'cores': threads, 'kind': 'port', 'blas': vendor, 'host_cpus': os.cpu_count(),
+            'sample': f'{sample}; median of {calls} calls after one warm-up, {med:.3f} s per call, '
+                      f'{sum(times):.1f} s total'}
+
+
+def _measured_peaks(torch, dev):
+    """Measured FP64 matrix and HBM copy rates of this device beside the datasheet peaks: the
+    vendor DGEMM (torch.matmul on 8192 x 8192 float64), and a 4 GiB device-to-device copy."""
+    out = {}
+    a = torch.randn(8192, 8192, dtype=torch.float64, device=dev)
+    b = torch.randn(8192, 8192, dtype=torch.float64, device=dev)
+    torch.matmul(a, b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        c = torch.matmul(a, b)
+    torch.cuda.synchronize()
+    out['dgemm_vendor_tflops'] = round(5 * 2 * 8192 ** 3 / (time.perf_counter() - t0) / 1e12, 2)
+    del a, b, c
+    n = (4 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out['copy_gbs'] = round(5 * 2 * n * 8 / (time.perf_counter() - t0) / 1e9, 1)
+    del src, dst
+    out['datasheet_fp64_matrix_tflops'] = PEAK_FP64_TFLOPS
+    out['datasheet_hbm_gbs'] = PEAK_HBM_GBS
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -81,7 +141,7 @@ def main():
     ap.add_argument('--serve-rate', type=float, default=4.0, help='request arrivals per second')
     ap.add_argument('--crossed-points', type=int, default=512,
                     help='points of the crossed-grating leg (orders (7,7), 3 layers of 256x256; 0 skips)')
-    ap.add_argument('--uneven-problems', type=int, default=0,
+    ap.add_argument('--uneven-problems', type=int, default=512,
                     help='problems of the uneven-batch leg (first of the seeded 2048; 0 skips)')
     ap.add_argument('--serve-grad-fraction', type=float, default=0.3,
                     help='fraction of served requests that also ask for the raster gradient')
@@ -91,6 +151,7 @@ def main():
                     help='runs of the 1D orders-(10,0) grating, eager vs captured graph (0 skips)')
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='run one device copy of this size first (HBM counter calibration)')
+    ap.add_argument('--no-peaks', action='store_true', help='skip the measured DGEMM / copy peaks')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the points of a batch are '
@@ -272,16 +333,9 @@ def main():
             errs.append(max(float(np.abs(got['incident'][k] - ref['incident'][k]).max())
                             for k in ('R', 'T')))
         med = float(np.median(times))
-        threads = os.cpu_count()
-        try:
-            from threadpoolctl import threadpool_info
-            blas = [t['num_threads'] for t in threadpool_info() if t.get('user_api') == 'blas']
-            if blas:
-                threads = max(blas)
-        except Exception:
-            pass
+        vendor, threads = _blas()
         cpu = {'value': round(1.0 / med, 6), 'unit': 'solves/s', 'cores': threads,
-               'kind': 'port',
+               'kind': 'port', 'blas': vendor, 'host_cpus': os.cpu_count(),
                'sample': f'{n_cpu} points of the same workload (4 layers, orders (9,9), every '
                          f'{stride}th wavelength) through the NumPy FP64 oracle after one warm-up; '
                          f'median {med:.2f} s per point, {sum(times):.1f} s total'}
@@ -395,14 +449,34 @@ def main():
                 eng.run_bucket(bucket, [e_conv], dev, graph_slot=slot)
                 torch.cuda.synchronize()
             dev_t[mode] = (time.perf_counter() - t0) / args.small_runs
+        # the configuration itself: exactly 900 nm, against the oracle
+        res900 = rcwa.run(prob1(p1['wavelength']), numerics=num1, backend='torch', device='cuda:0')
+        from oracle import rcwa_oracle as orc
+
+        def oracle1(i=0):
+            orc.solve_raster(g1['rasters'], g1['thickness'], g1['eps_top'], g1['eps_bottom'],
+                             g1['period'], p1['wavelength'], p1['theta'], None, 's', g1['orders'])
+            return 1
+        ref1 = orc.solve_raster(g1['rasters'], g1['thickness'], g1['eps_top'], g1['eps_bottom'],
+                                g1['period'], p1['wavelength'], p1['theta'], None, 's',
+                                g1['orders'])['incident']
+        centre = g1['orders'][0]
         small = {'workload': '1D binary grating, period 700 nm, one 300 nm layer, orders (10,0) = 21 '
-                             'harmonics, one wavelength per run',
+                             'harmonics, one wavelength per run (timed runs step the wavelength by '
+                             '0.01 nm; the values below are at exactly 900 nm)',
                  'runs': args.small_runs,
                  'lifecycle_eager_ms': round(timings['eager'] * 1e3, 4),
                  'lifecycle_graph_ms': round(timings['graph'] * 1e3, 4),
                  'solve_eager_ms': round(dev_t['eager'] * 1e3, 4),
                  'solve_graph_ms': round(dev_t['graph'] * 1e3, 4),
-                 'R_sum': float(res1.R.sum()), 'T_sum': float(res1.T.sum())}
+                 'wavelength_nm': p1['wavelength'],
+                 'R_sum': float(res900.R.sum()), 'T_sum': float(res900.T.sum()),
+                 'T_00': float(res900.T[0, 0, centre]),
+                 'solve_path': res900.diagnostics['solve_path'][0],
+                 'oracle_max_abs_err_RT': float(max(np.abs(res900.R[0] - ref1['R']).max(),
+                                                    np.abs(res900.T[0] - ref1['T']).max())),
+                 'cpu_baseline': None if args.no_cpu else _cpu_baseline(
+                     oracle1, 5, 'solves/s', 'the NumPy FP64 oracle on the same point')}
 
     # the crossed grating: 3 layers of 256x256 binary rasters, orders (7,7) = 225 harmonics,
     # points over wavelength, angle and polarization in one batched launch sequence (assembly
@@ -438,7 +512,29 @@ def main():
                    'points': len(w2['points']), 's': round(dt2, 4),
                    'solves_per_s': round(len(w2['points']) / dt2, 3),
                    'max_abs_R_plus_T_minus_1': err2, 'flagged_points': int(((f2 & 3) != 0).sum()),
-                   'grazing_or_defective_layer_solves': int(((f2 & 12) != 0).sum())}
+                   'grazing_layer_solves': int(((f2 & _native.FLAG_GRAZING) != 0).sum()),
+                   'near_defective_pair_layer_solves': int(((f2 & _native.FLAG_EP) != 0).sum()),
+                   'loud_pair_layer_solves': int(((f2 & _native.FLAG_EP_LOUD) != 0).sum())}
+        if not args.no_cpu:
+            from oracle import rcwa_oracle as orc
+            sample2 = list(range(0, len(w2['points']), max(1, len(w2['points']) // 5)))[:5]
+
+            def oracle2(i):
+                p = w2['points'][sample2[i % len(sample2)]]
+                return orc.solve_raster(w2['rasters'], w2['thickness'], w2['eps_top'],
+                                        w2['eps_bottom'], w2['period'], p['wavelength'],
+                                        p['theta'], p['phi'], p['polarization'], h2)
+            errs2 = []
+            for j, i in enumerate(sample2):
+                ref2 = oracle2(j)['incident']
+                got2 = eng.unpack(e2, a2, p2, i, h2[0], h2[1], eng.PATH_2D)['incident']
+                errs2.append(max(float(np.abs(got2[k] - ref2[k]).max()) for k in ('R', 'T')))
+            crossed['oracle_max_abs_err_RT'] = max(errs2)
+            crossed['oracle_points'] = sample2
+            crossed['cpu_baseline'] = _cpu_baseline(
+                lambda i: (oracle2(i), 1)[1], len(sample2), 'solves/s',
+                f'{len(sample2)} points of the crossed grating (every '
+                f'{max(1, len(w2["points"]) // 5)}th) through the NumPy FP64 oracle')
 
     # the uneven batch: problems of different order windows and layer counts served at once,
     # size-bucketed by (orders, layers), one launch sequence per bucket on three streams
@@ -458,7 +554,31 @@ def main():
                   'problems': len(specs3), 'points': rep3['points'], 's': round(rep3['wall_s'], 3),
                   'solves_per_s': round(rep3['solves_per_s'], 3), 'launches': rep3['launches'],
                   'failed': sum(1 for v in res3.values() if isinstance(v, Exception))}
+        if not args.no_cpu:
+            from oracle import rcwa_oracle as orc
+            small3 = [i for i, s3 in enumerate(specs3) if s3['orders'][0] <= 6][:5]
 
+            def oracle3(j):
+                s3 = specs3[small3[j % len(small3)]]
+                p = s3['points'][0]
+                return orc.solve_raster(s3['rasters'], s3['thickness'], s3['eps_top'],
+                                        s3['eps_bottom'], s3['period'], p['wavelength'], p['theta'],
+                                        p['phi'], p['polarization'], s3['orders'])
+            errs3 = []
+            for j, i in enumerate(small3):
+                if isinstance(res3[i], Exception):
+                    continue
+                ref3 = oracle3(j)['incident']
+                errs3.append(max(float(np.abs(res3[i].R[0] - ref3['R']).max()),
+                                 float(np.abs(res3[i].T[0] - ref3['T']).max())))
+            uneven['oracle_max_abs_err_RT'] = max(errs3) if errs3 else None
+            uneven['oracle_problems'] = small3
+            uneven['cpu_baseline'] = _cpu_baseline(
+                lambda j: (oracle3(j), 1)[1], len(small3), 'solves/s',
+                f'the first {len(small3)} problems with orders <= 6 per axis through the NumPy '
+                'FP64 oracle (the set mean is heavier: orders up to 12)')
+
+    peaks = None if args.no_peaks else _measured_peaks(torch, dev)
     fl_pt = workloads.flops_per_point((hu, hv), L)
     line = {
         'metric': 'RCWA solves/s @ (9,9) orders FP64 vs plain launch; p50/p99 serve latency',
@@ -474,7 +594,7 @@ def main():
         'plain_solves_per_s': round(plain_rate, 4) if plain_rate else None,
         'batched_vs_plain': round(batched_rate / plain_rate, 3) if plain_rate else None,
         'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
-        'roofline': roofline, 'cpu_baseline': cpu, 'stages': stages,
+        'roofline': roofline, 'cpu_baseline': cpu, 'stages': stages, 'measured_peaks': peaks,
         'gradient': grad,
         'serve': serve_report,
         'launch_bound': small,
