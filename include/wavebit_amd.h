@@ -178,6 +178,33 @@ int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const 
                      const double* epz_i, double* eff, double* amp, double* pe, int* flags,
                      void* stream);
 
+/* The layer modes of a batch, the first half of wb_solve: for each (point, layer) pair b = p L + l
+ * the raw layer eigenmodes of rcwa/_engine/solve_2d.py:2303-2581 layer_modes_2d (W = [Ex; Ey]
+ * eigenvectors of Omega2 with unit columns, V = Omega_R W / q, q by the SMM root rule) or of
+ * rcwa/_engine/solve_1d.py:151-253 layer_modes_1d (TE: V = W q; TM: V = epx^-1 W q), before the
+ * s/p rotation of solve_2d.py:2584-2616. W, V: P L x n x n complex; q: P L x n complex. flags
+ * P x L (or NULL): WB_FLAG_NOCONVERGE, the pair screen's bits and, with rebuild = 1 on the 2D
+ * path, WB_FLAG_REBUILT / WB_FLAG_DEFECTIVE (the grazing clusters re-derived as
+ * wb_solve_rebuild does, n <= 2048). The modes depend on the point's wavelength, angles, frame
+ * and half-space media and on the layer's conv matrices, not on its thickness. */
+size_t wb_layer_modes_workspace_size(int P, int L, int hu, int hv, int mode, int rebuild);
+int wb_layer_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                   const int* conv_idx, const double* epx, const double* epy, const double* epz_i,
+                   double* W, double* V, double* q, int* flags, int rebuild, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* The second half of wb_solve: the layer S-matrices, the cascade and the efficiencies from modes
+ * of wb_layer_modes (rcwa/_engine/scattering.py:1185-1324 _smm_sp from its per-layer modes).
+ * Pair b = p L + l takes the modes at index modes_idx[b] of W, V, q (NULL: index b), so modes
+ * computed once serve runs that change only thicknesses, and one wb_layer_modes call can serve
+ * several calls with different layer counts. flags P x L (or NULL) report this call's singular
+ * factorizations only; OR them with the modes' flags. smat is NULL or wb_solve_scattering's
+ * output. The workspace is wb_solve_workspace_size's. Results equal wb_solve's to the bit. */
+int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                        const double* thick, const double* W, const double* V, const double* q,
+                        const int* modes_idx, double* eff, double* amp, double* pe, int* flags,
+                        double* smat, void* workspace, size_t workspace_bytes, void* stream);
+
 /* wb_solve (2D path, WB_MODE_2D) with every grazing layer's mode cluster re-derived from the
  * coupled first-order problem, as the reference always does (rcwa/_engine/solve_2d.py:118-228
  * _rebuild_grazing_cluster, entered at :2463-2485). Meant for the points whose wb_solve flags

@@ -1,0 +1,59 @@
+"""The split solve on the GPU: ``wb_layer_modes`` then ``wb_solve_from_modes`` equal ``wb_solve`` to
+the bit (2D and both 1D paths); one modes batch over buckets of different layer counts serves each
+bucket's cascade exactly as its own fused solve; modes kept from one run serve a run with new
+thicknesses exactly as a fresh solve of those thicknesses (the resident eigensystems)."""
+import numpy as np
+import pytest
+import torch
+
+from wavebit_amd import _native
+from wavebit_amd.rcwa import _engine as eng
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device('cuda:0')
+
+
+def _bucket(rng, L, P, hu, hv, path, n_raster=10):
+    ny = n_raster if path == eng.PATH_2D else 1
+    eps = rng.uniform(1, 9, (L, ny, n_raster)) + 1j * rng.uniform(0, 0.2, (L, ny, n_raster))
+    conv = [_native.assemble_li(torch.from_numpy(eps).to(DEV), hu, hv)[:3]]
+    b = eng.Bucket(hu=hu, hv=hv, path=path, n_layers=L)
+    for _ in range(P):
+        pol = (0.6, 0.8j) if path == eng.PATH_2D else ('s' if path == eng.PATH_TE else 'p')
+        b.points.append(eng.Point(stack=0, wavelength=float(rng.uniform(0.6, 1.1)),
+                                  theta=float(rng.uniform(0, 0.4)),
+                                  phi=float(rng.uniform(0, 6)) if path == eng.PATH_2D else None,
+                                  polarization=pol, eps_top=1.0, eps_bottom=2.25 + 0.1j,
+                                  period=(0.9, 1.0), thickness=tuple(rng.uniform(0.05, 0.3, L))))
+    return b, conv
+
+
+def _equal(a, b, path):
+    nv = 3 if path == eng.PATH_2D else 1        # the 1D paths fill the first variant only
+    for x, y in zip(a[:3], b[:3]):
+        assert torch.equal(x[:, :nv], y[:, :nv]), (x[:, :nv] - y[:, :nv]).abs().max()
+    assert torch.equal(a[3], b[3])
+
+
+@pytest.mark.parametrize('path,hu,hv', [(eng.PATH_2D, 3, 2), (eng.PATH_TE, 8, 0), (eng.PATH_TM, 8, 0)])
+def test_window_of_mixed_layer_counts_equals_fused_solves(path, hu, hv):
+    rng = np.random.default_rng(hu * 10 + len(path))
+    buckets = [_bucket(rng, L, P, hu, hv, path) for L, P in ((1, 3), (3, 2), (2, 4))]
+    out, modes = eng.run_window([b for b, _ in buckets], [c for _, c in buckets], DEV)
+    for (b, conv), got in zip(buckets, out):
+        want = eng.run_bucket(b, conv, DEV)
+        _equal(got[:4], want[:4], path)
+    W, V, q, flags, tables = modes
+    assert W.shape[0] == sum(b.n_layers * len(b.points) for b, _ in buckets)
+
+
+def test_resident_modes_serve_new_thicknesses_exactly():
+    rng = np.random.default_rng(5)
+    b, conv = _bucket(rng, 3, 4, 4, 4, eng.PATH_2D)
+    _, modes = eng.run_window([b], [conv], DEV)
+    import dataclasses
+    b2 = dataclasses.replace(b, points=[dataclasses.replace(p, thickness=tuple(np.asarray(p.thickness) * 1.7))
+                                        for p in b.points])
+    got, _ = eng.run_window([b2], [conv], DEV, modes=modes)
+    want = eng.run_bucket(b2, conv, DEV)
+    _equal(got[0][:4], want[:4], eng.PATH_2D)

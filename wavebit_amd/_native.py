@@ -35,7 +35,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
            'wb_gemm_tuning_clear', 'wb_solve_noalloc', 'wb_solve_rebuild_workspace_size',
-           'wb_solve_rebuild')
+           'wb_solve_rebuild', 'wb_layer_modes_workspace_size', 'wb_layer_modes',
+           'wb_solve_from_modes')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -84,6 +85,12 @@ def load_library():
             'wb_solve_rebuild_workspace_size': (c_sz, [c_int, c_int, c_int, c_int]),
             'wb_solve_rebuild': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+            'wb_layer_modes_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int, c_int]),
+            'wb_layer_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_sz, c_vp]),
+            'wb_solve_from_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                            c_sz, c_vp]),
             'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
@@ -360,6 +367,62 @@ def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=Fals
                                        _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
                                        _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
                                        _ptr(ws), nbytes, _stream(device)), 'solve_rebuild')
+    return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
+
+
+def layer_modes(pts, conv_idx, epx, epy, epz_i, hu, hv, mode, rebuild=False, workspace=None):
+    """The raw layer modes of every (point, layer) pair; see ``wb_layer_modes``. ``conv_idx`` is
+    (P, L). Returns (W, V, q, flags): (P L, n, n), (P L, n, n), (P L, n), (P, L)."""
+    device = epx.device
+    ctx = context(device)
+    P, L = conv_idx.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    n = 2 * N if mode == MODE_2D else N
+    _require(pts, torch.float64, device, 'pts')
+    _require(conv_idx, torch.int32, device, 'conv_idx')
+    for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
+        _require(t, torch.complex128, device, name)
+    W = torch.empty((P * L, n, n), dtype=torch.complex128, device=device)
+    V = torch.empty((P * L, n, n), dtype=torch.complex128, device=device)
+    q = torch.empty((P * L, n), dtype=torch.complex128, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_layer_modes_workspace_size(P, L, hu, hv, mode, int(bool(rebuild)))
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_layer_modes(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(conv_idx),
+                                     _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(W), _ptr(V), _ptr(q),
+                                     _ptr(flags), int(bool(rebuild)), _ptr(workspace), nbytes,
+                                     _stream(device)), 'layer_modes')
+    return W, V, q, flags
+
+
+def solve_from_modes(pts, thick, W, V, q, modes_idx, hu, hv, mode, scattering=False, workspace=None):
+    """The S-matrix half of ``solve`` from modes of ``layer_modes``; see ``wb_solve_from_modes``.
+    ``modes_idx`` (P, L) int32 or None. Returns (eff, amp, pe, flags) and, with ``scattering``,
+    smat; flags report this call's factorizations only."""
+    device = W.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    n = 2 * N if mode == MODE_2D else N
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    for name, t in (('W', W), ('V', V), ('q', q)):
+        _require(t, torch.complex128, device, name)
+    if modes_idx is not None:
+        _require(modes_idx, torch.int32, device, 'modes_idx')
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    smat = torch.empty((P, 4, n, n), dtype=torch.complex128, device=device) if scattering else None
+    nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve_from_modes(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
+                                          _ptr(W), _ptr(V), _ptr(q), _ptr(modes_idx), _ptr(eff),
+                                          _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
+                                          _ptr(workspace), nbytes, _stream(device)), 'solve_from_modes')
     return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
 
 

@@ -259,6 +259,59 @@ int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pt
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_rebuild");
 }
 
+size_t wb_layer_modes_workspace_size(int P, int L, int hu, int hv, int mode, int rebuild) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
+  const size_t base = align_up(wb_solve_workspace_bytes(P, L, hu, hv, mode));
+  if (!rebuild || mode != WB_MODE_2D) return base;
+  return base + wb_graze_bytes(P * L, 2 * (2 * hu + 1) * (2 * hv + 1));
+}
+
+int wb_layer_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                   const int* conv_idx, const double* epx, const double* epy, const double* epz_i,
+                   double* W, double* V, double* q, int* flags, int rebuild, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_layer_modes: bad dimensions");
+  if (!pts || !conv_idx || !epx || !epy || !epz_i || !W || !V || !q)
+    return fail(ctx, WB_BADARG, "wb_layer_modes: null buffer");
+  if (rebuild && mode == WB_MODE_2D && 2 * (2 * hu + 1) * (2 * hv + 1) > WB_GRAZE_NMAX)
+    return fail(ctx, WB_UNSUPPORTED, "wb_layer_modes: the grazing rebuild takes mode orders up to 2048");
+  if (workspace_bytes < wb_layer_modes_workspace_size(P, L, hu, hv, mode, rebuild))
+    return fail(ctx, WB_OOM, "wb_layer_modes: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = nullptr; a.conv_idx = conv_idx;
+  a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.eff = nullptr; a.amp = nullptr; a.pe = nullptr; a.flags = flags; a.ck = nullptr;
+  a.modes_out = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q;
+  if (rebuild && mode == WB_MODE_2D)
+    a.graze = (char*)workspace + align_up(wb_solve_workspace_bytes(P, L, hu, hv, mode));
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_layer_modes");
+}
+
+int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                        const double* thick, const double* W, const double* V, const double* q,
+                        const int* modes_idx, double* eff, double* amp, double* pe, int* flags,
+                        double* smat, void* workspace, size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_solve_from_modes: bad dimensions");
+  if (!pts || !thick || !W || !V || !q || !eff || !amp || !pe)
+    return fail(ctx, WB_BADARG, "wb_solve_from_modes: null buffer");
+  if (workspace_bytes < wb_solve_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_from_modes: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = nullptr;
+  a.epx = nullptr; a.epy = nullptr; a.epz = nullptr;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
+  a.modes_in = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q; a.modes_idx = modes_idx;
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_from_modes");
+}
+
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                         const double* thick, const int* conv_idx, const double* epx,
                         const double* epy, const double* epz_i, double* eff, double* amp, double* pe,

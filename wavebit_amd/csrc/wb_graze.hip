@@ -316,6 +316,22 @@ __global__ void k_graze_patch(int N, int L, const cd* ov, cd* bigW, cd* bigV, cd
   if (r == 0) q[b * n + c] = g.qc[b * K + i];
 }
 
+// The rebuilt columns over columns J of the raw (unrotated) W and V of the split solve's modes
+// stage (pair b at b n^2), and their roots over q.
+__global__ void k_graze_patch_raw(int n, cd* W, cd* V, cd* q, wb_graze_ws g) {
+  const long long b = blockIdx.y;
+  if (g.status[b] != WB_GRAZE_REBUILT) return;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * K) return;
+  const int r = (int)(e / K), i = (int)(e % K);
+  if (i >= g.jn[b]) return;
+  const int c = g.jidx[b * K + i];
+  const long long base = b * (long long)n * n;
+  W[base + (long long)r * n + c] = g.WC[b * (long long)n * K + (long long)r * K + i];
+  V[base + (long long)r * n + c] = g.VC[b * (long long)n * K + (long long)r * K + i];
+  if (r == 0) q[b * n + c] = g.qc[b * K + i];
+}
+
 // Flags: a rebuilt layer is marked WB_FLAG_REBUILT and its pair screen cleared (the reference
 // returns the rebuilt modes without the pair gates); a defective cluster is WB_FLAG_DEFECTIVE.
 __global__ void k_graze_flags(int B, const int* status, int* flags) {
@@ -421,6 +437,18 @@ hipError_t wb_graze_patch(int N, int L, int B, const cd* ov, cd* bigW, cd* bigV,
                           const wb_graze_ws& g, int* flags, hipStream_t st) {
   hipLaunchKernelGGL(k_graze_patch, dim3((unsigned)(((long long)N * K + 255) / 256), (unsigned)B),
                      dim3(256), 0, st, N, L, ov, bigW, bigV, q, g);
+  WB_CHECK_LAUNCH();
+  if (flags) {
+    hipLaunchKernelGGL(k_graze_flags, dim3((B + 255) / 256), dim3(256), 0, st, B, g.status, flags);
+    WB_CHECK_LAUNCH();
+  }
+  return hipSuccess;
+}
+
+hipError_t wb_graze_patch_raw(int n, int B, cd* W, cd* V, cd* q, const wb_graze_ws& g, int* flags,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_graze_patch_raw, dim3((unsigned)(((long long)n * K + 255) / 256), (unsigned)B),
+                     dim3(256), 0, st, n, W, V, q, g);
   WB_CHECK_LAUNCH();
   if (flags) {
     hipLaunchKernelGGL(k_graze_flags, dim3((B + 255) / 256), dim3(256), 0, st, B, g.status, flags);

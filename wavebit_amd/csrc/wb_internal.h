@@ -75,6 +75,14 @@ struct wb_solve_args {
   cd* smat = nullptr;       // optional P x 4 x n x n stack scattering matrix (S11, S12, S21, S22)
   void* graze = nullptr;    // 2D: scratch of the grazing-cluster rebuild (wb_graze_bytes), or
                             // nullptr for no rebuild
+  // Split solve. modes_out: compute the layer modes only and write them raw (unrotated W, V and
+  // q, pair b at b n^2 / b n) to modes_W / modes_V / modes_q, then return. modes_in: take the
+  // modes of pair b from index modes_idx[b] (or b) of those arrays instead of computing them.
+  int modes_out = 0, modes_in = 0;
+  cd* modes_W = nullptr;
+  cd* modes_V = nullptr;
+  cd* modes_q = nullptr;
+  const int* modes_idx = nullptr;
 };
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

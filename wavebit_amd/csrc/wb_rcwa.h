@@ -75,6 +75,8 @@ hipError_t wb_graze_stage(int N, int L, int B, const cd* w, const cd* W, BatchPt
                           hipStream_t st);
 hipError_t wb_graze_patch(int N, int L, int B, const cd* ov, cd* bigW, cd* bigV, cd* q,
                           const wb_graze_ws& g, int* flags, hipStream_t st);
+hipError_t wb_graze_patch_raw(int n, int B, cd* W, cd* V, cd* q, const wb_graze_ws& g, int* flags,
+                              hipStream_t st);
 
 size_t wb_adjoint_workspace_bytes(int P, int L, int hu, int hv, int mode);
 struct wb_adjoint_args {

@@ -232,6 +232,62 @@ __global__ void k_modes_post_1d(int N, const cd* Bw, const cd* w, cd* V, cd* q) 
   V[b * (long long)N * N + e] = cmul(Bw[b * (long long)N * N + e], qc);
 }
 
+// 2D raw modes for the split solve: q = mode_q(w) and V = Omega_R W / q unrotated (the values
+// k_modes_post_2d rotates), W copied; pair b of the output at b n^2.
+__global__ void k_modes_raw_2d(int N, int L, const cd* W, const cd* Vg, const cd* w, const cd* ov,
+                               cd* Wout, cd* Vout, cd* q) {
+  const long long b = blockIdx.y;
+  const int p = (int)(b / L);
+  const int n = 2 * N;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)N * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const long long base = b * (long long)n * n;
+  const cd qc = mode_q(w[b * n + c]);
+  if (r == 0) q[b * n + c] = qc;
+  const cd kx = o[OV_KX * N + r], ky = o[OV_KY * N + r];
+  const cd Wt = W[base + (long long)r * n + c], Wb = W[base + (long long)(N + r) * n + c];
+  cd Vt = cadd(Vg[base + (long long)r * n + c],
+               cadd(cmul(cneg(cmul(kx, ky)), Wt), cmul(cmul(kx, kx), Wb)));
+  cd Vb = cadd(Vg[base + (long long)(N + r) * n + c],
+               cadd(cmul(cneg(cmul(ky, ky)), Wt), cmul(cmul(ky, kx), Wb)));
+  Wout[base + (long long)r * n + c] = Wt;
+  Wout[base + (long long)(N + r) * n + c] = Wb;
+  Vout[base + (long long)r * n + c] = cdiv(Vt, qc);
+  Vout[base + (long long)(N + r) * n + c] = cdiv(Vb, qc);
+}
+
+// The split solve's input modes: pair b's raw W, V, q from index src = idx ? idx[b] : b of the
+// caller's arrays, rotated into the s/p basis as k_modes_post_2d rotates them (2D), or copied
+// (1D: big_W = W, big_V = V).
+__global__ void k_modes_take(int N, int n, int L, int mode, const cd* Win, const cd* Vin,
+                             const cd* qin, const int* idx, const cd* ov, cd* bigW, cd* bigV, cd* q) {
+  const long long b = blockIdx.y;
+  const long long src = idx ? (long long)idx[b] : b;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long sb = src * (long long)n * n, db = b * (long long)n * n;
+  if (mode != 0) {
+    if (e >= (long long)n * n) return;
+    bigW[db + e] = Win[sb + e];
+    bigV[db + e] = Vin[sb + e];
+    if (e < n) q[b * n + e] = qin[src * n + e];
+    return;
+  }
+  if (e >= (long long)N * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const int p = (int)(b / L);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const double fc = o[OV_FC * N + r].x, fs = o[OV_FS * N + r].x;
+  if (r == 0) q[b * n + c] = qin[src * n + c];
+  const cd Wt = Win[sb + (long long)r * n + c], Wb = Win[sb + (long long)(N + r) * n + c];
+  const cd Vt = Vin[sb + (long long)r * n + c], Vb = Vin[sb + (long long)(N + r) * n + c];
+  bigW[db + (long long)r * n + c] = csub(cscale(Wb, fc), cscale(Wt, fs));
+  bigW[db + (long long)(N + r) * n + c] = cadd(cscale(Wt, fc), cscale(Wb, fs));
+  bigV[db + (long long)r * n + c] = cadd(cscale(Vt, fc), cscale(Vb, fs));
+  bigV[db + (long long)(N + r) * n + c] = csub(cscale(Vb, fc), cscale(Vt, fs));
+}
+
 // A = Wi diag(wg) + Vi diag(vg), B = Wi diag(wg) - Vi diag(vg) for the gap of each pair.
 // 2D gap: wg = [1, -i y], vg = [-i y, 1]; 1D TE: wg = 1, vg = -i y; 1D TM: same with z_p = y.
 __global__ void k_ab(int N, int n, int L, int mode, const cd* Wi, const cd* Vi, const cd* ov,
@@ -761,7 +817,14 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
 
   // ---------------------------------------------------------------- layer modes
   cd *bigW, *bigV;
-  if (mode == 0) {
+  if (a.modes_in) {
+    // the modes of an earlier wb_layer_modes call (resident eigensystems, batches whose layer
+    // counts differ): rotated into the s/p basis as the fused path does
+    hipLaunchKernelGGL(k_modes_take, grid2(mode == 0 ? (long long)N * n : nn, B), dim3(256), 0, st, N,
+                       n, L, mode, a.modes_W, a.modes_V, a.modes_q, a.modes_idx, ov, M[1], M[2], q);
+    WB_CHECK_LAUNCH();
+    bigW = M[1]; bigV = M[2];
+  } else if (mode == 0) {
     hipLaunchKernelGGL(k_omega_pre, grid2(NN, B), dim3(256), 0, st, N, L, EPZ, ov, M[1], M[0]);
     WB_CHECK_LAUNCH();
     // Omega2[:, 0:N] += [B; E] epx ; Omega2[:, N:2N] += [Cc; D] epy
@@ -790,6 +853,17 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                              S(M[4], nn), n, B, st))) return e;
     if ((e = wb_zgemm_launch(N, n, N, one, EPX, N, S(M[3], nn), n, zero,
                              S(M[4] + (long long)N * n, nn), n, B, st))) return e;
+    if (a.modes_out) {
+      // the split solve's first half: raw W, V, q to the caller, the rebuilt grazing columns over
+      // them, and stop
+      hipLaunchKernelGGL(k_modes_raw_2d, grid2((long long)N * n, B), dim3(256), 0, st, N, L, M[3],
+                         M[4], w, ov, a.modes_W, a.modes_V, a.modes_q);
+      WB_CHECK_LAUNCH();
+      if (graze) {
+        if ((e = wb_graze_patch_raw(n, B, a.modes_W, a.modes_V, a.modes_q, G, a.flags, st))) return e;
+      }
+      return hipSuccess;
+    }
     hipLaunchKernelGGL(k_modes_post_2d, grid2((long long)N * n, B), dim3(256), 0, st, N, L, M[3],
                        M[4], w, ov, M[1], M[2], q);
     WB_CHECK_LAUNCH();
@@ -821,10 +895,13 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       Bw = M[4];
     }
     // W stays in M3; V into M2
-    hipLaunchKernelGGL(k_modes_post_1d, grid2(NN, B), dim3(256), 0, st, N, Bw, w, M[2], q);
+    hipLaunchKernelGGL(k_modes_post_1d, grid2(NN, B), dim3(256), 0, st, N, Bw, w,
+                       a.modes_out ? a.modes_V : M[2], a.modes_out ? a.modes_q : q);
     WB_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, S(M[3], nn), N, S(M[1], nn), N);
+    hipLaunchKernelGGL(k_copy, grid2(NN, B), dim3(256), 0, st, N, N, S(M[3], nn), N,
+                       S(a.modes_out ? a.modes_W : M[1], nn), N);
     WB_CHECK_LAUNCH();
+    if (a.modes_out) return hipSuccess;
     bigW = M[1]; bigV = M[2];
   }
 

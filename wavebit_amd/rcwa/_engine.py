@@ -185,7 +185,9 @@ def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
     number of points solved again."""
     if bucket.path != PATH_2D:
         return 0
-    bad = np.flatnonzero(np.any(np.asarray(flags) & _native.FLAG_GRAZING, axis=1))
+    fl = np.asarray(flags)
+    untried = (fl & _native.FLAG_GRAZING) & ~(fl & (_native.FLAG_REBUILT | _native.FLAG_DEFECTIVE))
+    bad = np.flatnonzero(np.any(untried != 0, axis=1))
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
     if not bad.size or 2 * N > _native.GRAZE_NMAX:
         return 0
@@ -199,6 +201,78 @@ def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
     if smat is not None:
         smat[bad] = host[4]
     return int(bad.size)
+
+
+def _pair_inputs(buckets, convs, device):
+    """One pair per (bucket, point, layer): its point record, the index of its layer's conv
+    matrices in the concatenation of every bucket's stacks, and per bucket the (P, L) table of
+    pair indices. Returns (pts (pairs, 16), conv_idx (pairs, 1), epx, epy, epz, tables)."""
+    recs, cidx, tables = [], [], []
+    base = 0
+    for bucket, conv in zip(buckets, convs):
+        L = bucket.n_layers
+        table = np.empty((len(bucket.points), L), dtype=np.int32)
+        for p, pt in enumerate(bucket.points):
+            rec = point_record(pt, bucket.path)
+            for l in range(L):
+                table[p, l] = len(recs)
+                recs.append(rec)
+                cidx.append(base + pt.stack * L + l)
+        tables.append(table)
+        base += sum(int(c[0].shape[0]) for c in conv)
+    epx = torch.cat([c[0] for conv in convs for c in conv]).contiguous()
+    epy = torch.cat([c[1] for conv in convs for c in conv]).contiguous()
+    epz = torch.cat([c[2] for conv in convs for c in conv]).contiguous()
+    pts = torch.from_numpy(np.array(recs, dtype=np.float64)).to(device)
+    idx = torch.from_numpy(np.array(cidx, dtype=np.int32).reshape(-1, 1)).to(device)
+    return pts, idx, epx, epy, epz, tables
+
+
+def window_modes(buckets, convs, device):
+    """The layer modes of every (point, layer) pair of several buckets that share an order window
+    and path, whatever their layer counts, in one ``wb_layer_modes`` launch sequence (one
+    eigensolve batch); pairs with a grazing layer are solved again with the cluster rebuilt.
+    Returns (W, V, q, flags (pairs,), tables): ``tables[i][p, l]`` is the pair index of layer l of
+    point p of bucket i."""
+    b0 = buckets[0]
+    mode = _MODE[b0.path]
+    pts, idx, epx, epy, epz, tables = _pair_inputs(buckets, convs, device)
+    W, V, q, flags = _native.layer_modes(pts, idx, epx, epy, epz, b0.hu, b0.hv, mode)
+    flags = flags.reshape(-1)
+    N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
+    if mode == _native.MODE_2D and 2 * N <= _native.GRAZE_NMAX:
+        bad = torch.nonzero(flags & _native.FLAG_GRAZING).reshape(-1)
+        if bad.numel():
+            W2, V2, q2, f2 = _native.layer_modes(pts[bad].contiguous(), idx[bad].contiguous(), epx,
+                                                 epy, epz, b0.hu, b0.hv, mode, rebuild=True)
+            W[bad], V[bad], q[bad], flags[bad] = W2, V2, q2, f2.reshape(-1)
+    return W, V, q, flags, tables
+
+
+def run_window(buckets, convs, device, modes=None):
+    """Solve buckets of one order window and path with different layer counts: their modes in one
+    batch (``window_modes``, or ``modes`` from an earlier call on the same points), then each
+    bucket's S-matrices and cascade from them (``wb_solve_from_modes``). Results equal
+    ``run_bucket``'s per bucket to the bit. Returns [(eff, amp, pe, flags, N)] per bucket, device
+    tensors, flags combining the modes' and the cascade's, and the modes used."""
+    b0 = buckets[0]
+    mode = _MODE[b0.path]
+    N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
+    if modes is None:
+        modes = window_modes(buckets, convs, device)
+    W, V, q, mflags, tables = modes
+    out = []
+    for bucket, table in zip(buckets, tables):
+        P, L = len(bucket.points), bucket.n_layers
+        rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+        thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+        tab = torch.from_numpy(table).to(device)
+        eff, amp, pe, flags = _native.solve_from_modes(
+            torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device), W, V, q, tab,
+            bucket.hu, bucket.hv, mode)
+        flags = flags | mflags[tab.reshape(-1).long()].reshape(P, L)
+        out.append((eff, amp, pe, flags, N))
+    return out, modes
 
 
 def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
