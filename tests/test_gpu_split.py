@@ -57,3 +57,34 @@ def test_resident_modes_serve_new_thicknesses_exactly():
     got, _ = eng.run_window([b2], [conv], DEV, modes=modes)
     want = eng.run_bucket(b2, conv, DEV)
     _equal(got[0][:4], want[:4], eng.PATH_2D)
+
+
+def test_prepared_keeps_layer_eigensystems_for_a_thickness_sweep():
+    """A preparation above the captured-graph size keeps its layer modes on the device: a run that
+    changes only the thicknesses reuses them (no eigensolve) and equals a fresh preparation's run
+    of the same thicknesses to the bit; new wavelengths recompute them."""
+    import wavebit_amd
+    from wavebit_amd import rcwa
+    rng = np.random.default_rng(9)
+    eps = rng.uniform(1, 9, (3, 16, 16)) + 1j * rng.uniform(0, 0.1, (3, 16, 16))
+
+    def problem(thick, wls):
+        return rcwa.Problem(
+            stack=rcwa.Stack(layers=[rcwa.Layer(thickness=t, epsilon=e) for t, e in zip(thick, eps)],
+                             top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25),
+                             period=(0.8, 0.9)),
+            excitation=rcwa.Excitation(wavelength=wls, sources=(rcwa.PlaneWave(polarization='p',
+                                                                               theta=0.2, phi=0.5),)),
+            units=wavebit_amd.Units(length='um'))
+    wls = list(np.linspace(0.7, 1.0, 8))
+    num = rcwa.Numerics(orders=(6, 6))
+    prep = rcwa.prepare(problem([0.1, 0.2, 0.15], wls), numerics=num, backend='torch', device='cuda:0')
+    prep.run(problem([0.1, 0.2, 0.15], wls))
+    first = prep._cache.modes
+    assert first is not None
+    res = prep.run(problem([0.12, 0.25, 0.11], wls))
+    assert prep._cache.modes[1] is first[1]           # the same device buffers: no eigensolve
+    fresh = rcwa.run(problem([0.12, 0.25, 0.11], wls), numerics=num, backend='torch', device='cuda:0')
+    assert np.array_equal(res.R, fresh.R) and np.array_equal(res.T, fresh.T)
+    prep.run(problem([0.12, 0.25, 0.11], [w * 1.01 for w in wls]))
+    assert prep._cache.modes[1] is not first[1]

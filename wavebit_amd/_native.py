@@ -370,6 +370,10 @@ def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=Fals
     return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
 
 
+def layer_modes_bytes(P, L, hu, hv, mode, rebuild=False):
+    return int(load_library().wb_layer_modes_workspace_size(P, L, hu, hv, mode, int(bool(rebuild))))
+
+
 def layer_modes(pts, conv_idx, epx, epy, epz_i, hu, hv, mode, rebuild=False, workspace=None):
     """The raw layer modes of every (point, layer) pair; see ``wb_layer_modes``. ``conv_idx`` is
     (P, L). Returns (W, V, q, flags): (P L, n, n), (P L, n, n), (P L, n), (P, L)."""

@@ -203,6 +203,19 @@ def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
     return int(bad.size)
 
 
+# the point-record entries a layer's modes depend on: the wavelength, the incident in-plane
+# wavevector and the cell (the frame azimuth, the half-spaces and the polarization enter later)
+_MODES_FIELDS = (0, 1, 2, 4, 5, 14, 15)
+
+
+def modes_key(bucket):
+    """What the layer modes of a bucket's points depend on besides their conv matrices: the
+    records' wavelength, incident wavevector and cell entries, as bytes."""
+    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+    return (bucket.hu, bucket.hv, bucket.n_layers, rec[:, _MODES_FIELDS].tobytes(),
+            tuple(p.stack for p in bucket.points))
+
+
 def _pair_inputs(buckets, convs, device):
     """One pair per (bucket, point, layer): its point record, the index of its layer's conv
     matrices in the concatenation of every bucket's stacks, and per bucket the (P, L) table of

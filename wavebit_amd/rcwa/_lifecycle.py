@@ -395,20 +395,24 @@ class _Cache:
     def __init__(self, device):
         self.conv = _engine.ConvCache(device)
         self.graph = [None, None]    # captured solve of this preparation (launch-bound sizes)
+        self.modes = None            # (key, modes) of the last run's layer eigensystems
 
     def sharing_conv(self):
-        """A cache with the same convolution matrices and no captured solve (for the points a
-        detuning ladder solves again)."""
+        """A cache with the same convolution matrices and no captured solve or modes (for the
+        points a detuning ladder solves again)."""
         other = _Cache.__new__(_Cache)
-        other.conv, other.graph = self.conv, [None, None]
+        other.conv, other.graph, other.modes = self.conv, [None, None], None
         return other
+
+    @staticmethod
+    def key(rasters, hu, hv, factorization='li'):
+        return (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
+                rasters.shape, hu, hv, factorization)
 
     def get(self, rasters, hu, hv, factorization='li'):
         import torch
-        key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
-               rasters.shape, hu, hv, factorization)
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.conv.device)
-        return self.conv.get(key, eps, hu, hv, factorization)
+        return self.conv.get(self.key(rasters, hu, hv, factorization), eps, hu, hv, factorization)
 
 
 @dataclass(frozen=True)
@@ -721,6 +725,15 @@ def _solve_resolved(r, cache):
     if r.scattering:
         eff, amp, pe, flags, _, smat = _engine.run_bucket_scattering(bucket, [conv], device)
         smat = smat.cpu().numpy()
+    elif _resident_modes(bucket):
+        # the layer eigensystems stay on the device between runs: a run that changes only
+        # thicknesses (or the polarization) takes them from the last run and skips the eigensolves
+        mkey = (_Cache.key(r.rasters, r.hu, r.hv, r.factorization), r.path,
+                _engine.modes_key(bucket))
+        kept = cache.modes[1] if cache.modes is not None and cache.modes[0] == mkey else None
+        outs, modes = _engine.run_window([bucket], [[conv]], device, modes=kept)
+        cache.modes = (mkey, modes)
+        eff, amp, pe, flags, _ = outs[0]
     else:
         eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
     eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
@@ -729,6 +742,22 @@ def _solve_resolved(r, cache):
         smats = _scattering_matrices(r, smat)
     per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
     return per_wl, flags, smats
+
+
+# Preparations whose fused solve needs more workspace than a captured graph is kept for, and whose
+# modes take at most this many bytes, keep their layer eigensystems resident between runs.
+RESIDENT_MODES_BYTES = 32 << 30
+
+
+def _resident_modes(bucket):
+    mode = _engine._MODE[bucket.path]
+    P, L = len(bucket.points), bucket.n_layers
+    N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
+    n = 2 * N if mode == _engine._native.MODE_2D else N
+    from . import graphs
+    if graphs.worth_capturing(P, L, bucket.hu, bucket.hv, mode):
+        return False
+    return P * L * (2 * n * n + n) * 16 <= RESIDENT_MODES_BYTES
 
 
 def _scattering_matrices(r, smat):

@@ -1,9 +1,11 @@
 """Continuous-batching RCWA server on one GPU.
 
 Requests (a ``Problem`` with its ``Numerics``) arrive over time. Each is resolved into plane-wave
-points; points wait in per-bucket queues keyed by (order window, layer count, solve path), and
-a launch packs queued points of one bucket -- from any number of requests -- into one
-``wb_solve`` launch sequence. Launches run on several HIP streams at once, so work admitted
+points; points wait in per-window queues keyed by (order window, solve path), and a launch
+packs queued points of one window -- from any number of requests, whatever their layer counts --
+into one launch sequence: the layer eigensolves of all its (point, layer) pairs as one batch
+(``wb_layer_modes``), then one cascade per layer count (``wb_solve_from_modes``). Launches run on
+several HIP streams at once, so work admitted
 while others are on the device overlaps with them. Admission is by HBM footprint: a launch
 is started only while the workspaces of all launches in flight plus the device-resident
 convolution matrices fit the memory budget; a request's points are admitted as slots free
@@ -13,7 +15,8 @@ last point has been solved; its latency is completion time minus arrival time.
 
 A request may also ask for the gradient of its total transmitted efficiency (the sum of T over
 its wavelengths and orders) with respect to every raster pixel and layer thickness. Such a
-request runs as a launch of its own through the hand-written reverse pass (forward with
+request runs as launches of its own (its points in chunks that fit half the HBM budget; the
+chunks' gradients add up) through the hand-written reverse pass (forward with
 per-layer checkpoints, then the adjoint), on a server stream and under the same HBM
 admission, driven by a worker thread so the loop keeps admitting and launching forward work
 meanwhile; its result carries ``diagnostics['gradient']``.
@@ -159,20 +162,41 @@ class Server:
                 stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                 polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                 period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct), conv))
-        key = (r.hu, r.hv, r.path, r.rasters.shape[0])
+        # forward points queue per order window and path, whatever their layer count: a launch
+        # batches the eigensolves of all its (point, layer) pairs, then runs one cascade per
+        # layer count (engine.run_window)
+        key = (r.hu, r.hv, r.path)
+        work.layers = r.rasters.shape[0]
         return work, key, pts
 
-    def _launch_bytes(self, key, P):
-        hu, hv, path, L = key
+    def _launch_bytes(self, key, items):
+        """Device bytes of a window launch over ``items``: the modes of every pair and the
+        workspace that computes them, and the largest cascade workspace plus the outputs."""
+        hu, hv, path = key
+        mode = _engine._MODE[path]
         N = (2 * hu + 1) * (2 * hv + 1)
-        out = P * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8 + L * 4)
-        return _native.solve_workspace_bytes(P, L, hu, hv, _engine._MODE[path]) + out
+        n = 2 * N if mode == _native.MODE_2D else N
+        groups = collections.Counter(int(conv[0].shape[0]) for _, _, _, conv in items)
+        pairs = sum(L * c for L, c in groups.items())
+        modes = pairs * (2 * n * n + n) * 16 + _native.layer_modes_bytes(pairs, 1, hu, hv, mode)
+        cascade = max(_native.solve_workspace_bytes(c, L, hu, hv, mode) for L, c in groups.items())
+        out = len(items) * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8) + pairs * 8
+        return modes + cascade + out
 
     def _gradient_bytes(self, key, P):
         hu, hv, path, L = key
         mode = _engine._MODE[path]
         return (2 * _native.solve_workspace_bytes(P, L, hu, hv, mode)
                 + _native.checkpoint_bytes(P, L, hu, hv, mode))
+
+    def _gradient_chunk(self, key, n_points):
+        """Points per forward-and-reverse launch of a gradient request: all of them when they fit
+        half the HBM budget, else the most that do (at least one). The objective, a sum over the
+        points, is additive, so the chunks' gradients add up to the request's."""
+        P = n_points
+        while P > 1 and self._gradient_bytes(key, P) > self.budget // 2:
+            P = (P + 1) // 2
+        return P
 
     def _gradient(self, work, stream):
         """Forward with checkpoints and the reverse pass for one request, on ``stream``: fills
@@ -187,18 +211,23 @@ class Server:
                                 polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                                 period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
                   for wl in r.wavelengths]
+        chunk = self._gradient_chunk((r.hu, r.hv, r.path, r.rasters.shape[0]), len(points))
+        parts = []
         with torch.cuda.stream(stream):
             eps = torch.tensor(np.ascontiguousarray(r.rasters), dtype=torch.complex128, device=self.device,
                                requires_grad=True)
             thick = torch.tensor(np.asarray(r.thickness, dtype=np.float64), device=self.device,
                                  requires_grad=True)
-            eff, amp, pe, flags = adjoint.solve(eps, thick, points, r.hu, r.hv, self.device,
-                                                factorization=r.factorization)
-            (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
+            for c0 in range(0, len(points), chunk):
+                eff, amp, pe, flags = adjoint.solve(eps, thick, points[c0:c0 + chunk], r.hu, r.hv,
+                                                    self.device, factorization=r.factorization)
+                (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()      # accumulates into .grad
+                parts.append(tuple(t.detach().cpu().numpy() for t in (eff, amp, pe, flags)))
+                del eff, amp, pe, flags
             d_eps, d_t = eps.grad.detach().cpu().numpy(), thick.grad.detach().cpu().numpy()
             if r.side == 'bottom':               # solved on the mirrored stack: back to the declared order
                 d_eps, d_t = d_eps[::-1].copy(), d_t[::-1].copy()
-            eff_h, amp_h, pe_h, fl = (t.detach().cpu().numpy() for t in (eff, amp, pe, flags))
+            eff_h, amp_h, pe_h, fl = (np.concatenate([p_[i] for p_ in parts]) for i in range(4))
         for i in range(work.n_points):
             work.flags |= int(np.bitwise_or.reduce(fl[i]))
             work.results[i] = _engine.unpack(eff_h, amp_h, pe_h, i, r.hu, r.hv, r.path)
@@ -216,39 +245,47 @@ class Server:
         return replace(result, diagnostics=diag)
 
     def _launch(self, key, items, stream):
-        hu, hv, path, L = key
-        convs, index = [], {}
-        pts = []
-        for work, i, pt, conv in items:
-            cid = id(conv[0])
-            if cid not in index:
-                index[cid] = len(convs)
-                convs.append(conv)
-            pts.append(dataclasses.replace(pt, stack=index[cid]))   # every other field as admitted
-        bucket = _engine.Bucket(hu=hu, hv=hv, path=path, n_layers=L, points=pts)
+        """One window launch: the items grouped by layer count into buckets (each with its own
+        conv stacks), their modes in one batch, one cascade per bucket."""
+        hu, hv, path = key
+        groups = collections.OrderedDict()
+        for item in items:
+            groups.setdefault(int(item[3][0].shape[0]), []).append(item)
+        buckets, convs, members = [], [], []
+        for L, group in groups.items():
+            cs, index, pts = [], {}, []
+            for work, i, pt, conv in group:
+                cid = id(conv[0])
+                if cid not in index:
+                    index[cid] = len(cs)
+                    cs.append(conv)
+                pts.append(dataclasses.replace(pt, stack=index[cid]))   # every other field as admitted
+            buckets.append(_engine.Bucket(hu=hu, hv=hv, path=path, n_layers=L, points=pts))
+            convs.append(cs)
+            members.append(group)
         with torch.cuda.stream(stream):
-            eff, amp, pe, flags, _ = _engine.run_bucket(bucket, convs, self.device)
+            outs, _ = _engine.run_window(buckets, convs, self.device)
             done = torch.cuda.Event()
             done.record(stream)
-        return dict(key=key, items=items, out=(eff, amp, pe, flags), event=done, bucket=bucket,
-                    convs=convs,
-                    bytes=self._launch_bytes(key, len(items)), stream=stream)
+        return dict(key=key, items=items, outs=outs, members=members, buckets=buckets, convs=convs,
+                    event=done, bytes=self._launch_bytes(key, items), stream=stream)
 
     def _finish(self, launch, now, finished):
-        eff, amp, pe, flags = (t.cpu().numpy() for t in launch['out'])
-        hu, hv, path, L = launch['key']
-        # points with a grazing layer are solved again with the cluster rebuilt, on the launch's
-        # stream (its outputs are complete: the launch's event has been reached)
-        with torch.cuda.stream(launch['stream']):
-            _engine.rebuild_grazing(launch['bucket'], launch['convs'], self.device, eff, amp, pe,
-                                    flags)
-        for p, (work, i, _, _) in enumerate(launch['items']):
-            work.flags |= int(np.bitwise_or.reduce(flags[p]))
-            work.results[i] = _engine.unpack(eff, amp, pe, p, hu, hv, path)
-            work.done += 1
-            if work.done == work.n_points:
-                work.t_done = now
-                finished.append(work)
+        hu, hv, path = launch['key']
+        for out, bucket, convs, group in zip(launch['outs'], launch['buckets'], launch['convs'],
+                                             launch['members']):
+            eff, amp, pe, flags = (t.cpu().numpy() for t in out[:4])
+            # a grazing cluster the window's rebuild did not settle is tried once more through
+            # the fused rebuild, on the launch's stream (its outputs are complete)
+            with torch.cuda.stream(launch['stream']):
+                _engine.rebuild_grazing(bucket, convs, self.device, eff, amp, pe, flags)
+            for p, (work, i, _, _) in enumerate(group):
+                work.flags |= int(np.bitwise_or.reduce(flags[p]))
+                work.results[i] = _engine.unpack(eff, amp, pe, p, hu, hv, path)
+                work.done += 1
+                if work.done == work.n_points:
+                    work.t_done = now
+                    finished.append(work)
 
     def _result(self, work):
         r = work.resolved
@@ -335,7 +372,8 @@ class Server:
                 self.stats['gradient_requests'] += 1
             if grads and grad_job is None and free_streams:
                 work, key = grads[0]
-                need = self._gradient_bytes(key, work.n_points)
+                gkey = key + (work.layers,)
+                need = self._gradient_bytes(gkey, self._gradient_chunk(gkey, work.n_points))
                 if self.conv.bytes + used + need <= self.budget or not inflight:
                     grads.popleft()
                     stream = free_streams.pop()
@@ -353,9 +391,11 @@ class Server:
                     break
                 q = queues[key]
                 P = min(len(q), self.max_points)
-                while P > 1 and self.conv.bytes + used + self._launch_bytes(key, P) > self.budget:
+                while P > 1 and (self.conv.bytes + used
+                                 + self._launch_bytes(key, list(itertools.islice(q, P))) > self.budget):
                     P //= 2
-                if self.conv.bytes + used + self._launch_bytes(key, P) > self.budget and inflight:
+                if (self.conv.bytes + used + self._launch_bytes(key, list(itertools.islice(q, P)))
+                        > self.budget and inflight):
                     continue
                 items = [q.popleft() for _ in range(P)]
                 if not q:
