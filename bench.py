@@ -143,6 +143,13 @@ def main():
                     help='points of the crossed-grating leg (orders (7,7), 3 layers of 256x256; 0 skips)')
     ap.add_argument('--uneven-problems', type=int, default=512,
                     help='problems of the uneven-batch leg (first of the seeded 2048; 0 skips)')
+    ap.add_argument('--sweep-runs', type=int, default=5,
+                    help='thickness-sweep runs of the prepared (9,9) problem on resident modes (0 skips)')
+    ap.add_argument('--cfg5-requests', type=int, default=12,
+                    help='prefix of the seeded serving stream (orders 5..15, 1-10 layers, 64^2-256^2, '
+                         '1-64 wavelengths, 30%% gradient) served in the cfg5 leg (0 skips)')
+    ap.add_argument('--cfg5-rate', type=float, default=0.25,
+                    help='cfg5 request arrivals per second (about what the device sustains)')
     ap.add_argument('--serve-grad-fraction', type=float, default=0.3,
                     help='fraction of served requests that also ask for the raster gradient')
     ap.add_argument('--autotune', type=int, default=1,
@@ -400,6 +407,75 @@ def main():
                                   'U[1,16]+iU[0,1], 1-8 wavelengths in [400,1000] nm, Poisson arrivals, '
                                   f'{args.serve_grad_fraction:.0%} also asking for the raster gradient'}
 
+    # resident eigensystems: the prepared (9,9) problem run again with new thicknesses takes its
+    # layer modes from the last run (no eigensolve), the S-matrix work and cascade only
+    sweep = None
+    if args.sweep_runs > 0:
+        import wavebit_amd
+        from wavebit_amd import rcwa
+
+        def prob4(thick):
+            layers = [rcwa.Layer(thickness=float(t), epsilon=e) for t, e in zip(thick, wl['rasters'])]
+            return rcwa.Problem(
+                stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(wl['eps_top']),
+                                 bottom=wavebit_amd.Material(wl['eps_bottom']), period=wl['period']),
+                excitation=rcwa.Excitation(wavelength=[p['wavelength'] for p in wl['points']],
+                                           sources=(rcwa.PlaneWave(polarization='p', theta=0.0, phi=0.0),)),
+                units=wavebit_amd.Units(length='nm'))
+        num4 = rcwa.Numerics(orders=(hu, hv))
+        prep4 = rcwa.prepare(prob4(wl['thickness']), numerics=num4, backend='torch', device='cuda:0')
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prep4.run(prob4(wl['thickness']))
+        first_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for i in range(args.sweep_runs):
+            res4 = prep4.run(prob4(np.asarray(wl['thickness']) * (1.0 + 0.05 * (i + 1))))
+        sweep_s = (time.perf_counter() - t0) / args.sweep_runs
+        fresh4 = rcwa.run(prob4(np.asarray(wl['thickness']) * (1.0 + 0.05 * args.sweep_runs)),
+                          numerics=num4, backend='torch', device='cuda:0')
+        sweep = {'workload': 'the benchmark problem through rcwa.prepare / Prepared.run (64 '
+                             'wavelengths, host results), then runs with every thickness scaled',
+                 'first_run_s': round(first_s, 4), 'runs': args.sweep_runs,
+                 'thickness_run_s': round(sweep_s, 4),
+                 'thickness_sweep_solves_per_s': round(P / sweep_s, 3),
+                 'equal_to_fresh_run': bool(np.array_equal(res4.R, fresh4.R)
+                                            and np.array_equal(res4.T, fresh4.T))}
+        del prep4, res4, fresh4
+
+    # cfg5: a prefix of the seeded serving stream at its stated distribution
+    cfg5 = None
+    if args.cfg5_requests > 0:
+        from wavebit_amd.rcwa import serve
+        specs5 = workloads.serving_requests(args.cfg5_requests, seed=0)
+        arrivals5 = serve.poisson_arrivals(len(specs5), rate=args.cfg5_rate, seed=1)
+        reqs5 = []
+        for i, (s5, t5) in enumerate(zip(specs5, arrivals5)):
+            prob5, num5 = workloads.to_problem(s5)
+            reqs5.append(serve.Request(problem=prob5, numerics=num5, arrival=float(t5), id=i,
+                                       gradient=bool(s5['gradient'])))
+        srv5 = serve.Server(device=dev, streams=3, max_points_per_launch=64)
+        res5, rep5 = srv5.serve(reqs5, realtime=True, progress=lambda i, lat: print(
+            f'cfg5 request {i} done, latency {lat:.2f} s', file=sys.stderr, flush=True))
+        srv5.close()
+        cfg5 = {'stream': 'workloads.serving_requests(seed=0): orders per axis U{5..15}, 1-10 layers '
+                          'of 64^2/128^2/256^2 rasters eps U[1,16]+iU[0,1], 1-64 wavelengths in '
+                          '[400,1000] nm, 30% also asking for the raster gradient; Poisson arrivals',
+                'requests': rep5['requests'], 'arrival_rate_per_s': args.cfg5_rate,
+                'prefix_of': 10000,
+                'orders': [s5['orders'][0] for s5 in specs5],
+                'layers': [len(s5['thickness']) for s5 in specs5],
+                'wavelengths': [len(s5['wavelengths']) for s5 in specs5],
+                'gradient': [bool(s5['gradient']) for s5 in specs5],
+                'points': rep5['points'], 'wall_s': round(rep5['wall_s'], 3),
+                'solves_per_s': round(rep5['solves_per_s'], 4),
+                'p50_latency_s': round(rep5['p50_latency_s'], 4),
+                'p99_latency_s': round(rep5['p99_latency_s'], 4),
+                'launches': rep5['launches'], 'gradient_requests': rep5['gradient_requests'],
+                'failed_requests': sum(1 for v in res5.values() if isinstance(v, Exception)),
+                'errors': sorted({type(v).__name__ for v in res5.values() if isinstance(v, Exception)})}
+        del srv5, res5
+
     # the launch-bound end: the 1D binary grating at orders (10,0) through Prepared.run, eager
     # launch sequence vs the captured HIP graph it replays from its second run on
     small = None
@@ -597,6 +673,8 @@ def main():
         'roofline': roofline, 'cpu_baseline': cpu, 'stages': stages, 'measured_peaks': peaks,
         'gradient': grad,
         'serve': serve_report,
+        'thickness_sweep': sweep,
+        'serve_cfg5': cfg5,
         'launch_bound': small,
         'crossed_grating': crossed,
         'uneven_batch': uneven,

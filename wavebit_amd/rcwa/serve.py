@@ -301,9 +301,10 @@ class Server:
         return pack(per, r.hu, r.hv, r.wavelengths, work.checks, diag, prov, side=r.side)
 
     # ------------------------------------------------------------------ the loop
-    def serve(self, requests, realtime=True):
+    def serve(self, requests, realtime=True, progress=None):
         """Run a request stream to completion. ``arrival`` times are seconds after the start
-        (``realtime=False`` admits everything at once). Returns (results by id, report)."""
+        (``realtime=False`` admits everything at once); ``progress(id, latency_s)``, when given,
+        is called as each request completes. Returns (results by id, report)."""
         reqs = sorted(requests, key=lambda q: q.arrival)
         for i, q in enumerate(reqs):
             if q.id < 0:
@@ -352,6 +353,8 @@ class Server:
             for work in finished:
                 results[work.req.id] = self._result(work)
                 latencies.append(work.t_done - work.req.arrival)
+                if progress is not None:
+                    progress(work.req.id, latencies[-1])
             # gradient requests: retire a finished one, start the next when a stream and its
             # memory are free (one at a time, on the worker thread)
             started = False
@@ -364,6 +367,8 @@ class Server:
                 work.t_done = getattr(work, 't_grad_done', time.perf_counter() - t0)
                 results[work.req.id] = res
                 latencies.append(work.t_done - work.req.arrival)
+                if progress is not None:
+                    progress(work.req.id, latencies[-1])
                 if not isinstance(res, Exception):
                     solved_points += work.n_points
                 free_streams.append(stream)
