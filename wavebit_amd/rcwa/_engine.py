@@ -241,19 +241,20 @@ def _pair_inputs(buckets, convs, device):
     return pts, idx, epx, epy, epz, tables
 
 
-def window_modes(buckets, convs, device):
+def window_modes(buckets, convs, device, rebuild=True, inputs=None):
     """The layer modes of every (point, layer) pair of several buckets that share an order window
     and path, whatever their layer counts, in one ``wb_layer_modes`` launch sequence (one
-    eigensolve batch); pairs with a grazing layer are solved again with the cluster rebuilt.
-    Returns (W, V, q, flags (pairs,), tables): ``tables[i][p, l]`` is the pair index of layer l of
-    point p of bucket i."""
+    eigensolve batch); with ``rebuild``, pairs with a grazing layer are solved again with the
+    cluster rebuilt (that check reads the flags, so it waits for the eigensolves; the server
+    leaves it to the completion of its launch instead). Returns (W, V, q, flags (pairs,), tables):
+    ``tables[i][p, l]`` is the pair index of layer l of point p of bucket i."""
     b0 = buckets[0]
     mode = _MODE[b0.path]
-    pts, idx, epx, epy, epz, tables = _pair_inputs(buckets, convs, device)
+    pts, idx, epx, epy, epz, tables = inputs or _pair_inputs(buckets, convs, device)
     W, V, q, flags = _native.layer_modes(pts, idx, epx, epy, epz, b0.hu, b0.hv, mode)
     flags = flags.reshape(-1)
     N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
-    if mode == _native.MODE_2D and 2 * N <= _native.GRAZE_NMAX:
+    if rebuild and mode == _native.MODE_2D and 2 * N <= _native.GRAZE_NMAX:
         bad = torch.nonzero(flags & _native.FLAG_GRAZING).reshape(-1)
         if bad.numel():
             W2, V2, q2, f2 = _native.layer_modes(pts[bad].contiguous(), idx[bad].contiguous(), epx,
@@ -262,27 +263,35 @@ def window_modes(buckets, convs, device):
     return W, V, q, flags, tables
 
 
-def run_window(buckets, convs, device, modes=None):
+def run_window(buckets, convs, device, modes=None, rebuild=True):
     """Solve buckets of one order window and path with different layer counts: their modes in one
     batch (``window_modes``, or ``modes`` from an earlier call on the same points), then each
     bucket's S-matrices and cascade from them (``wb_solve_from_modes``). Results equal
-    ``run_bucket``'s per bucket to the bit. Returns [(eff, amp, pe, flags, N)] per bucket, device
-    tensors, flags combining the modes' and the cascade's, and the modes used."""
+    ``run_bucket``'s per bucket to the bit. Every host input goes to the device before the first
+    launch, so with ``rebuild=False`` nothing here waits for the device. Returns
+    [(eff, amp, pe, flags, N)] per bucket, device tensors, flags combining the modes' and the
+    cascade's, and the modes used."""
     b0 = buckets[0]
     mode = _MODE[b0.path]
     N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
-    if modes is None:
-        modes = window_modes(buckets, convs, device)
-    W, V, q, mflags, tables = modes
-    out = []
-    for bucket, table in zip(buckets, tables):
+    staged = []
+    for bucket in buckets:
         P, L = len(bucket.points), bucket.n_layers
         rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
         thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
-        tab = torch.from_numpy(table).to(device)
-        eff, amp, pe, flags = _native.solve_from_modes(
-            torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device), W, V, q, tab,
-            bucket.hu, bucket.hv, mode)
+        staged.append((torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device)))
+    if modes is None:
+        inputs = _pair_inputs(buckets, convs, device)
+        tabs = [torch.from_numpy(t).to(device) for t in inputs[5]]
+        modes = window_modes(buckets, convs, device, rebuild=rebuild, inputs=inputs)
+    else:
+        tabs = [torch.from_numpy(t).to(device) for t in modes[4]]
+    W, V, q, mflags, _ = modes
+    out = []
+    for bucket, (pts, thick), tab in zip(buckets, staged, tabs):
+        P, L = len(bucket.points), bucket.n_layers
+        eff, amp, pe, flags = _native.solve_from_modes(pts, thick, W, V, q, tab, bucket.hu,
+                                                       bucket.hv, mode)
         flags = flags | mflags[tab.reshape(-1).long()].reshape(P, L)
         out.append((eff, amp, pe, flags, N))
     return out, modes

@@ -110,10 +110,14 @@ class ConvLRU:
 
 
 class Server:
-    """Continuous batching with HBM-footprint admission and overlapping streams."""
+    """Continuous batching with HBM-footprint admission and overlapping streams. Forward points
+    queue per order window and path; a layer count with ``window_below`` or more queued points
+    launches alone, smaller counts share window launches (one eigensolve batch, one cascade
+    per layer count)."""
 
     def __init__(self, device='cuda:0', streams=3, memory_fraction=0.6, max_points_per_launch=64,
-                 conv_cache_fraction=0.1):
+                 conv_cache_fraction=0.1, window_below=16):
+        self.window_below = max(1, window_below)
         self.device = torch.device(device)
         _native.context(self.device)
         free, total = torch.cuda.mem_get_info(self.device)
@@ -162,9 +166,8 @@ class Server:
                 stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
                 polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                 period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct), conv))
-        # forward points queue per order window and path, whatever their layer count: a launch
-        # batches the eigensolves of all its (point, layer) pairs, then runs one cascade per
-        # layer count (engine.run_window)
+        # forward points queue per order window and path, whatever their layer count; _take
+        # decides per launch whether one layer count goes alone or several share a window
         key = (r.hu, r.hv, r.path)
         work.layers = r.rasters.shape[0]
         return work, key, pts
@@ -172,11 +175,15 @@ class Server:
     def _launch_bytes(self, key, items):
         """Device bytes of a window launch over ``items``: the modes of every pair and the
         workspace that computes them, and the largest cascade workspace plus the outputs."""
-        hu, hv, path = key
+        hu, hv, path = key[:3]
         mode = _engine._MODE[path]
         N = (2 * hu + 1) * (2 * hv + 1)
         n = 2 * N if mode == _native.MODE_2D else N
         groups = collections.Counter(int(conv[0].shape[0]) for _, _, _, conv in items)
+        if len(groups) == 1:                 # one layer count: the fused solve
+            (L, P), = groups.items()
+            out = P * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8 + L * 4)
+            return _native.solve_workspace_bytes(P, L, hu, hv, mode) + out
         pairs = sum(L * c for L, c in groups.items())
         modes = pairs * (2 * n * n + n) * 16 + _native.layer_modes_bytes(pairs, 1, hu, hv, mode)
         cascade = max(_native.solve_workspace_bytes(c, L, hu, hv, mode) for L, c in groups.items())
@@ -244,10 +251,29 @@ class Server:
                             'd_epsilon': grads[0], 'd_thickness': grads[1]}
         return replace(result, diagnostics=diag)
 
+    def _take(self, q):
+        """The items of the next launch from one order window's queue (not yet removed). A
+        layer count with at least ``window_below`` queued points goes alone, as one fused solve
+        of up to ``max_points_per_launch`` points (the count queued earliest first); otherwise
+        the oldest points share a window launch whatever their layer counts. On the uneven
+        2048-problem batch (orders 3..12, 1-8 layers) single-count launches of ~25 points run
+        39.7 solves/s against 25.5 for windows of 32-64 mixed points, while on its first 384
+        problems (~5 points per count) windows run 27.2 against 20.5: windows pay off only for
+        counts too small to fill a launch."""
+        counts = collections.Counter(int(item[3][0].shape[0]) for item in q)
+        full = [L for L, c in counts.items() if c >= min(self.window_below, self.max_points)]
+        if full:
+            first = {}
+            for n, item in enumerate(q):
+                first.setdefault(int(item[3][0].shape[0]), n)
+            L = min(full, key=first.__getitem__)
+            return [item for item in q if int(item[3][0].shape[0]) == L][:self.max_points]
+        return list(itertools.islice(q, min(len(q), self.max_points)))
+
     def _launch(self, key, items, stream):
         """One window launch: the items grouped by layer count into buckets (each with its own
         conv stacks), their modes in one batch, one cascade per bucket."""
-        hu, hv, path = key
+        hu, hv, path = key[:3]
         groups = collections.OrderedDict()
         for item in items:
             groups.setdefault(int(item[3][0].shape[0]), []).append(item)
@@ -264,14 +290,18 @@ class Server:
             convs.append(cs)
             members.append(group)
         with torch.cuda.stream(stream):
-            outs, _ = _engine.run_window(buckets, convs, self.device)
+            if len(buckets) == 1:
+                outs = [_engine.run_bucket(buckets[0], convs[0], self.device)]
+            else:
+                # no wait on the device here: grazing points are settled when the launch completes
+                outs, _ = _engine.run_window(buckets, convs, self.device, rebuild=False)
             done = torch.cuda.Event()
             done.record(stream)
         return dict(key=key, items=items, outs=outs, members=members, buckets=buckets, convs=convs,
                     event=done, bytes=self._launch_bytes(key, items), stream=stream)
 
     def _finish(self, launch, now, finished):
-        hu, hv, path = launch['key']
+        hu, hv, path = launch['key'][:3]
         for out, bucket, convs, group in zip(launch['outs'], launch['buckets'], launch['convs'],
                                              launch['members']):
             eff, amp, pe, flags = (t.cpu().numpy() for t in out[:4])
@@ -377,7 +407,7 @@ class Server:
                 self.stats['gradient_requests'] += 1
             if grads and grad_job is None and free_streams:
                 work, key = grads[0]
-                gkey = key + (work.layers,)
+                gkey = key[:3] + (work.layers,)
                 need = self._gradient_bytes(gkey, self._gradient_chunk(gkey, work.n_points))
                 if self.conv.bytes + used + need <= self.budget or not inflight:
                     grads.popleft()
@@ -395,14 +425,14 @@ class Server:
                 if not free_streams:
                     break
                 q = queues[key]
-                P = min(len(q), self.max_points)
-                while P > 1 and (self.conv.bytes + used
-                                 + self._launch_bytes(key, list(itertools.islice(q, P))) > self.budget):
-                    P //= 2
-                if (self.conv.bytes + used + self._launch_bytes(key, list(itertools.islice(q, P)))
-                        > self.budget and inflight):
+                items = self._take(q)
+                while len(items) > 1 and self.conv.bytes + used + self._launch_bytes(key, items) > self.budget:
+                    items = items[:len(items) // 2]
+                if self.conv.bytes + used + self._launch_bytes(key, items) > self.budget and inflight:
                     continue
-                items = [q.popleft() for _ in range(P)]
+                taken = set(map(id, items))
+                queues[key] = q = collections.deque(it for it in q if id(it) not in taken)
+                P = len(items)
                 if not q:
                     order.pop(key, None)
                 stream = free_streams.pop()
@@ -418,6 +448,11 @@ class Server:
                 elif pending and realtime:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
         wall = time.perf_counter() - t0
+        # hand the stream's cached workspaces back: launches of other sizes later (another
+        # server, a prepared run) would otherwise find the device full of blocks they cannot
+        # reuse, and kernels whose scratch the HIP runtime allocates on demand would fail
+        torch.cuda.synchronize(self.device)
+        torch.cuda.empty_cache()
         lat = np.array(latencies) if latencies else np.zeros(1)
         report = {'requests': len(reqs), 'points': solved_points, 'wall_s': wall,
                   'solves_per_s': solved_points / wall if wall > 0 else 0.0,
