@@ -16,6 +16,13 @@ Sources (paths relative to the reference root):
   rule) totals and per-order efficiencies at four samplings per pixel, double-precision-phase
   flavour, for the two rectangular cells and the hexagonal one; the scene table is
   ``tests/references/_fmmax_laurent_cases.py:39-71``.
+* ``tests/fixtures/legacy_rcwa/golden_field.npz`` -- the reference's own in-layer field cells
+  (its drift detector), downward frame, corner-unit incidence; inputs at
+  ``tests/fixtures/legacy_rcwa/gen_goldens_field.py:84-108`` (layers listed top-to-bottom),
+  sampling ``sample_x=5, sample_y=4, sample_z=6`` (``:117``).
+* ``tests/references/golden_fmmax_field.npz`` -- fmmax 1.7.1 in-layer E/H at cell-centred
+  points: the x-only grating and uniform slab (25 orders) and the crossed patch and its
+  uniform control at 49 orders; scene table ``tests/references/_fmmax_field_cases.py``.
 """
 import json
 import sys
@@ -33,6 +40,8 @@ LEGACY_CASES = ['te_1layer', 'tm_1layer', 'te_oblique', 'tm_oblique', 'te_2layer
                 'te_same_mat', '2d_normal', '2d_oblique', '2d_2layer', '2d_cfs']
 
 FMMAX_LAURENT_CASES = ['rod_eps12_rect', 'hole_metal_rect', 'cross_eps6_hex']
+
+FIELD_CASES = ['te_1d', 'tm_1d', '1d_2layer', 'conical', '2d', '2d_2layer', '2d_first_order']
 
 # tests/references/test_reticolo_li_nesting.py:34-39 -- RETICOLO V10 total T, (TE, TM).
 LI_NESTING_FIXED = {'3': [0.9822061471, 0.8913451356], '5': [0.9816211625, 0.8830613359]}
@@ -62,11 +71,21 @@ def main():
                                       and (k.endswith('__orders') or '__phase64__' in k))}
     np.savez_compressed(HERE / 'fmmax_laurent.npz', **laurent)
 
+    gf = np.load(REF / 'tests/fixtures/legacy_rcwa/golden_field.npz', allow_pickle=False)
+    fields = {k: gf[k] for k in gf.files
+              if k.split('__')[0] in FIELD_CASES and k.split('__')[1] in ('default', 'all', 'grating_type')}
+    np.savez_compressed(HERE / 'legacy_field.npz', **fields)
+
+    ff = np.load(REF / 'tests/references/golden_fmmax_field.npz', allow_pickle=False)
+    fmmax_field = {k: ff[k] for k in ff.files if '__9' not in k}
+    np.savez_compressed(HERE / 'fmmax_field.npz', **fmmax_field)
+
     (HERE / 'li_nesting_fixed.json').write_text(json.dumps(
         {'source': 'tests/references/test_reticolo_li_nesting.py:34-39 (RETICOLO V10, nesting fixed)',
          'tolerance': 1e-9, 'total_T_te_tm': LI_NESTING_FIXED}, indent=1))
     print('wrote', len(out), 'reticolo arrays,', len(keep), 'legacy arrays,', len(laurent),
-          'fmmax Laurent arrays')
+          'fmmax Laurent arrays,', len(fields), 'legacy field arrays,', len(fmmax_field),
+          'fmmax field arrays')
 
 
 if __name__ == '__main__':
