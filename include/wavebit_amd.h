@@ -218,8 +218,37 @@ int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pt
                      const double* epz_i, double* eff, double* amp, double* pe, int* flags,
                      double* smat, void* workspace, size_t workspace_bytes, void* stream);
 
-/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
- * is stored k x m, B n x k). */
+/* Field state: wb_solve_from_modes plus each layer's mode amplitudes for the point's own
+ * incidence, the state rcwa/_engine/scattering.py:236-333 (_field_state) leaves for field
+ * reconstruction (rcwa/_engine/fields.py). amps: P L x 2 x n complex, pair b = p L + l holding
+ * the down-going amplitudes anchored at the layer's top face, then the up-going ones anchored at
+ * its bottom face, in the basis of the modes (W, V, q) it was given: the field in the layer at
+ * height z above its bottom face is E_t = W (a_up + a_down), with a_up = exp(-k0 q z) up and
+ * a_down = exp(k0 q (z - d)) down. The incidence is unit E at the cell centre (a_s s + a_p p on
+ * the 2D path; the s or p wave of the incident order on the 1D ones). The other outputs are
+ * wb_solve_from_modes'. The workspace is wb_field_state_workspace_size's. */
+size_t wb_field_state_workspace_size(int P, int L, int hu, int hv, int mode);
+int wb_field_state(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                   const double* thick, const double* W, const double* V, const double* q,
+                   const int* modes_idx, double* amps, double* eff, double* amp, double* pe,
+                   int* flags, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The six field components (Ex, Ey, Ez, Hx, Hy, Hz, H in units of E) in one layer of one point,
+ * public frame (+z up, x and y from the cell centre), the expansion of
+ * rcwa/_engine/field_result.py:109-114 over the components of rcwa/_engine/fields.py:503-814:
+ * out (nz x nxy x 6 complex, device) = scale[c] x component c at local heights z[j] (device,
+ * measured up from the layer's bottom face, 0 <= z <= thickness) and points xy (device, nxy
+ * x, y pairs). pt: the point's parameter record (device, as wb_solve's pts); W, V (n x n), q (n)
+ * and amps (2n) of the layer's pair from wb_layer_modes / wb_field_state; epz_i (N x N) the
+ * layer's inverse-rule matrix (not read on the TE path); scale: host array of 6 complex factors
+ * (1/Z0 on H gives A/m for E in V/m). The workspace is wb_field_sample_workspace_size's. */
+size_t wb_field_sample_workspace_size(int hu, int hv, int mode, int nz, int nxy);
+int wb_field_sample(wb_ctx* ctx, int hu, int hv, int mode, const double* pt, double thickness,
+                    const double* W, const double* V, const double* q, const double* epz_i,
+                    const double* amps, int nz, const double* z, int nxy, const double* xy,
+                    const double* scale, double* out, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
 /* wb_solve plus the stack's scattering matrix per point: smat is P x 4 x n x n complex,
  * n = 2N on the 2D path (s amplitudes over p port variables) and N on the 1D paths, blocks in
  * the order (S11, S12, S21, S22), incidence-relative for top incidence: S11 reflects back into
@@ -233,6 +262,8 @@ int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
                         int* flags, double* smat, void* workspace, size_t workspace_bytes,
                         void* stream);
 
+/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; op 0 = as stored, 1 = conjugate transpose (then A
+ * is stored k x m, B n x k). */
 int wb_zgemm_batched_ops(wb_ctx* ctx, int opa, int opb, int m, int n, int k, const double* alpha,
                          const double* A, int lda, long long strideA, const double* B, int ldb,
                          long long strideB, const double* beta, double* C, int ldc,

@@ -239,7 +239,7 @@ def test_detuning_ladder_moves_wavelength_then_angle_once_each(monkeypatch):
         calls.append((tuple(sub.wavelengths), sub.source.theta))
         moved_theta = sub.source.theta != theta
         flags = np.array([[0 if moved_theta else _native.FLAG_GRAZING] for _ in sub.wavelengths])
-        return [('solved', float(w), sub.source.theta) for w in sub.wavelengths], flags, None
+        return [('solved', float(w), sub.source.theta) for w in sub.wavelengths], flags, None, None
 
     class Cache:
         def sharing_conv(self):

@@ -1,0 +1,190 @@
+"""In-layer fields on the GPU (``Outputs(fields=True)``, ``Result.field``): the HIP path
+(wb_layer_modes -> wb_field_state -> wb_field_sample) against the reference's frozen field
+cells, the oracle's reconstruction on the same inputs, and at full size through properties the
+fields must have whatever their size: flux through every height of a lossless stack equal to the
+transmission, tangential continuity at internal faces, explicit heights equal to counted ones.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import wavebit_amd
+from oracle import rcwa_fields as orf
+from wavebit_amd import rcwa
+from wavebit_amd.errors import UnsupportedFeatureError, ValidationError
+from wavebit_amd.units import Z0
+
+from test_oracle_fields import LEGACY, RES, legacy_expected
+
+pytestmark = pytest.mark.gpu
+RUN = dict(backend='torch', device='cuda:0')
+H_UNITS = np.array([1, 1, 1, 1 / Z0, 1 / Z0, 1 / Z0])
+
+
+def _problem(eps, d, top, bot, period, wls, theta, phi, pol, side='top', units='um', amplitude=1.0,
+             fields=True):
+    layers = [rcwa.Layer(thickness=t, epsilon=np.ascontiguousarray(e)) for t, e in zip(d, eps)]
+    return rcwa.Problem(
+        stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(top),
+                         bottom=wavebit_amd.Material(bot), period=period),
+        excitation=rcwa.Excitation(wavelength=list(wls), sources=(rcwa.PlaneWave(
+            polarization=pol, theta=theta, phi=phi, side=side, amplitude=amplitude),)),
+        outputs=rcwa.Outputs(fields=fields), units=wavebit_amd.Units(length=units))
+
+
+def _legacy_run(name):
+    c = LEGACY[name]
+    eps = np.asarray(c['eps'], dtype=np.complex128)[::-1]
+    pol = c['pol'] if 'pol' in c else (np.sin(c['psi']), np.cos(c['psi']))
+    prob = _problem(eps, list(c['d'])[::-1], c['top'], c['bot'], c['period'], [c['wl']],
+                    c['theta'], c['phi'], pol)
+    return rcwa.run(prob, numerics=rcwa.Numerics(orders=c['M']), **RUN)
+
+
+@pytest.mark.parametrize('name', list(LEGACY))
+def test_device_fields_match_the_references_frozen_cells(name):
+    """Every path the reference froze (1D TE/TM, two layers, conical, 2D, two-layer 2D, the
+    first-order scene): Result.field at the archive's sampling against the archive restated in
+    the public frame, 1e-11 (the values reach ~9)."""
+    g = np.load(Path(__file__).parent / 'golden' / 'legacy_field.npz')
+    res = _legacy_run(name)
+    nx, ny, nz = RES
+    coupled = int(g[f'{name}__grating_type']) != 0
+    f = res.field(x=nx, y=ny if coupled else 1, z=nz)
+    got = f.values[0] / H_UNITS
+    if coupled:
+        want = legacy_expected(name, g[f'{name}__all'])[0]
+    else:
+        cols = [1, 3, 5] if LEGACY[name]['pol'] == 's' else [4, 0, 2]
+        want = legacy_expected(name, g[f'{name}__default'])
+        got = got[..., cols]
+        absent = [c for c in range(6) if c not in cols]
+        assert np.abs(f.values[0][..., absent]).max() == 0.0
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-11, np.abs(got - want).max()
+    assert f.values.shape[-1] == 6 and f.components == ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
+    assert f.units['Ex'] == 'V/m' and f.units['Hx'] == 'A/m' and f.units['coordinates'] == 'um'
+    assert f.coordinates['z'].shape == (nz * len(LEGACY[name]['d']),)
+
+
+def _lossless_stack(L, n, seed):
+    rng = np.random.default_rng(seed)
+    return np.where(rng.random((L, n, n)) < 0.5, 6.25, 1.69).astype(np.complex128)
+
+
+def _flux(values):
+    """Cell-averaged z Poynting flux per height of a (z, y, x, 6) cell (H in A/m): the mean of
+    1/2 Re(E x H*)_z over a uniform grid that excludes the repeated edge."""
+    v = values[:, :-1, :-1]
+    s = 0.5 * np.real(v[..., 0] * np.conj(v[..., 4]) - v[..., 1] * np.conj(v[..., 3]))
+    return s.mean(axis=(1, 2))
+
+
+@pytest.mark.parametrize('orders,pol,side', [((5, 5), (0.6, 0.8j), 'top'), ((4, 3), 'p', 'bottom'),
+                                             ((9, 0), 's', 'top')])
+def test_flux_through_a_lossless_stack_is_the_transmission(orders, pol, side):
+    """Every height of a lossless three-layer stack carries T times the incident flux (down for
+    top incidence, up for bottom), and the tangential fields and H_z are continuous at the
+    internal faces; the device matches the oracle (top incidence) at every sample."""
+    hu, hv = orders
+    ny = 1 if hv == 0 else 16
+    eps = _lossless_stack(3, 16, 5)[:, :ny]
+    d = [0.21, 0.34, 0.15]
+    top, bot, theta, phi = 2.25, 1.44, 0.35, (None if hv == 0 and isinstance(pol, str) else 0.6)
+    wl = 0.7
+    res = rcwa.run(_problem(eps, d, top, bot, (0.9, 0.8), [wl], theta, phi, pol, side=side),
+                   numerics=rcwa.Numerics(orders=orders), **RUN)
+    nxy = 2 * (2 * hu + 1) + 1
+    f = res.field(x=nxy, y=nxy if hv else 1, z=5)
+    v = f.values[0]
+    n_lit = np.sqrt(top if side == 'top' else bot)
+    incident = 0.5 * n_lit * np.cos(theta) / Z0
+    sign = -1.0 if side == 'top' else 1.0
+    T = res.T[0].sum()
+    flux = _flux(v) if hv else 0.5 * np.real(v[:, 0, :-1, 0] * np.conj(v[:, 0, :-1, 4])
+                                             - v[:, 0, :-1, 1] * np.conj(v[:, 0, :-1, 3])).mean(axis=1)
+    assert np.abs(flux - sign * T * incident).max() < 1e-9 * incident, (flux / incident, T)
+    scale = np.abs(v).max()
+    for face in (1, 2):
+        below, above = v[5 * face - 1], v[5 * face]
+        for c in (0, 1, 3, 4, 5):
+            assert np.abs(below[..., c] - above[..., c]).max() < 1e-10 * scale, (face, c)
+    if side == 'top':
+        st = orf.solve_state(eps, d, top, bot, (0.9, 0.8), wl, theta, phi, pol, orders)
+        want = orf.public(orf.sample(st, nxy, nxy if hv else 1, 5))
+        assert np.abs(v - want).max() < 1e-10 * np.abs(want).max(), np.abs(v - want).max()
+
+
+def test_bottom_incidence_is_the_mirrored_stack():
+    """Lit from below, the field is the top-lit field of the stack turned upside down, read at
+    the mirrored heights with the mirror parity (E_z, H_x, H_y flip)."""
+    eps = _lossless_stack(2, 8, 9) + 0.05j
+    d = [0.25, 0.4]
+    args = dict(period=(0.8, 0.8), wls=[0.75], theta=0.3, phi=0.2, pol=(0.3, 0.7 + 0.2j))
+    up = rcwa.run(_problem(eps, d, 1.21, 2.25, side='bottom', **args),
+                  numerics=rcwa.Numerics(orders=(3, 3)), **RUN).field(x=7, y=6, z=4).values[0]
+    flipped = rcwa.run(_problem(eps[::-1], d[::-1], 2.25, 1.21, side='top', **args),
+                       numerics=rcwa.Numerics(orders=(3, 3)), **RUN).field(x=7, y=6, z=4).values[0]
+    want = flipped[::-1] * np.array([1, 1, -1, -1, -1, 1])
+    assert np.abs(up - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_full_size_crossed_grating_conserves_flux_and_matches_the_oracle():
+    """(9, 9), four 128 x 128 layers (the differentiation workload's stack, made lossless), two
+    wavelengths: flux equals T at every sampled height; one layer's cell against the oracle."""
+    rng = np.random.default_rng(21)
+    eps = np.where(rng.random((4, 128, 128)) < 0.4, 4.0, 1.0).astype(np.complex128)
+    d = [120.0, 90.0, 150.0, 60.0]
+    wls = [520.0, 610.0]
+    res = rcwa.run(_problem(eps, d, 1.0, 2.25, (1000.0, 1000.0), wls, 0.2, 0.3, 'p', units='nm'),
+                   numerics=rcwa.Numerics(orders=(9, 9)), **RUN)
+    f = res.field(x=33, y=33, z=3)
+    for p, wl in enumerate(wls):
+        flux = _flux(f.values[p])
+        incident = 0.5 * np.cos(0.2) / Z0
+        assert np.abs(flux + res.T[p].sum() * incident).max() < 1e-8 * incident, flux / incident
+    st = orf.solve_state(eps, d, 1.0, 2.25, (1000.0, 1000.0), wls[0], 0.2, 0.3, 'p', (9, 9))
+    want = orf.public(orf.sample(st, 9, 7, [30.0, 200.0, 330.0]))
+    got = res.field(x=9, y=7, z=[30.0, 200.0, 330.0]).values[0]
+    assert np.abs(got - want).max() < 1e-9 * np.abs(want).max(), np.abs(got - want).max()
+
+
+def test_explicit_heights_are_the_counted_ones_and_faces_take_the_upper_layer():
+    eps = _lossless_stack(2, 8, 3)
+    d = [0.2, 0.3]
+    res = rcwa.run(_problem(eps, d, 1.0, 2.25, (0.7, 0.7), [0.8, 0.9], 0.1, 0.2, 's', amplitude=2.0 - 1.0j),
+                   numerics=rcwa.Numerics(orders=(3, 3)), **RUN)
+    counted = res.field(x=5, y=4, z=3)
+    zc = counted.coordinates['z']
+    np.testing.assert_allclose(zc, [0.0, 0.1, 0.2, 0.2, 0.35, 0.5], atol=1e-15)
+    order = [4, 0, 1, 3, 5]          # out of order; 0.2 is the internal face: the upper layer
+    explicit = res.field(x=counted.coordinates['x'], y=counted.coordinates['y'], z=zc[order])
+    assert np.abs(explicit.values - counted.values[:, order]).max() < 1e-13 * np.abs(counted.values).max()
+    plain_res = rcwa.run(_problem(eps, d, 1.0, 2.25, (0.7, 0.7), [0.8, 0.9], 0.1, 0.2, 's'),
+                         numerics=rcwa.Numerics(orders=(3, 3)), **RUN)
+    plain = plain_res.field(x=5, y=4, z=3)
+    assert np.abs(counted.values - (2.0 - 1.0j) * plain.values).max() < 1e-13 * np.abs(counted.values).max()
+    np.testing.assert_array_equal(res.R, plain_res.R)
+
+
+def test_field_requests_are_refused_by_name():
+    eps = _lossless_stack(1, 4, 1)
+    res = rcwa.run(_problem(eps, [0.2], 1.0, 1.0, (0.5, 0.5), [0.6], 0.0, 0.0, 's'),
+                   numerics=rcwa.Numerics(orders=(1, 1)), **RUN)
+    with pytest.raises(UnsupportedFeatureError) as e:
+        res.field(x=3, y=3, z=3, reconstruction='lalanne')
+    assert e.value.code == 'field.reconstruction'
+    with pytest.raises(ValidationError) as e:
+        res.field(x=3, y=3, z=3, reconstruction='fancy')
+    assert e.value.code == 'field.reconstruction'
+    for bad in (dict(x=0, y=3, z=3), dict(x=3.0, y=3, z=3), dict(x=3, y=[[0.1]], z=3),
+                dict(x=3, y=3, z=[0.3]), dict(x=3, y=3, z=[np.nan])):
+        with pytest.raises(ValidationError) as e:
+            res.field(**bad)
+        assert e.value.code == 'field.coordinates', bad
+    plain = rcwa.run(_problem(eps, [0.2], 1.0, 1.0, (0.5, 0.5), [0.6], 0.0, 0.0, 's', fields=False),
+                     numerics=rcwa.Numerics(orders=(1, 1)), **RUN)
+    with pytest.raises(UnsupportedFeatureError) as e:
+        plain.field(x=3, y=3, z=3)
+    assert e.value.code == 'outputs.unavailable'

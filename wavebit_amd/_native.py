@@ -36,7 +36,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
            'wb_gemm_tuning_clear', 'wb_solve_noalloc', 'wb_solve_rebuild_workspace_size',
            'wb_solve_rebuild', 'wb_layer_modes_workspace_size', 'wb_layer_modes',
-           'wb_solve_from_modes')
+           'wb_solve_from_modes', 'wb_field_state_workspace_size', 'wb_field_state',
+           'wb_field_sample_workspace_size', 'wb_field_sample')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -91,6 +92,14 @@ def load_library():
             'wb_solve_from_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                             c_sz, c_vp]),
+            'wb_field_state_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_field_state': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                       c_vp]),
+            'wb_field_sample_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_field_sample': (c_int, [c_vp, c_int, c_int, c_int, c_vp, ctypes.c_double, c_vp, c_vp,
+                                        c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_dp, c_vp,
+                                        c_vp, c_sz, c_vp]),
             'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
@@ -428,6 +437,68 @@ def solve_from_modes(pts, thick, W, V, q, modes_idx, hu, hv, mode, scattering=Fa
                                           _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
                                           _ptr(workspace), nbytes, _stream(device)), 'solve_from_modes')
     return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
+
+
+def field_state(pts, thick, W, V, q, modes_idx, hu, hv, mode, workspace=None):
+    """``solve_from_modes`` plus each pair's mode amplitudes for the point's own incidence; see
+    ``wb_field_state``. Returns (amps, eff, amp, pe, flags), amps (P L, 2, n)."""
+    device = W.device
+    ctx = context(device)
+    P, L = thick.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    n = 2 * N if mode == MODE_2D else N
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    for name, t in (('W', W), ('V', V), ('q', q)):
+        _require(t, torch.complex128, device, name)
+    if modes_idx is not None:
+        _require(modes_idx, torch.int32, device, 'modes_idx')
+    amps = torch.empty((P * L, 2, n), dtype=torch.complex128, device=device)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_field_state_workspace_size(P, L, hu, hv, mode)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_field_state(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick), _ptr(W),
+                                     _ptr(V), _ptr(q), _ptr(modes_idx), _ptr(amps), _ptr(eff),
+                                     _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
+                                     _stream(device)), 'field_state')
+    return amps, eff, amp, pe, flags
+
+
+def field_sample_bytes(hu, hv, mode, nz, nxy):
+    return int(load_library().wb_field_sample_workspace_size(hu, hv, mode, nz, nxy))
+
+
+def field_sample(pt, thickness, W, V, q, epz_i, amps, z, xy, hu, hv, mode, scale, out=None,
+                 workspace=None):
+    """Fields of one layer of one point; see ``wb_field_sample``. ``pt`` (PT_STRIDE,) float64,
+    ``W``/``V`` (n, n), ``q`` (n,), ``epz_i`` (N, N) or None on the TE path, ``amps`` (2, n), ``z``
+    (nz,) local heights, ``xy`` (nxy, 2); ``scale`` six complex factors. Returns (nz, nxy, 6)."""
+    device = W.device
+    ctx = context(device)
+    _require(pt, torch.float64, device, 'pt')
+    _require(z, torch.float64, device, 'z')
+    _require(xy, torch.float64, device, 'xy')
+    for name, t in (('W', W), ('V', V), ('q', q), ('amps', amps)):
+        _require(t, torch.complex128, device, name)
+    if epz_i is not None:
+        _require(epz_i, torch.complex128, device, 'epz_i')
+    nz, nxy = int(z.shape[0]), int(xy.shape[0])
+    if out is None:
+        out = torch.empty((nz, nxy, 6), dtype=torch.complex128, device=device)
+    _require(out, torch.complex128, device, 'out')
+    sc = (ctypes.c_double * 12)(*[v for c in scale for v in (complex(c).real, complex(c).imag)])
+    nbytes = ctx.lib.wb_field_sample_workspace_size(hu, hv, mode, nz, nxy)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_field_sample(ctx.handle, hu, hv, mode, _ptr(pt), float(thickness), _ptr(W),
+                                      _ptr(V), _ptr(q), _ptr(epz_i), _ptr(amps), nz, _ptr(z), nxy,
+                                      _ptr(xy), sc, _ptr(out), _ptr(workspace), nbytes,
+                                      _stream(device)), 'field_sample')
+    return out
 
 
 SOLVE_NOALLOC_BYTES = 64 << 20   # WB_SOLVE_NOALLOC_BYTES

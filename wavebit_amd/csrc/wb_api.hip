@@ -312,6 +312,65 @@ int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_from_modes");
 }
 
+size_t wb_field_state_workspace_size(int P, int L, int hu, int hv, int mode) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
+  const int N = (2 * hu + 1) * (2 * hv + 1);
+  const int n = (mode == WB_MODE_2D) ? 2 * N : N;
+  return align_up(wb_solve_workspace_bytes(P, L, hu, hv, mode)) +
+         wb_field_extra_bytes(P, L, n, mode == WB_MODE_2D ? 2 : 1);
+}
+
+int wb_field_state(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                   const double* thick, const double* W, const double* V, const double* q,
+                   const int* modes_idx, double* amps, double* eff, double* amp, double* pe,
+                   int* flags, void* workspace, size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG, "wb_field_state: bad dimensions");
+  if (!pts || !thick || !W || !V || !q || !amps || !eff || !amp || !pe)
+    return fail(ctx, WB_BADARG, "wb_field_state: null buffer");
+  if (workspace_bytes < wb_field_state_workspace_size(P, L, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_field_state: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = nullptr;
+  a.epx = nullptr; a.epy = nullptr; a.epz = nullptr;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr;
+  a.modes_in = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q; a.modes_idx = modes_idx;
+  a.field_amps = (cd*)amps;
+  a.field_ws = (char*)workspace + align_up(wb_solve_workspace_bytes(P, L, hu, hv, mode));
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_field_state");
+}
+
+size_t wb_field_sample_workspace_size(int hu, int hv, int mode, int nz, int nxy) {
+  if (hu < 0 || hv < 0 || mode < 0 || mode > 2 || nz <= 0 || nxy <= 0) return 0;
+  return wb_field_sample_bytes(hu, hv, mode, nz, nxy);
+}
+
+int wb_field_sample(wb_ctx* ctx, int hu, int hv, int mode, const double* pt, double thickness,
+                    const double* W, const double* V, const double* q, const double* epz_i,
+                    const double* amps, int nz, const double* z, int nxy, const double* xy,
+                    const double* scale, double* out, void* workspace, size_t workspace_bytes,
+                    void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0) || nz <= 0 || nxy <= 0 ||
+      !(thickness >= 0.0))
+    return fail(ctx, WB_BADARG, "wb_field_sample: bad dimensions");
+  if (!pt || !W || !V || !q || !amps || !z || !xy || !scale || !out || (mode != WB_MODE_1D_TE && !epz_i))
+    return fail(ctx, WB_BADARG, "wb_field_sample: null buffer");
+  if (workspace_bytes < wb_field_sample_workspace_size(hu, hv, mode, nz, nxy))
+    return fail(ctx, WB_OOM, "wb_field_sample: workspace too small");
+  cd f[6];
+  for (int k = 0; k < 6; ++k) f[k] = mkc(scale[2 * k], scale[2 * k + 1]);
+  return hip_status(ctx,
+                    wb_field_sample_launch(hu, hv, mode, pt, thickness, (const cd*)W, (const cd*)V,
+                                           (const cd*)q, (const cd*)epz_i, (const cd*)amps, nz, z,
+                                           nxy, xy, f, (cd*)out, workspace, (hipStream_t)stream),
+                    "wb_field_sample");
+}
+
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                         const double* thick, const int* conv_idx, const double* epx,
                         const double* epy, const double* epz_i, double* eff, double* amp, double* pe,

@@ -83,6 +83,12 @@ struct wb_solve_args {
   cd* modes_V = nullptr;
   cd* modes_q = nullptr;
   const int* modes_idx = nullptr;
+  // Field state: each pair's mode amplitudes for the public incidence (pair b at b 2n: the
+  // down-going ones anchored at the layer's top face, then the up-going ones anchored at its
+  // bottom face), from the cascade's gap waves; field_ws the scratch of wb_field_extra_bytes.
+  // Not combined with smat.
+  cd* field_amps = nullptr;
+  void* field_ws = nullptr;
 };
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

@@ -20,6 +20,26 @@ enum {
 };
 // The cell: PT_PX / PT_PY are a1x / a2y of the lattice vectors a1 = (a1x, a1y), a2 = (a2x, a2y)
 // (the periods of a rectangular cell), PT_A1Y / PT_A2X its skew entries (zero when rectangular).
+
+// The normalized in-plane wavevector of order r (u fastest, r = (n + hv) nu + m + hu) of a point:
+// order (m, n) sits at k_inc + m b1 + n b2 (reference lattice.py:899-944), on a rectangular cell
+// kx0 + m wl/Lx and ky0 + n wl/Ly, on a skewed one the closed form over det(a1, a2); the 1D
+// paths have ky = 0.
+__device__ inline void order_k(const double* pt, int hu, int hv, int mode, int r, double* kx,
+                               double* ky) {
+  const int nu = 2 * hu + 1;
+  const int mi = r % nu - hu, ni = r / nu - hv;
+  const double wl = pt[PT_WL];
+  const double a1x = pt[PT_PX], a2y = pt[PT_PY], a1y = pt[PT_A1Y], a2x = pt[PT_A2X];
+  if (a1y == 0.0 && a2x == 0.0) {
+    *kx = pt[PT_KX0] + (double)mi * (wl / a1x);
+    *ky = (mode == 0) ? pt[PT_KY0] + (double)ni * (wl / a2y) : 0.0;
+  } else {
+    const double lam = wl / (a1x * a2y - a2x * a1y);
+    *kx = pt[PT_KX0] + lam * (a2y * (double)mi - a1y * (double)ni);
+    *ky = pt[PT_KY0] + lam * (-a2x * (double)mi + a1x * (double)ni);
+  }
+}
 // per-order vectors kept per point: kx, ky, kz_kt, kz_kb, gap y, Fc, Fs  (7 x N complex)
 enum { OV_KX = 0, OV_KY, OV_KZT, OV_KZB, OV_Y, OV_FC, OV_FS, OV_COUNT };
 // per-point diagonal blocks (n each): ref11, ref12, ref21, ref22, trn11, trn12, trn21, trn22
@@ -91,3 +111,26 @@ struct wb_adjoint_args {
   double* d_thick;            // P x L (overwritten)
 };
 hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st);
+
+// Field state (wb_fields.hip): scratch beyond the solve workspace that keeps every split plane's
+// down-going gap columns and the per-pair amplitude partials, and the stage run after the
+// cascade's back-sweep.
+size_t wb_field_extra_bytes(int P, int L, int n, int nrhs);
+struct wb_field_ws {
+  cd *d, *u, *t;            // P x (L+1) planes of n x nrhs (down, up), B x 3 blocks of n x nrhs
+};
+wb_field_ws wb_field_carve(void* ws, int P, int L, int n, int nrhs);
+// Gamma_0 at G0 (P x n x n), Gamma_l (l >= 1) the S11 half of M6's slot l - 1 (row stride 2n),
+// A / B per pair in M4 / M5, q per pair; writes a.field_amps.
+hipError_t wb_field_amps_stage(const wb_solve_args& a, int N, int n, int nrhs, const cd* G0,
+                               const cd* M6, const cd* A, const cd* B, const cd* q,
+                               const wb_field_ws& F, hipStream_t st);
+// Fields of one layer of one point at nz local heights z (device, within [0, d]) and nxy
+// centre-origin points xy (device, x y pairs): out (nz x nxy x 6) = scale[c] x component c.
+// W, V (n x n, unrotated), q (n) and amps (2n: down-going, then up-going) of the layer's pair,
+// epz (N x N) its inverse-rule matrix (unused on the TE path); scale a host array of 6.
+size_t wb_field_sample_bytes(int hu, int hv, int mode, int nz, int nxy);
+hipError_t wb_field_sample_launch(int hu, int hv, int mode, const double* pt, double d, const cd* W,
+                                  const cd* V, const cd* q, const cd* epz, const cd* amps, int nz,
+                                  const double* z, int nxy, const double* xy, const cd* scale,
+                                  cd* out, void* ws, hipStream_t st);

@@ -65,26 +65,14 @@ __device__ inline cd pt_c(const double* p, int off) { return mkc(p[off], p[off +
 
 // One thread per (point, order). mode: 0 = 2D coupled, 1 = 1D TE, 2 = 1D TM.
 __global__ void k_point_setup(int P, int hu, int hv, int mode, const double* pts, cd* ov, cd* dg) {
-  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv;
+  const int N = (2 * hu + 1) * (2 * hv + 1);
   const int n = (mode == 0) ? 2 * N : N;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)P * N) return;
   const int p = (int)(gid / N), r = (int)(gid % N);
   const double* pt = pts + (long long)p * PT_STRIDE;
-  const int mi = r % nu - hu, ni = r / nu - hv;
-  const double wl = pt[PT_WL];
-  // order (m, n) sits at k_inc + m b1 + n b2 (reference lattice.py:899-944): on a rectangular
-  // cell kx0 + m wl/Lx and ky0 + n wl/Ly; on a skewed one the closed form over det(a1, a2)
-  const double a1x = pt[PT_PX], a2y = pt[PT_PY], a1y = pt[PT_A1Y], a2x = pt[PT_A2X];
   double kx, ky;
-  if (a1y == 0.0 && a2x == 0.0) {
-    kx = pt[PT_KX0] + (double)mi * (wl / a1x);
-    ky = (mode == 0) ? pt[PT_KY0] + (double)ni * (wl / a2y) : 0.0;
-  } else {
-    const double lam = wl / (a1x * a2y - a2x * a1y);
-    kx = pt[PT_KX0] + lam * (a2y * (double)mi - a1y * (double)ni);
-    ky = pt[PT_KY0] + lam * (-a2x * (double)mi + a1x * (double)ni);
-  }
+  order_k(pt, hu, hv, mode, r, &kx, &ky);
   const cd kpar2 = mkc(kx * kx + ky * ky, 0.0);
   // kernel top = scene bottom (substrate), kernel bottom = scene top (superstrate)
   const cd e_kt = pt_c(pt, PT_EBOT), e_kb = pt_c(pt, PT_ETOP);
@@ -775,6 +763,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if (graze) G = wb_graze_carve(a.graze, B, n);
   wb_ck CK;
   if (keep) CK = wb_ck_carve(a.ck, P, L, a.hu, a.hv, mode);
+  const bool fields = a.field_amps != nullptr;
+  if (fields && (a.smat || !a.field_ws)) return hipErrorInvalidValue;
+  wb_field_ws FW{};
+  if (fields) FW = wb_field_carve(a.field_ws, P, L, n, nrhs);
   hipError_t e;
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0), mone = mkc(-1.0, 0.0);
   // per-stage timing: the layer eigensolves vs everything else of the solve (algorithmic flops:
@@ -1003,7 +995,9 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   }
   // final interface: u = (I - t11 G_L)^-1 t12 e ; R = t22 e + t21 G_L u
   const int last_ld = ldg;
-  BatchPtr Tf = bp_strided(G0, nn);   // Gamma_0 no longer needed
+  // Gamma_0 is no longer needed, unless the field state reads it (then M3, free since the
+  // layer S-matrices were formed, takes the factors)
+  BatchPtr Tf = bp_strided(fields ? M[3] : G0, nn);
   hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, Gam, last_ld, Tf, n);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tf, n, dg, (int)DG_T11);
@@ -1028,6 +1022,11 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                        S(CK.d + (long long)L * vs, dstride), nrhs);
     WB_CHECK_LAUNCH();
   }
+  if (fields) {
+    hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(vec[0], vs), nrhs,
+                       S(FW.d + (long long)L * vs, dstride), nrhs);
+    WB_CHECK_LAUNCH();
+  }
   // gu = G_L u
   if ((e = wb_zgemm_launch(n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero, S(vec[1], vs),
                            nrhs, P, st))) return e;
@@ -1048,8 +1047,14 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                          S(CK.d + (long long)l * vs, dstride), nrhs);
       WB_CHECK_LAUNCH();
     }
+    if (fields) {
+      hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(dnext, vs), nrhs,
+                         S(FW.d + (long long)l * vs, dstride), nrhs);
+      WB_CHECK_LAUNCH();
+    }
     cd* tmp = dcur; dcur = dnext; dnext = tmp;
   }
+  if (fields && (e = wb_field_amps_stage(a, N, n, nrhs, G0, M6, M[4], M[5], q, FW, st))) return e;
   hipLaunchKernelGGL(k_transmit, grid2(vs, P), dim3(256), 0, st, n, nrhs, dg, dcur, vec[3]);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_efficiency, dim3(P), dim3(256), 0, st, a.hu, a.hv, mode, a.pts, ov, vec[2],

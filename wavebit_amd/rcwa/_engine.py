@@ -149,6 +149,13 @@ def point_record(pt, path):
             a_s.real, a_s.imag, a_p.real, a_p.imag, a1y, a2x]
 
 
+def cell_of(period):
+    """``(a1x, a2y, a1y, a2x)`` of a period ``(Lx, Ly)`` or of a skewed cell's 4-tuple."""
+    if len(period) == 4:
+        return tuple(float(v) for v in period)
+    return (float(period[0]), float(period[1]) if len(period) > 1 else 1.0, 0.0, 0.0)
+
+
 def order_kxy(kx0, ky0, wavelength, period, hu, hv):
     """kx/k0, ky/k0 of the retained orders: vectors (2hu+1,), (2hv+1,) on a rectangular cell;
     on a skewed one (``period`` = (a1x, a2y, a1y, a2x)) two (2hv+1, 2hu+1) grids (reference
@@ -295,6 +302,28 @@ def run_window(buckets, convs, device, modes=None, rebuild=True):
         flags = flags | mflags[tab.reshape(-1).long()].reshape(P, L)
         out.append((eff, amp, pe, flags, N))
     return out, modes
+
+
+def run_fields(bucket, conv, device):
+    """Solve a bucket keeping what its fields are sampled from: every pair's layer modes (grazing
+    clusters rebuilt, ``window_modes``) and mode amplitudes for the point's own incidence
+    (``wb_field_state``). Returns device tensors (eff, amp, pe, flags) and the state: pts, W, V,
+    q, amps (P L, 2, n), the (P, L) pair table, the concatenated epz_i and the (P, L) index of
+    each layer's conv matrices in it."""
+    mode = _MODE[bucket.path]
+    P, L = len(bucket.points), bucket.n_layers
+    W, V, q, mflags, tables = window_modes([bucket], [conv], device)
+    tab = torch.from_numpy(tables[0]).to(device)
+    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+    pts = torch.from_numpy(rec).to(device)
+    amps, eff, amp, pe, flags = _native.field_state(pts, torch.from_numpy(thick).to(device), W, V, q,
+                                                    tab, bucket.hu, bucket.hv, mode)
+    flags = flags | mflags[tab.reshape(-1).long()].reshape(P, L)
+    epz = torch.cat([c[2] for c in conv]).contiguous()
+    conv_idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int64)
+    state = dict(pts=pts, W=W, V=V, q=q, amps=amps, table=tables[0], epz=epz, conv_idx=conv_idx)
+    return eff, amp, pe, flags, state
 
 
 def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):

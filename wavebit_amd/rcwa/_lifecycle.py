@@ -33,6 +33,7 @@ from .numerics import Numerics
 from .problem import Problem
 from .sources import PlaneWave
 from . import _engine
+from .fields import FieldRecord
 from .results import pack
 
 
@@ -160,6 +161,8 @@ class _Resolved:
     factorization: str = 'li'    # 'li' (the default) or 'laurent'
     coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
     scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
+    fields: bool = False             # Outputs(fields=True): keep the in-layer field state
+    bare: bool = False               # the stack declared no layers
     thetas: tuple = None             # per-point effective theta after a detuning ladder
     side: str = 'top'                # the lit half-space; 'bottom' is solved on the mirrored stack
     k_direct: tuple = None           # PlaneWave(kx=, ky=): radians per length unit
@@ -187,9 +190,6 @@ def _resolve(problem, numerics, execution):
             "nests no rules -- it takes the direct rule for every product, so there is no "
             "inverse for an order to matter to. It selects the order Li's rules are nested in, "
             "so it needs factorization='li'.", code='rcwa.unsupported', path='numerics.li_nesting')
-    if problem.outputs.fields:
-        raise UnsupportedFeatureError('Field reconstruction is not implemented by this engine.',
-                                      code='rcwa.unsupported', path='outputs.fields')
     if not problem.outputs.diffraction:
         raise UnsupportedFeatureError('Outputs(diffraction=False) leaves nothing this engine computes.',
                                       code='rcwa.unsupported', path='outputs.diffraction')
@@ -293,7 +293,8 @@ def _resolve(problem, numerics, execution):
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
                      hu=hu, hv=hv, uniform=uniform, factorization=factorization,
                      coefficients=numerics.fourier_coefficients or 'sampled',
-                     scattering=bool(problem.outputs.scattering), side=src.side, k_direct=k_direct)
+                     scattering=bool(problem.outputs.scattering), side=src.side, k_direct=k_direct,
+                     fields=bool(problem.outputs.fields), bare=not layers)
 
 
 def _cell(lattice):
@@ -456,8 +457,12 @@ class Prepared:
         if _live(problem):
             return _run_live(self, problem, r, checks)
         try:
-            per_wl, flags, smats = _solve_resolved(r, self._cache)
-            r, per_wl, flags, smats = _detune(r, self._cache, per_wl, flags, smats, self.numerics)
+            per_wl, flags, smats, record = _solve_resolved(r, self._cache)
+            r2, per_wl, flags, smats = _detune(r, self._cache, per_wl, flags, smats, self.numerics)
+            if r.fields and r2 is not r:
+                # the ladder moved points: the field state is taken at the effective points
+                record = _solve_resolved(r2, self._cache.sharing_conv())[3]
+            r = r2
         except FloatingPointError as error:
             raise NumericalError(str(error), code='rcwa.singular', path='problem') from error
         except ValueError as error:
@@ -479,7 +484,7 @@ class Prepared:
                 'effective_theta': r.thetas or tuple(r.source.theta for _ in r.wavelengths),
                 'exceptional_points': ep_counts}
         result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, scattering=smats,
-                      side=r.side)
+                      side=r.side, fields=record)
         if not result.validity.checks['numerical_finite']:
             raise NumericalError('The requested solve produced nonfinite observables; no implicit '
                                  'detuning was applied.', code='rcwa.nonfinite', path='result')
@@ -500,11 +505,12 @@ def _run_live(prepared, problem, r, checks):
         raise UnsupportedFeatureError(
             'A differentiable run solves the requested point; detuning is applied to concrete '
             'runs only.', code='rcwa.gradient', path='numerics.detuning')
-    if r.scattering:
+    if r.scattering or r.fields:
+        what = 'scattering' if r.scattering else 'fields'
         raise UnsupportedFeatureError(
             'Gradients are implemented for the diffraction outputs; a differentiable run cannot '
-            'also keep the scattering matrix. Solve it with concrete values.',
-            code='rcwa.gradient', path='outputs.scattering')
+            f'also keep the {"scattering matrix" if r.scattering else "field state"}. Solve it with '
+            'concrete values.', code='rcwa.gradient', path=f'outputs.{what}')
     device = torch.device(prepared._cache.conv.device)
     _engine._native.context(device)
     layers = problem.stack.layers
@@ -671,7 +677,8 @@ def _detune(r, cache, per_wl, flags, smats, numerics):
         trial_theta = theta0 * (1 + step) if knob == 'theta' else None
         sub = replace(r, wavelengths=trial_wl,
                       source=replace(r.source, theta=trial_theta) if trial_theta is not None else r.source)
-        sub_per, sub_flags, sub_smats = _solve_resolved(sub, cache.sharing_conv())
+        sub_per, sub_flags, sub_smats, _ = _solve_resolved(replace(sub, fields=False),
+                                                           cache.sharing_conv())
         still = []
         for j, p in enumerate(bad):
             if np.any(sub_flags[j] & fatal):
@@ -715,14 +722,26 @@ def _solve_resolved(r, cache):
     _engine._native.context(device)          # raises EngineUnavailable without a GPU or library
     conv = cache.get(r.rasters, r.hu, r.hv, r.factorization)
     bucket = _engine.Bucket(hu=r.hu, hv=r.hv, path=r.path, n_layers=r.rasters.shape[0])
-    for wl in r.wavelengths:
+    for p, wl in enumerate(r.wavelengths):
         bucket.points.append(_engine.Point(
-            stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
-            polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
-            period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct))
+            stack=0, wavelength=float(wl), theta=r.thetas[p] if r.thetas else r.source.theta,
+            phi=r.source.phi, polarization=r.polarization, eps_top=r.eps_top,
+            eps_bottom=r.eps_bottom, period=r.period, thickness=tuple(float(t) for t in r.thickness),
+            k_direct=r.k_direct))
     smats = None
     smat = None
-    if r.scattering:
+    record = None
+    if r.fields:
+        eff, amp, pe, flags, state = _engine.run_fields(bucket, [conv], device)
+        record = FieldRecord(device=device, hu=r.hu, hv=r.hv, mode=_engine._MODE[r.path],
+                             thickness=r.thickness[::-1] if r.side == 'bottom' else r.thickness,
+                             cell=_engine.cell_of(r.period), mirrored=r.side == 'bottom',
+                             amplitude=_scalar(r.source.amplitude, 'excitation.sources.amplitude',
+                                               real=False),
+                             bare=r.bare, **state)
+        if r.scattering:
+            smat = _engine.run_bucket_scattering(bucket, [conv], device)[5].cpu().numpy()
+    elif r.scattering:
         eff, amp, pe, flags, _, smat = _engine.run_bucket_scattering(bucket, [conv], device)
         smat = smat.cpu().numpy()
     elif _resident_modes(bucket):
@@ -741,7 +760,7 @@ def _solve_resolved(r, cache):
     if smat is not None:
         smats = _scattering_matrices(r, smat)
     per_wl = [_engine.unpack(eff, amp, pe, p, r.hu, r.hv, r.path) for p in range(len(bucket.points))]
-    return per_wl, flags, smats
+    return per_wl, flags, smats, record
 
 
 # Preparations whose fused solve needs more workspace than a captured graph is kept for, and whose

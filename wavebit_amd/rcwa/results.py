@@ -87,6 +87,7 @@ class Result:
     diagnostics: object
     provenance: object
     _scattering: object = None       # tuple of ScatteringMatrix, one per wavelength
+    _fields: object = None           # fields.FieldRecord of an Outputs(fields=True) run
 
     def __post_init__(self):
         for name in ('outputs', 'diagnostics', 'provenance'):
@@ -128,6 +129,17 @@ class Result:
                                           code='outputs.unavailable', path='outputs.absorption')
         return self.outputs['absorption']
 
+    def field(self, *, x, y, z, reconstruction='plain'):
+        """Collocated complex phasors at Cartesian sample points, shape
+        ``(wavelength, z, y, x, component)``, E in V/m and H in A/m (reference
+        ``results.py:149-227``). ``x``, ``y`` from the cell centre and ``z`` up from the bottom
+        interface, in the problem's length unit; an integer count samples uniformly instead (the
+        cell across, every layer face to face) and the returned ``coordinates`` hold the points
+        sampled. Interfaces select the upper layer. Only ``reconstruction='plain'`` is
+        implemented."""
+        from .fields import field
+        return field(self, x, y, z, reconstruction)
+
     @property
     def scattering(self):
         """The stack's ``ScatteringMatrix`` per wavelength (reference ``results.py:136-141``)."""
@@ -138,7 +150,7 @@ class Result:
 
 
 def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, scattering=None,
-         side='top'):
+         side='top', fields=None):
     """Stack per-wavelength solve outputs (dicts from the engine) into a ``Result``.
 
     NaN-masks every masked output when a validity check failed (reference
@@ -179,7 +191,9 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, s
     if scattering is not None:
         scattering = tuple(m._replace(S11=m.S11 * scale, S12=m.S12 * scale, S21=m.S21 * scale,
                                       S22=m.S22 * scale) for m in scattering)
-    return Result(outputs=outputs, wavelengths=ws, _scattering=scattering,
+    if fields is not None and not combined:
+        fields.amplitude = complex('nan')      # sampled fields follow the admission like the rest
+    return Result(outputs=outputs, wavelengths=ws, _scattering=scattering, _fields=fields,
                   validity=Validity(checks={**checks, 'numerical_finite': combined},
                                     convergence={'status': 'not assessed',
                                                  'requires': 'independent refinement study'}),
