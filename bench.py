@@ -159,6 +159,8 @@ def main():
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='run one device copy of this size first (HBM counter calibration)')
     ap.add_argument('--no-peaks', action='store_true', help='skip the measured DGEMM / copy peaks')
+    ap.add_argument('--field-samples', type=int, default=64,
+                    help='fields leg: x and y samples per cell (0 skips the leg); 16 heights per layer')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the points of a batch are '
@@ -443,6 +445,65 @@ def main():
                                             and np.array_equal(res4.T, fresh4.T))}
         del prep4, res4, fresh4
 
+    # in-layer fields of the benchmark problem: the run that keeps the field state (layer modes and
+    # mode amplitudes), then Result.field on a field_samples^2 x (16 per layer) grid of every
+    # wavelength; the flux through every sampled height checked against T (the stack is lossless)
+    fields_leg = None
+    if args.field_samples > 0:
+        import wavebit_amd
+        from wavebit_amd import rcwa
+        from wavebit_amd.units import Z0
+        layers_f = [rcwa.Layer(thickness=float(t), epsilon=e) for t, e in zip(wl['thickness'], wl['rasters'])]
+        prob_f = rcwa.Problem(
+            stack=rcwa.Stack(layers=layers_f, top=wavebit_amd.Material(wl['eps_top']),
+                             bottom=wavebit_amd.Material(wl['eps_bottom']), period=wl['period']),
+            excitation=rcwa.Excitation(wavelength=[p['wavelength'] for p in wl['points']],
+                                       sources=(rcwa.PlaneWave(polarization='p', theta=0.0, phi=0.0),)),
+            outputs=rcwa.Outputs(fields=True), units=wavebit_amd.Units(length='nm'))
+        num_f = rcwa.Numerics(orders=(hu, hv))
+        rcwa.run(prob_f, numerics=num_f, backend='torch', device='cuda:0')     # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res_f = rcwa.run(prob_f, numerics=num_f, backend='torch', device='cuda:0')
+        field_run_s = time.perf_counter() - t0
+        ns, nz_l = args.field_samples, 16
+        res_f.field(x=ns, y=ns, z=2)                                             # warm
+        torch.cuda.synchronize()
+        _native.timing_enable(True)
+        _native.timing_read('zgemm', reset=True)
+        t0 = time.perf_counter()
+        vals_dev, _ = res_f._fields.sample(ns, ns, nz_l)          # the sampling, values on the device
+        torch.cuda.synchronize()
+        device_s = time.perf_counter() - t0
+        _native.timing_enable(False)
+        g_ms, g_cnt, g_fl = _native.timing_read('zgemm', reset=True)
+        del vals_dev
+        t0 = time.perf_counter()
+        fs = res_f.field(x=ns, y=ns, z=nz_l)                        # the public call, host arrays
+        sample_s = time.perf_counter() - t0
+        n_samples = P * L * nz_l * ns * ns
+        v = fs.values[:, :, :-1, :-1]
+        flux = 0.5 * np.real(v[..., 0] * np.conj(v[..., 4]) - v[..., 1] * np.conj(v[..., 3])).mean(axis=(2, 3))
+        n_top = np.sqrt(complex(wl['eps_top'])).real
+        incident = 0.5 * n_top / Z0
+        T_tot = res_f.T.sum(axis=(1, 2))
+        lossless = all(np.isrealobj(r) or not np.any(np.imag(r)) for r in wl['rasters'])
+        fields_leg = {
+            'workload': f'the benchmark problem with Outputs(fields=True), then Result.field on '
+                        f'{ns} x {ns} points x {nz_l} heights per layer for all {P} wavelengths',
+            'field_run_s': round(field_run_s, 4),
+            'field_run_solves_per_s': round(P / field_run_s, 3),
+            'sample_device_s': round(device_s, 4), 'samples': n_samples,
+            'device_samples_per_s': round(n_samples / device_s, 1),
+            'field_call_s': round(sample_s, 4),
+            'field_call_note': 'Result.field including the copy of every value to host memory',
+            'zgemm_ms': round(g_ms, 3), 'zgemm_launches': g_cnt,
+            'zgemm_tflops_per_s': round(g_fl / (g_ms * 1e-3) / 1e12, 3) if g_ms > 0 else None,
+            'zgemm_frac_of_fp64_peak': round(g_fl / (g_ms * 1e-3) / 78.6e12, 4) if g_ms > 0 else None,
+            'flux_vs_T_max_rel_err': (float(np.max(np.abs(flux + T_tot[:, None] * incident)) / incident)
+                                      if lossless else None)}
+        del res_f, fs, v
+
     # cfg5: a prefix of the seeded serving stream at its stated distribution
     cfg5 = None
     if args.cfg5_requests > 0:
@@ -674,6 +735,7 @@ def main():
         'gradient': grad,
         'serve': serve_report,
         'thickness_sweep': sweep,
+        'fields': fields_leg,
         'serve_cfg5': cfg5,
         'launch_bound': small,
         'crossed_grating': crossed,

@@ -233,18 +233,21 @@ int wb_field_state(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const do
                    const int* modes_idx, double* amps, double* eff, double* amp, double* pe,
                    int* flags, void* workspace, size_t workspace_bytes, void* stream);
 
-/* The six field components (Ex, Ey, Ez, Hx, Hy, Hz, H in units of E) in one layer of one point,
- * public frame (+z up, x and y from the cell centre), the expansion of
- * rcwa/_engine/field_result.py:109-114 over the components of rcwa/_engine/fields.py:503-814:
- * out (nz x nxy x 6 complex, device) = scale[c] x component c at local heights z[j] (device,
- * measured up from the layer's bottom face, 0 <= z <= thickness) and points xy (device, nxy
- * x, y pairs). pt: the point's parameter record (device, as wb_solve's pts); W, V (n x n), q (n)
- * and amps (2n) of the layer's pair from wb_layer_modes / wb_field_state; epz_i (N x N) the
- * layer's inverse-rule matrix (not read on the TE path); scale: host array of 6 complex factors
- * (1/Z0 on H gives A/m for E in V/m). The workspace is wb_field_sample_workspace_size's. */
-size_t wb_field_sample_workspace_size(int hu, int hv, int mode, int nz, int nxy);
-int wb_field_sample(wb_ctx* ctx, int hu, int hv, int mode, const double* pt, double thickness,
-                    const double* W, const double* V, const double* q, const double* epz_i,
+/* The six field components (Ex, Ey, Ez, Hx, Hy, Hz, H in units of E) of P points x L layers at
+ * nz heights per layer, public frame (+z up, x and y from the cell centre): the expansion of
+ * rcwa/_engine/field_result.py:109-114 over the components of rcwa/_engine/fields.py:503-814, in
+ * one batched launch sequence. out (P x L nz x nxy x 6 complex, device) = scale[c] x component c
+ * at local heights z[l nz + j] (device, L x nz, measured up from layer l's bottom face, within
+ * its thickness) and points xy (device, nxy x, y pairs). Pair b = p L + l: pts and thick as
+ * wb_solve's; its modes at modes_idx[b] (NULL: b) of W, V (n x n) and q (n) from
+ * wb_layer_modes, its amplitudes at b of amps (2n) from wb_field_state, its inverse-rule matrix
+ * at conv_idx[b] (NULL: b) of epz_i (N x N; not read on the TE path); scale: host array of 6
+ * complex factors (1/Z0 on H gives A/m for E in V/m). The workspace is
+ * wb_field_sample_workspace_size's. */
+size_t wb_field_sample_workspace_size(int P, int L, int hu, int hv, int mode, int nz, int nxy);
+int wb_field_sample(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                    const double* thick, const double* W, const double* V, const double* q,
+                    const int* modes_idx, const double* epz_i, const int* conv_idx,
                     const double* amps, int nz, const double* z, int nxy, const double* xy,
                     const double* scale, double* out, void* workspace, size_t workspace_bytes,
                     void* stream);

@@ -96,10 +96,10 @@ def load_library():
             'wb_field_state': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
                                        c_vp]),
-            'wb_field_sample_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
-            'wb_field_sample': (c_int, [c_vp, c_int, c_int, c_int, c_vp, ctypes.c_double, c_vp, c_vp,
-                                        c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_dp, c_vp,
-                                        c_vp, c_sz, c_vp]),
+            'wb_field_sample_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
+            'wb_field_sample': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
+                                        c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_dp,
+                                        c_vp, c_vp, c_sz, c_vp]),
             'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
@@ -468,36 +468,40 @@ def field_state(pts, thick, W, V, q, modes_idx, hu, hv, mode, workspace=None):
     return amps, eff, amp, pe, flags
 
 
-def field_sample_bytes(hu, hv, mode, nz, nxy):
-    return int(load_library().wb_field_sample_workspace_size(hu, hv, mode, nz, nxy))
+def field_sample_bytes(P, L, hu, hv, mode, nz, nxy):
+    return int(load_library().wb_field_sample_workspace_size(P, L, hu, hv, mode, nz, nxy))
 
 
-def field_sample(pt, thickness, W, V, q, epz_i, amps, z, xy, hu, hv, mode, scale, out=None,
+def field_sample(pts, thick, W, V, q, modes_idx, epz_i, conv_idx, amps, z, xy, hu, hv, mode, scale,
                  workspace=None):
-    """Fields of one layer of one point; see ``wb_field_sample``. ``pt`` (PT_STRIDE,) float64,
-    ``W``/``V`` (n, n), ``q`` (n,), ``epz_i`` (N, N) or None on the TE path, ``amps`` (2, n), ``z``
-    (nz,) local heights, ``xy`` (nxy, 2); ``scale`` six complex factors. Returns (nz, nxy, 6)."""
+    """Fields of P points x L layers; see ``wb_field_sample``. ``pts`` (P, 16), ``thick`` (P, L),
+    ``W``/``V`` (pairs, n, n), ``q`` (pairs, n), ``modes_idx`` / ``conv_idx`` (P, L) int32 or None,
+    ``epz_i`` (C, N, N) or None on the TE path, ``amps`` (P L, 2, n), ``z`` (L, nz) local heights,
+    ``xy`` (nxy, 2); ``scale`` six complex factors. Returns (P, L nz, nxy, 6)."""
     device = W.device
     ctx = context(device)
-    _require(pt, torch.float64, device, 'pt')
+    P, L = thick.shape
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
     _require(z, torch.float64, device, 'z')
     _require(xy, torch.float64, device, 'xy')
     for name, t in (('W', W), ('V', V), ('q', q), ('amps', amps)):
         _require(t, torch.complex128, device, name)
     if epz_i is not None:
         _require(epz_i, torch.complex128, device, 'epz_i')
-    nz, nxy = int(z.shape[0]), int(xy.shape[0])
-    if out is None:
-        out = torch.empty((nz, nxy, 6), dtype=torch.complex128, device=device)
-    _require(out, torch.complex128, device, 'out')
+    for name, t in (('modes_idx', modes_idx), ('conv_idx', conv_idx)):
+        if t is not None:
+            _require(t, torch.int32, device, name)
+    nz, nxy = int(z.shape[1]), int(xy.shape[0])
+    out = torch.empty((P, L * nz, nxy, 6), dtype=torch.complex128, device=device)
     sc = (ctypes.c_double * 12)(*[v for c in scale for v in (complex(c).real, complex(c).imag)])
-    nbytes = ctx.lib.wb_field_sample_workspace_size(hu, hv, mode, nz, nxy)
+    nbytes = ctx.lib.wb_field_sample_workspace_size(P, L, hu, hv, mode, nz, nxy)
     if workspace is None or workspace.numel() < nbytes:
         workspace = _workspace(nbytes, device)
-    ctx.check(ctx.lib.wb_field_sample(ctx.handle, hu, hv, mode, _ptr(pt), float(thickness), _ptr(W),
-                                      _ptr(V), _ptr(q), _ptr(epz_i), _ptr(amps), nz, _ptr(z), nxy,
-                                      _ptr(xy), sc, _ptr(out), _ptr(workspace), nbytes,
-                                      _stream(device)), 'field_sample')
+    ctx.check(ctx.lib.wb_field_sample(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick), _ptr(W),
+                                      _ptr(V), _ptr(q), _ptr(modes_idx), _ptr(epz_i), _ptr(conv_idx),
+                                      _ptr(amps), nz, _ptr(z), nxy, _ptr(xy), sc, _ptr(out),
+                                      _ptr(workspace), nbytes, _stream(device)), 'field_sample')
     return out
 
 

@@ -343,31 +343,34 @@ int wb_field_state(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const do
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_field_state");
 }
 
-size_t wb_field_sample_workspace_size(int hu, int hv, int mode, int nz, int nxy) {
-  if (hu < 0 || hv < 0 || mode < 0 || mode > 2 || nz <= 0 || nxy <= 0) return 0;
-  return wb_field_sample_bytes(hu, hv, mode, nz, nxy);
+size_t wb_field_sample_workspace_size(int P, int L, int hu, int hv, int mode, int nz, int nxy) {
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || nz <= 0 || nxy <= 0) return 0;
+  return wb_field_sample_bytes(P, L, hu, hv, mode, nz, nxy);
 }
 
-int wb_field_sample(wb_ctx* ctx, int hu, int hv, int mode, const double* pt, double thickness,
-                    const double* W, const double* V, const double* q, const double* epz_i,
+int wb_field_sample(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
+                    const double* thick, const double* W, const double* V, const double* q,
+                    const int* modes_idx, const double* epz_i, const int* conv_idx,
                     const double* amps, int nz, const double* z, int nxy, const double* xy,
                     const double* scale, double* out, void* workspace, size_t workspace_bytes,
                     void* stream) {
   int s = enter(ctx);
   if (s) return s;
-  if (hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0) || nz <= 0 || nxy <= 0 ||
-      !(thickness >= 0.0))
+  if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0) ||
+      nz <= 0 || nxy <= 0)
     return fail(ctx, WB_BADARG, "wb_field_sample: bad dimensions");
-  if (!pt || !W || !V || !q || !amps || !z || !xy || !scale || !out || (mode != WB_MODE_1D_TE && !epz_i))
+  if (!pts || !thick || !W || !V || !q || !amps || !z || !xy || !scale || !out ||
+      (mode != WB_MODE_1D_TE && !epz_i))
     return fail(ctx, WB_BADARG, "wb_field_sample: null buffer");
-  if (workspace_bytes < wb_field_sample_workspace_size(hu, hv, mode, nz, nxy))
+  if (workspace_bytes < wb_field_sample_workspace_size(P, L, hu, hv, mode, nz, nxy))
     return fail(ctx, WB_OOM, "wb_field_sample: workspace too small");
   cd f[6];
   for (int k = 0; k < 6; ++k) f[k] = mkc(scale[2 * k], scale[2 * k + 1]);
   return hip_status(ctx,
-                    wb_field_sample_launch(hu, hv, mode, pt, thickness, (const cd*)W, (const cd*)V,
-                                           (const cd*)q, (const cd*)epz_i, (const cd*)amps, nz, z,
-                                           nxy, xy, f, (cd*)out, workspace, (hipStream_t)stream),
+                    wb_field_sample_launch(P, L, hu, hv, mode, pts, thick, (const cd*)W, (const cd*)V,
+                                           (const cd*)q, modes_idx, (const cd*)epz_i, conv_idx,
+                                           (const cd*)amps, nz, z, nxy, xy, f, (cd*)out, workspace,
+                                           (hipStream_t)stream),
                     "wb_field_sample");
 }
 

@@ -67,59 +67,74 @@ __global__ void k_field_amps(int n, int L, int nrhs, int mode, const double* pts
   amps[b * 2 * n + n + r] = up;
 }
 
-// Propagated amplitudes at the samples: Pp = a_up + a_down, Pm = a_down - a_up (n x nz,
-// row = mode). One thread per (mode, sample).
-__global__ void k_field_prop(int n, int nz, const double* pt, double d, const cd* q,
-                             const cd* amps, const double* z, cd* Pp, cd* Pm) {
+// Propagated amplitudes at the samples of every pair: Pp = a_up + a_down, Pm = a_down - a_up
+// (pair b: n x nz, row = mode, column = sample j of layer l = b % L at local height z[l nz + j]).
+__global__ void k_field_prop(int n, int L, int nz, const double* pts, const double* thick, const cd* q,
+                             const int* modes_idx, const cd* amps, const double* z, cd* Pp, cd* Pm) {
+  const long long b = blockIdx.y;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)n * nz) return;
   const int i = (int)(e / nz), j = (int)(e % nz);
-  const double k0 = 2.0 * PI / pt[PT_WL];
-  const cd qi = q[i];
-  const double zj = z[j];
-  const cd up = cmul(cexp_(cscale(qi, -k0 * zj)), amps[n + i]);
-  const cd down = cmul(cexp_(cscale(qi, k0 * (zj - d))), amps[i]);
-  Pp[e] = cadd(up, down);
-  Pm[e] = csub(down, up);
+  const long long p = b / L;
+  const int l = (int)(b % L);
+  const double k0 = 2.0 * PI / pts[p * PT_STRIDE + PT_WL];
+  const long long mb = modes_idx ? modes_idx[b] : b;
+  const cd qi = q[mb * n + i];
+  const double zj = z[(long long)l * nz + j];
+  const double d = thick[b];
+  const cd up = cmul(cexp_(cscale(qi, -k0 * zj)), amps[b * 2 * n + n + i]);
+  const cd down = cmul(cexp_(cscale(qi, k0 * (zj - d))), amps[b * 2 * n + i]);
+  const long long o = b * (long long)n * nz + e;
+  Pp[o] = cadd(up, down);
+  Pm[o] = csub(down, up);
 }
 
-// The longitudinal partial that epz_i multiplies: 2D Kx Uy - Ky Ux, TM Kx M (N x nz).
-__global__ void k_field_tz(int hu, int hv, int mode, int nz, const double* pt, const cd* S,
+// The longitudinal partial that epz_i multiplies: 2D Kx Uy - Ky Ux, TM Kx M (pair b: N x nz).
+__global__ void k_field_tz(int hu, int hv, int mode, int L, int nz, const double* pts, const cd* S,
                            const cd* U, cd* T) {
   const int N = (2 * hu + 1) * (2 * hv + 1);
+  const int n = (mode == 0) ? 2 * N : N;
+  const long long b = blockIdx.y;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)N * nz) return;
   const int o = (int)(e / nz), j = (int)(e % nz);
   double kx, ky;
-  order_k(pt, hu, hv, mode, o, &kx, &ky);
-  if (mode == 0) {
-    T[e] = csub(cscale(U[(long long)(N + o) * nz + j], kx), cscale(U[(long long)o * nz + j], ky));
-  } else {
-    T[e] = cscale(S[(long long)o * nz + j], kx);
-  }
+  order_k(pts + (b / L) * PT_STRIDE, hu, hv, mode, o, &kx, &ky);
+  const cd* Sb = S + b * (long long)n * nz;
+  const cd* Ub = U + b * (long long)n * nz;
+  cd v;
+  if (mode == 0) v = csub(cscale(Ub[(long long)(N + o) * nz + j], kx), cscale(Ub[(long long)o * nz + j], ky));
+  else v = cscale(Sb[(long long)o * nz + j], kx);
+  T[b * (long long)N * nz + e] = v;
 }
 
 struct Scale6 { cd c[6]; };
 
-// Coefficients C (N x 6 nz, column j 6 + component) of the six components, with the centre
-// parity of each order and the caller's component factors.
-__global__ void k_field_coef(int hu, int hv, int mode, int nz, const double* pt, const cd* S,
+// Coefficients of the six components with the centre parity of each order and the caller's
+// component factors: point p's block is N x 6 L nz, column (l nz + j) 6 + component.
+__global__ void k_field_coef(int hu, int hv, int mode, int L, int nz, const double* pts, const cd* S,
                              const cd* U, const cd* Ez, Scale6 f, cd* C) {
   const int nu = 2 * hu + 1;
   const int N = nu * (2 * hv + 1);
+  const int n = (mode == 0) ? 2 * N : N;
+  const long long b = blockIdx.y;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)N * nz) return;
   const int o = (int)(e / nz), j = (int)(e % nz);
+  const long long p = b / L;
+  const int l = (int)(b % L);
   double kx, ky;
-  order_k(pt, hu, hv, mode, o, &kx, &ky);
+  order_k(pts + p * PT_STRIDE, hu, hv, mode, o, &kx, &ky);
   const int m = o % nu - hu, nn = o / nu - hv;
   const double par = ((m + nn) & 1) ? -1.0 : 1.0;
+  const cd* Sb = S + b * (long long)n * nz;
+  const cd* Ub = U + b * (long long)n * nz;
   const cd zero = mkc(0.0, 0.0);
   cd v[6];
   if (mode == 0) {
-    const cd mx = S[(long long)o * nz + j], my = S[(long long)(N + o) * nz + j];
-    const cd ux = U[(long long)o * nz + j], uy = U[(long long)(N + o) * nz + j];
-    const cd ez = Ez[(long long)o * nz + j];
+    const cd mx = Sb[(long long)o * nz + j], my = Sb[(long long)(N + o) * nz + j];
+    const cd ux = Ub[(long long)o * nz + j], uy = Ub[(long long)(N + o) * nz + j];
+    const cd ez = Ez[b * (long long)N * nz + e];
     v[0] = mx;
     v[1] = my;
     v[2] = mkc(ez.y, -ez.x);                        // -i epz_i (Kx Uy - Ky Ux)
@@ -127,67 +142,72 @@ __global__ void k_field_coef(int hu, int hv, int mode, int nz, const double* pt,
     v[4] = mkc(-uy.y, uy.x);                        // i Uy
     v[5] = csub(cscale(my, kx), cscale(mx, ky));
   } else if (mode == 1) {
-    const cd mm = S[(long long)o * nz + j], u = U[(long long)o * nz + j];
+    const cd mm = Sb[(long long)o * nz + j], u = Ub[(long long)o * nz + j];
     v[0] = zero; v[1] = mm; v[2] = zero;
     v[3] = mkc(-u.y, u.x);                          // i U
     v[4] = zero;
     v[5] = cscale(mm, kx);
   } else {
-    const cd mm = S[(long long)o * nz + j], u = U[(long long)o * nz + j];
-    const cd ez = Ez[(long long)o * nz + j];
+    const cd mm = Sb[(long long)o * nz + j], u = Ub[(long long)o * nz + j];
+    const cd ez = Ez[b * (long long)N * nz + e];
     v[0] = mkc(u.y, -u.x);                          // -i U
     v[1] = zero;
     v[2] = cneg(ez);                                // -epz_i Kx M
     v[3] = zero; v[4] = mm; v[5] = zero;
   }
-  cd* c = C + (long long)o * 6 * nz + (long long)j * 6;
+  const long long cols = 6LL * L * nz;
+  cd* c = C + p * (long long)N * cols + (long long)o * cols + ((long long)l * nz + j) * 6;
   for (int k = 0; k < 6; ++k) c[k] = cscale(cmul(f.c[k], v[k]), par);
 }
 
-// E (nxy x N): exp(i k0 (x kx + y ky)) at centre-origin sample points.
-__global__ void k_field_phase(int hu, int hv, int mode, int nxy, const double* pt, const double* xy,
+// E_p (nxy x N): exp(i k0 (x kx + y ky)) at centre-origin sample points, per point p.
+__global__ void k_field_phase(int hu, int hv, int mode, int nxy, const double* pts, const double* xy,
                               cd* E) {
   const int N = (2 * hu + 1) * (2 * hv + 1);
+  const long long p = blockIdx.y;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)nxy * N) return;
   const int s = (int)(e / N), o = (int)(e % N);
+  const double* pt = pts + p * PT_STRIDE;
   double kx, ky;
   order_k(pt, hu, hv, mode, o, &kx, &ky);
   const double k0 = 2.0 * PI / pt[PT_WL];
   const double ph = k0 * (xy[2 * s] * kx + xy[2 * s + 1] * ky);
   double sn, cs;
   sincos(ph, &sn, &cs);
-  E[e] = mkc(cs, sn);
+  E[p * (long long)nxy * N + e] = mkc(cs, sn);
 }
 
-// out (nz x nxy x 6) from F (nxy x 6 nz).
-__global__ void k_field_permute(int nz, int nxy, const cd* F, cd* out) {
+// out_p (L nz x nxy x 6) from F_p (nxy x 6 L nz).
+__global__ void k_field_permute(int rows, int nxy, const cd* F, cd* out) {
+  const long long p = blockIdx.y;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = (long long)nz * nxy * 6;
-  if (e >= total) return;
+  const long long per = (long long)rows * nxy * 6;
+  if (e >= per) return;
   const int c = (int)(e % 6);
   const long long rest = e / 6;
   const int s = (int)(rest % nxy), j = (int)(rest / nxy);
-  out[e] = F[(long long)s * 6 * nz + (long long)j * 6 + c];
+  out[p * per + e] = F[p * per + (long long)s * 6 * rows + (long long)j * 6 + c];
 }
 
 struct SampleWs {
   cd *Pp, *Pm, *S, *U, *T, *Ez, *C, *E, *F;
 };
 
-SampleWs sample_carve(void* ws, int N, int n, int nz, int nxy) {
+SampleWs sample_carve(void* ws, int P, int L, int N, int n, int nz, int nxy) {
   char* p = (char*)ws;
   auto take = [&](size_t elems) { char* r = p; p += align_up(elems * sizeof(cd)); return (cd*)r; };
+  const size_t B = (size_t)P * L;
   SampleWs w;
-  w.Pp = take((size_t)n * nz);
-  w.Pm = take((size_t)n * nz);
-  w.S = take((size_t)n * nz);
-  w.U = take((size_t)n * nz);
-  w.T = take((size_t)N * nz);
-  w.Ez = take((size_t)N * nz);
-  w.C = take((size_t)N * 6 * nz);
-  w.E = take((size_t)nxy * N);
-  w.F = take((size_t)nxy * 6 * nz);
+  w.Pp = take(B * n * nz);
+  w.Pm = take(B * n * nz);
+  w.S = take(B * n * nz);
+  w.U = take(B * n * nz);
+  w.T = take(B * N * nz);
+  w.Ez = take(B * N * nz);
+  w.C = take(B * N * 6 * nz);
+  w.E = take((size_t)P * nxy * N);
+  w.F = take(B * (size_t)nxy * 6 * nz);
   return w;
 }
 
@@ -249,48 +269,58 @@ hipError_t wb_field_amps_stage(const wb_solve_args& a, int N, int n, int nrhs, c
   return hipGetLastError();
 }
 
-size_t wb_field_sample_bytes(int hu, int hv, int mode, int nz, int nxy) {
+size_t wb_field_sample_bytes(int P, int L, int hu, int hv, int mode, int nz, int nxy) {
   const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
   const size_t n = (mode == 0) ? 2 * N : N;
-  return 4 * align_up(n * nz * sizeof(cd)) + 2 * align_up(N * nz * sizeof(cd)) +
-         align_up(N * 6 * nz * sizeof(cd)) + align_up((size_t)nxy * N * sizeof(cd)) +
-         align_up((size_t)nxy * 6 * nz * sizeof(cd)) + 256;
+  const size_t B = (size_t)P * L;
+  return 4 * align_up(B * n * nz * sizeof(cd)) + 2 * align_up(B * N * nz * sizeof(cd)) +
+         align_up(B * N * 6 * nz * sizeof(cd)) + align_up((size_t)P * nxy * N * sizeof(cd)) +
+         align_up(B * (size_t)nxy * 6 * nz * sizeof(cd)) + 256;
 }
 
-hipError_t wb_field_sample_launch(int hu, int hv, int mode, const double* pt, double d, const cd* W,
-                                  const cd* V, const cd* q, const cd* epz, const cd* amps, int nz,
-                                  const double* z, int nxy, const double* xy, const cd* scale,
-                                  cd* out, void* ws, hipStream_t st) {
+hipError_t wb_field_sample_launch(int P, int L, int hu, int hv, int mode, const double* pts,
+                                  const double* thick, const cd* W, const cd* V, const cd* q,
+                                  const int* modes_idx, const cd* epz, const int* conv_idx,
+                                  const cd* amps, int nz, const double* z, int nxy, const double* xy,
+                                  const cd* scale, cd* out, void* ws, hipStream_t st) {
   const int N = (2 * hu + 1) * (2 * hv + 1);
   const int n = (mode == 0) ? 2 * N : N;
-  const SampleWs w = sample_carve(ws, N, n, nz, nxy);
+  const int B = P * L;
+  const long long nn = (long long)n * n;
+  const SampleWs w = sample_carve(ws, P, L, N, n, nz, nxy);
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0);
   hipError_t e;
-  hipLaunchKernelGGL(k_field_prop, grid2((long long)n * nz, 1), dim3(256), 0, st, n, nz, pt, d, q, amps,
-                     z, w.Pp, w.Pm);
+  hipLaunchKernelGGL(k_field_prop, grid2((long long)n * nz, B), dim3(256), 0, st, n, L, nz, pts, thick, q,
+                     modes_idx, amps, z, w.Pp, w.Pm);
   if ((e = hipGetLastError())) return e;
-  if ((e = wb_zgemm_launch(n, nz, n, one, bp_strided(W, 0), n, bp_strided(w.Pp, 0), nz, zero,
-                           bp_strided(w.S, 0), nz, 1, st))) return e;
-  if ((e = wb_zgemm_launch(n, nz, n, one, bp_strided(V, 0), n, bp_strided(w.Pm, 0), nz, zero,
-                           bp_strided(w.U, 0), nz, 1, st))) return e;
+  const BatchPtr Wb = modes_idx ? bp_gather(W, nn, modes_idx) : bp_strided(W, nn);
+  const BatchPtr Vb = modes_idx ? bp_gather(V, nn, modes_idx) : bp_strided(V, nn);
+  const long long vz = (long long)n * nz;
+  if ((e = wb_zgemm_launch(n, nz, n, one, Wb, n, bp_strided(w.Pp, vz), nz, zero, bp_strided(w.S, vz),
+                           nz, B, st))) return e;
+  if ((e = wb_zgemm_launch(n, nz, n, one, Vb, n, bp_strided(w.Pm, vz), nz, zero, bp_strided(w.U, vz),
+                           nz, B, st))) return e;
+  const long long tz = (long long)N * nz;
   if (mode != 1) {
-    hipLaunchKernelGGL(k_field_tz, grid2((long long)N * nz, 1), dim3(256), 0, st, hu, hv, mode, nz, pt,
-                       w.S, w.U, w.T);
+    hipLaunchKernelGGL(k_field_tz, grid2(tz, B), dim3(256), 0, st, hu, hv, mode, L, nz, pts, w.S, w.U, w.T);
     if ((e = hipGetLastError())) return e;
-    if ((e = wb_zgemm_launch(N, nz, N, one, bp_strided(epz, 0), N, bp_strided(w.T, 0), nz, zero,
-                             bp_strided(w.Ez, 0), nz, 1, st))) return e;
+    const BatchPtr Zb = conv_idx ? bp_gather(epz, (long long)N * N, conv_idx) : bp_strided(epz, (long long)N * N);
+    if ((e = wb_zgemm_launch(N, nz, N, one, Zb, N, bp_strided(w.T, tz), nz, zero, bp_strided(w.Ez, tz), nz,
+                             B, st))) return e;
   }
   Scale6 f;
   for (int k = 0; k < 6; ++k) f.c[k] = scale[k];
-  hipLaunchKernelGGL(k_field_coef, grid2((long long)N * nz, 1), dim3(256), 0, st, hu, hv, mode, nz, pt,
-                     w.S, w.U, w.Ez, f, w.C);
+  hipLaunchKernelGGL(k_field_coef, grid2(tz, B), dim3(256), 0, st, hu, hv, mode, L, nz, pts, w.S, w.U,
+                     w.Ez, f, w.C);
   if ((e = hipGetLastError())) return e;
-  hipLaunchKernelGGL(k_field_phase, grid2((long long)nxy * N, 1), dim3(256), 0, st, hu, hv, mode, nxy,
-                     pt, xy, w.E);
+  hipLaunchKernelGGL(k_field_phase, grid2((long long)nxy * N, P), dim3(256), 0, st, hu, hv, mode, nxy, pts,
+                     xy, w.E);
   if ((e = hipGetLastError())) return e;
-  if ((e = wb_zgemm_launch(nxy, 6 * nz, N, one, bp_strided(w.E, 0), N, bp_strided(w.C, 0), 6 * nz,
-                           zero, bp_strided(w.F, 0), 6 * nz, 1, st))) return e;
-  hipLaunchKernelGGL(k_field_permute, grid2((long long)nz * nxy * 6, 1), dim3(256), 0, st, nz, nxy,
+  const int cols = 6 * L * nz;
+  if ((e = wb_zgemm_launch(nxy, cols, N, one, bp_strided(w.E, (long long)nxy * N), N,
+                           bp_strided(w.C, (long long)N * cols), cols, zero,
+                           bp_strided(w.F, (long long)nxy * cols), cols, P, st))) return e;
+  hipLaunchKernelGGL(k_field_permute, grid2((long long)L * nz * nxy * 6, P), dim3(256), 0, st, L * nz, nxy,
                      w.F, out);
   return hipGetLastError();
 }

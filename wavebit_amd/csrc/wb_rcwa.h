@@ -125,12 +125,15 @@ wb_field_ws wb_field_carve(void* ws, int P, int L, int n, int nrhs);
 hipError_t wb_field_amps_stage(const wb_solve_args& a, int N, int n, int nrhs, const cd* G0,
                                const cd* M6, const cd* A, const cd* B, const cd* q,
                                const wb_field_ws& F, hipStream_t st);
-// Fields of one layer of one point at nz local heights z (device, within [0, d]) and nxy
-// centre-origin points xy (device, x y pairs): out (nz x nxy x 6) = scale[c] x component c.
-// W, V (n x n, unrotated), q (n) and amps (2n: down-going, then up-going) of the layer's pair,
-// epz (N x N) its inverse-rule matrix (unused on the TE path); scale a host array of 6.
-size_t wb_field_sample_bytes(int hu, int hv, int mode, int nz, int nxy);
-hipError_t wb_field_sample_launch(int hu, int hv, int mode, const double* pt, double d, const cd* W,
-                                  const cd* V, const cd* q, const cd* epz, const cd* amps, int nz,
-                                  const double* z, int nxy, const double* xy, const cd* scale,
-                                  cd* out, void* ws, hipStream_t st);
+// Fields of P points x L layers at nz local heights per layer (z: L x nz, device, within each
+// layer's [0, d]) and nxy centre-origin points xy (device, x y pairs): out (P x L nz x nxy x 6)
+// = scale[c] x component c. Pair b = p L + l takes its modes at modes_idx[b] (or b) of W, V
+// (n x n, unrotated) and q (n), its amplitudes at b of amps (2n: down-going, then up-going), its
+// inverse-rule matrix at conv_idx[b] (or b) of epz (N x N, unused on the TE path) and its
+// thickness at thick[b]; scale is a host array of 6.
+size_t wb_field_sample_bytes(int P, int L, int hu, int hv, int mode, int nz, int nxy);
+hipError_t wb_field_sample_launch(int P, int L, int hu, int hv, int mode, const double* pts,
+                                  const double* thick, const cd* W, const cd* V, const cd* q,
+                                  const int* modes_idx, const cd* epz, const int* conv_idx,
+                                  const cd* amps, int nz, const double* z, int nxy, const double* xy,
+                                  const cd* scale, cd* out, void* ws, hipStream_t st);

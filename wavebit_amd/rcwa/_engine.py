@@ -309,7 +309,7 @@ def run_fields(bucket, conv, device):
     clusters rebuilt, ``window_modes``) and mode amplitudes for the point's own incidence
     (``wb_field_state``). Returns device tensors (eff, amp, pe, flags) and the state: pts, W, V,
     q, amps (P L, 2, n), the (P, L) pair table, the concatenated epz_i and the (P, L) index of
-    each layer's conv matrices in it."""
+    each layer's conv matrices in it (device int32), and the (P, L) thicknesses."""
     mode = _MODE[bucket.path]
     P, L = len(bucket.points), bucket.n_layers
     W, V, q, mflags, tables = window_modes([bucket], [conv], device)
@@ -317,12 +317,14 @@ def run_fields(bucket, conv, device):
     rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
     thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
     pts = torch.from_numpy(rec).to(device)
-    amps, eff, amp, pe, flags = _native.field_state(pts, torch.from_numpy(thick).to(device), W, V, q,
-                                                    tab, bucket.hu, bucket.hv, mode)
+    thick_d = torch.from_numpy(thick).to(device)
+    amps, eff, amp, pe, flags = _native.field_state(pts, thick_d, W, V, q, tab, bucket.hu, bucket.hv,
+                                                    mode)
     flags = flags | mflags[tab.reshape(-1).long()].reshape(P, L)
     epz = torch.cat([c[2] for c in conv]).contiguous()
-    conv_idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int64)
-    state = dict(pts=pts, W=W, V=V, q=q, amps=amps, table=tables[0], epz=epz, conv_idx=conv_idx)
+    conv_idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
+    state = dict(pts=pts, thick=thick_d, W=W, V=V, q=q, amps=amps, table=tab,
+                 epz=epz, conv_idx=torch.from_numpy(conv_idx).to(device))
     return eff, amp, pe, flags, state
 
 

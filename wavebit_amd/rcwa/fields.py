@@ -2,10 +2,11 @@
 ``src/wavebit/rcwa/results.py:149-227``, ``rcwa/_engine/field_result.py``, ``rcwa/_engine/fields.py``).
 
 A run with ``Outputs(fields=True)`` keeps, on the device, every layer's modes (``W``, ``V``,
-``q`` from ``wb_layer_modes``) and mode amplitudes (``wb_field_state``); sampling launches
-``wb_field_sample`` per (wavelength, layer) over the heights that fall in the layer, and the
-values come back as host arrays of shape ``(wavelength, z, y, x, component)``, ``E`` in V/m per
-unit incident amplitude times the source's ``amplitude``, ``H`` in A/m.
+``q`` from ``wb_layer_modes``) and mode amplitudes (``wb_field_state``); sampling is one batched
+``wb_field_sample`` launch sequence over every (wavelength, layer) pair (chunked to bound its
+workspace), each layer taking the heights that fall in it, and the values come back as host
+arrays of shape ``(wavelength, z, y, x, component)``, ``E`` in V/m per unit incident amplitude
+times the source's ``amplitude``, ``H`` in A/m.
 
 Sampling rules (the reference's, ``field_result.py:85-114``, ``:117-160``, ``:330-363``):
 ``x`` and ``y`` are measured from the cell centre; an integer count spans the cell
@@ -28,7 +29,7 @@ COMPONENTS = ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
 RECONSTRUCTIONS = ('plain', 'lalanne', 'split', 'splitw')
 _FACE_ULPS_PER_LAYER = 4          # field_result.py:122
 _MIRROR = (1.0, 1.0, -1.0, -1.0, -1.0, 1.0)   # z -> -z: E polar, H axial
-_CHUNK_BYTES = 512 << 20          # largest expansion operand of one sampling launch
+_WORKSPACE_BYTES = 2 << 30        # largest workspace of one sampling launch sequence
 
 
 @dataclass(frozen=True)
@@ -122,21 +123,21 @@ def heights(thickness, z):
 class FieldRecord:
     """The device state a run's fields are sampled from: per wavelength (point ``p``) and layer
     (kernel index ``l``, bottom to top of the solved stack) the modes at pair ``table[p, l]`` of
-    ``W``, ``V``, ``q``, the amplitudes at ``amps[p L + l]`` and the inverse-rule matrix
-    ``epz[conv_idx[p, l]]``. ``mirrored``: the stack was solved upside down (bottom incidence),
-    so public layer ``j`` is kernel layer ``L - 1 - j`` and the components take the mirror
-    parity."""
+    ``W``, ``V``, ``q``, the amplitudes at ``amps[p L + l]``, the inverse-rule matrix
+    ``epz[conv_idx[p, l]]`` and the thickness ``thick[p, l]``. ``mirrored``: the stack was solved
+    upside down (bottom incidence), so public layer ``j`` is kernel layer ``L - 1 - j``, heights
+    run down from its top face and the components take the mirror parity."""
 
-    def __init__(self, *, device, hu, hv, mode, pts, W, V, q, amps, table, epz, conv_idx,
+    def __init__(self, *, device, hu, hv, mode, pts, thick, W, V, q, amps, table, epz, conv_idx,
                  thickness, cell, mirrored, amplitude, bare=False):
         self.device, self.hu, self.hv, self.mode = device, hu, hv, mode
-        self.pts, self.W, self.V, self.q, self.amps = pts, W, V, q, amps
-        self.table, self.epz, self.conv_idx = np.asarray(table), epz, np.asarray(conv_idx)
+        self.pts, self.thick, self.W, self.V, self.q, self.amps = pts, thick, W, V, q, amps
+        self.table, self.epz, self.conv_idx = table, epz, conv_idx
         self.thickness = np.asarray(thickness, dtype=np.float64)       # public order
         self.cell, self.mirrored, self.amplitude, self.bare = cell, mirrored, complex(amplitude), bare
 
     def sample(self, x, y, z):
-        """``(values (wavelength, z, y, x, 6), coordinates)`` on the device."""
+        """``(values (wavelength, z, y, x, 6), coordinates)``, values on the device."""
         import torch
         from .. import _native
         X, Y, x_rep, y_rep = xy_grid(self.cell, x, y)
@@ -146,32 +147,51 @@ class FieldRecord:
         nxy = nx * ny
         P = self.pts.shape[0]
         L = self.thickness.shape[0]
-        N = (2 * self.hu + 1) * (2 * self.hv + 1)
-        values = torch.empty((P, zz.shape[0], nxy, 6), dtype=torch.complex128, device=self.device)
-        par = _MIRROR if self.mirrored else (1.0,) * 6
-        scale = [self.amplitude * par[c] * (1.0 if c < 3 else 1.0 / Z0) for c in range(6)]
-        # samples per launch: the expansion operands (nxy x N and nxy x 6 nz) stay under a bound
-        nz_max = max(1, _CHUNK_BYTES // (16 * 6 * nxy))
-        for j in range(L):
+        # each kernel layer's samples, padded to a common count: (public index, local height)
+        per = []
+        for lk in range(L):
+            j = L - 1 - lk if self.mirrored else lk
             sel = np.flatnonzero(owner == j)
-            if not sel.size:
-                continue
-            lk = L - 1 - j if self.mirrored else j
             d = float(self.thickness[j])
             local = np.clip(zz[sel] - edges[j], 0.0, d)
-            if self.mirrored:
-                local = d - local
-            for c0 in range(0, sel.size, nz_max):
-                part = sel[c0:c0 + nz_max]
-                zl = torch.from_numpy(local[c0:c0 + nz_max].copy()).to(self.device)
-                dst = torch.from_numpy(part).to(self.device)
-                for p in range(P):
-                    b = int(self.table[p, lk])
-                    out = _native.field_sample(
-                        self.pts[p], d, self.W[b], self.V[b], self.q[b],
-                        None if self.mode == _native.MODE_1D_TE else self.epz[int(self.conv_idx[p, lk])],
-                        self.amps[p * L + lk], zl, xy, self.hu, self.hv, self.mode, scale)
-                    values[p].index_copy_(0, dst, out)
+            per.append((sel, d - local if self.mirrored else local))
+        nz = max(1, max(sel.size for sel, _ in per))
+        par = _MIRROR if self.mirrored else (1.0,) * 6
+        scale = [self.amplitude * par[c] * (1.0 if c < 3 else 1.0 / Z0) for c in range(6)]
+        # chunks of heights and of points that keep the workspace under its bound
+        nzc = nz
+        while nzc > 1 and _native.field_sample_bytes(1, L, self.hu, self.hv, self.mode, nzc, nxy) > _WORKSPACE_BYTES:
+            nzc = (nzc + 1) // 2
+        pc = P
+        while pc > 1 and _native.field_sample_bytes(pc, L, self.hu, self.hv, self.mode, nzc, nxy) > _WORKSPACE_BYTES:
+            pc = (pc + 1) // 2
+        values = torch.empty((P, zz.shape[0], nxy, 6), dtype=torch.complex128, device=self.device)
+        epz = None if self.mode == _native.MODE_1D_TE else self.epz
+        workspace = None
+        for z0 in range(0, nz, nzc):
+            zl = np.zeros((L, nzc))
+            src, dst = [], []
+            for lk, (sel, local) in enumerate(per):
+                part = sel[z0:z0 + nzc]
+                zl[lk, :part.size] = local[z0:z0 + nzc]
+                src.extend(lk * nzc + np.arange(part.size))
+                dst.extend(part)
+            if not src:
+                continue
+            zl_d = torch.from_numpy(zl).to(self.device)
+            src_d = torch.as_tensor(np.asarray(src, dtype=np.int64), device=self.device)
+            dst_d = torch.as_tensor(np.asarray(dst, dtype=np.int64), device=self.device)
+            for p0 in range(0, P, pc):
+                p1 = min(P, p0 + pc)
+                if workspace is None:
+                    workspace = torch.empty(_native.field_sample_bytes(p1 - p0, L, self.hu, self.hv,
+                                                                       self.mode, nzc, nxy),
+                                            dtype=torch.uint8, device=self.device)
+                out = _native.field_sample(self.pts[p0:p1], self.thick[p0:p1], self.W, self.V, self.q,
+                                           self.table[p0:p1], epz, self.conv_idx[p0:p1],
+                                           self.amps[p0 * L:p1 * L], zl_d, xy, self.hu, self.hv,
+                                           self.mode, scale, workspace=workspace)
+                values[p0:p1].index_copy_(1, dst_d, out.index_select(1, src_d))
         values = values.reshape(P, zz.shape[0], ny, nx, 6)
         coords = {'x': x_rep, 'y': y_rep, 'z': zz if is_count(z) else z}
         return values, coords
