@@ -19,6 +19,17 @@
 // deferred updates. Shifts are the eigenvalues of the trailing M x M block (one wave's QR
 // iteration, wave_qr); active blocks of at most nmin rows finish with a single-shift iteration
 // held in LDS by one wave (small_block).
+//
+// Before every sweep, aggressive early deflation (Braman, Byers & Mathias, SIAM J. Matrix Anal.
+// Appl. 23 (2002) 948-973; LAPACK zlaqr3) takes the trailing window of the active block to Schur
+// form, deflates each eigenvalue whose spike entry s conj(U[0][j]) is negligible, moves the
+// others to the top of the window by swaps of the triangular form (ztrexc's rotation), and
+// returns the window to Hessenberg form (a reflector that folds the spike into one entry, then
+// Householder reduction of the undeflated part); the undeflated eigenvalues are the next
+// sweep's shifts. The window's own QR runs on one wave, so its cost grows as its cube: on the
+// (9,9) layer matrices (n = 722, 256 per launch) windows of 16-24 rows take the iteration from
+// 371 to 271-278 ms per launch, 32 rows to 301, 48 to 358 and 64 to 464 (a CPU model of the
+// sweep counts: 93 sweeps without the window, 68 at 24 rows, 46 at 48).
 #include "wb_internal.h"
 #include "wb_timing.h"
 #include <float.h>
@@ -41,6 +52,11 @@ namespace {
 #ifndef WB_HQR_BC
 #define WB_HQR_BC 2        // rotations per register chunk of the band updates
 #endif
+#ifndef WB_HQR_AED
+#define WB_HQR_AED 20      // aggressive-early-deflation window (<= 64 rows; 0 turns it off)
+#endif
+static_assert(WB_HQR_AED == 0 || (WB_HQR_AED <= 64 && WB_HQR_AED >= WB_HQR_M),
+              "the deflation window fits one wave and holds a sweep's shifts");
 constexpr int NT = WB_HQR_NT;
 static_assert(NT % 64 == 0, "whole waves");
 
@@ -343,6 +359,166 @@ __device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* w
   if (zh) uh_times_cols(Z + (long long)lo * n, n, m, n, U, ld);        // Z^H rows lo..hi
   else rows_times_u(Z + lo, n, n, m, U, ld);                          // Z columns lo..hi
   return info;
+}
+
+// --------------------------------------------------------------------- aggressive early deflation
+// The routines below run on one wave over a window of m <= 64 rows in LDS: T (row stride ld) and
+// U its accumulated unitary. Lane j owns column j (or row j) of a step; LDS accesses of one wave
+// stay in program order, so no barriers are needed.
+
+// Swap the diagonal entries k, k+1 of the upper-triangular T (ztrexc's step): the rotation with
+// Q [T(k,k+1); T(k+1,k+1) - T(k,k)] = [r; 0] applied to rows k, k+1 right of the pair, columns
+// k, k+1 above it and the columns of U.
+__device__ void wave_swap(cd* T, cd* U, int m, int ld, int k) {
+  const int lane = threadIdx.x & 63;
+  const cd t11 = T[k * ld + k], t22 = T[(k + 1) * ld + k + 1];
+  cd rr;
+  const Rot r = givens(T[k * ld + k + 1], csub(t22, t11), &rr);
+  if (lane >= k + 2 && lane < m) rot_left(r, T[k * ld + lane], T[(k + 1) * ld + lane]);
+  if (lane < k) rot_right(r, T[lane * ld + k], T[lane * ld + k + 1]);
+  if (lane < m) rot_right(r, U[lane * ld + k], U[lane * ld + k + 1]);
+  if (lane == 0) {
+    T[k * ld + k] = t22;
+    T[(k + 1) * ld + k + 1] = t11;
+  }
+}
+
+// Householder reflector (zlarfg) of x = (alpha, x[1..len-1]): (I - tau v v^H)^H x = beta e1 with
+// v[0] = 1; writes v (length len) to vout and returns tau (0 when x is already beta e1).
+__device__ cd wave_reflector(cd* vout, const cd* x, int stride, int len, double* beta_out) {
+  const int lane = threadIdx.x & 63;
+  cd xi = (lane >= 1 && lane < len) ? x[lane * stride] : mkc(0.0, 0.0);
+  double ss = cnorm2(xi);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) ss += __shfl_xor(ss, off);
+  const cd alpha = x[0];
+  if (ss == 0.0 && alpha.y == 0.0) {
+    if (lane < len) vout[lane] = (lane == 0) ? mkc(1.0, 0.0) : mkc(0.0, 0.0);
+    *beta_out = alpha.x;
+    return mkc(0.0, 0.0);
+  }
+  const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + ss), alpha.x);
+  const cd tau = mkc((beta - alpha.x) / beta, -alpha.y / beta);
+  const cd scale = crecip(mkc(alpha.x - beta, alpha.y));
+  if (lane < len) vout[lane] = (lane == 0) ? mkc(1.0, 0.0) : cmul(xi, scale);
+  *beta_out = beta;
+  return tau;
+}
+
+// rows r0..r0+len-1 of T, columns c0..m-1: T <- (I - conj(tau) v v^H) T (lane per column)
+__device__ void wave_refl_left(cd* T, int ld, int m, int r0, int len, int c0, const cd* v, cd tau) {
+  const int lane = threadIdx.x & 63;
+  if (lane >= c0 && lane < m) {
+    cd w = mkc(0.0, 0.0);
+    for (int j = 0; j < len; ++j) w = cadd(w, cmulc(v[j], T[(r0 + j) * ld + lane]));
+    const cd f = cmul(cconj(tau), w);
+    for (int j = 0; j < len; ++j) T[(r0 + j) * ld + lane] = csub(T[(r0 + j) * ld + lane], cmul(v[j], f));
+  }
+}
+
+// rows 0..rows-1 of A, columns c0..c0+len-1: A <- A (I - tau v v^H) (lane per row)
+__device__ void wave_refl_right(cd* A, int ld, int rows, int c0, int len, const cd* v, cd tau) {
+  const int lane = threadIdx.x & 63;
+  if (lane < rows) {
+    cd w = mkc(0.0, 0.0);
+    for (int j = 0; j < len; ++j) w = cadd(w, cmul(A[lane * ld + c0 + j], v[j]));
+    const cd f = cmul(tau, w);
+    for (int j = 0; j < len; ++j) A[lane * ld + c0 + j] = csub(A[lane * ld + c0 + j], cmul(f, cconj(v[j])));
+  }
+}
+
+// The deflation of one window (one wave): T, U hold the window's Schur form and unitary, s its
+// coupling H(kw, kw-1). Deflates from the bottom every eigenvalue with |s| |U(0, j)| <=
+// max(smlnum, ulp |T(j, j)|) and moves the others to the top; returns ns, the undeflated count,
+// with their eigenvalues copied to eig[0..ns-1]. When some deflated (ns < m) and s != 0, the
+// undeflated part is returned to Hessenberg form: a reflector folds the spike into entry 0, then
+// Householder reduction of T(0:ns, 0:ns), every transformation also applied to U.
+__device__ int wave_aed(cd* T, cd* U, int m, int ld, cd s, double smlnum, cd* eig, cd* v) {
+  const int lane = threadIdx.x & 63;
+  int ns = m, ilst = 0;
+  const double as = cabs1(s);
+  while (ilst < ns) {
+    double foo = cabs1(T[(ns - 1) * ld + ns - 1]);
+    if (foo == 0.0) foo = as;
+    if (as * cabs1(U[ns - 1]) <= fmax(smlnum, ULP * foo)) {
+      --ns;                                   // deflatable: stays at the bottom
+    } else {
+      for (int k = ns - 2; k >= ilst; --k) wave_swap(T, U, m, ld, k);
+      ++ilst;                                 // undeflatable: moved to the top
+    }
+  }
+  if (lane < ns) eig[lane] = T[lane * ld + lane];
+  if (ns == m || as == 0.0 || ns <= 1) return ns;
+  // fold the spike s conj(U[0][0..ns-1]) into its first entry
+  double beta;
+  {
+    const cd x = (lane < ns) ? cconj(U[lane]) : mkc(0.0, 0.0);
+    if (lane < ns) v[64 + lane] = x;           // scratch copy of the spike direction
+  }
+  const cd tau0 = wave_reflector(v, v + 64, 1, ns, &beta);
+  wave_refl_left(T, ld, m, 0, ns, 0, v, tau0);
+  wave_refl_right(T, ld, ns, 0, ns, v, tau0);
+  wave_refl_right(U, ld, m, 0, ns, v, tau0);
+  // Hessenberg reduction of T(0:ns, 0:ns) (columns ns.. see the left reflectors only)
+  for (int i = 0; i + 2 < ns; ++i) {
+    const int len = ns - i - 1;
+    const cd tau = wave_reflector(v, T + (i + 1) * ld + i, ld, len, &beta);
+    wave_refl_left(T, ld, m, i + 1, len, i + 1, v, tau);
+    if (lane == 0) T[(i + 1) * ld + i] = mkc(beta, 0.0);
+    if (lane >= i + 2 && lane < ns) T[lane * ld + i] = mkc(0.0, 0.0);
+    wave_refl_right(T, ld, ns, i + 1, len, v, tau);
+    wave_refl_right(U, ld, m, i + 1, len, v, tau);
+  }
+  return ns;
+}
+
+// Aggressive early deflation on the trailing jw rows of the active block [lo, hi] (all threads;
+// the window in the dynamic LDS). Returns the number of eigenvalues deflated (their rows end
+// upper triangular, split off by zero subdiagonals), or -1 when the window's QR did not converge
+// (H untouched); the undeflated eigenvalues (ns of them) are written to eig (LDS) and *ns_out.
+__device__ __attribute__((noinline)) int aed_window(cd* H, cd* Z, int zh, int n, int lo, int hi, int jw,
+                                                    cd* lds, cd* eig, int* s_int) {
+  ASSUME_LDS(lds);
+  ASSUME_LDS(eig);
+  ASSUME_LDS(s_int);
+  const int t = threadIdx.x;
+  const int kw = hi - jw + 1;
+  const int ld = (jw % 2) ? jw : jw + 1;
+  cd* T = lds;
+  cd* U = lds + jw * ld;
+  cd* v = U + jw * ld;                          // 128 entries of reflector scratch
+  cd* wtmp = v + 128;                           // the window's eigenvalues from its QR
+  for (int e = t; e < jw * jw; e += NT) {
+    const int r = e / jw, c = e % jw;
+    T[r * ld + c] = (r > c + 1) ? mkc(0.0, 0.0) : H[(long long)(kw + r) * n + kw + c];
+    U[r * ld + c] = mkc(r == c ? 1.0 : 0.0, 0.0);
+  }
+  const cd s = (kw > lo) ? H[(long long)kw * n + kw - 1] : mkc(0.0, 0.0);
+  __syncthreads();
+  if (t < 64) {
+    const double smlnum = SAFMIN * ((double)n / ULP);
+    const int bad = wave_qr(T, U, jw, ld, n, wtmp);
+    int ns = -1;
+    if (!bad) ns = wave_aed(T, U, jw, ld, s, smlnum, eig, v);
+    if (t == 0) s_int[0] = ns;
+  }
+  __syncthreads();
+  const int ns = s_int[0];
+  if (ns < 0) return -1;
+  const int nd = jw - ns;
+  if (nd == 0) return 0;                        // nothing deflated: H is left as it was
+  // the window back into H: T (Hessenberg over the undeflated rows, triangular below), the spike
+  for (int e = t; e < jw * jw; e += NT) {
+    const int r = e / jw, c = e % jw;
+    H[(long long)(kw + r) * n + kw + c] = (r > c + 1) ? mkc(0.0, 0.0) : T[r * ld + c];
+  }
+  if (t == 0 && kw > lo) H[(long long)kw * n + kw - 1] = cmul(s, cconj(U[0]));
+  __syncthreads();
+  rows_times_u(H + kw, n, kw, jw, U, ld);                                  // rows above
+  uh_times_cols(H + (long long)kw * n + hi + 1, n, jw, n - hi - 1, U, ld);  // right of it
+  if (zh) uh_times_cols(Z + (long long)kw * n, n, jw, n, U, ld);
+  else rows_times_u(Z + kw, n, n, jw, U, ld);
+  return nd;
 }
 
 // Compiler fence on values: the asm "reads and writes" x, so everything computing x is
@@ -924,15 +1100,35 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       __syncthreads();
       continue;
     }
+    int hs = hi;                 // last row of the sweep
+    bool have = false;           // shifts from the deflation window
+#if WB_HQR_AED
+    {
+      const int jw = (WB_HQR_AED < hi - lo + 1) ? WB_HQR_AED : hi - lo + 1;
+      const int nd = aed_window(H, Z, zh, n, lo, hi, jw, Hw, s_small, s_k);
+      if (nd > 0) {
+        stall = 0;
+        hs = hi - nd;
+        // a large deflation (LAPACK's nibble, 14% of the block) or a block left small: split
+        // off what deflated and look again before sweeping
+        if (100 * nd > 14 * (hi - lo + 1) || hs - lo + 1 <= nmin) continue;
+      }
+      if (nd >= 0 && jw - nd >= M) {
+        if (t < M) s_sh[t] = s_small[jw - nd - M + t];
+        have = true;
+      }
+    }
+#endif
     if (++sweeps > max_sweeps) {
       if (t == 0) info[b] = hi + 1;
       return;
     }
-    // shifts: eigenvalues of the trailing M x M block (one wave; exceptional shifts when
-    // stalled or when that small iteration fails)
+    // shifts: the deflation window's undeflated eigenvalues, else those of the trailing M x M
+    // block (one wave); exceptional shifts when stalled or when that small iteration fails
     ++stall;
-    {
-      const int b0 = hi - M + 1;
+    if (!have || stall % 6 == 0) {
+      const int b0 = hs - M + 1;
+      __syncthreads();
       for (int e = t; e < M * M; e += NT) {
         const int r = e / M, c = e % M;
         s_small[e] = (r > c + 1) ? mkc(0.0, 0.0) : H[(long long)(b0 + r) * n + b0 + c];
@@ -941,14 +1137,14 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       if (t < 64) {
         const int bad = wave_qr(s_small, nullptr, M, M, M, s_sh);
         if (t == 0 && (bad || stall % 6 == 0)) {
-          const double ex = 0.75 * cabs1(H[(long long)hi * n + hi - 1]);
+          const double ex = 0.75 * cabs1(H[(long long)hs * n + hs - 1]);
           for (int j = 0; j < M; ++j)
-            s_sh[j] = cadd(H[(long long)(hi - (j % 2)) * n + hi - (j % 2)], mkc(ex * (1.0 + 0.1 * j), 0.0));
+            s_sh[j] = cadd(H[(long long)(hs - (j % 2)) * n + hs - (j % 2)], mkc(ex * (1.0 + 0.1 * j), 0.0));
         }
       }
     }
     __syncthreads();
-    multishift_sweep<M>(H, Z, zh, n, lo, hi, s_sh, W, Hw, s_rot, rot_all);
+    multishift_sweep<M>(H, Z, zh, n, lo, hs, s_sh, W, Hw, s_rot, rot_all);
   }
 }
 
@@ -963,9 +1159,12 @@ static size_t hqr_lds_bytes(int W) {
   const size_t rots = sizeof(Rot) * (size_t)(W + 3 * M) * M;
   const int m = hqr_nmin(), ld = (m % 2) ? m : m + 1;
   const size_t small = 2 * sizeof(cd) * (size_t)m * ld;
+  constexpr int jw = WB_HQR_AED, ldj = (jw % 2) ? jw : jw + 1;
+  const size_t aed = sizeof(cd) * (2 * (size_t)jw * ldj + 192);   // T, U, reflector, eigenvalues
   size_t need = window;
   if (rots > need) need = rots;
   if (small > need) need = small;
+  if (aed > need) need = aed;
   return need;
 }
 
