@@ -316,6 +316,12 @@ def main():
                     'avg_launch_ms': round(k['total_ms'] / k['launches'], 4),
                     'flops_per_launch': flops_per_launch,
                     'share_of_step': round(k['total_ms'] / 1e3 / dt, 4)}
+    # every timed kernel's rate against the FP64 peak (algorithmic flops / HIP-event time)
+    kernel_rates = {nm: {'avg_launch_ms': round(k['total_ms'] / k['launches'], 4),
+                         'achieved_tflops': round(k['flops'] / (k['total_ms'] * 1e-3) / 1e12, 3),
+                         'frac': round(k['flops'] / (k['total_ms'] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                         'share_of_step': round(k['total_ms'] / 1e3 / dt, 4)}
+                    for nm, k in kernels.items() if k['flops'] > 0 and k['total_ms'] > 0}
 
     # CPU baseline (oracle, NumPy FP64) on a bounded sample; parity of the same points.
     # One untimed warm-up solve, then the median per-point time of the sample.
@@ -731,7 +737,8 @@ def main():
         'plain_solves_per_s': round(plain_rate, 4) if plain_rate else None,
         'batched_vs_plain': round(batched_rate / plain_rate, 3) if plain_rate else None,
         'algorithmic_tflops_per_s': round(fl_pt * batched_rate / 1e12, 4),
-        'roofline': roofline, 'cpu_baseline': cpu, 'stages': stages, 'measured_peaks': peaks,
+        'roofline': roofline, 'kernel_rates': kernel_rates, 'cpu_baseline': cpu, 'stages': stages,
+        'measured_peaks': peaks,
         'gradient': grad,
         'serve': serve_report,
         'thickness_sweep': sweep,
