@@ -57,13 +57,23 @@ typedef enum {
   WB_OK = 0,
   WB_BADARG = 1,      /* an argument is out of range or inconsistent */
   WB_OOM = 2,         /* the workspace given is smaller than the query returned */
-  WB_SINGULAR = 3,    /* reserved: singular factorizations are reported through flags/info */
+  WB_SINGULAR = 3,    /* a factorization was singular (wb_flags_status of a batch's flags) */
   WB_NONFINITE = 4,   /* reserved for callers that check outputs */
-  WB_NOCONVERGE = 5,  /* reserved: eigensolver non-convergence is reported through flags/info */
-  WB_FLAGGED = 6,     /* reserved */
+  WB_NOCONVERGE = 5,  /* an eigensolve did not converge (wb_flags_status of a batch's flags) */
+  WB_FLAGGED = 6,     /* a pair carries a bit of the caller's notice mask (wb_flags_status) */
   WB_HIP = 7,         /* a HIP runtime call failed; see wb_last_error */
   WB_UNSUPPORTED = 8  /* the combination asked for is not implemented (e.g. a 1D adjoint) */
 } wb_status;
+
+/* The status of a batch from the per-pair flags a solve wrote (host copy, count entries):
+ * WB_NOCONVERGE if any pair's eigensolve did not converge, else WB_SINGULAR if any factorization
+ * was singular, else WB_FLAGGED if any pair carries a bit of notice_mask (e.g. WB_FLAG_GRAZING |
+ * WB_FLAG_EP_LOUD), else WB_OK. The solve entries themselves return WB_OK whenever they ran:
+ * a batch holds independent points and the reference decides per point (its solves take one
+ * wavelength each), so numerical outcomes are reported per (point, layer) pair in flags and a
+ * failed point never fails its batch; this helper is the batch-level reading for C callers.
+ * Needs no context or device. */
+int wb_flags_status(const int* flags, long long count, int notice_mask);
 
 /* wb_solve mode */
 enum { WB_MODE_2D = 0, WB_MODE_1D_TE = 1, WB_MODE_1D_TM = 2 };

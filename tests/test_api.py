@@ -45,6 +45,16 @@ def test_workspace_queries_need_no_gpu():
     assert lib.wb_assemble_li_workspace_size(1, 8, 8, 3, 3) > 0
 
 
+def test_flags_status_reads_a_batch_on_the_host():
+    """wb_flags_status needs no device: non-convergence before singularity before notices."""
+    F = _native
+    assert F.flags_status([0, 0, 0]) == 'WB_OK'
+    assert F.flags_status([0, F.FLAG_GRAZING], notice_mask=F.FLAG_GRAZING) == 'WB_FLAGGED'
+    assert F.flags_status([0, F.FLAG_GRAZING]) == 'WB_OK'
+    assert F.flags_status([F.FLAG_SINGULAR, F.FLAG_EP_LOUD], notice_mask=F.FLAG_EP_LOUD) == 'WB_SINGULAR'
+    assert F.flags_status([[F.FLAG_SINGULAR], [F.FLAG_NOCONVERGE]]) == 'WB_NOCONVERGE'
+
+
 def test_check_reports_estimates_without_solving():
     rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(3, 3)), backend='torch', device='cuda')
     assert rep.ok

@@ -37,7 +37,7 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_gemm_tuning_clear', 'wb_solve_noalloc', 'wb_solve_rebuild_workspace_size',
            'wb_solve_rebuild', 'wb_layer_modes_workspace_size', 'wb_layer_modes',
            'wb_solve_from_modes', 'wb_field_state_workspace_size', 'wb_field_state',
-           'wb_field_sample_workspace_size', 'wb_field_sample')
+           'wb_field_sample_workspace_size', 'wb_field_sample', 'wb_flags_status')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -100,6 +100,7 @@ def load_library():
             'wb_field_sample': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_dp,
                                         c_vp, c_vp, c_sz, c_vp]),
+            'wb_flags_status': (c_int, [c_vp, c_ll, c_int]),
             'wb_solve_noalloc': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
             'wb_zgemm_batched_ops': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
@@ -503,6 +504,18 @@ def field_sample(pts, thick, W, V, q, modes_idx, epz_i, conv_idx, amps, z, xy, h
                                       _ptr(amps), nz, _ptr(z), nxy, _ptr(xy), sc, _ptr(out),
                                       _ptr(workspace), nbytes, _stream(device)), 'field_sample')
     return out
+
+
+STATUS_NAMES = {0: 'WB_OK', 1: 'WB_BADARG', 2: 'WB_OOM', 3: 'WB_SINGULAR', 4: 'WB_NONFINITE',
+                5: 'WB_NOCONVERGE', 6: 'WB_FLAGGED', 7: 'WB_HIP', 8: 'WB_UNSUPPORTED'}
+
+
+def flags_status(flags, notice_mask=0):
+    """``wb_flags_status`` of a host array of flags: the batch-level status name."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(flags, dtype=np.int32)).reshape(-1)
+    return STATUS_NAMES[int(load_library().wb_flags_status(a.ctypes.data_as(ctypes.c_void_p), a.size,
+                                                             int(notice_mask)))]
 
 
 SOLVE_NOALLOC_BYTES = 64 << 20   # WB_SOLVE_NOALLOC_BYTES

@@ -312,6 +312,16 @@ int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_from_modes");
 }
 
+int wb_flags_status(const int* flags, long long count, int notice_mask) {
+  if (!flags || count < 0) return WB_BADARG;
+  int all = 0;
+  for (long long i = 0; i < count; ++i) all |= flags[i];
+  if (all & WB_FLAG_NOCONVERGE) return WB_NOCONVERGE;
+  if (all & WB_FLAG_SINGULAR) return WB_SINGULAR;
+  if (all & notice_mask) return WB_FLAGGED;
+  return WB_OK;
+}
+
 size_t wb_field_state_workspace_size(int P, int L, int hu, int hv, int mode) {
   if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
   const int N = (2 * hu + 1) * (2 * hv + 1);
