@@ -24,7 +24,8 @@ H_UNITS = np.array([1, 1, 1, 1 / Z0, 1 / Z0, 1 / Z0])
 
 def _problem(eps, d, top, bot, period, wls, theta, phi, pol, side='top', units='um', amplitude=1.0,
              fields=True):
-    layers = [rcwa.Layer(thickness=t, epsilon=np.ascontiguousarray(e)) for t, e in zip(d, eps)]
+    layers = [rcwa.Layer(thickness=t, epsilon=np.ascontiguousarray(e) if isinstance(e, np.ndarray) else e)
+              for t, e in zip(d, eps)]
     return rcwa.Problem(
         stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(top),
                          bottom=wavebit_amd.Material(bot), period=period),
@@ -188,3 +189,19 @@ def test_field_requests_are_refused_by_name():
     with pytest.raises(UnsupportedFeatureError) as e:
         plain.field(x=3, y=3, z=3)
     assert e.value.code == 'outputs.unavailable'
+
+
+def test_the_server_and_differentiable_runs_refuse_field_outputs():
+    import torch
+    from wavebit_amd.rcwa import serve
+    eps = np.full((1, 4, 4), 4.0 + 0j)
+    concrete = _problem(eps, [0.2], 1.0, 2.25, (0.5, 0.5), [0.8], 0.1, 0.2, 's')
+    with serve.Server(streams=1) as srv:
+        results, _ = srv.serve([serve.Request(concrete, rcwa.Numerics(orders=(1, 1)), 0.0, 0)],
+                               realtime=False)
+    assert isinstance(results[0], UnsupportedFeatureError) and results[0].code == 'rcwa.unsupported'
+    live = torch.full((1, 4, 4), 4.0, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.run(_problem(live, [0.2], 1.0, 2.25, (0.5, 0.5), [0.8], 0.1, 0.2, 's'),
+                 numerics=rcwa.Numerics(orders=(1, 1)), **RUN)
+    assert e.value.code == 'rcwa.gradient' and e.value.issue.path == 'outputs.fields'

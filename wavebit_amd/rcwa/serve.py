@@ -148,6 +148,10 @@ class Server:
             raise UnsupportedFeatureError(
                 'The server returns diffraction outputs; solve a scattering-matrix request with '
                 'rcwa.run.', code='rcwa.unsupported', path='outputs.scattering')
+        if r.fields:
+            raise UnsupportedFeatureError(
+                'The server returns diffraction outputs; solve a field request with rcwa.run.',
+                code='rcwa.unsupported', path='outputs.fields')
         if req.numerics.detuning:
             raise UnsupportedFeatureError(
                 'The server solves the requested points; run a request with detuning through '

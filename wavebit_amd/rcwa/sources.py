@@ -1,8 +1,9 @@
 """Plane-wave source declaration (reference ``src/wavebit/rcwa/sources.py:7-44``).
 
 ``polarization`` is ``'s'``, ``'p'`` or a complex ``(s, p)`` Jones pair normalised to unit
-electric magnitude; ``theta``/``phi`` in radians. Direct ``kx``/``ky`` and bottom incidence are
-declared for parity and refused at admission by this engine.
+electric magnitude; ``theta``/``phi`` in radians, or direct ``kx``/``ky`` in radians per length
+unit; ``side='bottom'`` lights the stack from the bottom half-space; ``amplitude`` scales the
+sampled fields (V/m).
 """
 from dataclasses import dataclass
 
