@@ -358,3 +358,57 @@ def test_bottom_incidence_gradients_match_central_differences_of_the_oracle():
         tm[l] -= h
         fd = (total(eps, tp) - total(eps, tm)) / (2 * h)
         assert abs(g_t[l] - fd) <= 1e-6 * abs(fd) + 5e-9, (l, g_t[l], fd)
+
+
+@pytest.mark.parametrize('pol', ['s', 'p'])
+def test_pure_1d_gradients_match_central_differences_of_the_1d_oracle(pol):
+    """A one-row raster with a polarization word routes to the pure TE/TM path; its gradients
+    (through the coupled path at hv = 0) match central differences of the oracle's own 1D route,
+    the efficiencies equal that route's, and the amplitudes are the concrete 1D solve's."""
+    from wavebit_amd import rcwa
+    from oracle import rcwa_oracle as orc
+    import wavebit_amd
+    rng = np.random.default_rng(23 if pol == 's' else 29)
+    L, Nx, M = 2, 12, (6, 0)
+    eps = rng.uniform(1, 9, (L, 1, Nx))
+    thick = np.array([0.21, 0.14])
+    wls = [0.8, 0.93]
+    eps_t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    thick_t = torch.tensor(thick, dtype=torch.float64, requires_grad=True)
+
+    def problem(e, t):
+        return rcwa.Problem(
+            stack=rcwa.Stack(layers=[rcwa.Layer(thickness=t[i], epsilon=e[i]) for i in range(L)],
+                             top=wavebit_amd.Material(1.0), bottom=wavebit_amd.Material(2.25),
+                             period=(0.77, 0.77)),
+            excitation=rcwa.Excitation(wavelength=wls, sources=(rcwa.PlaneWave(polarization=pol, theta=0.25),)),
+            units=wavebit_amd.Units(length='um'))
+    res = rcwa.run(problem(eps_t, thick_t), numerics=rcwa.Numerics(orders=M), backend='torch', device=DEV)
+    assert res.diagnostics['solve_path'][0] == ('1d_te' if pol == 's' else '1d_tm')
+    loss = res.T.sum()
+    loss.backward()
+    g_eps, g_t = eps_t.grad.numpy(), thick_t.grad.numpy()
+
+    def total(e, t):
+        return sum(float(orc.solve_raster(e.astype(np.complex128), t, 1.0, 2.25, (0.77, 0.77), wl, 0.25, None,
+                                          pol, M)['incident']['T'].sum()) for wl in wls)
+
+    assert abs(float(loss.detach()) - total(eps, thick)) < 1e-10
+    concrete = rcwa.run(problem(eps, thick), numerics=rcwa.Numerics(orders=M), backend='torch', device=DEV)
+    assert np.abs(res.R.detach().cpu().numpy() - concrete.R).max() < 1e-12
+    np.testing.assert_array_equal(res.reflection.amplitude_ratio_s, concrete.reflection.amplitude_ratio_s)
+    np.testing.assert_array_equal(res.transmission.amplitude_ratio_p, concrete.transmission.amplitude_ratio_p)
+    h = 1e-6
+    for _ in range(5):
+        l, x = int(rng.integers(0, L)), int(rng.integers(0, Nx))
+        ep, em = eps.copy(), eps.copy()
+        ep[l, 0, x] += h
+        em[l, 0, x] -= h
+        fd = (total(ep, thick) - total(em, thick)) / (2 * h)
+        assert abs(g_eps[l, 0, x] - fd) <= 1e-6 * abs(fd) + 5e-9, (l, x, g_eps[l, 0, x], fd)
+    for l in range(L):
+        tp, tm = thick.copy(), thick.copy()
+        tp[l] += h
+        tm[l] -= h
+        fd = (total(eps, tp) - total(eps, tm)) / (2 * h)
+        assert abs(g_t[l] - fd) <= 1e-6 * abs(fd) + 5e-9, (l, g_t[l], fd)

@@ -497,10 +497,11 @@ def _run_live(prepared, problem, r, checks):
     import torch
     from dataclasses import replace
     from . import adjoint
-    if r.uniform or r.path != _engine.PATH_2D:
+    one_d = r.path in (_engine.PATH_TE, _engine.PATH_TM)
+    if r.uniform or not (r.path == _engine.PATH_2D or one_d):
         raise UnsupportedFeatureError(
-            'Gradients are implemented for raster stacks on the 2D coupled path; this problem '
-            f'routes to {r.path!r}.', code='rcwa.gradient', path='problem')
+            'Gradients are implemented for raster stacks; this problem routes to '
+            f'{r.path!r} with uniform layers.', code='rcwa.gradient', path='problem')
     if prepared.numerics.detuning:
         raise UnsupportedFeatureError(
             'A differentiable run solves the requested point; detuning is applied to concrete '
@@ -523,7 +524,13 @@ def _run_live(prepared, problem, r, checks):
                          else torch.tensor(float(l.thickness), dtype=torch.float64) for l in layers])
     if r.side == 'bottom':                   # the mirrored stack (see _resolve); flip is differentiable
         eps, thick = torch.flip(eps, [0]), torch.flip(thick, [0])
-    points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta, phi=r.source.phi,
+    # The pure TE/TM routes differentiate through the coupled path at hv = 0, phi = 0: a one-row
+    # (y-invariant) raster there decouples into the same TE (s) and TM (p) problems, the 2D Li
+    # rules reducing to the 1D ones, so its efficiencies and their gradients are the 1D route's
+    # (the reference differentiates that route by stock autograd, solve_1d.py:151-253). The
+    # amplitudes, whose frame signs differ between the routes, come from a concrete 1D solve.
+    points = [_engine.Point(stack=0, wavelength=float(wl), theta=r.source.theta,
+                            phi=0.0 if one_d else r.source.phi,
                             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                             period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
               for wl in r.wavelengths]
@@ -539,12 +546,16 @@ def _run_live(prepared, problem, r, checks):
         from .diagnostics import ExceptionalPointWarning
         warnings.warn(notice, ExceptionalPointWarning, stacklevel=3)
     eff_h, amp_h, pe_h = (t.detach().cpu().numpy() for t in (eff, amp, pe))
-    per_wl = [_engine.unpack(eff_h, amp_h, pe_h, p, r.hu, r.hv, r.path) for p in range(len(points))]
+    if one_d:
+        per_wl = _solve_resolved(r, prepared._cache)[0]
+    else:
+        per_wl = [_engine.unpack(eff_h, amp_h, pe_h, p, r.hu, r.hv, r.path) for p in range(len(points))]
     prov = {'solver': 'rcwa', 'engine': 'wavebit_amd/hip-gfx950 (differentiable)',
             'length_units': problem.units.length, 'execution': prepared.execution,
             'numerics': prepared.numerics}
     diag = {'report': prepared.report, 'solve_path': tuple(r.path for _ in r.wavelengths),
-            'gradient': 'hand-written adjoint (raster permittivity, layer thickness)'}
+            'gradient': 'hand-written adjoint (raster permittivity, layer thickness)'
+                        + (' through the coupled path at hv = 0' if one_d else '')}
     result = pack(per_wl, r.hu, r.hv, r.wavelengths, checks, diag, prov, side=r.side)
     if not result.validity.checks['numerical_finite']:
         raise NumericalError('The requested solve produced nonfinite observables.',
