@@ -215,6 +215,24 @@ int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
                         const int* modes_idx, double* eff, double* amp, double* pe, int* flags,
                         double* smat, void* workspace, size_t workspace_bytes, void* stream);
 
+/* wb_solve_from_modes over points whose layer counts differ, as one cascade: each point runs the
+ * reference's per-problem cascade over its own layers (rcwa/_engine/scattering.py:1185-1324
+ * _smm_sp, called once per problem), here batched so that cascade step l takes layer l of every
+ * point that has it. layers (host memory, P entries) are the points' layer counts, each in
+ * 1..64 and non-increasing (order the points so). Pairs are numbered layer-major: with P_l the
+ * number of points with more than l layers and base_l = P_0 + ... + P_{l-1}, layer l of point p
+ * is pair base_l + p. thick (B doubles, B = the sum of the layer counts), modes_idx (B ints, or
+ * NULL: index b) and flags (B ints, or NULL) follow that numbering; pts, eff, amp and pe are per
+ * point as in wb_solve. No scattering output. Results equal wb_solve_from_modes' for each point
+ * within rounding (the same operations per point in batches of other sizes). The workspace is
+ * wb_solve_ragged_workspace_size's. */
+size_t wb_solve_ragged_workspace_size(int P, const int* layers, int hu, int hv, int mode);
+int wb_solve_from_modes_ragged(wb_ctx* ctx, int P, const int* layers, int hu, int hv, int mode,
+                               const double* pts, const double* thick, const double* W,
+                               const double* V, const double* q, const int* modes_idx, double* eff,
+                               double* amp, double* pe, int* flags, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
 /* wb_solve (2D path, WB_MODE_2D) with every grazing layer's mode cluster re-derived from the
  * coupled first-order problem, as the reference always does (rcwa/_engine/solve_2d.py:118-228
  * _rebuild_grazing_cluster, entered at :2463-2485). Meant for the points whose wb_solve flags

@@ -35,16 +35,57 @@ def _equal(a, b, path):
     assert torch.equal(a[3], b[3])
 
 
+def _close(a, b, path):
+    """The ragged cascade runs each point's operations in batches of other sizes: equal within
+    rounding (1e-12 of each output's scale), flags equal."""
+    nv = 3 if path == eng.PATH_2D else 1
+    for x, y in zip(a[:3], b[:3]):
+        x, y = x[:, :nv], y[:, :nv]
+        scale = float(y.abs().max())
+        assert float((x - y).abs().max()) <= 1e-12 * max(scale, 1e-300), float((x - y).abs().max())
+    assert torch.equal(a[3], b[3])
+
+
+@pytest.mark.parametrize('ragged', [False, True])
 @pytest.mark.parametrize('path,hu,hv', [(eng.PATH_2D, 3, 2), (eng.PATH_TE, 8, 0), (eng.PATH_TM, 8, 0)])
-def test_window_of_mixed_layer_counts_equals_fused_solves(path, hu, hv):
+def test_window_of_mixed_layer_counts_equals_fused_solves(path, hu, hv, ragged):
+    """One modes batch over buckets of three layer counts; per-bucket cascades equal each bucket's
+    fused solve to the bit, the one ragged cascade within rounding."""
     rng = np.random.default_rng(hu * 10 + len(path))
     buckets = [_bucket(rng, L, P, hu, hv, path) for L, P in ((1, 3), (3, 2), (2, 4))]
-    out, modes = eng.run_window([b for b, _ in buckets], [c for _, c in buckets], DEV)
+    out, modes = eng.run_window([b for b, _ in buckets], [c for _, c in buckets], DEV,
+                                ragged=ragged)
     for (b, conv), got in zip(buckets, out):
         want = eng.run_bucket(b, conv, DEV)
-        _equal(got[:4], want[:4], path)
+        (_close if ragged else _equal)(got[:4], want[:4], path)
     W, V, q, flags, tables = modes
     assert W.shape[0] == sum(b.n_layers * len(b.points) for b, _ in buckets)
+
+
+def test_ragged_cascade_over_eight_layer_counts_with_shared_counts():
+    """Layer counts 1..8 (two buckets share a count, buckets given in no order): one ragged cascade
+    equals the per-bucket cascades of the same modes within rounding, and every point's R + T
+    is the same."""
+    rng = np.random.default_rng(21)
+    spec = [(3, 2), (8, 1), (1, 4), (5, 3), (3, 3), (2, 2), (7, 1), (4, 2), (6, 1)]
+    buckets = [_bucket(rng, L, P, 3, 3, eng.PATH_2D, n_raster=12) for L, P in spec]
+    bs, cs = [b for b, _ in buckets], [c for _, c in buckets]
+    got, modes = eng.run_window(bs, cs, DEV, ragged=True)
+    want, _ = eng.run_window(bs, cs, DEV, modes=modes, ragged=False)
+    for g, w in zip(got, want):
+        _close(g[:4], w[:4], eng.PATH_2D)
+
+
+def test_ragged_entry_refuses_unordered_layer_counts():
+    rng = np.random.default_rng(3)
+    b, conv = _bucket(rng, 2, 2, 2, 2, eng.PATH_2D)
+    _, modes = eng.run_window([b], [conv], DEV)
+    W, V, q = modes[:3]
+    pts = torch.zeros((2, 16), dtype=torch.float64, device=DEV)
+    thick = torch.full((3,), 0.1, dtype=torch.float64, device=DEV)
+    with pytest.raises(Exception) as e:
+        _native.solve_from_modes_ragged(pts, thick, W, V, q, None, [1, 2], 2, 2, eng._MODE[eng.PATH_2D])
+    assert 'non-increasing' in str(e.value)
 
 
 def test_resident_modes_serve_new_thicknesses_exactly():

@@ -9,6 +9,7 @@ import ctypes
 import threading
 from pathlib import Path
 
+import numpy as np
 import torch
 
 _LIB_PATH = Path(__file__).resolve().parent / 'libwavebit_hip.so'
@@ -21,6 +22,7 @@ MODE_2D, MODE_1D_TE, MODE_1D_TM = 0, 1, 2
 FLAG_NOCONVERGE, FLAG_SINGULAR, FLAG_EP, FLAG_GRAZING = 1, 2, 4, 8
 FLAG_REBUILT, FLAG_DEFECTIVE, FLAG_EP_LOUD = 16, 32, 64
 GRAZE_NMAX = 2048            # largest mode order wb_solve_rebuild takes
+RAGGED_MAX = 64              # largest layer count wb_solve_from_modes_ragged takes
 PT_STRIDE = 16
 
 EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb_zgemm_batched',
@@ -37,7 +39,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_gemm_tuning_clear', 'wb_solve_noalloc', 'wb_solve_rebuild_workspace_size',
            'wb_solve_rebuild', 'wb_layer_modes_workspace_size', 'wb_layer_modes',
            'wb_solve_from_modes', 'wb_field_state_workspace_size', 'wb_field_state',
-           'wb_field_sample_workspace_size', 'wb_field_sample', 'wb_flags_status')
+           'wb_field_sample_workspace_size', 'wb_field_sample', 'wb_flags_status',
+           'wb_solve_ragged_workspace_size', 'wb_solve_from_modes_ragged')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -92,6 +95,10 @@ def load_library():
             'wb_solve_from_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                             c_sz, c_vp]),
+            'wb_solve_ragged_workspace_size': (c_sz, [c_int, c_vp, c_int, c_int, c_int]),
+            'wb_solve_from_modes_ragged': (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_vp, c_vp,
+                                                   c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                   c_vp, c_sz, c_vp]),
             'wb_field_state_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_field_state': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                        c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
@@ -328,6 +335,13 @@ def solve_workspace_bytes(P, L, hu, hv, mode):
     return int(load_library().wb_solve_workspace_size(P, L, hu, hv, mode))
 
 
+def solve_ragged_workspace_bytes(layers, hu, hv, mode):
+    """Workspace of ``solve_from_modes_ragged`` for points with these layer counts (any order)."""
+    lay = np.ascontiguousarray(sorted(layers, reverse=True), dtype=np.int32)
+    return int(load_library().wb_solve_ragged_workspace_size(
+        len(lay), lay.ctypes.data_as(ctypes.c_void_p), hu, hv, mode))
+
+
 def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
     """Batch RCWA solve; see ``wb_solve``. Returns (eff, amp, pe, flags)."""
     device = epx.device
@@ -438,6 +452,61 @@ def solve_from_modes(pts, thick, W, V, q, modes_idx, hu, hv, mode, scattering=Fa
                                           _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
                                           _ptr(workspace), nbytes, _stream(device)), 'solve_from_modes')
     return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
+
+
+def solve_from_modes_ragged(pts, thick, W, V, q, modes_idx, layers, hu, hv, mode, workspace=None):
+    """``solve_from_modes`` over points whose layer counts differ, as one cascade; see
+    ``wb_solve_from_modes_ragged``. ``layers`` (host ints, non-increasing) are the points' layer
+    counts; ``thick`` (B,) float64 and ``modes_idx`` (B,) int32 or None are numbered layer-major
+    (layer l of point p at base_l + p). Returns (eff, amp, pe, flags), flags (B,) in the same
+    numbering and reporting this call's factorizations only."""
+    device = W.device
+    ctx = context(device)
+    lay = np.ascontiguousarray(layers, dtype=np.int32)
+    P = int(lay.shape[0])
+    B = int(lay.sum())
+    N = (2 * hu + 1) * (2 * hv + 1)
+    _require(pts, torch.float64, device, 'pts')
+    _require(thick, torch.float64, device, 'thick')
+    for name, t in (('W', W), ('V', V), ('q', q)):
+        _require(t, torch.complex128, device, name)
+    if modes_idx is not None:
+        _require(modes_idx, torch.int32, device, 'modes_idx')
+    if tuple(thick.shape) != (B,) or (modes_idx is not None and tuple(modes_idx.shape) != (B,)):
+        raise ValueError(f'thick and modes_idx: expected ({B},) in the layer-major numbering')
+    lay_p = lay.ctypes.data_as(ctypes.c_void_p)
+    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
+    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
+    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
+    flags = torch.empty((B,), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_solve_ragged_workspace_size(P, lay_p, hu, hv, mode)
+    if nbytes == 0:
+        raise ValueError('layers: expected non-increasing layer counts in 1..64')
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_solve_from_modes_ragged(ctx.handle, P, lay_p, hu, hv, mode, _ptr(pts),
+                                                 _ptr(thick), _ptr(W), _ptr(V), _ptr(q),
+                                                 _ptr(modes_idx), _ptr(eff), _ptr(amp), _ptr(pe),
+                                                 _ptr(flags), _ptr(workspace), nbytes,
+                                                 _stream(device)), 'solve_from_modes_ragged')
+    return eff, amp, pe, flags
+
+
+def ragged_order(layers):
+    """The layer-major numbering of ``wb_solve_from_modes_ragged`` for layer counts in any order:
+    (order, pair) with ``order`` the points sorted by non-increasing layer count (stable) and
+    ``pair[k]`` the (sorted point, layer) of pair k, as two int arrays."""
+    lay = np.asarray(layers, dtype=np.int64)
+    order = np.argsort(-lay, kind='stable')
+    sl = lay[order]
+    pts_, lays_ = [], []
+    for l in range(int(sl.max()) if sl.size else 0):
+        cnt = int(np.count_nonzero(sl > l))
+        pts_.append(np.arange(cnt))
+        lays_.append(np.full(cnt, l))
+    if not pts_:
+        return order, (np.zeros(0, np.int64), np.zeros(0, np.int64))
+    return order, (np.concatenate(pts_), np.concatenate(lays_))
 
 
 def field_state(pts, thick, W, V, q, modes_idx, hu, hv, mode, workspace=None):

@@ -1,5 +1,6 @@
 // C ABI of the wavebit_amd engine: contexts, argument checks, workspace carving and the
 // launches behind each entry point declared in include/wavebit_amd.h.
+#include <climits>
 #include <string>
 #include <cstring>
 
@@ -310,6 +311,59 @@ int wb_solve_from_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
   a.modes_in = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q; a.modes_idx = modes_idx;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_from_modes");
+}
+
+// Step counts of a ragged call: steps[l] = points with more than l layers. Returns the pair
+// count, or -1 if the layer counts are not a non-increasing sequence in [1, WB_RAGGED_MAX].
+static long long ragged_steps(int P, const int* layers, int* steps, int* Lmax) {
+  if (P <= 0 || !layers) return -1;
+  long long B = 0;
+  for (int p = 0; p < P; ++p) {
+    if (layers[p] < 1 || layers[p] > WB_RAGGED_MAX || (p > 0 && layers[p] > layers[p - 1])) return -1;
+    B += layers[p];
+  }
+  *Lmax = layers[0];
+  for (int l = 0; l < *Lmax; ++l) {
+    int c = 0;
+    while (c < P && layers[c] > l) ++c;
+    steps[l] = c;
+  }
+  return B;
+}
+
+size_t wb_solve_ragged_workspace_size(int P, const int* layers, int hu, int hv, int mode) {
+  int steps[WB_RAGGED_MAX], Lmax = 0;
+  const long long B = ragged_steps(P, layers, steps, &Lmax);
+  if (B < 0 || B > INT_MAX || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
+  return align_up(wb_solve_workspace_bytes_b(P, (int)B, hu, hv, mode)) + align_up((size_t)(B + P) * sizeof(int));
+}
+
+int wb_solve_from_modes_ragged(wb_ctx* ctx, int P, const int* layers, int hu, int hv, int mode,
+                               const double* pts, const double* thick, const double* W,
+                               const double* V, const double* q, const int* modes_idx, double* eff,
+                               double* amp, double* pe, int* flags, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  int steps[WB_RAGGED_MAX], Lmax = 0;
+  const long long B = ragged_steps(P, layers, steps, &Lmax);
+  if (B < 0 || B > INT_MAX || hu < 0 || hv < 0 || mode < 0 || mode > 2 || (mode != 0 && hv != 0))
+    return fail(ctx, WB_BADARG,
+                "wb_solve_from_modes_ragged: bad dimensions (layer counts must be non-increasing, "
+                "1..64)");
+  if (!pts || !thick || !W || !V || !q || !eff || !amp || !pe)
+    return fail(ctx, WB_BADARG, "wb_solve_from_modes_ragged: null buffer");
+  if (workspace_bytes < wb_solve_ragged_workspace_size(P, layers, hu, hv, mode))
+    return fail(ctx, WB_OOM, "wb_solve_from_modes_ragged: workspace too small");
+  wb_solve_args a;
+  a.P = P; a.L = Lmax; a.hu = hu; a.hv = hv; a.mode = mode;
+  a.pts = pts; a.thick = thick; a.conv_idx = nullptr;
+  a.epx = nullptr; a.epy = nullptr; a.epz = nullptr;
+  a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = nullptr;
+  a.modes_in = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q; a.modes_idx = modes_idx;
+  a.steps = steps;
+  a.ragged_tab = (int*)((char*)workspace + align_up(wb_solve_workspace_bytes_b(P, (int)B, hu, hv, mode)));
+  return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_from_modes_ragged");
 }
 
 int wb_flags_status(const int* flags, long long count, int notice_mask) {

@@ -89,6 +89,17 @@ struct wb_solve_args {
   // Not combined with smat.
   cd* field_amps = nullptr;
   void* field_ws = nullptr;
+  // Ragged solve (with modes_in only; no ck, smat, field_amps or graze): the points' layer counts
+  // differ. steps[l] (host memory, l < L) is the number of points with more than l layers: the
+  // points are ordered by non-increasing layer count, steps[0] = P and L is the largest count
+  // (at most WB_RAGGED_MAX). Pair (p, l) is numbered layer-major, base_l + p with base_l =
+  // steps[0] + ... + steps[l-1], so each cascade step is a prefix of the points; thick,
+  // modes_idx and flags follow that numbering. ragged_tab: B + P ints of device scratch.
+  const int* steps = nullptr;
+  int* ragged_tab = nullptr;
 };
+#define WB_RAGGED_MAX 64
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode);
+// The solve workspace for B pairs in all (a ragged solve).
+size_t wb_solve_workspace_bytes_b(int P, int B, int hu, int hv, int mode);
 hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st);

@@ -63,6 +63,8 @@ struct wb_solve_ws {
   void* luws;              // scratch of the blocked many-right-hand-side solves
 };
 wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode);
+// The same for B pairs in all (a ragged solve: B = the sum of the points' layer counts).
+wb_solve_ws wb_solve_carve_b(void* ws, int P, int B, int hu, int hv, int mode);
 
 // Checkpoint kept by a solve for its adjoint (2D path): per pair the unrotated eigenvectors W,
 // Wi = (R W)^-1, Vi = (R' V)^-1 and the layer S11; per point the transmitted columns d_l,

@@ -248,9 +248,10 @@ __global__ void k_modes_raw_2d(int N, int L, const cd* W, const cd* Vg, const cd
 
 // The split solve's input modes: pair b's raw W, V, q from index src = idx ? idx[b] : b of the
 // caller's arrays, rotated into the s/p basis as k_modes_post_2d rotates them (2D), or copied
-// (1D: big_W = W, big_V = V).
+// (1D: big_W = W, big_V = V). ppt (or NULL: pair b is point b / L) gives each pair's point.
 __global__ void k_modes_take(int N, int n, int L, int mode, const cd* Win, const cd* Vin,
-                             const cd* qin, const int* idx, const cd* ov, cd* bigW, cd* bigV, cd* q) {
+                             const cd* qin, const int* idx, const int* ppt, const cd* ov, cd* bigW,
+                             cd* bigV, cd* q) {
   const long long b = blockIdx.y;
   const long long src = idx ? (long long)idx[b] : b;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,7 +265,7 @@ __global__ void k_modes_take(int N, int n, int L, int mode, const cd* Win, const
   }
   if (e >= (long long)N * n) return;
   const int r = (int)(e / n), c = (int)(e % n);
-  const int p = (int)(b / L);
+  const int p = ppt ? ppt[b] : (int)(b / L);
   const cd* o = ov + (long long)p * OV_COUNT * N;
   const double fc = o[OV_FC * N + r].x, fs = o[OV_FS * N + r].x;
   if (r == 0) q[b * n + c] = qin[src * n + c];
@@ -278,10 +279,10 @@ __global__ void k_modes_take(int N, int n, int L, int mode, const cd* Win, const
 
 // A = Wi diag(wg) + Vi diag(vg), B = Wi diag(wg) - Vi diag(vg) for the gap of each pair.
 // 2D gap: wg = [1, -i y], vg = [-i y, 1]; 1D TE: wg = 1, vg = -i y; 1D TM: same with z_p = y.
-__global__ void k_ab(int N, int n, int L, int mode, const cd* Wi, const cd* Vi, const cd* ov,
-                     cd* A, cd* B) {
+__global__ void k_ab(int N, int n, int L, int mode, const int* ppt, const cd* Wi, const cd* Vi,
+                     const cd* ov, cd* A, cd* B) {
   const long long b = blockIdx.y;
-  const int p = (int)(b / L);
+  const int p = ppt ? ppt[b] : (int)(b / L);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long long)n * n) return;
   const int c = (int)(e % n);
@@ -303,11 +304,12 @@ __global__ void k_ab(int N, int n, int L, int mode, const cd* Wi, const cd* Vi, 
 
 // XYX[r][c] = X_r Y[r][c] X_c with X = exp(-k0 q d).
 // Layer propagation diagonal x_r = exp(-k0 d q_r) per (pair, mode).
-__global__ void k_xvec(int n, int L, const cd* q, const double* pts, const double* thick, cd* xv) {
+__global__ void k_xvec(int n, int L, const int* ppt, const cd* q, const double* pts,
+                       const double* thick, cd* xv) {
   const long long b = blockIdx.y;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  const int p = (int)(b / L);
+  const int p = ppt ? ppt[b] : (int)(b / L);
   const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
   xv[b * n + r] = cexp_(cscale(q[b * n + r], -k0 * thick[b]));
 }
@@ -642,6 +644,34 @@ __global__ void k_flag(int count, int spread, const int* info, int bit, int* fla
   if (info[b / spread] != 0) flags[b] |= bit;
 }
 
+// Ragged batches: flags[b] |= bit for the pairs b of the points p < count with info[p] != 0
+__global__ void k_flag_points(int B, int count, const int* ppt, const int* info, int bit, int* flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int p = ppt[b];
+  if (p < count && info[p] != 0) flags[b] |= bit;
+}
+
+// Ragged batches (points ordered by non-increasing layer count, pair (p, l) numbered base[l] + p):
+// each pair's point, and each point's last pair.
+struct RaggedBases {
+  int L;
+  int base[WB_RAGGED_MAX + 1];
+};
+__global__ void k_ragged_tables(int P, RaggedBases rb, int* ppt, int* last) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rb.base[rb.L]) {
+    int l = 0;
+    while (l + 1 < rb.L && rb.base[l + 1] <= i) ++l;
+    ppt[i] = i - rb.base[l];
+  }
+  if (i < P) {
+    int l = 0;   // the point's top layer: the last step with more than i points
+    while (l + 1 < rb.L && rb.base[l + 2] - rb.base[l + 1] > i) ++l;
+    last[i] = rb.base[l] + i;
+  }
+}
+
 __global__ void k_zero_flags(int count, int* flags) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < count) flags[b] = 0;
@@ -664,9 +694,13 @@ inline dim3 grid2(long long elems, int batch) {
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
+  return wb_solve_workspace_bytes_b(P, P * L, hu, hv, mode);
+}
+
+size_t wb_solve_workspace_bytes_b(int P, int Bp, int hu, int hv, int mode) {
   const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
   const size_t n = (mode == 0) ? 2 * N : N;
-  const size_t B = (size_t)P * L;
+  const size_t B = (size_t)Bp;
   const size_t nn = n * n * sizeof(cd);
   size_t s = 0;
   s += align_up(B * nn) * 6;          // M0..M5
@@ -685,9 +719,13 @@ size_t wb_solve_workspace_bytes(int P, int L, int hu, int hv, int mode) {
 }
 
 wb_solve_ws wb_solve_carve(void* ws, int P, int L, int hu, int hv, int mode) {
+  return wb_solve_carve_b(ws, P, P * L, hu, hv, mode);
+}
+
+wb_solve_ws wb_solve_carve_b(void* ws, int P, int Bp, int hu, int hv, int mode) {
   const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1);
   const size_t n = (mode == 0) ? 2 * N : N;
-  const size_t B = (size_t)P * L;
+  const size_t B = (size_t)Bp;
   const size_t nn = n * n;
   char* p = (char*)ws;
   auto take = [&](size_t bytes) { char* r = p; p += align_up(bytes); return (void*)r; };
@@ -736,10 +774,28 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   const int P = a.P, L = a.L, mode = a.mode;
   const int N = (2 * a.hu + 1) * (2 * a.hv + 1);
   const int n = (mode == 0) ? 2 * N : N;
-  const int B = P * L;
+  // a ragged solve: the pair count and each cascade step's first pair (layer-major numbering)
+  const bool ragged = a.steps != nullptr;
+  if (ragged && (!a.modes_in || a.ck || a.smat || a.field_amps || a.graze || !a.ragged_tab ||
+                 L > WB_RAGGED_MAX))
+    return hipErrorInvalidValue;
+  RaggedBases rb{};
+  rb.L = L;
+  int B = P * L;
+  if (ragged) {
+    B = 0;
+    for (int l = 0; l < L; ++l) { rb.base[l] = B; B += a.steps[l]; }
+    rb.base[L] = B;
+  }
   const long long nn = (long long)n * n;
   const long long NN = (long long)N * N;
-  const wb_solve_ws W_ = wb_solve_carve(ws, P, L, a.hu, a.hv, mode);
+  const wb_solve_ws W_ = wb_solve_carve_b(ws, P, B, a.hu, a.hv, mode);
+  int* ppt = ragged ? a.ragged_tab : nullptr;   // each pair's point
+  int* last = ragged ? a.ragged_tab + B : nullptr;   // each point's top-layer pair
+  if (ragged) {
+    hipLaunchKernelGGL(k_ragged_tables, dim3((B + 255) / 256), dim3(256), 0, st, P, rb, ppt, last);
+    WB_CHECK_LAUNCH();
+  }
   cd* M[6];
   for (int i = 0; i < 6; ++i) M[i] = W_.M[i];
   cd* M6 = W_.M6;
@@ -773,7 +829,8 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   // EIG(n) = 100 n^3 per layer-point; Omega^2 + V + A, B and S_layer = 84 n^3 per layer-point
   // and the cascade 96 (L+1) n^3 per point)
   const double n3 = (double)n * n * n;
-  WbStageTimer smat(WB_T_STAGE_SMAT, st, (84.0 * B + 96.0 * (L + 1) * (double)P) * n3, WB_STAGE_COARSE);
+  const double steps_all = ragged ? (double)B + P : (double)(L + 1) * P;   // sum of (L_p + 1)
+  WbStageTimer smat(WB_T_STAGE_SMAT, st, (84.0 * B + 96.0 * steps_all) * n3, WB_STAGE_COARSE);
   auto eig = [&]() -> hipError_t {
     smat.stop();
     smat.flops = 0.0;
@@ -794,6 +851,17 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
       WB_CHECK_LAUNCH();                                                                     \
     }                                                                                        \
   } while (0)
+  // a per-point factorization's failures (info over the first `count` points) onto each of the
+  // point's pairs
+  auto flag_points = [&](const int* inf, int count, int bit) -> hipError_t {
+    if (!a.flags) return hipSuccess;
+    if (ragged)
+      hipLaunchKernelGGL(k_flag_points, dim3((B + 255) / 256), dim3(256), 0, st, B, count, ppt, inf,
+                         bit, a.flags);
+    else
+      hipLaunchKernelGGL(k_flag, dim3((B + 255) / 256), dim3(256), 0, st, B, L, inf, bit, a.flags);
+    return hipGetLastError();
+  };
   if (a.flags) {
     hipLaunchKernelGGL(k_zero_flags, dim3((B + 255) / 256), dim3(256), 0, st, B, a.flags);
     WB_CHECK_LAUNCH();
@@ -813,7 +881,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     // the modes of an earlier wb_layer_modes call (resident eigensystems, batches whose layer
     // counts differ): rotated into the s/p basis as the fused path does
     hipLaunchKernelGGL(k_modes_take, grid2(mode == 0 ? (long long)N * n : nn, B), dim3(256), 0, st, N,
-                       n, L, mode, a.modes_W, a.modes_V, a.modes_q, a.modes_idx, ov, M[1], M[2], q);
+                       n, L, mode, a.modes_W, a.modes_V, a.modes_q, a.modes_idx, ppt, ov, M[1], M[2], q);
     WB_CHECK_LAUNCH();
     bigW = M[1]; bigV = M[2];
   } else if (mode == 0) {
@@ -916,7 +984,8 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     WB_CHECK_LAUNCH();
   }
   // A -> M4, B -> M5
-  hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, M[0], M[3], ov, M[4], M[5]);
+  hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, ppt, M[0], M[3], ov, M[4],
+                     M[5]);
   WB_CHECK_LAUNCH();
   // Y = B A^-1 from A^H Y^H = B^H: LU of a copy of A in M1, Y^H -> M2 ; XYX -> M3
   hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[4], nn), n, S(M[1], nn), n);
@@ -928,7 +997,8 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   WB_CHECK_LAUNCH();
   if ((e = wb_getrs_blocked(n, n, S(M[1], nn), n, ipiv[0], S(M[2], nn), n, B, 1, W_.luws, st))) return e;
   cd* xv = tau;   // the Householder scalars of the eigensolve are no longer needed
-  hipLaunchKernelGGL(k_xvec, dim3((n + 255) / 256, B), dim3(256), 0, st, n, L, q, a.pts, a.thick, xv);
+  hipLaunchKernelGGL(k_xvec, dim3((n + 255) / 256, B), dim3(256), 0, st, n, L, ppt, q, a.pts, a.thick,
+                     xv);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ctrans, tgrid, dim3(256), 0, st, n, M[2], M[3], (const cd*)xv);
   WB_CHECK_LAUNCH();
@@ -963,36 +1033,47 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M6, n2), 2 * n, S(M[3], nn), n);
     WB_CHECK_LAUNCH();
   }
-  // pair (p, l) lives at index p * L + l; per-point views step by L pairs
+  // Cascade step l takes layer l of the points that have it: every point (pair p * L + l, a
+  // stride of L pairs) or, ragged, the first steps[l] points (pairs base_l + p, consecutive).
+  // Pivots of step l: a P x n slice (ragged: from base_l n on).
+  auto step_points = [&](int l) { return ragged ? a.steps[l] : P; };
+  auto step_pairs = [&](cd* base, long long per_pair, int l) {
+    return ragged ? bp_strided(base + (long long)rb.base[l] * per_pair, per_pair)
+                  : bp_strided(base + (long long)l * per_pair, (long long)L * per_pair);
+  };
+  auto step_piv = [&](int l) { return ipiv[2] + (ragged ? (long long)rb.base[l] : (long long)l * P) * n; };
   hipLaunchKernelGGL(k_gamma0, grid2(nn, P), dim3(256), 0, st, n, dg, G0);
   WB_CHECK_LAUNCH();
   BatchPtr Gam = bp_strided(G0, nn);
   int ldg = n;
   for (int l = 0; l < L; ++l) {
-    BatchPtr s11 = bp_strided(M6 + (long long)l * n2, (long long)L * n2);
+    const int Pl = step_points(l);
+    BatchPtr s11 = step_pairs(M6, n2, l);
     BatchPtr s12 = bp_offset(s11, n);
-    BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);     // -> LU_l
-    BatchPtr Xs = bp_strided(M[1] + (long long)l * nn, (long long)L * nn);
-    BatchPtr G2 = bp_strided(M[2] + (long long)l * nn, (long long)L * nn);
+    BatchPtr Tm = step_pairs(M[0], nn, l);     // -> LU_l
+    BatchPtr Xs = step_pairs(M[1], nn, l);
+    BatchPtr G2 = step_pairs(M[2], nn, l);
     // T = s11 G ; Mx = I - T ; LU(Mx)
-    if ((e = wb_zgemm_launch(n, n, n, one, s11, 2 * n, Gam, ldg, zero, Tm, n, P, st))) return e;
-    hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tm, n, nullptr, 0);
+    if ((e = wb_zgemm_launch(n, n, n, one, s11, 2 * n, Gam, ldg, zero, Tm, n, Pl, st))) return e;
+    hipLaunchKernelGGL(k_i_minus, grid2(nn, Pl), dim3(256), 0, st, n, Tm, n, nullptr, 0);
     WB_CHECK_LAUNCH();
-    // pivots of (p, l): a P-strided slice per layer
-    int* piv_pl = ipiv[2] + (long long)l * P * n;
+    int* piv_pl = step_piv(l);
     int* info_pl = info[3];
-    if ((e = wb_getrf_launch(n, Tm, n, piv_pl, info_pl, P, st))) return e;
-    FLAG(B, L, info_pl, WB_FLAG_SINGULAR);
+    if ((e = wb_getrf_launch(n, Tm, n, piv_pl, info_pl, Pl, st))) return e;
+    if ((e = flag_points(info_pl, Pl, WB_FLAG_SINGULAR))) return e;
     // Xs = Mx^-1 s12
-    hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, s12, 2 * n, Xs, n);
+    hipLaunchKernelGGL(k_copy, grid2(nn, Pl), dim3(256), 0, st, n, n, s12, 2 * n, Xs, n);
     WB_CHECK_LAUNCH();
-    if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, Xs, n, P, 0, W_.luws, st))) return e;
-    // G2 = G Xs ; s11 += s12 G2  (s11 becomes Gamma_{l+1})
-    if ((e = wb_zgemm_launch(n, n, n, one, Gam, ldg, Xs, n, zero, G2, n, P, st))) return e;
-    if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, G2, n, one, s11, 2 * n, P, st))) return e;
+    if ((e = wb_getrs_blocked(n, n, Tm, n, piv_pl, Xs, n, Pl, 0, W_.luws, st))) return e;
+    // G2 = G Xs ; s11 += s12 G2  (s11 becomes Gamma_{l+1}; ragged, the next step's points are a
+    // prefix of this step's, so the same view serves it)
+    if ((e = wb_zgemm_launch(n, n, n, one, Gam, ldg, Xs, n, zero, G2, n, Pl, st))) return e;
+    if ((e = wb_zgemm_launch(n, n, n, one, s12, 2 * n, G2, n, one, s11, 2 * n, Pl, st))) return e;
     Gam = s11;
     ldg = 2 * n;
   }
+  // ragged: each point's Gamma_{L_p} sits in its top layer's pair
+  if (ragged) Gam = bp_gather(M6, n2, last);
   // final interface: u = (I - t11 G_L)^-1 t12 e ; R = t22 e + t21 G_L u
   const int last_ld = ldg;
   // Gamma_0 is no longer needed, unless the field state reads it (then M3, free since the
@@ -1004,7 +1085,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   WB_CHECK_LAUNCH();
   int* piv_f = ipiv[3];
   if ((e = wb_getrf_launch(n, Tf, n, piv_f, info[3], P, st))) return e;
-  FLAG(B, L, info[3], WB_FLAG_SINGULAR);
+  if ((e = flag_points(info[3], P, WB_FLAG_SINGULAR))) return e;
   double* sigc = nullptr;
   if (mode != 0) {
     hipLaunchKernelGGL(k_sigma_c, dim3((P + 255) / 256), dim3(256), 0, st, P, a.hu, ov, N, sig);
@@ -1032,16 +1113,31 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                            nrhs, P, st))) return e;
   hipLaunchKernelGGL(k_reflect, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[1], vec[2]);
   WB_CHECK_LAUNCH();
-  // sweep: d_{l} = (I - s11 G_{l})^-1 s12 d_{l+1}, layers top to bottom
+  // sweep: d_{l} = (I - s11 G_{l})^-1 s12 d_{l+1}, layers top to bottom. Ragged, a point joins
+  // at its top layer with d = u: u is kept in vec[3] (free until the transmitted columns) and
+  // copied in for the points that join at each step.
   cd* dcur = vec[0];
   cd* dnext = vec[1];
+  if (ragged) {
+    hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(vec[0], vs), nrhs,
+                       S(vec[3], vs), nrhs);
+    WB_CHECK_LAUNCH();
+  }
   for (int l = L - 1; l >= 0; --l) {
-    BatchPtr s12 = bp_offset(bp_strided(M6 + (long long)l * n2, (long long)L * n2), n);
-    BatchPtr Tm = bp_strided(M[0] + (long long)l * nn, (long long)L * nn);
-    int* piv_pl = ipiv[2] + (long long)l * P * n;
+    const int Pl = step_points(l);
+    const int joined = (ragged && l < L - 1) ? a.steps[l + 1] : Pl;
+    if (joined < Pl) {
+      hipLaunchKernelGGL(k_copy, grid2(vs, Pl - joined), dim3(256), 0, st, n, nrhs,
+                         S(vec[3] + (long long)joined * vs, vs), nrhs,
+                         S(dcur + (long long)joined * vs, vs), nrhs);
+      WB_CHECK_LAUNCH();
+    }
+    BatchPtr s12 = bp_offset(step_pairs(M6, n2, l), n);
+    BatchPtr Tm = step_pairs(M[0], nn, l);
+    int* piv_pl = step_piv(l);
     if ((e = wb_zgemm_launch(n, nrhs, n, one, s12, 2 * n, S(dcur, vs), nrhs, zero, S(dnext, vs),
-                             nrhs, P, st))) return e;
-    if ((e = wb_getrs_launch(n, nrhs, Tm, n, piv_pl, S(dnext, vs), nrhs, P, st))) return e;
+                             nrhs, Pl, st))) return e;
+    if ((e = wb_getrs_launch(n, nrhs, Tm, n, piv_pl, S(dnext, vs), nrhs, Pl, st))) return e;
     if (keep) {
       hipLaunchKernelGGL(k_copy, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(dnext, vs), nrhs,
                          S(CK.d + (long long)l * vs, dstride), nrhs);

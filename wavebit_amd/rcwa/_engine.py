@@ -270,31 +270,77 @@ def window_modes(buckets, convs, device, rebuild=True, inputs=None):
     return W, V, q, flags, tables
 
 
-def run_window(buckets, convs, device, modes=None, rebuild=True):
+def _ragged_inputs(buckets, tables, device):
+    """Host side of one ragged cascade over several buckets: the points ordered by non-increasing
+    layer count (the buckets' points kept together), pairs numbered layer-major (layer l of point
+    p at base_l + p). Returns the staged (pts, thick (B,), modes idx (B,)), the layer counts, each
+    bucket's first point and its (P, L) table of pair numbers (device int64)."""
+    order = sorted(range(len(buckets)), key=lambda i: -buckets[i].n_layers)
+    start, recs, lays = {}, [], []
+    for i in order:
+        start[i] = len(lays)
+        recs.extend(point_record(p, buckets[i].path) for p in buckets[i].points)
+        lays.extend([buckets[i].n_layers] * len(buckets[i].points))
+    P, Lmax = len(lays), max(lays)
+    thick = np.zeros((P, Lmax))
+    tab = np.zeros((P, Lmax), dtype=np.int32)
+    for i, b in enumerate(buckets):
+        s, L = start[i], b.n_layers
+        thick[s:s + len(b.points), :L] = np.array([p.thickness for p in b.points]).reshape(-1, L)
+        tab[s:s + len(b.points), :L] = tables[i]
+    _, (pp, ll) = _native.ragged_order(lays)
+    pair = np.full((P, Lmax), -1, dtype=np.int64)
+    pair[pp, ll] = np.arange(pp.size)
+    pairs = [torch.from_numpy(pair[start[i]:start[i] + len(b.points), :b.n_layers].copy()).to(device)
+             for i, b in enumerate(buckets)]
+    staged = (torch.from_numpy(np.array(recs, dtype=np.float64)).to(device),
+              torch.from_numpy(np.ascontiguousarray(thick[pp, ll])).to(device),
+              torch.from_numpy(np.ascontiguousarray(tab[pp, ll])).to(device))
+    return staged, lays, [start[i] for i in range(len(buckets))], pairs
+
+
+def run_window(buckets, convs, device, modes=None, rebuild=True, ragged=True):
     """Solve buckets of one order window and path with different layer counts: their modes in one
-    batch (``window_modes``, or ``modes`` from an earlier call on the same points), then each
-    bucket's S-matrices and cascade from them (``wb_solve_from_modes``). Results equal
-    ``run_bucket``'s per bucket to the bit. Every host input goes to the device before the first
-    launch, so with ``rebuild=False`` nothing here waits for the device. Returns
+    batch (``window_modes``, or ``modes`` from an earlier call on the same points), then the
+    S-matrices and one cascade over every point (``wb_solve_from_modes_ragged``: cascade step l
+    takes layer l of each point that has it). With ``ragged=False``, a single bucket or more than
+    ``RAGGED_MAX`` layers, one ``wb_solve_from_modes`` per bucket instead, whose results equal
+    ``run_bucket``'s to the bit; the ragged cascade does the same operations per point in
+    batches of other sizes (equal within rounding). Every host input goes to the device before
+    the first launch, so with ``rebuild=False`` nothing here waits for the device. Returns
     [(eff, amp, pe, flags, N)] per bucket, device tensors, flags combining the modes' and the
     cascade's, and the modes used."""
     b0 = buckets[0]
     mode = _MODE[b0.path]
     N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
-    staged = []
-    for bucket in buckets:
-        P, L = len(bucket.points), bucket.n_layers
-        rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
-        thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
-        staged.append((torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device)))
+    inputs = None
     if modes is None:
         inputs = _pair_inputs(buckets, convs, device)
-        tabs = [torch.from_numpy(t).to(device) for t in inputs[5]]
-        modes = window_modes(buckets, convs, device, rebuild=rebuild, inputs=inputs)
+    tables = inputs[5] if modes is None else modes[4]
+    tabs = [torch.from_numpy(t).to(device) for t in tables]
+    ragged = (ragged and len(buckets) > 1 and
+              max(b.n_layers for b in buckets) <= _native.RAGGED_MAX)
+    if ragged:
+        (pts_r, thick_r, idx_r), lays, starts, pairs = _ragged_inputs(buckets, tables, device)
     else:
-        tabs = [torch.from_numpy(t).to(device) for t in modes[4]]
+        staged = []
+        for bucket in buckets:
+            P, L = len(bucket.points), bucket.n_layers
+            rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
+            thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
+            staged.append((torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device)))
+    if modes is None:
+        modes = window_modes(buckets, convs, device, rebuild=rebuild, inputs=inputs)
     W, V, q, mflags, _ = modes
     out = []
+    if ragged:
+        eff, amp, pe, flags = _native.solve_from_modes_ragged(pts_r, thick_r, W, V, q, idx_r, lays,
+                                                              b0.hu, b0.hv, mode)
+        for bucket, s, pair, tab in zip(buckets, starts, pairs, tabs):
+            P, L = len(bucket.points), bucket.n_layers
+            f = flags[pair] | mflags[tab.reshape(-1).long()].reshape(P, L)
+            out.append((eff[s:s + P], amp[s:s + P], pe[s:s + P], f, N))
+        return out, modes
     for bucket, (pts, thick), tab in zip(buckets, staged, tabs):
         P, L = len(bucket.points), bucket.n_layers
         eff, amp, pe, flags = _native.solve_from_modes(pts, thick, W, V, q, tab, bucket.hu,

@@ -4,7 +4,8 @@ Requests (a ``Problem`` with its ``Numerics``) arrive over time. Each is resolve
 points; points wait in per-window queues keyed by (order window, solve path), and a launch
 packs queued points of one window -- from any number of requests, whatever their layer counts --
 into one launch sequence: the layer eigensolves of all its (point, layer) pairs as one batch
-(``wb_layer_modes``), then one cascade per layer count (``wb_solve_from_modes``). Launches run on
+(``wb_layer_modes``), then one cascade over every point whatever its layer count
+(``wb_solve_from_modes_ragged``: step l takes layer l of the points that have it). Launches run on
 several HIP streams at once, so work admitted
 while others are on the device overlaps with them. Admission is by HBM footprint: a launch
 is started only while the workspaces of all launches in flight plus the device-resident
@@ -112,8 +113,8 @@ class ConvLRU:
 class Server:
     """Continuous batching with HBM-footprint admission and overlapping streams. Forward points
     queue per order window and path; a layer count with ``window_below`` or more queued points
-    launches alone, smaller counts share window launches (one eigensolve batch, one cascade
-    per layer count)."""
+    launches alone, smaller counts share window launches (one eigensolve batch, one ragged
+    cascade)."""
 
     def __init__(self, device='cuda:0', streams=3, memory_fraction=0.6, max_points_per_launch=64,
                  conv_cache_fraction=0.1, window_below=16):
@@ -178,7 +179,7 @@ class Server:
 
     def _launch_bytes(self, key, items):
         """Device bytes of a window launch over ``items``: the modes of every pair and the
-        workspace that computes them, and the largest cascade workspace plus the outputs."""
+        workspace that computes them, and the ragged cascade's workspace plus the outputs."""
         hu, hv, path = key[:3]
         mode = _engine._MODE[path]
         N = (2 * hu + 1) * (2 * hv + 1)
@@ -190,7 +191,11 @@ class Server:
             return _native.solve_workspace_bytes(P, L, hu, hv, mode) + out
         pairs = sum(L * c for L, c in groups.items())
         modes = pairs * (2 * n * n + n) * 16 + _native.layer_modes_bytes(pairs, 1, hu, hv, mode)
-        cascade = max(_native.solve_workspace_bytes(c, L, hu, hv, mode) for L, c in groups.items())
+        if max(groups) <= _native.RAGGED_MAX:
+            cascade = _native.solve_ragged_workspace_bytes(
+                [L for L, c in groups.items() for _ in range(c)], hu, hv, mode)
+        else:
+            cascade = max(_native.solve_workspace_bytes(c, L, hu, hv, mode) for L, c in groups.items())
         out = len(items) * (3 * 4 * N * 8 + 3 * 4 * N * 16 + 3 * 8) + pairs * 8
         return modes + cascade + out
 
@@ -260,10 +265,11 @@ class Server:
         layer count with at least ``window_below`` queued points goes alone, as one fused solve
         of up to ``max_points_per_launch`` points (the count queued earliest first); otherwise
         the oldest points share a window launch whatever their layer counts. On the uneven
-        2048-problem batch (orders 3..12, 1-8 layers) single-count launches of ~25 points run
-        39.7 solves/s against 25.5 for windows of 32-64 mixed points, while on its first 384
-        problems (~5 points per count) windows run 27.2 against 20.5: windows pay off only for
-        counts too small to fill a launch."""
+        2048-problem batch (orders 3..12, 1-8 layers, ~25 points per count and window) this
+        policy runs 43.5 solves/s (80 launches) against 39.4 for windows of up to 128 mixed
+        points (33 launches); on its first 512 problems (~6 points per count) it makes every
+        launch a window: 46.3 solves/s in 10 launches, one per order window, as windows alone
+        (46.9)."""
         counts = collections.Counter(int(item[3][0].shape[0]) for item in q)
         full = [L for L, c in counts.items() if c >= min(self.window_below, self.max_points)]
         if full:
@@ -276,7 +282,7 @@ class Server:
 
     def _launch(self, key, items, stream):
         """One window launch: the items grouped by layer count into buckets (each with its own
-        conv stacks), their modes in one batch, one cascade per bucket."""
+        conv stacks), their modes in one batch, one ragged cascade over all of them."""
         hu, hv, path = key[:3]
         groups = collections.OrderedDict()
         for item in items:
