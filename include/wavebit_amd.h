@@ -19,6 +19,10 @@
  *                    rcwa/_engine/factorization.py:1016-1034 to_conv_mat_raster_laurent
  *                    (laurent_conv_matrices, :955-1000; _laurent_conv, :944-952), the
  *                    Numerics(factorization='laurent') opt-in, one call for a layer stack;
+ *   wb_assemble_nvf  rcwa/_engine/factorization.py:2840-2915 to_conv_mat_raster_nvf
+ *                    (nvf_conv_matrices, :2796-2815; normal_vector_field, :1180-1404; the
+ *                    closing _nvf_assemble, :2461-2540), Numerics(factorization='nvf') with
+ *                    the default normal field ('distance') and projector ('linear');
  *   wb_solve         rcwa/_engine/scattering.py:1386-1456 smm_2d and :704-902 smm_1d as the
  *                    engine calls them (rcwa/_engine/engine.py:3283-3316, 3336-3434), for a
  *                    batch of plane-wave points instead of one wavelength per call
@@ -156,11 +160,33 @@ int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, i
                         double* epx, double* epy, double* epz_i, void* workspace,
                         size_t workspace_bytes, int* info, void* stream);
 
+/* Normal-vector-field convolution matrices (sampled coefficients) of L raster layers eps
+ * (L, Ny, Nx): with E = [[eps]], A = [[1/eps]]^-1, Delta = E - A and C_ab the point-sample
+ * series of n_a n_b (n the unit normal of the periodic distance to the interfaces, on the raster
+ * block-repeated to 4h + 1 samples per axis),
+ *   epx = E - (Delta C_xx + C_xx Delta)/2, epy = E - (Delta C_yy + C_yy Delta)/2,
+ *   epxy = (Delta C_xy + C_xy Delta)/2 (the solve's epxy and epyx slot alike), epz_i = E^-1.
+ * lattice: host array (a1x, a1y, a2x, a2y), the cell basis the raster indexes (the normal is
+ * taken in Cartesian components through the cell's metric), or NULL for square pixels in an
+ * axis-aligned cell. hu, hv <= 15. info: 3 L device ints: the inversions of E, of [[1/eps]],
+ * then 1 for a layer holding a value that is not a passive dielectric (Re eps <= 0 or
+ * Im eps < 0), where the reference floors the singular values of [[1/eps]]
+ * (factorization.py:1968-2160) and this builder does not: such layers' outputs are the
+ * unfloored closing, which the caller refuses or accepts. */
+size_t wb_assemble_nvf_workspace_size(int L, int Ny, int Nx, int hu, int hv);
+int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                    const double* lattice, double* epx, double* epy, double* epz_i, double* epxy,
+                    void* workspace, size_t workspace_bytes, int* info, void* stream);
+
 /* Batch solve of P plane-wave points, each through L layers (bottom to top), top incidence.
  *   pts       P x WB_PT_STRIDE doubles (layout above)
  *   thick     P x L layer thicknesses
  *   conv_idx  P x L: which (epx, epy, epz_i) matrix set layer l of point p uses
  *   epx, epy, epz_i   C x N x N complex
+ *   epxy, epyx        C x N x N complex, the off-diagonal slots of an NVF layer (2D path;
+ *                     the same array twice for wb_assemble_nvf's symmetric projector), or both
+ *                     NULL for Li's and Laurent's rules (smm_2d's epxy_conv_all /
+ *                     epyx_conv_all, rcwa/_engine/solve_2d.py:2393-2411)
  * Outputs:
  *   eff  P x 3 x 4 x N doubles: incidence variant (as given, pure s, pure p) x (R_s, R_p, T_s,
  *        T_p) per order, orders flattened row-major over (2hv+1, 2hu+1). 1D modes fill
@@ -171,7 +197,8 @@ int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, i
 size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode);
 int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
              const double* thick, const int* conv_idx, const double* epx, const double* epy,
-             const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+             const double* epz_i, const double* epxy, const double* epyx, double* eff,
+             double* amp, double* pe, int* flags,
              void* workspace, size_t workspace_bytes, void* stream);
 
 /* wb_solve without a workspace argument, for callers that cannot allocate device memory (code
@@ -181,7 +208,7 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
  * the process ends; the call allocates nothing. Problems whose wb_solve_workspace_size exceeds
  * WB_SOLVE_NOALLOC_BYTES are refused with WB_OOM. Every call on one device shares that device's
  * array: calls that overlap on one device (two streams or two threads) race, so serialize them.
- * Arguments and outputs are wb_solve's. */
+ * Arguments and outputs are wb_solve's, less the off-diagonal slots (Li and Laurent layers). */
 #define WB_SOLVE_NOALLOC_BYTES (64u << 20)
 int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                      const double* thick, const int* conv_idx, const double* epx, const double* epy,
@@ -200,7 +227,7 @@ int wb_solve_noalloc(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const 
 size_t wb_layer_modes_workspace_size(int P, int L, int hu, int hv, int mode, int rebuild);
 int wb_layer_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                    const int* conv_idx, const double* epx, const double* epy, const double* epz_i,
-                   double* W, double* V, double* q, int* flags, int rebuild, void* workspace,
+                   const double* epxy, const double* epyx, double* W, double* V, double* q, int* flags, int rebuild, void* workspace,
                    size_t workspace_bytes, void* stream);
 
 /* The second half of wb_solve: the layer S-matrices, the cascade and the efficiencies from modes
@@ -243,7 +270,7 @@ int wb_solve_from_modes_ragged(wb_ctx* ctx, int P, const int* layers, int hu, in
 size_t wb_solve_rebuild_workspace_size(int P, int L, int hu, int hv);
 int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pts,
                      const double* thick, const int* conv_idx, const double* epx, const double* epy,
-                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     const double* epz_i, const double* epxy, const double* epyx, double* eff, double* amp, double* pe, int* flags,
                      double* smat, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Field state: wb_solve_from_modes plus each layer's mode amplitudes for the point's own
@@ -289,7 +316,8 @@ int wb_field_sample(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const d
  * The workspace is wb_solve_workspace_size's. */
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                         const double* thick, const int* conv_idx, const double* epx,
-                        const double* epy, const double* epz_i, double* eff, double* amp, double* pe,
+                        const double* epy, const double* epz_i, const double* epxy,
+                        const double* epyx, double* eff, double* amp, double* pe,
                         int* flags, double* smat, void* workspace, size_t workspace_bytes,
                         void* stream);
 

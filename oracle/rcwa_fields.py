@@ -121,8 +121,11 @@ def state_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, 
     S_bottom = orc.s_reflection(W_kt, V_kt, W_gap, V_gap)
     S_top = orc.s_transmission(W_kb, V_kb, W_gap, V_gap)
     layers = []
-    for li, (epx, epy, epz_i) in enumerate(conv):
-        W, V, q, status = orc.layer_modes_2d_screened(kxf, kyf, epx, epy, epz_i)
+    for li, slots in enumerate(conv):
+        epx, epy, epz_i = slots[:3]
+        epxy, epyx = slots[3:5] if len(slots) == 5 else (None, None)
+        W, V, q, status = orc.layer_modes_2d_screened(kxf, kyf, epx, epy, epz_i, 'rebuild', epxy,
+                                                      epyx)
         big_W, big_V = orc.sp_modes_2d(W, V, Fc, Fs, N)
         A, B = orc.ab_matrices(big_W, big_V, W_gap, V_gap)
         s11, s12, _, _ = orc.s_layer(A, B, np.exp(-k0 * thickness[li] * q))
@@ -168,7 +171,7 @@ def state_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot
     layers = []
     rule = orc.conv_rule(factorization, coefficients)
     for li in range(len(thickness)):
-        epx, epy, epz_i = rule(eps_layers[li], hu, 0)
+        epx, epy, epz_i = rule(eps_layers[li], hu, 0)[:3]
         W, V, q = orc.layer_modes_1d(tetm, kx, epx, epy, epz_i)
         A, B = orc.ab_matrices(W, V, I, V_gap)
         s11, s12, _, _ = orc.s_layer(A, B, np.exp(-k0 * thickness[li] * q))
@@ -298,6 +301,10 @@ def solve_state(eps_layers, thickness, eps_top, eps_bot, period, wavelength, the
     else:
         pol = np.asarray(polarization, dtype=np.complex128)
         pol = pol / np.sqrt(np.sum(np.abs(pol) ** 2))
-    rule = orc.conv_rule(factorization, coefficients)
+    lattice = None
+    if factorization == 'nvf':
+        from oracle.rcwa_nvf import lattice_of
+        lattice = lattice_of(period)
+    rule = orc.conv_rule(factorization, coefficients, lattice)
     conv = [rule(e, hu, hv) for e in eps_layers]
     return state_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, pol, hu, hv)

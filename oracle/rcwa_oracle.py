@@ -194,8 +194,14 @@ CONV_RULES = {'li': li_conv_matrices, 'laurent': laurent_conv_matrices,
               ('laurent', 'integrated'): laurent_conv_matrices_integrated}
 
 
-def conv_rule(factorization='li', coefficients='sampled'):
-    """The oracle's builder for one (factorization, fourier_coefficients) pair."""
+def conv_rule(factorization='li', coefficients='sampled', lattice=None):
+    """The oracle's builder for one (factorization, fourier_coefficients) pair. NVF
+    (``oracle/rcwa_nvf.py``) also reads the cell basis ``lattice`` (``None``: square pixels)
+    and returns five slots ``(epx, epy, epz_i, epxy, epyx)``; the others return three."""
+    if factorization == 'nvf':
+        from oracle import rcwa_nvf
+        return lambda e, hu, hv: rcwa_nvf.nvf_conv_matrices(e, hu, hv, lattice=lattice,
+                                                            coefficients=coefficients)
     return CONV_RULES[factorization if coefficients in (None, 'sampled') else (factorization, coefficients)]
 
 
@@ -336,9 +342,11 @@ def redheffer_star(I, S_a, S_b):
 
 # --------------------------------------------------------------------------- layer modes
 
-def layer_modes_2d(kx, ky, epx, epy, epz_i):
+def layer_modes_2d(kx, ky, epx, epy, epz_i, epxy=None, epyx=None):
     """``(W, V, q)`` of one 2D layer, non-magnetic, no exceptional-point rebuild
-    (``solve_2d.py:2303-2448``)."""
+    (``solve_2d.py:2303-2448``). ``epxy`` / ``epyx``: the off-diagonal slots of an NVF layer
+    (``None``: zero), entering ``Omega^2`` and ``Omega_R`` as ``solve_2d.py:2393-2411`` writes
+    them."""
     N = kx.shape[0]
     I = np.eye(N, dtype=np.complex128)
     Kx, Ky = np.diag(kx), np.diag(ky)
@@ -346,10 +354,13 @@ def layer_modes_2d(kx, ky, epx, epy, epz_i):
     D = (ky[:, None] * epz_i) * ky[None, :] - I
     Cc = (kx[:, None] * epz_i) * ky[None, :]
     E = (ky[:, None] * epz_i) * kx[None, :]
-    O2 = np.block([[Ky @ Ky + B @ epx, Cc @ epy - Kx @ Ky],
-                   [E @ epx - Ky @ Kx, Kx @ Kx + D @ epy]])
+    zero = np.zeros((N, N), dtype=np.complex128)
+    exy = zero if epxy is None else epxy
+    eyx = zero if epyx is None else epyx
+    O2 = np.block([[Ky @ Ky + B @ epx - Cc @ eyx, Cc @ epy - Kx @ Ky - B @ exy],
+                   [E @ epx - Ky @ Kx - D @ eyx, Kx @ Kx + D @ epy - E @ exy]])
     w, W = np.linalg.eig(O2)
-    OR = np.block([[-(Kx @ Ky), Kx @ Kx - epy], [epx - Ky @ Ky, Ky @ Kx]])
+    OR = np.block([[-(Kx @ Ky) + eyx, Kx @ Kx - epy], [epx - Ky @ Ky, Ky @ Kx - exy]])
     q = mode_q(w)
     V = (OR @ W) / q
     return W, V, q, O2, OR
@@ -464,7 +475,8 @@ def pair_sines(w, W):
     return np.sqrt(np.maximum(0.0, 1.0 - cos ** 2))
 
 
-def layer_modes_2d_screened(kx, ky, epx, epy, epz_i, exceptional_points='rebuild'):
+def layer_modes_2d_screened(kx, ky, epx, epy, epz_i, exceptional_points='rebuild', epxy=None,
+                            epyx=None):
     """``layer_modes_2d`` with the reference's Rayleigh/EP handling (solve_2d.py:2450-2581):
     a grazing layer's cluster is rebuilt (``rebuild_grazing_cluster``: ``'rebuilt'``, or
     ``'kept'`` when the products disagree on the cluster); otherwise, under
@@ -473,7 +485,7 @@ def layer_modes_2d_screened(kx, ky, epx, epy, epz_i, exceptional_points='rebuild
     whose construction is within 1e-10). Returns ``(W, V, q, status)``, status one of
     ``'plain'``, ``'rebuilt'``, ``'kept'``, ``'pair'`` (a gated pair above the loud sine)."""
     with np.errstate(divide='ignore', invalid='ignore'):
-        W, V, q, _, OR = layer_modes_2d(kx, ky, epx, epy, epz_i)
+        W, V, q, _, OR = layer_modes_2d(kx, ky, epx, epy, epz_i, epxy, epyx)
     w = q ** 2
     if _small(w, GRAZE_REL).size:
         rebuilt = rebuild_grazing_cluster(w, W, V, q, omega_left(kx, ky, epz_i), OR)
@@ -558,7 +570,7 @@ def smm_1d(tetm, wavelength, period_x, eps_layers, thickness, eps_top, eps_bot, 
     S_trn = s_transmission(I, V_kb, I, V_gap)
     S = S_ref
     for li in range(len(thickness)):
-        epx, epy, epz_i = conv_rule(factorization, coefficients)(eps_layers[li], hu, 0)
+        epx, epy, epz_i = conv_rule(factorization, coefficients)(eps_layers[li], hu, 0)[:3]
         W, V, q = layer_modes_1d(tetm, kx, epx, epy, epz_i)
         X = np.exp(-k0 * thickness[li] * q)
         A, B = ab_matrices(W, V, I, V_gap)
@@ -667,8 +679,11 @@ def smm_2d(wavelength, period, conv, thickness, eps_top, eps_bot, theta, phi, po
     S_trn = s_transmission(W_kb, V_kb, W_gap, V_gap)
     modes = []
     status = []
-    for li, (epx, epy, epz_i) in enumerate(conv):
-        W, V, q, st = layer_modes_2d_screened(kxf, kyf, epx, epy, epz_i, exceptional_points)
+    for li, slots in enumerate(conv):
+        epx, epy, epz_i = slots[:3]
+        epxy, epyx = slots[3:5] if len(slots) == 5 else (None, None)
+        W, V, q, st = layer_modes_2d_screened(kxf, kyf, epx, epy, epz_i, exceptional_points,
+                                              epxy, epyx)
         status.append(st)
         big_W, big_V = sp_modes_2d(W, V, Fc, Fs, N)
         X = np.exp(-k0 * thickness[li] * q)
@@ -745,7 +760,11 @@ def solve_raster(eps_layers, thickness, eps_top, eps_bot, period, wavelength, th
     are the lattice vectors (a skewed cell; Laurent's rule). ``factorization``: ``'li'`` (the
     default) or ``'laurent'``; ``coefficients``: ``'sampled'`` (the default) or ``'integrated'``.
     """
-    rule = conv_rule(factorization, coefficients)
+    lattice = None
+    if factorization == 'nvf':
+        from oracle.rcwa_nvf import lattice_of
+        lattice = lattice_of(period)
+    rule = conv_rule(factorization, coefficients, lattice)
     eps_layers = np.asarray(eps_layers, dtype=np.complex128)
     hu, hv = orders
     one_row = eps_layers.shape[1] == 1

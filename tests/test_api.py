@@ -85,7 +85,10 @@ def test_cpu_backends_are_refused_not_rerouted(kwargs, code):
 
 
 @pytest.mark.parametrize('numerics', [rcwa.Numerics(orders=(1, 1), algorithm='etm'),
-                                      rcwa.Numerics(orders=(1, 1), factorization='nvf'),
+                                      rcwa.Numerics(orders=(1, 1), factorization='nvf',
+                                                    raster_normal_method='contour'),
+                                      rcwa.Numerics(orders=(1, 1), factorization='nvf',
+                                                    nvf_projector='circular'),
                                       rcwa.Numerics(orders=(1, 1), li_nesting='symmetric'),
                                       rcwa.Numerics(orders=(1, 1), formulation='first_order')])
 def test_non_default_numerics_are_unsupported(numerics):
@@ -101,6 +104,56 @@ def test_laurent_is_accepted_and_reported():
     assert rep.resolved['factorization'] == 'laurent' and rep.resolved['li_nesting'] is None
     rep = rcwa.check(_problem(), numerics=rcwa.Numerics(orders=(2, 2)), backend='torch', device='cuda')
     assert rep.resolved['factorization'] == 'li' and rep.resolved['li_nesting'] == 'fixed'
+
+
+@pytest.mark.parametrize('kw', [dict(), dict(raster_normal_method='distance'),
+                                dict(nvf_projector='linear'), dict(fourier_coefficients='integrated')])
+def test_nvf_is_accepted_with_its_default_construction(kw):
+    numerics = rcwa.Numerics(orders=(2, 2), factorization='nvf', **kw)
+    rep = rcwa.check(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert rep.ok and rep.resolved['factorization'] == 'nvf' and rep.resolved['li_nesting'] is None
+
+
+@pytest.mark.parametrize('kw, path', [(dict(li_nesting='fixed'), 'numerics.li_nesting'),
+                                      (dict(raster_normal_method='distance', factorization='li'),
+                                       'numerics.raster_normal_method'),
+                                      (dict(nvf_projector='linear', factorization='laurent'),
+                                       'numerics.nvf_projector')])
+def test_options_without_a_reader_are_refused(kw, path):
+    """Reference engine.py:5807-5850, 5887-5899: a normal-field option without NVF, or a nesting
+    under NVF, is refused rather than ignored."""
+    kw = dict(kw)
+    numerics = rcwa.Numerics(orders=(1, 1), factorization=kw.pop('factorization', 'nvf'), **kw)
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(_problem(), numerics=numerics, backend='torch', device='cuda')
+    assert e.value.code == 'rcwa.unsupported' and e.value.path == path
+
+
+def test_nvf_refuses_a_patterned_metal_layer_but_takes_a_uniform_one():
+    eps = np.ones((6, 6), dtype=complex)
+    eps[2:4, 2:4] = -20.0 + 1.0j
+    bad = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.1, epsilon=eps)], top=wavebit_amd.Material(1.0),
+                         bottom=wavebit_amd.Material(2.25), period=(0.5, 0.5)),
+        excitation=rcwa.Excitation(wavelength=0.7, sources=(rcwa.PlaneWave(polarization='s'),)))
+    with pytest.raises(UnsupportedFeatureError) as e:
+        rcwa.prepare(bad, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'),
+                     backend='torch', device='cuda')
+    assert e.value.path == 'stack.layers[0].epsilon'
+    film = np.full((6, 6), -20.0 + 1.0j)
+    ok = rcwa.Problem(
+        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.01, epsilon=film)], top=wavebit_amd.Material(1.0),
+                         bottom=wavebit_amd.Material(2.25), period=(0.5, 0.5)),
+        excitation=rcwa.Excitation(wavelength=0.7, sources=(rcwa.PlaneWave(polarization='s'),)))
+    assert rcwa.check(ok, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'),
+                      backend='torch', device='cuda').ok
+
+
+def test_nvf_workspace_queries_need_no_gpu():
+    lib = _native.load_library()
+    assert lib.wb_assemble_nvf_workspace_size(2, 8, 8, 3, 3) > lib.wb_assemble_laurent_workspace_size(2, 8, 8, 3, 3)
+    assert lib.wb_assemble_nvf_workspace_size(0, 8, 8, 3, 3) == 0
+    assert lib.wb_assemble_nvf_workspace_size(1, 8, 8, 16, 3) == 0
 
 
 def test_integrated_coefficients_are_accepted_and_reported():

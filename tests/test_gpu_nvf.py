@@ -1,0 +1,160 @@
+"""The normal-vector-field factorization (``Numerics(factorization='nvf')``) on the GPU.
+
+* Assembly (``wb_assemble_nvf``) against the oracle (``oracle.rcwa_nvf.nvf_conv_matrices``, SciPy's
+  distance transform underneath) at 1e-12 of the largest entry: square pixels, a raster lifted to
+  its 4h + 1 floor, non-square pixels in a rectangular cell, a skewed cell, a one-row raster (the
+  constant normal), a uniform layer and a 128 x 128 raster.
+* Solves through the public lifecycle against the oracle per order within 1e-10 (two layers,
+  oblique Jones incidence; rectangular and skewed cells; integrated coefficients), energy on a
+  lossless stack, the pure 1D route equal to Li's, the scattering output and the fields against
+  the oracle, and a served NVF request equal to its run alone.
+"""
+import numpy as np
+import pytest
+import torch
+
+import wavebit_amd
+from oracle import rcwa_fields as orf
+from oracle import rcwa_nvf as nvf
+from oracle import rcwa_oracle as orc
+from wavebit_amd import _native, rcwa
+from wavebit_amd.rcwa import serve
+
+from test_oracle_nvf import _blobs
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device('cuda:0')
+RUN = dict(backend='torch', device='cuda:0')
+
+
+def _stack(L, shape, seed, values=(1.0, 6.25)):
+    return np.stack([_blobs(shape, seed + 7 * l, values) for l in range(L)])
+
+
+CASES = {
+    'square': (lambda: _stack(2, (16, 16), 1), 3, 3, None),
+    'lifted': (lambda: _stack(1, (12, 12), 2, (1.0, 4.0 + 0.3j)), 3, 3, None),
+    'rect_pixels': (lambda: _stack(1, (20, 14), 3), 2, 3, np.diag([1.0, 0.75])),
+    'skewed': (lambda: _stack(1, (18, 18), 4, (2.25, 9.0)), 3, 3, np.array([[1.0, 0.4], [0.0, 0.9]])),
+    'one_row': (lambda: np.array([[[1, 1, 12.25, 12.25, 3 + 0.1j, 1, 2, 2]]], dtype=np.complex128),
+                5, 0, np.diag([0.7, 1.0])),
+    'uniform': (lambda: np.full((1, 5, 5), 2.5 + 0.1j), 2, 2, np.diag([1.0, 1.0])),
+    'large': (lambda: _stack(1, (128, 128), 5), 7, 7, np.diag([1.0, 1.0])),
+}
+
+
+@pytest.mark.parametrize('name', list(CASES))
+def test_nvf_assembly_matches_the_oracle(name):
+    make, hu, hv, lattice = CASES[name]
+    eps = make()
+    print(f'nvf assembly {name}: {eps.shape} orders ({hu}, {hv})', flush=True)
+    epx, epy, epz_i, epxy, info = _native.assemble_nvf(torch.from_numpy(eps).to(DEV), hu, hv, lattice)
+    info = info.cpu().numpy()
+    assert not info[:2 * eps.shape[0]].any() and not info[2 * eps.shape[0]:].any()
+    for l in range(eps.shape[0]):
+        want = nvf.nvf_conv_matrices(eps[l], hu, hv, lattice=lattice)
+        for got, w, slot in zip((epx[l], epy[l], epz_i[l], epxy[l]), want[:4], ('epx', 'epy', 'epz_i', 'epxy')):
+            got = got.cpu().numpy()
+            err = np.abs(got - w).max()
+            assert err <= 1e-12 * max(1.0, np.abs(want[0]).max()), (name, slot, err)
+
+
+def test_nvf_assembly_reports_a_layer_that_is_not_a_passive_dielectric():
+    eps = _stack(2, (10, 10), 6, (1.0, 4.0))
+    eps[1, 3:6, 3:6] = -12.0 + 0.8j
+    *_, info = _native.assemble_nvf(torch.from_numpy(eps).to(DEV), 2, 2)
+    assert info.cpu().numpy()[4:].tolist() == [0, 1]
+
+
+def _problem(eps, d, period, pol, wls, theta=0.25, phi=0.6, top=1.0, bottom=2.25, outputs=None):
+    layers = [rcwa.Layer(thickness=t, epsilon=e) for t, e in zip(d, eps)]
+    cell = dict(lattice=period) if np.shape(period) == (2, 2) else dict(period=period)
+    kw = {} if outputs is None else dict(outputs=outputs)
+    return rcwa.Problem(
+        stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(top),
+                         bottom=wavebit_amd.Material(bottom), **cell),
+        excitation=rcwa.Excitation(wavelength=list(wls), sources=(rcwa.PlaneWave(
+            polarization=pol, theta=theta, phi=phi),)), units=wavebit_amd.Units(length='um'), **kw)
+
+
+@pytest.mark.parametrize('period, coef', [((1.0, 1.0), 'sampled'), ((0.9, 0.8), 'sampled'),
+                                          (np.array([[1.0, 0.4], [0.0, 0.9]]), 'sampled'),
+                                          ((1.0, 1.0), 'integrated')])
+def test_nvf_run_matches_the_oracle_per_order(period, coef):
+    eps = _stack(2, (16, 16), 11, (1.0, 4.0 + 0.2j))
+    d, wls, pol = [0.21, 0.13], [0.71, 0.83], (0.6, 0.8j)
+    numerics = rcwa.Numerics(orders=(3, 3), factorization='nvf', fourier_coefficients=coef)
+    res = rcwa.run(_problem(eps, d, period, pol, wls), numerics=numerics, **RUN)
+    for i, wl in enumerate(wls):
+        want = orc.solve_raster(eps, d, 1.0, 2.25, np.asarray(period), wl, 0.25, 0.6, pol, (3, 3),
+                                factorization='nvf', coefficients=coef)['incident']
+        for k, got in (('R', res.R[i]), ('T', res.T[i])):
+            assert np.abs(np.asarray(got) - want[k]).max() < 1e-10, (k, wl)
+
+
+def test_nvf_lossless_stack_conserves_energy_at_size():
+    n = 64
+    y, x = np.mgrid[:n, :n]
+    disc = np.where((x - 31.5) ** 2 + (y - 27.5) ** 2 <= 18 ** 2, 6.25, 1.0).astype(np.complex128)
+    ring = np.where(np.abs(np.hypot(x - 20.5, y - 40.5) - 12) <= 4, 2.25, 1.0).astype(np.complex128)
+    wls = [0.62, 0.7, 0.78, 0.91]
+    res = rcwa.run(_problem([disc, ring], [0.18, 0.25], (1.0, 1.0), 'p', wls, top=1.0, bottom=1.0),
+                   numerics=rcwa.Numerics(orders=(5, 5), factorization='nvf'), **RUN)
+    tot = np.asarray(res.R).sum(axis=(1, 2)) + np.asarray(res.T).sum(axis=(1, 2))
+    assert np.abs(tot - 1.0).max() < 1e-10, tot
+
+
+@pytest.mark.parametrize('pol', ['s', 'p'])
+def test_nvf_on_a_one_row_raster_is_the_li_answer(pol):
+    eps = np.array([[[1, 1, 12.25, 12.25, 12.25, 1, 1, 1]]], dtype=np.complex128)
+    prob = _problem(eps, [0.3], (0.7, 0.7), pol, [0.9], theta=0.2, phi=None)
+    a = rcwa.run(prob, numerics=rcwa.Numerics(orders=(10, 0), factorization='nvf'), **RUN)
+    b = rcwa.run(prob, numerics=rcwa.Numerics(orders=(10, 0)), **RUN)
+    assert a.diagnostics['solve_path'][0] == ('1d_te' if pol == 's' else '1d_tm')
+    assert np.abs(np.asarray(a.R) - np.asarray(b.R)).max() < 1e-12
+    assert np.abs(np.asarray(a.T) - np.asarray(b.T)).max() < 1e-12
+
+
+def test_nvf_scattering_and_fields_match_the_oracle():
+    eps = _stack(1, (14, 14), 21, (1.0, 4.0))
+    d, wl = [0.24], 0.76
+    prob = _problem(eps, d, (0.9, 0.9), 'p', [wl], outputs=rcwa.Outputs(scattering=True, fields=True))
+    res = rcwa.run(prob, numerics=rcwa.Numerics(orders=(2, 2), factorization='nvf'), **RUN)
+    want = orc.solve_raster(eps, d, 1.0, 2.25, (0.9, 0.9), wl, 0.25, 0.6, 'p', (2, 2),
+                            factorization='nvf')
+    S = res.scattering[0]
+    ws = want['scattering']
+    for got, key in zip(S.blocks, ('S11', 'S12', 'S21', 'S22')):
+        w = ws[key]
+        assert np.abs(np.asarray(got) - w).max() < 1e-10 * max(1.0, np.abs(w).max()), key
+    st = orf.solve_state(eps, d, 1.0, 2.25, (0.9, 0.9), wl, 0.25, 0.6, 'p', (2, 2), factorization='nvf')
+    fw = orf.public(orf.sample(st, 5, 4, [0.05, 0.12, 0.2]))
+    fg = res.field(x=5, y=4, z=[0.05, 0.12, 0.2]).values[0]
+    assert np.abs(fg - fw).max() < 1e-9 * np.abs(fw).max()
+
+
+def test_a_served_nvf_request_equals_its_run_alone():
+    eps = _stack(2, (12, 12), 31)
+    numerics = rcwa.Numerics(orders=(2, 2), factorization='nvf')
+    li = rcwa.Numerics(orders=(2, 2))
+    prob = _problem(eps, [0.2, 0.1], (1.0, 1.0), 's', [0.7, 0.8, 0.9])
+    srv = serve.Server(streams=2)
+    results, rep = srv.serve([serve.Request(problem=prob, numerics=numerics, arrival=0.0, id=0),
+                              serve.Request(problem=prob, numerics=li, arrival=0.0, id=1)],
+                             realtime=False)
+    for got, num in ((results[0], numerics), (results[1], li)):
+        assert not isinstance(got, Exception), got
+        alone = rcwa.run(prob, numerics=num, **RUN)
+        assert np.abs(np.asarray(got.R) - np.asarray(alone.R)).max() < 1e-12
+    assert np.abs(np.asarray(results[0].R) - np.asarray(results[1].R)).max() > 1e-6
+
+
+def test_nvf_gradients_are_refused_by_name():
+    eps = torch.tensor(_stack(1, (8, 8), 41).real, dtype=torch.float64, requires_grad=True)
+    layers = [rcwa.Layer(thickness=0.2, epsilon=eps[0])]
+    prob = rcwa.Problem(stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(1.0),
+                                         bottom=wavebit_amd.Material(2.25), period=(1.0, 1.0)),
+                        excitation=rcwa.Excitation(wavelength=0.8, sources=(rcwa.PlaneWave(polarization='p'),)))
+    with pytest.raises(wavebit_amd.errors.UnsupportedFeatureError) as e:
+        rcwa.run(prob, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'), **RUN)
+    assert e.value.code == 'rcwa.gradient'

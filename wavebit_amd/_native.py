@@ -40,7 +40,8 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_rebuild', 'wb_layer_modes_workspace_size', 'wb_layer_modes',
            'wb_solve_from_modes', 'wb_field_state_workspace_size', 'wb_field_state',
            'wb_field_sample_workspace_size', 'wb_field_sample', 'wb_flags_status',
-           'wb_solve_ragged_workspace_size', 'wb_solve_from_modes_ragged')
+           'wb_solve_ragged_workspace_size', 'wb_solve_from_modes_ragged',
+           'wb_assemble_nvf_workspace_size', 'wb_assemble_nvf')
 
 TIMER_IDS = {'zgemm': 0, 'getrf_panel': 1, 'laswp_trsm': 2, 'trsm_diag': 3, 'gehrd': 4,
              'unghr': 5, 'hqr': 6, 'trevc': 7, 'colnorm': 8, 'assemble': 9, 'stage_eig': 10,
@@ -85,13 +86,19 @@ def load_library():
                                        c_vp, c_vp, c_sz, c_vp, c_vp]),
             'wb_solve_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
-                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+                                 c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                 c_vp]),
+            'wb_assemble_nvf_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_assemble_nvf': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
+                                        c_vp, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp]),
             'wb_solve_rebuild_workspace_size': (c_sz, [c_int, c_int, c_int, c_int]),
             'wb_solve_rebuild': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
-                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_sz, c_vp]),
             'wb_layer_modes_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int, c_int]),
             'wb_layer_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
-                                       c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_sz, c_vp]),
+                                       c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_sz,
+                                       c_vp]),
             'wb_solve_from_modes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                             c_sz, c_vp]),
@@ -136,7 +143,7 @@ def load_library():
                                                     c_vp]),
             'wb_solve_scattering': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                                            c_vp, c_sz, c_vp]),
+                                            c_vp, c_vp, c_vp, c_sz, c_vp]),
             'wb_timing_enable': (c_int, [c_int]),
             'wb_gemm_tuning_configs': (c_int, []),
             'wb_gemm_tuning_config_name': (ctypes.c_char_p, [c_int]),
@@ -216,6 +223,14 @@ def _require(t, dtype, device, name):
     if t.dtype != dtype or t.device != device or not t.is_contiguous():
         raise ValueError(f'{name}: expected a contiguous {dtype} tensor on {device}, got '
                          f'{t.dtype} on {t.device} (contiguous={t.is_contiguous()})')
+
+
+def _offdiag_ok(epxy, epx):
+    if epxy is not None:
+        _require(epxy, torch.complex128, epx.device, 'epxy')
+        if epxy.shape != epx.shape:
+            raise ValueError(f'epxy: expected the shape of epx {tuple(epx.shape)}, got '
+                             f'{tuple(epxy.shape)}')
 
 
 def _workspace(nbytes, device):
@@ -322,13 +337,45 @@ def assemble_laurent(eps, hu, hv):
     return out[0], out[1], out[2], info
 
 
-def assemble(eps, hu, hv, factorization='li'):
-    """The convolution matrices of ``factorization`` ('li' or 'laurent') for a raster stack."""
+def assemble_nvf(eps, hu, hv, lattice=None):
+    """NVF convolution matrices of a raster stack eps (L, Ny, Nx) -> (epx, epy, epz_i, epxy,
+    info): epxy is the solve's epxy and epyx slot alike; ``lattice`` the cell basis (2 x 2,
+    columns a1 and a2) or None for square pixels; info holds 3 L ints (the last L: layers that
+    are not passive dielectrics)."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    L, Ny, Nx = eps.shape
+    N = (2 * hu + 1) * (2 * hv + 1)
+    out = [torch.empty((L, N, N), dtype=torch.complex128, device=device) for _ in range(4)]
+    info = torch.empty((3 * L,), dtype=torch.int32, device=device)
+    nbytes = ctx.lib.wb_assemble_nvf_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    cell = None
+    if lattice is not None:
+        A = np.asarray(lattice, dtype=np.float64).reshape(2, 2)
+        cell = (ctypes.c_double * 4)(A[0, 0], A[1, 0], A[0, 1], A[1, 1])
+    ctx.check(ctx.lib.wb_assemble_nvf(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, cell, _ptr(out[0]),
+                                      _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _ptr(ws), nbytes,
+                                      _ptr(info), _stream(device)), 'assemble_nvf')
+    return out[0], out[1], out[2], out[3], info
+
+
+def assemble(eps, hu, hv, factorization='li', lattice=None):
+    """The convolution matrices of ``factorization`` ('li', 'laurent' or 'nvf') for a raster
+    stack: (epx, epy, epz_i, epxy, info), epxy None for the diagonal rules."""
     if factorization == 'laurent':
-        return assemble_laurent(eps, hu, hv)
+        return _with_offdiag(assemble_laurent(eps, hu, hv))
     if factorization == 'li':
-        return assemble_li(eps, hu, hv)
+        return _with_offdiag(assemble_li(eps, hu, hv))
+    if factorization == 'nvf':
+        return assemble_nvf(eps, hu, hv, lattice)
     raise ValueError(f'no assembly for factorization={factorization!r}')
+
+
+def _with_offdiag(res):
+    epx, epy, epz_i, info = res
+    return epx, epy, epz_i, None, info
 
 
 def solve_workspace_bytes(P, L, hu, hv, mode):
@@ -342,8 +389,9 @@ def solve_ragged_workspace_bytes(layers, hu, hv, mode):
         len(lay), lay.ctypes.data_as(ctypes.c_void_p), hu, hv, mode))
 
 
-def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
-    """Batch RCWA solve; see ``wb_solve``. Returns (eff, amp, pe, flags)."""
+def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None, epxy=None):
+    """Batch RCWA solve; see ``wb_solve``. ``epxy``: the NVF off-diagonal slot (epxy = epyx),
+    or None. Returns (eff, amp, pe, flags)."""
     device = epx.device
     ctx = context(device)
     P, L = thick.shape
@@ -360,14 +408,16 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None):
     nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
     if workspace is None or workspace.numel() < nbytes:
         workspace = _workspace(nbytes, device)
+    _offdiag_ok(epxy, epx)
     ctx.check(ctx.lib.wb_solve(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
-                               _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(eff),
+                               _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(epxy),
+                               _ptr(epxy), _ptr(eff),
                                _ptr(amp), _ptr(pe), _ptr(flags), _ptr(workspace), nbytes,
                                _stream(device)), 'solve')
     return eff, amp, pe, flags
 
 
-def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=False):
+def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=False, epxy=None):
     """``solve`` of 2D points with every grazing layer's mode cluster rebuilt; see
     ``wb_solve_rebuild``. Returns (eff, amp, pe, flags) and, with ``scattering``, smat."""
     device = epx.device
@@ -387,8 +437,10 @@ def solve_rebuild(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, scattering=Fals
     smat = torch.empty((P, 4, n, n), dtype=torch.complex128, device=device) if scattering else None
     nbytes = ctx.lib.wb_solve_rebuild_workspace_size(P, L, hu, hv)
     ws = _workspace(nbytes, device)
+    _offdiag_ok(epxy, epx)
     ctx.check(ctx.lib.wb_solve_rebuild(ctx.handle, P, L, hu, hv, _ptr(pts), _ptr(thick),
                                        _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                       _ptr(epxy), _ptr(epxy),
                                        _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
                                        _ptr(ws), nbytes, _stream(device)), 'solve_rebuild')
     return (eff, amp, pe, flags, smat) if scattering else (eff, amp, pe, flags)
@@ -398,7 +450,8 @@ def layer_modes_bytes(P, L, hu, hv, mode, rebuild=False):
     return int(load_library().wb_layer_modes_workspace_size(P, L, hu, hv, mode, int(bool(rebuild))))
 
 
-def layer_modes(pts, conv_idx, epx, epy, epz_i, hu, hv, mode, rebuild=False, workspace=None):
+def layer_modes(pts, conv_idx, epx, epy, epz_i, hu, hv, mode, rebuild=False, workspace=None,
+                epxy=None):
     """The raw layer modes of every (point, layer) pair; see ``wb_layer_modes``. ``conv_idx`` is
     (P, L). Returns (W, V, q, flags): (P L, n, n), (P L, n, n), (P L, n), (P, L)."""
     device = epx.device
@@ -417,8 +470,10 @@ def layer_modes(pts, conv_idx, epx, epy, epz_i, hu, hv, mode, rebuild=False, wor
     nbytes = ctx.lib.wb_layer_modes_workspace_size(P, L, hu, hv, mode, int(bool(rebuild)))
     if workspace is None or workspace.numel() < nbytes:
         workspace = _workspace(nbytes, device)
+    _offdiag_ok(epxy, epx)
     ctx.check(ctx.lib.wb_layer_modes(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(conv_idx),
-                                     _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(W), _ptr(V), _ptr(q),
+                                     _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(epxy), _ptr(epxy),
+                                     _ptr(W), _ptr(V), _ptr(q),
                                      _ptr(flags), int(bool(rebuild)), _ptr(workspace), nbytes,
                                      _stream(device)), 'layer_modes')
     return W, V, q, flags
@@ -620,7 +675,7 @@ def solve_noalloc(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, out=None)
     return eff, amp, pe, flags
 
 
-def solve_scattering(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode):
+def solve_scattering(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, epxy=None):
     """``solve`` plus the stack's scattering matrix; see ``wb_solve_scattering``.
     Returns (eff, amp, pe, flags, smat) with smat (P, 4, n, n): S11, S12, S21, S22."""
     device = epx.device
@@ -640,8 +695,10 @@ def solve_scattering(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode):
     smat = torch.empty((P, 4, n, n), dtype=torch.complex128, device=device)
     nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
     ws = _workspace(nbytes, device)
+    _offdiag_ok(epxy, epx)
     ctx.check(ctx.lib.wb_solve_scattering(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
                                           _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                          _ptr(epxy), _ptr(epxy),
                                           _ptr(eff), _ptr(amp), _ptr(pe), _ptr(flags), _ptr(smat),
                                           _ptr(ws), nbytes, _stream(device)), 'solve_scattering')
     return eff, amp, pe, flags, smat

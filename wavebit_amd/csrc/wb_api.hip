@@ -1,6 +1,8 @@
 // C ABI of the wavebit_amd engine: contexts, argument checks, workspace carving and the
 // launches behind each entry point declared in include/wavebit_amd.h.
+#include <algorithm>
 #include <climits>
+#include <cmath>
 #include <string>
 #include <cstring>
 
@@ -180,6 +182,64 @@ int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, i
                     "wb_assemble_laurent");
 }
 
+size_t wb_assemble_nvf_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15) return 0;
+  return wb_assemble_nvf_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                    const double* lattice, double* epx, double* epy, double* epz_i, double* epxy,
+                    void* workspace, size_t workspace_bytes, int* info, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15 || !info)
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf: bad dimensions (need L, Ny, Nx >= 1, 0 <= hu, hv <= 15)");
+  if (!eps || !epx || !epy || !epz_i || !epxy)
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf: null buffer");
+  if (workspace_bytes < wb_assemble_nvf_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_nvf: workspace too small");
+  // the raster lifted to 4 h + 1 samples per axis (factorization.py:811-827)
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const int Y = Ny * ((Ny < Kv) ? Kv / Ny + 1 : 1), X = Nx * ((Nx < Ku) ? Ku / Nx + 1 : 1);
+  // the cell basis, columns a1 = (a1x, a1y), a2 = (a2x, a2y); square pixels when NULL
+  double a1x = 1.0, a1y = 0.0, a2x = 0.0, a2y = 1.0;
+  if (lattice) {
+    a1x = lattice[0]; a1y = lattice[1]; a2x = lattice[2]; a2y = lattice[3];
+    if (!(a1x * a2y - a2x * a1y > 0.0))
+      return fail(ctx, WB_BADARG, "wb_assemble_nvf: the lattice columns must form a right-handed basis");
+  } else {
+    a1x = (double)X; a2y = (double)Y;
+  }
+  double metric[6] = {1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+  int cst = 0;
+  double ux = 0.0, uy = 0.0;
+  if (Y == 1 || X == 1) {
+    // a one-cell axis: the normal along b1 (one row) or b2 (one column), rows of the basis
+    // inverse (factorization.py:1337-1375)
+    const double det = a1x * a2y - a2x * a1y;
+    const double b0 = (Y == 1) ? a2y / det : -a1y / det;
+    const double b1 = (Y == 1) ? -a2x / det : a1x / det;
+    const double h = std::hypot(b0, b1);
+    cst = 1; ux = b0 / h; uy = b1 / h;
+  } else {
+    // index steps in units of one u-step, and J^-T (factorization.py:1110-1133)
+    double j00 = a1x / X, j10 = a1y / X, j01 = a2x / Y, j11 = a2y / Y;
+    const double scale = std::hypot(j00, j10);
+    j00 /= scale; j10 /= scale; j01 /= scale; j11 /= scale;
+    const double step_j = std::hypot(j01, j11), step_i = std::hypot(j00, j10);
+    const bool same = std::fabs(step_j - step_i) <= 1e-12 * std::max(std::fabs(step_j), std::fabs(step_i));
+    const double det = j00 * j11 - j01 * j10;
+    metric[0] = j11 / det; metric[1] = -j10 / det; metric[2] = -j01 / det; metric[3] = j00 / det;
+    metric[4] = same ? 1.0 : step_j;
+    metric[5] = same ? 1.0 : step_i;
+  }
+  return hip_status(ctx,
+                    wb_assemble_nvf_launch((const cd*)eps, L, Ny, Nx, hu, hv, metric, cst, ux, uy,
+                                           (cd*)epx, (cd*)epy, (cd*)epz_i, (cd*)epxy, workspace, info,
+                                           (hipStream_t)stream),
+                    "wb_assemble_nvf");
+}
+
 size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode) {
   if (P <= 0 || L <= 0 || hu < 0 || hv < 0 || mode < 0 || mode > 2) return 0;
   return wb_solve_workspace_bytes(P, L, hu, hv, mode);
@@ -187,7 +247,8 @@ size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode) {
 
 int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
              const double* thick, const int* conv_idx, const double* epx, const double* epy,
-             const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+             const double* epz_i, const double* epxy,
+             const double* epyx, double* eff, double* amp, double* pe, int* flags,
              void* workspace, size_t workspace_bytes, void* stream) {
   int s = enter(ctx);
   if (s) return s;
@@ -201,6 +262,9 @@ int wb_solve(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* 
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
   a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  if ((epxy == nullptr) != (epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_solve: epxy and epyx are given together or not at all");
+  a.epxy = (const cd*)epxy; a.epyx = (const cd*)epyx;
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve");
 }
@@ -239,7 +303,8 @@ size_t wb_solve_rebuild_workspace_size(int P, int L, int hu, int hv) {
 
 int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pts,
                      const double* thick, const int* conv_idx, const double* epx, const double* epy,
-                     const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                     const double* epz_i, const double* epxy,
+             const double* epyx, double* eff, double* amp, double* pe, int* flags,
                      double* smat, void* workspace, size_t workspace_bytes, void* stream) {
   int s = enter(ctx);
   if (s) return s;
@@ -255,6 +320,9 @@ int wb_solve_rebuild(wb_ctx* ctx, int P, int L, int hu, int hv, const double* pt
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = WB_MODE_2D;
   a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  if ((epxy == nullptr) != (epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_solve_rebuild: epxy and epyx are given together or not at all");
+  a.epxy = (const cd*)epxy; a.epyx = (const cd*)epyx;
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
   a.graze = (char*)workspace + align_up(wb_solve_workspace_bytes(P, L, hu, hv, WB_MODE_2D));
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_rebuild");
@@ -268,7 +336,8 @@ size_t wb_layer_modes_workspace_size(int P, int L, int hu, int hv, int mode, int
 }
 
 int wb_layer_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
-                   const int* conv_idx, const double* epx, const double* epy, const double* epz_i,
+                   const int* conv_idx, const double* epx, const double* epy, const double* epz_i, const double* epxy,
+             const double* epyx,
                    double* W, double* V, double* q, int* flags, int rebuild, void* workspace,
                    size_t workspace_bytes, void* stream) {
   int s = enter(ctx);
@@ -285,6 +354,9 @@ int wb_layer_modes(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const do
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
   a.pts = pts; a.thick = nullptr; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  if ((epxy == nullptr) != (epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_layer_modes: epxy and epyx are given together or not at all");
+  a.epxy = (const cd*)epxy; a.epyx = (const cd*)epyx;
   a.eff = nullptr; a.amp = nullptr; a.pe = nullptr; a.flags = flags; a.ck = nullptr;
   a.modes_out = 1; a.modes_W = (cd*)W; a.modes_V = (cd*)V; a.modes_q = (cd*)q;
   if (rebuild && mode == WB_MODE_2D)
@@ -440,7 +512,8 @@ int wb_field_sample(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const d
 
 int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                         const double* thick, const int* conv_idx, const double* epx,
-                        const double* epy, const double* epz_i, double* eff, double* amp, double* pe,
+                        const double* epy, const double* epz_i, const double* epxy,
+             const double* epyx, double* eff, double* amp, double* pe,
                         int* flags, double* smat, void* workspace, size_t workspace_bytes,
                         void* stream) {
   int s = enter(ctx);
@@ -455,6 +528,9 @@ int wb_solve_scattering(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, con
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
   a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  if ((epxy == nullptr) != (epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_solve_scattering: epxy and epyx are given together or not at all");
+  a.epxy = (const cd*)epxy; a.epyx = (const cd*)epyx;
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = nullptr; a.smat = (cd*)smat;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_scattering");
 }

@@ -636,3 +636,354 @@ hipError_t wb_assemble_laurent_adjoint_launch(const cd* eps, int L, int Ny, int 
   WB_CHECK_LAUNCH();
   return hipSuccess;
 }
+
+// ------------------------------------------------------------------- normal-vector-field rule
+// NVF with the default construction (reference factorization.py:811-878 the floor lift,
+// :1054-1404 the normal field from the periodic distance map, :1934-1966 the projector series,
+// :2461-2540 the closing, :2796-2815 the layer's builder): per layer
+//   E = [[eps]], A = [[1/eps]]^-1, Delta = E - A, C_ab = [[n_a n_b]] (point-sample series of the
+//   normal field on the raster lifted to 4 h + 1 samples per axis),
+//   epx = E - (Delta C_xx + C_xx Delta)/2, epy = E - (Delta C_yy + C_yy Delta)/2,
+//   epxy = epyx = (Delta C_xy + C_xy Delta)/2, epz_i = E^-1.
+// The normal field: the interface mask (pixels differing from a 4-neighbour, wrapping), the
+// exact periodic Euclidean distance to it in two separable passes (per column the nearest mask
+// row in either direction around the period, then per row the lower envelope over every
+// column, brute force from LDS), its wrapping central differences carried to Cartesian
+// components by the cell's J^-T, normalised where the length is at least 1e-9. A raster with a
+// one-cell axis (after the lift) has the constant normal the host passes.
+namespace {
+
+constexpr double NVF_FLAT_PIXEL = 1e-9;      // factorization.py:1462-1464
+
+// Point-sample series along one axis, orders -2h..2h: sum_j (a_j / n) w^{kj} times the
+// half-pixel phase e^{-i pi k / n} (factorization.py:322-338, 398-440): no box transform and no
+// base split.
+__device__ inline cd point_series_lds(const cd* s, const cd* tw, int n, int k) {
+  double sr = 0.0, si = 0.0;
+  const double inv_n = 1.0 / (double)n;
+  const int kmod = ((k % n) + n) % n;
+  int idx = 0;
+  for (int j = 0; j < n; ++j) {
+    const cd d = cscale(s[j], inv_n);
+    const cd w = tw[idx];
+    sr += d.x * w.x - d.y * w.y;
+    si += d.x * w.y + d.y * w.x;
+    idx += kmod;
+    if (idx >= n) idx -= n;
+  }
+  double sn, c;
+  sincospi(-(double)k / (double)n, &sn, &c);
+  return cmul(mkc(sr, si), mkc(c, sn));
+}
+
+// The same sum with the twiddles evaluated per term (series longer than SERIES_LDS_MAX).
+__device__ inline cd point_series_global(const cd* g, long long stride, int n, int k) {
+  double sr = 0.0, si = 0.0;
+  const double inv_n = 1.0 / (double)n;
+  for (int j = 0; j < n; ++j) {
+    const cd d = cscale(g[(long long)j * stride], inv_n);
+    long long idx = ((long long)k * j) % n;
+    if (idx < 0) idx += n;
+    double s, c;
+    sincospi(-2.0 * (double)idx / (double)n, &s, &c);
+    sr += d.x * c - d.y * s;
+    si += d.x * s + d.y * c;
+  }
+  double sn, c;
+  sincospi(-(double)k / (double)n, &sn, &c);
+  return cmul(mkc(sr, si), mkc(c, sn));
+}
+
+// lines[g][ju][kv] = v-series of column ju of grid g (g over the S grids of Ny x Nx samples)
+__global__ __launch_bounds__(64) void k_point_v(const cd* f, int Ny, int Nx, int hv, cd* lines) {
+  extern __shared__ cd sm[];
+  const int ju = blockIdx.x, g = blockIdx.y;
+  const int Kv = 4 * hv + 1;
+  const cd* col = f + (long long)g * Ny * Nx + ju;
+  cd* out = lines + ((long long)g * Nx + ju) * Kv;
+  if (Ny <= SERIES_LDS_MAX) {
+    stage_series(col, Nx, Ny, false, sm, sm + Ny);
+    __syncthreads();
+    for (int kk = threadIdx.x; kk < Kv; kk += blockDim.x)
+      out[kk] = point_series_lds(sm, sm + Ny, Ny, kk - 2 * hv);
+  } else {
+    for (int kk = threadIdx.x; kk < Kv; kk += blockDim.x)
+      out[kk] = point_series_global(col, Nx, Ny, kk - 2 * hv);
+  }
+}
+
+// c2d[g][kv][ku] = u-series over ju of lines[g][ju][kv]
+__global__ __launch_bounds__(64) void k_point_u(const cd* lines, int Nx, int hu, int hv, cd* c2d) {
+  extern __shared__ cd sm[];
+  const int kv = blockIdx.x, g = blockIdx.y;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const cd* row = lines + ((long long)g * Nx) * Kv + kv;
+  cd* out = c2d + ((long long)g * Kv + kv) * Ku;
+  if (Nx <= SERIES_LDS_MAX) {
+    stage_series(row, Kv, Nx, false, sm, sm + Nx);
+    __syncthreads();
+    for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x)
+      out[kk] = point_series_lds(sm, sm + Nx, Nx, kk - 2 * hu);
+  } else {
+    for (int kk = threadIdx.x; kk < Ku; kk += blockDim.x)
+      out[kk] = point_series_global(row, Kv, Nx, kk - 2 * hu);
+  }
+}
+
+__device__ inline bool cne(cd a, cd b) { return a.x != b.x || a.y != b.y; }
+
+// Interface mask of the lifted raster (Ny rv x Nx ru; sample (J, I) holds eps[J / rv][I / ru]):
+// a sample differing from one of its four neighbours, neighbours wrapping
+// (factorization.py:1467-1481).
+__global__ void k_nvf_mask(const cd* eps, int L, int Ny, int Nx, int rv, int ru, unsigned char* mask) {
+  const int Y = Ny * rv, X = Nx * ru;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L * Y * X) return;
+  const int l = (int)(t / ((long long)Y * X));
+  const int J = (int)((t / X) % Y), I = (int)(t % X);
+  const cd* e = eps + (long long)l * Ny * Nx;
+  auto at = [&](int j, int i) { return e[(long long)(j / rv) * Nx + i / ru]; };
+  const cd v = at(J, I);
+  const bool m = cne(v, at((J + 1) % Y, I)) || cne(v, at((J + Y - 1) % Y, I)) ||
+                 cne(v, at(J, (I + 1) % X)) || cne(v, at(J, (I + X - 1) % X));
+  mask[t] = m ? 1 : 0;
+}
+
+// Column pass: g[J][I] = (sv d)^2 with d the distance, around the period, from row J to the
+// nearest mask row of column I (+inf for a column without one). One thread per (layer, column):
+// a forward and a backward sweep over two laps of the column.
+__global__ void k_nvf_edt_cols(const unsigned char* mask, int L, int Y, int X, double sv, double* g) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L * X) return;
+  const int l = (int)(t / X), I = (int)(t % X);
+  const unsigned char* m = mask + (long long)l * Y * X + I;
+  double* out = g + (long long)l * Y * X + I;
+  const int NONE = -1;
+  int last = NONE;
+  for (int s = 0; s < 2 * Y; ++s) {                 // forward: nearest mask at or above
+    const int J = s < Y ? s : s - Y;
+    if (m[(long long)J * X]) last = s;
+    if (s >= Y) out[(long long)J * X] = (last == NONE) ? HUGE_VAL : (double)(s - last);
+  }
+  last = NONE;
+  for (int s = 2 * Y - 1; s >= 0; --s) {            // backward: nearest mask at or below
+    const int J = s < Y ? s : s - Y;
+    if (m[(long long)J * X]) last = s;
+    if (s < Y) {
+      const double up = (last == NONE) ? HUGE_VAL : (double)(last - s);
+      const double d = fmin(out[(long long)J * X], up);
+      if (d == HUGE_VAL) {
+        out[(long long)J * X] = HUGE_VAL;
+      } else {
+        const double a = d * sv;
+        out[(long long)J * X] = __dmul_rn(a, a);
+      }
+    }
+  }
+}
+
+// Row pass: dist[J][I] = sqrt(min over I' of g[J][I'] + (su d(I, I'))^2), d around the period,
+// the sum in the order (v term) + (u term) without contraction. One workgroup per (layer, row),
+// the row's g in LDS when it fits.
+__global__ __launch_bounds__(256) void k_nvf_edt_rows(const double* g, int L, int Y, int X, double su,
+                                                      double* dist) {
+  extern __shared__ double gs[];
+  const int J = blockIdx.x, l = blockIdx.y;
+  const double* row = g + ((long long)l * Y + J) * X;
+  const bool lds = X <= 8192;
+  if (lds)
+    for (int i = threadIdx.x; i < X; i += blockDim.x) gs[i] = row[i];
+  __syncthreads();
+  const double* src = lds ? gs : row;
+  for (int I = threadIdx.x; I < X; I += blockDim.x) {
+    double best = HUGE_VAL;
+    for (int i2 = 0; i2 < X; ++i2) {
+      const double gv = src[i2];
+      if (gv == HUGE_VAL) continue;
+      int d = I - i2;
+      if (d < 0) d = -d;
+      if (X - d < d) d = X - d;
+      const double a = (double)d * su;
+      const double v = __dadd_rn(gv, __dmul_rn(a, a));
+      best = fmin(best, v);
+    }
+    dist[((long long)l * Y + J) * X + I] = (best == HUGE_VAL) ? HUGE_VAL : sqrt(best);
+  }
+}
+
+// The normal field and its three products n_x^2, n_x n_y, n_y^2 as complex samples:
+// prod[slot][l][J][I]. cst: the constant normal (ux, uy) of a raster with a one-cell axis.
+// A layer with no interface (every distance infinite) gets the zero field.
+__global__ void k_nvf_normal(const double* dist, int L, int Y, int X, double jt00, double jt01,
+                             double jt10, double jt11, int cst, double ux, double uy, cd* prod) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)L * Y * X;
+  if (t >= plane) return;
+  double nx, ny;
+  if (cst) {
+    nx = ux; ny = uy;
+  } else {
+    const int l = (int)(t / ((long long)Y * X));
+    const int J = (int)((t / X) % Y), I = (int)(t % X);
+    const double* d = dist + (long long)l * Y * X;
+    const double dJp = d[(long long)((J + 1) % Y) * X + I], dJm = d[(long long)((J + Y - 1) % Y) * X + I];
+    const double dIp = d[(long long)J * X + (I + 1) % X], dIm = d[(long long)J * X + (I + X - 1) % X];
+    if (dJp == HUGE_VAL || dJm == HUGE_VAL || dIp == HUGE_VAL || dIm == HUGE_VAL) {
+      nx = 0.0; ny = 0.0;
+    } else {
+      const double gj = (dJp - dJm) / 2.0, gi = (dIp - dIm) / 2.0;
+      const double gx = __dadd_rn(__dmul_rn(jt00, gi), __dmul_rn(jt01, gj));
+      const double gy = __dadd_rn(__dmul_rn(jt10, gi), __dmul_rn(jt11, gj));
+      double norm = hypot(gx, gy);
+      if (norm < NVF_FLAT_PIXEL) norm = 1.0;
+      nx = gx / norm; ny = gy / norm;
+    }
+  }
+  prod[t] = mkc(__dmul_rn(nx, nx), 0.0);
+  prod[plane + t] = mkc(__dmul_rn(nx, ny), 0.0);
+  prod[2 * plane + t] = mkc(__dmul_rn(ny, ny), 0.0);
+}
+
+// D = E - A in place over n complex entries (D holds A)
+__global__ void k_sub_from(const cd* E, cd* D, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) D[i] = csub(E[i], D[i]);
+}
+
+// out = E - (X + Y) / 2 (sub), or (X + Y) / 2 (!sub), over n complex entries
+__global__ void k_nvf_close(const cd* E, const cd* X, const cd* Y, int sub, cd* out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cd h = cscale(cadd(X[i], Y[i]), 0.5);
+  out[i] = sub ? csub(E[i], h) : h;
+}
+
+// active[l] = 1 when layer l holds a value that is not a passive dielectric (Re eps <= 0 or
+// Im eps < 0): the reference floors the singular values of [[1/eps]] there
+// (factorization.py:1968-2160), which this builder does not.
+__global__ void k_nvf_passive(const cd* eps, int L, long long per, int* active) {
+  const int l = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per) return;
+  const cd v = eps[(long long)l * per + i];
+  if (!(v.x > 0.0 && v.y >= 0.0)) atomicOr(active + l, 1);
+}
+
+}  // namespace
+
+size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const size_t rv = (Ny < (int)Kv) ? Kv / Ny + 1 : 1, ru = (Nx < (int)Ku) ? Ku / Nx + 1 : 1;
+  const size_t Y = Ny * rv, X = Nx * ru, P = (size_t)L * Y * X;
+  size_t b = 0;
+  b += sizeof(cd) * 2 * L * Nx * Kv + 256;      // column series of eps and 1/eps
+  b += sizeof(cd) * 2 * L * Kv * Ku + 256;      // their 2D coefficients
+  b += P + 256;                                 // interface mask
+  b += sizeof(double) * 2 * P + 512;            // column pass, distances
+  b += sizeof(cd) * 3 * P + 256;                // the three projector products
+  b += sizeof(cd) * 3 * L * X * Kv + 256;       // their column series
+  b += sizeof(cd) * 3 * L * Kv * Ku + 256;      // their 2D coefficients
+  b += sizeof(cd) * 7 * L * N * N + 7 * 256;    // E, Delta, C_xx, C_xy, C_yy, two products
+  b += sizeof(int) * 2 * L * N + 256;           // pivots
+  return b + 1024;
+}
+
+hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                  const double* metric, int cst, double ux, double uy, cd* epx,
+                                  cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
+                                  hipStream_t stream) {
+  const double* jt = metric;
+  const double step_v = metric[4], step_u = metric[5];
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const int rv = (Ny < Kv) ? Kv / Ny + 1 : 1, ru = (Nx < Ku) ? Ku / Nx + 1 : 1;
+  const int Y = Ny * rv, X = Nx * ru;
+  const long long P = (long long)L * Y * X;
+  const long long NN = (long long)N * N;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
+  cd* lines = (cd*)take(sizeof(cd) * 2 * (size_t)L * Nx * Kv);
+  cd* c2d = (cd*)take(sizeof(cd) * 2 * (size_t)L * Kv * Ku);
+  unsigned char* mask = (unsigned char*)take((size_t)P);
+  double* gcol = (double*)take(sizeof(double) * (size_t)P);
+  double* dist = (double*)take(sizeof(double) * (size_t)P);
+  cd* prod = (cd*)take(sizeof(cd) * 3 * (size_t)P);
+  cd* plines = (cd*)take(sizeof(cd) * 3 * (size_t)L * X * Kv);
+  cd* pc2d = (cd*)take(sizeof(cd) * 3 * (size_t)L * Kv * Ku);
+  cd* E = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  cd* D = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  cd* C = (cd*)take(sizeof(cd) * 3 * (size_t)L * NN);
+  cd* T1 = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  cd* T2 = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  int* ipiv = (int*)take(sizeof(int) * 2 * (size_t)L * N);
+  const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0);
+  const dim3 blk(256);
+  auto grid1 = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
+  hipError_t e;
+  {
+    WbStageTimer timer(WB_T_ASSEMBLE, stream, 0.0, WB_STAGE_BOTH);
+    // E and [[1/eps]] from the raster itself (a block repeat has the same staircase series)
+    if ((e = series_v_launch(eps, L, Ny, Nx, hv, lines, stream)) != hipSuccess) return e;
+    if ((e = series_2d_launch(lines, L, Nx, hu, hv, c2d, stream)) != hipSuccess) return e;
+    if ((e = series_2d_launch(lines + (size_t)L * Nx * Kv, L, Nx, hu, hv, c2d + (size_t)L * Kv * Ku,
+                              stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), blk, 0, stream, c2d, L, hu,
+                       hv, E, T1, (cd*)nullptr);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), L), blk, 0, stream,
+                       c2d + (size_t)L * Kv * Ku, L, hu, hv, T2, (cd*)nullptr, (cd*)nullptr);
+    WB_CHECK_LAUNCH();
+    // the normal field on the lifted raster and its three products
+    if (!cst) {
+      hipLaunchKernelGGL(k_nvf_mask, grid1(P), blk, 0, stream, eps, L, Ny, Nx, rv, ru, mask);
+      WB_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_nvf_edt_cols, grid1((long long)L * X), blk, 0, stream, mask, L, Y, X, step_v,
+                         gcol);
+      WB_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_nvf_edt_rows, dim3(Y, L), blk, X <= 8192 ? X * sizeof(double) : 0, stream, gcol,
+                         L, Y, X, step_u, dist);
+      WB_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_nvf_normal, grid1(P), blk, 0, stream, dist, L, Y, X, jt[0], jt[1], jt[2], jt[3],
+                       cst, ux, uy, prod);
+    WB_CHECK_LAUNCH();
+    // C_xx, C_xy, C_yy: point-sample series, then the direct rule's layout
+    const size_t lv = Y <= SERIES_LDS_MAX ? 2 * Y * sizeof(cd) : 0;
+    hipLaunchKernelGGL(k_point_v, dim3(X, 3 * L), dim3(64), lv, stream, prod, Y, X, hv, plines);
+    WB_CHECK_LAUNCH();
+    const size_t lu = X <= SERIES_LDS_MAX ? 2 * X * sizeof(cd) : 0;
+    hipLaunchKernelGGL(k_point_u, dim3(Kv, 3 * L), dim3(64), lu, stream, plines, X, hu, hv, pc2d);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), 3 * L), blk, 0, stream, pc2d,
+                       3 * L, hu, hv, C, (cd*)nullptr, (cd*)nullptr);
+    WB_CHECK_LAUNCH();
+  }
+  // info: [0, L) the inversions of E, [L, 2L) those of [[1/eps]], [2L, 3L) non-passive layers
+  if ((e = wb_zero_int_launch(info + 2 * L, L, stream)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nvf_passive, dim3((unsigned)(((long long)Ny * Nx + 255) / 256), L), blk, 0,
+                     stream, eps, L, (long long)Ny * Nx, info + 2 * L);
+  WB_CHECK_LAUNCH();
+  // epz_i = E^-1 (from the copy in T1), A = [[1/eps]]^-1 into D, Delta = E - A
+  if ((e = wb_inverse_launch(N, bp_strided(T1, NN), N, bp_strided(epz_i, NN), N, ipiv, info, L, stream)))
+    return e;
+  if ((e = wb_inverse_launch(N, bp_strided(T2, NN), N, bp_strided(D, NN), N, ipiv + (size_t)L * N,
+                             info + L, L, stream)))
+    return e;
+  hipLaunchKernelGGL(k_sub_from, grid1((long long)L * NN), blk, 0, stream, E, D, (long long)L * NN);
+  WB_CHECK_LAUNCH();
+  // the closing, one projector at a time: T1 = Delta C, T2 = C Delta, then the slot
+  cd* outs[3] = {epx, epxy, epy};
+  const int subs[3] = {1, 0, 1};
+  for (int s = 0; s < 3; ++s) {
+    const cd* Cs = C + (size_t)s * L * NN;
+    if ((e = wb_zgemm_launch(N, N, N, one, bp_strided(D, NN), N, bp_strided(Cs, NN), N, zero,
+                             bp_strided(T1, NN), N, L, stream))) return e;
+    if ((e = wb_zgemm_launch(N, N, N, one, bp_strided(Cs, NN), N, bp_strided(D, NN), N, zero,
+                             bp_strided(T2, NN), N, L, stream))) return e;
+    hipLaunchKernelGGL(k_nvf_close, grid1((long long)L * NN), blk, 0, stream, E, T1, T2, subs[s], outs[s],
+                       (long long)L * NN);
+    WB_CHECK_LAUNCH();
+  }
+  return hipSuccess;
+}

@@ -64,11 +64,12 @@ __device__ inline double block_max(double x, double* sh) {
   return r;
 }
 
-// Dense Omega_L = [[-Cc, B], [-D, E]] and Omega_R = [[-KxKy, Kx^2 - epy], [epx - Ky^2, KyKx]] of
-// each pair: Omega_L at olr + b * 2 n^2, Omega_R n^2 after it (B = Kx epz_i Kx - I,
-// Cc = Kx epz_i Ky, D = Ky epz_i Ky - I, E = Ky epz_i Kx; solve_2d.py:2377-2392, 2474-2476).
-__global__ void k_omega_lr(int N, int L, BatchPtr epx, BatchPtr epy, BatchPtr epz, const cd* ov,
-                           cd* olr) {
+// Dense Omega_L = [[-Cc, B], [-D, E]] and Omega_R = [[-KxKy + epyx, Kx^2 - epy],
+// [epx - Ky^2, KyKx - epxy]] of each pair (the off-diagonal slots only when offd): Omega_L at
+// olr + b * 2 n^2, Omega_R n^2 after it (B = Kx epz_i Kx - I, Cc = Kx epz_i Ky,
+// D = Ky epz_i Ky - I, E = Ky epz_i Kx; solve_2d.py:2377-2411, 2474-2476).
+__global__ void k_omega_lr(int N, int L, BatchPtr epx, BatchPtr epy, BatchPtr epz, BatchPtr epxy,
+                           BatchPtr epyx, int offd, const cd* ov, cd* olr) {
   const long long b = blockIdx.y;
   const int p = (int)(b / L);
   const int n = 2 * N;
@@ -90,10 +91,16 @@ __global__ void k_omega_lr(int N, int L, BatchPtr epx, BatchPtr epy, BatchPtr ep
   OL[(long long)(N + i) * n + j] = cneg(Dm);
   OL[(long long)(N + i) * n + N + j] = cmul(cmul(kyi, z), kxj);
   const cd ex = epx.ptr(b)[e], ey = epy.ptr(b)[e];
-  OR[(long long)i * n + j] = dg ? cneg(cmul(kxi, kyi)) : zero;
+  cd o00 = dg ? cneg(cmul(kxi, kyi)) : zero;
+  cd o11 = dg ? cmul(kyi, kxi) : zero;
+  if (offd) {
+    o00 = cadd(o00, epyx.ptr(b)[e]);
+    o11 = csub(o11, epxy.ptr(b)[e]);
+  }
+  OR[(long long)i * n + j] = o00;
   OR[(long long)i * n + N + j] = dg ? csub(cmul(kxi, kxi), ey) : cneg(ey);
   OR[(long long)(N + i) * n + j] = dg ? csub(ex, cmul(kyi, kyi)) : ex;
-  OR[(long long)(N + i) * n + N + j] = dg ? cmul(kyi, kxi) : zero;
+  OR[(long long)(N + i) * n + N + j] = o11;
 }
 
 // Orthonormal basis (n x K, row-major, columns >= k zero) of columns idx[0..k) of the n x n
@@ -396,7 +403,8 @@ wb_graze_ws wb_graze_carve(void* ws, int B, int n) {
 }
 
 hipError_t wb_graze_stage(int N, int L, int B, const cd* w, const cd* W, BatchPtr epx,
-                          BatchPtr epy, BatchPtr epz, const cd* ov, cd* olr, cd* H, cd* Z, cd* X,
+                          BatchPtr epy, BatchPtr epz, BatchPtr epxy, BatchPtr epyx, int offd,
+                          const cd* ov, cd* olr, cd* H, cd* Z, cd* X,
                           cd* Vh, cd* wh, cd* tau, void* rotws, const wb_graze_ws& g,
                           hipStream_t st) {
   const int n = 2 * N;
@@ -405,7 +413,7 @@ hipError_t wb_graze_stage(int N, int L, int B, const cd* w, const cd* W, BatchPt
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0);
   hipError_t e;
   hipLaunchKernelGGL(k_omega_lr, dim3((unsigned)(((long long)N * N + 255) / 256), (unsigned)B),
-                     dim3(256), 0, st, N, L, epx, epy, epz, ov, olr);
+                     dim3(256), 0, st, N, L, epx, epy, epz, epxy, epyx, offd, ov, olr);
   WB_CHECK_LAUNCH();
   const BatchPtr OLp = bp_strided(olr, 2 * nn), ORp = bp_strided(olr + nn, 2 * nn);
   if ((e = wb_zgemm_launch(n, n, n, one, ORp, n, OLp, n, zero, bp_strided(H, nn), n, B, st))) return e;

@@ -53,6 +53,14 @@ size_t wb_assemble_laurent_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu
 hipError_t wb_assemble_laurent_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
                                               const cd* epz_i, const cd* d_epx, const cd* d_epy,
                                               const cd* d_epz, cd* d_eps, void* ws, hipStream_t stream);
+size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+// metric (host): the cell's J^-T (row-major 2 x 2) in units of one u-step, then the v- and
+// u-steps in those units; cst: a one-cell axis after the lift, whose constant normal is
+// (ux, uy). info: 3 L ints.
+hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                  const double* metric, int cst, double ux, double uy, cd* epx,
+                                  cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
+                                  hipStream_t stream);
 size_t wb_assemble_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
                                       const cd* epx, const cd* epz_i, const cd* d_epx,
@@ -67,6 +75,11 @@ struct wb_solve_args {
   const cd* epx;            // C x N x N
   const cd* epy;
   const cd* epz;
+  // NVF layers (2D path): the off-diagonal slots epxy / epyx (C x N x N each, gathered by
+  // conv_idx like the others; the same array twice for a symmetric projector), or nullptr for
+  // the diagonal rules (Li, Laurent), whose off-diagonal slots are zero.
+  const cd* epxy = nullptr;
+  const cd* epyx = nullptr;
   double* eff;              // P x 3 x 4 x N
   cd* amp;                  // P x 3 x 4 x N
   double* pe;               // P x 3

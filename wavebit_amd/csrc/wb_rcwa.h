@@ -91,8 +91,10 @@ struct wb_graze_ws {
 };
 size_t wb_graze_bytes(int B, int n);
 wb_graze_ws wb_graze_carve(void* ws, int B, int n);
+// offd: the NVF off-diagonal slots epxy / epyx enter Omega_R (else they are zero and unread).
 hipError_t wb_graze_stage(int N, int L, int B, const cd* w, const cd* W, BatchPtr epx,
-                          BatchPtr epy, BatchPtr epz, const cd* ov, cd* olr, cd* H, cd* Z, cd* X,
+                          BatchPtr epy, BatchPtr epz, BatchPtr epxy, BatchPtr epyx, int offd,
+                          const cd* ov, cd* olr, cd* H, cd* Z, cd* X,
                           cd* Vh, cd* wh, cd* tau, void* rotws, const wb_graze_ws& g,
                           hipStream_t st);
 hipError_t wb_graze_patch(int N, int L, int B, const cd* ov, cd* bigW, cd* bigV, cd* q,

@@ -873,6 +873,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   BatchPtr EPX = bp_gather(a.epx, NN, a.conv_idx);
   BatchPtr EPY = bp_gather(a.epy, NN, a.conv_idx);
   BatchPtr EPZ = bp_gather(a.epz, NN, a.conv_idx);
+  // NVF: the off-diagonal slots (solve_2d.py:2393-2411); nullptr for the diagonal rules
+  const int offd = (a.epxy != nullptr && a.epyx != nullptr) ? 1 : 0;
+  BatchPtr EPXY = bp_gather(offd ? a.epxy : a.epx, NN, a.conv_idx);
+  BatchPtr EPYX = bp_gather(offd ? a.epyx : a.epx, NN, a.conv_idx);
   auto S = [&](cd* base, long long stride) { return bp_strided(base, stride); };
 
   // ---------------------------------------------------------------- layer modes
@@ -891,6 +895,13 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1], nn), N, EPX, N, one, S(M[0], nn), n, B, st))) return e;
     if ((e = wb_zgemm_launch(n, N, N, one, S(M[1] + (long long)n * N, nn), N, EPY, N, one,
                              S(M[0] + N, nn), n, B, st))) return e;
+    if (offd) {
+      // Omega2[:, 0:N] -= [Cc; D] epyx ; Omega2[:, N:2N] -= [B; E] epxy
+      if ((e = wb_zgemm_launch(n, N, N, mone, S(M[1] + (long long)n * N, nn), N, EPYX, N, one,
+                               S(M[0], nn), n, B, st))) return e;
+      if ((e = wb_zgemm_launch(n, N, N, mone, S(M[1], nn), N, EPXY, N, one, S(M[0] + N, nn), n, B,
+                               st))) return e;
+    }
     if ((e = eig())) return e;
     FLAG(B, 1, info[0], WB_FLAG_NOCONVERGE);
     if (a.flags && n <= EP_NMAX) {
@@ -905,14 +916,21 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     // (Omega_L | Omega_R), q (eigenvalues of H), tau and the rotation staging, all free until the
     // V GEMMs and the s/p rotation below
     if (graze) {
-      if ((e = wb_graze_stage(N, L, B, w, M[3], EPX, EPY, EPZ, ov, M6, M[0], M[1], M[2], M[5], q, tau,
-                              rotws, G, st))) return e;
+      if ((e = wb_graze_stage(N, L, B, w, M[3], EPX, EPY, EPZ, EPXY, EPYX, offd, ov, M6, M[0], M[1],
+                              M[2], M[5], q, tau, rotws, G, st))) return e;
     }
     // V GEMMs: top = -epy W_bot, bottom = epx W_top (into M4)
     if ((e = wb_zgemm_launch(N, n, N, mone, EPY, N, S(M[3] + (long long)N * n, nn), n, zero,
                              S(M[4], nn), n, B, st))) return e;
     if ((e = wb_zgemm_launch(N, n, N, one, EPX, N, S(M[3], nn), n, zero,
                              S(M[4] + (long long)N * n, nn), n, B, st))) return e;
+    if (offd) {
+      // top += epyx W_top, bottom -= epxy W_bot
+      if ((e = wb_zgemm_launch(N, n, N, one, EPYX, N, S(M[3], nn), n, one, S(M[4], nn), n, B,
+                               st))) return e;
+      if ((e = wb_zgemm_launch(N, n, N, mone, EPXY, N, S(M[3] + (long long)N * n, nn), n, one,
+                               S(M[4] + (long long)N * n, nn), n, B, st))) return e;
+    }
     if (a.modes_out) {
       // the split solve's first half: raw W, V, q to the caller, the rebuilt grazing columns over
       // them, and stop

@@ -120,14 +120,11 @@ class ConvCache:
         self.device = device
         self._store = {}
 
-    def get(self, key, eps, hu, hv, factorization='li'):
+    def get(self, key, eps, hu, hv, factorization='li', period=None):
         hit = self._store.get(key)
         if hit is not None:
             return hit
-        epx, epy, epz_i, info = _native.assemble(eps, hu, hv, factorization)
-        if int(info.abs().sum()) != 0:
-            raise FloatingPointError('a layer\'s convolution matrix is singular')
-        self._store[key] = (epx, epy, epz_i)
+        self._store[key] = assemble_set(eps, hu, hv, factorization, period)
         return self._store[key]
 
     def __len__(self):
@@ -135,6 +132,36 @@ class ConvCache:
 
     def clear(self):
         self._store.clear()
+
+
+def lattice_of(period):
+    """The cell basis (2 x 2, columns a1 and a2) of a period ``(Lx, Ly)`` or of a skewed cell's
+    ``(a1x, a2y, a1y, a2x)``: what the NVF normal field measures its pixels in (the reference
+    passes ``diag(period)`` for an axis-aligned cell, ``engine.py:1975-2006``)."""
+    a1x, a2y, a1y, a2x = cell_of(period)
+    return np.array([[a1x, a2x], [a1y, a2y]], dtype=np.float64)
+
+
+def assemble_set(eps, hu, hv, factorization='li', period=None):
+    """One raster stack's conv matrices on the device: ``(epx, epy, epz_i, epxy)``, ``epxy``
+    (the NVF off-diagonal slot, epxy = epyx) None for the diagonal rules. Raises
+    ``FloatingPointError`` when a layer's matrix is singular."""
+    lattice = lattice_of(period) if (factorization == 'nvf' and period is not None) else None
+    epx, epy, epz_i, epxy, info = _native.assemble(eps, hu, hv, factorization, lattice)
+    L = eps.shape[0]
+    inv_info = info[:2 * L] if factorization == 'nvf' else info
+    if int(inv_info.abs().sum()) != 0:
+        raise FloatingPointError('a layer\'s convolution matrix is singular')
+    return epx, epy, epz_i, epxy
+
+
+def offdiag_cat(sets):
+    """The NVF off-diagonal slots of several stacks' conv sets, concatenated as their epx are
+    (zeros for a stack of a diagonal rule), or None when no stack has one."""
+    if all(len(c) < 4 or c[3] is None for c in sets):
+        return None
+    return torch.cat([c[3] if len(c) > 3 and c[3] is not None else torch.zeros_like(c[0])
+                      for c in sets]).contiguous()
 
 
 def point_record(pt, path):
@@ -201,7 +228,7 @@ def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
     sub = Bucket(hu=bucket.hu, hv=bucket.hv, path=bucket.path, n_layers=bucket.n_layers,
                  points=[bucket.points[i] for i in bad])
     out = _native.solve_rebuild(*bucket_inputs(sub, conv, device), bucket.hu, bucket.hv,
-                                scattering=smat is not None)
+                                scattering=smat is not None, epxy=offdiag_cat(conv))
     host = [t.cpu().numpy() for t in out]
     for dst, src in zip((eff, amp, pe, flags), host[:4]):
         dst[bad] = src
@@ -226,7 +253,8 @@ def modes_key(bucket):
 def _pair_inputs(buckets, convs, device):
     """One pair per (bucket, point, layer): its point record, the index of its layer's conv
     matrices in the concatenation of every bucket's stacks, and per bucket the (P, L) table of
-    pair indices. Returns (pts (pairs, 16), conv_idx (pairs, 1), epx, epy, epz, tables)."""
+    pair indices. Returns (pts (pairs, 16), conv_idx (pairs, 1), epx, epy, epz, tables, epxy),
+    epxy None unless a stack holds NVF layers."""
     recs, cidx, tables = [], [], []
     base = 0
     for bucket, conv in zip(buckets, convs):
@@ -243,9 +271,10 @@ def _pair_inputs(buckets, convs, device):
     epx = torch.cat([c[0] for conv in convs for c in conv]).contiguous()
     epy = torch.cat([c[1] for conv in convs for c in conv]).contiguous()
     epz = torch.cat([c[2] for conv in convs for c in conv]).contiguous()
+    epxy = offdiag_cat([c for conv in convs for c in conv])
     pts = torch.from_numpy(np.array(recs, dtype=np.float64)).to(device)
     idx = torch.from_numpy(np.array(cidx, dtype=np.int32).reshape(-1, 1)).to(device)
-    return pts, idx, epx, epy, epz, tables
+    return pts, idx, epx, epy, epz, tables, epxy
 
 
 def window_modes(buckets, convs, device, rebuild=True, inputs=None):
@@ -257,15 +286,16 @@ def window_modes(buckets, convs, device, rebuild=True, inputs=None):
     ``tables[i][p, l]`` is the pair index of layer l of point p of bucket i."""
     b0 = buckets[0]
     mode = _MODE[b0.path]
-    pts, idx, epx, epy, epz, tables = inputs or _pair_inputs(buckets, convs, device)
-    W, V, q, flags = _native.layer_modes(pts, idx, epx, epy, epz, b0.hu, b0.hv, mode)
+    pts, idx, epx, epy, epz, tables, epxy = inputs or _pair_inputs(buckets, convs, device)
+    W, V, q, flags = _native.layer_modes(pts, idx, epx, epy, epz, b0.hu, b0.hv, mode, epxy=epxy)
     flags = flags.reshape(-1)
     N = (2 * b0.hu + 1) * (2 * b0.hv + 1)
     if rebuild and mode == _native.MODE_2D and 2 * N <= _native.GRAZE_NMAX:
         bad = torch.nonzero(flags & _native.FLAG_GRAZING).reshape(-1)
         if bad.numel():
             W2, V2, q2, f2 = _native.layer_modes(pts[bad].contiguous(), idx[bad].contiguous(), epx,
-                                                 epy, epz, b0.hu, b0.hv, mode, rebuild=True)
+                                                 epy, epz, b0.hu, b0.hv, mode, rebuild=True,
+                                                 epxy=epxy)
             W[bad], V[bad], q[bad], flags[bad] = W2, V2, q2, f2.reshape(-1)
     return W, V, q, flags, tables
 
@@ -377,16 +407,18 @@ def run_fields(bucket, conv, device):
 def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
     """Solve every point of a bucket; returns device tensors (eff, amp, pe, flags) and N.
 
-    ``conv`` is a list of (epx, epy, epz_i) per stack, each (L, N, N) on the device.
-    ``graph_slot`` (a two-element list [graph, last shapes], or None) holds a
-    ``graphs.SolveGraph`` reused while the shapes match: launch-bound sizes run eagerly the first
-    time, are captured the second time the same shapes come, and are replayed afterwards.
+    ``conv`` is a list of (epx, epy, epz_i, epxy) per stack, each (L, N, N) on the device (epxy
+    None for the diagonal rules). ``graph_slot`` (a two-element list [graph, last shapes], or
+    None) holds a ``graphs.SolveGraph`` reused while the shapes match: launch-bound sizes run
+    eagerly the first time, are captured the second time the same shapes come, and are replayed
+    afterwards (diagonal rules only).
     """
     P, L = len(bucket.points), bucket.n_layers
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
     pts_d, thick_d, idx_d, epx, epy, epz = bucket_inputs(bucket, conv, device)
+    epxy = offdiag_cat(conv)
     mode = _MODE[bucket.path]
-    if graph_slot is not None:
+    if graph_slot is not None and epxy is None:
         from . import graphs
         g = graph_slot[0]
         if g is not None and g.matches(pts_d, thick_d, epx, bucket.hu, bucket.hv, mode):
@@ -401,7 +433,7 @@ def run_bucket(bucket, conv, device, workspace=None, graph_slot=None):
             return eff, amp, pe, flags, N
         graph_slot[1] = key
     eff, amp, pe, flags = _native.solve(pts_d, thick_d, idx_d, epx, epy, epz, bucket.hu,
-                                        bucket.hv, mode, workspace)
+                                        bucket.hv, mode, workspace, epxy=epxy)
     return eff, amp, pe, flags, N
 
 
@@ -410,7 +442,8 @@ def run_bucket_scattering(bucket, conv, device):
     (eff, amp, pe, flags, N, smat) with smat (P, 4, n, n) on the device."""
     N = (2 * bucket.hu + 1) * (2 * bucket.hv + 1)
     eff, amp, pe, flags, smat = _native.solve_scattering(
-        *bucket_inputs(bucket, conv, device), bucket.hu, bucket.hv, _MODE[bucket.path])
+        *bucket_inputs(bucket, conv, device), bucket.hu, bucket.hv, _MODE[bucket.path],
+        epxy=offdiag_cat(conv))
     return eff, amp, pe, flags, N, smat
 
 

@@ -158,7 +158,7 @@ class _Resolved:
     hu: int
     hv: int
     uniform: bool
-    factorization: str = 'li'    # 'li' (the default) or 'laurent'
+    factorization: str = 'li'    # 'li' (the default), 'laurent' or 'nvf'
     coefficients: str = 'sampled'    # 'sampled' (the default on a raster) or 'integrated'
     scattering: bool = False         # Outputs(scattering=True): keep the stack's S-matrix
     fields: bool = False             # Outputs(fields=True): keep the in-layer field state
@@ -180,7 +180,8 @@ def _resolve(problem, numerics, execution):
         raise UnsupportedFeatureError(
             f'numerics.{bad[0]}={bad[1]!r} is not implemented by this engine (it implements the '
             'raster defaults: SMM, Li rules with fixed nesting, sampled or integrated '
-            "coefficients, and the opt-in factorization='laurent').",
+            "coefficients, and the opt-in factorization='laurent' and factorization='nvf' with "
+            "its default normal field ('distance') and projector ('linear')).",
             code='rcwa.unsupported', path=f'numerics.{bad[0]}')
     factorization = numerics.factorization or 'li'
     if factorization == 'laurent' and numerics.li_nesting is not None:
@@ -190,6 +191,20 @@ def _resolve(problem, numerics, execution):
             "nests no rules -- it takes the direct rule for every product, so there is no "
             "inverse for an order to matter to. It selects the order Li's rules are nested in, "
             "so it needs factorization='li'.", code='rcwa.unsupported', path='numerics.li_nesting')
+    if factorization == 'nvf' and numerics.li_nesting is not None:
+        raise UnsupportedFeatureError(
+            f"li_nesting={numerics.li_nesting!r} has no reader here: factorization='nvf' nests no "
+            "rules -- it takes the direct rule both ways and closes the discontinuity with the "
+            "normal-field projector. It selects the order Li's rules are nested in, so it needs "
+            "factorization='li'.", code='rcwa.unsupported', path='numerics.li_nesting')
+    for name, value in (('raster_normal_method', numerics.raster_normal_method),
+                        ('nvf_projector', numerics.nvf_projector)):
+        # reference engine.py:5807-5850: the normal-field options are read by NVF only
+        if value is not None and factorization != 'nvf':
+            raise UnsupportedFeatureError(
+                f"{name}={value!r} has no reader here: factorization={factorization!r} builds no "
+                "normal field. It selects the raster NVF construction, so it needs "
+                "factorization='nvf'.", code='rcwa.unsupported', path=f'numerics.{name}')
     if not problem.outputs.diffraction:
         raise UnsupportedFeatureError('Outputs(diffraction=False) leaves nothing this engine computes.',
                                       code='rcwa.unsupported', path='outputs.diffraction')
@@ -202,13 +217,20 @@ def _resolve(problem, numerics, execution):
             raise ValidationError('period must be two positive lengths.', code='stack.period',
                                   path='stack.period')
     skewed = len(period) == 4
-    if skewed and (numerics.factorization or 'li') != 'laurent':
+    if skewed and factorization == 'li':
         # reference: Li's inverse rules run along the lattice axes and are refused on a skewed
-        # cell; Laurent's rule names no axis (factorization.py:991-994)
+        # cell; Laurent's rule names no axis (factorization.py:991-994), and NVF's tensor is
+        # written in Cartesian components (factorization.py:1283-1295)
         raise UnsupportedFeatureError(
             "factorization='li' runs its inverse rules along the cell's axes, which a skewed "
-            "lattice does not have: pass factorization='laurent' on this cell.",
+            "lattice does not have: pass factorization='laurent' or 'nvf' on this cell.",
             code='rcwa.unsupported', path='numerics.factorization')
+    if skewed and factorization == 'nvf' and numerics.fourier_coefficients == 'integrated':
+        # reference factorization.py:3264-3268: ('integrated', 'nvf') is refused on a skewed cell
+        raise UnsupportedFeatureError(
+            "fourier_coefficients='integrated' with factorization='nvf' is not solved on a skewed "
+            "cell: use the sampled coefficients there.", code='rcwa.unsupported',
+            path='numerics.fourier_coefficients')
     sources = problem.excitation.sources
     if len(sources) != 1:
         raise UnsupportedFeatureError('One incident plane wave per problem is implemented.',
@@ -288,6 +310,19 @@ def _resolve(problem, numerics, execution):
         raise UnsupportedFeatureError('A skewed lattice is solved on the 2D coupled path: give a '
                                       'Jones polarization or phi, or more than one raster row.',
                                       code='rcwa.unsupported', path='stack.lattice')
+    if factorization == 'nvf' and not uniform:
+        # reference factorization.py:1968-2160: on a layer that is not a passive dielectric
+        # (Re eps <= 0 or Im eps < 0) the inverse rule [[1/eps]] is floored in its singular values
+        # before the closing; this engine builds the unfloored closing only, so it takes such a
+        # layer when its values are all equal (the floor provably never binds there)
+        for i, layer in enumerate(rasters):
+            passive = np.all((layer.real > 0) & (layer.imag >= 0))
+            if not passive and not np.all(layer == layer.flat[0]):
+                raise UnsupportedFeatureError(
+                    "factorization='nvf' on a patterned layer holding a metal or gain value (Re eps "
+                    "<= 0 or Im eps < 0) needs the singular-value floor of [[1/eps]], which this "
+                    "engine does not build: use factorization='li' or 'laurent' for this stack.",
+                    code='rcwa.unsupported', path=f'stack.layers[{i}].epsilon')
     ws = _wavelengths(problem)
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
@@ -406,14 +441,17 @@ class _Cache:
         return other
 
     @staticmethod
-    def key(rasters, hu, hv, factorization='li'):
+    def key(rasters, hu, hv, factorization='li', period=None):
+        # NVF's normal field measures the pixels in the cell's metric: the cell is part of the key
         return (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
-                rasters.shape, hu, hv, factorization)
+                rasters.shape, hu, hv, factorization,
+                _engine.cell_of(period) if factorization == 'nvf' else None)
 
-    def get(self, rasters, hu, hv, factorization='li'):
+    def get(self, rasters, hu, hv, factorization='li', period=None):
         import torch
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.conv.device)
-        return self.conv.get(self.key(rasters, hu, hv, factorization), eps, hu, hv, factorization)
+        return self.conv.get(self.key(rasters, hu, hv, factorization, period), eps, hu, hv,
+                             factorization, period)
 
 
 @dataclass(frozen=True)
@@ -506,6 +544,10 @@ def _run_live(prepared, problem, r, checks):
         raise UnsupportedFeatureError(
             'A differentiable run solves the requested point; detuning is applied to concrete '
             'runs only.', code='rcwa.gradient', path='numerics.detuning')
+    if r.factorization == 'nvf':
+        raise UnsupportedFeatureError(
+            "Gradients are implemented under factorization='li' and 'laurent'; an NVF stack is "
+            'solved with concrete values.', code='rcwa.gradient', path='numerics.factorization')
     if r.scattering or r.fields:
         what = 'scattering' if r.scattering else 'fields'
         raise UnsupportedFeatureError(
@@ -731,7 +773,7 @@ def _solve_resolved(r, cache):
     import torch
     device = torch.device(cache.conv.device)
     _engine._native.context(device)          # raises EngineUnavailable without a GPU or library
-    conv = cache.get(r.rasters, r.hu, r.hv, r.factorization)
+    conv = cache.get(r.rasters, r.hu, r.hv, r.factorization, r.period)
     bucket = _engine.Bucket(hu=r.hu, hv=r.hv, path=r.path, n_layers=r.rasters.shape[0])
     for p, wl in enumerate(r.wavelengths):
         bucket.points.append(_engine.Point(
@@ -758,7 +800,7 @@ def _solve_resolved(r, cache):
     elif _resident_modes(bucket):
         # the layer eigensystems stay on the device between runs: a run that changes only
         # thicknesses (or the polarization) takes them from the last run and skips the eigensolves
-        mkey = (_Cache.key(r.rasters, r.hu, r.hv, r.factorization), r.path,
+        mkey = (_Cache.key(r.rasters, r.hu, r.hv, r.factorization, r.period), r.path,
                 _engine.modes_key(bucket))
         kept = cache.modes[1] if cache.modes is not None and cache.modes[0] == mkey else None
         outs, modes = _engine.run_window([bucket], [[conv]], device, modes=kept)

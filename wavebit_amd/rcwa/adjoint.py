@@ -30,7 +30,7 @@ class RasterSolve(torch.autograd.Function):
         thick_d = thick.detach().to(torch.float64).contiguous()
         L = eps_c.shape[0]
         P = pts.shape[0]
-        epx, epy, epz, info = _native.assemble(eps_c, hu, hv, factorization)
+        epx, epy, epz, _, info = _native.assemble(eps_c, hu, hv, factorization)
         thick_pl = thick_d.reshape(1, L).expand(P, L).contiguous()
         conv_idx = torch.arange(L, dtype=torch.int32, device=eps_c.device).reshape(1, L).expand(P, L).contiguous()
         eff, amp, pe, flags, state = _native.solve_keep(pts, thick_pl, conv_idx, epx, epy, epz, hu, hv,

@@ -73,9 +73,10 @@ class Numerics:
     def unsupported(self):
         """The first non-default choice this engine does not implement, as (field, value)."""
         for name, ok in (('algorithm', ('smm',)), ('fourier_coefficients', (None, 'sampled', 'integrated')),
-                         ('factorization', (None, 'li', 'laurent')), ('formulation', ('auto', 'second_order')),
+                         ('factorization', (None, 'li', 'laurent', 'nvf')),
+                         ('formulation', ('auto', 'second_order')),
                          ('li_nesting', (None, 'fixed')),
-                         ('raster_normal_method', (None,)), ('nvf_projector', (None,)),
+                         ('raster_normal_method', (None, 'distance')), ('nvf_projector', (None, 'linear')),
                          ('shape_nvf_grid_size', (None,)), ('half_space_merge_tolerance', (0.0,)),
                          ('wall_resolution', (None,)), ('shape_normal_method', (None,)),
                          ('slicing', (None,))):

@@ -86,9 +86,10 @@ class ConvLRU:
         self.hits = 0
         self.misses = 0
 
-    def get(self, rasters, hu, hv, factorization='li'):
+    def get(self, rasters, hu, hv, factorization='li', period=None):
         key = (hashlib.blake2b(np.ascontiguousarray(rasters).tobytes(), digest_size=16).hexdigest(),
-               rasters.shape, hu, hv, factorization)
+               rasters.shape, hu, hv, factorization,
+               _engine.cell_of(period) if factorization == 'nvf' else None)
         hit = self._d.get(key)
         if hit is not None:
             self._d.move_to_end(key)
@@ -96,18 +97,21 @@ class ConvLRU:
             return hit
         self.misses += 1
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.device)
-        epx, epy, epz, info = _native.assemble(eps, hu, hv, factorization)
-        if int(info.abs().sum()) != 0:
+        try:
+            entry = _engine.assemble_set(eps, hu, hv, factorization, period)
+        except FloatingPointError:
             raise NumericalError('a layer convolution matrix is singular', code='rcwa.singular',
-                                 path='stack.layers')
-        entry = (epx, epy, epz)
-        size = 3 * epx.numel() * 16
+                                 path='stack.layers') from None
         self._d[key] = entry
-        self.bytes += size
+        self.bytes += self._size(entry)
         while self.bytes > self.budget and len(self._d) > 1:
             _, old = self._d.popitem(last=False)
-            self.bytes -= 3 * old[0].numel() * 16
+            self.bytes -= self._size(old)
         return entry
+
+    @staticmethod
+    def _size(entry):
+        return sum(t.numel() * 16 for t in entry if t is not None)
 
 
 class Server:
@@ -153,6 +157,10 @@ class Server:
             raise UnsupportedFeatureError(
                 'The server returns diffraction outputs; solve a field request with rcwa.run.',
                 code='rcwa.unsupported', path='outputs.fields')
+        if req.gradient and r.factorization == 'nvf':
+            raise UnsupportedFeatureError(
+                "Gradient requests are served under factorization='li' and 'laurent'.",
+                code='rcwa.gradient', path='numerics.factorization')
         if req.numerics.detuning:
             raise UnsupportedFeatureError(
                 'The server solves the requested points; run a request with detuning through '
@@ -162,7 +170,7 @@ class Server:
         if bad:
             raise ValidationError(f'runtime guard {bad[0]} failed', code=f'prepared.{bad[0]}',
                                   path='problem')
-        conv = self.conv.get(r.rasters, r.hu, r.hv, r.factorization)
+        conv = self.conv.get(r.rasters, r.hu, r.hv, r.factorization, r.period)
         work = _Work(req=req, resolved=r, n_points=len(r.wavelengths), t_admit=now)
         work.checks = checks
         pts = []
