@@ -168,15 +168,16 @@ int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, i
  *   epxy = (Delta C_xy + C_xy Delta)/2 (the solve's epxy and epyx slot alike), epz_i = E^-1.
  * lattice: host array (a1x, a1y, a2x, a2y), the cell basis the raster indexes (the normal is
  * taken in Cartesian components through the cell's metric), or NULL for square pixels in an
- * axis-aligned cell. hu, hv <= 15. info: 3 L device ints: the inversions of E, of [[1/eps]],
- * then 1 for a layer holding a value that is not a passive dielectric (Re eps <= 0 or
- * Im eps < 0), where the reference floors the singular values of [[1/eps]]
- * (factorization.py:1968-2160) and this builder does not: such layers' outputs are the
- * unfloored closing, which the caller refuses or accepts. */
+ * axis-aligned cell. floor = 1: on every layer holding a value that is not a passive dielectric
+ * (Re eps <= 0 or Im eps < 0), the singular values of [[1/eps]] below 1 / (2 max|eps|) are
+ * raised to it before the closing (factorization.py:1968-2160; one N x N Hermitian Schur
+ * decomposition per layer of the call, the passive layers' matrices unchanged); floor = 0 skips
+ * that work for a caller that knows every layer is passive. hu, hv <= 15. info: 3 L device ints:
+ * the inversions of E, of [[1/eps]], then 1 for a layer that is not a passive dielectric. */
 size_t wb_assemble_nvf_workspace_size(int L, int Ny, int Nx, int hu, int hv);
 int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
-                    const double* lattice, double* epx, double* epy, double* epz_i, double* epxy,
-                    void* workspace, size_t workspace_bytes, int* info, void* stream);
+                    const double* lattice, int floor, double* epx, double* epy, double* epz_i,
+                    double* epxy, void* workspace, size_t workspace_bytes, int* info, void* stream);
 
 /* Batch solve of P plane-wave points, each through L layers (bottom to top), top incidence.
  *   pts       P x WB_PT_STRIDE doubles (layout above)

@@ -129,23 +129,16 @@ def test_options_without_a_reader_are_refused(kw, path):
     assert e.value.code == 'rcwa.unsupported' and e.value.path == path
 
 
-def test_nvf_refuses_a_patterned_metal_layer_but_takes_a_uniform_one():
+def test_nvf_takes_metal_layers():
+    """Reference factorization.py:1968-2160: a layer that is not a passive dielectric is solved
+    with [[1/eps]]'s singular values floored (on the device); it is not refused."""
     eps = np.ones((6, 6), dtype=complex)
     eps[2:4, 2:4] = -20.0 + 1.0j
-    bad = rcwa.Problem(
+    metal = rcwa.Problem(
         stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.1, epsilon=eps)], top=wavebit_amd.Material(1.0),
                          bottom=wavebit_amd.Material(2.25), period=(0.5, 0.5)),
         excitation=rcwa.Excitation(wavelength=0.7, sources=(rcwa.PlaneWave(polarization='s'),)))
-    with pytest.raises(UnsupportedFeatureError) as e:
-        rcwa.prepare(bad, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'),
-                     backend='torch', device='cuda')
-    assert e.value.path == 'stack.layers[0].epsilon'
-    film = np.full((6, 6), -20.0 + 1.0j)
-    ok = rcwa.Problem(
-        stack=rcwa.Stack(layers=[rcwa.Layer(thickness=0.01, epsilon=film)], top=wavebit_amd.Material(1.0),
-                         bottom=wavebit_amd.Material(2.25), period=(0.5, 0.5)),
-        excitation=rcwa.Excitation(wavelength=0.7, sources=(rcwa.PlaneWave(polarization='s'),)))
-    assert rcwa.check(ok, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'),
+    assert rcwa.check(metal, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'),
                       backend='torch', device='cuda').ok
 
 

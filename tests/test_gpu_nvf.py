@@ -66,6 +66,42 @@ def test_nvf_assembly_reports_a_layer_that_is_not_a_passive_dielectric():
     assert info.cpu().numpy()[4:].tolist() == [0, 1]
 
 
+@pytest.mark.parametrize('shape, hu, values', [((16, 16), 3, (1.0, -25.0 + 1.5j)),
+                                               ((24, 24), 7, (2.25, -20.0 + 1.0j))])
+def test_nvf_floors_the_inverse_rule_of_a_metal_layer_like_the_oracle(shape, hu, values):
+    # a gold-like inclusion: [[1/eps]] has singular values below 1 / (2 max|eps|), which the
+    # closing raises to that floor (factorization.py:2046-2160); a passive layer beside it and a
+    # run without the floor stage show what the floor changes
+    eps = np.stack([_blobs(shape, 12, values), _blobs(shape, 13, (1.0, 4.0))])
+    lattice = np.eye(2)
+    d = torch.from_numpy(eps).to(DEV)
+    got = _native.assemble_nvf(d, hu, hu, lattice, floor=True)
+    raw = _native.assemble_nvf(d, hu, hu, lattice, floor=False)
+    want = nvf.nvf_conv_matrices(eps[0], hu, hu, lattice=lattice)
+    B = orc.conv_matrix(1.0 / eps[0], hu, hu)
+    s = np.linalg.svd(B, compute_uv=False)
+    assert np.any(s < 1.0 / (2.0 * np.abs(eps[0]).max()))          # the floor binds here
+    for g, w, slot in zip(got[:4], want[:4], ('epx', 'epy', 'epz_i', 'epxy')):
+        err = np.abs(g[0].cpu().numpy() - w).max()
+        assert err <= 1e-10 * max(1.0, np.abs(w).max()), (slot, err)
+    assert np.abs(raw[0][0].cpu().numpy() - want[0]).max() > 1e-6    # unfloored differs
+    for g, r in zip(got[:4], raw[:4]):
+        assert torch.equal(g[1], r[1])                                   # passive layer untouched
+
+
+def test_nvf_metal_stack_run_matches_the_oracle():
+    eps = np.stack([_blobs((16, 16), 14, (1.0, -25.0 + 1.5j)), _blobs((16, 16), 15, (2.25, 6.25))])
+    d, wls = [0.05, 0.12], [0.807, 0.85]
+    res = rcwa.run(_problem(eps, d, (0.6, 0.6), 's', wls, theta=0.1, phi=0.2),
+                   numerics=rcwa.Numerics(orders=(3, 3), factorization='nvf'), **RUN)
+    for i, wl in enumerate(wls):
+        want = orc.solve_raster(eps, d, 1.0, 2.25, (0.6, 0.6), wl, 0.1, 0.2, 's', (3, 3),
+                                factorization='nvf')['incident']
+        for k, got in (('R', res.R[i]), ('T', res.T[i])):
+            assert np.abs(np.asarray(got) - want[k]).max() < 1e-9, (k, wl)
+        assert 0.0 < want['R'].sum() + want['T'].sum() < 1.0
+
+
 def _problem(eps, d, period, pol, wls, theta=0.25, phi=0.6, top=1.0, bottom=2.25, outputs=None):
     layers = [rcwa.Layer(thickness=t, epsilon=e) for t, e in zip(d, eps)]
     cell = dict(lattice=period) if np.shape(period) == (2, 2) else dict(period=period)

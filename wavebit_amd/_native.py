@@ -89,8 +89,8 @@ def load_library():
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
                                  c_vp]),
             'wb_assemble_nvf_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
-            'wb_assemble_nvf': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_vp,
-                                        c_vp, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp]),
+            'wb_assemble_nvf': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_dp, c_int,
+                                        c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp, c_vp]),
             'wb_solve_rebuild_workspace_size': (c_sz, [c_int, c_int, c_int, c_int]),
             'wb_solve_rebuild': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
@@ -337,11 +337,12 @@ def assemble_laurent(eps, hu, hv):
     return out[0], out[1], out[2], info
 
 
-def assemble_nvf(eps, hu, hv, lattice=None):
+def assemble_nvf(eps, hu, hv, lattice=None, floor=True):
     """NVF convolution matrices of a raster stack eps (L, Ny, Nx) -> (epx, epy, epz_i, epxy,
     info): epxy is the solve's epxy and epyx slot alike; ``lattice`` the cell basis (2 x 2,
-    columns a1 and a2) or None for square pixels; info holds 3 L ints (the last L: layers that
-    are not passive dielectrics)."""
+    columns a1 and a2) or None for square pixels; ``floor``: floor [[1/eps]]'s singular values on
+    the layers that are not passive dielectrics (False skips the work for an all-passive stack);
+    info holds 3 L ints (the last L: layers that are not passive dielectrics)."""
     device = eps.device
     ctx = context(device)
     _require(eps, torch.complex128, device, 'eps')
@@ -355,13 +356,14 @@ def assemble_nvf(eps, hu, hv, lattice=None):
     if lattice is not None:
         A = np.asarray(lattice, dtype=np.float64).reshape(2, 2)
         cell = (ctypes.c_double * 4)(A[0, 0], A[1, 0], A[0, 1], A[1, 1])
-    ctx.check(ctx.lib.wb_assemble_nvf(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, cell, _ptr(out[0]),
+    ctx.check(ctx.lib.wb_assemble_nvf(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, cell, int(bool(floor)),
+                                      _ptr(out[0]),
                                       _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _ptr(ws), nbytes,
                                       _ptr(info), _stream(device)), 'assemble_nvf')
     return out[0], out[1], out[2], out[3], info
 
 
-def assemble(eps, hu, hv, factorization='li', lattice=None):
+def assemble(eps, hu, hv, factorization='li', lattice=None, floor=True):
     """The convolution matrices of ``factorization`` ('li', 'laurent' or 'nvf') for a raster
     stack: (epx, epy, epz_i, epxy, info), epxy None for the diagonal rules."""
     if factorization == 'laurent':
@@ -369,7 +371,7 @@ def assemble(eps, hu, hv, factorization='li', lattice=None):
     if factorization == 'li':
         return _with_offdiag(assemble_li(eps, hu, hv))
     if factorization == 'nvf':
-        return assemble_nvf(eps, hu, hv, lattice)
+        return assemble_nvf(eps, hu, hv, lattice, floor)
     raise ValueError(f'no assembly for factorization={factorization!r}')
 
 

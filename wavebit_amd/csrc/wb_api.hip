@@ -188,8 +188,8 @@ size_t wb_assemble_nvf_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
 }
 
 int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
-                    const double* lattice, double* epx, double* epy, double* epz_i, double* epxy,
-                    void* workspace, size_t workspace_bytes, int* info, void* stream) {
+                    const double* lattice, int floor, double* epx, double* epy, double* epz_i,
+                    double* epxy, void* workspace, size_t workspace_bytes, int* info, void* stream) {
   int s = enter(ctx);
   if (s) return s;
   if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15 || !info)
@@ -235,7 +235,7 @@ int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int h
   }
   return hip_status(ctx,
                     wb_assemble_nvf_launch((const cd*)eps, L, Ny, Nx, hu, hv, metric, cst, ux, uy,
-                                           (cd*)epx, (cd*)epy, (cd*)epz_i, (cd*)epxy, workspace, info,
+                                           floor ? 1 : 0, (cd*)epx, (cd*)epy, (cd*)epz_i, (cd*)epxy, workspace, info,
                                            (hipStream_t)stream),
                     "wb_assemble_nvf");
 }

@@ -869,6 +869,47 @@ __global__ void k_nvf_passive(const cd* eps, int L, long long per, int* active) 
   if (!(v.x > 0.0 && v.y >= 0.0)) atomicOr(active + l, 1);
 }
 
+// f[l] = 1 / (INVERSE_RULE_BOUND max|eps_l|) on a layer flagged active (not a passive
+// dielectric), 0 elsewhere (factorization.py:1968-2043: the floor of [[1/eps]]'s singular
+// values, bound 2). One workgroup per layer.
+__global__ __launch_bounds__(256) void k_nvf_floor_value(const cd* eps, long long per, const int* active,
+                                                         double* f) {
+  __shared__ double red[256];
+  const int l = blockIdx.x;
+  double m = 0.0;
+  for (long long i = threadIdx.x; i < per; i += blockDim.x) {
+    const cd v = eps[(long long)l * per + i];
+    m = fmax(m, hypot(v.x, v.y));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) f[l] = active[l] ? 1.0 / (2.0 * red[0]) : 0.0;
+}
+
+// c[l][k] = f / s_k - 1 for every singular value s_k = sqrt(eigenvalue k of B^H B) below the
+// layer's floor f, else 0: B + (B Z) diag(c) Z^H is U max(S, f) V^H with V = Z.
+__global__ void k_nvf_floor_gain(int N, int L, const cd* w, const double* f, double* c) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L * N) return;
+  const int l = (int)(t / N);
+  const double s = sqrt(fmax(w[t].x, 0.0));
+  const double fl = f[l];
+  c[t] = (s < fl) ? fl / s - 1.0 : 0.0;
+}
+
+// Y[l][:, k] *= c[l][k]
+__global__ void k_nvf_scale_cols(int N, int L, cd* Y, const double* c) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L * N * N) return;
+  const long long l = t / ((long long)N * N);
+  const int k = (int)(t % N);
+  Y[t] = cscale(Y[t], c[l * N + k]);
+}
+
 }  // namespace
 
 size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
@@ -886,12 +927,17 @@ size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
   b += sizeof(cd) * 3 * L * Kv * Ku + 256;      // their 2D coefficients
   b += sizeof(cd) * 7 * L * N * N + 7 * 256;    // E, Delta, C_xx, C_xy, C_yy, two products
   b += sizeof(int) * 2 * L * N + 256;           // pivots
+  // the singular-value floor: eigenvalues, reflector scalars, rotation staging, statuses, the
+  // floor and the gains (its n x n scratch is the closing's buffers, free at that point)
+  b += sizeof(cd) * 2 * L * N + 512;
+  b += wb_hqr_ms_rot_bytes((int)N, L) + 256;
+  b += sizeof(int) * L + sizeof(double) * L + sizeof(double) * L * N + 768;
   return b + 1024;
 }
 
 hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
-                                  const double* metric, int cst, double ux, double uy, cd* epx,
-                                  cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
+                                  const double* metric, int cst, double ux, double uy, int floor,
+                                  cd* epx, cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
                                   hipStream_t stream) {
   const double* jt = metric;
   const double step_v = metric[4], step_u = metric[5];
@@ -917,6 +963,12 @@ hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, 
   cd* T1 = (cd*)take(sizeof(cd) * (size_t)L * NN);
   cd* T2 = (cd*)take(sizeof(cd) * (size_t)L * NN);
   int* ipiv = (int*)take(sizeof(int) * 2 * (size_t)L * N);
+  cd* fw = (cd*)take(sizeof(cd) * (size_t)L * N);
+  cd* ftau = (cd*)take(sizeof(cd) * (size_t)L * N);
+  void* frot = (void*)take(wb_hqr_ms_rot_bytes(N, L));
+  int* finfo = (int*)take(sizeof(int) * (size_t)L);
+  double* fval = (double*)take(sizeof(double) * (size_t)L);
+  double* fgain = (double*)take(sizeof(double) * (size_t)L * N);
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0);
   const dim3 blk(256);
   auto grid1 = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
@@ -955,14 +1007,38 @@ hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, 
     const size_t lu = X <= SERIES_LDS_MAX ? 2 * X * sizeof(cd) : 0;
     hipLaunchKernelGGL(k_point_u, dim3(Kv, 3 * L), dim3(64), lu, stream, plines, X, hu, hv, pc2d);
     WB_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), 3 * L), blk, 0, stream, pc2d,
-                       3 * L, hu, hv, C, (cd*)nullptr, (cd*)nullptr);
-    WB_CHECK_LAUNCH();
   }
   // info: [0, L) the inversions of E, [L, 2L) those of [[1/eps]], [2L, 3L) non-passive layers
   if ((e = wb_zero_int_launch(info + 2 * L, L, stream)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_nvf_passive, dim3((unsigned)(((long long)Ny * Nx + 255) / 256), L), blk, 0,
                      stream, eps, L, (long long)Ny * Nx, info + 2 * L);
+  WB_CHECK_LAUNCH();
+  if (floor) {
+    // [[1/eps]] (T2) with its singular values below 1 / (2 max|eps|) raised to that floor on the
+    // non-passive layers (factorization.py:2046-2160): the Schur vectors Z of the Hermitian
+    // B^H B are B's right singular vectors, and B + (B Z) diag(f / s - 1 where s < f) Z^H is
+    // U max(S, f) V^H; a layer without a value below the floor (every passive one) gets B back.
+    // Scratch: D (B^H B, then T), C (the Schur stage's W and X, then B Z), C + 2 L NN (Z).
+    hipLaunchKernelGGL(k_nvf_floor_value, dim3(L), dim3(256), 0, stream, eps, (long long)Ny * Nx,
+                       info + 2 * L, fval);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, one, bp_strided(T2, NN), N, bp_strided(T2, NN), N, zero,
+                                 bp_strided(D, NN), N, L, stream))) return e;
+    cd* Zs = C + 2 * (size_t)L * NN;
+    int zh = 0;
+    if ((e = wb_schur_launch(N, D, fw, C, Zs, C + (size_t)L * NN, ftau, frot, finfo, L, &zh, stream)))
+      return e;
+    hipLaunchKernelGGL(k_nvf_floor_gain, grid1((long long)L * N), blk, 0, stream, N, L, fw, fval, fgain);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_zgemm_ops_launch(0, zh ? 1 : 0, N, N, N, one, bp_strided(T2, NN), N, bp_strided(Zs, NN), N,
+                                 zero, bp_strided(C, NN), N, L, stream))) return e;
+    hipLaunchKernelGGL(k_nvf_scale_cols, grid1((long long)L * NN), blk, 0, stream, N, L, C, fgain);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_zgemm_ops_launch(0, zh ? 0 : 1, N, N, N, one, bp_strided(C, NN), N, bp_strided(Zs, NN), N,
+                                 one, bp_strided(T2, NN), N, L, stream))) return e;
+  }
+  hipLaunchKernelGGL(k_direct_bttb, dim3((unsigned)((NN + 255) / 256), 3 * L), blk, 0, stream, pc2d,
+                     3 * L, hu, hv, C, (cd*)nullptr, (cd*)nullptr);
   WB_CHECK_LAUNCH();
   // epz_i = E^-1 (from the copy in T1), A = [[1/eps]]^-1 into D, Delta = E - A
   if ((e = wb_inverse_launch(N, bp_strided(T1, NN), N, bp_strided(epz_i, NN), N, ipiv, info, L, stream)))

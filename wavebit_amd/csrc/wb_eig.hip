@@ -642,3 +642,27 @@ hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, vo
   WB_TIMED(WB_T_COLNORM, stream, hipLaunchKernelGGL(k_colnorm, dim3(batch), dim3(NT), 0, stream, n, W));
   return hipGetLastError();
 }
+
+// Schur form only: T in A (upper triangular), the Schur vectors in Z (conjugate-transposed, Z^H,
+// when *zh is set on return), eigenvalues in w. W and X are n x n scratch per matrix.
+hipError_t wb_schur_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
+                           int batch, int* zh, hipStream_t stream) {
+  *zh = 0;
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipError_t z = wb_zero_int_launch(info, batch, stream);
+  if (z != hipSuccess) return z;
+  const bool blocked = n >= WB_BLOCKED_HESSENBERG_MIN;
+  if (blocked) {
+    hipError_t e = wb_gehrd_blocked(n, A, W, X, batch, stream);
+    if (e != hipSuccess) return e;
+    e = wb_unghr_blocked(n, W, X, Z, batch, stream);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(k_gehrd, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, A, tau);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_unghr, dim3(batch), dim3(NTH), sizeof(cd) * (size_t)n, stream, n, A, tau, Z);
+    WB_CHECK_LAUNCH();
+  }
+  *zh = blocked ? 1 : 0;
+  return wb_hqr_ms_launch(n, A, Z, blocked ? 1 : 0, w, info, rot_ws, batch, stream);
+}

@@ -40,6 +40,10 @@ void* wb_noalloc_scratch(int device);
 hipError_t wb_zgeev_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
                            int batch, hipStream_t stream);
 size_t wb_hqr_ms_rot_bytes(int n, int batch);
+// The Schur form alone: T in A, the Schur vectors in Z (as Z^H when *zh is set on return);
+// W, X: n x n scratch per matrix, tau: n per matrix.
+hipError_t wb_schur_launch(int n, cd* A, cd* w, cd* W, cd* Z, cd* X, cd* tau, void* rot_ws, int* info,
+                           int batch, int* zh, hipStream_t stream);
 // zh: Z is held conjugate-transposed (Z^H) when nonzero.
 hipError_t wb_hqr_ms_launch(int n, cd* H, cd* Z, int zh, cd* w, int* info, void* rot_ws, int batch,
                             hipStream_t stream);
@@ -57,9 +61,11 @@ size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 // metric (host): the cell's J^-T (row-major 2 x 2) in units of one u-step, then the v- and
 // u-steps in those units; cst: a one-cell axis after the lift, whose constant normal is
 // (ux, uy). info: 3 L ints.
+// floor: floor the singular values of [[1/eps]] on the non-passive layers (one N x N Hermitian
+// Schur decomposition per layer of the call).
 hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
-                                  const double* metric, int cst, double ux, double uy, cd* epx,
-                                  cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
+                                  const double* metric, int cst, double ux, double uy, int floor,
+                                  cd* epx, cd* epy, cd* epz_i, cd* epxy, void* ws, int* info,
                                   hipStream_t stream);
 size_t wb_assemble_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 hipError_t wb_assemble_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,

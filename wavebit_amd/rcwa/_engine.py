@@ -120,11 +120,11 @@ class ConvCache:
         self.device = device
         self._store = {}
 
-    def get(self, key, eps, hu, hv, factorization='li', period=None):
+    def get(self, key, eps, hu, hv, factorization='li', period=None, host=None):
         hit = self._store.get(key)
         if hit is not None:
             return hit
-        self._store[key] = assemble_set(eps, hu, hv, factorization, period)
+        self._store[key] = assemble_set(eps, hu, hv, factorization, period, host)
         return self._store[key]
 
     def __len__(self):
@@ -142,12 +142,15 @@ def lattice_of(period):
     return np.array([[a1x, a2x], [a1y, a2y]], dtype=np.float64)
 
 
-def assemble_set(eps, hu, hv, factorization='li', period=None):
+def assemble_set(eps, hu, hv, factorization='li', period=None, host=None):
     """One raster stack's conv matrices on the device: ``(epx, epy, epz_i, epxy)``, ``epxy``
-    (the NVF off-diagonal slot, epxy = epyx) None for the diagonal rules. Raises
-    ``FloatingPointError`` when a layer's matrix is singular."""
+    (the NVF off-diagonal slot, epxy = epyx) None for the diagonal rules. ``host``: the rasters
+    as a host array, which decides whether NVF needs its singular-value floor (a value that is
+    not a passive dielectric); without it the floor stage runs. Raises ``FloatingPointError``
+    when a layer's matrix is singular."""
     lattice = lattice_of(period) if (factorization == 'nvf' and period is not None) else None
-    epx, epy, epz_i, epxy, info = _native.assemble(eps, hu, hv, factorization, lattice)
+    floor = host is None or not bool(np.all((host.real > 0) & (host.imag >= 0)))
+    epx, epy, epz_i, epxy, info = _native.assemble(eps, hu, hv, factorization, lattice, floor)
     L = eps.shape[0]
     inv_info = info[:2 * L] if factorization == 'nvf' else info
     if int(inv_info.abs().sum()) != 0:

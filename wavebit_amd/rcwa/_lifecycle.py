@@ -310,19 +310,6 @@ def _resolve(problem, numerics, execution):
         raise UnsupportedFeatureError('A skewed lattice is solved on the 2D coupled path: give a '
                                       'Jones polarization or phi, or more than one raster row.',
                                       code='rcwa.unsupported', path='stack.lattice')
-    if factorization == 'nvf' and not uniform:
-        # reference factorization.py:1968-2160: on a layer that is not a passive dielectric
-        # (Re eps <= 0 or Im eps < 0) the inverse rule [[1/eps]] is floored in its singular values
-        # before the closing; this engine builds the unfloored closing only, so it takes such a
-        # layer when its values are all equal (the floor provably never binds there)
-        for i, layer in enumerate(rasters):
-            passive = np.all((layer.real > 0) & (layer.imag >= 0))
-            if not passive and not np.all(layer == layer.flat[0]):
-                raise UnsupportedFeatureError(
-                    "factorization='nvf' on a patterned layer holding a metal or gain value (Re eps "
-                    "<= 0 or Im eps < 0) needs the singular-value floor of [[1/eps]], which this "
-                    "engine does not build: use factorization='li' or 'laurent' for this stack.",
-                    code='rcwa.unsupported', path=f'stack.layers[{i}].epsilon')
     ws = _wavelengths(problem)
     return _Resolved(rasters=rasters, thickness=thickness, eps_top=eps_top, eps_bottom=eps_bottom,
                      period=period, wavelengths=ws, source=src, polarization=pol, path=path,
@@ -451,7 +438,7 @@ class _Cache:
         import torch
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.conv.device)
         return self.conv.get(self.key(rasters, hu, hv, factorization, period), eps, hu, hv,
-                             factorization, period)
+                             factorization, period, rasters)
 
 
 @dataclass(frozen=True)

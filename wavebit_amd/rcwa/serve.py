@@ -98,7 +98,7 @@ class ConvLRU:
         self.misses += 1
         eps = torch.from_numpy(np.ascontiguousarray(rasters)).to(self.device)
         try:
-            entry = _engine.assemble_set(eps, hu, hv, factorization, period)
+            entry = _engine.assemble_set(eps, hu, hv, factorization, period, rasters)
         except FloatingPointError:
             raise NumericalError('a layer convolution matrix is singular', code='rcwa.singular',
                                  path='stack.layers') from None
