@@ -147,7 +147,8 @@ def test_detuning_ladder_moves_a_refused_graze_off_the_condition():
 
 def test_scattering_output_at_a_graze_is_rebuilt():
     """Outputs(scattering=True) at a pillar graze: the rebuilt solve's specular columns are the
-    reported amplitudes, and the efficiencies equal the plain rebuilt run's to the bit."""
+    reported amplitudes, and the efficiencies equal the plain rebuilt run's (within rounding: the
+    plain solve's reflection-matrix cascade orders the same algebra differently)."""
     stack = _stack(1.5)
     wl = PILLAR_GRAZES[0]
     plain, _ = _run(stack, [wl])
@@ -156,7 +157,8 @@ def test_scattering_output_at_a_graze_is_rebuilt():
                         outputs=rcwa.Outputs(scattering=True))
     res = rcwa.run(prob, numerics=rcwa.Numerics(orders=(5, 1)), **RUN)
     assert res.diagnostics['exceptional_points']['grazing_rebuilt_layer_solves'] == 1
-    assert np.array_equal(res.R, plain.R) and np.array_equal(res.T, plain.T)
+    # at the graze itself the two orderings part at ~3e-11: the parity bar, not the bit
+    assert np.abs(res.R - plain.R).max() < 1e-10 and np.abs(res.T - plain.T).max() < 1e-10
     S = res.scattering[0]
     N = 33
     s_inc = res.diagnostics['incidence_variants']['s_inc']

@@ -37,7 +37,10 @@ def _device_blocks(eps, d, top, bot, period, wl, theta, phi, pol, M):
     assert int((flags & 3).sum()) == 0          # no failed eigensolve or factorization
     eff0, amp0, pe0, flags0, _ = eng.run_bucket(b, conv, DEV)
     v = 3 if path == eng.PATH_2D else 1          # the 1D paths fill incidence variant 0 only
-    assert torch.equal(eff[:, :v], eff0[:, :v]) and torch.equal(amp[:, :v], amp0[:, :v])
+    # the scattering pass forms the layer S-matrices; the plain solve's reflection-matrix cascade
+    # is the same algebra in another order
+    assert (eff[:, :v] - eff0[:, :v]).abs().max() < 1e-12
+    assert (amp[:, :v] - amp0[:, :v]).abs().max() < 1e-12 * max(1.0, float(amp0.abs().max()))
     return dict(zip(KEYS, smat[0].cpu().numpy())), path
 
 
@@ -106,7 +109,7 @@ def test_run_keeps_the_scattering_matrix_and_it_cascades():
                      backend='torch', device='cuda:0')
     with pytest.raises(UnsupportedFeatureError):
         plain.scattering
-    assert np.array_equal(plain.R, whole.R)
+    assert np.abs(plain.R - whole.R).max() < 1e-12
 
 
 def test_gradient_runs_and_the_server_refuse_the_scattering_output():

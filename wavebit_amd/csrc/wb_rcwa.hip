@@ -357,6 +357,132 @@ __global__ void k_scale_rows_x(int n, cd* E, const cd* xv) {
   *x = cmul(xv[b * n + r], *x);
 }
 
+// ------------------------------------------------------------ the reflection-matrix cascade
+// The layer S-matrices and their Redheffer stars fused into one recursion on the reflection
+// matrix Gamma of the structure below each gap (the same matrices as s11 + s12 G (I - s11 G)^-1
+// s12 with the gap-embedded s11, s12 of scattering.py:163-200, by the interface relations those
+// are built from). Per layer, with A = W^-1 Wg + V^-1 Vg, B = W^-1 Wg - V^-1 Vg (the gap to the
+// layer's modes) and A' = Wg^-1 W + Vg^-1 V, B' = Wg^-1 W - Vg^-1 V (the layer's modes to the gap):
+//   R_b = (B + A G)(A + B G)^-1  (the modal reflection at the layer's lower face),
+//   R_t = X R_b X,  G' = (B' + A' R_t)(A' + B' R_t)^-1.
+// It is carried transposed (G^T), so the two right divisions are left solves on the factors of
+// P^T = A^T + G^T B^T and P'^T = A'^T + R_t^T B'^T, and each pair of products is one GEMM launch.
+// The transmitted gap waves follow from d_l = 4 P_l^-1 X_l P'_l^-1 d_{l+1}, taken in conjugated
+// variables so the transposed factors solve them with the conjugate-transposed solve.
+
+// A'^T | B'^T (n x 2n, row stride 2n) from the s/p-basis W, V of each pair, through 32 x 32 LDS
+// tiles so reads and writes are both coalesced. Gap: as k_ab.
+__global__ __launch_bounds__(256) void k_apbp_t(int N, int n, int L, int mode, const int* ppt,
+                                                const cd* W, const cd* V, const cd* ov, cd* out) {
+  __shared__ cd ta[32][33];
+  __shared__ cd tb[32][33];
+  const long long b = blockIdx.z;
+  const int p = ppt ? ppt[b] : (int)(b / L);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  const cd* Wb = W + b * (long long)n * n;
+  const cd* Vb = V + b * (long long)n * n;
+  cd* D = out + b * 2LL * n * n;
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;    // tile of the output (transposed)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {                         // source row i = c0 + k, column r0 + tx
+    const int i = c0 + k, j = r0 + tx;
+    if (i < n && j < n) {
+      cd wg, vg;
+      if (mode == 0) {
+        const double y = o[OV_Y * N + (i % N)].x;
+        if (i < N) { wg = mkc(1.0, 0.0); vg = mkc(0.0, -y); }
+        else { wg = mkc(0.0, -y); vg = mkc(1.0, 0.0); }
+      } else {
+        const double y = o[OV_Y * N + i].x;
+        wg = mkc(1.0, 0.0); vg = mkc(0.0, -y);
+      }
+      const cd Pw = cdiv(Wb[(long long)i * n + j], wg), Qv = cdiv(Vb[(long long)i * n + j], vg);
+      ta[k][tx] = cadd(Pw, Qv);
+      tb[k][tx] = csub(Pw, Qv);
+    }
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {                         // output row r0 + k, column c0 + tx
+    const int r = r0 + k, c = c0 + tx;
+    if (r < n && c < n) {
+      D[(long long)r * 2 * n + c] = ta[tx][k];
+      D[(long long)r * 2 * n + n + c] = tb[tx][k];
+    }
+  }
+}
+
+// A^T, B^T of each pair from Wh = W^-H, Vh = V^-H (A^T[i][j] = wg_i conj(Wh[i][j]) +
+// vg_i conj(Vh[i][j]), B^T with the minus sign).
+__global__ void k_ab_t(int N, int n, int L, int mode, const int* ppt, const cd* Wh, const cd* Vh,
+                       const cd* ov, cd* At, cd* Bt) {
+  const long long b = blockIdx.y;
+  const int p = ppt ? ppt[b] : (int)(b / L);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n);
+  const cd* o = ov + (long long)p * OV_COUNT * N;
+  cd wg, vg;
+  if (mode == 0) {
+    const double y = o[OV_Y * N + (r % N)].x;
+    if (r < N) { wg = mkc(1.0, 0.0); vg = mkc(0.0, -y); }
+    else { wg = mkc(0.0, -y); vg = mkc(1.0, 0.0); }
+  } else {
+    const double y = o[OV_Y * N + r].x;
+    wg = mkc(1.0, 0.0); vg = mkc(0.0, -y);
+  }
+  const long long idx = b * (long long)n * n + e;
+  const cd P = cmul(wg, cconj(Wh[idx])), Q = cmul(vg, cconj(Vh[idx]));
+  At[idx] = cadd(P, Q);
+  Bt[idx] = csub(P, Q);
+}
+
+// d1 += s1, d2 += s2 over n x n per batch element (rows strided by the given leading dimensions)
+__global__ void k_add_two(int n, BatchPtr d1, int ld1, BatchPtr s1, int lds1, BatchPtr d2, int ld2,
+                          BatchPtr s2, int lds2) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  cd* x = d1.ptr(b) + (long long)r * ld1 + c;
+  *x = cadd(*x, s1.ptr(b)[(long long)r * lds1 + c]);
+  cd* y = d2.ptr(b) + (long long)r * ld2 + c;
+  *y = cadd(*y, s2.ptr(b)[(long long)r * lds2 + c]);
+}
+
+// R[r][c] *= x_r x_c (x of the batch element's pair)
+__global__ void k_xscale2(int n, BatchPtr R, BatchPtr xv) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const cd* x = xv.ptr(b);
+  cd* p = R.ptr(b) + e;
+  *p = cmul(cmul(x[r], *p), x[c]);
+}
+
+// v[r][j] = scale * (xv ? conj(x_r) : 1) * conj?(v[r][j]) over n x nrhs per batch element
+__global__ void k_vec_conj_scale(int n, int nrhs, BatchPtr v, BatchPtr xv, double scale, int conj_in) {
+  const long long b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nrhs) return;
+  const int r = e / nrhs;
+  cd* p = v.ptr(b) + e;
+  cd t = conj_in ? cconj(*p) : *p;
+  if (xv.base) t = cmul(cconj(xv.ptr(b)[r]), t);
+  *p = cscale(t, scale);
+}
+
+// M^T = I - G^T diag(t11): the transposed final-interface matrix from the transposed Gamma_L,
+// into T (row stride n).
+__global__ void k_i_minus_t(int n, BatchPtr G, int ldg, const cd* dg, int dg_block, cd* T) {
+  const long long b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  const cd v = cmul(G.ptr(b)[(long long)r * ldg + c], dg[b * DG_COUNT * n + dg_block * n + c]);
+  T[b * (long long)n * n + e] = csub(r == c ? mkc(1.0, 0.0) : mkc(0.0, 0.0), v);
+}
+
 // Gamma_0: full matrix with the diagonal of S_ref's 22 block.
 __global__ void k_gamma0(int n, const cd* dg, cd* G) {
   const long long p = blockIdx.y;
@@ -821,6 +947,9 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if (keep) CK = wb_ck_carve(a.ck, P, L, a.hu, a.hv, mode);
   const bool fields = a.field_amps != nullptr;
   if (fields && (a.smat || !a.field_ws)) return hipErrorInvalidValue;
+  // the plain solve takes the reflection-matrix cascade; the checkpointed forward, the scattering
+  // output and the field state keep the layer S-matrices their later passes read
+  const bool rmat = !keep && !a.smat && !fields;
   wb_field_ws FW{};
   if (fields) FW = wb_field_carve(a.field_ws, P, L, n, nrhs);
   hipError_t e;
@@ -984,11 +1113,19 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   }
 
   // ---------------------------------------------------------------- layer S-matrices
-  // Wi = bigW^-1 (M0), Vi = bigV^-1 (M3)
+  if (rmat) {
+    // A'^T | B'^T into M6 while W and V are still whole
+    const dim3 tg((n + 31) / 32, (n + 31) / 32, B);
+    hipLaunchKernelGGL(k_apbp_t, tg, dim3(256), 0, st, N, n, L, mode, ppt, bigW, bigV, ov, M6);
+    WB_CHECK_LAUNCH();
+  }
+  // Wi = bigW^-1 (M0), Vi = bigV^-1 (M3); the cascade's transposed form takes W^-H, V^-H
+  const int inv_conj = rmat ? 1 : 0;
   if ((e = wb_getrf_launch(n, S(bigW, nn), n, ipiv[0], info[1], B, st))) return e;
   FLAG(B, 1, info[1], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[0], nn), n, B, st))) return e;
-  if ((e = wb_getrs_blocked(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, 0, W_.luws, st))) return e;
+  if ((e = wb_getrs_blocked(n, n, S(bigW, nn), n, ipiv[0], S(M[0], nn), n, B, inv_conj, W_.luws, st)))
+    return e;
   if (keep) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[0], nn), n, S(CK.Wi, nn), n);
     WB_CHECK_LAUNCH();
@@ -996,11 +1133,23 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   if ((e = wb_getrf_launch(n, S(bigV, nn), n, ipiv[1], info[2], B, st))) return e;
   FLAG(B, 1, info[2], WB_FLAG_SINGULAR);
   if ((e = wb_set_identity_launch(n, S(M[3], nn), n, B, st))) return e;
-  if ((e = wb_getrs_blocked(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, 0, W_.luws, st))) return e;
+  if ((e = wb_getrs_blocked(n, n, S(bigV, nn), n, ipiv[1], S(M[3], nn), n, B, inv_conj, W_.luws, st)))
+    return e;
   if (keep) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[3], nn), n, S(CK.Vi, nn), n);
     WB_CHECK_LAUNCH();
   }
+  cd* xv = tau;   // the Householder scalars of the eigensolve are no longer needed
+  hipLaunchKernelGGL(k_xvec, dim3((n + 255) / 256, B), dim3(256), 0, st, n, L, ppt, q, a.pts, a.thick,
+                     xv);
+  WB_CHECK_LAUNCH();
+  const long long n2 = 2 * nn;
+  if (rmat) {
+    // A^T -> M4, B^T -> M5
+    hipLaunchKernelGGL(k_ab_t, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, ppt, M[0], M[3], ov, M[4],
+                       M[5]);
+    WB_CHECK_LAUNCH();
+  } else {
   // A -> M4, B -> M5
   hipLaunchKernelGGL(k_ab, grid2(nn, B), dim3(256), 0, st, N, n, L, mode, ppt, M[0], M[3], ov, M[4],
                      M[5]);
@@ -1014,10 +1163,6 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(k_ctrans, tgrid, dim3(256), 0, st, n, M[5], M[2], (const cd*)nullptr);
   WB_CHECK_LAUNCH();
   if ((e = wb_getrs_blocked(n, n, S(M[1], nn), n, ipiv[0], S(M[2], nn), n, B, 1, W_.luws, st))) return e;
-  cd* xv = tau;   // the Householder scalars of the eigensolve are no longer needed
-  hipLaunchKernelGGL(k_xvec, dim3((n + 255) / 256, B), dim3(256), 0, st, n, L, ppt, q, a.pts, a.thick,
-                     xv);
-  WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ctrans, tgrid, dim3(256), 0, st, n, M[2], M[3], (const cd*)xv);
   WB_CHECK_LAUNCH();
   // Mm = A - XYX B -> M0
@@ -1025,7 +1170,6 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   WB_CHECK_LAUNCH();
   if ((e = wb_zgemm_launch(n, n, n, mone, S(M[3], nn), n, S(M[5], nn), n, one, S(M[0], nn), n, B, st))) return e;
   // E1 = XYX A - B (cols 0..n-1 of M6), E2 = X (A - Y B) (cols n..2n-1)
-  const long long n2 = 2 * nn;
   hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M[5], nn), n, S(M6, n2), 2 * n);
   WB_CHECK_LAUNCH();
   if ((e = wb_zgemm_launch(n, n, n, one, S(M[3], nn), n, S(M[4], nn), n, mone, S(M6, n2), 2 * n, B, st))) return e;
@@ -1043,6 +1187,7 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, grid2(nn, B), dim3(256), 0, st, n, n, S(M6, n2), 2 * n, S(CK.S11, nn), n);
     WB_CHECK_LAUNCH();
   }
+  }   // the layer S-matrices (not formed on the reflection-matrix path)
 
   // ---------------------------------------------------------------- cascade
   // the bottom-lit recursion of the scattering output needs every layer's S11, which the
@@ -1064,7 +1209,49 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   WB_CHECK_LAUNCH();
   BatchPtr Gam = bp_strided(G0, nn);
   int ldg = n;
-  for (int l = 0; l < L; ++l) {
+  // two same-shape outputs of one GEMM launch: element 2j + k of the grid is operand k of pair j
+  auto pair_grid = [](BatchPtr v, long long second_off) {
+    BatchPtr g = v; g.inner = 2; g.s_inner = second_off; return g;
+  };
+  auto step_piv_p = [&](int l) { return ipiv[1] + (ragged ? (long long)rb.base[l] : (long long)l * P) * n; };
+  if (rmat) {
+    // G^T recursion: P^T = A^T + G^T B^T, Q^T = B^T + G^T A^T, R_b^T = P^-T Q^T, R_t^T = X R_b^T X,
+    // P'^T = A'^T + R_t^T B'^T, G'^T = P'^-T (B'^T + R_t^T A'^T). Per pair: A^T M4, B^T M5,
+    // A'^T | B'^T M6; LU(P^T) M1, LU(P'^T) M0 (kept for the transmitted sweep), G'^T M2, M3 scratch.
+    const long long off43 = M[4] - M[5], off31 = M[3] - M[1], off20 = M[2] - M[0];
+    for (int l = 0; l < L; ++l) {
+      const int Pl = step_points(l);
+      BatchPtr At = step_pairs(M[4], nn, l), Bt = step_pairs(M[5], nn, l);
+      BatchPtr Pt = step_pairs(M[1], nn, l), Qt = step_pairs(M[3], nn, l);
+      BatchPtr Ppt = step_pairs(M[0], nn, l), Gn = step_pairs(M[2], nn, l);
+      BatchPtr ApBp = step_pairs(M6, n2, l);                  // A'^T (cols 0..n-1), B'^T (n..2n-1)
+      BatchPtr xl = step_pairs(xv, n, l);
+      // [G^T B^T | G^T A^T] -> M1, M3
+      if ((e = wb_zgemm_launch(n, n, n, one, pair_grid(Gam, 0), ldg, pair_grid(Bt, off43), n, zero,
+                               pair_grid(Pt, off31), n, 2 * Pl, st))) return e;
+      hipLaunchKernelGGL(k_add_two, grid2(nn, Pl), dim3(256), 0, st, n, Pt, n, At, n, Qt, n, Bt, n);
+      WB_CHECK_LAUNCH();
+      int* pp = step_piv_p(l);
+      if ((e = wb_getrf_launch(n, Pt, n, pp, info[3], Pl, st))) return e;
+      if ((e = flag_points(info[3], Pl, WB_FLAG_SINGULAR))) return e;
+      if ((e = wb_getrs_blocked(n, n, Pt, n, pp, Qt, n, Pl, 0, W_.luws, st))) return e;
+      hipLaunchKernelGGL(k_xscale2, grid2(nn, Pl), dim3(256), 0, st, n, Qt, xl);
+      WB_CHECK_LAUNCH();
+      // [R_t^T B'^T | R_t^T A'^T] -> M0, M2
+      if ((e = wb_zgemm_launch(n, n, n, one, pair_grid(Qt, 0), n, pair_grid(bp_offset(ApBp, n), -(long long)n),
+                               2 * n, zero, pair_grid(Ppt, off20), n, 2 * Pl, st))) return e;
+      hipLaunchKernelGGL(k_add_two, grid2(nn, Pl), dim3(256), 0, st, n, Ppt, n, ApBp, 2 * n, Gn, n,
+                         bp_offset(ApBp, n), 2 * n);
+      WB_CHECK_LAUNCH();
+      int* ppp = step_piv(l);
+      if ((e = wb_getrf_launch(n, Ppt, n, ppp, info[3], Pl, st))) return e;
+      if ((e = flag_points(info[3], Pl, WB_FLAG_SINGULAR))) return e;
+      if ((e = wb_getrs_blocked(n, n, Ppt, n, ppp, Gn, n, Pl, 0, W_.luws, st))) return e;
+      Gam = Gn;
+      ldg = n;
+    }
+  }
+  for (int l = 0; l < (rmat ? 0 : L); ++l) {
     const int Pl = step_points(l);
     BatchPtr s11 = step_pairs(M6, n2, l);
     BatchPtr s12 = bp_offset(s11, n);
@@ -1091,16 +1278,23 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     ldg = 2 * n;
   }
   // ragged: each point's Gamma_{L_p} sits in its top layer's pair
-  if (ragged) Gam = bp_gather(M6, n2, last);
+  if (ragged) Gam = rmat ? bp_gather(M[2], nn, last) : bp_gather(M6, n2, last);
   // final interface: u = (I - t11 G_L)^-1 t12 e ; R = t22 e + t21 G_L u
   const int last_ld = ldg;
   // Gamma_0 is no longer needed, unless the field state reads it (then M3, free since the
   // layer S-matrices were formed, takes the factors)
   BatchPtr Tf = bp_strided(fields ? M[3] : G0, nn);
-  hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, Gam, last_ld, Tf, n);
-  WB_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tf, n, dg, (int)DG_T11);
-  WB_CHECK_LAUNCH();
+  if (rmat) {
+    // the transposed I - t11 Gamma_L, from G^T
+    hipLaunchKernelGGL(k_i_minus_t, grid2(nn, P), dim3(256), 0, st, n, Gam, last_ld, dg, (int)DG_T11,
+                       G0);
+    WB_CHECK_LAUNCH();
+  } else {
+    hipLaunchKernelGGL(k_copy, grid2(nn, P), dim3(256), 0, st, n, n, Gam, last_ld, Tf, n);
+    WB_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_i_minus, grid2(nn, P), dim3(256), 0, st, n, Tf, n, dg, (int)DG_T11);
+    WB_CHECK_LAUNCH();
+  }
   int* piv_f = ipiv[3];
   if ((e = wb_getrf_launch(n, Tf, n, piv_f, info[3], P, st))) return e;
   if ((e = flag_points(info[3], P, WB_FLAG_SINGULAR))) return e;
@@ -1113,7 +1307,15 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   const long long vs = (long long)n * nrhs;
   hipLaunchKernelGGL(k_rhs_init, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[0]);
   WB_CHECK_LAUNCH();
-  if ((e = wb_getrs_launch(n, nrhs, Tf, n, piv_f, S(vec[0], vs), nrhs, P, st))) return e;
+  const BatchPtr nox = bp_strided(nullptr, 0);
+  if (rmat) {
+    // conj(u) = solve of (M^T)^H x = conj(rhs): the transmitted sweep runs in conjugated variables
+    hipLaunchKernelGGL(k_vec_conj_scale, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(vec[0], vs), nox, 1.0, 1);
+    WB_CHECK_LAUNCH();
+    if ((e = wb_getrs_conj_launch(n, nrhs, Tf, n, piv_f, S(vec[0], vs), nrhs, P, st))) return e;
+  } else {
+    if ((e = wb_getrs_launch(n, nrhs, Tf, n, piv_f, S(vec[0], vs), nrhs, P, st))) return e;
+  }
   // per-point transmitted columns d_l (l = 0..L) kept for the adjoint
   const long long dstride = (long long)(L + 1) * vs;
   if (keep) {
@@ -1126,9 +1328,16 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                        S(FW.d + (long long)L * vs, dstride), nrhs);
     WB_CHECK_LAUNCH();
   }
-  // gu = G_L u
-  if ((e = wb_zgemm_launch(n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero, S(vec[1], vs),
-                           nrhs, P, st))) return e;
+  // gu = G_L u (from G^T: conj(gu) = (G^T)^H conj(u))
+  if (rmat) {
+    if ((e = wb_zgemm_ops_launch(1, 0, n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero,
+                                 S(vec[1], vs), nrhs, P, st))) return e;
+    hipLaunchKernelGGL(k_vec_conj_scale, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(vec[1], vs), nox, 1.0, 1);
+    WB_CHECK_LAUNCH();
+  } else {
+    if ((e = wb_zgemm_launch(n, nrhs, n, one, Gam, last_ld, S(vec[0], vs), nrhs, zero, S(vec[1], vs),
+                             nrhs, P, st))) return e;
+  }
   hipLaunchKernelGGL(k_reflect, grid2(vs, P), dim3(256), 0, st, N, n, nrhs, dg, sigc, vec[1], vec[2]);
   WB_CHECK_LAUNCH();
   // sweep: d_{l} = (I - s11 G_{l})^-1 s12 d_{l+1}, layers top to bottom. Ragged, a point joins
@@ -1150,6 +1359,17 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
                          S(dcur + (long long)joined * vs, vs), nrhs);
       WB_CHECK_LAUNCH();
     }
+    if (rmat) {
+      // conj(d_l) = 4 conj(X) . P^-H-solve of conj(d_{l+1}) through both transposed factors
+      if ((e = wb_getrs_conj_launch(n, nrhs, step_pairs(M[0], nn, l), n, step_piv(l), S(dcur, vs), nrhs,
+                                    Pl, st))) return e;
+      hipLaunchKernelGGL(k_vec_conj_scale, grid2(vs, Pl), dim3(256), 0, st, n, nrhs, S(dcur, vs),
+                         step_pairs(xv, n, l), 4.0, 0);
+      WB_CHECK_LAUNCH();
+      if ((e = wb_getrs_conj_launch(n, nrhs, step_pairs(M[1], nn, l), n, step_piv_p(l), S(dcur, vs), nrhs,
+                                    Pl, st))) return e;
+      continue;
+    }
     BatchPtr s12 = bp_offset(step_pairs(M6, n2, l), n);
     BatchPtr Tm = step_pairs(M[0], nn, l);
     int* piv_pl = step_piv(l);
@@ -1169,6 +1389,10 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
     cd* tmp = dcur; dcur = dnext; dnext = tmp;
   }
   if (fields && (e = wb_field_amps_stage(a, N, n, nrhs, G0, M6, M[4], M[5], q, FW, st))) return e;
+  if (rmat) {
+    hipLaunchKernelGGL(k_vec_conj_scale, grid2(vs, P), dim3(256), 0, st, n, nrhs, S(dcur, vs), nox, 1.0, 1);
+    WB_CHECK_LAUNCH();
+  }
   hipLaunchKernelGGL(k_transmit, grid2(vs, P), dim3(256), 0, st, n, nrhs, dg, dcur, vec[3]);
   WB_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_efficiency, dim3(P), dim3(256), 0, st, a.hu, a.hv, mode, a.pts, ov, vec[2],
