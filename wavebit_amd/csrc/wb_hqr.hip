@@ -30,6 +30,19 @@
 // (9,9) layer matrices (n = 722, 256 per launch) windows of 16-24 rows take the iteration from
 // 371 to 271-278 ms per launch, 32 rows to 301, 48 to 358 and 64 to 464 (a CPU model of the
 // sweep counts: 93 sweeps without the window, 68 at 24 rows, 46 at 48).
+//
+// The single-shift iteration (wave_qr) also deflates a subdiagonal at or below dfl =
+// ulp ||H||_F (the Frobenius norm taken once before the iteration; it is invariant under the
+// unitary similarity) once the bottom subdiagonal it drives down has stagnated there. Setting
+// such an entry to zero perturbs H by ulp ||H|| in norm, the size of the rounding every rotation
+// already commits, so the Schur form stays normwise backward stable. Without it an eigenvalue
+// far below ||H|| in magnitude must reach a subdiagonal below ulp times its own size, which
+// rounding at ulp ||H|| does not allow, and the iteration stalls until exceptional shifts (a
+// 21-harmonic grating operator with eigenvalues from -11 to 159: 205 single-shift iterations
+// without the floor, 40 with it, eigenvalues within 1.2e-13 of the Hermitian solver's). Only a
+// stagnated entry takes the floor: while the entry still converges (a tiny eigenvalue, or a
+// cluster at a grazing order, computed to relative accuracy) the LAPACK tests alone decide, and
+// the subdiagonal split and the window's spike test keep them throughout.
 #include "wb_internal.h"
 #include "wb_timing.h"
 #include <float.h>
@@ -104,6 +117,28 @@ __device__ inline void rot_right(const Rot& q, cd& a0, cd& a1) {
   a1 = mkc(b1x, b1y);
 }
 
+// givens() without its scaling where none is needed: with max(|Re f|, |Im f|) in
+// [2^-250, 2^250] and g's components at most 2^250, |f|^2 (|f|^2 + |g|^2) stays a normal double, so
+// c = |f|^2 q, s = f conj(g) q, r = f d^2 q with q = 1 / (|f| d) need no exponent handling; other
+// inputs (f = 0, extreme magnitudes) take givens(). f and g are the same on every lane, so the
+// wave takes one branch.
+__device__ inline Rot givens_fast(cd f, cd g, cd* r) {
+  const double fm = fmax(fabs(f.x), fabs(f.y)), gm = fmax(fabs(g.x), fabs(g.y));
+  if (fm >= 0x1p-250 && fm <= 0x1p250 && gm <= 0x1p250) {
+    const double f2 = cnorm2(f), d2 = f2 + cnorm2(g);
+    const double iq = rsqrt_pos(f2 * d2);
+    Rot q;
+    q.c = f2 * iq;
+    q.sx = (f.x * g.x + f.y * g.y) * iq;
+    q.sy = (f.y * g.x - f.x * g.y) * iq;
+    const double t = d2 * iq;
+    *r = mkc(f.x * t, f.y * t);
+    return q;
+  }
+  return givens(f, g, r);
+}
+
+
 // Largest k in (lo, hi] with a negligible subdiagonal (Ahues-Tisseur test), else lo. With
 // `relaxed` (the iteration has stalled) the conventional test |h(k,k-1)| <= ulp (|h(k-1,k-1)| +
 // |h(k,k)|) alone decides: still a normwise backward-stable deflation, and the one that lets an
@@ -163,8 +198,14 @@ __device__ inline cd readlane_c(cd x, int src) {
 // LDS on one wave with no barriers, every rotation also applied to U (the block's accumulated
 // unitary); the rows of H above the block, its columns right of the block and the Schur vectors
 // then get U in three small products with all threads. Returns 0, or i + 1 if eigenvalue i of
-// the block did not converge.
-__device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
+// the block did not converge. Once the bottom subdiagonal T(i, i-1) has stagnated (after two
+// iterations it is at or below dfl, the matrix's normwise floor, and fell by less than half in
+// the last one), subdiagonals at or below dfl deflate too.
+// WZ: U (LDS, like T) takes every rotation; without it U is not read.
+template <bool WZ>
+__device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w, double dfl) {
+  ASSUME_LDS(T);
+  if constexpr (WZ) ASSUME_LDS(U);
   const int lane = threadIdx.x & 63;
   const double smlnum = SAFMIN * ((double)n / ULP);
   int i = m - 1;
@@ -174,13 +215,17 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
   while (i >= 0) {
     int l = 0;
     bool converged = false;
+    double prev = 0.0;   // |T(i, i-1)| one iteration back
     for (int its = 0; its <= itmax; ++its) {
+      const double hb = (i > 0) ? cabs1(TT(i, i - 1)) : 0.0;
+      const double floor_now = (its >= 2 && hb <= dfl && hb > 0.5 * prev) ? dfl : 0.0;
+      prev = hb;
       // largest k in (l, i] with a negligible subdiagonal (Ahues-Tisseur), lane per k
       const int k = l + 1 + lane;
       bool small = false;
       if (k <= i) {
         const cd hk = TT(k, k - 1);
-        small = cabs1(hk) <= smlnum;
+        small = cabs1(hk) <= fmax(smlnum, floor_now);
         if (!small) {
           double tst = cabs1(TT(k - 1, k - 1)) + cabs1(TT(k, k));
           if (tst == 0.0) {
@@ -231,31 +276,138 @@ __device__ int wave_qr(cd* T, cd* U, int m, int ld, int n, cd* w) {
         const double sc = cabs1(v1) + cabs1(v2);
         if (sc != 0.0) { v1 = cscale(v1, 1.0 / sc); v2 = cscale(v2, 1.0 / sc); }
       }
+      if (m <= 32) {
+        // Blocks of at most 32 rows: one rotation application serves both operations. Lanes
+        // 0..31 take the row operation (lane j: rows kk, kk+1 of column j >= kk), lanes 32..63 the
+        // column operation (lane 32 + r: columns kk, kk+1 of row r <= kk+2); the column operation
+        // is the row operation's formula with conj(s). The four entries both touch (rows and
+        // columns kk, kk+1) are taken from the row lanes, which finish them with the column
+        // rotation after a readlane exchange between lanes kk and kk+1; the column lanes of rows
+        // kk, kk+1 are not stored. Loads are unconditional at clamped in-block indices.
+        const bool rowh = lane < 32;
+        const int q = lane & 31;
+        const int qu = q < m ? q : m - 1;
+        for (int kk = l; kk < i; ++kk) {
+          cd rr;
+          const Rot r = givens_fast(v1, v2, &rr);
+          const int rmax = (kk + 2 < i) ? kk + 2 : i;
+          cd u0, u1;
+          if (WZ) { u0 = U[qu * ld + kk]; u1 = U[qu * ld + kk + 1]; }
+          const bool act = rowh ? (q >= kk && q < m) : (q <= rmax);
+          const int qc = rowh ? qu : (q <= rmax ? q : rmax);
+          const int o0 = rowh ? kk * ld + qc : qc * ld + kk;
+          const int o1 = rowh ? o0 + ld : o0 + 1;
+          cd x0 = T[o0], x1 = T[o1];
+          Rot g = r;
+          if (!rowh) g.sy = -g.sy;
+          rot_left(g, x0, x1);
+          // the 2 x 2 block: lanes kk, kk+1 hold columns kk, kk+1 of G B; (G B) G^H takes each
+          // column as c times itself plus (conj(s) | -s) times the other
+          {
+            const cd p00 = readlane_c(x0, kk), p10 = readlane_c(x1, kk);
+            const cd p01 = readlane_c(x0, kk + 1), p11 = readlane_c(x1, kk + 1);
+            const bool left = (lane == kk);
+            const cd o_0 = left ? p01 : p00, o_1 = left ? p11 : p10;
+            // h = conj(s) on lane kk, -s on lane kk+1
+            const double hx = left ? r.sx : -r.sx, hy = -r.sy;
+            const cd y0 = mkc(fma(r.c, x0.x, hx * o_0.x - hy * o_0.y), fma(r.c, x0.y, hx * o_0.y + hy * o_0.x));
+            const cd y1 = mkc(fma(r.c, x1.x, hx * o_1.x - hy * o_1.y), fma(r.c, x1.y, hx * o_1.y + hy * o_1.x));
+            if (rowh && (lane == kk || lane == kk + 1)) { x0 = y0; x1 = y1; }
+          }
+          if (act && (rowh || (q != kk && q != kk + 1))) { T[o0] = x0; T[o1] = x1; }
+          if (kk > l && lane == 0) {
+            TT(kk, kk - 1) = rr;
+            TT(kk + 1, kk - 1) = mkc(0.0, 0.0);
+          }
+          if (WZ) {
+            rot_right(r, u0, u1);
+            if (lane < m) { U[q * ld + kk] = u0; U[q * ld + kk + 1] = u1; }
+          }
+          // the next rotation's inputs: T(kk+1, kk) from the row lane kk, T(kk+2, kk) from the
+          // column lane of row kk+2
+          if (kk + 1 < i) {
+            v1 = readlane_c(x1, kk);
+            v2 = readlane_c(x0, 32 + kk + 2);
+          }
+        }
+        continue;
+      }
+      // The chase keeps what it rotates in registers: lane j holds ra = T(kk, j), rb = T(kk+1, j)
+      // for the row operation (columns j >= kk), lane r holds ca = T(r, kk), cb = T(r, kk+1) for
+      // the column operation (rows r <= kk+2) and u0, u1 = U(r, kk), U(r, kk+1). The 2 x 2 block
+      // both operations touch passes from the row to the column layout by readlane, so the
+      // serial path of a step is the rotation, two small updates and the readlanes. Each finished
+      // row kk and column kk is written back as the chase moves on, and the next row kk+2 and
+      // column kk+2 are read from LDS at the end of a step (LDS accesses of one wave stay in
+      // program order: a read issued after a write returns it) into trow / tcol / tu, taken into
+      // rb / cb / u1 only after the next rotation is formed, so the reads' latency overlaps it.
+      // The loads are unconditional at clamped in-block indices; the values are selected after.
+      const cd z0 = mkc(0.0, 0.0);
+      cd ra = z0, ca = z0, u0 = z0, rb, cb, u1 = z0, trow, tcol, tu = z0, b21;
+      {
+        const int rmax0 = (l + 2 < i) ? l + 2 : i;
+        const int lr = lane < m ? lane : m - 1;
+        const int lc = lane <= rmax0 ? lane : rmax0;
+        ra = TT(l, lr);
+        trow = TT(l + 1, lr);
+        b21 = TT(l + 1, l);
+        ca = TT(lc, l);
+        tcol = TT(lc, l + 1);
+        if (WZ) { u0 = U[lr * ld + l]; tu = U[lr * ld + l + 1]; }
+      }
       for (int kk = l; kk < i; ++kk) {
         cd rr;
         const Rot r = givens(v1, v2, &rr);
+        const int rmax = (kk + 2 < i) ? kk + 2 : i;
+        // rows kk+1 and columns kk+1 read at the end of the last step (row kk+1's column kk
+        // entry and the column's rows beyond rmax as the layouts need them)
+        rb = (lane > kk && lane < m) ? trow : b21;
+        cb = (lane <= rmax) ? tcol : z0;
+        if (WZ) u1 = tu;
+        // row operation on columns kk..m-1
+        if (lane >= kk && lane < m) rot_left(r, ra, rb);
+        // rows kk, kk+1 of columns kk, kk+1 into the column layout
+        {
+          const cd a00 = readlane_c(ra, kk), a01 = readlane_c(ra, kk + 1);
+          const cd a10 = readlane_c(rb, kk), a11 = readlane_c(rb, kk + 1);
+          if (lane == kk) { ca = a00; cb = a01; }
+          if (lane == kk + 1) { ca = a10; cb = a11; }
+        }
+        // column operation on rows 0..rmax, and on U
+        if (lane <= rmax) rot_right(r, ca, cb);
+        if (WZ && lane < m) rot_right(r, u0, u1);
+        // the next rotation's inputs (rows kk+1, kk+2 of column kk)
+        if (kk + 1 < i) {
+          v1 = readlane_c(ca, kk + 1);
+          v2 = readlane_c(ca, kk + 2);
+        }
+        // row kk beyond the block (its columns kk, kk+1 are in the column layout), the chased
+        // entries of column kk-1, then column kk
+        if (lane >= kk + 2 && lane < m) TT(kk, lane) = ra;
         if (kk > l && lane == 0) {
           TT(kk, kk - 1) = rr;
-          TT(kk + 1, kk - 1) = mkc(0.0, 0.0);
+          TT(kk + 1, kk - 1) = z0;
         }
-        // row operation on columns kk..m-1, then the column operation on rows 0..min(kk+2, i)
-        // and on U (LDS accesses of one wave stay in program order)
-        if (lane >= kk && lane < m) rot_left(r, TT(kk, lane), TT(kk + 1, lane));
-        const int rmax = (kk + 2 < i) ? kk + 2 : i;
-        cd col = mkc(0.0, 0.0);
-        if (lane <= rmax) {
-          cd x0 = TT(lane, kk), x1 = TT(lane, kk + 1);
-          rot_right(r, x0, x1);
-          TT(lane, kk) = x0;
-          TT(lane, kk + 1) = x1;
-          col = x0;
-        }
-        if (U && lane < m) rot_right(r, U[lane * ld + kk], U[lane * ld + kk + 1]);
-        // the next rotation's inputs (rows kk+1, kk+2 of column kk) straight from the lanes that
-        // just computed them
+        if (lane <= rmax) TT(lane, kk) = ca;
+        if (WZ && lane < m) U[lane * ld + kk] = u0;
         if (kk + 1 < i) {
-          v1 = readlane_c(col, kk + 1);
-          v2 = readlane_c(col, kk + 2);
+          // step to rows / columns kk+1, kk+2: row kk+1 with its column kk+1 entry from the
+          // column operation, column kk+1 (row kk+3 is zero in Hessenberg form); row kk+2 (its
+          // column kk+1 entry, b21, likewise from the column operation) and column kk+2 are read
+          const cd b11 = readlane_c(cb, kk + 1);
+          b21 = readlane_c(cb, kk + 2);
+          const int rmax2 = (kk + 3 < i) ? kk + 3 : i;
+          trow = TT(kk + 2, lane < m ? lane : m - 1);
+          tcol = TT(lane <= rmax2 ? lane : rmax2, kk + 2);
+          if (WZ) tu = U[(lane < m ? lane : m - 1) * ld + kk + 2];
+          ra = (lane == kk + 1) ? b11 : rb;
+          ca = (lane <= rmax) ? cb : z0;
+          u0 = u1;
+        } else {
+          // the last step: row i beyond the block, column i, U column i
+          if (lane >= kk + 2 && lane < m) TT(kk + 1, lane) = rb;
+          if (lane <= rmax) TT(lane, kk + 1) = cb;
+          if (WZ && lane < m) U[lane * ld + kk + 1] = u1;
         }
       }
     }
@@ -329,7 +481,7 @@ __device__ void uh_times_cols(cd* B, int ldb, int m, int cols, const cd* U, int 
 
 static_assert(WB_HQR_NMIN <= 64 && WB_HQR_NMIN >= WB_HQR_M + 2, "small blocks (at most nmin rows) fit one wave");
 __device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* wout, int n, int lo, int hi, cd* lds,
-                                                     int* s_info) {
+                                                     int* s_info, double dfl) {
   ASSUME_LDS(lds);
   ASSUME_LDS(s_info);
   const int t = threadIdx.x;
@@ -345,7 +497,7 @@ __device__ __attribute__((noinline)) int small_block(cd* H, cd* Z, int zh, cd* w
   if (t == 0) *s_info = 0;
   __syncthreads();
   if (t < 64) {
-    const int info = wave_qr(T, U, m, ld, n, wout + lo);
+    const int info = wave_qr<true>(T, U, m, ld, n, wout + lo, dfl);
     if (t == 0) *s_info = info ? lo + info : 0;
   }
   __syncthreads();
@@ -477,7 +629,7 @@ __device__ int wave_aed(cd* T, cd* U, int m, int ld, cd s, double smlnum, cd* ei
 // upper triangular, split off by zero subdiagonals), or -1 when the window's QR did not converge
 // (H untouched); the undeflated eigenvalues (ns of them) are written to eig (LDS) and *ns_out.
 __device__ __attribute__((noinline)) int aed_window(cd* H, cd* Z, int zh, int n, int lo, int hi, int jw,
-                                                    cd* lds, cd* eig, int* s_int) {
+                                                    cd* lds, cd* eig, int* s_int, double dfl) {
   ASSUME_LDS(lds);
   ASSUME_LDS(eig);
   ASSUME_LDS(s_int);
@@ -497,7 +649,7 @@ __device__ __attribute__((noinline)) int aed_window(cd* H, cd* Z, int zh, int n,
   __syncthreads();
   if (t < 64) {
     const double smlnum = SAFMIN * ((double)n / ULP);
-    const int bad = wave_qr(T, U, jw, ld, n, wtmp);
+    const int bad = wave_qr<true>(T, U, jw, ld, n, wtmp, dfl);
     int ns = -1;
     if (!bad) ns = wave_aed(T, U, jw, ld, s, smlnum, eig, v);
     if (t == 0) s_int[0] = ns;
@@ -1078,6 +1230,24 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
   int stall = 0;
   int sweeps = 0;
   const int max_sweeps = 40 * (n / M + 10);
+  // the normwise deflation floor ulp ||H||_F (a wave per row of the Hessenberg part)
+  double dfl;
+  {
+    __shared__ double s_nrm[NT / 64];
+    const int lane = t & 63, wave = t >> 6;
+    double acc = 0.0;
+    for (int r = wave; r < n; r += NT / 64) {
+      const cd* row = H + (long long)r * n;
+      for (int c = (r > 0 ? r - 1 : 0) + lane; c < n; c += 64) acc += cnorm2(row[c]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) s_nrm[wave] = acc;
+    __syncthreads();
+    double sum = 0.0;
+    for (int w = 0; w < NT / 64; ++w) sum += s_nrm[w];
+    dfl = ULP * sqrt(sum);
+  }
   while (hi >= 0) {
     lo = find_split(H, n, 0, hi, s_k, stall >= 3);
     if (lo > 0 && t == 0) H[(long long)lo * n + lo - 1] = mkc(0.0, 0.0);
@@ -1090,7 +1260,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       continue;
     }
     if (hi - lo + 1 <= nmin) {
-      const int r = small_block(H, Z, zh, wout, n, lo, hi, Hw, &s_k[0]);
+      const int r = small_block(H, Z, zh, wout, n, lo, hi, Hw, &s_k[0], dfl);
       if (r != 0) {
         if (t == 0) info[b] = r;
         return;
@@ -1105,7 +1275,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
 #if WB_HQR_AED
     {
       const int jw = (WB_HQR_AED < hi - lo + 1) ? WB_HQR_AED : hi - lo + 1;
-      const int nd = aed_window(H, Z, zh, n, lo, hi, jw, Hw, s_small, s_k);
+      const int nd = aed_window(H, Z, zh, n, lo, hi, jw, Hw, s_small, s_k, dfl);
       if (nd > 0) {
         stall = 0;
         hs = hi - nd;
@@ -1135,7 +1305,7 @@ __global__ __launch_bounds__(NT) void k_hqr_ms(int n, cd* Hb, cd* Zb, int zh, cd
       }
       __syncthreads();
       if (t < 64) {
-        const int bad = wave_qr(s_small, nullptr, M, M, M, s_sh);
+        const int bad = wave_qr<false>(s_small, nullptr, M, M, M, s_sh, dfl);
         if (t == 0 && (bad || stall % 6 == 0)) {
           const double ex = 0.75 * cabs1(H[(long long)hs * n + hs - 1]);
           for (int j = 0; j < M; ++j)
