@@ -50,14 +50,17 @@ def _close(a, b, path):
 @pytest.mark.parametrize('path,hu,hv', [(eng.PATH_2D, 3, 2), (eng.PATH_TE, 8, 0), (eng.PATH_TM, 8, 0)])
 def test_window_of_mixed_layer_counts_equals_fused_solves(path, hu, hv, ragged):
     """One modes batch over buckets of three layer counts; per-bucket cascades equal each bucket's
-    fused solve to the bit, the one ragged cascade within rounding."""
+    fused solve to the bit, the one ragged cascade within rounding. (A 1D bucket of at most 32
+    harmonics solves alone in the one-workgroup kernel, the same algebra in another order:
+    within rounding there too.)"""
     rng = np.random.default_rng(hu * 10 + len(path))
     buckets = [_bucket(rng, L, P, hu, hv, path) for L, P in ((1, 3), (3, 2), (2, 4))]
     out, modes = eng.run_window([b for b, _ in buckets], [c for _, c in buckets], DEV,
                                 ragged=ragged)
+    small = path != eng.PATH_2D and 2 * hu + 1 <= 32
     for (b, conv), got in zip(buckets, out):
         want = eng.run_bucket(b, conv, DEV)
-        (_close if ragged else _equal)(got[:4], want[:4], path)
+        (_close if ragged or small else _equal)(got[:4], want[:4], path)
     W, V, q, flags, tables = modes
     assert W.shape[0] == sum(b.n_layers * len(b.points) for b, _ in buckets)
 

@@ -31,7 +31,6 @@ inline dim3 grid2(long long elems, int batch) {
   return dim3((unsigned)((elems + 255) / 256), (unsigned)batch);
 }
 
-__device__ inline cd pt_c(const double* p, int off) { return mkc(p[off], p[off + 1]); }
 
 __device__ inline cd x_of(cd q, double k0d) { return cexp_(cscale(q, -k0d)); }
 

@@ -32,7 +32,6 @@ inline dim3 grid2(long long elems, int batch) {
   return dim3((unsigned)((elems + 255) / 256), (unsigned)batch);
 }
 
-__device__ inline cd pt_c(const double* p, int off) { return mkc(p[off], p[off + 1]); }
 
 // Combine the basis columns of the three partials of pair b into the public incidence and pick
 // the face each up-going mode is read from. One thread per (row, pair).

@@ -31,37 +31,10 @@
 //                 sign C S C of rcwa/_engine/engine.py:2995-3009.
 #include "wb_rcwa.h"
 #include "wb_timing.h"
+#include <cstdlib>
 
 namespace {
 
-constexpr double PI = 3.141592653589793;
-constexpr double HALF_PI = 1.5707963267948966;
-
-__device__ inline cd boundary_kz(cd eps, cd kpar2) {
-  cd v = csqrt_p(csub(eps, kpar2));
-  if (v.x + v.y < 0.0) v = cneg(v);
-  return v;
-}
-
-// Real-angle order azimuth (kx, ky real): atan with the half-turn on Re(kx) < 0, the
-// +-pi/2 - kx/ky column and the origin rule; returns (cos, sin).
-__device__ inline void order_frame(double kx, double ky, double phi, double* c, double* s) {
-  double varphi;
-  if (kx == 0.0) {
-    if (ky == 0.0) {
-      double sp, cp;
-      sincos(phi, &sp, &cp);
-      varphi = atan(sp / cp) + (cp < 0.0 ? PI : 0.0);
-    } else {
-      varphi = (ky < 0.0 ? -HALF_PI : HALF_PI) - kx / ky;
-    }
-  } else {
-    varphi = atan(ky / kx) + (kx < 0.0 ? PI : 0.0);
-  }
-  sincos(varphi, s, c);
-}
-
-__device__ inline cd pt_c(const double* p, int off) { return mkc(p[off], p[off + 1]); }
 
 // One thread per (point, order). mode: 0 = 2D coupled, 1 = 1D TE, 2 = 1D TM.
 __global__ void k_point_setup(int P, int hu, int hv, int mode, const double* pts, cd* ov, cd* dg) {
@@ -70,60 +43,8 @@ __global__ void k_point_setup(int P, int hu, int hv, int mode, const double* pts
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)P * N) return;
   const int p = (int)(gid / N), r = (int)(gid % N);
-  const double* pt = pts + (long long)p * PT_STRIDE;
-  double kx, ky;
-  order_k(pt, hu, hv, mode, r, &kx, &ky);
-  const cd kpar2 = mkc(kx * kx + ky * ky, 0.0);
-  // kernel top = scene bottom (substrate), kernel bottom = scene top (superstrate)
-  const cd e_kt = pt_c(pt, PT_EBOT), e_kb = pt_c(pt, PT_ETOP);
-  const cd n_kt = csqrt_p(e_kt), n_kb = csqrt_p(e_kb);
-  const cd kz_kt = boundary_kz(cmul(n_kt, n_kt), kpar2);
-  const cd kz_kb = boundary_kz(cmul(n_kb, n_kb), kpar2);
-  const double y = sqrt(1.0 + (kx * kx + ky * ky));
-  double fc = 1.0, fs = 0.0;
-  if (mode == 0) order_frame(kx, ky, pt[PT_PHI], &fc, &fs);
-  cd* o = ov + (long long)p * OV_COUNT * N;
-  o[OV_KX * N + r] = mkc(kx, 0.0);
-  o[OV_KY * N + r] = mkc(ky, 0.0);
-  o[OV_KZT * N + r] = kz_kt;
-  o[OV_KZB * N + r] = kz_kb;
-  o[OV_Y * N + r] = mkc(y, 0.0);
-  o[OV_FC * N + r] = mkc(fc, 0.0);
-  o[OV_FS * N + r] = mkc(fs, 0.0);
-  cd* d = dg + (long long)p * DG_COUNT * n;
-  // half-space -> gap interface blocks, diagonal: P = Wg^-1 Wh, Q = Vg^-1 Vh
-  for (int side = 0; side < 2; ++side) {
-    const cd kz = side == 0 ? kz_kt : kz_kb;
-    const cd e = side == 0 ? e_kt : e_kb;
-    const int halves = (mode == 0) ? 2 : 1;
-    for (int h = 0; h < halves; ++h) {
-      cd Pd, Qd;
-      if (mode == 0) {
-        if (h == 0) { Pd = mkc(1.0, 0.0); Qd = cscale(kz, 1.0 / y); }
-        else { Pd = cdiv(kz, cscale(e, y)); Qd = mkc(1.0, 0.0); }
-      } else if (mode == 1) {
-        Pd = mkc(1.0, 0.0); Qd = cscale(kz, 1.0 / y);
-      } else {
-        Pd = mkc(1.0, 0.0); Qd = cdiv(kz, cscale(e, y));
-      }
-      const cd A = cadd(Pd, Qd), B = csub(Pd, Qd);
-      const cd BA = cdiv(B, A);
-      const cd two_A = cdiv(mkc(2.0, 0.0), A);
-      const cd half = cscale(csub(A, cmul(B, BA)), 0.5);
-      const int idx = h * N + r;
-      if (side == 0) {   // S_ref (start port, scene bottom)
-        d[DG_R11 * n + idx] = cneg(BA);
-        d[DG_R12 * n + idx] = two_A;
-        d[DG_R21 * n + idx] = half;
-        d[DG_R22 * n + idx] = BA;
-      } else {           // S_trn (end port, scene top)
-        d[DG_T11 * n + idx] = BA;
-        d[DG_T12 * n + idx] = half;
-        d[DG_T21 * n + idx] = two_A;
-        d[DG_T22 * n + idx] = cneg(BA);
-      }
-    }
-  }
+  point_order_setup(pts + (long long)p * PT_STRIDE, hu, hv, mode, r,
+                    ov + (long long)p * OV_COUNT * N, dg + (long long)p * DG_COUNT * n);
 }
 
 // 2D: BE = [[Kx epz_i Kx - I],[Ky epz_i Kx]], CD = [[Kx epz_i Ky],[Ky epz_i Ky - I]] (n x N each,
@@ -310,7 +231,7 @@ __global__ void k_xvec(int n, int L, const int* ppt, const cd* q, const double* 
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int p = ppt ? ppt[b] : (int)(b / L);
-  const double k0 = 2.0 * PI / pts[(long long)p * PT_STRIDE + PT_WL];
+  const double k0 = 2.0 * WB_PI / pts[(long long)p * PT_STRIDE + PT_WL];
   xv[b * n + r] = cexp_(cscale(q[b * n + r], -k0 * thick[b]));
 }
 
@@ -660,107 +581,16 @@ __global__ void k_transmit(int n, int nrhs, const cd* dg, const cd* d0, cd* ctrn
 }
 
 // Efficiencies, amplitudes and power entering, one workgroup per point.
-// eff: [P][3][4][N] doubles (R_s, R_p, T_s, T_p); amp: [P][3][4][N] complex (r_s, r_p, t_s, t_p);
-// pe: [P][3]. 1D fills variant 0 only.
 __global__ __launch_bounds__(256) void k_efficiency(int hu, int hv, int mode, const double* pts,
                                                     const cd* ov, const cd* cref, const cd* ctrn,
                                                     double* eff, cd* amp, double* pe) {
+  __shared__ double red[3 * 256];
   const int p = blockIdx.x;
-  const int nu = 2 * hu + 1, nv = 2 * hv + 1, N = nu * nv;
+  const int N = (2 * hu + 1) * (2 * hv + 1);
   const int n = (mode == 0) ? 2 * N : N;
   const int nrhs = (mode == 0) ? 2 : 1;
-  const double* pt = pts + (long long)p * PT_STRIDE;
-  const cd* o = ov + (long long)p * OV_COUNT * N;
-  const cd* cr = cref + (long long)p * n * nrhs;
-  const cd* ct = ctrn + (long long)p * n * nrhs;
-  // incidence side ("ref") is the scene top = kernel bottom
-  const cd e_ref = pt_c(pt, PT_ETOP), e_trn = pt_c(pt, PT_EBOT);
-  const cd n_ref = csqrt_p(pt_c(pt, PT_ETOP)), n_trn = csqrt_p(pt_c(pt, PT_EBOT));
-  const int c0 = N / 2;
-  const cd kz_c = o[OV_KZB * N + c0];
-  __shared__ double red[3][256];
-  const int t = threadIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
-  if (mode == 0) {
-    const cd as = pt_c(pt, PT_AS), ap = pt_c(pt, PT_AP);
-    // the three incidence variants as (alpha, beta) weights of the (s, p) basis columns
-    const cd mi_n = cmul(mkc(0.0, -1.0), n_ref);
-    const double cos_half_pi = cos(HALF_PI), sin_half_pi = sin(HALF_PI);
-    cd al[3] = {as, mkc(sin_half_pi, 0.0), mkc(0.0, 0.0)};
-    cd be[3] = {cmul(mi_n, ap), cmul(mi_n, mkc(cos_half_pi, 0.0)), cmul(mi_n, mkc(1.0, 0.0))};
-    const double s_power = kz_c.x;
-    const double p_power = cnorm2(n_ref) * cdiv(kz_c, e_ref).x;
-    double den[3] = {cnorm2(as) * s_power + cnorm2(ap) * p_power, s_power, p_power};
-    for (int r = t; r < N; r += 256) {
-      const int mi = r % nu - hu, ni = r / nu - hv;
-      const double par = ((mi + ni) & 1) ? -1.0 : 1.0;
-      const cd kzr = o[OV_KZB * N + r], kzt = o[OV_KZT * N + r];
-      const double adm_s_ref = kzr.x, adm_p_ref = cdiv(kzr, e_ref).x;
-      const double adm_s_trn = kzt.x, adm_p_trn = cdiv(kzt, e_trn).x;
-      for (int v = 0; v < 3; ++v) {
-        const cd Rs = cadd(cmul(cr[r * 2 + 0], al[v]), cmul(cr[r * 2 + 1], be[v]));
-        const cd Rp = cneg(cadd(cmul(cr[(N + r) * 2 + 0], al[v]), cmul(cr[(N + r) * 2 + 1], be[v])));
-        const cd Ts = cadd(cmul(ct[r * 2 + 0], al[v]), cmul(ct[r * 2 + 1], be[v]));
-        const cd Tp = cadd(cmul(ct[(N + r) * 2 + 0], al[v]), cmul(ct[(N + r) * 2 + 1], be[v]));
-        double* ef = eff + (((long long)p * 3 + v) * 4) * N;
-        ef[0 * N + r] = cnorm2(Rs) * adm_s_ref / den[v];
-        ef[1 * N + r] = cnorm2(Rp) * adm_p_ref / den[v];
-        ef[2 * N + r] = cnorm2(Ts) * adm_s_trn / den[v];
-        ef[3 * N + r] = cnorm2(Tp) * adm_p_trn / den[v];
-        cd* am = amp + (((long long)p * 3 + v) * 4) * N;
-        am[0 * N + r] = cscale(Rs, par);
-        am[1 * N + r] = cscale(cdiv(Rp, mi_n), par);
-        am[2 * N + r] = cscale(Ts, par);
-        am[3 * N + r] = cscale(cdiv(Tp, cmul(mkc(0.0, -1.0), n_trn)), par);
-        // net power at the lit face: incident column entries are nonzero at the specular order
-        const cd inc_s = (r == c0) ? al[v] : mkc(0.0, 0.0);
-        const cd inc_p = (r == c0) ? be[v] : mkc(0.0, 0.0);
-        const cd ps = cmul(cadd(inc_s, Rs), cconj(cmul(kzr, csub(inc_s, Rs))));
-        const cd pp = cmul(cadd(inc_p, Rp), cconj(cmul(cdiv(kzr, e_ref), csub(inc_p, Rp))));
-        acc[v] += (ps.x + pp.x) / den[v];
-      }
-    }
-  } else {
-    const int tm = (mode == 2);
-    const double sig_c = (o[OV_KX * N + c0].x < 0.0) ? -1.0 : 1.0;
-    const cd adm_c = tm ? cdiv(kz_c, e_ref) : kz_c;
-    const double incoming = adm_c.x;
-    for (int r = t; r < N; r += 256) {
-      const int mi = r % nu - hu;
-      const double par = (mi & 1) ? -1.0 : 1.0;
-      const double sig = (o[OV_KX * N + r].x < 0.0) ? -1.0 : 1.0;
-      const cd kzr = o[OV_KZB * N + r], kzt = o[OV_KZT * N + r];
-      const cd adm_ref = tm ? cdiv(kzr, e_ref) : kzr;
-      const cd adm_trn = tm ? cdiv(kzt, e_trn) : kzt;
-      const cd R = cr[r], T = ct[r];
-      double* ef = eff + ((long long)p * 3 * 4) * N;
-      cd* am = amp + ((long long)p * 3 * 4) * N;
-      const double er = cnorm2(R) * adm_ref.x / incoming;
-      const double et = cnorm2(T) * adm_trn.x / incoming;
-      const cd zero = mkc(0.0, 0.0);
-      if (!tm) {
-        ef[0 * N + r] = er; ef[1 * N + r] = 0.0; ef[2 * N + r] = et; ef[3 * N + r] = 0.0;
-        am[0 * N + r] = cscale(R, sig * par); am[1 * N + r] = zero;
-        am[2 * N + r] = cscale(T, sig * par); am[3 * N + r] = zero;
-      } else {
-        ef[0 * N + r] = 0.0; ef[1 * N + r] = er; ef[2 * N + r] = 0.0; ef[3 * N + r] = et;
-        am[0 * N + r] = zero; am[1 * N + r] = cscale(R, sig * par);
-        am[2 * N + r] = zero; am[3 * N + r] = cscale(cmul(T, cdiv(n_ref, n_trn)), sig * par);
-      }
-      const cd inc = (r == c0) ? mkc(sig_c, 0.0) : mkc(0.0, 0.0);
-      const cd pw = cmul(cadd(inc, R), cconj(cmul(adm_ref, csub(inc, R))));
-      acc[0] += pw.x / incoming;
-    }
-  }
-  for (int v = 0; v < 3; ++v) red[v][t] = acc[v];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int v = 0; v < 3; ++v) red[v][t] += red[v][t + s];
-    __syncthreads();
-  }
-  if (t == 0)
-    for (int v = 0; v < 3; ++v) pe[p * 3 + v] = red[v][0];
+  point_efficiency<256>(p, hu, hv, mode, pts, ov + (long long)p * OV_COUNT * N, cref + (long long)p * n * nrhs,
+                   ctrn + (long long)p * n * nrhs, eff, amp, pe, red);
 }
 
 // flags[b] |= bit where info[b] != 0 (b over pairs, or over points spread to their pairs)
@@ -916,6 +746,15 @@ hipError_t wb_solve_launch(const wb_solve_args& a, void* ws, hipStream_t st) {
   const long long nn = (long long)n * n;
   const long long NN = (long long)N * N;
   const wb_solve_ws W_ = wb_solve_carve_b(ws, P, B, a.hu, a.hv, mode);
+  // 1D points with at most 32 harmonics: the whole plain solve in one launch, one workgroup per
+  // point (WAVEBIT_AMD_FUSED_SMALL=0 keeps them on the batched kernels below)
+  if (wb_small_1d_eligible(a)) {
+    const char* env = getenv("WAVEBIT_AMD_FUSED_SMALL");
+    if (!(env && env[0] == '0')) {
+      return wb_small_1d_launch(a, W_, bp_gather(a.epx, NN, a.conv_idx), bp_gather(a.epy, NN, a.conv_idx),
+                                bp_gather(a.epz, NN, a.conv_idx), st);
+    }
+  }
   int* ppt = ragged ? a.ragged_tab : nullptr;   // each pair's point
   int* last = ragged ? a.ragged_tab + B : nullptr;   // each point's top-layer pair
   if (ragged) {
