@@ -167,6 +167,9 @@ def offdiag_cat(sets):
                       for c in sets]).contiguous()
 
 
+PT_STRIDE = 16   # doubles per point record (WB_PT_STRIDE)
+
+
 def point_record(pt, path):
     """The WB_PT_* record of one point."""
     kx0, ky0 = incident_k(pt.eps_top, pt.theta, pt.phi, pt.wavelength, pt.k_direct)
@@ -203,15 +206,25 @@ def order_kxy(kx0, ky0, wavelength, period, hu, hv):
 def bucket_inputs(bucket, conv, device):
     """The device inputs of a bucket's solve: (pts, thick, conv_idx, epx, epy, epz_i)."""
     P, L = len(bucket.points), bucket.n_layers
-    # concatenate the stacks' matrices: stack s layer l -> s * L + l
-    epx = torch.cat([c[0] for c in conv]).contiguous()
-    epy = torch.cat([c[1] for c in conv]).contiguous()
-    epz = torch.cat([c[2] for c in conv]).contiguous()
-    rec = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64)
-    thick = np.array([p.thickness for p in bucket.points], dtype=np.float64).reshape(P, L)
-    idx = np.array([[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32)
-    return (torch.from_numpy(rec).to(device), torch.from_numpy(thick).to(device),
-            torch.from_numpy(idx).to(device), epx, epy, epz)
+    # concatenate the stacks' matrices: stack s layer l -> s * L + l (one stack: as they are)
+    if len(conv) == 1:
+        epx, epy, epz = (c.contiguous() for c in conv[0][:3])
+    else:
+        epx = torch.cat([c[0] for c in conv]).contiguous()
+        epy = torch.cat([c[1] for c in conv]).contiguous()
+        epz = torch.cat([c[2] for c in conv]).contiguous()
+    # the point records, thicknesses and conv indices in one host buffer: one upload
+    nr, nt, ni = P * PT_STRIDE, P * L, (P * L + 1) // 2
+    host = np.zeros(nr + nt + ni, dtype=np.float64)
+    host[:nr] = np.array([point_record(p, bucket.path) for p in bucket.points], dtype=np.float64).ravel()
+    host[nr:nr + nt] = np.array([p.thickness for p in bucket.points], dtype=np.float64).ravel()
+    host[nr + nt:].view(np.int32)[:P * L] = np.array(
+        [[p.stack * L + l for l in range(L)] for p in bucket.points], dtype=np.int32).ravel()
+    dev = torch.from_numpy(host).to(device)
+    pts = dev[:nr].view(P, PT_STRIDE)
+    thick = dev[nr:nr + nt].view(P, L)
+    idx = dev[nr + nt:].view(torch.int32)[:P * L].view(P, L)
+    return pts, thick, idx, epx, epy, epz
 
 
 def rebuild_grazing(bucket, conv, device, eff, amp, pe, flags, smat=None):
