@@ -100,6 +100,84 @@ __device__ void w_identity(int n, cd* A) {
   }
 }
 
+// C = D + op(A) op(B), op the conjugate transpose where ha / hb is set (D may be null or C).
+__device__ void w_mm_op(int n, int ha, const cd* A, int hb, const cd* B, const cd* D, cd* C) {
+  for (int e = threadIdx.x; e < n * n; e += ST) {
+    const int i = e / n, j = e % n;
+    cd acc = D ? AT(D, i, j) : mkc(0.0, 0.0);
+    for (int k = 0; k < n; ++k) {
+      const cd a = ha ? cconj(AT(A, k, i)) : AT(A, i, k);
+      const cd b = hb ? cconj(AT(B, j, k)) : AT(B, k, j);
+      acc = cadd(acc, cmul(a, b));
+    }
+    AT(C, i, j) = acc;
+  }
+}
+
+// Whether A equals A^H to 64 ulp of its largest entry.
+__device__ bool w_is_herm(int n, const cd* A) {
+  double dmax = 0.0, amax = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += ST) {
+    const int i = e / n, j = e % n;
+    dmax = fmax(dmax, cabs1(csub(AT(A, i, j), cconj(AT(A, j, i)))));
+    amax = fmax(amax, cabs1(AT(A, i, j)));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off));
+    amax = fmax(amax, __shfl_xor(amax, off));
+  }
+  return dmax <= 64.0 * ULP * amax;
+}
+
+// A := (A + A^H) / 2.
+__device__ void w_symmetrize(int n, cd* A) {
+  for (int e = threadIdx.x; e < n * n; e += ST) {
+    const int i = e / n, j = e % n;
+    if (i <= j) {
+      const cd h = cscale(cadd(AT(A, i, j), cconj(AT(A, j, i))), 0.5);
+      AT(A, i, j) = h;
+      AT(A, j, i) = cconj(h);
+    }
+  }
+}
+
+// Cholesky A = C C^H in place (C lower triangular with a positive real diagonal, the upper part
+// cleared): lane i updates row i. Returns 1 when a pivot is not positive and finite (A is not
+// positive definite), else 0.
+__device__ int w_chol(int n, cd* A) {
+  const int i = threadIdx.x & 63;
+  for (int k = 0; k < n; ++k) {
+    const double akk = AT(A, k, k).x;
+    if (!(akk > 0.0) || !isfinite(akk)) return 1;
+    const double c = sqrt(akk), ic = 1.0 / c;
+    if (i == k) AT(A, k, k) = mkc(c, 0.0);
+    if (i > k && i < n) AT(A, i, k) = cscale(AT(A, i, k), ic);
+    if (i > k && i < n) {
+      const cd lik = AT(A, i, k);
+      for (int j = k + 1; j <= i; ++j) AT(A, i, j) = csub(AT(A, i, j), cmul(lik, cconj(AT(A, j, k))));
+    }
+  }
+  for (int e = threadIdx.x; e < n * n; e += ST) {
+    const int r = e / n, c = e % n;
+    if (c > r) AT(A, r, c) = mkc(0.0, 0.0);
+  }
+  return 0;
+}
+
+// X = C^-1 for lower triangular C: lane j solves C x = e_j by forward substitution.
+__device__ void w_trinv_lower(int n, const cd* C, cd* X) {
+  const int j = threadIdx.x & 63;
+  if (j < n) {
+    for (int r = 0; r < j; ++r) AT(X, r, j) = mkc(0.0, 0.0);
+    for (int r = j; r < n; ++r) {
+      cd acc = mkc(r == j ? 1.0 : 0.0, 0.0);
+      for (int k = j; k < r; ++k) acc = csub(acc, cmul(AT(C, r, k), AT(X, k, j)));
+      AT(X, r, j) = cdiv(acc, AT(C, r, r));
+    }
+  }
+}
+
 // A = P L U in place (LAPACK getrf: piv[k] is the row swapped with row k, whole rows swapped;
 // the pivot is the largest |re| + |im|, the first on ties). Lane (j, h) = (lane & 31, lane >> 5)
 // updates column j's rows of parity h. Returns the first zero pivot's column + 1, or 0.
@@ -311,6 +389,13 @@ __device__ void w_gehrd(int n, cd* A, cd* Z, SmallLds& S) {
 // LAPACK's test e_k^2 <= ulp^2 |d_k d_k+1| + safmin. On return Z holds the orthonormal
 // eigenvectors of the original operator and w its (real) eigenvalues; returns 0, or 1 when an
 // eigenvalue did not converge in 30 n sweeps.
+__device__ inline double readlane_d(double x, int src) {
+  const unsigned long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffull), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 __device__ int w_tridiag_qr(int n, const cd* H, cd* Z, cd* w) {
   const int lane = threadIdx.x & 63;
   double dl = (lane < n) ? AT(H, lane, lane).x : 0.0;
@@ -335,17 +420,20 @@ __device__ int w_tridiag_qr(int n, const cd* H, cd* Z, cd* w) {
     if (lo == hi) { --hi; continue; }
     if (++sweeps > 30 * n) { bad = 1; break; }
     // Wilkinson shift from the trailing 2 x 2
-    const double a = __shfl(dl, hi - 1), bb = __shfl(el, hi - 1), c = __shfl(dl, hi);
+    const double a = readlane_d(dl, hi - 1), bb = readlane_d(el, hi - 1), c = readlane_d(dl, hi);
     const double delta = 0.5 * (a - c);
     const double rt = sqrt(delta * delta + bb * bb);
     const double den = delta + (delta >= 0.0 ? rt : -rt);
     const double mu = (den != 0.0) ? c - bb * bb / den : c;
-    double dk = __shfl(dl, lo), ek = __shfl(el, lo);
+    double dk = readlane_d(dl, lo), ek = readlane_d(el, lo);
     double x = dk - mu, z = ek;
     cd z0 = (lane < n) ? AT(Z, lane, lo) : mkc(0.0, 0.0);
+    // d_k+1 and e_k+1 are read before the step that changes them: from a copy taken here, so the
+    // reads do not wait on the chain of updates
+    const double dsnap = dl, esnap = el;
     for (int k = lo; k < hi; ++k) {
-      const double dk1 = __shfl(dl, k + 1);
-      const double ek1 = (k + 1 < hi) ? __shfl(el, k + 1) : 0.0;
+      const double dk1 = readlane_d(dsnap, k + 1);
+      const double ek1 = (k + 1 < hi) ? readlane_d(esnap, k + 1) : 0.0;
       const cd z1 = (lane < n) ? AT(Z, lane, k + 1) : mkc(0.0, 0.0);
       const double rr2 = x * x + z * z;
       double cs = 1.0, sn = 0.0, r = x;
@@ -454,7 +542,7 @@ __global__ __launch_bounds__(ST) void k_small_1d(int hu, int L, int mode, const 
   for (int l = 0; l < L; ++l) {
     const long long b = (long long)p * L + l;
     int lflags = 0;   // the pair's flags
-    // operator -> M0 (TM: epx in M2 for the modes' V)
+    // operator: TE Kx^2 - epy -> M0; TM B = Kx epz_i Kx - I -> M1 and epx -> M2 (A = epx B)
     if (!tm) {
       const cd* EY = epy.ptr(b);
       for (int e = lane; e < n * n; e += ST) {
@@ -473,58 +561,67 @@ __global__ __launch_bounds__(ST) void k_small_1d(int hu, int L, int mode, const 
         AT(M[1], i, j) = v;
         AT(M[2], i, j) = EX[e];
       }
-      w_mm(n, M[2], M[1], nullptr, M[0]);
     }
-    // a Hermitian operator (TE through a lossless layer: Kx^2 - epy with epy Hermitian to within
-    // rounding) takes the symmetric eigensolver, symmetrized first
-    bool herm = false;
+    // The eigensolver. A lossless layer makes the problem Hermitian to within rounding: TE's
+    // operator itself (eig 1: the symmetric solver, W unitary); TM's pencil, epx B with epx
+    // Hermitian positive definite and B Hermitian (eig 2: with epx = C C^H the similar operator
+    // C^H B C is Hermitian; its eigenvectors Y give W = C Y, V = epx^-1 W q = C^-H Y q and
+    // W^-1 = Y^H C^-1). Otherwise (eig 0) the Schur iteration of the full operator.
+    int eig = 0;
     if (!tm) {
-      double dmax = 0.0, amax = 0.0;
-      for (int e = lane; e < n * n; e += ST) {
-        const int i = e / n, j = e % n;
-        dmax = fmax(dmax, cabs1(csub(AT(M[0], i, j), cconj(AT(M[0], j, i)))));
-        amax = fmax(amax, cabs1(AT(M[0], i, j)));
+      if (w_is_herm(n, M[0])) { w_symmetrize(n, M[0]); eig = 1; }
+    } else {
+      if (w_is_herm(n, M[1]) && w_is_herm(n, M[2])) {
+        for (int e = lane; e < n * n; e += ST) AT(M[3], e / n, e % n) = AT(M[2], e / n, e % n);
+        w_symmetrize(n, M[3]);
+        if (w_chol(n, M[3]) == 0) eig = 2;
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        dmax = fmax(dmax, __shfl_xor(dmax, off));
-        amax = fmax(amax, __shfl_xor(amax, off));
+      if (eig == 2) {
+        w_symmetrize(n, M[1]);
+        w_trinv_lower(n, M[3], M[6]);                    // C^-1
+        w_mm_op(n, 0, M[1], 0, M[3], nullptr, M[7]);     // B C
+        w_mm_op(n, 1, M[3], 0, M[7], nullptr, M[0]);     // C^H B C
+        w_symmetrize(n, M[0]);
+      } else {
+        w_mm(n, M[2], M[1], nullptr, M[0]);              // epx B
       }
-      herm = dmax <= 64.0 * ULP * amax;
-      if (herm)
-        for (int e = lane; e < n * n; e += ST) {
-          const int i = e / n, j = e % n;
-          if (i <= j) {
-            const cd h = cscale(cadd(AT(M[0], i, j), cconj(AT(M[0], j, i))), 0.5);
-            AT(M[0], i, j) = h;
-            AT(M[0], j, i) = cconj(h);
-          }
-        }
     }
-    // eigensystem: Hessenberg with Z in M1; general: Schur form, eigenvectors X in M3, W = Z X
-    // in M4; Hermitian: tridiagonal QR, W = Z
+    // Hessenberg with Z in M1; general: Schur form, eigenvectors X in M3, W = Z X in M4;
+    // Hermitian: tridiagonal QR, the eigenvectors in M1
     w_identity(n, M[1]);
     w_gehrd(n, M[0], M[1], S);
-    if (herm) {
+    if (eig) {
       if (w_tridiag_qr(n, M[0], M[1], S.w)) lflags |= WB_FLAG_NOCONVERGE;
-      for (int e = lane; e < n * n; e += ST) AT(M[4], e / n, e % n) = AT(M[1], e / n, e % n);
+      if (eig == 1) {
+        for (int e = lane; e < n * n; e += ST) AT(M[4], e / n, e % n) = AT(M[1], e / n, e % n);
+        w_colnorm(n, M[4]);
+      } else {
+        w_mm_op(n, 0, M[3], 0, M[1], nullptr, M[4]);     // W = C Y
+      }
     } else {
       const double dfl = ULP * w_hess_norm(n, M[0]);
       if (wave_qr<true>(M[0], M[1], n, SLD, n, S.w, dfl)) lflags |= WB_FLAG_NOCONVERGE;
       w_trevc(n, M[0], M[3]);
       w_mm(n, M[1], M[3], nullptr, M[4]);
+      w_colnorm(n, M[4]);
     }
-    w_colnorm(n, M[4]);
     if (lane < n) S.q[lane] = mode_q(S.w[lane]);
-    // V = W q (TE) or epx^-1 W q (TM) -> M5
-    for (int e = lane; e < n * n; e += ST) {
-      const int i = e / n, j = e % n;
-      AT(M[5], i, j) = cmul(AT(M[4], i, j), S.q[j]);
-    }
-    if (tm) {
-      for (int e = lane; e < n * n; e += ST) AT(M[6], e / n, e % n) = AT(M[2], e / n, e % n);
-      w_getrf(n, M[6], S.piv);
-      w_getrs(n, n, M[6], S.piv, M[5], S);
+    // V = W q (TE) or epx^-1 W q (TM) -> M5 (eig 2: C^-H Y q, and W^-1 = Y^H C^-1 -> M0 while
+    // C^-1 is still in M6)
+    if (eig == 2) {
+      w_mm_op(n, 1, M[6], 0, M[1], nullptr, M[5]);
+      for (int e = lane; e < n * n; e += ST) AT(M[5], e / n, e % n) = cmul(AT(M[5], e / n, e % n), S.q[e % n]);
+      w_mm_op(n, 1, M[1], 0, M[6], nullptr, M[0]);
+    } else {
+      for (int e = lane; e < n * n; e += ST) {
+        const int i = e / n, j = e % n;
+        AT(M[5], i, j) = cmul(AT(M[4], i, j), S.q[j]);
+      }
+      if (tm) {
+        for (int e = lane; e < n * n; e += ST) AT(M[6], e / n, e % n) = AT(M[2], e / n, e % n);
+        w_getrf(n, M[6], S.piv);
+        w_getrs(n, n, M[6], S.piv, M[5], S);
+      }
     }
     // A'^T -> M6, B'^T -> M7 (gap: wg = 1, vg = -i y)
     for (int e = lane; e < n * n; e += ST) {
@@ -535,12 +632,12 @@ __global__ __launch_bounds__(ST) void k_small_1d(int hu, int L, int mode, const 
       AT(M[6], j, i) = cadd(Pw, Qv);
       AT(M[7], j, i) = csub(Pw, Qv);
     }
-    // W^-1 -> M0 (W^H for the Hermitian operator's unitary W); V^-1 = q^-1 W^-1 (TE) or
+    // W^-1 -> M0 (eig 1: W^H, W unitary; eig 2: formed above); V^-1 = q^-1 W^-1 (TE) or
     // q^-1 W^-1 epx (TM) -> M1
     int iw = 0;
-    if (herm) {
+    if (eig == 1) {
       for (int e = lane; e < n * n; e += ST) AT(M[0], e / n, e % n) = cconj(AT(M[4], e % n, e / n));
-    } else {
+    } else if (eig == 0) {
       iw = w_getrf(n, M[4], S.piv);
       w_identity(n, M[0]);
       w_getrs(n, n, M[4], S.piv, M[0], S);
