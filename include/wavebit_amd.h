@@ -345,22 +345,26 @@ int wb_zgetrs_batched_ops(wb_ctx* ctx, int op, int n, int nrhs, const double* LU
  *   d_epx, d_epy, d_epz   n_conv x N x N complex: the cotangents (d/dRe + i d/dIm of a real
  *                         objective) of the conv matrices, summed over the (point, layer)
  *                         pairs that use each slot
+ *   d_epxy, d_epyx        likewise for the off-diagonal blocks when the solve had them (an
+ *                         NVF stack: epxy and epyx both given to wb_solve_keep and here;
+ *                         both NULL otherwise, and then d_epxy, d_epyx may be NULL too)
  *   d_thick               P x L doubles: d/d thickness of each (point, layer).
  * The eigenbasis is held fixed and the modal constants vary as a matrix in it (the generator
  * rule), which gives the exact derivative of the efficiencies. */
 size_t wb_solve_checkpoint_size(int P, int L, int hu, int hv, int mode);
 int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                   const double* thick, const int* conv_idx, const double* epx, const double* epy,
-                  const double* epz_i, double* eff, double* amp, double* pe, int* flags,
-                  void* workspace, size_t workspace_bytes, void* checkpoint,
-                  size_t checkpoint_bytes, void* stream);
+                  const double* epz_i, const double* epxy, const double* epyx, double* eff,
+                  double* amp, double* pe, int* flags, void* workspace, size_t workspace_bytes,
+                  void* checkpoint, size_t checkpoint_bytes, void* stream);
 size_t wb_solve_adjoint_workspace_size(int P, int L, int hu, int hv, int mode);
 int wb_solve_adjoint(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                      const double* thick, const int* conv_idx, int n_conv, const double* epx,
-                     const double* epy, const double* epz_i, const void* solve_workspace,
-                     const void* checkpoint, const double* eff_bar, double* d_epx, double* d_epy,
-                     double* d_epz, double* d_thick, void* workspace, size_t workspace_bytes,
-                     void* stream);
+                     const double* epy, const double* epz_i, const double* epxy,
+                     const double* epyx, const void* solve_workspace, const void* checkpoint,
+                     const double* eff_bar, double* d_epx, double* d_epy, double* d_epz,
+                     double* d_epxy, double* d_epyx, double* d_thick, void* workspace,
+                     size_t workspace_bytes, void* stream);
 
 /* Cotangent of the raster eps (L, Ny, Nx) from those of wb_assemble_li's outputs; epx and epz_i
  * are that call's outputs for the same eps. hu, hv <= 15. */
@@ -377,6 +381,19 @@ int wb_assemble_laurent_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, i
                                 const double* epz_i, const double* d_epx, const double* d_epy,
                                 const double* d_epz, double* d_eps, void* workspace,
                                 size_t workspace_bytes, void* stream);
+
+/* Cotangent of the raster eps (L, Ny, Nx) from those of wb_assemble_nvf's outputs (floor 0):
+ * d_off is the sum of the cotangents of the two off-diagonal blocks (epxy = epyx). The normal
+ * field is held fixed -- it is the geometry of the interfaces, which an infinitesimal change of
+ * the values does not move (the reference's gradient treats it so: tests/test_frozen_geometry.py
+ * :644) -- and only passive layers (Re eps > 0, Im eps >= 0 everywhere, where the floor of
+ * factorization.py:1968-2160 is inactive) are differentiated; the caller refuses the others.
+ * The forward is rebuilt inside (its workspace is part of this one). hu, hv <= 15. */
+size_t wb_assemble_nvf_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv);
+int wb_assemble_nvf_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                            const double* lattice, const double* d_epx, const double* d_epy,
+                            const double* d_epz, const double* d_off, double* d_eps,
+                            void* workspace, size_t workspace_bytes, void* stream);
 
 /* Per-kernel and per-stage timing (no reference counterpart: measurement support).
  * wb_timing_enable(1) brackets the launches of the kernels below by hipEvent records on their

@@ -183,14 +183,3 @@ def test_a_served_nvf_request_equals_its_run_alone():
         alone = rcwa.run(prob, numerics=num, **RUN)
         assert np.abs(np.asarray(got.R) - np.asarray(alone.R)).max() < 1e-12
     assert np.abs(np.asarray(results[0].R) - np.asarray(results[1].R)).max() > 1e-6
-
-
-def test_nvf_gradients_are_refused_by_name():
-    eps = torch.tensor(_stack(1, (8, 8), 41).real, dtype=torch.float64, requires_grad=True)
-    layers = [rcwa.Layer(thickness=0.2, epsilon=eps[0])]
-    prob = rcwa.Problem(stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(1.0),
-                                         bottom=wavebit_amd.Material(2.25), period=(1.0, 1.0)),
-                        excitation=rcwa.Excitation(wavelength=0.8, sources=(rcwa.PlaneWave(polarization='p'),)))
-    with pytest.raises(wavebit_amd.errors.UnsupportedFeatureError) as e:
-        rcwa.run(prob, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'), **RUN)
-    assert e.value.code == 'rcwa.gradient'

@@ -1,0 +1,171 @@
+"""Gradients under the normal-vector-field rule: the device reverse pass (``wb_solve_adjoint``
+with the off-diagonal blocks, then ``wb_assemble_nvf_adjoint``) against central differences of
+the CPU oracle's NVF solve with the normal field held at the unperturbed raster's -- the
+derivative the reference's gradient takes (tests/test_frozen_geometry.py:644: "the NVF gradient
+holds the interfaces fixed"). A perturbed pixel would otherwise add or remove an interface and
+move the normal field by a finite step."""
+import numpy as np
+import pytest
+import torch
+
+import wavebit_amd
+from oracle import rcwa_nvf as nvf
+from oracle import rcwa_oracle as orc
+from wavebit_amd import rcwa
+from wavebit_amd.rcwa import _engine, adjoint
+
+from test_oracle_nvf import _blobs
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device('cuda:0')
+RUN = dict(backend='torch', device=DEV)
+
+
+def _d5(f, h=1e-3):
+    """Fourth-order five-point difference of f at 0 (truncation ~h^4 f^(5), rounding ~eps/h)."""
+    return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+
+def _frozen(monkeypatch, base, hu, hv, lattice):
+    """Make the oracle's NVF builder use each base layer's own normal field."""
+    normals = [nvf.normal_vector_field(nvf.lift(b, hu, hv), lattice) for b in base]
+
+    def rule(factorization='li', coefficients='sampled', lattice=None):
+        def build(e, hu_, hv_):
+            layer = int(np.argmin([np.abs(e - b).max() for b in base]))
+            return nvf.nvf_conv_matrices(e, hu_, hv_, lattice=lattice, normal=normals[layer])
+        return build
+    monkeypatch.setattr(orc, 'conv_rule', rule)
+
+
+def _total(eps, thick, period, points, pol, orders):
+    return sum(float(orc.solve_raster(eps, thick, 1.0, 2.25, np.asarray(period), wl, th, ph, pol,
+                                      orders, factorization='nvf')['incident']['T'].sum())
+               for wl, th, ph in points)
+
+
+def test_solve_adjoint_off_diagonal_cotangents_match_differences_of_the_forward():
+    """d_epxy + d_epyx of wb_solve_adjoint (an NVF stack's shared off-diagonal slot) and the
+    diagonal slots' cotangents against central differences of the device forward."""
+    from wavebit_amd import _native
+    hu, hv, L, P = 2, 1, 2, 3
+    eps = np.stack([_blobs((10, 12), 5, (1.0, 4.0 + 0.2j)), _blobs((10, 12), 8, (2.0, 7.0))])
+    epx, epy, epz, epxy, info = _native.assemble_nvf(torch.from_numpy(eps).to(DEV), hu, hv,
+                                                      nvf.lattice_of((0.8, 0.7)), floor=False)
+    assert int(info[:2 * L].abs().sum()) == 0
+    thick = [0.21, 0.16]
+    pts = [_engine.Point(stack=0, wavelength=0.8 + 0.1 * i, theta=0.1 + 0.15 * i, phi=0.3 + 0.5 * i,
+                         polarization=('s', 'p', (0.6, 0.8j))[i], eps_top=1.0, eps_bottom=2.25,
+                         period=(0.8, 0.7), thickness=tuple(thick)) for i in range(P)]
+    rec = adjoint.point_records(pts, DEV)
+    th = torch.tensor(np.tile(thick, (P, 1)), dtype=torch.float64, device=DEV)
+    idx = torch.tensor(np.tile(np.arange(L, dtype=np.int32), (P, 1)), device=DEV)
+    N = (2 * hu + 1) * (2 * hv + 1)
+    eb = np.random.default_rng(3).normal(size=(P, 3, 4, N))
+    eff, _, _, flags, state = _native.solve_keep(rec, th, idx, epx, epy, epz, hu, hv, epxy=epxy,
+                                                 epyx=epxy)
+    assert int((flags & 3).sum()) == 0
+    d = _native.solve_adjoint(state, torch.from_numpy(eb).to(DEV))
+    assert len(d) == 6
+    grads = [d[0], d[1], d[2], d[3] + d[4]]
+    grads = [g.cpu().numpy() for g in grads]
+
+    def objective(cv):
+        e, *_ = _native.solve(rec, th, idx, cv[0], cv[1], cv[2], hu, hv, _native.MODE_2D, epxy=cv[3])
+        return float(np.sum(e.cpu().numpy() * eb))
+    assert abs(objective([epx, epy, epz, epxy]) - float(np.sum(eff.cpu().numpy() * eb))) < 1e-12
+    rng = np.random.default_rng(11)
+    for which in range(4):
+        for _ in range(3):
+            l = int(rng.integers(0, L))
+            i, j = (int(v) for v in rng.integers(0, N, 2))
+            for dz in (1.0, 1j):
+                def shifted(t):
+                    cv = [c.clone() for c in (epx, epy, epz, epxy)]
+                    cv[which][l, i, j] += t * dz
+                    return objective(cv)
+                fd = _d5(shifted)
+                an = grads[which][l, i, j].real if dz == 1.0 else grads[which][l, i, j].imag
+                assert abs(an - fd) <= 1e-6 * abs(fd) + 1e-8, (which, l, i, j, dz, an, fd)
+
+
+@pytest.mark.parametrize('period', [(1.0, 1.0), (0.9, 0.7), np.array([[1.0, 0.35], [0.0, 0.9]])])
+def test_nvf_gradient_matches_frozen_normal_differences(monkeypatch, period):
+    hu = hv = 2
+    eps = np.stack([_blobs((12, 12), 3, (1.0, 4.0 + 0.3j)), _blobs((12, 12), 9, (2.25, 6.0))])
+    thick = [0.17, 0.23]
+    pts = [(0.83, 0.2, 0.5), (1.07, 0.0, 0.0)]
+    pol = 'p'
+    lattice = nvf.lattice_of(period)
+    points = [_engine.Point(stack=0, wavelength=wl, theta=th, phi=ph, polarization=pol,
+                            eps_top=1.0, eps_bottom=2.25, period=_engine_period(period),
+                            thickness=tuple(thick)) for wl, th, ph in pts]
+    eps_t = torch.tensor(eps, requires_grad=True)
+    thick_t = torch.tensor(thick, dtype=torch.float64, requires_grad=True)
+    eff, amp, pe, flags = adjoint.solve(eps_t, thick_t, points, hu, hv, DEV, factorization='nvf',
+                                        lattice=lattice)
+    assert int((flags & 3).sum()) == 0
+    obj = (eff[:, 0, 2] + eff[:, 0, 3]).sum()
+    obj.backward()
+    g_eps, g_t = eps_t.grad.numpy(), thick_t.grad.numpy()
+
+    _frozen(monkeypatch, list(eps), hu, hv, lattice)
+    base = _total(eps, thick, period, pts, pol, (hu, hv))
+    assert abs(float(obj.detach()) - base) < 1e-10
+    scale = np.abs(g_eps).max()
+    # pixels on and off the interfaces of both layers, real and imaginary directions
+    pix = [(0, 0, 0), (0, 5, 6), (1, 3, 9), (1, 11, 2), (0, 8, 1), (1, 6, 6)]
+    for where in pix:
+        for unit, part in ((1.0, 'real'), (1j, 'imag')):
+            def shifted(t):
+                e = eps.copy()
+                e[where] += t * unit
+                return _total(e, thick, period, pts, pol, (hu, hv))
+            fd = _d5(shifted)
+            got = getattr(g_eps[where], part)
+            assert abs(got - fd) <= 1e-6 * abs(fd) + 1e-8 * scale, (where, part, got, fd)
+    for l in range(2):
+        def shifted(t):
+            tt = list(thick)
+            tt[l] += t
+            return _total(eps, tt, period, pts, pol, (hu, hv))
+        fd = _d5(shifted, 1e-4)     # evanescent orders: thickness derivatives grow as (k0 |q|)^n
+        assert abs(g_t[l] - fd) <= 1e-6 * abs(fd) + 1e-8, (l, g_t[l], fd)
+
+
+def _engine_period(period):
+    """The engine's period record: (Lx, Ly) or a skewed cell's (a1x, a2y, a1y, a2x)."""
+    p = np.asarray(period, dtype=np.float64)
+    if p.shape == (2,):
+        return tuple(p)
+    return (p[0, 0], p[1, 1], p[1, 0], p[0, 1])
+
+
+def test_nvf_run_gradient_equals_the_reverse_pass_and_refuses_metal():
+    """rcwa.run with tensor leaves under NVF gives the autograd function's gradient; a stack
+    with a metal layer (where the reference floors [[1/eps]]) is refused by name."""
+    eps0 = np.stack([_blobs((10, 10), 4, (1.0, 5.0))]).real
+    eps = torch.tensor(eps0, dtype=torch.float64, requires_grad=True)
+    layers = [rcwa.Layer(thickness=0.2, epsilon=eps[0])]
+    prob = rcwa.Problem(stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(1.0),
+                                         bottom=wavebit_amd.Material(2.25), period=(1.0, 1.0)),
+                        excitation=rcwa.Excitation(wavelength=0.8, sources=(rcwa.PlaneWave(polarization='p'),)))
+    res = rcwa.run(prob, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'), **RUN)
+    res.T.sum().backward()
+    g_run = eps.grad.numpy().copy()
+    e2 = torch.tensor(eps0, dtype=torch.float64, requires_grad=True)
+    pt = _engine.Point(stack=0, wavelength=0.8, theta=0.0, phi=0.0, polarization='p', eps_top=1.0,
+                       eps_bottom=2.25, period=(1.0, 1.0), thickness=(0.2,))
+    eff, _, _, _ = adjoint.solve(e2, torch.tensor([0.2], dtype=torch.float64), [pt], 1, 1, DEV,
+                                 factorization='nvf', lattice=nvf.lattice_of((1.0, 1.0)))
+    (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()
+    assert np.abs(g_run - e2.grad.numpy()).max() < 1e-12 * np.abs(g_run).max() + 1e-15
+    metal = torch.tensor(eps0 * 0 + np.where(eps0 > 2, -20.0 + 1.0j, 1.0), dtype=torch.complex128,
+                         requires_grad=True)
+    layers = [rcwa.Layer(thickness=0.2, epsilon=metal[0])]
+    prob = rcwa.Problem(stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(1.0),
+                                         bottom=wavebit_amd.Material(2.25), period=(1.0, 1.0)),
+                        excitation=rcwa.Excitation(wavelength=0.8, sources=(rcwa.PlaneWave(polarization='p'),)))
+    with pytest.raises(wavebit_amd.errors.UnsupportedFeatureError) as e:
+        rcwa.run(prob, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'), **RUN)
+    assert e.value.code == 'rcwa.gradient'

@@ -33,6 +33,7 @@ EXPORTS = ('wb_version', 'wb_ctx_create', 'wb_ctx_destroy', 'wb_last_error', 'wb
            'wb_solve_adjoint', 'wb_assemble_li_adjoint_workspace_size', 'wb_assemble_li_adjoint',
            'wb_assemble_laurent_workspace_size', 'wb_assemble_laurent',
            'wb_assemble_laurent_adjoint_workspace_size', 'wb_assemble_laurent_adjoint',
+           'wb_assemble_nvf_adjoint_workspace_size', 'wb_assemble_nvf_adjoint',
            'wb_solve_scattering', 'wb_timing_enable', 'wb_timing_read', 'wb_gemm_tuning_configs',
            'wb_gemm_tuning_config_name', 'wb_gemm_tuning_record', 'wb_gemm_autotune_size',
            'wb_gemm_autotune', 'wb_gemm_tuning_table', 'wb_gemm_tuning_load', 'wb_gemm_tuning_set',
@@ -124,12 +125,12 @@ def load_library():
                                               c_vp, c_int, c_ll, c_int, c_vp]),
             'wb_solve_checkpoint_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve_keep': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
-                                      c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp,
-                                      c_sz, c_vp]),
+                                      c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                      c_sz, c_vp, c_sz, c_vp]),
             'wb_solve_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_solve_adjoint': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                          c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                                         c_vp, c_vp, c_vp, c_sz, c_vp]),
+                                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
             'wb_assemble_li_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
             'wb_assemble_li_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
                                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
@@ -141,6 +142,10 @@ def load_library():
             'wb_assemble_laurent_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int,
                                                     c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
                                                     c_vp]),
+            'wb_assemble_nvf_adjoint_workspace_size': (c_sz, [c_int, c_int, c_int, c_int, c_int]),
+            'wb_assemble_nvf_adjoint': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int,
+                                                c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                                c_vp]),
             'wb_solve_scattering': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                             c_vp, c_vp, c_vp, c_sz, c_vp]),
@@ -337,6 +342,14 @@ def assemble_laurent(eps, hu, hv):
     return out[0], out[1], out[2], info
 
 
+def _cell(lattice):
+    """The cell basis (2 x 2, columns a1 and a2) as the C-ABI's four doubles, or None."""
+    if lattice is None:
+        return None
+    A = np.asarray(lattice, dtype=np.float64).reshape(2, 2)
+    return (ctypes.c_double * 4)(A[0, 0], A[1, 0], A[0, 1], A[1, 1])
+
+
 def assemble_nvf(eps, hu, hv, lattice=None, floor=True):
     """NVF convolution matrices of a raster stack eps (L, Ny, Nx) -> (epx, epy, epz_i, epxy,
     info): epxy is the solve's epxy and epyx slot alike; ``lattice`` the cell basis (2 x 2,
@@ -352,11 +365,8 @@ def assemble_nvf(eps, hu, hv, lattice=None, floor=True):
     info = torch.empty((3 * L,), dtype=torch.int32, device=device)
     nbytes = ctx.lib.wb_assemble_nvf_workspace_size(L, Ny, Nx, hu, hv)
     ws = _workspace(nbytes, device)
-    cell = None
-    if lattice is not None:
-        A = np.asarray(lattice, dtype=np.float64).reshape(2, 2)
-        cell = (ctypes.c_double * 4)(A[0, 0], A[1, 0], A[0, 1], A[1, 1])
-    ctx.check(ctx.lib.wb_assemble_nvf(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, cell, int(bool(floor)),
+    ctx.check(ctx.lib.wb_assemble_nvf(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv, _cell(lattice),
+                                      int(bool(floor)),
                                       _ptr(out[0]),
                                       _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _ptr(ws), nbytes,
                                       _ptr(info), _stream(device)), 'assemble_nvf')
@@ -748,10 +758,12 @@ def checkpoint_bytes(P, L, hu, hv, mode=MODE_2D):
     return int(load_library().wb_solve_checkpoint_size(P, L, hu, hv, mode))
 
 
-def solve_keep(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode=MODE_2D):
+def solve_keep(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode=MODE_2D, epxy=None,
+               epyx=None):
     """``solve`` that also keeps the checkpoint of its reverse pass. Returns
     (eff, amp, pe, flags, state); ``state`` holds the workspace and checkpoint tensors and is
-    consumed by ``solve_adjoint``."""
+    consumed by ``solve_adjoint``. ``epxy``/``epyx``: the off-diagonal blocks of an NVF stack
+    (both or neither)."""
     device = epx.device
     ctx = context(device)
     P, L = thick.shape
@@ -761,6 +773,11 @@ def solve_keep(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode=MODE_2D):
     _require(conv_idx, torch.int32, device, 'conv_idx')
     for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
         _require(t, torch.complex128, device, name)
+    if (epxy is None) != (epyx is None):
+        raise ValueError('solve_keep: give both epxy and epyx, or neither')
+    for name, t in (('epxy', epxy), ('epyx', epyx)):
+        if t is not None:
+            _require(t, torch.complex128, device, name)
     eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
     amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
     pe = torch.empty((P, 3), dtype=torch.float64, device=device)
@@ -770,17 +787,19 @@ def solve_keep(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode=MODE_2D):
     ws = _workspace(nbytes, device)
     ck = _workspace(ckbytes, device)
     ctx.check(ctx.lib.wb_solve_keep(ctx.handle, P, L, hu, hv, mode, _ptr(pts), _ptr(thick),
-                                    _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i), _ptr(eff),
-                                    _ptr(amp), _ptr(pe), _ptr(flags), _ptr(ws), nbytes, _ptr(ck),
-                                    ckbytes, _stream(device)), 'solve_keep')
+                                    _ptr(conv_idx), _ptr(epx), _ptr(epy), _ptr(epz_i),
+                                    _ptr(epxy), _ptr(epyx), _ptr(eff), _ptr(amp), _ptr(pe),
+                                    _ptr(flags), _ptr(ws), nbytes, _ptr(ck), ckbytes,
+                                    _stream(device)), 'solve_keep')
     state = dict(ws=ws, ck=ck, pts=pts, thick=thick, conv_idx=conv_idx, epx=epx, epy=epy,
-                 epz_i=epz_i, hu=hu, hv=hv, mode=mode)
+                 epz_i=epz_i, epxy=epxy, epyx=epyx, hu=hu, hv=hv, mode=mode)
     return eff, amp, pe, flags, state
 
 
 def solve_adjoint(state, eff_bar, n_conv=None):
     """Reverse pass of ``solve_keep``: cotangent of eff (P, 3, 4, N) -> (d_epx, d_epy, d_epz,
-    d_thick); conv cotangents are summed per conv slot."""
+    d_thick), with (d_epxy, d_epyx) before d_thick when the solve had off-diagonal blocks;
+    conv cotangents are summed per conv slot."""
     epx = state['epx']
     device = epx.device
     ctx = context(device)
@@ -791,16 +810,23 @@ def solve_adjoint(state, eff_bar, n_conv=None):
     eff_bar = eff_bar.to(device=device, dtype=torch.float64).contiguous()
     if tuple(eff_bar.shape) != (P, 3, 4, N):
         raise ValueError(f'eff_bar must have shape {(P, 3, 4, N)}, got {tuple(eff_bar.shape)}')
-    d = [torch.empty((C, N, N), dtype=torch.complex128, device=device) for _ in range(3)]
+    offd = state.get('epxy') is not None
+    d = [torch.empty((C, N, N), dtype=torch.complex128, device=device)
+         for _ in range(5 if offd else 3)]
     d_thick = torch.empty((P, L), dtype=torch.float64, device=device)
     nbytes = ctx.lib.wb_solve_adjoint_workspace_size(P, L, hu, hv, mode)
     ws = _workspace(nbytes, device)
     ctx.check(ctx.lib.wb_solve_adjoint(ctx.handle, P, L, hu, hv, mode, _ptr(state['pts']),
                                        _ptr(state['thick']), _ptr(state['conv_idx']), C, _ptr(epx),
-                                       _ptr(state['epy']), _ptr(state['epz_i']), _ptr(state['ws']),
-                                       _ptr(state['ck']), _ptr(eff_bar), _ptr(d[0]), _ptr(d[1]),
-                                       _ptr(d[2]), _ptr(d_thick), _ptr(ws), nbytes,
+                                       _ptr(state['epy']), _ptr(state['epz_i']),
+                                       _ptr(state.get('epxy')), _ptr(state.get('epyx')),
+                                       _ptr(state['ws']), _ptr(state['ck']), _ptr(eff_bar),
+                                       _ptr(d[0]), _ptr(d[1]), _ptr(d[2]),
+                                       _ptr(d[3] if offd else None), _ptr(d[4] if offd else None),
+                                       _ptr(d_thick), _ptr(ws), nbytes,
                                        _stream(device)), 'solve_adjoint')
+    if offd:
+        return d[0], d[1], d[2], d[3], d[4], d_thick
     return d[0], d[1], d[2], d_thick
 
 
@@ -839,6 +865,26 @@ def assemble_laurent_adjoint(eps, hu, hv, epz_i, d_epx, d_epy, d_epz):
                                                   _ptr(epz_i), _ptr(d_epx), _ptr(d_epy),
                                                   _ptr(d_epz), _ptr(d_eps), _ptr(ws), nbytes,
                                                   _stream(device)), 'assemble_laurent_adjoint')
+    return d_eps
+
+
+def assemble_nvf_adjoint(eps, hu, hv, d_epx, d_epy, d_epz, d_off, lattice=None):
+    """Cotangent of the raster stack eps (L, Ny, Nx) from those of ``assemble_nvf``'s outputs
+    (without the floor; ``d_off`` the summed cotangent of the epxy and epyx slots), the normal
+    field held fixed."""
+    device = eps.device
+    ctx = context(device)
+    _require(eps, torch.complex128, device, 'eps')
+    for name, t in (('d_epx', d_epx), ('d_epy', d_epy), ('d_epz', d_epz), ('d_off', d_off)):
+        _require(t, torch.complex128, device, name)
+    L, Ny, Nx = eps.shape
+    d_eps = torch.empty_like(eps)
+    nbytes = ctx.lib.wb_assemble_nvf_adjoint_workspace_size(L, Ny, Nx, hu, hv)
+    ws = _workspace(nbytes, device)
+    ctx.check(ctx.lib.wb_assemble_nvf_adjoint(ctx.handle, _ptr(eps), L, Ny, Nx, hu, hv,
+                                              _cell(lattice), _ptr(d_epx), _ptr(d_epy),
+                                              _ptr(d_epz), _ptr(d_off), _ptr(d_eps), _ptr(ws),
+                                              nbytes, _stream(device)), 'assemble_nvf_adjoint')
     return d_eps
 
 

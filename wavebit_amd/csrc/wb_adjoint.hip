@@ -274,8 +274,10 @@ __global__ void k_be_cd(int N, BatchPtr epz, const cd* ov, cd* becd) {
 }
 
 // Per pair: epz-bar from BEb / CDb (n x N each, in T2), and the Omega_R blocks of ORt (= Ub W^H
-// before the R'^T row rotation) added into epx-bar / epy-bar. E3 holds [epx | epy | epz] bars.
-__global__ void k_adj_conv(int N, const cd* BECDb, const cd* ORt, const cd* ov, cd* E3) {
+// before the R'^T row rotation) added into epx-bar / epy-bar (and, with the off-diagonal slots,
+// Omega_R[:N, :N] = -KxKy + epyx into epyx-bar, Omega_R[N:, N:] = KyKx - epxy into epxy-bar).
+// E3 holds [epx | epy | epz | epxy | epyx] bars.
+__global__ void k_adj_conv(int N, const cd* BECDb, const cd* ORt, const cd* ov, int offd, cd* E3) {
   const long long p = blockIdx.y;
   const int n = 2 * N;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -291,7 +293,7 @@ __global__ void k_adj_conv(int N, const cd* BECDb, const cd* ORt, const cd* ov, 
   z = cadd(z, cmul(cmul(kyi, be[(long long)(N + i) * N + j]), kxj));
   z = cadd(z, cmul(cmul(kxi, cdm[(long long)i * N + j]), kyj));
   z = cadd(z, cmul(cmul(kyi, cdm[(long long)(N + i) * N + j]), kyj));
-  cd* E = E3 + p * 3 * NN;
+  cd* E = E3 + p * 5 * NN;
   E[2 * NN + e] = z;
   // R'^T U: top = c U_top - s U_bot, bottom = s U_top + c U_bot (c, s of the row's order)
   const double ci = o[OV_FC * N + i].x, si = o[OV_FS * N + i].x;
@@ -302,11 +304,17 @@ __global__ void k_adj_conv(int N, const cd* BECDb, const cd* ORt, const cd* ov, 
   const cd or_bl = cadd(cscale(top_l, si), cscale(bot_l, ci));   // ORb[N:, :N]
   E[e] = cadd(E[e], or_bl);                 // Omega_R[N:, :N] = epx - Ky^2
   E[NN + e] = csub(E[NN + e], or_tr);       // Omega_R[:N, N:] = Kx^2 - epy
+  if (offd) {
+    const cd or_tl = csub(cscale(top_l, ci), cscale(bot_l, si));   // ORb[:N, :N]
+    const cd or_br = cadd(cscale(top_r, si), cscale(bot_r, ci));   // ORb[N:, N:]
+    E[4 * NN + e] = cadd(E[4 * NN + e], or_tl);
+    E[3 * NN + e] = csub(E[3 * NN + e], or_br);
+  }
 }
 
 // out[c] += sum over points p with conv_idx[p * L + l] == c of E3[p][slot] (slot grid z).
 __global__ void k_reduce_conv(int P, int L, int l, long long NN, const int* conv_idx, const cd* E3,
-                              cd* d_epx, cd* d_epy, cd* d_epz) {
+                              cd* d_epx, cd* d_epy, cd* d_epz, cd* d_epxy, cd* d_epyx) {
   const int c = blockIdx.y, slot = blockIdx.z;
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= NN) return;
@@ -314,11 +322,12 @@ __global__ void k_reduce_conv(int P, int L, int l, long long NN, const int* conv
   bool any = false;
   for (int p = 0; p < P; ++p) {
     if (conv_idx[(long long)p * L + l] != c) continue;
-    acc = cadd(acc, E3[((long long)p * 3 + slot) * NN + e]);
+    acc = cadd(acc, E3[((long long)p * 5 + slot) * NN + e]);
     any = true;
   }
   if (!any) return;
-  cd* out = (slot == 0 ? d_epx : slot == 1 ? d_epy : d_epz) + (long long)c * NN + e;
+  cd* const outs[5] = {d_epx, d_epy, d_epz, d_epxy, d_epyx};
+  cd* out = outs[slot] + (long long)c * NN + e;
   *out = cadd(*out, acc);
 }
 
@@ -349,7 +358,7 @@ AdjWs adj_carve(void* ws, int P, int L, int n, int N, bool count_only, size_t* b
                  &a.BECD};
   for (cd** m : mats) *m = (cd*)take((size_t)P * nn);
   a.SB = (cd*)take((size_t)P * nn * 2);
-  a.E3 = (cd*)take((size_t)P * 3 * N * N * sizeof(cd));
+  a.E3 = (cd*)take((size_t)P * 5 * N * N * sizeof(cd));
   cd** vecs[] = {&a.crb, &a.ctb, &a.db, &a.gub, &a.ub, &a.tv};
   for (cd** v : vecs) *v = (cd*)take((size_t)P * n * 2 * sizeof(cd));
   a.Zs = (cd*)take((size_t)P * L * n * 2 * sizeof(cd));
@@ -391,9 +400,14 @@ hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st)
   auto LY = [&](const cd* base, long long per, int l) { return bp_strided(base + (long long)l * per, (long long)L * per); };
   const long long dstride = (long long)(L + 1) * vs;
 
+  // the NVF off-diagonal slots (epxy in Omega2's second column block and Omega_R's bottom-right
+  // block, epyx in the first and the top-left), both or neither
+  const int offd = (f.epxy != nullptr && f.epyx != nullptr && a.d_epxy != nullptr && a.d_epyx != nullptr) ? 1 : 0;
+  if ((f.epxy != nullptr) != (a.d_epxy != nullptr)) return hipErrorInvalidValue;
   // outputs start at zero (fill kernels: no memset nodes in a capturable sequence)
   const long long conv_elems = (long long)a.n_conv * NN;
-  for (cd* out : {a.d_epx, a.d_epy, a.d_epz}) {
+  for (cd* out : {a.d_epx, a.d_epy, a.d_epz, a.d_epxy, a.d_epyx}) {
+    if (!out) continue;
     hipLaunchKernelGGL(k_zero_c, dim3((unsigned)((conv_elems + 255) / 256)), dim3(256), 0, st,
                        conv_elems, out);
     WB_CHECK_LAUNCH();
@@ -547,17 +561,31 @@ hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st)
     BatchPtr EPZ = bp_gather(f.epz, NN, f.conv_idx + l, L);
     hipLaunchKernelGGL(k_be_cd, grid2(NN, P), dim3(256), 0, st, N, EPZ, W.ov, T.BECD);
     WB_CHECK_LAUNCH();
-    BatchPtr E3x = S(T.E3, 3 * NN), E3y = S(T.E3 + NN, 3 * NN);
+    BatchPtr E3x = S(T.E3, 5 * NN), E3y = S(T.E3 + NN, 5 * NN);
     if ((e = wb_zgemm_ops_launch(1, 0, N, N, n, one, TT(T.BECD), N, TT(T.Bb), n, zero, E3x, N, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(1, 0, N, N, n, one, S(T.BECD + (long long)n * N, nn), N,
                                  S(T.Bb + N, nn), n, zero, E3y, N, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(0, 1, n, N, N, one, TT(T.Bb), n, EPX, N, zero, TT(T.T2), N, P, st))) return e;
     if ((e = wb_zgemm_ops_launch(0, 1, n, N, N, one, S(T.Bb + N, nn), n, EPY, N, zero,
                                  S(T.T2 + (long long)n * N, nn), N, P, st))) return e;
-    hipLaunchKernelGGL(k_adj_conv, grid2(NN, P), dim3(256), 0, st, N, T.T2, T.Ab, W.ov, T.E3);
+    if (offd) {
+      // Omega2[:, :N] -= CD epyx, Omega2[:, N:] -= BE epxy: the slots' bars, and BE / CD's
+      BatchPtr EPXY = bp_gather(f.epxy, NN, f.conv_idx + l, L);
+      BatchPtr EPYX = bp_gather(f.epyx, NN, f.conv_idx + l, L);
+      BatchPtr E3xy = S(T.E3 + 3 * NN, 5 * NN), E3yx = S(T.E3 + 4 * NN, 5 * NN);
+      if ((e = wb_zgemm_ops_launch(1, 0, N, N, n, mone, TT(T.BECD), N, S(T.Bb + N, nn), n, zero, E3xy, N,
+                                   P, st))) return e;
+      if ((e = wb_zgemm_ops_launch(1, 0, N, N, n, mone, S(T.BECD + (long long)n * N, nn), N, TT(T.Bb), n,
+                                   zero, E3yx, N, P, st))) return e;
+      if ((e = wb_zgemm_ops_launch(0, 1, n, N, N, mone, S(T.Bb + N, nn), n, EPXY, N, one, TT(T.T2), N, P,
+                                   st))) return e;
+      if ((e = wb_zgemm_ops_launch(0, 1, n, N, N, mone, TT(T.Bb), n, EPYX, N, one,
+                                   S(T.T2 + (long long)n * N, nn), N, P, st))) return e;
+    }
+    hipLaunchKernelGGL(k_adj_conv, grid2(NN, P), dim3(256), 0, st, N, T.T2, T.Ab, W.ov, offd, T.E3);
     WB_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_reduce_conv, dim3((unsigned)((NN + 255) / 256), a.n_conv, 3), dim3(256), 0, st,
-                       P, L, l, NN, f.conv_idx, T.E3, a.d_epx, a.d_epy, a.d_epz);
+    hipLaunchKernelGGL(k_reduce_conv, dim3((unsigned)((NN + 255) / 256), a.n_conv, 3 + 2 * offd), dim3(256),
+                       0, st, P, L, l, NN, f.conv_idx, T.E3, a.d_epx, a.d_epy, a.d_epz, a.d_epxy, a.d_epyx);
     WB_CHECK_LAUNCH();
   }
   return hipSuccess;

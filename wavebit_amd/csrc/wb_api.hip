@@ -182,6 +182,49 @@ int wb_assemble_laurent(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, i
                     "wb_assemble_laurent");
 }
 
+// The NVF builder's geometry from the raster's shape and the cell basis: metric = J^-T (4) and
+// the index steps (2), or a constant normal (cst, ux, uy) for a one-cell axis. False for a basis
+// that is not right-handed.
+static bool nvf_metric(int Ny, int Nx, int hu, int hv, const double* lattice, double* metric, int* cst,
+                       double* ux, double* uy) {
+  // the raster lifted to 4 h + 1 samples per axis (factorization.py:811-827)
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const int Y = Ny * ((Ny < Kv) ? Kv / Ny + 1 : 1), X = Nx * ((Nx < Ku) ? Ku / Nx + 1 : 1);
+  // the cell basis, columns a1 = (a1x, a1y), a2 = (a2x, a2y); square pixels when NULL
+  double a1x = 1.0, a1y = 0.0, a2x = 0.0, a2y = 1.0;
+  if (lattice) {
+    a1x = lattice[0]; a1y = lattice[1]; a2x = lattice[2]; a2y = lattice[3];
+    if (!(a1x * a2y - a2x * a1y > 0.0))
+      return false;
+  } else {
+    a1x = (double)X; a2y = (double)Y;
+  }
+  const double unit[6] = {1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+  for (int k = 0; k < 6; ++k) metric[k] = unit[k];
+  *cst = 0; *ux = 0.0; *uy = 0.0;
+  if (Y == 1 || X == 1) {
+    // a one-cell axis: the normal along b1 (one row) or b2 (one column), rows of the basis
+    // inverse (factorization.py:1337-1375)
+    const double det = a1x * a2y - a2x * a1y;
+    const double b0 = (Y == 1) ? a2y / det : -a1y / det;
+    const double b1 = (Y == 1) ? -a2x / det : a1x / det;
+    const double h = std::hypot(b0, b1);
+    *cst = 1; *ux = b0 / h; *uy = b1 / h;
+  } else {
+    // index steps in units of one u-step, and J^-T (factorization.py:1110-1133)
+    double j00 = a1x / X, j10 = a1y / X, j01 = a2x / Y, j11 = a2y / Y;
+    const double scale = std::hypot(j00, j10);
+    j00 /= scale; j10 /= scale; j01 /= scale; j11 /= scale;
+    const double step_j = std::hypot(j01, j11), step_i = std::hypot(j00, j10);
+    const bool same = std::fabs(step_j - step_i) <= 1e-12 * std::max(std::fabs(step_j), std::fabs(step_i));
+    const double det = j00 * j11 - j01 * j10;
+    metric[0] = j11 / det; metric[1] = -j10 / det; metric[2] = -j01 / det; metric[3] = j00 / det;
+    metric[4] = same ? 1.0 : step_j;
+    metric[5] = same ? 1.0 : step_i;
+  }
+  return true;
+}
+
 size_t wb_assemble_nvf_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
   if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15) return 0;
   return wb_assemble_nvf_workspace_bytes(L, Ny, Nx, hu, hv);
@@ -198,46 +241,45 @@ int wb_assemble_nvf(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int h
     return fail(ctx, WB_BADARG, "wb_assemble_nvf: null buffer");
   if (workspace_bytes < wb_assemble_nvf_workspace_size(L, Ny, Nx, hu, hv))
     return fail(ctx, WB_OOM, "wb_assemble_nvf: workspace too small");
-  // the raster lifted to 4 h + 1 samples per axis (factorization.py:811-827)
-  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
-  const int Y = Ny * ((Ny < Kv) ? Kv / Ny + 1 : 1), X = Nx * ((Nx < Ku) ? Ku / Nx + 1 : 1);
-  // the cell basis, columns a1 = (a1x, a1y), a2 = (a2x, a2y); square pixels when NULL
-  double a1x = 1.0, a1y = 0.0, a2x = 0.0, a2y = 1.0;
-  if (lattice) {
-    a1x = lattice[0]; a1y = lattice[1]; a2x = lattice[2]; a2y = lattice[3];
-    if (!(a1x * a2y - a2x * a1y > 0.0))
-      return fail(ctx, WB_BADARG, "wb_assemble_nvf: the lattice columns must form a right-handed basis");
-  } else {
-    a1x = (double)X; a2y = (double)Y;
-  }
-  double metric[6] = {1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+  double metric[6];
   int cst = 0;
   double ux = 0.0, uy = 0.0;
-  if (Y == 1 || X == 1) {
-    // a one-cell axis: the normal along b1 (one row) or b2 (one column), rows of the basis
-    // inverse (factorization.py:1337-1375)
-    const double det = a1x * a2y - a2x * a1y;
-    const double b0 = (Y == 1) ? a2y / det : -a1y / det;
-    const double b1 = (Y == 1) ? -a2x / det : a1x / det;
-    const double h = std::hypot(b0, b1);
-    cst = 1; ux = b0 / h; uy = b1 / h;
-  } else {
-    // index steps in units of one u-step, and J^-T (factorization.py:1110-1133)
-    double j00 = a1x / X, j10 = a1y / X, j01 = a2x / Y, j11 = a2y / Y;
-    const double scale = std::hypot(j00, j10);
-    j00 /= scale; j10 /= scale; j01 /= scale; j11 /= scale;
-    const double step_j = std::hypot(j01, j11), step_i = std::hypot(j00, j10);
-    const bool same = std::fabs(step_j - step_i) <= 1e-12 * std::max(std::fabs(step_j), std::fabs(step_i));
-    const double det = j00 * j11 - j01 * j10;
-    metric[0] = j11 / det; metric[1] = -j10 / det; metric[2] = -j01 / det; metric[3] = j00 / det;
-    metric[4] = same ? 1.0 : step_j;
-    metric[5] = same ? 1.0 : step_i;
-  }
+  if (!nvf_metric(Ny, Nx, hu, hv, lattice, metric, &cst, &ux, &uy))
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf: the lattice columns must form a right-handed basis");
   return hip_status(ctx,
                     wb_assemble_nvf_launch((const cd*)eps, L, Ny, Nx, hu, hv, metric, cst, ux, uy,
                                            floor ? 1 : 0, (cd*)epx, (cd*)epy, (cd*)epz_i, (cd*)epxy, workspace, info,
                                            (hipStream_t)stream),
                     "wb_assemble_nvf");
+}
+
+size_t wb_assemble_nvf_adjoint_workspace_size(int L, int Ny, int Nx, int hu, int hv) {
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15) return 0;
+  return wb_assemble_nvf_adjoint_workspace_bytes(L, Ny, Nx, hu, hv);
+}
+
+int wb_assemble_nvf_adjoint(wb_ctx* ctx, const double* eps, int L, int Ny, int Nx, int hu, int hv,
+                            const double* lattice, const double* d_epx, const double* d_epy,
+                            const double* d_epz, const double* d_off, double* d_eps, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  int s = enter(ctx);
+  if (s) return s;
+  if (L <= 0 || Ny <= 0 || Nx <= 0 || hu < 0 || hv < 0 || hu > 15 || hv > 15)
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf_adjoint: bad dimensions (need L, Ny, Nx >= 1, 0 <= hu, hv <= 15)");
+  if (!eps || !d_epx || !d_epy || !d_epz || !d_off || !d_eps)
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf_adjoint: null buffer");
+  if (workspace_bytes < wb_assemble_nvf_adjoint_workspace_size(L, Ny, Nx, hu, hv))
+    return fail(ctx, WB_OOM, "wb_assemble_nvf_adjoint: workspace too small");
+  double metric[6];
+  int cst = 0;
+  double ux = 0.0, uy = 0.0;
+  if (!nvf_metric(Ny, Nx, hu, hv, lattice, metric, &cst, &ux, &uy))
+    return fail(ctx, WB_BADARG, "wb_assemble_nvf_adjoint: the lattice columns must form a right-handed basis");
+  return hip_status(ctx,
+                    wb_assemble_nvf_adjoint_launch((const cd*)eps, L, Ny, Nx, hu, hv, metric, cst, ux, uy,
+                                                   (const cd*)d_epx, (const cd*)d_epy, (const cd*)d_epz,
+                                                   (const cd*)d_off, (cd*)d_eps, workspace, (hipStream_t)stream),
+                    "wb_assemble_nvf_adjoint");
 }
 
 size_t wb_solve_workspace_size(int P, int L, int hu, int hv, int mode) {
@@ -575,7 +617,8 @@ size_t wb_solve_checkpoint_size(int P, int L, int hu, int hv, int mode) {
 
 int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                   const double* thick, const int* conv_idx, const double* epx, const double* epy,
-                  const double* epz_i, double* eff, double* amp, double* pe, int* flags,
+                  const double* epz_i, const double* epxy, const double* epyx, double* eff,
+                  double* amp, double* pe, int* flags,
                   void* workspace, size_t workspace_bytes, void* checkpoint,
                   size_t checkpoint_bytes, void* stream) {
   int s = enter(ctx);
@@ -586,6 +629,8 @@ int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const dou
     return fail(ctx, WB_BADARG, "wb_solve_keep: bad dimensions");
   if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !eff || !amp || !pe || !checkpoint)
     return fail(ctx, WB_BADARG, "wb_solve_keep: null buffer");
+  if ((epxy == nullptr) != (epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_solve_keep: pass both off-diagonal slots or neither");
   if (workspace_bytes < wb_solve_workspace_size(P, L, hu, hv, mode))
     return fail(ctx, WB_OOM, "wb_solve_keep: workspace too small");
   if (checkpoint_bytes < wb_solve_checkpoint_size(P, L, hu, hv, mode))
@@ -594,6 +639,7 @@ int wb_solve_keep(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const dou
   a.P = P; a.L = L; a.hu = hu; a.hv = hv; a.mode = mode;
   a.pts = pts; a.thick = thick; a.conv_idx = conv_idx;
   a.epx = (const cd*)epx; a.epy = (const cd*)epy; a.epz = (const cd*)epz_i;
+  a.epxy = (const cd*)epxy; a.epyx = (const cd*)epyx;
   a.eff = eff; a.amp = (cd*)amp; a.pe = pe; a.flags = flags; a.ck = checkpoint;
   return hip_status(ctx, wb_solve_launch(a, workspace, (hipStream_t)stream), "wb_solve_keep");
 }
@@ -605,10 +651,11 @@ size_t wb_solve_adjoint_workspace_size(int P, int L, int hu, int hv, int mode) {
 
 int wb_solve_adjoint(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const double* pts,
                      const double* thick, const int* conv_idx, int n_conv, const double* epx,
-                     const double* epy, const double* epz_i, const void* solve_workspace,
-                     const void* checkpoint, const double* eff_bar, double* d_epx, double* d_epy,
-                     double* d_epz, double* d_thick, void* workspace, size_t workspace_bytes,
-                     void* stream) {
+                     const double* epy, const double* epz_i, const double* epxy,
+                     const double* epyx, const void* solve_workspace, const void* checkpoint,
+                     const double* eff_bar, double* d_epx, double* d_epy, double* d_epz,
+                     double* d_epxy, double* d_epyx, double* d_thick, void* workspace,
+                     size_t workspace_bytes, void* stream) {
   int s = enter(ctx);
   if (s) return s;
   if (mode != 0)
@@ -618,17 +665,22 @@ int wb_solve_adjoint(wb_ctx* ctx, int P, int L, int hu, int hv, int mode, const 
   if (!pts || !thick || !conv_idx || !epx || !epy || !epz_i || !solve_workspace || !checkpoint ||
       !eff_bar || !d_epx || !d_epy || !d_epz || !d_thick)
     return fail(ctx, WB_BADARG, "wb_solve_adjoint: null buffer");
+  if ((epxy == nullptr) != (epyx == nullptr) || (epxy == nullptr) != (d_epxy == nullptr) ||
+      (epxy == nullptr) != (d_epyx == nullptr))
+    return fail(ctx, WB_BADARG, "wb_solve_adjoint: the off-diagonal slots and their cotangents go together");
   if (workspace_bytes < wb_solve_adjoint_workspace_size(P, L, hu, hv, mode))
     return fail(ctx, WB_OOM, "wb_solve_adjoint: workspace too small");
   wb_solve_args f;
   f.P = P; f.L = L; f.hu = hu; f.hv = hv; f.mode = mode;
   f.pts = pts; f.thick = thick; f.conv_idx = conv_idx;
   f.epx = (const cd*)epx; f.epy = (const cd*)epy; f.epz = (const cd*)epz_i;
+  f.epxy = (const cd*)epxy; f.epyx = (const cd*)epyx;
   f.eff = nullptr; f.amp = nullptr; f.pe = nullptr; f.flags = nullptr;
   f.ck = const_cast<void*>(checkpoint);
   wb_adjoint_args a;
   a.fwd = &f; a.n_conv = n_conv; a.solve_ws = solve_workspace; a.ck = checkpoint;
   a.eff_bar = eff_bar; a.d_epx = (cd*)d_epx; a.d_epy = (cd*)d_epy; a.d_epz = (cd*)d_epz;
+  a.d_epxy = (cd*)d_epxy; a.d_epyx = (cd*)d_epyx;
   a.d_thick = d_thick;
   return hip_status(ctx, wb_adjoint_launch(a, workspace, (hipStream_t)stream), "wb_solve_adjoint");
 }

@@ -912,6 +912,48 @@ __global__ void k_nvf_scale_cols(int N, int L, cd* Y, const double* c) {
 
 }  // namespace
 
+// The NVF builder's workspace: the series, the normal field's buffers, the layer matrices
+// (E, Delta, the three projectors C_xx | C_xy | C_yy, two products) and the floor's scratch.
+struct NvfWs {
+  cd *lines, *c2d, *prod, *plines, *pc2d, *E, *D, *C, *T1, *T2, *fw, *ftau;
+  unsigned char* mask;
+  double *gcol, *dist, *fval, *fgain;
+  int *ipiv, *finfo;
+  void* frot;
+};
+static NvfWs nvf_carve(void* ws, int L, int Ny, int Nx, int hu, int hv) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const int rv = (Ny < Kv) ? Kv / Ny + 1 : 1, ru = (Nx < Ku) ? Ku / Nx + 1 : 1;
+  const int Y = Ny * rv, X = Nx * ru;
+  const long long P = (long long)L * Y * X;
+  const long long NN = (long long)N * N;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
+  NvfWs w;
+  w.lines = (cd*)take(sizeof(cd) * 2 * (size_t)L * Nx * Kv);
+  w.c2d = (cd*)take(sizeof(cd) * 2 * (size_t)L * Kv * Ku);
+  w.mask = (unsigned char*)take((size_t)P);
+  w.gcol = (double*)take(sizeof(double) * (size_t)P);
+  w.dist = (double*)take(sizeof(double) * (size_t)P);
+  w.prod = (cd*)take(sizeof(cd) * 3 * (size_t)P);
+  w.plines = (cd*)take(sizeof(cd) * 3 * (size_t)L * X * Kv);
+  w.pc2d = (cd*)take(sizeof(cd) * 3 * (size_t)L * Kv * Ku);
+  w.E = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  w.D = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  w.C = (cd*)take(sizeof(cd) * 3 * (size_t)L * NN);
+  w.T1 = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  w.T2 = (cd*)take(sizeof(cd) * (size_t)L * NN);
+  w.ipiv = (int*)take(sizeof(int) * 2 * (size_t)L * N);
+  w.fw = (cd*)take(sizeof(cd) * (size_t)L * N);
+  w.ftau = (cd*)take(sizeof(cd) * (size_t)L * N);
+  w.frot = (void*)take(wb_hqr_ms_rot_bytes(N, L));
+  w.finfo = (int*)take(sizeof(int) * (size_t)L);
+  w.fval = (double*)take(sizeof(double) * (size_t)L);
+  w.fgain = (double*)take(sizeof(double) * (size_t)L * N);
+  return w;
+}
+
 size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
   const size_t nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
   const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
@@ -947,28 +989,13 @@ hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, 
   const int Y = Ny * rv, X = Nx * ru;
   const long long P = (long long)L * Y * X;
   const long long NN = (long long)N * N;
-  char* p = (char*)ws;
-  auto take = [&](size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
-  cd* lines = (cd*)take(sizeof(cd) * 2 * (size_t)L * Nx * Kv);
-  cd* c2d = (cd*)take(sizeof(cd) * 2 * (size_t)L * Kv * Ku);
-  unsigned char* mask = (unsigned char*)take((size_t)P);
-  double* gcol = (double*)take(sizeof(double) * (size_t)P);
-  double* dist = (double*)take(sizeof(double) * (size_t)P);
-  cd* prod = (cd*)take(sizeof(cd) * 3 * (size_t)P);
-  cd* plines = (cd*)take(sizeof(cd) * 3 * (size_t)L * X * Kv);
-  cd* pc2d = (cd*)take(sizeof(cd) * 3 * (size_t)L * Kv * Ku);
-  cd* E = (cd*)take(sizeof(cd) * (size_t)L * NN);
-  cd* D = (cd*)take(sizeof(cd) * (size_t)L * NN);
-  cd* C = (cd*)take(sizeof(cd) * 3 * (size_t)L * NN);
-  cd* T1 = (cd*)take(sizeof(cd) * (size_t)L * NN);
-  cd* T2 = (cd*)take(sizeof(cd) * (size_t)L * NN);
-  int* ipiv = (int*)take(sizeof(int) * 2 * (size_t)L * N);
-  cd* fw = (cd*)take(sizeof(cd) * (size_t)L * N);
-  cd* ftau = (cd*)take(sizeof(cd) * (size_t)L * N);
-  void* frot = (void*)take(wb_hqr_ms_rot_bytes(N, L));
-  int* finfo = (int*)take(sizeof(int) * (size_t)L);
-  double* fval = (double*)take(sizeof(double) * (size_t)L);
-  double* fgain = (double*)take(sizeof(double) * (size_t)L * N);
+  const NvfWs w = nvf_carve(ws, L, Ny, Nx, hu, hv);
+  cd *lines = w.lines, *c2d = w.c2d, *prod = w.prod, *plines = w.plines, *pc2d = w.pc2d;
+  unsigned char* mask = w.mask;
+  double *gcol = w.gcol, *dist = w.dist, *fval = w.fval, *fgain = w.fgain;
+  cd *E = w.E, *D = w.D, *C = w.C, *T1 = w.T1, *T2 = w.T2, *fw = w.fw, *ftau = w.ftau;
+  int *ipiv = w.ipiv, *finfo = w.finfo;
+  void* frot = w.frot;
   const cd one = mkc(1.0, 0.0), zero = mkc(0.0, 0.0);
   const dim3 blk(256);
   auto grid1 = [](long long n) { return dim3((unsigned)((n + 255) / 256)); };
@@ -1061,5 +1088,100 @@ hipError_t wb_assemble_nvf_launch(const cd* eps, int L, int Ny, int Nx, int hu, 
                        (long long)L * NN);
     WB_CHECK_LAUNCH();
   }
+  return hipSuccess;
+}
+
+// ------------------------------------------------------------------- NVF adjoint
+// Cotangent of the raster from those of the NVF builder's outputs, the normal field held fixed
+// (it is geometry: the interface mask and the distance map do not move under an infinitesimal
+// change of the values, which is how the reference's gradient treats it; only a layer whose
+// floor stays inactive -- a passive dielectric -- is differentiated). With the forward's
+// E, A = [[1/eps]]^-1, Delta = E - A and the projectors C_ab rebuilt in the workspace:
+//   Delta-bar = -(Xb C_xx^H + C_xx^H Xb + Yb C_yy^H + C_yy^H Yb)/2 + (Ob C_xy^H + C_xy^H Ob)/2,
+//   E-bar = Xb + Yb + Delta-bar - Z^H Zb Z^H (Z = E^-1),  [[1/eps]]-bar = A^H Delta-bar A^H,
+// (Xb, Yb, Zb the cotangents of epx, epy, epz_i; Ob the sum of the two off-diagonal slots'),
+// then the direct rule's and the two series' adjoints (eps and 1/eps) as for Laurent's rule.
+size_t wb_assemble_nvf_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv) {
+  const size_t N = (size_t)(2 * hu + 1) * (2 * hv + 1), NN = N * N;
+  const size_t Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  size_t b = wb_assemble_nvf_workspace_bytes(L, Ny, Nx, hu, hv) + 256;
+  b += sizeof(cd) * 7 * (size_t)L * NN + 7 * 256;            // forward outputs, three bars
+  b += sizeof(int) * 3 * (size_t)L + 256;                     // the builder's statuses
+  b += sizeof(cd) * 2 * (size_t)L * Nx * Kv + 256;            // series cotangents
+  b += sizeof(cd) * 2 * (size_t)L * Kv * Ku + 256;
+  return b + 1024;
+}
+
+namespace {
+// out += a + b + c over n complex entries
+__global__ void k_acc3(const cd* a, const cd* b, const cd* c, cd* out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cadd(out[i], cadd(cadd(a[i], b[i]), c[i]));
+}
+}  // namespace
+
+hipError_t wb_assemble_nvf_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                          const double* metric, int cst, double ux, double uy,
+                                          const cd* d_epx, const cd* d_epy, const cd* d_epz,
+                                          const cd* d_off, cd* d_eps, void* ws, hipStream_t stream) {
+  const int nv = 2 * hv + 1, nu = 2 * hu + 1, N = nu * nv;
+  const int Kv = 4 * hv + 1, Ku = 4 * hu + 1;
+  const long long NN = (long long)N * N, LNN = (long long)L * NN;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* r = p; p += (bytes + 255) / 256 * 256; return r; };
+  void* fws = take(wb_assemble_nvf_workspace_bytes(L, Ny, Nx, hu, hv));
+  cd* fx = (cd*)take(sizeof(cd) * LNN);
+  cd* fy = (cd*)take(sizeof(cd) * LNN);
+  cd* fz = (cd*)take(sizeof(cd) * LNN);
+  cd* fxy = (cd*)take(sizeof(cd) * LNN);
+  cd* Db = (cd*)take(sizeof(cd) * LNN);
+  cd* Eb = (cd*)take(sizeof(cd) * LNN);
+  cd* Tm = (cd*)take(sizeof(cd) * LNN);
+  int* info = (int*)take(sizeof(int) * 3 * (size_t)L);
+  cd* lines_bar = (cd*)take(sizeof(cd) * 2 * (size_t)L * Nx * Kv);
+  cd* c2d_bar = (cd*)take(sizeof(cd) * 2 * (size_t)L * Kv * Ku);
+  hipError_t e;
+  // the forward without the floor: E, Delta and the projectors stay in its workspace
+  if ((e = wb_assemble_nvf_launch(eps, L, Ny, Nx, hu, hv, metric, cst, ux, uy, 0, fx, fy, fz, fxy, fws,
+                                  info, stream)))
+    return e;
+  const NvfWs w = nvf_carve(fws, L, Ny, Nx, hu, hv);
+  const cd* Cxx = w.C;
+  const cd* Cxy = w.C + LNN;
+  const cd* Cyy = w.C + 2 * LNN;
+  auto S = [&](const cd* x) { return bp_strided(x, NN); };
+  const cd half = mkc(0.5, 0.0), mhalf = mkc(-0.5, 0.0), one = mkc(1.0, 0.0), mone = mkc(-1.0, 0.0),
+           zero = mkc(0.0, 0.0);
+  // Delta-bar
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, mhalf, S(d_epx), N, S(Cxx), N, zero, S(Db), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mhalf, S(Cxx), N, S(d_epx), N, one, S(Db), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, mhalf, S(d_epy), N, S(Cyy), N, one, S(Db), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mhalf, S(Cyy), N, S(d_epy), N, one, S(Db), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, half, S(d_off), N, S(Cxy), N, one, S(Db), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, half, S(Cxy), N, S(d_off), N, one, S(Db), N, L, stream))) return e;
+  // E-bar = Xb + Yb + Delta-bar - Z^H Zb Z^H
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, one, S(d_epz), N, S(fz), N, zero, S(Tm), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, mone, S(fz), N, S(Tm), N, zero, S(Eb), N, L, stream))) return e;
+  hipLaunchKernelGGL(k_acc3, dim3((unsigned)((LNN + 255) / 256)), dim3(256), 0, stream, d_epx, d_epy, Db, Eb,
+                     LNN);
+  WB_CHECK_LAUNCH();
+  // [[1/eps]]-bar = A^H Delta-bar A^H with A = E - Delta (into fy, fx as scratch)
+  hipLaunchKernelGGL(k_sub_from, dim3((unsigned)((LNN + 255) / 256)), dim3(256), 0, stream, w.E, w.D, LNN);
+  WB_CHECK_LAUNCH();
+  if ((e = wb_zgemm_ops_launch(0, 1, N, N, N, one, S(Db), N, S(w.D), N, zero, S(fx), N, L, stream))) return e;
+  if ((e = wb_zgemm_ops_launch(1, 0, N, N, N, one, S(w.D), N, S(fx), N, zero, S(fy), N, L, stream))) return e;
+  // the direct rule's, then the two series' adjoints
+  hipLaunchKernelGGL(k_direct_adj, dim3((Kv * Ku + 255) / 256, L), dim3(256), 0, stream, Eb, L, hu, hv, c2d_bar);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_direct_adj, dim3((Kv * Ku + 255) / 256, L), dim3(256), 0, stream, fy, L, hu, hv,
+                     c2d_bar + (size_t)L * Kv * Ku);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series2d_adj, dim3(Kv, L), dim3(64), 0, stream, c2d_bar, L, Nx, hu, hv, lines_bar, 0);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series2d_adj, dim3(Kv, L), dim3(64), 0, stream, c2d_bar + (size_t)L * Kv * Ku, L, Nx,
+                     hu, hv, lines_bar + (size_t)L * Nx * Kv, 0);
+  WB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_series_v_adj, dim3(Nx, L), dim3(64), 0, stream, eps, lines_bar, L, Ny, Nx, hv, d_eps, 1);
+  WB_CHECK_LAUNCH();
   return hipSuccess;
 }

@@ -57,6 +57,11 @@ size_t wb_assemble_laurent_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu
 hipError_t wb_assemble_laurent_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
                                               const cd* epz_i, const cd* d_epx, const cd* d_epy,
                                               const cd* d_epz, cd* d_eps, void* ws, hipStream_t stream);
+size_t wb_assemble_nvf_adjoint_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
+hipError_t wb_assemble_nvf_adjoint_launch(const cd* eps, int L, int Ny, int Nx, int hu, int hv,
+                                          const double* metric, int cst, double ux, double uy,
+                                          const cd* d_epx, const cd* d_epy, const cd* d_epz,
+                                          const cd* d_off, cd* d_eps, void* ws, hipStream_t stream);
 size_t wb_assemble_nvf_workspace_bytes(int L, int Ny, int Nx, int hu, int hv);
 // metric (host): the cell's J^-T (row-major 2 x 2) in units of one u-step, then the v- and
 // u-steps in those units; cst: a one-cell axis after the lift, whose constant normal is

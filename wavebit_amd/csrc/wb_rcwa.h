@@ -309,6 +309,8 @@ struct wb_adjoint_args {
   cd* d_epx;                  // n_conv x N x N (overwritten)
   cd* d_epy;
   cd* d_epz;
+  cd* d_epxy = nullptr;       // the NVF off-diagonal slots' cotangents (with fwd->epxy / epyx)
+  cd* d_epyx = nullptr;
   double* d_thick;            // P x L (overwritten)
 };
 hipError_t wb_adjoint_launch(const wb_adjoint_args& a, void* ws, hipStream_t st);

@@ -531,10 +531,6 @@ def _run_live(prepared, problem, r, checks):
         raise UnsupportedFeatureError(
             'A differentiable run solves the requested point; detuning is applied to concrete '
             'runs only.', code='rcwa.gradient', path='numerics.detuning')
-    if r.factorization == 'nvf':
-        raise UnsupportedFeatureError(
-            "Gradients are implemented under factorization='li' and 'laurent'; an NVF stack is "
-            'solved with concrete values.', code='rcwa.gradient', path='numerics.factorization')
     if r.scattering or r.fields:
         what = 'scattering' if r.scattering else 'fields'
         raise UnsupportedFeatureError(
@@ -553,6 +549,11 @@ def _run_live(prepared, problem, r, checks):
                          else torch.tensor(float(l.thickness), dtype=torch.float64) for l in layers])
     if r.side == 'bottom':                   # the mirrored stack (see _resolve); flip is differentiable
         eps, thick = torch.flip(eps, [0]), torch.flip(thick, [0])
+    if r.factorization == 'nvf' and not adjoint.passive(eps):
+        raise UnsupportedFeatureError(
+            "Gradients under factorization='nvf' are implemented for passive dielectric layers "
+            '(Re eps > 0, Im eps >= 0 everywhere); a stack with metal or gain is solved with '
+            'concrete values.', code='rcwa.gradient', path='numerics.factorization')
     # The pure TE/TM routes differentiate through the coupled path at hv = 0, phi = 0: a one-row
     # (y-invariant) raster there decouples into the same TE (s) and TM (p) problems, the 2D Li
     # rules reducing to the 1D ones, so its efficiencies and their gradients are the 1D route's
@@ -563,8 +564,9 @@ def _run_live(prepared, problem, r, checks):
                             polarization=r.polarization, eps_top=r.eps_top, eps_bottom=r.eps_bottom,
                             period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
               for wl in r.wavelengths]
+    lattice = _engine.lattice_of(r.period) if r.factorization == 'nvf' else None
     eff, amp, pe, flags = adjoint.solve(eps, thick.to(device), points, r.hu, r.hv, device,
-                                        factorization=r.factorization)
+                                        factorization=r.factorization, lattice=lattice)
     fl_h = flags.cpu().numpy()
     error = flag_error(fl_h, prepared.numerics)
     if error is not None:

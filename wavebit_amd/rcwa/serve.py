@@ -157,10 +157,12 @@ class Server:
             raise UnsupportedFeatureError(
                 'The server returns diffraction outputs; solve a field request with rcwa.run.',
                 code='rcwa.unsupported', path='outputs.fields')
-        if req.gradient and r.factorization == 'nvf':
+        if req.gradient and r.factorization == 'nvf' and not bool(
+                np.all((r.rasters.real > 0) & (r.rasters.imag >= 0))):
             raise UnsupportedFeatureError(
-                "Gradient requests are served under factorization='li' and 'laurent'.",
-                code='rcwa.gradient', path='numerics.factorization')
+                "Gradient requests under factorization='nvf' are served for passive dielectric "
+                'layers (Re eps > 0, Im eps >= 0 everywhere).', code='rcwa.gradient',
+                path='numerics.factorization')
         if req.numerics.detuning:
             raise UnsupportedFeatureError(
                 'The server solves the requested points; run a request with detuning through '
@@ -236,6 +238,7 @@ class Server:
                                 period=r.period, thickness=tuple(float(t) for t in r.thickness), k_direct=r.k_direct)
                   for wl in r.wavelengths]
         chunk = self._gradient_chunk((r.hu, r.hv, r.path, r.rasters.shape[0]), len(points))
+        lattice = _engine.lattice_of(r.period) if r.factorization == 'nvf' else None
         parts = []
         with torch.cuda.stream(stream):
             eps = torch.tensor(np.ascontiguousarray(r.rasters), dtype=torch.complex128, device=self.device,
@@ -244,7 +247,8 @@ class Server:
                                  requires_grad=True)
             for c0 in range(0, len(points), chunk):
                 eff, amp, pe, flags = adjoint.solve(eps, thick, points[c0:c0 + chunk], r.hu, r.hv,
-                                                    self.device, factorization=r.factorization)
+                                                    self.device, factorization=r.factorization,
+                                                    lattice=lattice)
                 (eff[:, 0, 2] + eff[:, 0, 3]).sum().backward()      # accumulates into .grad
                 parts.append(tuple(t.detach().cpu().numpy() for t in (eff, amp, pe, flags)))
                 del eff, amp, pe, flags
