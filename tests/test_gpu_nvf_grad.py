@@ -89,10 +89,16 @@ def test_solve_adjoint_off_diagonal_cotangents_match_differences_of_the_forward(
                 assert abs(an - fd) <= 1e-6 * abs(fd) + 1e-8, (which, l, i, j, dz, an, fd)
 
 
-@pytest.mark.parametrize('period', [(1.0, 1.0), (0.9, 0.7), np.array([[1.0, 0.35], [0.0, 0.9]])])
-def test_nvf_gradient_matches_frozen_normal_differences(monkeypatch, period):
-    hu = hv = 2
-    eps = np.stack([_blobs((12, 12), 3, (1.0, 4.0 + 0.3j)), _blobs((12, 12), 9, (2.25, 6.0))])
+# (cell, raster shape, orders): square, rectangular and skewed cells; a raster smaller than
+# 4 h + 1 samples per axis (the normal field on the lifted raster, the series on the original)
+CELLS = [((1.0, 1.0), (12, 12), (2, 2)), ((0.9, 0.7), (12, 12), (2, 2)),
+         (np.array([[1.0, 0.35], [0.0, 0.9]]), (12, 12), (2, 2)), ((0.8, 0.9), (6, 5), (2, 1))]
+
+
+@pytest.mark.parametrize('period, shape, orders', CELLS)
+def test_nvf_gradient_matches_frozen_normal_differences(monkeypatch, period, shape, orders):
+    hu, hv = orders
+    eps = np.stack([_blobs(shape, 3, (1.0, 4.0 + 0.3j)), _blobs(shape, 9, (2.25, 6.0))])
     thick = [0.17, 0.23]
     pts = [(0.83, 0.2, 0.5), (1.07, 0.0, 0.0)]
     pol = 'p'
@@ -115,7 +121,7 @@ def test_nvf_gradient_matches_frozen_normal_differences(monkeypatch, period):
     scale = np.abs(g_eps).max()
     # pixels on and off the interfaces of both layers, real and imaginary directions
     pix = [(0, 0, 0), (0, 5, 6), (1, 3, 9), (1, 11, 2), (0, 8, 1), (1, 6, 6)]
-    for where in pix:
+    for where in ((l, y % shape[0], x % shape[1]) for l, y, x in pix):
         for unit, part in ((1.0, 'real'), (1j, 'imag')):
             def shifted(t):
                 e = eps.copy()
@@ -169,3 +175,33 @@ def test_nvf_run_gradient_equals_the_reverse_pass_and_refuses_metal():
     with pytest.raises(wavebit_amd.errors.UnsupportedFeatureError) as e:
         rcwa.run(prob, numerics=rcwa.Numerics(orders=(1, 1), factorization='nvf'), **RUN)
     assert e.value.code == 'rcwa.gradient'
+
+
+def test_a_served_nvf_gradient_request_equals_the_differentiable_run():
+    """A gradient request under NVF through the server: its d(sum of T)/d(rasters, thicknesses)
+    equals the differentiable ``rcwa.run`` of the same problem."""
+    from wavebit_amd.rcwa import serve
+    base = np.stack([_blobs((10, 10), 4, (1.0, 5.0 + 0.1j)), _blobs((10, 10), 6, (2.0, 3.5))])
+    thick0 = [0.15, 0.2]
+    numerics = rcwa.Numerics(orders=(2, 2), factorization='nvf')
+
+    def problem(eps, thick):
+        layers = [rcwa.Layer(thickness=thick[l], epsilon=eps[l]) for l in range(2)]
+        return rcwa.Problem(stack=rcwa.Stack(layers=layers, top=wavebit_amd.Material(1.0),
+                                             bottom=wavebit_amd.Material(2.25), period=(0.9, 0.8)),
+                            excitation=rcwa.Excitation(wavelength=[0.75, 0.9], sources=(
+                                rcwa.PlaneWave(polarization='s', theta=0.2, phi=0.4),)))
+    srv = serve.Server(streams=2)
+    results, report = srv.serve([serve.Request(problem=problem(base, thick0), numerics=numerics,
+                                               arrival=0.0, id=0, gradient=True)], realtime=False)
+    res = results[0]
+    assert not isinstance(res, Exception), res
+    assert report['gradient_requests'] == 1
+    eps = torch.tensor(base, dtype=torch.complex128, requires_grad=True)
+    thick = torch.tensor(thick0, dtype=torch.float64, requires_grad=True)
+    out = rcwa.run(problem(eps, thick), numerics=numerics, **RUN)
+    out.T.sum().backward()
+    g = res.diagnostics['gradient']
+    ge, gt = eps.grad.numpy(), thick.grad.numpy()
+    assert np.abs(g['d_epsilon'] - ge).max() <= 1e-9 * np.abs(ge).max()
+    assert np.abs(g['d_thickness'] - gt).max() <= 1e-9 * np.abs(gt).max()
