@@ -401,6 +401,42 @@ def solve_ragged_workspace_bytes(layers, hu, hv, mode):
         len(lay), lay.ctypes.data_as(ctypes.c_void_p), hu, hv, mode))
 
 
+def _packed_outputs(P, L, N, device):
+    """eff (P, 3, 4, N), amp (P, 3, 4, N) complex, pe (P, 3) and flags (P, L) int32 as views of
+    one device buffer, so that ``host_copy`` brings a solve's results back in a single copy."""
+    nf = (P * L + 1) // 2
+    buf = torch.empty(36 * P * N + 3 * P + nf, dtype=torch.float64, device=device)
+    return packed_views(buf, P, L, N)
+
+
+def packed_views(buf, P, L, N):
+    """(eff, amp, pe, flags) views of a packed output buffer (see ``_packed_outputs``)."""
+    na, ne = 24 * P * N, 12 * P * N
+    amp = buf[:na].view(torch.complex128).view(P, 3, 4, N)
+    eff = buf[na:na + ne].view(P, 3, 4, N)
+    pe = buf[na + ne:na + ne + 3 * P].view(P, 3)
+    flags = buf[na + ne + 3 * P:].view(torch.int32)[:P * L].view(P, L)
+    eff._wb_packed = (buf, amp, pe, flags)
+    return eff, amp, pe, flags
+
+
+def host_copy(eff, amp, pe, flags):
+    """Host numpy arrays of a solve's (eff, amp, pe, flags): one device-to-host copy when the
+    four are ``solve``'s packed views, else one copy each."""
+    packed = getattr(eff, '_wb_packed', None)
+    if packed is None or packed[1] is not amp or packed[2] is not pe or packed[3] is not flags:
+        return tuple(t.cpu().numpy() for t in (eff, amp, pe, flags))
+    buf = packed[0]
+    h = buf.cpu()
+    P, _, _, N = eff.shape
+    L = flags.shape[1]
+    na, ne = 24 * P * N, 12 * P * N
+    return (h[na:na + ne].view(P, 3, 4, N).numpy(),
+            h[:na].view(torch.complex128).view(P, 3, 4, N).numpy(),
+            h[na + ne:na + ne + 3 * P].view(P, 3).numpy(),
+            h[na + ne + 3 * P:].view(torch.int32)[:P * L].view(P, L).numpy())
+
+
 def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None, epxy=None):
     """Batch RCWA solve; see ``wb_solve``. ``epxy``: the NVF off-diagonal slot (epxy = epyx),
     or None. Returns (eff, amp, pe, flags)."""
@@ -413,10 +449,7 @@ def solve(pts, thick, conv_idx, epx, epy, epz_i, hu, hv, mode, workspace=None, e
     _require(conv_idx, torch.int32, device, 'conv_idx')
     for name, t in (('epx', epx), ('epy', epy), ('epz_i', epz_i)):
         _require(t, torch.complex128, device, name)
-    eff = torch.empty((P, 3, 4, N), dtype=torch.float64, device=device)
-    amp = torch.empty((P, 3, 4, N), dtype=torch.complex128, device=device)
-    pe = torch.empty((P, 3), dtype=torch.float64, device=device)
-    flags = torch.empty((P, L), dtype=torch.int32, device=device)
+    eff, amp, pe, flags = _packed_outputs(P, L, N, device)
     nbytes = ctx.lib.wb_solve_workspace_size(P, L, hu, hv, mode)
     if workspace is None or workspace.numel() < nbytes:
         workspace = _workspace(nbytes, device)

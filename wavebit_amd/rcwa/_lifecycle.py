@@ -797,7 +797,7 @@ def _solve_resolved(r, cache):
         eff, amp, pe, flags, _ = outs[0]
     else:
         eff, amp, pe, flags, _ = _engine.run_bucket(bucket, [conv], device, graph_slot=cache.graph)
-    eff, amp, pe, flags = (t.cpu().numpy() for t in (eff, amp, pe, flags))
+    eff, amp, pe, flags = _engine._native.host_copy(eff, amp, pe, flags)
     _engine.rebuild_grazing(bucket, [conv], device, eff, amp, pe, flags, smat)
     if smat is not None:
         smats = _scattering_matrices(r, smat)

@@ -65,7 +65,9 @@ class SolveGraph:
         self.epy.copy_(epy)
         self.epz.copy_(epz_i)
         self.graph.replay()
-        return tuple(t.clone() for t in self.out)
+        P, L = self.thick.shape
+        N = (2 * self.hu + 1) * (2 * self.hv + 1)
+        return _native.packed_views(self.out[0]._wb_packed[0].clone(), P, L, N)
 
 
 def worth_capturing(P, L, hu, hv, mode):
