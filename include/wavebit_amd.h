@@ -37,11 +37,14 @@
  *                    the dense inv / solve / matmul calls of rcwa/_engine/scattering.py:116-200
  *                    (_ab_matrices, _s_layer, redheffer_star) and _common/backend.py:506-542
  *                    (linear_solve).
- *   wb_solve_keep / wb_solve_adjoint / wb_assemble_li_adjoint / wb_assemble_laurent_adjoint
+ *   wb_solve_keep / wb_solve_adjoint / wb_assemble_li_adjoint / wb_assemble_laurent_adjoint /
+ *   wb_assemble_nvf_adjoint
  *                    the reverse pass PyTorch autograd takes through the same path
  *                    (loss.backward() after Prepared.run, rcwa/_lifecycle.py:785-786), with
  *                    the eigensolve's generator rule of rcwa/_engine/modal_gradient.py:161-251
- *                    and the divided-difference exponential of :446-470 written by hand;
+ *                    and the divided-difference exponential of :446-470 written by hand; under
+ *                    NVF through the builder of factorization.py:2796-2915 with the normal
+ *                    field held fixed (tests/test_frozen_geometry.py:644);
  *   wb_zgemm_batched_ops / wb_zgetrs_batched_ops
  *                    the conjugate-transposed products and solves of that reverse pass
  *                    (torch matmul / linalg.solve backward).

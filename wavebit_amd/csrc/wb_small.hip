@@ -427,20 +427,25 @@ __device__ int w_tridiag_qr(int n, const cd* H, cd* Z, cd* w) {
     const double mu = (den != 0.0) ? c - bb * bb / den : c;
     double dk = readlane_d(dl, lo), ek = readlane_d(el, lo);
     double x = dk - mu, z = ek;
-    cd z0 = (lane < n) ? AT(Z, lane, lo) : mkc(0.0, 0.0);
+    // Z's rows by lane; lanes past n read row 0 and store into the padding column, which holds
+    // no entry: the stores stay unconditional, so the next column's load, issued before this
+    // column's store, is waited for without draining the store
+    const int zr = (lane < n) ? lane : 0;
+    cd z0 = AT(Z, zr, lo);
+    cd z1 = AT(Z, zr, lo + 1);
     // d_k+1 and e_k+1 are read before the step that changes them: from a copy taken here, so the
     // reads do not wait on the chain of updates
     const double dsnap = dl, esnap = el;
     for (int k = lo; k < hi; ++k) {
       const double dk1 = readlane_d(dsnap, k + 1);
-      const double ek1 = (k + 1 < hi) ? readlane_d(esnap, k + 1) : 0.0;
-      const cd z1 = (lane < n) ? AT(Z, lane, k + 1) : mkc(0.0, 0.0);
+      const double enx = readlane_d(esnap, k + 1);
+      const double ek1 = (k + 1 < hi) ? enx : 0.0;
+      const cd z2 = AT(Z, zr, (k + 2 < n) ? k + 2 : n - 1);
+      // the rotation, branch-free (x = z = 0 gives the identity)
       const double rr2 = x * x + z * z;
-      double cs = 1.0, sn = 0.0, r = x;
-      if (rr2 > 0.0) {
-        const double ir = rsqrt_pos(rr2);
-        cs = x * ir; sn = z * ir; r = rr2 * ir;
-      }
+      const bool nz = rr2 > 0.0;
+      const double ir = rsqrt_pos(nz ? rr2 : 1.0);
+      const double cs = nz ? x * ir : 1.0, sn = nz ? z * ir : 0.0, r = nz ? rr2 * ir : x;
       if (k > lo && lane == k - 1) el = r;
       const double c2 = cs * cs, s2 = sn * sn, cssn = cs * sn;
       const double ndk = c2 * dk + 2.0 * cssn * ek + s2 * dk1;
@@ -448,8 +453,10 @@ __device__ int w_tridiag_qr(int n, const cd* H, cd* Z, cd* w) {
       const double nek = cssn * (dk1 - dk) + (c2 - s2) * ek;
       if (lane == k) { dl = ndk; el = nek; }
       // Z columns k, k+1: Z <- Z G^T
-      if (lane < n) AT(Z, lane, k) = mkc(cs * z0.x + sn * z1.x, cs * z0.y + sn * z1.y);
+      cd* dst = (lane < n) ? &AT(Z, lane, k) : &AT(Z, 0, SLD - 1);
+      *dst = mkc(cs * z0.x + sn * z1.x, cs * z0.y + sn * z1.y);
       z0 = mkc(cs * z1.x - sn * z0.x, cs * z1.y - sn * z0.y);
+      z1 = z2;
       x = nek;
       z = sn * ek1;
       dk = ndk1;
@@ -457,7 +464,10 @@ __device__ int w_tridiag_qr(int n, const cd* H, cd* Z, cd* w) {
       if (k + 1 < hi && lane == k + 1) el = ek;
       if (k + 1 == hi && lane == hi) dl = ndk1;
     }
-    if (lane < n) AT(Z, lane, hi) = z0;
+    {
+      cd* dst = (lane < n) ? &AT(Z, lane, hi) : &AT(Z, 0, SLD - 1);
+      *dst = z0;
+    }
   }
   if (lane < n) w[lane] = mkc(dl, 0.0);
   return bad;
