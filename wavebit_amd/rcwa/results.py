@@ -160,8 +160,12 @@ def pack(per_wavelength, hu, hv, wavelengths, checks, diagnostics, provenance, s
     orders_v = np.arange(-hv, hv + 1)
     ws = np.asarray(wavelengths, dtype=np.float64)
 
+    stacked = {}
+
     def stack(key):
-        return np.stack([np.asarray(d['incident'][key]) for d in per_wavelength])
+        if key not in stacked:
+            stacked[key] = np.stack([np.asarray(d['incident'][key]) for d in per_wavelength])
+        return stacked[key]
 
     finite = all(np.all(np.isfinite(stack(k))) for k in ('r_s', 'r_p', 't_s', 't_p', 'R', 'T'))
     combined = bool(finite and all(bool(v) for v in checks.values()))
